@@ -1,0 +1,1177 @@
+/*
+ * CPU oracle for MOHID Water's tracer-transport step -- TEST INFRASTRUCTURE.
+ *
+ * A plain-C restatement of the reference algorithm, routine by routine and in
+ * the reference's loop order (so that every cell sees its floating-point
+ * operations in the reference's order).  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg use it, always as the checker.
+ *
+ * Reference files (relative to the reference root):
+ *   AD = Software/MOHIDBase2/ModuleAdvectionDiffusion.F90
+ *   MF = Software/MOHIDBase1/ModuleFunctions.F90
+ *   GD = Software/MOHIDBase1/ModuleGlobalData.F90
+ *
+ * Parity status of each routine (how it is pinned) is listed in the test
+ * suite: the 1-D face builders and THOMASZ_NewType2 are compared bit for bit
+ * with the reference's own ModuleFunctions compiled from source (oracle/_ref);
+ * the AdvectionDiffusion orchestration is pinned by conservation, constancy,
+ * linearity and closed-form checks (ModuleAdvectionDiffusion itself needs
+ * HDF5-backed grid modules and does not build here).
+ *
+ * Build with -ffp-contract=off: the reference's operation order is the
+ * contract, fused multiply-adds would change the rounding.
+ */
+#include "advdiff_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* GD:143, GD:1810-1812, GD:1831-1832 (default-real-8 literals). */
+#define NULL_REAL (-9.9E15)
+#define MIN_VALUE (1.e-16)
+#define UPWIND1 1
+#define UPWIND2 2
+#define UPWIND3 3
+#define P2_TVD 4
+#define CENTRALDIF 5
+#define LEAPFROG 6
+#define MINMOD 1
+#define VANLEER 2
+#define MUSCL 3
+#define SUPERBEE 4
+#define PDM 5
+#define EXPLICIT 0.0
+#define IMPLICIT 1.0
+
+struct ora_instance {
+    ora_size3d size, ws;
+    int docycle, vertical1d, xzflow, nthreads;
+    long nI, nJ, nK, n3, n2;
+    ora_geometry g;
+    /* persistent arrays of T_AdvectionDiffusion (AD:165-381), initialised to
+     * null_real as in AllocateVariables (AD:537-684) */
+    double *KhX, *KhY, *KzZ;                  /* Diffusion_Coeficient X/Y/Z */
+    double *DHU, *DHV, *DVC;                  /* Diff_H_Const_U/V, Diff_V_Const */
+    double *D, *E, *F, *TI;                   /* COEF3%D/E/F, TICOEF3 */
+    double *xC, *xD, *xE, *xF;                /* COEF3_HorAdvXX */
+    double *yC, *yD, *yE, *yF;                /* COEF3_HorAdvYY */
+    double *zC, *zD, *zE, *zF;                /* COEF3_VertAdv */
+    double *vecW, *vecG;                      /* THOMAS%VEC(tid)%W/G */
+    long vec_len;
+    /* Set_Internal_State memory (AD:5746-5835) */
+    int st_hadv, st_vadv, st_vdif, st_hdif;
+    double ext_DTProp, last_calc;
+    int ext_AdvMethodH, ext_TVDH, ext_AdvMethodV, ext_TVDV;
+    double ext_SchmidtCoef_V, ext_SchmidtBackground_V, ext_Schmidt_H;
+    int first_call;
+};
+
+#define I3(me, i, j, k) (((long)(i) - (me)->size.ILB) + (me)->nI * (((long)(j) - (me)->size.JLB) + (me)->nJ * ((long)(k) - (me)->size.KLB)))
+#define I2(me, i, j) (((long)(i) - (me)->size.ILB) + (me)->nI * ((long)(j) - (me)->size.JLB))
+
+static double fmin2(double a, double b) { return a < b ? a : b; }
+static double fmax2(double a, double b) { return a > b ? a : b; }
+
+/* ------------------------------------------------------------------------ */
+/* MF:11127-11140 FaceConcUpFirstOrder */
+static void face_up1(double c[4], double q)
+{
+    c[0] = c[1] = c[2] = c[3] = 0.0;
+    if (q > 0.0) c[1] = 1.0; else c[2] = 1.0;
+}
+
+/* MF:11056-11086 FaceConcUpSecondOrder */
+static void face_up2(double c[4], double q)
+{
+    c[0] = c[1] = c[2] = c[3] = 0.0;
+    if (q > 0.0) {
+        c[0] = -1.0 / 8.0; c[1] = 6.0 / 8.0; c[2] = 3.0 / 8.0;
+    } else if (q < 0.0) {
+        c[3] = -1.0 / 8.0; c[2] = 6.0 / 8.0; c[1] = 3.0 / 8.0;
+    } else {
+        face_up1(c, q);
+    }
+}
+
+/* MF:11088-11125 FaceConcUpThirdOrder */
+static void face_up3(double c4[4], double q, double cr)
+{
+    double c = (1.0 - 2.0 * fabs(cr)) / 6.0;
+    double a = 0.5 + c;
+    double b = 0.5 - c;
+    double d = (1.0 - fabs(cr)) / 2.0;
+    if (q > 0.0) {
+        c4[0] = -(d * b); c4[1] = 1.0 + d * (b - a); c4[2] = d * a; c4[3] = 0.0;
+    } else if (q < 0.0) {
+        c4[3] = -(d * b); c4[2] = 1.0 + d * (b - a); c4[1] = d * a; c4[0] = 0.0;
+    } else {
+        face_up1(c4, q);
+    }
+}
+
+/* MF:11045-11053 Courant */
+static double courant(double q, double v, double dt) { return q * dt / v; }
+
+static double clamp_dc(double dc)
+{
+    if (fabs(dc) < MIN_VALUE) dc = (dc >= 0.0) ? MIN_VALUE : -MIN_VALUE;
+    return dc;
+}
+
+static double superbee(double r)
+{
+    return fmax2(fmax2(0.0, fmin2(1.0, 2.0 * r)), fmin2(r, 2.0));
+}
+
+/* MF:10702-10894 ComputeAdvectionFace; P, V, du are the 4-point stencil
+ * (Fortran index 1..4 -> C 0..3). Returns 0 or ORA_ERR_METHOD. */
+static int advection_face(const double P[4], const double V[4], const double du[4],
+                          double dt, double Q, double volrelmax, int method,
+                          int tvd, int nearb, int upwind2, double CFace[4])
+{
+    double cup1[4], high[4];
+    double cr = 0.0, theta = 0.0, dc, r, volrel = 0.0;
+    int m;
+
+    face_up1(cup1, Q);
+    if (!nearb) {
+        double aux;
+        if (Q > 0.0) {
+            cr = courant(Q, V[1], dt);
+            aux = fmin2(fmin2(V[0], V[1]), V[2]);
+            volrel = fmax2(fmax2(V[0], V[1]), V[2]) / aux;
+        } else {
+            cr = courant(Q, V[2], dt);
+            aux = fmin2(fmin2(V[1], V[2]), V[3]);
+            volrel = fmax2(fmax2(V[1], V[2]), V[3]) / aux;
+        }
+    }
+
+    if (method == UPWIND1 || (nearb && upwind2)) {
+        for (m = 0; m < 4; m++) high[m] = cup1[m];
+        theta = 0.0;
+    } else if (method == UPWIND2 && !nearb) {
+        face_up2(high, Q);
+        theta = (volrel > volrelmax) ? 0.0 : 1.0;
+    } else if (method == UPWIND3 && !nearb) {
+        face_up3(high, Q, cr);
+        theta = (volrel > volrelmax) ? 0.0 : 1.0;
+    } else if (method == CENTRALDIF || method == LEAPFROG) {
+        double auxl = du[2] / (du[1] + du[2]);
+        double auxr = du[1] / (du[1] + du[2]);
+        high[0] = 0.0; high[1] = auxl; high[2] = auxr; high[3] = 0.0;
+        theta = 1.0;
+    } else if (method == P2_TVD && !nearb) {
+        high[0] = high[1] = high[2] = high[3] = 0.0;
+        if (Q > 0.0) {
+            high[2] = 1.0;
+            dc = (P[2] - P[1]) / (du[2] + du[1]);
+            dc = clamp_dc(dc);
+            r = (P[1] - P[0]) / (du[1] + du[0]) / dc;
+        } else {
+            high[1] = 1.0;
+            dc = (P[1] - P[2]) / (du[2] + du[1]);
+            dc = clamp_dc(dc);
+            r = (P[2] - P[3]) / (du[2] + du[3]) / dc;
+        }
+        if (tvd == MINMOD) {
+            theta = fmax2(0.0, fmin2(1.0, r));
+        } else if (tvd == VANLEER) {
+            theta = (r < 0.0) ? 0.0 : 2.0 * r / (1.0 + r);
+        } else if (tvd == MUSCL) {
+            theta = fmax2(0.0, fmin2(fmin2(2.0, 2.0 * r), (1.0 + r) / 2.0));
+        } else if (tvd == SUPERBEE) {
+            theta = superbee(r);
+        } else if (tvd == PDM) {
+            double a = 0.5 + (1.0 - 2.0 * fabs(cr)) / 6.0;
+            double b = 0.5 - (1.0 - 2.0 * fabs(cr)) / 6.0;
+            double aux = a + b * r;
+            if (fabs(cr) < MIN_VALUE) cr = MIN_VALUE;
+            theta = fmax2(0.0, fmin2(fmin2(aux, 2.0 / (1.0 - cr)), 2.0 * r / cr));
+        } else {
+            return ORA_ERR_METHOD;
+        }
+        theta = 0.5 * theta * (1.0 - cr);
+    } else {
+        return ORA_ERR_METHOD;
+    }
+    for (m = 0; m < 4; m++) CFace[m] = (1.0 - theta) * cup1[m] + theta * high[m];
+    return 0;
+}
+
+/* MF:10534-10591 ComputeAdvection1D_V2, strided.  Element m (1-based) of a
+ * line is ptr[(m-1)*stride]. */
+static int adv1d_v2_strided(int ilb, int iub, double dt,
+                            const double *du, long sdu,
+                            const double *prop, const double *q, const double *v,
+                            const int32_t *cp, long s,
+                            double *C, double *D, double *E, double *F,
+                            int method, int tvd, double volrelmax, int upwind2)
+{
+#define L(p, m, st) (p)[((long)(m) - 1) * (st)]
+    int i, rc = 0;
+    for (i = ilb; i <= iub; i++) {
+        if (L(cp, i - 1, s) == 1 && L(cp, i, s) == 1) {
+            double QF = L(q, i, s);
+            int nearb = 0;
+            double P4[4], du4[4], V4[4], CF[4];
+            if (QF > 0.0) {
+                if (L(cp, i - 2, s) != 1) nearb = 1;
+            } else {
+                if (L(cp, i + 1, s) != 1) nearb = 1;
+            }
+            P4[0] = L(prop, i - 2, s); P4[1] = L(prop, i - 1, s);
+            P4[2] = L(prop, i, s);     P4[3] = L(prop, i + 1, s);
+            du4[0] = L(du, i - 2, sdu); du4[1] = L(du, i - 1, sdu);
+            du4[2] = L(du, i, sdu);     du4[3] = L(du, i + 1, sdu);
+            V4[0] = L(v, i - 2, s); V4[1] = L(v, i - 1, s);
+            V4[2] = L(v, i, s);     V4[3] = L(v, i + 1, s);
+            rc |= advection_face(P4, V4, du4, dt, QF, volrelmax, method, tvd,
+                                 nearb, upwind2, CF);
+            L(C, i, s) = QF * CF[0];
+            L(D, i, s) = QF * CF[1];
+            L(E, i, s) = QF * CF[2];
+            L(F, i, s) = QF * CF[3];
+        }
+    }
+    return rc;
+#undef L
+}
+
+/* MF:10642-10699 ComputeAdvection1D_TVD_SuperBee_2 with
+ * ComputeAdvectionFace_TVD_Superbee_1/_2 (MF:10968-11043), strided. */
+static void adv1d_sb2_strided(int ilb, int iub, double dt,
+                              const double *du, long sdu,
+                              const double *prop, const double *q, const double *v,
+                              const int32_t *cp, long s, double *D, double *E)
+{
+#define L(p, m, st) (p)[((long)(m) - 1) * (st)]
+    int i;
+    for (i = ilb; i <= iub; i++) {
+        if (L(cp, i - 1, s) == 1 && L(cp, i, s) == 1) {
+            double QF = L(q, i, s);
+            if (QF > 0.0) {
+                if (L(cp, i - 2, s) != 1) {
+                    L(D, i, s) = QF;
+                } else {
+                    /* _Superbee_1: V4(2)=V(i-1), Prop4(1..3), du4(1..3) */
+                    double cr = courant(QF, L(v, i - 1, s), dt);
+                    double dc = (L(prop, i, s) - L(prop, i - 1, s)) /
+                                (L(du, i, sdu) + L(du, i - 1, sdu));
+                    double r, theta;
+                    dc = clamp_dc(dc);
+                    r = (L(prop, i - 1, s) - L(prop, i - 2, s)) /
+                        (L(du, i - 1, sdu) + L(du, i - 2, sdu)) / dc;
+                    theta = superbee(r);
+                    theta = 0.5 * theta * (1.0 - cr);
+                    L(D, i, s) = QF * (1.0 - theta);
+                    L(E, i, s) = QF * theta;
+                }
+            } else {
+                if (L(cp, i + 1, s) != 1) {
+                    L(E, i, s) = QF;
+                } else {
+                    /* _Superbee_2: V4(3)=V(i), Prop4(2..4), du4(2..4) */
+                    double cr = courant(QF, L(v, i, s), dt);
+                    double dc = (L(prop, i - 1, s) - L(prop, i, s)) /
+                                (L(du, i, sdu) + L(du, i - 1, sdu));
+                    double r, theta;
+                    dc = clamp_dc(dc);
+                    r = (L(prop, i, s) - L(prop, i + 1, s)) /
+                        (L(du, i, sdu) + L(du, i + 1, sdu)) / dc;
+                    theta = superbee(r);
+                    theta = 0.5 * theta * (1.0 - cr);
+                    L(D, i, s) = QF * theta;
+                    L(E, i, s) = QF * (1.0 - theta);
+                }
+            }
+        }
+    }
+#undef L
+}
+
+int ora_advection_face(const double P[4], const double V[4], const double du[4],
+                       double dt, double Q, double volrelmax, int method, int tvd,
+                       int nearb, int upwind2, double CFace[4])
+{
+    return advection_face(P, V, du, dt, Q, volrelmax, method, tvd, nearb, upwind2, CFace);
+}
+
+int ora_advection1d_v2(int n, int ilb, int iub, double dt, const double *du,
+                       const double *prop, const double *q, const double *v,
+                       const int32_t *cp, double *C, double *D, double *E,
+                       double *F, int method, int tvd, double volrelmax,
+                       int upwind2)
+{
+    /* the reference reads element i-2 of a face whose two cells compute */
+    if (ilb < 2 || iub > n - 1 || (ilb == 2 && cp[0] == 1)) return ORA_ERR_BADARG;
+    return adv1d_v2_strided(ilb, iub, dt, du, 1, prop, q, v, cp, 1, C, D, E, F,
+                            method, tvd, volrelmax, upwind2);
+}
+
+void ora_advection1d_tvd_superbee_2(int n, int ilb, int iub, double dt,
+                                    const double *du, const double *prop,
+                                    const double *q, const double *v,
+                                    const int32_t *cp, double *D, double *E)
+{
+    if (ilb < 2 || iub > n - 1 || (ilb == 2 && cp[0] == 1)) return;
+    adv1d_sb2_strided(ilb, iub, dt, du, 1, prop, q, v, cp, 1, D, E);
+}
+
+/* ------------------------------------------------------------------------ */
+/* MF:4026-4123 THOMASZ_NewType2 for one column (i, j). */
+static void thomasz_column(const ora_instance *me_or_null, const ora_size3d *size,
+                           const ora_size3d *ws, long nI, long nJ,
+                           int i, int j, const double *D, const double *E,
+                           const double *F, const double *TI, double *res,
+                           double *W, double *G)
+{
+    (void)me_or_null;
+#define X3(k) (((long)i - size->ILB) + nI * (((long)j - size->JLB) + nJ * ((long)(k) - size->KLB)))
+    int KLB = ws->KLB, KUB = ws->KUB, K, II;
+    W[KLB] = -F[X3(1)] / E[X3(1)];
+    G[KLB] = TI[X3(1)] / E[X3(1)];
+    for (K = KLB + 1; K <= KUB + 1; K++) {
+        double aux = E[X3(K)] + D[X3(K)] * W[K - 1];
+        if (fabs(aux) > 0.0) {
+            W[K] = -F[X3(K)] / aux;
+            G[K] = (TI[X3(K)] - D[X3(K)] * G[K - 1]) / aux;
+        }
+        /* else: the reference silently keeps the row's previous W/G (MF:4092-4098) */
+    }
+    res[X3(KUB + 1)] = G[KUB + 1];
+    for (II = KLB + 1; II <= KUB + 1; II++) {
+        int MM = KUB + KLB + 1 - II;
+        res[X3(MM)] = W[MM] * res[X3(MM + 1)] + G[MM];
+    }
+#undef X3
+}
+
+void ora_thomasz(const ora_size3d *size, const ora_size3d *ws,
+                 const double *D, const double *E, const double *F,
+                 const double *TI, const int32_t *waterpoints, double *res,
+                 double *W, double *G)
+{
+    long nI = size->IUB - size->ILB + 1, nJ = size->JUB - size->JLB + 1;
+    int i, j;
+    for (j = ws->JLB; j <= ws->JUB; j++)
+        for (i = ws->ILB; i <= ws->IUB; i++) {
+            long top = ((long)i - size->ILB) + nI * (((long)j - size->JLB) + nJ * ((long)ws->KUB - size->KLB));
+            if (waterpoints[top] == 1)
+                thomasz_column(NULL, size, ws, nI, nJ, i, j, D, E, F, TI, res, W, G);
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+static double *alloc_fill(long n, double v)
+{
+    double *p = (double *)malloc(sizeof(double) * (size_t)n);
+    long q;
+    if (!p) return NULL;
+    for (q = 0; q < n; q++) p[q] = v;
+    return p;
+}
+
+int ora_start(const ora_size3d *size, const ora_size3d *worksize,
+              int docycle_method, int vertical1d, int xzflow, int nthreads,
+              const ora_geometry *geom, ora_instance **out)
+{
+    ora_instance *me;
+    double **arrs[22];
+    int a;
+    if (!size || !worksize || !geom || !out) return ORA_ERR_BADARG;
+    /* MOHID's invariant Size = WorkSize +/- 1 (GEO:815-816) */
+    if (size->ILB != worksize->ILB - 1 || size->IUB != worksize->IUB + 1 ||
+        size->JLB != worksize->JLB - 1 || size->JUB != worksize->JUB + 1 ||
+        size->KLB != worksize->KLB - 1 || size->KUB != worksize->KUB + 1 ||
+        worksize->ILB != 1 || worksize->JLB != 1 || worksize->KLB != 1)
+        return ORA_ERR_BADARG;
+    me = (ora_instance *)calloc(1, sizeof(*me));
+    if (!me) return ORA_UNKNOWN;
+    me->size = *size;
+    me->ws = *worksize;
+    me->docycle = docycle_method;
+    me->vertical1d = vertical1d;
+    me->xzflow = xzflow;
+    me->nthreads = nthreads > 0 ? nthreads : 1;
+    me->nI = size->IUB - size->ILB + 1;
+    me->nJ = size->JUB - size->JLB + 1;
+    me->nK = size->KUB - size->KLB + 1;
+    me->n2 = me->nI * me->nJ;
+    me->n3 = me->n2 * me->nK;
+    me->g = *geom;
+    arrs[0] = &me->KhX; arrs[1] = &me->KhY; arrs[2] = &me->KzZ;
+    arrs[3] = &me->DHU; arrs[4] = &me->DHV; arrs[5] = &me->DVC;
+    arrs[6] = &me->D; arrs[7] = &me->E; arrs[8] = &me->F; arrs[9] = &me->TI;
+    arrs[10] = &me->xC; arrs[11] = &me->xD; arrs[12] = &me->xE; arrs[13] = &me->xF;
+    arrs[14] = &me->yC; arrs[15] = &me->yD; arrs[16] = &me->yE; arrs[17] = &me->yF;
+    arrs[18] = &me->zC; arrs[19] = &me->zD; arrs[20] = &me->zE; arrs[21] = &me->zF;
+    for (a = 0; a < 22; a++) {
+        *arrs[a] = alloc_fill(me->n3, NULL_REAL);
+        if (!*arrs[a]) { ora_kill(me); return ORA_UNKNOWN; }
+    }
+    {
+        /* VEC%W/G(IJKLB:IJKUB), one pair per thread (AD:624-637) */
+        int lo = size->ILB, hi = size->IUB;
+        if (size->JLB < lo) lo = size->JLB;
+        if (size->KLB < lo) lo = size->KLB;
+        if (size->JUB > hi) hi = size->JUB;
+        if (size->KUB > hi) hi = size->KUB;
+        me->vec_len = (long)hi + 2;  /* indexed directly by k (lo >= 0) */
+        (void)lo;
+        me->vecW = alloc_fill(me->vec_len * me->nthreads, NULL_REAL);
+        me->vecG = alloc_fill(me->vec_len * me->nthreads, NULL_REAL);
+        if (!me->vecW || !me->vecG) { ora_kill(me); return ORA_UNKNOWN; }
+    }
+    me->ext_DTProp = NULL_REAL;
+    me->last_calc = -HUGE_VAL;
+    me->first_call = 1;
+    *out = me;
+    return ORA_SUCCESS;
+}
+
+void ora_kill(ora_instance *me)
+{
+    if (!me) return;
+    free(me->KhX); free(me->KhY); free(me->KzZ);
+    free(me->DHU); free(me->DHV); free(me->DVC);
+    free(me->D); free(me->E); free(me->F); free(me->TI);
+    free(me->xC); free(me->xD); free(me->xE); free(me->xF);
+    free(me->yC); free(me->yD); free(me->yE); free(me->yF);
+    free(me->zC); free(me->zD); free(me->zE); free(me->zF);
+    free(me->vecW); free(me->vecG);
+    free(me);
+}
+
+const double *ora_internal(const ora_instance *me, const char *name)
+{
+    if (!strcmp(name, "D")) return me->D;
+    if (!strcmp(name, "E")) return me->E;
+    if (!strcmp(name, "F")) return me->F;
+    if (!strcmp(name, "TI")) return me->TI;
+    if (!strcmp(name, "KhX")) return me->KhX;
+    if (!strcmp(name, "KhY")) return me->KhY;
+    if (!strcmp(name, "KzZ")) return me->KzZ;
+    return NULL;
+}
+
+static void fill(double *p, long n, double v)
+{
+    long q;
+#pragma omp parallel for schedule(static)
+    for (q = 0; q < n; q++) p[q] = v;
+}
+
+/* ------------------------------------------------------------------------ */
+/* AD:5746-5835 Set_Internal_State (open-boundary part: ReferenceProp is
+ * never associated here). */
+static void set_internal_state(ora_instance *me, const ora_args *a)
+{
+    if (me->ext_DTProp != a->DTProp || me->last_calc != a->Now) {
+        me->st_vadv = me->st_hadv = me->st_vdif = me->st_hdif = 1;
+    } else {
+        me->st_hadv = (me->ext_AdvMethodH != a->AdvMethodH ||
+                       me->ext_TVDH != a->TVDLimitationH ||
+                       me->ext_AdvMethodH == P2_TVD);
+        me->st_vadv = (me->ext_AdvMethodV != a->AdvMethodV ||
+                       me->ext_TVDV != a->TVDLimitationV ||
+                       me->ext_AdvMethodV == P2_TVD);
+        me->st_vdif = (me->ext_SchmidtCoef_V != a->SchmidtCoef_V ||
+                       me->ext_SchmidtBackground_V != a->SchmidtBackground_V ||
+                       a->NullDif);
+        me->st_hdif = (me->ext_Schmidt_H != a->schmidt_H || a->NullDif);
+    }
+}
+
+/* AD:2364-2443 Convert_Dif_Vertical */
+static void convert_dif_vertical(ora_instance *me, const ora_args *a)
+{
+    const ora_size3d *w = &me->ws;
+    int k;
+    for (k = w->KLB + 1; k <= w->KUB; k++) {
+        int j;
+#pragma omp parallel for schedule(static)
+        for (j = w->JLB; j <= w->JUB; j++) {
+            int i;
+            for (i = w->ILB; i <= w->IUB; i++) {
+                long x = I3(me, i, j, k);
+                if (a->ComputeFacesW3D[x] == 1) {
+                    me->KzZ[x] = a->SchmidtCoef_V * a->Diff_V[x] + a->SchmidtBackground_V;
+                    if (a->NullDif && a->Wflux_Z[x] == 0.0) me->KzZ[x] = 0.0;
+                }
+            }
+        }
+    }
+}
+
+/* AD:2453-2595 Convert_Visc_Dif_Horizontal (and the _opt variant,
+ * AD:2598-2675, which is the same without NoDifFlux/NullDif). */
+static void convert_visc_dif_horizontal(ora_instance *me, const ora_args *a, int nulldif)
+{
+    const ora_size3d *w = &me->ws;
+    const double *DUX = me->g.DUX, *DVY = me->g.DVY, *nu = a->Visc_H;
+    int k;
+    for (k = w->KLB; k <= w->KUB; k++) {
+        int j;
+#pragma omp parallel for schedule(static)
+        for (j = w->JLB; j <= w->JUB; j++) {
+            int i;
+            for (i = w->ILB; i <= w->IUB; i++) {
+                long x = I3(me, i, j, k);
+                if (a->ComputeFacesU3D[x] == 1) {
+                    me->KhX[x] = a->schmidt_H *
+                                 (nu[x] * DUX[I2(me, i, j - 1)] +
+                                  nu[I3(me, i, j - 1, k)] * DUX[I2(me, i, j)]) /
+                                 (DUX[I2(me, i, j)] + DUX[I2(me, i, j - 1)]);
+                }
+            }
+        }
+    }
+    for (k = w->KLB; k <= w->KUB; k++) {
+        int j;
+#pragma omp parallel for schedule(static)
+        for (j = w->JLB; j <= w->JUB; j++) {
+            int i;
+            for (i = w->ILB; i <= w->IUB; i++) {
+                long x = I3(me, i, j, k);
+                if (a->ComputeFacesV3D[x] == 1) {
+                    me->KhY[x] = a->schmidt_H *
+                                 (nu[x] * DVY[I2(me, i - 1, j)] +
+                                  nu[I3(me, i - 1, j, k)] * DVY[I2(me, i, j)]) /
+                                 (DVY[I2(me, i, j)] + DVY[I2(me, i - 1, j)]);
+                }
+            }
+        }
+    }
+    if (nulldif) {
+        long x;
+        for (k = w->KLB; k <= w->KUB; k++) {
+            int j, i;
+            for (j = w->JLB; j <= w->JUB; j++)
+                for (i = w->ILB; i <= w->IUB; i++) {
+                    x = I3(me, i, j, k);
+                    if (a->ComputeFacesU3D[x] == 1 && a->Wflux_X[x] == 0.0) me->KhX[x] = 0.0;
+                }
+            for (j = w->JLB; j <= w->JUB; j++)
+                for (i = w->ILB; i <= w->IUB; i++) {
+                    x = I3(me, i, j, k);
+                    if (a->ComputeFacesV3D[x] == 1 && a->Wflux_Y[x] == 0.0) me->KhY[x] = 0.0;
+                }
+        }
+    }
+}
+
+/* AD:1514-1559 Compute_DifH_Constants.  The reference loops over Size; at
+ * the outer halo the DZX/DZY index would leave the array, which only happens
+ * when a halo face is flagged computable -- never in MOHID's maps. */
+static void compute_difh_constants(ora_instance *me, const ora_args *a)
+{
+    const ora_size3d *s = &me->size;
+    int k;
+#pragma omp parallel for schedule(static)
+    for (k = s->KLB; k <= s->KUB; k++) {
+        int j, i;
+        for (j = s->JLB; j <= s->JUB; j++)
+            for (i = s->ILB; i <= s->IUB; i++) {
+                long x = I3(me, i, j, k);
+                if (a->ComputeFacesV3D[x] == 1 && i - 1 >= s->ILB)
+                    me->DHV[x] = me->KhY[x] * me->g.AreaV[x] / me->g.DZY[I2(me, i - 1, j)];
+            }
+    }
+#pragma omp parallel for schedule(static)
+    for (k = s->KLB; k <= s->KUB; k++) {
+        int j, i;
+        for (j = s->JLB; j <= s->JUB; j++)
+            for (i = s->ILB; i <= s->IUB; i++) {
+                long x = I3(me, i, j, k);
+                if (a->ComputeFacesU3D[x] == 1 && j - 1 >= s->JLB)
+                    me->DHU[x] = me->KhX[x] * me->g.AreaU[x] / me->g.DZX[I2(me, i, j - 1)];
+            }
+    }
+}
+
+/* AD:1563-1619 Compute_DifV_Constants (SmallDepths absent) */
+static void compute_difv_constants(ora_instance *me, const ora_args *a)
+{
+    const ora_size3d *w = &me->ws;
+    int k;
+#pragma omp parallel for schedule(static)
+    for (k = w->KLB; k <= w->KUB; k++) {
+        int j, i;
+        for (j = w->JLB; j <= w->JUB; j++)
+            for (i = w->ILB; i <= w->IUB; i++) {
+                long x = I3(me, i, j, k);
+                if (a->ComputeFacesW3D[x] == 1) {
+                    double auxk = me->KzZ[x] * me->g.DUX[I2(me, i, j)] * me->g.DVY[I2(me, i, j)];
+                    me->DVC[x] = auxk / me->g.DZZ[I3(me, i, j, k - 1)];
+                }
+            }
+    }
+}
+
+/* AD:3966-4021 VolumeVariation */
+static void volume_variation(ora_instance *me, const ora_args *a, const double *P)
+{
+    const ora_size3d *w = &me->ws;
+    int k;
+    for (k = w->KLB; k <= w->KUB; k++) {
+        int j;
+#pragma omp parallel for schedule(static)
+        for (j = w->JLB; j <= w->JUB; j++) {
+            int i;
+            for (i = w->ILB; i <= w->IUB; i++) {
+                long x = I3(me, i, j, k);
+                if (a->OpenPoints3D[x] == 1) {
+                    me->TI[x] = P[x] * (a->VolumeZOld[x] / a->VolumeZ[x]);
+                    if (k == w->KUB) {
+                        double dtv = a->DTProp / a->VolumeZ[x];
+                        me->E[x] = 1.0 + dtv * a->Wflux_Z[I3(me, i, j, k + 1)];
+                    } else if (a->Optimize) {
+                        me->E[x] = 1.0;
+                    }
+                } else {
+                    me->TI[x] = P[x];
+                }
+            }
+        }
+    }
+}
+
+/* AD:5156-5214 HorizontalDiffusionXX / AD:5216-5265 HorizontalDiffusionXX2 */
+static void horizontal_diffusion_xx(ora_instance *me, const ora_args *a, const double *P)
+{
+    const ora_size3d *w = &me->ws;
+    double dt = a->DTProp;
+    int k;
+#pragma omp parallel for schedule(static)
+    for (k = w->KLB; k <= w->KUB; k++) {
+        int j, i;
+        for (j = w->JLB; j <= w->JUB; j++)
+            for (i = w->ILB; i <= w->IUB; i++) {
+                long x = I3(me, i, j, k), xm = I3(me, i, j - 1, k);
+                if (a->ComputeFacesU3D[x] != 1) continue;
+                if (a->Optimize) {
+                    double grad = P[x] - P[xm];
+                    double auxj = me->DHU[x] * grad * dt;
+                    me->TI[xm] = me->TI[xm] + auxj / a->VolumeZ[xm];
+                    me->TI[x] = me->TI[x] - auxj / a->VolumeZ[x];
+                } else {
+                    double auxj = me->KhX[x] * me->g.AreaU[x] / me->g.DZX[I2(me, i, j - 1)];
+                    me->TI[xm] = me->TI[xm] + auxj * dt / a->VolumeZ[xm] * (P[x] - P[xm]);
+                    me->TI[x] = me->TI[x] - auxj * dt / a->VolumeZ[x] * (P[x] - P[xm]);
+                }
+            }
+    }
+}
+
+/* AD:5313-5365 HorizontalDiffusionYY / AD:5267-5309 HorizontalDiffusionYY2 */
+static void horizontal_diffusion_yy(ora_instance *me, const ora_args *a, const double *P)
+{
+    const ora_size3d *w = &me->ws;
+    double dt = a->DTProp;
+    int k;
+    for (k = w->KLB; k <= w->KUB; k++) {
+        int j;
+#pragma omp parallel for schedule(static)
+        for (j = w->JLB; j <= w->JUB; j++) {
+            int i;
+            for (i = w->ILB; i <= w->IUB; i++) {
+                long x = I3(me, i, j, k), xm = I3(me, i - 1, j, k);
+                if (a->ComputeFacesV3D[x] != 1) continue;
+                if (a->Optimize) {
+                    double grad = P[x] - P[xm];
+                    double auxi = me->DHV[x] * dt * grad;
+                    me->TI[xm] = me->TI[xm] + auxi / a->VolumeZ[xm];
+                    me->TI[x] = me->TI[x] - auxi / a->VolumeZ[x];
+                } else {
+                    double auxi = me->KhY[x] * me->g.AreaV[x] / me->g.DZY[I2(me, i - 1, j)];
+                    me->TI[xm] = me->TI[xm] + auxi * dt / a->VolumeZ[xm] * (P[x] - P[xm]);
+                    me->TI[x] = me->TI[x] - auxi * dt / a->VolumeZ[x] * (P[x] - P[xm]);
+                }
+            }
+        }
+    }
+}
+
+/* AD:4368-4540 HorizontalAdvectionXX (+ _Explicit, AD:4544-4582) */
+static int horizontal_advection_xx(ora_instance *me, const ora_args *a, const double *P)
+{
+    const ora_size3d *w = &me->ws, *s = &me->size;
+    double dt = a->DTProp;
+    long sj = me->nI;      /* stride of a j-line */
+    int rc = 0, k;
+    if (me->st_hadv) {
+        for (k = w->KLB; k <= w->KUB; k++) {
+            int i;
+#pragma omp parallel for schedule(static) reduction(| : rc)
+            for (i = w->ILB; i <= w->IUB; i++) {
+                long b3 = I3(me, i, s->JLB, k), b2 = I2(me, i, s->JLB);
+                if (a->Optimize)
+                    adv1d_sb2_strided(w->JLB + 1, w->JUB + 1, dt, me->g.DUX + b2, sj,
+                                      P + b3, a->Wflux_X + b3, a->VolumeZ + b3,
+                                      a->OpenPoints3D + b3, sj, me->xD + b3, me->xE + b3);
+                else
+                    rc |= adv1d_v2_strided(w->JLB + 1, w->JUB + 1, dt, me->g.DUX + b2, sj,
+                                           P + b3, a->Wflux_X + b3, a->VolumeZ + b3,
+                                           a->OpenPoints3D + b3, sj,
+                                           me->xC + b3, me->xD + b3, me->xE + b3, me->xF + b3,
+                                           a->AdvMethodH, a->TVDLimitationH,
+                                           a->VolumeRelMax, a->Upwind2H);
+            }
+        }
+    }
+    if (a->ImpExp_AdvXX != EXPLICIT) return rc ? rc : ORA_ERR_UNSUPPORTED;
+#pragma omp parallel for schedule(static)
+    for (k = w->KLB; k <= w->KUB; k++) {
+        int j, i;
+        for (j = w->JLB; j <= w->JUB; j++)
+            for (i = w->ILB; i <= w->IUB; i++) {
+                long x = I3(me, i, j, k), xm = I3(me, i, j - 1, k);
+                double f;
+                if (a->ComputeFacesU3D[x] != 1) continue;
+                if (a->Optimize)
+                    f = me->xD[x] * P[xm] + me->xE[x] * P[x];
+                else
+                    f = me->xC[x] * P[I3(me, i, j - 2, k)] + me->xD[x] * P[xm] +
+                        me->xE[x] * P[x] + me->xF[x] * P[I3(me, i, j + 1, k)];
+                me->TI[x] = me->TI[x] + f * dt / a->VolumeZ[x];
+                me->TI[xm] = me->TI[xm] - f * dt / a->VolumeZ[xm];
+            }
+    }
+    return rc;
+}
+
+/* AD:4739-4910 HorizontalAdvectionYY (+ _Explicit, AD:4914-4953) */
+static int horizontal_advection_yy(ora_instance *me, const ora_args *a, const double *P)
+{
+    const ora_size3d *w = &me->ws, *s = &me->size;
+    double dt = a->DTProp;
+    int rc = 0, k;
+    if (me->st_hadv) {
+        for (k = w->KLB; k <= w->KUB; k++) {
+            int j;
+#pragma omp parallel for schedule(static) reduction(| : rc)
+            for (j = w->JLB; j <= w->JUB; j++) {
+                long b3 = I3(me, s->ILB, j, k), b2 = I2(me, s->ILB, j);
+                if (a->Optimize)
+                    adv1d_sb2_strided(w->ILB + 1, w->IUB + 1, dt, me->g.DVY + b2, 1,
+                                      P + b3, a->Wflux_Y + b3, a->VolumeZ + b3,
+                                      a->OpenPoints3D + b3, 1, me->yD + b3, me->yE + b3);
+                else
+                    rc |= adv1d_v2_strided(w->ILB + 1, w->IUB + 1, dt, me->g.DVY + b2, 1,
+                                           P + b3, a->Wflux_Y + b3, a->VolumeZ + b3,
+                                           a->OpenPoints3D + b3, 1,
+                                           me->yC + b3, me->yD + b3, me->yE + b3, me->yF + b3,
+                                           a->AdvMethodH, a->TVDLimitationH,
+                                           a->VolumeRelMax, a->Upwind2H);
+            }
+        }
+    }
+    if (a->ImpExp_AdvYY != EXPLICIT) return rc ? rc : ORA_ERR_UNSUPPORTED;
+    for (k = w->KLB; k <= w->KUB; k++) {
+        int j;
+#pragma omp parallel for schedule(static)
+        for (j = w->JLB; j <= w->JUB; j++) {
+            int i;
+            for (i = w->ILB; i <= w->IUB; i++) {
+                long x = I3(me, i, j, k), xm = I3(me, i - 1, j, k);
+                double f;
+                if (a->ComputeFacesV3D[x] != 1) continue;
+                if (a->Optimize)
+                    f = me->yD[x] * P[xm] + me->yE[x] * P[x];
+                else
+                    f = me->yC[x] * P[I3(me, i - 2, j, k)] + me->yD[x] * P[xm] +
+                        me->yE[x] * P[x] + me->yF[x] * P[I3(me, i + 1, j, k)];
+                me->TI[x] = me->TI[x] + f * dt / a->VolumeZ[x];
+                me->TI[xm] = me->TI[xm] - f * dt / a->VolumeZ[xm];
+            }
+        }
+    }
+    return rc;
+}
+
+/* AD:2708-2784 VerticalDiffusion (theta-scheme, non-optimised path) */
+static void vertical_diffusion(ora_instance *me, const ora_args *a, const double *P)
+{
+    const ora_size3d *w = &me->ws;
+    double dt = a->DTProp, th = a->ImpExp_DifV;
+    int k;
+    for (k = w->KLB; k <= w->KUB; k++) {
+        int j;
+#pragma omp parallel for schedule(static)
+        for (j = w->JLB; j <= w->JUB; j++) {
+            int i;
+            for (i = w->ILB; i <= w->IUB; i++) {
+                long x = I3(me, i, j, k), xm = I3(me, i, j, k - 1);
+                double auxk, aux1, aux2;
+                if (a->ComputeFacesW3D[x] != 1) continue;
+                auxk = me->KzZ[x] * me->g.DUX[I2(me, i, j)] * me->g.DVY[I2(me, i, j)] /
+                       me->g.DZZ[xm];
+                aux1 = auxk * dt / a->VolumeZ[xm];
+                aux2 = auxk * dt / a->VolumeZ[x];
+                me->E[xm] = me->E[xm] + aux1 * th;
+                me->F[xm] = me->F[xm] - aux1 * th;
+                me->TI[xm] = me->TI[xm] + aux1 * (P[x] - P[xm]) * (1.0 - th);
+                me->D[x] = me->D[x] - aux2 * th;
+                me->E[x] = me->E[x] + aux2 * th;
+                me->TI[x] = me->TI[x] - aux2 * (P[x] - P[xm]) * (1.0 - th);
+            }
+        }
+    }
+}
+
+/* AD:2788-2937 VerticalDiffusion2 (optimised path) */
+static void vertical_diffusion2(ora_instance *me, const ora_args *a, const double *P)
+{
+    const ora_size3d *w = &me->ws;
+    double dt = a->DTProp;
+    int implicit = a->ImpExp_DifV > 0.0;
+    if (me->docycle == 1) {
+        int j;
+#pragma omp parallel for schedule(static)
+        for (j = w->JLB; j <= w->JUB; j++) {
+            int i, k;
+            for (i = w->ILB; i <= w->IUB; i++) {
+                int kb;
+                double vb, pb = 0.0;
+                if (a->ComputeFacesW3D[I3(me, i, j, w->KUB)] != 1) continue;
+                kb = me->g.KFloorZ[I2(me, i, j)];
+                vb = a->VolumeZ[I3(me, i, j, kb)];
+                if (!implicit) pb = P[I3(me, i, j, kb)];
+                for (k = kb + 1; k <= w->KUB; k++) {
+                    long x = I3(me, i, j, k), xm = I3(me, i, j, k - 1);
+                    double auxk = me->DVC[x] * dt;
+                    double aux1 = auxk / vb;
+                    double aux2 = auxk / a->VolumeZ[x];
+                    vb = a->VolumeZ[x];
+                    if (implicit) {
+                        me->D[x] = me->D[x] - aux2;
+                        me->E[x] = me->E[x] + aux2;
+                        me->E[xm] = me->E[xm] + aux1;
+                        me->F[xm] = me->F[xm] - aux1;
+                    } else {
+                        double grad = P[x] - pb;
+                        me->TI[x] = me->TI[x] - aux2 * grad;
+                        me->TI[xm] = me->TI[xm] + aux1 * grad;
+                        pb = P[x];
+                    }
+                }
+            }
+        }
+    } else {
+        int k;
+        for (k = w->KLB; k <= w->KUB; k++) {
+            int j;
+#pragma omp parallel for schedule(static)
+            for (j = w->JLB; j <= w->JUB; j++) {
+                int i;
+                for (i = w->ILB; i <= w->IUB; i++) {
+                    long x = I3(me, i, j, k), xm = I3(me, i, j, k - 1);
+                    double auxk, aux1, aux2;
+                    if (a->ComputeFacesW3D[x] != 1) continue;
+                    auxk = me->DVC[x] * dt;
+                    aux1 = auxk / a->VolumeZ[xm];
+                    aux2 = auxk / a->VolumeZ[x];
+                    if (implicit) {
+                        me->D[x] = me->D[x] - aux2;
+                        me->E[x] = me->E[x] + aux2;
+                        me->E[xm] = me->E[xm] + aux1;
+                        me->F[xm] = me->F[xm] - aux1;
+                    } else {
+                        me->TI[x] = me->TI[x] - aux2 * (P[x] - P[xm]);
+                        me->TI[xm] = me->TI[xm] + aux1 * (P[x] - P[xm]);
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* AD:2941-3140 VerticalAdvection (+ _ExplicitScheme, AD:3144-3207) */
+static int vertical_advection(ora_instance *me, const ora_args *a, const double *P)
+{
+    const ora_size3d *w = &me->ws, *s = &me->size;
+    double dt = a->DTProp;
+    long sk = me->n2;
+    int rc = 0;
+    if (me->st_vadv) {
+        int j;
+#pragma omp parallel for schedule(static) reduction(| : rc)
+        for (j = w->JLB; j <= w->JUB; j++) {
+            int i;
+            for (i = w->ILB; i <= w->IUB; i++) {
+                long b3 = I3(me, i, j, s->KLB);
+                if (a->OpenPoints3D[I3(me, i, j, w->KUB)] != 1) continue;
+                if (a->Optimize)
+                    adv1d_sb2_strided(w->KLB + 1, w->KUB + 1, dt, me->g.DWZ + b3, sk,
+                                      P + b3, a->Wflux_Z + b3, a->VolumeZ + b3,
+                                      a->OpenPoints3D + b3, sk, me->zD + b3, me->zE + b3);
+                else
+                    rc |= adv1d_v2_strided(w->KLB + 1, w->KUB + 1, dt, me->g.DWZ + b3, sk,
+                                           P + b3, a->Wflux_Z + b3, a->VolumeZ + b3,
+                                           a->OpenPoints3D + b3, sk,
+                                           me->zC + b3, me->zD + b3, me->zE + b3, me->zF + b3,
+                                           a->AdvMethodV, a->TVDLimitationV,
+                                           a->VolumeRelMax, a->Upwind2V);
+            }
+        }
+    }
+    if (a->ImpExp_AdvV == EXPLICIT) {
+        if (a->Optimize) {
+            if (me->docycle == 1) {
+                int j;
+#pragma omp parallel for schedule(static)
+                for (j = w->JLB; j <= w->JUB; j++) {
+                    int i, k;
+                    for (i = w->ILB; i <= w->IUB; i++) {
+                        int kb;
+                        double vb, pb;
+                        if (a->ComputeFacesW3D[I3(me, i, j, w->KUB)] != 1) continue;
+                        kb = me->g.KFloorZ[I2(me, i, j)];
+                        vb = a->VolumeZ[I3(me, i, j, kb)];
+                        pb = P[I3(me, i, j, kb)];
+                        for (k = kb + 1; k <= w->KUB; k++) {
+                            long x = I3(me, i, j, k), xm = I3(me, i, j, k - 1);
+                            double f = (me->zD[x] * pb + me->zE[x] * P[x]) * dt;
+                            pb = P[x];
+                            me->TI[xm] = me->TI[xm] - f / vb;
+                            me->TI[x] = me->TI[x] + f / a->VolumeZ[x];
+                            vb = a->VolumeZ[x];
+                        }
+                    }
+                }
+            } else {
+                int k;
+                for (k = w->KLB; k <= w->KUB; k++) {
+                    int j;
+#pragma omp parallel for schedule(static)
+                    for (j = w->JLB; j <= w->JUB; j++) {
+                        int i;
+                        for (i = w->ILB; i <= w->IUB; i++) {
+                            long x = I3(me, i, j, k), xm = I3(me, i, j, k - 1);
+                            double f;
+                            if (a->ComputeFacesW3D[x] != 1) continue;
+                            f = (me->zD[x] * P[xm] + me->zE[x] * P[x]) * dt;
+                            me->TI[xm] = me->TI[xm] - f / a->VolumeZ[xm];
+                            me->TI[x] = me->TI[x] + f / a->VolumeZ[x];
+                        }
+                    }
+                }
+            }
+        } else {
+            int k;
+            for (k = w->KLB; k <= w->KUB; k++) {
+                int j;
+#pragma omp parallel for schedule(static)
+                for (j = w->JLB; j <= w->JUB; j++) {
+                    int i;
+                    for (i = w->ILB; i <= w->IUB; i++) {
+                        long x = I3(me, i, j, k), xm = I3(me, i, j, k - 1);
+                        double f;
+                        if (a->ComputeFacesW3D[x] != 1) continue;
+                        f = me->zC[x] * P[I3(me, i, j, k - 2)] + me->zD[x] * P[xm] +
+                            me->zE[x] * P[x] + me->zF[x] * P[I3(me, i, j, k + 1)];
+                        me->TI[xm] = me->TI[xm] - f * dt / a->VolumeZ[xm];
+                        me->TI[x] = me->TI[x] + f * dt / a->VolumeZ[x];
+                    }
+                }
+            }
+        }
+    } else if (a->ImpExp_AdvV == IMPLICIT) {
+        if (me->docycle == 1) {
+            int j;
+#pragma omp parallel for schedule(static)
+            for (j = w->JLB; j <= w->JUB; j++) {
+                int i, k;
+                for (i = w->ILB; i <= w->IUB; i++) {
+                    int kb;
+                    double vb;
+                    if (a->ComputeFacesW3D[I3(me, i, j, w->KUB)] != 1) continue;
+                    kb = me->g.KFloorZ[I2(me, i, j)];
+                    vb = a->VolumeZ[I3(me, i, j, kb)];
+                    for (k = kb + 1; k <= w->KUB; k++) {
+                        long x = I3(me, i, j, k), xm = I3(me, i, j, k - 1);
+                        double dt1 = dt / vb;
+                        double dt2 = dt / a->VolumeZ[x];
+                        vb = a->VolumeZ[x];
+                        me->D[x] = me->D[x] - me->zD[x] * dt2;
+                        me->E[x] = me->E[x] - me->zE[x] * dt2;
+                        me->E[xm] = me->E[xm] + me->zD[x] * dt1;
+                        me->F[xm] = me->F[xm] + me->zE[x] * dt1;
+                    }
+                }
+            }
+        } else {
+            int k;
+            for (k = w->KLB; k <= w->KUB; k++) {
+                int j;
+#pragma omp parallel for schedule(static)
+                for (j = w->JLB; j <= w->JUB; j++) {
+                    int i;
+                    for (i = w->ILB; i <= w->IUB; i++) {
+                        long x = I3(me, i, j, k), xm = I3(me, i, j, k - 1);
+                        double dt1, dt2;
+                        if (a->ComputeFacesW3D[x] != 1) continue;
+                        dt1 = dt / a->VolumeZ[xm];
+                        dt2 = dt / a->VolumeZ[x];
+                        me->D[x] = me->D[x] - me->zD[x] * dt2;
+                        me->E[x] = me->E[x] - me->zE[x] * dt2;
+                        me->E[xm] = me->E[xm] + me->zD[x] * dt1;
+                        me->F[xm] = me->F[xm] + me->zE[x] * dt1;
+                    }
+                }
+            }
+        }
+    } else {
+        return ORA_ERR_OPTION;  /* AD:3131 stop ERR01 */
+    }
+    return rc;
+}
+
+/* AD:1624-1922 AdvectionDiffusionIteration (explicit-horizontal branch) */
+static int iteration(ora_instance *me, const ora_args *a, double *P)
+{
+    const ora_size3d *w = &me->ws;
+    long n3 = me->n3;
+    int rc = 0;
+
+    if (a->Optimize) {
+        if (a->FirstProperty) {
+            fill(me->TI, n3, 0.0);
+            fill(me->E, n3, 1.0);
+        }
+        fill(me->D, n3, 0.0);
+        fill(me->F, n3, 0.0);
+    } else {
+        fill(me->D, n3, 0.0);
+        fill(me->E, n3, 1.0);
+        fill(me->F, n3, 0.0);
+        fill(me->TI, n3, 0.0);
+    }
+    if (me->st_hadv) {
+        if (a->Optimize) {
+            if (a->FirstProperty) {
+                fill(me->xD, n3, 0.0); fill(me->xE, n3, 0.0);
+                fill(me->yD, n3, 0.0); fill(me->yE, n3, 0.0);
+            }
+        } else {
+            fill(me->xC, n3, 0.0); fill(me->xD, n3, 0.0);
+            fill(me->xE, n3, 0.0); fill(me->xF, n3, 0.0);
+            fill(me->yC, n3, 0.0); fill(me->yD, n3, 0.0);
+            fill(me->yE, n3, 0.0); fill(me->yF, n3, 0.0);
+        }
+    }
+
+    volume_variation(me, a, P);
+    if (!me->vertical1d) {
+        horizontal_diffusion_xx(me, a, P);
+        if (!me->xzflow) horizontal_diffusion_yy(me, a, P);
+        rc = horizontal_advection_xx(me, a, P);
+        if (rc) return rc;
+        if (!me->xzflow) {
+            rc = horizontal_advection_yy(me, a, P);
+            if (rc) return rc;
+        }
+    }
+    if (w->KUB > 1) {
+        if (a->Optimize) vertical_diffusion2(me, a, P);
+        else vertical_diffusion(me, a, P);
+        if (!me->vertical1d) {
+            rc = vertical_advection(me, a, P);
+            if (rc) return rc;
+        }
+    }
+    /* AD:1753: TI = null_real on land points */
+    {
+        long q;
+#pragma omp parallel for schedule(static)
+        for (q = 0; q < n3; q++)
+            if (a->LandPoints3D[q] == 1) me->TI[q] = NULL_REAL;
+    }
+    /* AD:1858-1867 THOMASZ_NewType2 over WorkSize, OpenMP over j */
+    {
+        long nI = me->nI, nJ = me->nJ;
+        int j;
+        /* one VEC pair per thread: never run more threads than VEC slots
+         * (the reference's VEC(TID) overrun, AD:624-637) */
+#pragma omp parallel for schedule(dynamic, 1) num_threads(me->nthreads)
+        for (j = w->JLB; j <= w->JUB; j++) {
+            int tid = 0, i;
+            double *W, *G;
+#ifdef _OPENMP
+            tid = omp_get_thread_num();
+            if (tid >= me->nthreads) tid = me->nthreads - 1;
+#endif
+            W = me->vecW + (long)tid * me->vec_len;
+            G = me->vecG + (long)tid * me->vec_len;
+            for (i = w->ILB; i <= w->IUB; i++)
+                if (a->WaterPoints3D[I3(me, i, j, w->KUB)] == 1)
+                    thomasz_column(me, &me->size, w, nI, nJ, i, j, me->D, me->E,
+                                   me->F, me->TI, P, W, G);
+        }
+    }
+    return 0;
+}
+
+/* AD:1108-1509 AdvectionDiffusion */
+int ora_advection_diffusion(ora_instance *me, const ora_args *a, double *prop)
+{
+    int rc, optimize;
+    ora_args aa;
+    if (!me) return ORA_OFF_ERR;
+    if (!a || !prop) return ORA_ERR_BADARG;
+    /* AD:1231-1239 */
+    if ((a->ImpExp_AdvXX == IMPLICIT || a->ImpExp_AdvYY == IMPLICIT) &&
+        (a->AdvMethodH == UPWIND2 || a->AdvMethodH == UPWIND3))
+        return ORA_ERR_OPTION;
+    if (a->ImpExp_AdvV == IMPLICIT && (a->AdvMethodV == UPWIND2 || a->AdvMethodV == UPWIND3))
+        return ORA_ERR_OPTION;
+    /* AD:1340-1349 */
+    if (a->ImpExp_DifH != 0.0) return ORA_ERR_OPTION;
+    if (a->ImpExp_AdvXX == IMPLICIT && a->ImpExp_AdvYY == IMPLICIT) return ORA_ERR_OPTION;
+    /* implicit horizontal advection needs THOMAS_3D: not restated */
+    if (a->ImpExp_AdvXX != EXPLICIT || a->ImpExp_AdvYY != EXPLICIT) return ORA_ERR_UNSUPPORTED;
+
+    aa = *a;
+    optimize = a->Optimize && !me->vertical1d;   /* AD:1419 */
+    aa.Optimize = optimize;
+    set_internal_state(me, &aa);
+    me->ext_DTProp = a->DTProp;
+    me->ext_AdvMethodH = a->AdvMethodH;
+    me->ext_TVDH = a->TVDLimitationH;
+    me->ext_SchmidtCoef_V = a->SchmidtCoef_V;
+    me->ext_SchmidtBackground_V = a->SchmidtBackground_V;
+    me->ext_Schmidt_H = a->schmidt_H;
+
+    if (me->st_vdif) {
+        if (!optimize || a->FirstProperty) convert_dif_vertical(me, &aa);
+    }
+    if (me->st_hdif) {
+        if (optimize) {
+            if (a->FirstProperty) convert_visc_dif_horizontal(me, &aa, 0);
+        } else {
+            convert_visc_dif_horizontal(me, &aa, a->NullDif);
+        }
+    }
+    if (me->st_vadv) {
+        me->ext_AdvMethodV = a->AdvMethodV;
+        me->ext_TVDV = a->TVDLimitationV;
+        if (optimize) {
+            if (a->FirstProperty) { fill(me->zD, me->n3, 0.0); fill(me->zE, me->n3, 0.0); }
+        } else {
+            fill(me->zC, me->n3, 0.0); fill(me->zD, me->n3, 0.0);
+            fill(me->zE, me->n3, 0.0); fill(me->zF, me->n3, 0.0);
+        }
+    }
+    if (optimize && a->FirstProperty) {
+        compute_difh_constants(me, &aa);
+        compute_difv_constants(me, &aa);
+    }
+    rc = iteration(me, &aa, prop);
+    me->last_calc = a->Now;
+    return rc;
+}

@@ -1,0 +1,118 @@
+/*
+ * CPU oracle for MOHID Water's tracer-transport step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * Fortran algorithm (ModuleAdvectionDiffusion + the ModuleFunctions routines
+ * it calls), written in the reference's own loop order so that it reproduces
+ * the reference's floating-point operation order.  It is used by the tests,
+ * by the smoke check and as the CPU baseline of bench.py -- never by the
+ * product path in mohid_amd/.
+ *
+ * Precision: the reference is built with default-real-8
+ * (Solutions/mohid-in-linux/compile_mohid.sh:185), so every `real` of the
+ * reference is a double here.
+ *
+ * Array layout: Fortran column-major with the reference's halo, i fastest.
+ * A 3-D array spans Size = WorkSize +/- 1 in every direction
+ * (Software/MOHIDBase2/ModuleGeometry.F90:815-816); a 2-D array spans the
+ * (i, j) part of Size.
+ */
+#ifndef MOHID_ADVDIFF_ORACLE_H
+#define MOHID_ADVDIFF_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* T_Size3D, Software/MOHIDBase1/ModuleGlobalData.F90:2044-2052 */
+typedef struct {
+    int32_t ILB, IUB, JLB, JUB, KLB, KUB;
+} ora_size3d;
+
+/* Status codes, ModuleGlobalData.F90:180-199 (SUCCESS_=0, OFF_ERR_=1,
+ * UNKNOWN_=-99); the reference `stop`s on invalid options -- here those become
+ * the ORA_ERR_* codes so a test can force them from the host. */
+enum {
+    ORA_SUCCESS = 0,
+    ORA_OFF_ERR = 1,
+    ORA_UNKNOWN = -99,
+    ORA_ERR_OPTION = 200,      /* AdvectionDiffusion ERR02/ERR03/ERR100/ERR200 */
+    ORA_ERR_METHOD = 201,      /* "This method is not valid to compute Advection1D" */
+    ORA_ERR_UNSUPPORTED = 202, /* branch not restated (discharges, open boundary, ...) */
+    ORA_ERR_BADARG = 203
+};
+
+/* Geometry the reference fetches inside AdvectionDiffusion
+ * (ModuleAdvectionDiffusion.F90:1353-1401). */
+typedef struct {
+    const double *DUX, *DVY, *DZX, *DZY;   /* 2-D */
+    const int32_t *KFloorZ;                /* 2-D */
+    const double *AreaU, *AreaV;           /* 3-D */
+    const double *DZZ, *DWZ;               /* 3-D */
+} ora_geometry;
+
+/* Arguments of AdvectionDiffusion (ModuleAdvectionDiffusion.F90:1108-1198),
+ * restricted to the branches this oracle restates. */
+typedef struct {
+    double schmidt_H, SchmidtCoef_V, SchmidtBackground_V;
+    int32_t AdvMethodH, TVDLimitationH, AdvMethodV, TVDLimitationV;
+    int32_t Upwind2H, Upwind2V;
+    double VolumeRelMax;
+    double DTProp;
+    double ImpExp_AdvV, ImpExp_DifV, ImpExp_AdvXX, ImpExp_AdvYY, ImpExp_DifH;
+    int32_t NullDif;
+    const double *Wflux_X, *Wflux_Y, *Wflux_Z;
+    const double *VolumeZOld, *VolumeZ;
+    const int32_t *OpenPoints3D, *LandPoints3D;
+    const int32_t *ComputeFacesU3D, *ComputeFacesV3D, *ComputeFacesW3D;
+    const int32_t *WaterPoints3D;
+    const double *Visc_H, *Diff_V;
+    int32_t Optimize, FirstProperty;
+    double Now;                    /* model time of this call (ModuleTime) */
+} ora_args;
+
+typedef struct ora_instance ora_instance;
+
+/* StartAdvectionDiffusion (ModuleAdvectionDiffusion.F90:400-498). */
+int ora_start(const ora_size3d *size, const ora_size3d *worksize,
+              int docycle_method, int vertical1d, int xzflow, int nthreads,
+              const ora_geometry *geom, ora_instance **out);
+/* AdvectionDiffusion (ModuleAdvectionDiffusion.F90:1108-1509): PROP in place. */
+int ora_advection_diffusion(ora_instance *me, const ora_args *args, double *prop);
+/* KillAdvectionDiffusion (ModuleAdvectionDiffusion.F90:5849). */
+void ora_kill(ora_instance *me);
+/* Read back an internal array after a call (names: "D","E","F","TI"). */
+const double *ora_internal(const ora_instance *me, const char *name);
+
+/* THOMASZ_NewType2 (ModuleFunctions.F90:4026-4123) on explicit arrays.
+ * W, G: scratch of length >= KUB+2 (per-thread VEC in the reference). */
+void ora_thomasz(const ora_size3d *size, const ora_size3d *ws,
+                 const double *D, const double *E, const double *F,
+                 const double *TI, const int32_t *waterpoints, double *res,
+                 double *W, double *G);
+
+/* ComputeAdvectionFace (ModuleFunctions.F90:10702-10894) for one face;
+ * returns ORA_ERR_METHOD where the reference stops. */
+int ora_advection_face(const double P[4], const double V[4], const double du[4],
+                       double dt, double Q, double volrelmax, int method, int tvd,
+                       int nearb, int upwind2, double CFace[4]);
+
+/* ComputeAdvection1D_V2 (ModuleFunctions.F90:10534-10591) on 1-based line
+ * arrays of length n (element m of the Fortran dummy is x[m-1]). */
+int ora_advection1d_v2(int n, int ilb, int iub, double dt, const double *du,
+                       const double *prop, const double *q, const double *v,
+                       const int32_t *cp, double *C, double *D, double *E,
+                       double *F, int method, int tvd, double volrelmax,
+                       int upwind2);
+/* ComputeAdvection1D_TVD_SuperBee_2 (ModuleFunctions.F90:10642-10699). */
+void ora_advection1d_tvd_superbee_2(int n, int ilb, int iub, double dt,
+                                    const double *du, const double *prop,
+                                    const double *q, const double *v,
+                                    const int32_t *cp, double *D, double *E);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

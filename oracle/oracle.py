@@ -1,0 +1,210 @@
+"""ctypes front-end of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Loads oracle/liboracle.so (the C restatement, advdiff_oracle.c) and, when it
+has been built, oracle/_ref/libmohid_mf_ref.so (the reference's own
+ModuleFunctions compiled from source).  Only tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg import this module; the product package
+never does.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libmohid_mf_ref.so")
+
+_dp = ct.POINTER(ct.c_double)
+_ip = ct.POINTER(ct.c_int32)
+
+
+class Size3D(ct.Structure):
+    _fields_ = [(n, ct.c_int32) for n in ("ILB", "IUB", "JLB", "JUB", "KLB", "KUB")]
+
+
+class Geometry(ct.Structure):
+    _fields_ = [("DUX", _dp), ("DVY", _dp), ("DZX", _dp), ("DZY", _dp),
+                ("KFloorZ", _ip), ("AreaU", _dp), ("AreaV", _dp),
+                ("DZZ", _dp), ("DWZ", _dp)]
+
+
+class Args(ct.Structure):
+    _fields_ = [
+        ("schmidt_H", ct.c_double), ("SchmidtCoef_V", ct.c_double),
+        ("SchmidtBackground_V", ct.c_double),
+        ("AdvMethodH", ct.c_int32), ("TVDLimitationH", ct.c_int32),
+        ("AdvMethodV", ct.c_int32), ("TVDLimitationV", ct.c_int32),
+        ("Upwind2H", ct.c_int32), ("Upwind2V", ct.c_int32),
+        ("VolumeRelMax", ct.c_double), ("DTProp", ct.c_double),
+        ("ImpExp_AdvV", ct.c_double), ("ImpExp_DifV", ct.c_double),
+        ("ImpExp_AdvXX", ct.c_double), ("ImpExp_AdvYY", ct.c_double),
+        ("ImpExp_DifH", ct.c_double), ("NullDif", ct.c_int32),
+        ("Wflux_X", _dp), ("Wflux_Y", _dp), ("Wflux_Z", _dp),
+        ("VolumeZOld", _dp), ("VolumeZ", _dp),
+        ("OpenPoints3D", _ip), ("LandPoints3D", _ip),
+        ("ComputeFacesU3D", _ip), ("ComputeFacesV3D", _ip), ("ComputeFacesW3D", _ip),
+        ("WaterPoints3D", _ip), ("Visc_H", _dp), ("Diff_V", _dp),
+        ("Optimize", ct.c_int32), ("FirstProperty", ct.c_int32), ("Now", ct.c_double),
+    ]
+
+
+def _ptr(a: np.ndarray, kind):
+    return a.ctypes.data_as(kind)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
+        L = ct.CDLL(LIB_PATH)
+        L.ora_start.argtypes = [ct.POINTER(Size3D), ct.POINTER(Size3D), ct.c_int, ct.c_int,
+                                ct.c_int, ct.c_int, ct.POINTER(Geometry), ct.POINTER(ct.c_void_p)]
+        L.ora_start.restype = ct.c_int
+        L.ora_advection_diffusion.argtypes = [ct.c_void_p, ct.POINTER(Args), _dp]
+        L.ora_advection_diffusion.restype = ct.c_int
+        L.ora_kill.argtypes = [ct.c_void_p]
+        L.ora_internal.argtypes = [ct.c_void_p, ct.c_char_p]
+        L.ora_internal.restype = _dp
+        L.ora_thomasz.argtypes = [ct.POINTER(Size3D), ct.POINTER(Size3D), _dp, _dp, _dp, _dp,
+                                  _ip, _dp, _dp, _dp]
+        L.ora_advection1d_v2.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_double, _dp, _dp,
+                                         _dp, _dp, _ip, _dp, _dp, _dp, _dp, ct.c_int, ct.c_int,
+                                         ct.c_double, ct.c_int]
+        L.ora_advection1d_v2.restype = ct.c_int
+        L.ora_advection_face.argtypes = [_dp, _dp, _dp, ct.c_double, ct.c_double, ct.c_double,
+                                         ct.c_int, ct.c_int, ct.c_int, ct.c_int, _dp]
+        L.ora_advection_face.restype = ct.c_int
+        L.ora_advection1d_tvd_superbee_2.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_double,
+                                                     _dp, _dp, _dp, _dp, _ip, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    """The reference's own ModuleFunctions (compiled from source), or raise."""
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError(f"reference build missing: {REF_PATH} (run make -C oracle ref)")
+        R = ct.CDLL(REF_PATH)
+        R.ref_thomasz.argtypes = [_ip, _ip, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp, _ip, _dp,
+                                  _dp, _dp]
+        R.ref_advection1d_v2.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_double, _dp, _dp,
+                                         _dp, _dp, _ip, _dp, _dp, _dp, _dp, ct.c_int, ct.c_int,
+                                         ct.c_double, ct.c_int]
+        R.ref_advection1d_tvd_superbee_2.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_double,
+                                                     _dp, _dp, _dp, _dp, _ip, _dp, _dp]
+        R.ref_advection_face.argtypes = [_dp, _dp, _dp, ct.c_double, ct.c_double, ct.c_double,
+                                         ct.c_int, ct.c_int, ct.c_int, ct.c_int, _dp]
+        _ref = R
+    return _ref
+
+
+class Instance:
+    """One ModuleAdvectionDiffusion instance (StartAdvectionDiffusion .. Kill)."""
+
+    def __init__(self, case, docycle_method: int = 1, vertical1d: bool = False,
+                 xzflow: bool = False, nthreads: Optional[int] = None):
+        self.case = case
+        L = lib()
+        self._keep = []
+        g = case.geom
+        self.geom = Geometry(*[_ptr(self._k(np.asfortranarray(g[n], dtype=np.float64)), _dp)
+                               for n in ("DUX", "DVY", "DZX", "DZY")],
+                             _ptr(self._k(np.asfortranarray(g["KFloorZ"], dtype=np.int32)), _ip),
+                             *[_ptr(self._k(np.asfortranarray(g[n], dtype=np.float64)), _dp)
+                               for n in ("AreaU", "AreaV", "DZZ", "DWZ")])
+        self.size = Size3D(*case.size)
+        self.ws = Size3D(*case.worksize)
+        if nthreads is None:
+            nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        h = ct.c_void_p()
+        rc = L.ora_start(ct.byref(self.size), ct.byref(self.ws), docycle_method,
+                         int(vertical1d), int(xzflow), nthreads, ct.byref(self.geom),
+                         ct.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"ora_start failed: {rc}")
+        self.h = h
+        self.now = 0.0
+
+    def _k(self, a):
+        self._keep.append(a)
+        return a
+
+    def step(self, prop: np.ndarray, first_property: bool = True, now: Optional[float] = None,
+             hydro=None, **overrides) -> int:
+        """One AdvectionDiffusion call on `prop` (modified in place)."""
+        case = self.case
+        p = dict(case.params)
+        p.update(overrides)
+        hy = case.hydro if hydro is None else hydro
+        keep = []
+
+        def d(n):
+            a = np.asfortranarray(hy[n], dtype=np.float64)
+            keep.append(a)
+            return _ptr(a, _dp)
+
+        def i(n):
+            a = np.asfortranarray(hy[n], dtype=np.int32)
+            keep.append(a)
+            return _ptr(a, _ip)
+
+        args = Args(p["schmidt_H"], p["SchmidtCoef_V"], p["SchmidtBackground_V"],
+                    p["AdvMethodH"], p["TVDLimitationH"], p["AdvMethodV"], p["TVDLimitationV"],
+                    int(p["Upwind2H"]), int(p["Upwind2V"]), p["VolumeRelMax"], p["DTProp"],
+                    p["ImpExp_AdvV"], p["ImpExp_DifV"], p["ImpExp_AdvXX"], p["ImpExp_AdvYY"],
+                    p["ImpExp_DifH"], int(p["NullDif"]),
+                    d("Wflux_X"), d("Wflux_Y"), d("Wflux_Z"), d("VolumeZOld"), d("VolumeZ"),
+                    i("OpenPoints3D"), i("LandPoints3D"), i("ComputeFacesU3D"),
+                    i("ComputeFacesV3D"), i("ComputeFacesW3D"), i("WaterPoints3D"),
+                    d("Visc_H"), d("Diff_V"), int(p["Optimize"]), int(first_property),
+                    self.now if now is None else now)
+        assert prop.flags.f_contiguous and prop.dtype == np.float64
+        return lib().ora_advection_diffusion(self.h, ct.byref(args), _ptr(prop, _dp))
+
+    def internal(self, name: str) -> np.ndarray:
+        ptr = lib().ora_internal(self.h, name.encode())
+        n = int(np.prod(self.case.shape))
+        return np.ctypeslib.as_array(ptr, shape=(n,)).reshape(self.case.shape, order="F").copy()
+
+    def close(self):
+        if self.h:
+            lib().ora_kill(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_case(case, steps: int, nthreads: Optional[int] = None, tracers=None):
+    """Advance all tracers of `case` by `steps` steps with the oracle (the
+    reference's call pattern: one AdvectionDiffusion call per tracer per step,
+    FirstProperty on the first, WP:14776-14822).  Returns the new tracers."""
+    inst = Instance(case, nthreads=nthreads)
+    props = [np.array(t, order="F", copy=True) for t in (case.tracers if tracers is None else tracers)]
+    dt = case.params["DTProp"]
+    for s in range(steps):
+        now = (s + 1) * dt
+        for t, prop in enumerate(props):
+            rc = inst.step(prop, first_property=(t == 0), now=now)
+            if rc != 0:
+                raise RuntimeError(f"oracle step failed rc={rc}")
+    inst.close()
+    return props

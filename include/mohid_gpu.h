@@ -1,0 +1,168 @@
+/*
+ * mohid_gpu.h -- C ABI of the MI355X (gfx950) tracer-transport library.
+ *
+ * Drop-in for MOHID Water's tracer-transport step:
+ *   ModuleAdvectionDiffusion's explicit horizontal advection/diffusion and the
+ *   implicit vertical advection-diffusion solved per water column by
+ *   ModuleFunctions' THOMASZ_NewType2.
+ * Reference files (relative to the reference root):
+ *   AD = Software/MOHIDBase2/ModuleAdvectionDiffusion.F90
+ *   MF = Software/MOHIDBase1/ModuleFunctions.F90
+ *   GD = Software/MOHIDBase1/ModuleGlobalData.F90
+ *
+ * Conventions
+ *   - Arrays are Fortran column-major, i fastest, spanning Size = WorkSize +/- 1
+ *     (ModuleGeometry.F90:815-816), exactly what the Fortran caller holds; a
+ *     Fortran caller passes c_loc(a(lbound(a,1), lbound(a,2), lbound(a,3))).
+ *   - Every `real` of the reference is a double (default-real-8 production
+ *     build, Solutions/mohid-in-linux/compile_mohid.sh:185); masks are int32.
+ *   - Numeric enums (advection methods, TVD limiters, implicit/explicit
+ *     weights) take the reference's values (GD:1810-1832).
+ *   - All entry points return an int status: 0 = SUCCESS_, 1 = OFF_ERR_ (bad
+ *     handle), -99 = UNKNOWN_ (GD:180-199), or one of MGPU_ERR_*.  The library
+ *     never exits the process; where the reference stops
+ *     (AD:1231-1239, 1340-1349) the call returns MGPU_ERR_OPTION.
+ *   - Pointers are host pointers unless the call says `dev` / the args struct
+ *     sets device_pointers = 1, in which case they are device (HBM) pointers.
+ *   - Handles are not thread-safe (the reference's module is not re-entrant,
+ *     AD:385-386); each handle owns one HIP stream.
+ */
+#ifndef MOHID_GPU_H
+#define MOHID_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define MGPU_SUCCESS 0           /* SUCCESS_, GD:180 */
+#define MGPU_OFF_ERR 1           /* OFF_ERR_: handle not created / destroyed */
+#define MGPU_UNKNOWN (-99)       /* UNKNOWN_ */
+#define MGPU_ERR_HIP 100         /* a HIP runtime call failed */
+#define MGPU_ERR_BADARG 101      /* null pointer, bad bounds or sizes */
+#define MGPU_ERR_OPTION 102      /* option set the reference rejects with `stop` */
+#define MGPU_ERR_UNSUPPORTED 103 /* branch not offloaded: caller keeps the CPU path */
+#define MGPU_ERR_MASKS 104       /* masks inconsistent with KFloor under DOCYCLE_METHOD 1 */
+#define MGPU_ERR_STATE 105       /* Optimize call without a FirstProperty call this step */
+#define MGPU_ERR_NOMEM 106       /* device allocation failed */
+
+/* ---- precision of the batched transport -------------------------------- */
+#define MGPU_PREC_FP64 0         /* reference operation order, IEEE FP64 */
+#define MGPU_PREC_MIXED 1        /* FP32 sweeps + FP64 residual correction */
+
+/* T_Size3D (GD:2044-2052): ILB, IUB, JLB, JUB, KLB, KUB. */
+typedef struct {
+    int32_t ILB, IUB, JLB, JUB, KLB, KUB;
+} mgpu_size3d;
+
+/* Geometry AdvectionDiffusion fetches from HorizontalGrid / Geometry
+ * (AD:1353-1401): 2-D arrays span Size(i,j), 3-D arrays span Size. */
+typedef struct {
+    const double *DUX, *DVY, *DZX, *DZY;   /* HG metrics (2-D) */
+    const int32_t *KFloorZ;                /* GEO KFloor (2-D) */
+    const double *AreaU, *AreaV;           /* GEO areas (3-D) */
+    const double *DZZ, *DWZ;               /* GEO distances (3-D) */
+} mgpu_geometry;
+
+/* Arguments of AdvectionDiffusion (AD:1108-1198) in the reference's order.
+ * Options the GPU path does not implement are listed as `unsupported_*`:
+ * a non-zero value makes mgpu_ad_step return MGPU_ERR_UNSUPPORTED so the
+ * caller falls back to the Fortran path. */
+typedef struct {
+    double schmidt_H, SchmidtCoef_V, SchmidtBackground_V;
+    int32_t AdvMethodH, TVDLimitationH, AdvMethodV, TVDLimitationV;
+    int32_t Upwind2H, Upwind2V;
+    double VolumeRelMax;
+    int32_t AdvectionNudging, AdvectionNudgingCells;
+    double DTProp;
+    double ImpExp_AdvV, ImpExp_DifV, ImpExp_AdvXX, ImpExp_AdvYY, ImpExp_DifH;
+    int32_t NullDif;
+    const double *Wflux_X, *Wflux_Y, *Wflux_Z;
+    const double *VolumeZOld, *VolumeZ;
+    const int32_t *OpenPoints3D, *LandPoints3D;
+    const int32_t *ComputeFacesU3D, *ComputeFacesV3D, *ComputeFacesW3D;
+    const double *Visc_H, *Diff_V;
+    const int32_t *WaterPoints3D;
+    int32_t Optimize, FirstProperty;
+    /* present-and-set optional arguments the GPU path does not offload */
+    int32_t unsupported_CellFluxes, unsupported_ReferenceProp;
+    int32_t unsupported_BoundaryCondition; /* NullGradient_(4) / CyclicBoundary_(8) post-processing */
+    int32_t unsupported_SmallDepths, unsupported_NoAdvFlux, unsupported_NoDifFlux;
+    int32_t unsupported_Discharges;
+    int32_t device_pointers;               /* 1: every array above is a device pointer */
+} mgpu_ad_args;
+
+typedef struct mgpu_ad mgpu_ad;
+
+/* StartAdvectionDiffusion (AD:400-498) + the geometry it binds
+ * (AD:1353-1401).  docycle_method is ModuleModel's DOCYCLE_METHOD
+ * (ModuleModel.F90:641-643, default 1).  Geometry arrays are copied to the
+ * device (host pointers) and may be refreshed with mgpu_ad_set_geometry. */
+int mgpu_ad_create(const mgpu_size3d *size, const mgpu_size3d *worksize,
+                   int32_t docycle_method, int32_t vertical1d, int32_t xzflow,
+                   const mgpu_geometry *geometry, mgpu_ad **out);
+int mgpu_ad_set_geometry(mgpu_ad *ad, const mgpu_geometry *geometry);
+
+/* AdvectionDiffusion (AD:1108-1509): advances PROP by one DTProp in place.
+ * Host mode copies the inputs in and PROP out per call (PCIe-bound: use the
+ * device mode or the batched transport for performance). */
+int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *args, double *prop);
+
+/* KillAdvectionDiffusion (AD:5849-6066). */
+int mgpu_ad_destroy(mgpu_ad *ad);
+
+/* Number of zero-pivot Thomas rows met in the last step.  The reference
+ * skips such rows silently, keeping stale per-thread scratch (MF:4092-4098);
+ * the GPU path zeroes the row's forward coefficients and counts it here. */
+int mgpu_ad_zero_pivots(const mgpu_ad *ad, uint64_t *count);
+
+/* THOMASZ_NewType2 (MF:4026-4123): solve every water column
+ * (WaterPoints3D(i,j,WorkSize%KUB) == 1) of the tridiagonal system
+ * D(k) x(k-1) + E(k) x(k) + F(k) x(k+1) = TI(k), rows WorkSize%KLB..KUB+1,
+ * writing x into res.  dev = 1: all pointers are device pointers and the
+ * call is asynchronous on `stream` (a hipStream_t, may be null). */
+int mgpu_thomasz(const mgpu_size3d *size, const mgpu_size3d *worksize,
+                 const double *D, const double *E, const double *F,
+                 const double *TI, const int32_t *waterpoints, double *res,
+                 int32_t dev, void *stream);
+
+/* ---- batched, device-resident transport ----------------------------------
+ * The multi-tracer loop of ModuleWaterProperties::Advection_Diffusion_Processes
+ * (WP:14520-15169) for T tracers that share one set of fluxes, volumes,
+ * diffusivities and masks, kept in HBM across steps.  One step advances
+ * every tracer by DTProp with the reference's per-tracer semantics
+ * (Optimize/FirstProperty as WP:14576-14598 decides). */
+typedef struct mgpu_transport mgpu_transport;
+
+typedef struct {
+    int32_t ntracers;
+    int32_t precision;                    /* MGPU_PREC_FP64 / MGPU_PREC_MIXED */
+    int32_t docycle_method;
+    int32_t tile_i, tile_j;               /* 0: autotuned / default */
+} mgpu_transport_opts;
+
+int mgpu_transport_create(const mgpu_size3d *size, const mgpu_size3d *worksize,
+                          const mgpu_transport_opts *opts, const mgpu_geometry *geometry,
+                          mgpu_transport **out);
+/* Shared per-step inputs (the arrays of mgpu_ad_args; scalars from args). */
+int mgpu_transport_set_hydro(mgpu_transport *t, const mgpu_ad_args *args);
+int mgpu_transport_set_tracer(mgpu_transport *t, int32_t index, const double *prop, int32_t dev);
+int mgpu_transport_get_tracer(mgpu_transport *t, int32_t index, double *prop, int32_t dev);
+/* Advance all tracers nsteps steps.  use_graph = 1 captures the step loop in
+ * a hipGraph (re-used while nsteps is unchanged). */
+int mgpu_transport_run(mgpu_transport *t, int32_t nsteps, int32_t use_graph);
+/* Device pointer of tracer `index` (current state) and the handle's stream. */
+int mgpu_transport_tracer_ptr(mgpu_transport *t, int32_t index, double **dev_ptr);
+int mgpu_transport_stream(mgpu_transport *t, void **stream);
+int mgpu_transport_destroy(mgpu_transport *t);
+
+/* Library version string and the last HIP error message (thread-local). */
+const char *mgpu_version(void);
+const char *mgpu_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MOHID_GPU_H */

@@ -1,0 +1,144 @@
+"""ctypes binding of the C ABI declared in include/mohid_gpu.h.
+
+The shared library is built in-tree (mohid_amd/libmohid_gpu.so, see
+mohid_amd/build.py).  Loading fails loudly when it is missing: there is no
+CPU fallback in the product path.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmohid_gpu.so")
+
+_dp = ct.POINTER(ct.c_double)
+_ip = ct.POINTER(ct.c_int32)
+
+# status codes (include/mohid_gpu.h)
+MGPU_SUCCESS = 0
+MGPU_OFF_ERR = 1
+MGPU_UNKNOWN = -99
+MGPU_ERR_HIP = 100
+MGPU_ERR_BADARG = 101
+MGPU_ERR_OPTION = 102
+MGPU_ERR_UNSUPPORTED = 103
+MGPU_ERR_MASKS = 104
+MGPU_ERR_STATE = 105
+MGPU_ERR_NOMEM = 106
+MGPU_PREC_FP64 = 0
+MGPU_PREC_MIXED = 1
+
+STATUS_NAMES = {
+    0: "SUCCESS", 1: "OFF_ERR", -99: "UNKNOWN", 100: "ERR_HIP", 101: "ERR_BADARG",
+    102: "ERR_OPTION", 103: "ERR_UNSUPPORTED", 104: "ERR_MASKS", 105: "ERR_STATE",
+    106: "ERR_NOMEM",
+}
+
+
+class Size3D(ct.Structure):
+    _fields_ = [(n, ct.c_int32) for n in ("ILB", "IUB", "JLB", "JUB", "KLB", "KUB")]
+
+
+class Geometry(ct.Structure):
+    _fields_ = [("DUX", _dp), ("DVY", _dp), ("DZX", _dp), ("DZY", _dp),
+                ("KFloorZ", _ip), ("AreaU", _dp), ("AreaV", _dp), ("DZZ", _dp), ("DWZ", _dp)]
+
+
+class ADArgs(ct.Structure):
+    _fields_ = [
+        ("schmidt_H", ct.c_double), ("SchmidtCoef_V", ct.c_double),
+        ("SchmidtBackground_V", ct.c_double),
+        ("AdvMethodH", ct.c_int32), ("TVDLimitationH", ct.c_int32),
+        ("AdvMethodV", ct.c_int32), ("TVDLimitationV", ct.c_int32),
+        ("Upwind2H", ct.c_int32), ("Upwind2V", ct.c_int32),
+        ("VolumeRelMax", ct.c_double),
+        ("AdvectionNudging", ct.c_int32), ("AdvectionNudgingCells", ct.c_int32),
+        ("DTProp", ct.c_double),
+        ("ImpExp_AdvV", ct.c_double), ("ImpExp_DifV", ct.c_double),
+        ("ImpExp_AdvXX", ct.c_double), ("ImpExp_AdvYY", ct.c_double),
+        ("ImpExp_DifH", ct.c_double), ("NullDif", ct.c_int32),
+        ("Wflux_X", _dp), ("Wflux_Y", _dp), ("Wflux_Z", _dp),
+        ("VolumeZOld", _dp), ("VolumeZ", _dp),
+        ("OpenPoints3D", _ip), ("LandPoints3D", _ip),
+        ("ComputeFacesU3D", _ip), ("ComputeFacesV3D", _ip), ("ComputeFacesW3D", _ip),
+        ("Visc_H", _dp), ("Diff_V", _dp), ("WaterPoints3D", _ip),
+        ("Optimize", ct.c_int32), ("FirstProperty", ct.c_int32),
+        ("unsupported_CellFluxes", ct.c_int32), ("unsupported_ReferenceProp", ct.c_int32),
+        ("unsupported_BoundaryCondition", ct.c_int32), ("unsupported_SmallDepths", ct.c_int32),
+        ("unsupported_NoAdvFlux", ct.c_int32), ("unsupported_NoDifFlux", ct.c_int32),
+        ("unsupported_Discharges", ct.c_int32),
+        ("device_pointers", ct.c_int32),
+    ]
+
+
+class TransportOpts(ct.Structure):
+    _fields_ = [("ntracers", ct.c_int32), ("precision", ct.c_int32),
+                ("docycle_method", ct.c_int32), ("tile_i", ct.c_int32), ("tile_j", ct.c_int32)]
+
+
+# every symbol include/mohid_gpu.h declares
+EXPORTS = (
+    "mgpu_ad_create", "mgpu_ad_set_geometry", "mgpu_ad_step", "mgpu_ad_destroy",
+    "mgpu_ad_zero_pivots", "mgpu_thomasz",
+    "mgpu_transport_create", "mgpu_transport_set_hydro", "mgpu_transport_set_tracer",
+    "mgpu_transport_get_tracer", "mgpu_transport_run", "mgpu_transport_tracer_ptr",
+    "mgpu_transport_stream", "mgpu_transport_destroy",
+    "mgpu_version", "mgpu_last_error",
+)
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"HIP library not built: {LIB_PATH} (python -m mohid_amd.build)")
+    L = ct.CDLL(LIB_PATH)
+    vp = ct.c_void_p
+    L.mgpu_ad_create.argtypes = [ct.POINTER(Size3D), ct.POINTER(Size3D), ct.c_int32, ct.c_int32,
+                                 ct.c_int32, ct.POINTER(Geometry), ct.POINTER(vp)]
+    L.mgpu_ad_set_geometry.argtypes = [vp, ct.POINTER(Geometry)]
+    L.mgpu_ad_step.argtypes = [vp, ct.POINTER(ADArgs), vp]
+    L.mgpu_ad_destroy.argtypes = [vp]
+    L.mgpu_ad_zero_pivots.argtypes = [vp, ct.POINTER(ct.c_uint64)]
+    L.mgpu_thomasz.argtypes = [ct.POINTER(Size3D), ct.POINTER(Size3D), vp, vp, vp, vp, vp, vp,
+                               ct.c_int32, vp]
+    L.mgpu_transport_create.argtypes = [ct.POINTER(Size3D), ct.POINTER(Size3D),
+                                        ct.POINTER(TransportOpts), ct.POINTER(Geometry),
+                                        ct.POINTER(vp)]
+    L.mgpu_transport_set_hydro.argtypes = [vp, ct.POINTER(ADArgs)]
+    L.mgpu_transport_set_tracer.argtypes = [vp, ct.c_int32, vp, ct.c_int32]
+    L.mgpu_transport_get_tracer.argtypes = [vp, ct.c_int32, vp, ct.c_int32]
+    L.mgpu_transport_run.argtypes = [vp, ct.c_int32, ct.c_int32]
+    L.mgpu_transport_tracer_ptr.argtypes = [vp, ct.c_int32, ct.POINTER(vp)]
+    L.mgpu_transport_stream.argtypes = [vp, ct.POINTER(vp)]
+    L.mgpu_transport_destroy.argtypes = [vp]
+    L.mgpu_version.restype = ct.c_char_p
+    L.mgpu_last_error.restype = ct.c_char_p
+    for name in EXPORTS:
+        if name not in ("mgpu_version", "mgpu_last_error"):
+            getattr(L, name).restype = ct.c_int
+    _lib = L
+    return L
+
+
+class MgpuError(RuntimeError):
+    def __init__(self, where: str, code: int):
+        msg = f"{where} failed: {STATUS_NAMES.get(code, code)}"
+        try:
+            detail = lib().mgpu_last_error().decode()
+            if detail and code == MGPU_ERR_HIP:
+                msg += f" ({detail})"
+        except Exception:
+            pass
+        super().__init__(msg)
+        self.code = code
+
+
+def check(where: str, rc: int) -> None:
+    if rc != MGPU_SUCCESS:
+        raise MgpuError(where, rc)

@@ -1,0 +1,221 @@
+"""Python mirror of ModuleAdvectionDiffusion's public interface on the GPU.
+
+StartAdvectionDiffusion / AdvectionDiffusion / KillAdvectionDiffusion
+(Software/MOHIDBase2/ModuleAdvectionDiffusion.F90:400, 1108, 5849) and
+THOMASZ_NewType2 (Software/MOHIDBase1/ModuleFunctions.F90:4026) with the
+reference's argument names and meaning, calling the HIP library through its
+C ABI (include/mohid_gpu.h).  Arrays are numpy arrays in Fortran order
+spanning Size (host mode) or device tensors with the same layout (device
+mode: torch tensors on the GPU, passed by data pointer).
+
+Errors are raised as MgpuError carrying the library's status code, where the
+reference would `stop` or return a non-zero STAT.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+from . import _lib as L
+
+__all__ = ["AdvectionDiffusion", "thomasz", "Transport"]
+
+_GEOM_D = ("DUX", "DVY", "DZX", "DZY")
+_GEOM_3 = ("AreaU", "AreaV", "DZZ", "DWZ")
+_HYDRO_D = ("Wflux_X", "Wflux_Y", "Wflux_Z", "VolumeZOld", "VolumeZ")
+_MASKS = ("OpenPoints3D", "LandPoints3D", "ComputeFacesU3D", "ComputeFacesV3D",
+          "ComputeFacesW3D")
+
+
+def _is_device(a) -> bool:
+    return hasattr(a, "data_ptr") and getattr(a, "is_cuda", False)
+
+
+def _addr(a, keep: list, dtype):
+    """Address of `a` (numpy: a Fortran-ordered copy of dtype; torch: data_ptr)."""
+    if _is_device(a):
+        return ct.c_void_p(a.data_ptr())
+    arr = np.asfortranarray(a, dtype=dtype)
+    keep.append(arr)
+    return ct.c_void_p(arr.ctypes.data)
+
+
+def _geometry(geom: Dict[str, np.ndarray], keep: list) -> L.Geometry:
+    def d(n):
+        a = np.asfortranarray(geom[n], dtype=np.float64)
+        keep.append(a)
+        return a.ctypes.data_as(ct.POINTER(ct.c_double))
+    kf = np.asfortranarray(geom["KFloorZ"], dtype=np.int32)
+    keep.append(kf)
+    return L.Geometry(d("DUX"), d("DVY"), d("DZX"), d("DZY"),
+                      kf.ctypes.data_as(ct.POINTER(ct.c_int32)),
+                      d("AreaU"), d("AreaV"), d("DZZ"), d("DWZ"))
+
+
+def _args(hydro: Dict, params: Dict, keep: list, device: bool, optimize: bool,
+          first_property: bool) -> L.ADArgs:
+    def dp(n):
+        v = _addr(hydro[n], keep, np.float64)
+        return ct.cast(v, ct.POINTER(ct.c_double))
+
+    def ip(n):
+        v = _addr(hydro[n], keep, np.int32)
+        return ct.cast(v, ct.POINTER(ct.c_int32))
+
+    p = params
+    return L.ADArgs(
+        float(p["schmidt_H"]), float(p["SchmidtCoef_V"]), float(p["SchmidtBackground_V"]),
+        int(p["AdvMethodH"]), int(p["TVDLimitationH"]), int(p["AdvMethodV"]),
+        int(p["TVDLimitationV"]), int(bool(p["Upwind2H"])), int(bool(p["Upwind2V"])),
+        float(p["VolumeRelMax"]), int(bool(p.get("AdvectionNudging", False))),
+        int(p.get("AdvectionNudgingCells", 0)), float(p["DTProp"]),
+        float(p["ImpExp_AdvV"]), float(p["ImpExp_DifV"]), float(p["ImpExp_AdvXX"]),
+        float(p["ImpExp_AdvYY"]), float(p["ImpExp_DifH"]), int(bool(p["NullDif"])),
+        dp("Wflux_X"), dp("Wflux_Y"), dp("Wflux_Z"), dp("VolumeZOld"), dp("VolumeZ"),
+        ip("OpenPoints3D"), ip("LandPoints3D"), ip("ComputeFacesU3D"), ip("ComputeFacesV3D"),
+        ip("ComputeFacesW3D"), dp("Visc_H"), dp("Diff_V"), ip("WaterPoints3D"),
+        int(bool(optimize)), int(bool(first_property)),
+        int(bool(p.get("CellFluxes", False))), int(bool(p.get("ReferenceProp", False))),
+        int(p.get("BoundaryCondition", 1) in (4, 8)), int(bool(p.get("SmallDepths", False))),
+        int(bool(p.get("NoAdvFlux", False))), int(bool(p.get("NoDifFlux", False))),
+        int(bool(p.get("Discharges", False))), int(bool(device)))
+
+
+class AdvectionDiffusion:
+    """One GPU instance of the AdvectionDiffusion module.
+
+    ``StartAdvectionDiffusion`` <-> constructor, ``AdvectionDiffusion`` <->
+    ``__call__`` (PROP advanced in place), ``KillAdvectionDiffusion`` <->
+    ``kill``.
+    """
+
+    def __init__(self, size: Sequence[int], worksize: Sequence[int], geometry: Dict,
+                 docycle_method: int = 1, vertical1d: bool = False, xzflow: bool = False):
+        keep: list = []
+        g = _geometry(geometry, keep)
+        h = ct.c_void_p()
+        self._size = L.Size3D(*size)
+        self._ws = L.Size3D(*worksize)
+        L.check("mgpu_ad_create",
+                L.lib().mgpu_ad_create(ct.byref(self._size), ct.byref(self._ws),
+                                       int(docycle_method), int(vertical1d), int(xzflow),
+                                       ct.byref(g), ct.byref(h)))
+        self._h = h
+
+    def __call__(self, prop, hydro: Dict, params: Dict, optimize: Optional[bool] = None,
+                 first_property: bool = True) -> None:
+        if self._h is None:
+            raise L.MgpuError("AdvectionDiffusion", L.MGPU_OFF_ERR)
+        keep: list = []
+        device = _is_device(prop)
+        if optimize is None:
+            optimize = bool(params.get("Optimize", False))
+        args = _args(hydro, params, keep, device, optimize, first_property)
+        if device:
+            addr = ct.c_void_p(prop.data_ptr())
+        else:
+            if not (isinstance(prop, np.ndarray) and prop.flags.f_contiguous
+                    and prop.dtype == np.float64):
+                raise TypeError("PROP must be a Fortran-ordered float64 array")
+            addr = ct.c_void_p(prop.ctypes.data)
+        L.check("AdvectionDiffusion", L.lib().mgpu_ad_step(self._h, ct.byref(args), addr))
+
+    def zero_pivots(self) -> int:
+        c = ct.c_uint64()
+        L.check("mgpu_ad_zero_pivots", L.lib().mgpu_ad_zero_pivots(self._h, ct.byref(c)))
+        return int(c.value)
+
+    def kill(self) -> None:
+        if self._h is not None:
+            L.check("KillAdvectionDiffusion", L.lib().mgpu_ad_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.kill()
+        except Exception:
+            pass
+
+
+def thomasz(size, worksize, D, E, F, TI, waterpoints, res) -> None:
+    """THOMASZ_NewType2: solve the water columns into ``res`` (in place)."""
+    keep: list = []
+    dev = _is_device(res)
+    addrs = [_addr(a, keep, np.float64) for a in (D, E, F, TI)]
+    wp = _addr(waterpoints, keep, np.int32)
+    if not dev and not (isinstance(res, np.ndarray) and res.flags.f_contiguous
+                        and res.dtype == np.float64):
+        raise TypeError("RES must be a Fortran-ordered float64 array")
+    r = ct.c_void_p(res.data_ptr() if dev else res.ctypes.data)
+    s = L.Size3D(*size)
+    w = L.Size3D(*worksize)
+    L.check("THOMASZ_NewType2", L.lib().mgpu_thomasz(ct.byref(s), ct.byref(w), *addrs, wp, r,
+                                                     int(dev), None))
+
+
+class Transport:
+    """Batched, device-resident transport of several tracers that share one
+    hydrodynamic state (the tracer loop of
+    ModuleWaterProperties::Advection_Diffusion_Processes, WP:14520-15169)."""
+
+    def __init__(self, size, worksize, geometry: Dict, ntracers: int, precision: int = 0,
+                 docycle_method: int = 1, tile=(0, 0)):
+        keep: list = []
+        g = _geometry(geometry, keep)
+        opts = L.TransportOpts(int(ntracers), int(precision), int(docycle_method),
+                               int(tile[0]), int(tile[1]))
+        h = ct.c_void_p()
+        self._size = L.Size3D(*size)
+        self._ws = L.Size3D(*worksize)
+        L.check("mgpu_transport_create",
+                L.lib().mgpu_transport_create(ct.byref(self._size), ct.byref(self._ws),
+                                              ct.byref(opts), ct.byref(g), ct.byref(h)))
+        self._h = h
+        self.ntracers = ntracers
+
+    def set_hydro(self, hydro: Dict, params: Dict) -> None:
+        keep: list = []
+        dev = _is_device(hydro["Wflux_X"])
+        args = _args(hydro, params, keep, dev, params.get("Optimize", False), True)
+        L.check("mgpu_transport_set_hydro", L.lib().mgpu_transport_set_hydro(self._h,
+                                                                             ct.byref(args)))
+
+    def set_tracer(self, index: int, prop) -> None:
+        keep: list = []
+        dev = _is_device(prop)
+        L.check("mgpu_transport_set_tracer",
+                L.lib().mgpu_transport_set_tracer(self._h, index, _addr(prop, keep, np.float64),
+                                                  int(dev)))
+
+    def get_tracer(self, index: int, out=None):
+        shape = (self._size.IUB - self._size.ILB + 1, self._size.JUB - self._size.JLB + 1,
+                 self._size.KUB - self._size.KLB + 1)
+        if out is None:
+            out = np.empty(shape, dtype=np.float64, order="F")
+        dev = _is_device(out)
+        addr = ct.c_void_p(out.data_ptr() if dev else out.ctypes.data)
+        L.check("mgpu_transport_get_tracer",
+                L.lib().mgpu_transport_get_tracer(self._h, index, addr, int(dev)))
+        return out
+
+    def run(self, nsteps: int, use_graph: bool = True) -> None:
+        L.check("mgpu_transport_run", L.lib().mgpu_transport_run(self._h, int(nsteps),
+                                                                 int(bool(use_graph))))
+
+    def stream_handle(self) -> int:
+        s = ct.c_void_p()
+        L.check("mgpu_transport_stream", L.lib().mgpu_transport_stream(self._h, ct.byref(s)))
+        return s.value or 0
+
+    def close(self) -> None:
+        if self._h is not None:
+            L.check("mgpu_transport_destroy", L.lib().mgpu_transport_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,646 @@
+// C ABI of the transport library: handles, staging, validation, launches.
+//
+// Reference files (root-relative): AD = Software/MOHIDBase2/ModuleAdvectionDiffusion.F90,
+// MF = Software/MOHIDBase1/ModuleFunctions.F90, WP = Software/MOHIDWater/ModuleWaterProperties.F90.
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+
+#include "mgpu_internal.h"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+// copy the solved cells (water columns, rows KLB..KUB+1) of `src` into `dst`;
+// the reference writes exactly these cells of PROP (MF:4086-4107)
+__global__ void copy_back_kernel(Dims d, const uint8_t *__restrict__ mask,
+                                 const double *__restrict__ src, double *__restrict__ dst) {
+    const int i = d.wILB + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = d.wJLB + blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > d.wIUB || j > d.wJUB) return;
+    if (!(mask[d.idx(i, j, d.wKUB)] & M_WATER)) return;
+    for (int k = d.wKLB; k <= d.wKUB + 1; ++k) {
+        const long x = d.idx(i, j, k);
+        dst[x] = src[x];
+    }
+}
+
+int launch_copy_back(const Dims &d, const uint8_t *mask, const double *src, double *dst,
+                     hipStream_t s) {
+    const int ni = d.wIUB - d.wILB + 1, nj = d.wJUB - d.wJLB + 1;
+    dim3 b(64, 4), g((ni + 63) / 64, (nj + 3) / 4);
+    hipLaunchKernelGGL(copy_back_kernel, g, b, 0, s, d, mask, src, dst);
+    HIPCHK(hipGetLastError());
+    return MGPU_SUCCESS;
+}
+
+template <typename T>
+int dalloc(T **p, long n) {
+    *p = nullptr;
+    if (n <= 0) return MGPU_SUCCESS;
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)n);
+    if (e != hipSuccess) {
+        mgpu_set_error("hipMalloc", e);
+        *p = nullptr;
+        return MGPU_ERR_NOMEM;
+    }
+    return MGPU_SUCCESS;
+}
+
+template <typename T>
+void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+// Geometry held on the device.
+struct DevGeometry {
+    double *DUX = nullptr, *DVY = nullptr, *DZX = nullptr, *DZY = nullptr;
+    int32_t *KFloor = nullptr;
+    double *AreaU = nullptr, *AreaV = nullptr, *DZZ = nullptr, *DWZ = nullptr;
+
+    int alloc(const Dims &d) {
+        const long n2 = d.n2(), n3 = d.n3();
+        int rc = 0;
+        rc |= dalloc(&DUX, n2); rc |= dalloc(&DVY, n2); rc |= dalloc(&DZX, n2);
+        rc |= dalloc(&DZY, n2); rc |= dalloc(&KFloor, n2);
+        rc |= dalloc(&AreaU, n3); rc |= dalloc(&AreaV, n3); rc |= dalloc(&DZZ, n3);
+        rc |= dalloc(&DWZ, n3);
+        return rc ? MGPU_ERR_NOMEM : MGPU_SUCCESS;
+    }
+    int upload(const Dims &d, const mgpu_geometry *g, hipStream_t s) {
+        if (!g || !g->DUX || !g->DVY || !g->DZX || !g->DZY || !g->KFloorZ || !g->AreaU ||
+            !g->AreaV || !g->DZZ || !g->DWZ)
+            return MGPU_ERR_BADARG;
+        const size_t b2 = sizeof(double) * (size_t)d.n2(), b3 = sizeof(double) * (size_t)d.n3();
+        HIPCHK(hipMemcpyAsync(DUX, g->DUX, b2, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(DVY, g->DVY, b2, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(DZX, g->DZX, b2, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(DZY, g->DZY, b2, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(KFloor, g->KFloorZ, sizeof(int32_t) * (size_t)d.n2(),
+                              hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(AreaU, g->AreaU, b3, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(AreaV, g->AreaV, b3, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(DZZ, g->DZZ, b3, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(DWZ, g->DWZ, b3, hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return MGPU_SUCCESS;
+    }
+    void release() {
+        dfree(DUX); dfree(DVY); dfree(DZX); dfree(DZY); dfree(KFloor);
+        dfree(AreaU); dfree(AreaV); dfree(DZZ); dfree(DWZ);
+    }
+};
+
+// Hydrodynamic inputs held on the device (owned copies).
+struct DevHydro {
+    double *Qx = nullptr, *Qy = nullptr, *Qz = nullptr, *V = nullptr, *Vold = nullptr;
+    double *visc = nullptr, *diffv = nullptr;
+    int32_t *open = nullptr, *land = nullptr, *cfu = nullptr, *cfv = nullptr, *cfw = nullptr,
+            *water = nullptr;
+
+    int alloc(const Dims &d) {
+        const long n3 = d.n3();
+        int rc = 0;
+        rc |= dalloc(&Qx, n3); rc |= dalloc(&Qy, n3); rc |= dalloc(&Qz, n3);
+        rc |= dalloc(&V, n3); rc |= dalloc(&Vold, n3); rc |= dalloc(&visc, n3);
+        rc |= dalloc(&diffv, n3);
+        rc |= dalloc(&open, n3); rc |= dalloc(&land, n3); rc |= dalloc(&cfu, n3);
+        rc |= dalloc(&cfv, n3); rc |= dalloc(&cfw, n3); rc |= dalloc(&water, n3);
+        return rc ? MGPU_ERR_NOMEM : MGPU_SUCCESS;
+    }
+    int upload(const Dims &d, const mgpu_ad_args *a, hipStream_t s) {
+        const hipMemcpyKind kind =
+            a->device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        const size_t b = sizeof(double) * (size_t)d.n3(), bi = sizeof(int32_t) * (size_t)d.n3();
+        HIPCHK(hipMemcpyAsync(Qx, a->Wflux_X, b, kind, s));
+        HIPCHK(hipMemcpyAsync(Qy, a->Wflux_Y, b, kind, s));
+        HIPCHK(hipMemcpyAsync(Qz, a->Wflux_Z, b, kind, s));
+        HIPCHK(hipMemcpyAsync(V, a->VolumeZ, b, kind, s));
+        HIPCHK(hipMemcpyAsync(Vold, a->VolumeZOld, b, kind, s));
+        HIPCHK(hipMemcpyAsync(visc, a->Visc_H, b, kind, s));
+        HIPCHK(hipMemcpyAsync(diffv, a->Diff_V, b, kind, s));
+        HIPCHK(hipMemcpyAsync(open, a->OpenPoints3D, bi, kind, s));
+        HIPCHK(hipMemcpyAsync(land, a->LandPoints3D, bi, kind, s));
+        HIPCHK(hipMemcpyAsync(cfu, a->ComputeFacesU3D, bi, kind, s));
+        HIPCHK(hipMemcpyAsync(cfv, a->ComputeFacesV3D, bi, kind, s));
+        HIPCHK(hipMemcpyAsync(cfw, a->ComputeFacesW3D, bi, kind, s));
+        HIPCHK(hipMemcpyAsync(water, a->WaterPoints3D, bi, kind, s));
+        return MGPU_SUCCESS;
+    }
+    void release() {
+        dfree(Qx); dfree(Qy); dfree(Qz); dfree(V); dfree(Vold); dfree(visc); dfree(diffv);
+        dfree(open); dfree(land); dfree(cfu); dfree(cfv); dfree(cfw); dfree(water);
+    }
+};
+
+// Derived per-step fields: packed masks, kstart, face conductances, counters.
+struct DevDerived {
+    uint8_t *mask = nullptr;
+    int32_t *kstart = nullptr;
+    double *cu = nullptr, *cv = nullptr, *cw = nullptr;
+    unsigned long long *counters = nullptr;  // [0] zero pivots, [1] bad-method flag, [2] mask flags
+    int alloc(const Dims &d) {
+        const long n3 = d.n3();
+        int rc = 0;
+        rc |= dalloc(&mask, n3); rc |= dalloc(&kstart, d.n2());
+        rc |= dalloc(&cu, n3); rc |= dalloc(&cv, n3); rc |= dalloc(&cw, n3);
+        rc |= dalloc(&counters, 4);
+        if (rc) return MGPU_ERR_NOMEM;
+        // cells the prep kernel never writes must still be finite
+        HIPCHK(hipMemset(cu, 0, sizeof(double) * (size_t)n3));
+        HIPCHK(hipMemset(cv, 0, sizeof(double) * (size_t)n3));
+        HIPCHK(hipMemset(cw, 0, sizeof(double) * (size_t)n3));
+        HIPCHK(hipMemset(counters, 0, sizeof(unsigned long long) * 4));
+        return MGPU_SUCCESS;
+    }
+    void release() {
+        dfree(mask); dfree(kstart); dfree(cu); dfree(cv); dfree(cw); dfree(counters);
+    }
+    unsigned int *mask_flags() { return reinterpret_cast<unsigned int *>(counters + 2); }
+};
+
+// Checks of AdvectionDiffusion's option set (AD:1231-1239, 1340-1349) and of
+// the branches the GPU path offloads.
+int check_options(const mgpu_ad_args *a) {
+    const bool impXX = a->ImpExp_AdvXX == 1.0, impYY = a->ImpExp_AdvYY == 1.0;
+    if ((impXX || impYY) && (a->AdvMethodH == UPW2 || a->AdvMethodH == UPW3))
+        return MGPU_ERR_OPTION;  // ERR100
+    if (a->ImpExp_AdvV == 1.0 && (a->AdvMethodV == UPW2 || a->AdvMethodV == UPW3))
+        return MGPU_ERR_OPTION;  // ERR200
+    if (a->ImpExp_DifH != 0.0) return MGPU_ERR_OPTION;      // ERR02
+    if (impXX && impYY) return MGPU_ERR_OPTION;            // ERR03
+    if (a->ImpExp_AdvV != 0.0 && a->ImpExp_AdvV != 1.0) return MGPU_ERR_OPTION;  // VerticalAdvection ERR01
+    // implicit horizontal advection goes through THOMAS_3D (AD:4167-4255): CPU path
+    if (a->ImpExp_AdvXX != 0.0 || a->ImpExp_AdvYY != 0.0) return MGPU_ERR_UNSUPPORTED;
+    if (a->AdvectionNudging || a->unsupported_CellFluxes || a->unsupported_ReferenceProp ||
+        a->unsupported_BoundaryCondition || a->unsupported_SmallDepths ||
+        a->unsupported_NoAdvFlux || a->unsupported_NoDifFlux || a->unsupported_Discharges)
+        return MGPU_ERR_UNSUPPORTED;
+    if (!a->Wflux_X || !a->Wflux_Y || !a->Wflux_Z || !a->VolumeZOld || !a->VolumeZ ||
+        !a->OpenPoints3D || !a->LandPoints3D || !a->ComputeFacesU3D || !a->ComputeFacesV3D ||
+        !a->ComputeFacesW3D || !a->Visc_H || !a->Diff_V || !a->WaterPoints3D)
+        return MGPU_ERR_BADARG;
+    return MGPU_SUCCESS;
+}
+
+StepScalars make_scalars(const Dims &d, const mgpu_ad_args *a, int vertical1d, int xzflow) {
+    StepScalars sc;
+    sc.dt = a->DTProp;
+    sc.thV = a->ImpExp_DifV;
+    sc.volrelmax = a->VolumeRelMax;
+    sc.methodH = a->AdvMethodH; sc.tvdH = a->TVDLimitationH;
+    sc.methodV = a->AdvMethodV; sc.tvdV = a->TVDLimitationV;
+    sc.up2H = a->Upwind2H != 0; sc.up2V = a->Upwind2V != 0;
+    sc.optimize = (a->Optimize != 0) && !vertical1d;  // AD:1419
+    sc.vertical1d = vertical1d; sc.xzflow = xzflow;
+    sc.impAdvV = a->ImpExp_AdvV == 1.0;
+    sc.vert = d.wKUB > 1;                              // AD:1722
+    return sc;
+}
+
+SharedFields make_fields(const DevHydro &h, const DevDerived &dv, const DevGeometry &g) {
+    SharedFields f;
+    f.Qx = h.Qx; f.Qy = h.Qy; f.Qz = h.Qz; f.V = h.V; f.Vold = h.Vold;
+    f.cu = dv.cu; f.cv = dv.cv; f.cw = dv.cw;
+    f.DUX = g.DUX; f.DVY = g.DVY; f.DWZ = g.DWZ;
+    f.mask = dv.mask; f.kstart = dv.kstart;
+    return f;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+void mgpu_set_error(const char *what, hipError_t e) {
+    snprintf(g_err, sizeof(g_err), "%s: %s", what, hipGetErrorString(e));
+}
+
+Dims make_dims(const mgpu_size3d &s, const mgpu_size3d &w) {
+    Dims d;
+    d.ILB = s.ILB; d.IUB = s.IUB; d.JLB = s.JLB; d.JUB = s.JUB; d.KLB = s.KLB; d.KUB = s.KUB;
+    d.wILB = w.ILB; d.wIUB = w.IUB; d.wJLB = w.JLB; d.wJUB = w.JUB; d.wKLB = w.KLB; d.wKUB = w.KUB;
+    d.sj = (long)(s.IUB - s.ILB + 1);
+    d.sk = d.sj * (long)(s.JUB - s.JLB + 1);
+    return d;
+}
+
+// MOHID's invariant Size = WorkSize +/- 1 (ModuleGeometry.F90:815-816) with
+// WorkSize lower bounds 1; the 1-D face builders' index shift assumes it
+// (AD:4419, 2986).
+int check_dims(const mgpu_size3d *s, const mgpu_size3d *w) {
+    if (!s || !w) return MGPU_ERR_BADARG;
+    if (w->ILB != 1 || w->JLB != 1 || w->KLB != 1) return MGPU_ERR_BADARG;
+    if (w->IUB < 1 || w->JUB < 1 || w->KUB < 1) return MGPU_ERR_BADARG;
+    if (s->ILB != w->ILB - 1 || s->IUB != w->IUB + 1 || s->JLB != w->JLB - 1 ||
+        s->JUB != w->JUB + 1 || s->KLB != w->KLB - 1 || s->KUB != w->KUB + 1)
+        return MGPU_ERR_BADARG;
+    return MGPU_SUCCESS;
+}
+
+extern "C" const char *mgpu_last_error(void) { return g_err; }
+extern "C" const char *mgpu_version(void) { return "mohid-mi355x-transport 0.1 (gfx950)"; }
+
+// ---------------------------------------------------------------------------
+// AdvectionDiffusion drop-in
+struct mgpu_ad {
+    mgpu_size3d size, ws;
+    Dims d;
+    int docycle = 1, vertical1d = 0, xzflow = 0;
+    hipStream_t stream = nullptr;
+    DevGeometry geom;
+    DevHydro hydro;
+    DevDerived der;
+    double *prop = nullptr, *out = nullptr, *Wscr = nullptr;
+    bool have_faces = false;
+    unsigned long long last_zero_pivots = 0;
+};
+
+extern "C" int mgpu_ad_create(const mgpu_size3d *size, const mgpu_size3d *worksize,
+                              int32_t docycle_method, int32_t vertical1d, int32_t xzflow,
+                              const mgpu_geometry *geometry, mgpu_ad **out) {
+    if (!out) return MGPU_ERR_BADARG;
+    *out = nullptr;
+    int rc = check_dims(size, worksize);
+    if (rc) return rc;
+    mgpu_ad *ad = new (std::nothrow) mgpu_ad();
+    if (!ad) return MGPU_UNKNOWN;
+    ad->size = *size;
+    ad->ws = *worksize;
+    ad->d = make_dims(*size, *worksize);
+    ad->docycle = docycle_method;
+    ad->vertical1d = vertical1d != 0;
+    ad->xzflow = xzflow != 0;
+    hipError_t e = hipStreamCreate(&ad->stream);
+    if (e != hipSuccess) {
+        mgpu_set_error("hipStreamCreate", e);
+        delete ad;
+        return MGPU_ERR_HIP;
+    }
+    const long n3 = ad->d.n3();
+    rc = ad->geom.alloc(ad->d);
+    if (!rc) rc = ad->hydro.alloc(ad->d);
+    if (!rc) rc = ad->der.alloc(ad->d);
+    if (!rc) rc = dalloc(&ad->prop, n3);
+    if (!rc) rc = dalloc(&ad->out, n3);
+    if (!rc) rc = dalloc(&ad->Wscr, n3);
+    if (!rc) rc = ad->geom.upload(ad->d, geometry, ad->stream);
+    if (rc) {
+        mgpu_ad_destroy(ad);
+        return rc;
+    }
+    *out = ad;
+    return MGPU_SUCCESS;
+}
+
+extern "C" int mgpu_ad_set_geometry(mgpu_ad *ad, const mgpu_geometry *geometry) {
+    if (!ad) return MGPU_OFF_ERR;
+    return ad->geom.upload(ad->d, geometry, ad->stream);
+}
+
+extern "C" int mgpu_ad_destroy(mgpu_ad *ad) {
+    if (!ad) return MGPU_OFF_ERR;
+    if (ad->stream) (void)hipStreamSynchronize(ad->stream);
+    ad->geom.release();
+    ad->hydro.release();
+    ad->der.release();
+    dfree(ad->prop); dfree(ad->out); dfree(ad->Wscr);
+    if (ad->stream) (void)hipStreamDestroy(ad->stream);
+    delete ad;
+    return MGPU_SUCCESS;
+}
+
+extern "C" int mgpu_ad_zero_pivots(const mgpu_ad *ad, uint64_t *count) {
+    if (!ad) return MGPU_OFF_ERR;
+    if (!count) return MGPU_ERR_BADARG;
+    *count = ad->last_zero_pivots;
+    return MGPU_SUCCESS;
+}
+
+extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
+    if (!ad) return MGPU_OFF_ERR;
+    if (!a || !prop) return MGPU_ERR_BADARG;
+    int rc = check_options(a);
+    if (rc) return rc;
+    const Dims &d = ad->d;
+    hipStream_t s = ad->stream;
+    const StepScalars sc = make_scalars(d, a, ad->vertical1d, ad->xzflow);
+    const bool first = !sc.optimize || a->FirstProperty;
+    if (!first && !ad->have_faces) return MGPU_ERR_STATE;
+    const size_t b3 = sizeof(double) * (size_t)d.n3();
+
+    rc = ad->hydro.upload(d, a, s);
+    if (rc) return rc;
+    double *P = ad->prop;
+    if (a->device_pointers) P = prop;
+    else HIPCHK(hipMemcpyAsync(ad->prop, prop, b3, hipMemcpyHostToDevice, s));
+
+    HIPCHK(hipMemsetAsync(ad->der.counters, 0, sizeof(unsigned long long) * 4, s));
+    rc = launch_pack_masks(d, ad->hydro.open, ad->hydro.land, ad->hydro.cfu, ad->hydro.cfv,
+                           ad->hydro.cfw, ad->hydro.water, ad->geom.KFloor, ad->docycle,
+                           ad->der.mask, ad->der.kstart, ad->der.mask_flags(), s);
+    if (rc) return rc;
+    if (first) {
+        // Convert_Visc_Dif_Horizontal(_opt) + Convert_Dif_Vertical (+ the
+        // Optimize path's constants, AD:1421-1478); NullDif only on the
+        // non-optimised path (the _opt conversion ignores it, AD:2598-2675)
+        rc = launch_prep_faces(d, ad->hydro.visc, ad->hydro.diffv, ad->hydro.Qx, ad->hydro.Qy,
+                               ad->hydro.Qz, ad->geom.DUX, ad->geom.DVY, ad->geom.DZX,
+                               ad->geom.DZY, ad->geom.AreaU, ad->geom.AreaV, ad->geom.DZZ,
+                               ad->der.mask, a->schmidt_H, a->SchmidtCoef_V,
+                               a->SchmidtBackground_V, sc.optimize ? 0 : a->NullDif,
+                               ad->der.cu, ad->der.cv, ad->der.cw, s);
+        if (rc) return rc;
+        ad->have_faces = true;
+    }
+    const SharedFields f = make_fields(ad->hydro, ad->der, ad->geom);
+    rc = launch_transport_fp64(d, sc, f, P, ad->out, ad->Wscr, ad->der.counters, 0, 0, s);
+    if (rc) return rc;
+    rc = launch_copy_back(d, ad->der.mask, ad->out, P, s);
+    if (rc) return rc;
+    if (!a->device_pointers) HIPCHK(hipMemcpyAsync(prop, ad->prop, b3, hipMemcpyDeviceToHost, s));
+    unsigned long long cnt[4];
+    HIPCHK(hipMemcpyAsync(cnt, ad->der.counters, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    ad->last_zero_pivots = cnt[0];
+    if (cnt[1]) return MGPU_ERR_OPTION;          // method/limiter the reference stops on
+    const unsigned int mflags = (unsigned int)(cnt[2] & 0xffffffffu);
+    if (mflags) return MGPU_ERR_MASKS;
+    return MGPU_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+extern "C" int mgpu_thomasz(const mgpu_size3d *size, const mgpu_size3d *worksize,
+                            const double *D, const double *E, const double *F, const double *TI,
+                            const int32_t *waterpoints, double *res, int32_t dev, void *stream) {
+    int rc = check_dims(size, worksize);
+    if (rc) return rc;
+    if (!D || !E || !F || !TI || !waterpoints || !res) return MGPU_ERR_BADARG;
+    const Dims d = make_dims(*size, *worksize);
+    const long n3 = d.n3();
+    const size_t b3 = sizeof(double) * (size_t)n3;
+    hipStream_t s = (hipStream_t)stream;
+    double *W = nullptr;
+    if ((rc = dalloc(&W, n3))) return rc;
+    if (dev) {
+        rc = launch_thomasz(d, D, E, F, TI, waterpoints, res, W, s);
+        if (!rc) {
+            hipError_t e = hipStreamSynchronize(s);
+            if (e != hipSuccess) { mgpu_set_error("hipStreamSynchronize", e); rc = MGPU_ERR_HIP; }
+        }
+        dfree(W);
+        return rc;
+    }
+    double *dD = nullptr, *dE = nullptr, *dF = nullptr, *dT = nullptr, *dR = nullptr;
+    int32_t *dW = nullptr;
+    rc |= dalloc(&dD, n3); rc |= dalloc(&dE, n3); rc |= dalloc(&dF, n3);
+    rc |= dalloc(&dT, n3); rc |= dalloc(&dR, n3); rc |= dalloc(&dW, n3);
+    if (!rc) {
+        hipError_t e = hipSuccess;
+        if (e == hipSuccess) e = hipMemcpy(dD, D, b3, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dE, E, b3, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dF, F, b3, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dT, TI, b3, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dR, res, b3, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(dW, waterpoints, sizeof(int32_t) * (size_t)n3, hipMemcpyHostToDevice);
+        if (e != hipSuccess) { mgpu_set_error("hipMemcpy", e); rc = MGPU_ERR_HIP; }
+        if (!rc) rc = launch_thomasz(d, dD, dE, dF, dT, dW, dR, W, nullptr);
+        if (!rc) {
+            e = hipMemcpy(res, dR, b3, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) { mgpu_set_error("hipMemcpy", e); rc = MGPU_ERR_HIP; }
+        }
+    } else {
+        rc = MGPU_ERR_NOMEM;
+    }
+    dfree(dD); dfree(dE); dfree(dF); dfree(dT); dfree(dR); dfree(dW); dfree(W);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Batched device-resident transport
+struct mgpu_transport {
+    mgpu_size3d size, ws;
+    Dims d;
+    mgpu_transport_opts opts;
+    hipStream_t stream = nullptr;
+    DevGeometry geom;
+    DevHydro hydro;
+    DevDerived der;
+    bool have_hydro = false;
+    StepScalars sc;
+    double schmidtH = 1.0, schmidtV = 1.0, backgroundV = 0.0;
+    int nulldif = 0;
+    double **buf[2] = {nullptr, nullptr};
+    int cur = 0;
+    double *Wscr = nullptr;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    int graph_steps[2] = {0, 0};
+};
+
+static void drop_graphs(mgpu_transport *t) {
+    for (int p = 0; p < 2; ++p) {
+        if (t->exec[p]) (void)hipGraphExecDestroy(t->exec[p]);
+        if (t->graph[p]) (void)hipGraphDestroy(t->graph[p]);
+        t->exec[p] = nullptr;
+        t->graph[p] = nullptr;
+        t->graph_steps[p] = 0;
+    }
+}
+
+extern "C" int mgpu_transport_destroy(mgpu_transport *t) {
+    if (!t) return MGPU_OFF_ERR;
+    if (t->stream) (void)hipStreamSynchronize(t->stream);
+    drop_graphs(t);
+    t->geom.release();
+    t->hydro.release();
+    t->der.release();
+    for (int p = 0; p < 2; ++p) {
+        if (t->buf[p]) {
+            for (int q = 0; q < t->opts.ntracers; ++q) dfree(t->buf[p][q]);
+            delete[] t->buf[p];
+            t->buf[p] = nullptr;
+        }
+    }
+    dfree(t->Wscr);
+    if (t->stream) (void)hipStreamDestroy(t->stream);
+    delete t;
+    return MGPU_SUCCESS;
+}
+
+extern "C" int mgpu_transport_create(const mgpu_size3d *size, const mgpu_size3d *worksize,
+                                     const mgpu_transport_opts *opts,
+                                     const mgpu_geometry *geometry, mgpu_transport **out) {
+    if (!out || !opts) return MGPU_ERR_BADARG;
+    *out = nullptr;
+    int rc = check_dims(size, worksize);
+    if (rc) return rc;
+    if (opts->ntracers < 1 || opts->ntracers > 1024) return MGPU_ERR_BADARG;
+    if (opts->precision != MGPU_PREC_FP64 && opts->precision != MGPU_PREC_MIXED)
+        return MGPU_ERR_BADARG;
+    mgpu_transport *t = new (std::nothrow) mgpu_transport();
+    if (!t) return MGPU_UNKNOWN;
+    t->size = *size;
+    t->ws = *worksize;
+    t->d = make_dims(*size, *worksize);
+    t->opts = *opts;
+    hipError_t e = hipStreamCreate(&t->stream);
+    if (e != hipSuccess) {
+        mgpu_set_error("hipStreamCreate", e);
+        delete t;
+        return MGPU_ERR_HIP;
+    }
+    const long n3 = t->d.n3();
+    rc = t->geom.alloc(t->d);
+    if (!rc) rc = t->hydro.alloc(t->d);
+    if (!rc) rc = t->der.alloc(t->d);
+    if (!rc) rc = dalloc(&t->Wscr, n3);
+    for (int p = 0; p < 2 && !rc; ++p) {
+        t->buf[p] = new (std::nothrow) double *[opts->ntracers]();
+        if (!t->buf[p]) rc = MGPU_UNKNOWN;
+        for (int q = 0; q < opts->ntracers && !rc; ++q) rc = dalloc(&t->buf[p][q], n3);
+    }
+    if (!rc) rc = t->geom.upload(t->d, geometry, t->stream);
+    if (rc) {
+        mgpu_transport_destroy(t);
+        return rc;
+    }
+    *out = t;
+    return MGPU_SUCCESS;
+}
+
+extern "C" int mgpu_transport_set_hydro(mgpu_transport *t, const mgpu_ad_args *a) {
+    if (!t) return MGPU_OFF_ERR;
+    if (!a) return MGPU_ERR_BADARG;
+    int rc = check_options(a);
+    if (rc) return rc;
+    const int vertical1d = 0, xzflow = 0;
+    t->sc = make_scalars(t->d, a, vertical1d, xzflow);
+    // WP:14576-14598 picks the optimised path for >= 2 TVD/SuperBee tracers
+    t->schmidtH = a->schmidt_H;
+    t->schmidtV = a->SchmidtCoef_V;
+    t->backgroundV = a->SchmidtBackground_V;
+    t->nulldif = a->NullDif;
+    rc = t->hydro.upload(t->d, a, t->stream);
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(t->der.counters, 0, sizeof(unsigned long long) * 4, t->stream));
+    rc = launch_pack_masks(t->d, t->hydro.open, t->hydro.land, t->hydro.cfu, t->hydro.cfv,
+                           t->hydro.cfw, t->hydro.water, t->geom.KFloor, t->opts.docycle_method,
+                           t->der.mask, t->der.kstart, t->der.mask_flags(), t->stream);
+    if (rc) return rc;
+    unsigned long long cnt[4];
+    HIPCHK(hipMemcpyAsync(cnt, t->der.counters, sizeof(cnt), hipMemcpyDeviceToHost, t->stream));
+    HIPCHK(hipStreamSynchronize(t->stream));
+    if (cnt[2] & 0xffffffffu) return MGPU_ERR_MASKS;
+    drop_graphs(t);  // step scalars are baked into captured launches
+    t->have_hydro = true;
+    return MGPU_SUCCESS;
+}
+
+extern "C" int mgpu_transport_set_tracer(mgpu_transport *t, int32_t index, const double *prop,
+                                         int32_t dev) {
+    if (!t) return MGPU_OFF_ERR;
+    if (!prop || index < 0 || index >= t->opts.ntracers) return MGPU_ERR_BADARG;
+    const size_t b3 = sizeof(double) * (size_t)t->d.n3();
+    const hipMemcpyKind kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    // both ping-pong buffers start equal, so cells a step never writes
+    // (halos, land columns) stay identical in both
+    HIPCHK(hipMemcpyAsync(t->buf[0][index], prop, b3, kind, t->stream));
+    HIPCHK(hipMemcpyAsync(t->buf[1][index], prop, b3, kind, t->stream));
+    HIPCHK(hipStreamSynchronize(t->stream));
+    return MGPU_SUCCESS;
+}
+
+extern "C" int mgpu_transport_get_tracer(mgpu_transport *t, int32_t index, double *prop,
+                                         int32_t dev) {
+    if (!t) return MGPU_OFF_ERR;
+    if (!prop || index < 0 || index >= t->opts.ntracers) return MGPU_ERR_BADARG;
+    const size_t b3 = sizeof(double) * (size_t)t->d.n3();
+    HIPCHK(hipMemcpyAsync(prop, t->buf[t->cur][index], b3,
+                          dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, t->stream));
+    HIPCHK(hipStreamSynchronize(t->stream));
+    return MGPU_SUCCESS;
+}
+
+extern "C" int mgpu_transport_tracer_ptr(mgpu_transport *t, int32_t index, double **dev_ptr) {
+    if (!t) return MGPU_OFF_ERR;
+    if (!dev_ptr || index < 0 || index >= t->opts.ntracers) return MGPU_ERR_BADARG;
+    *dev_ptr = t->buf[t->cur][index];
+    return MGPU_SUCCESS;
+}
+
+extern "C" int mgpu_transport_stream(mgpu_transport *t, void **stream) {
+    if (!t) return MGPU_OFF_ERR;
+    if (!stream) return MGPU_ERR_BADARG;
+    *stream = (void *)t->stream;
+    return MGPU_SUCCESS;
+}
+
+// Enqueue `nsteps` steps starting from buffer parity `cur`.
+static int enqueue_steps(mgpu_transport *t, int nsteps, int cur) {
+    const Dims &d = t->d;
+    hipStream_t s = t->stream;
+    const SharedFields f = make_fields(t->hydro, t->der, t->geom);
+    for (int st = 0; st < nsteps; ++st) {
+        // first property of the step: face conductances (AD:1421-1478)
+        int rc = launch_prep_faces(d, t->hydro.visc, t->hydro.diffv, t->hydro.Qx, t->hydro.Qy,
+                                   t->hydro.Qz, t->geom.DUX, t->geom.DVY, t->geom.DZX,
+                                   t->geom.DZY, t->geom.AreaU, t->geom.AreaV, t->geom.DZZ,
+                                   t->der.mask, t->schmidtH, t->schmidtV, t->backgroundV,
+                                   t->sc.optimize ? 0 : t->nulldif, t->der.cu, t->der.cv,
+                                   t->der.cw, s);
+        if (rc) return rc;
+        for (int q = 0; q < t->opts.ntracers; ++q) {
+            if (t->opts.precision == MGPU_PREC_MIXED)
+                rc = launch_transport_mixed(d, t->sc, f, t->buf[cur][q], t->buf[cur ^ 1][q],
+                                            t->Wscr, t->der.counters, t->opts.tile_i,
+                                            t->opts.tile_j, s);
+            else
+                rc = launch_transport_fp64(d, t->sc, f, t->buf[cur][q], t->buf[cur ^ 1][q],
+                                           t->Wscr, t->der.counters, t->opts.tile_i,
+                                           t->opts.tile_j, s);
+            if (rc) return rc;
+        }
+        cur ^= 1;
+    }
+    return MGPU_SUCCESS;
+}
+
+extern "C" int mgpu_transport_run(mgpu_transport *t, int32_t nsteps, int32_t use_graph) {
+    if (!t) return MGPU_OFF_ERR;
+    if (nsteps < 0) return MGPU_ERR_BADARG;
+    if (!t->have_hydro) return MGPU_ERR_STATE;
+    if (nsteps == 0) return MGPU_SUCCESS;
+    if (!use_graph) {
+        int rc = enqueue_steps(t, nsteps, t->cur);
+        if (rc) return rc;
+    } else {
+        const int p = t->cur;
+        if (t->graph_steps[p] != nsteps) {
+            if (t->exec[p]) (void)hipGraphExecDestroy(t->exec[p]);
+            if (t->graph[p]) (void)hipGraphDestroy(t->graph[p]);
+            t->exec[p] = nullptr;
+            t->graph[p] = nullptr;
+            HIPCHK(hipStreamBeginCapture(t->stream, hipStreamCaptureModeThreadLocal));
+            int rc = enqueue_steps(t, nsteps, p);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(t->stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) {
+                mgpu_set_error("hipStreamEndCapture", e);
+                return MGPU_ERR_HIP;
+            }
+            t->graph[p] = g;
+            HIPCHK(hipGraphInstantiate(&t->exec[p], g, nullptr, nullptr, 0));
+            t->graph_steps[p] = nsteps;
+        }
+        HIPCHK(hipGraphLaunch(t->exec[p], t->stream));
+    }
+    t->cur ^= (nsteps & 1);
+    return MGPU_SUCCESS;
+}

@@ -1,0 +1,99 @@
+// Internal definitions shared by the HIP sources of the transport library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mohid_gpu.h"
+
+// GD:143, GD:1812 (default-real-8 literals)
+#define MGPU_NULL_REAL (-9.9E15)
+#define MGPU_MIN_VALUE (1.e-16)
+
+// GD:1810-1811
+enum { UPW1 = 1, UPW2 = 2, UPW3 = 3, TVD = 4, CDIF = 5, LEAPF = 6 };
+enum { LIM_MINMOD = 1, LIM_VANLEER = 2, LIM_MUSCL = 3, LIM_SUPERBEE = 4, LIM_PDM = 5 };
+
+// One byte per cell carries the six int32 masks of the reference.
+enum : uint8_t {
+    M_OPEN = 1,    // OpenPoints3D
+    M_LAND = 2,    // LandPoints3D
+    M_CFU = 4,     // ComputeFacesU3D (face between j-1 and j)
+    M_CFV = 8,     // ComputeFacesV3D (face between i-1 and i)
+    M_CFW = 16,    // ComputeFacesW3D (face between k-1 and k)
+    M_WATER = 32,  // WaterPoints3D
+};
+
+// Extents of a Size-shaped Fortran array and its WorkSize.
+struct Dims {
+    int ILB, IUB, JLB, JUB, KLB, KUB;       // Size
+    int wILB, wIUB, wJLB, wJUB, wKLB, wKUB; // WorkSize
+    long sj, sk;                            // strides of j and k (i is 1)
+    __host__ __device__ inline long idx(int i, int j, int k) const {
+        return (long)(i - ILB) + sj * (long)(j - JLB) + sk * (long)(k - KLB);
+    }
+    __host__ __device__ inline long idx2(int i, int j) const {
+        return (long)(i - ILB) + sj * (long)(j - JLB);
+    }
+    long n3() const { return sk * (long)(KUB - KLB + 1); }
+    long n2() const { return sk; }
+};
+
+Dims make_dims(const mgpu_size3d &size, const mgpu_size3d &ws);
+int check_dims(const mgpu_size3d *size, const mgpu_size3d *ws);
+
+// Error plumbing: every HIP call goes through HIPCHK, which records the
+// message and returns MGPU_ERR_HIP from the calling entry point.
+void mgpu_set_error(const char *what, hipError_t e);
+#define HIPCHK(x)                                   \
+    do {                                            \
+        hipError_t e_ = (x);                        \
+        if (e_ != hipSuccess) {                     \
+            mgpu_set_error(#x, e_);                 \
+            return MGPU_ERR_HIP;                    \
+        }                                           \
+    } while (0)
+
+// Scalars and pointers of one transport launch (passed by value).
+struct StepScalars {
+    double dt;
+    double thV;          // ImpExp_DifV
+    double volrelmax;
+    int methodH, tvdH, methodV, tvdV, up2H, up2V;
+    int optimize;        // AdvectionDiffusion's Optimize (after the Vertical1D rule)
+    int vertical1d, xzflow;
+    int impAdvV;         // ImpExp_AdvV == ImplicitScheme
+    int vert;            // WorkSize%KUB > 1
+};
+
+struct SharedFields {     // device pointers shared by all tracers
+    const double *Qx, *Qy, *Qz, *V, *Vold;
+    const double *cu, *cv, *cw;     // face conductances (prep kernel)
+    const double *DUX, *DVY, *DWZ;
+    const uint8_t *mask;
+    const int32_t *kstart;
+};
+
+// Kernel launchers (transport.hip)
+int launch_pack_masks(const Dims &d, const int32_t *open, const int32_t *land,
+                      const int32_t *cfu, const int32_t *cfv, const int32_t *cfw,
+                      const int32_t *water, const int32_t *kfloor, int docycle,
+                      uint8_t *mask, int32_t *kstart, unsigned int *flags, hipStream_t s);
+int launch_prep_faces(const Dims &d, const double *visc, const double *diffv,
+                      const double *Qx, const double *Qy, const double *Qz,
+                      const double *DUX, const double *DVY, const double *DZX,
+                      const double *DZY, const double *AreaU, const double *AreaV,
+                      const double *DZZ, const uint8_t *mask, double schmidtH,
+                      double schmidtV, double backgroundV, int nulldif,
+                      double *cu, double *cv, double *cw, hipStream_t s);
+int launch_transport_fp64(const Dims &d, const StepScalars &sc, const SharedFields &f,
+                          const double *P, double *out, double *Wscr,
+                          unsigned long long *zero_pivots, int tile_i, int tile_j,
+                          hipStream_t s);
+int launch_transport_mixed(const Dims &d, const StepScalars &sc, const SharedFields &f,
+                           const double *P, double *out, double *Wscr,
+                           unsigned long long *zero_pivots, int tile_i, int tile_j,
+                           hipStream_t s);
+int launch_thomasz(const Dims &d, const double *D, const double *E, const double *F,
+                   const double *TI, const int32_t *wp, double *res, double *Wscr,
+                   hipStream_t s);

@@ -1,0 +1,207 @@
+"""Parity of the HIP path with the oracle, through the C ABI, on the GPU.
+
+Every comparison runs the GPU library and the oracle (the reference's
+algorithm restated on the CPU) on identical seeded inputs.  The bar is the
+north-star tolerance for the FP64 path: |gpu - oracle| <= 1e-12 * |oracle|
+on every water cell after every step.  The FP64 kernels keep the reference's
+operation order and are compiled without contraction, so the tests also
+record whether the results are bit-identical.
+"""
+import numpy as np
+import pytest
+
+from mohid_amd import constants as C
+from oracle import oracle as ora
+
+from .cases import SMALL, make
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+
+
+def _ad():
+    from mohid_amd.advection_diffusion import AdvectionDiffusion, Transport, thomasz
+    return AdvectionDiffusion, Transport, thomasz
+
+
+def _rel_err(a, b, mask):
+    a = a[mask]
+    b = b[mask]
+    den = np.maximum(np.abs(b), 1e-300)
+    return float(np.max(np.abs(a - b) / den)) if a.size else 0.0
+
+
+def _solved_mask(case):
+    """Cells the reference writes: rows 1..KUB+1 of columns with
+    WaterPoints3D(i,j,KUB) == 1 (ModuleFunctions.F90:4086-4107)."""
+    wp = case.hydro["WaterPoints3D"]
+    kub = case.worksize[5]
+    m = np.zeros(wp.shape, dtype=bool)
+    col = wp[:, :, kub] == 1
+    m[:, :, 1:kub + 2] = col[:, :, None]
+    return m
+
+
+def test_thomasz_gpu_matches_oracle():
+    _, _, thomasz = _ad()
+    import ctypes as ct
+    rng = np.random.default_rng(5)
+    ni, nj, nk = 33, 21, 17
+    size = (0, ni + 1, 0, nj + 1, 0, nk + 1)
+    ws = (1, ni, 1, nj, 1, nk)
+    shape = (ni + 2, nj + 2, nk + 2)
+    D = np.asfortranarray(-rng.uniform(0, 1, shape))
+    F = np.asfortranarray(-rng.uniform(0, 1, shape))
+    E = np.asfortranarray(1.0 + rng.uniform(0, 1, shape) - D - F)
+    TI = np.asfortranarray(rng.normal(0, 3, shape))
+    WP = np.asfortranarray((rng.uniform(size=shape) < 0.8).astype(np.int32))
+    res_gpu = np.asfortranarray(rng.normal(size=shape))
+    res_ora = res_gpu.copy(order="F")
+    thomasz(size, ws, D, E, F, TI, WP, res_gpu)
+    w = np.zeros(nk + 3)
+    g = np.zeros(nk + 3)
+    dp = ct.POINTER(ct.c_double)
+    ora.lib().ora_thomasz(ct.byref(ora.Size3D(*size)), ct.byref(ora.Size3D(*ws)),
+                          D.ctypes.data_as(dp), E.ctypes.data_as(dp), F.ctypes.data_as(dp),
+                          TI.ctypes.data_as(dp), WP.ctypes.data_as(ct.POINTER(ct.c_int32)),
+                          res_ora.ctypes.data_as(dp), w.ctypes.data_as(dp), g.ctypes.data_as(dp))
+    assert np.array_equal(res_gpu, res_ora)
+
+
+@pytest.mark.parametrize("name", sorted(SMALL))
+def test_advection_diffusion_matches_oracle(name):
+    """Per-tracer AdvectionDiffusion calls (the WaterProperties call pattern,
+    WP:14776-14822) for 6 steps, GPU vs oracle after every call."""
+    AD, _, _ = _ad()
+    case = make(name)
+    steps = 6
+    gpu = AD(case.size, case.worksize, case.geom)
+    inst = ora.Instance(case, nthreads=1)
+    p_gpu = [np.array(t, order="F", copy=True) for t in case.tracers]
+    p_ora = [np.array(t, order="F", copy=True) for t in case.tracers]
+    solved = _solved_mask(case)
+    bitwise = True
+    for s in range(steps):
+        for t in range(len(p_gpu)):
+            first = t == 0
+            rc = inst.step(p_ora[t], first_property=first, now=(s + 1) * case.params["DTProp"])
+            assert rc == 0
+            gpu(p_gpu[t], case.hydro, case.params, first_property=first)
+            err = _rel_err(p_gpu[t], p_ora[t], solved)
+            assert err <= RTOL, (name, s, t, err)
+            # cells outside the solved columns are never written
+            assert np.array_equal(p_gpu[t][~solved], p_ora[t][~solved])
+            bitwise &= bool(np.array_equal(p_gpu[t], p_ora[t]))
+    gpu.kill()
+    inst.close()
+    print(f"{name}: bit-identical={bitwise}")
+
+
+@pytest.mark.parametrize("variant", [
+    dict(ImpExp_AdvV=C.EXPLICIT_SCHEME),
+    dict(ImpExp_DifV=C.EXPLICIT_SCHEME),
+    dict(ImpExp_DifV=0.5),
+    dict(AdvMethodH=C.UPWIND_ORDER2, AdvMethodV=C.UPWIND_ORDER2, ImpExp_AdvV=C.EXPLICIT_SCHEME),
+    dict(AdvMethodH=C.UPWIND_ORDER3, AdvMethodV=C.P2_TVD, TVDLimitationV=C.MINMOD),
+    dict(AdvMethodH=C.CENTRAL_DIF, AdvMethodV=C.CENTRAL_DIF),
+    dict(AdvMethodH=C.P2_TVD, TVDLimitationH=C.VANLEER, TVDLimitationV=C.MUSCL),
+    dict(AdvMethodH=C.P2_TVD, TVDLimitationH=C.PDM, AdvMethodV=C.UPWIND_ORDER1),
+    dict(NullDif=True),
+])
+def test_advection_diffusion_options(variant):
+    """Method/limiter/theta variants of the non-optimised path."""
+    AD, _, _ = _ad()
+    case = make("tvd_land_sigma")
+    params = dict(case.params)
+    params.update(variant)
+    params["Optimize"] = False
+    params.setdefault("Upwind2H", True)
+    params["Upwind2H"] = True
+    params["Upwind2V"] = True
+    gpu = AD(case.size, case.worksize, case.geom)
+    inst = ora.Instance(case, nthreads=1)
+    p_gpu = np.array(case.tracers[0], order="F", copy=True)
+    p_ora = p_gpu.copy(order="F")
+    solved = _solved_mask(case)
+    for s in range(4):
+        rc = inst.step(p_ora, first_property=True, now=(s + 1) * 30.0, **params)
+        assert rc == 0
+        gpu(p_gpu, case.hydro, params, first_property=True)
+        err = _rel_err(p_gpu, p_ora, solved)
+        assert err <= RTOL, (variant, s, err)
+    gpu.kill()
+
+
+def test_batched_transport_matches_oracle():
+    """Device-resident multi-tracer transport (hipGraph of 5 steps + 2 eager
+    steps) against the oracle's per-tracer Optimize call pattern."""
+    _, Transport, _ = _ad()
+    case = make("tvd_ragged_opt")
+    T = len(case.tracers)
+    tr = Transport(case.size, case.worksize, case.geom, T)
+    tr.set_hydro(case.hydro, case.params)
+    for t, p in enumerate(case.tracers):
+        tr.set_tracer(t, p)
+    tr.run(5, use_graph=True)
+    tr.run(2, use_graph=False)
+    ref = ora.run_case(case, 7, nthreads=1)
+    solved = _solved_mask(case)
+    for t in range(T):
+        got = tr.get_tracer(t)
+        err = _rel_err(got, ref[t], solved)
+        assert err <= RTOL, (t, err)
+        assert np.array_equal(got[~solved], case.tracers[t][~solved])
+    tr.close()
+
+
+def test_batched_graph_replays_agree():
+    """A captured step loop replayed several times gives the eager result."""
+    _, Transport, _ = _ad()
+    case = make("tvd_land_sigma")
+    T = len(case.tracers)
+    a = Transport(case.size, case.worksize, case.geom, T)
+    b = Transport(case.size, case.worksize, case.geom, T)
+    for tr in (a, b):
+        tr.set_hydro(case.hydro, case.params)
+        for t, p in enumerate(case.tracers):
+            tr.set_tracer(t, p)
+    for rep in range(3):
+        a.run(3, use_graph=True)
+        b.run(3, use_graph=False)
+        for t in range(T):
+            assert np.array_equal(a.get_tracer(t), b.get_tracer(t)), (rep, t)
+    a.close()
+    b.close()
+
+
+def test_rejects_like_the_reference():
+    """Option sets the reference stops on, and branches not offloaded."""
+    from mohid_amd import _lib as L
+    AD, _, _ = _ad()
+    case = make("upwind_flat")
+    gpu = AD(case.size, case.worksize, case.geom)
+    p = np.array(case.tracers[0], order="F", copy=True)
+    bad = [
+        (dict(ImpExp_DifH=1.0), L.MGPU_ERR_OPTION),               # AD:1340 ERR02
+        (dict(ImpExp_AdvXX=1.0, ImpExp_AdvYY=1.0), L.MGPU_ERR_OPTION),  # ERR03
+        (dict(ImpExp_AdvXX=1.0), L.MGPU_ERR_UNSUPPORTED),          # THOMAS_3D path
+        (dict(CellFluxes=True), L.MGPU_ERR_UNSUPPORTED),
+        (dict(Discharges=True), L.MGPU_ERR_UNSUPPORTED),
+    ]
+    for ov, code in bad:
+        params = dict(case.params)
+        params.update(ov)
+        with pytest.raises(L.MgpuError) as ei:
+            gpu(p, case.hydro, params)
+        assert ei.value.code == code, ov
+    # Optimize call without a FirstProperty call
+    params = dict(case.params, Optimize=True)
+    g2 = AD(case.size, case.worksize, case.geom)
+    with pytest.raises(L.MgpuError) as ei:
+        g2(p, case.hydro, params, first_property=False)
+    assert ei.value.code == L.MGPU_ERR_STATE
+    # the input is left untouched by rejected calls
+    assert np.array_equal(p, case.tracers[0])
+    gpu.kill()
+    g2.kill()

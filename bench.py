@@ -1,0 +1,245 @@
+"""Benchmark of the MI355X tracer-transport step.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3]
+                    [--precision fp64|mixed] [--no-graph] [--no-cpu]
+
+Workload (default): 8 tracers on a 1024 x 1024 x 40 Cartesian grid with ~30 %
+land and per-column KFloor uniform in 5..40, TVD-SuperBee advection in all
+directions, implicit vertical diffusion/advection, dt = 60 s -- the batched
+multi-tracer step the headline metric is quoted on.  Synthetic, seeded inputs
+(mohid_amd/fixtures.py); all inputs are resident in HBM before timing.
+
+One step = one advection-diffusion pass of every tracer (face conductances
+once per step, then one transport launch per tracer), as the reference's
+WaterProperties loop does per time step.  The timed region replays K such
+steps as one captured hipGraph.
+
+Prints ONE JSON line.  `value` is tracer cell-updates/s over water cells
+(T x N_wet x K / wall).  `roofline` is for the per-tracer transport kernel:
+algorithmic bytes per launch = (16 + 64/T) B x N_wet (FP64 byte model: one
+read and one write of the tracer plus eight shared FP64 fields amortised over
+T tracers) divided by the kernel's mean duration from HIP events recorded on
+the library's stream around each launch.  `cpu_baseline` times the CPU oracle
+(a restatement of the reference algorithm, OpenMP over the host cores) on a
+bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def _log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _traffic_from_profiles(kernel_key: str, workload: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
+    WRITE_SIZE), as recorded in profiles/*.md (a JSON block per workload)."""
+    prof = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(prof):
+        return None
+    for fn in sorted(os.listdir(prof)):
+        if not fn.endswith(".md"):
+            continue
+        text = open(os.path.join(prof, fn)).read()
+        for m in re.finditer(r"```json\n(.*?)\n```", text, re.S):
+            try:
+                rec = json.loads(m.group(1))
+            except ValueError:
+                continue
+            if rec.get("kernel") == kernel_key and rec.get("workload") == workload:
+                return rec.get("traffic_bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c3")
+    ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed"))
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-window", type=int, default=384)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--tile", default="0,0")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only (the transport step's "
+              "single-GPU kernel); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from mohid_amd import _lib
+    from mohid_amd import fixtures as fx
+    from mohid_amd.advection_diffusion import Transport
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(2)
+    torch.cuda.set_device(0)
+    _lib.lib()
+
+    cfg = fx.config(args.config)
+    t0 = time.time()
+    case = fx.make_case(cfg, device="cuda")
+    T = len(case.tracers)
+    nwet = case.n_wet
+    _log(f"case {cfg.name}: {cfg.ni}x{cfg.nj}x{cfg.nk}, {T} tracers, {nwet} water cells "
+         f"({time.time() - t0:.1f}s to build)")
+
+    prec = _lib.MGPU_PREC_MIXED if args.precision == "mixed" else _lib.MGPU_PREC_FP64
+    tile = tuple(int(x) for x in args.tile.split(","))
+    tr = Transport(case.size, case.worksize, case.geom, T, precision=prec, tile=tile)
+    tr.set_hydro(case.hydro, case.params)
+    for t, p in enumerate(case.tracers):
+        tr.set_tracer(t, p)
+    use_graph = not args.no_graph
+
+    # ---- warm-up
+    if args.warmup > 0:
+        tr.run(args.warmup, use_graph=use_graph)
+    torch.cuda.synchronize()
+
+    # ---- timed region: K steps (one captured graph when use_graph)
+    stream = torch.cuda.ExternalStream(tr.stream_handle())
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    if use_graph:
+        tr.run(args.steps, use_graph=True)   # capture + instantiate outside the timed region
+        torch.cuda.synchronize()
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    ev0.record(stream)
+    tr.run(args.steps, use_graph=use_graph)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    ev_ms = ev0.elapsed_time(ev1)
+    ms_per_step = 1e3 * wall / args.steps
+    value = T * nwet * args.steps / wall
+    _log(f"timed {args.steps} steps: {wall * 1e3:.2f} ms wall, {ev_ms:.2f} ms events, "
+         f"{value:.4e} cell-updates/s")
+
+    # ---- per-kernel durations (HIP events around every launch, eager steps)
+    ksteps = max(2, min(args.steps, 10))
+    tr.timing(True, ksteps)
+    tr.run(ksteps, use_graph=False)
+    prep_ms, trans_ms, nlaunch = tr.kernel_times()
+    tr.timing(False)
+    per_unit = 16.0 + 64.0 / T
+    alg_bytes = per_unit * nwet
+    achieved = alg_bytes / (trans_ms * 1e-3) / 1e9
+    kernel_key = f"transport_{args.precision}"
+    traffic = _traffic_from_profiles(kernel_key, cfg.name)
+    _log(f"transport kernel {trans_ms:.3f} ms/launch over {nlaunch} launches, "
+         f"prep {prep_ms:.3f} ms; {achieved:.1f} GB/s algorithmic")
+
+    out = {
+        "metric": "tracer cell-updates/s (%HBM peak), 1024^2x40x8 tracers, batched GPU step",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded fixtures, mohid_amd/fixtures.py)",
+        "config": {
+            "workload": f"{cfg.name}: {cfg.ni}x{cfg.nj}x{cfg.nk}, {T} tracers, "
+                        f"{cfg.vertical}, land {cfg.land_frac:.2f}, KFloor {cfg.kfloor}, "
+                        f"TVD-SuperBee H+V, dt {cfg.dt:g}s",
+            "n_wet_cells": nwet,
+            "precision": args.precision,
+            "graph": use_graph,
+            "optimize_path": bool(case.params["Optimize"]),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": kernel_key,
+            "kernel_ms": trans_ms,
+            "alg_bytes_per_launch": alg_bytes,
+            "prep_kernel_ms": prep_ms,
+        },
+    }
+
+    # ---- CPU baseline: the oracle on a bounded sample of the same workload
+    # (the same configuration on a smaller horizontal window), timed on the
+    # host cores; the GPU path runs the same sample for the parity figure
+    if not args.no_cpu:
+        from oracle import oracle as ora
+        nthreads = min(16, os.cpu_count() or 1)
+        scfg = fx.config(cfg.name, ni=min(cfg.ni, args.cpu_window),
+                         nj=min(cfg.nj, args.cpu_window))
+        scase = fx.make_case(scfg)
+        sT, swet = len(scase.tracers), scase.n_wet
+        inst = ora.Instance(scase, nthreads=nthreads)
+        props = [np.array(t, order="F", copy=True) for t in scase.tracers]
+        dt = scase.params["DTProp"]
+
+        def ostep(s):
+            for t, p in enumerate(props):
+                rc = inst.step(p, first_property=(t == 0), now=(s + 1) * dt)
+                assert rc == 0, rc
+
+        ostep(0)                      # untimed first step
+        c0 = time.perf_counter()
+        nsteps = 0
+        while nsteps < 2 or (time.perf_counter() - c0 < args.cpu_seconds and nsteps < 50):
+            ostep(nsteps + 1)
+            nsteps += 1
+        cwall = time.perf_counter() - c0
+        inst.close()
+        cval = sT * swet * nsteps / cwall
+        tr2 = Transport(scase.size, scase.worksize, scase.geom, sT, precision=prec, tile=tile)
+        tr2.set_hydro(scase.hydro, scase.params)
+        for t, p in enumerate(scase.tracers):
+            tr2.set_tracer(t, p)
+        tr2.run(1 + nsteps, use_graph=False)
+        wp = scase.hydro["WaterPoints3D"] == 1
+        worst = 0.0
+        for t in range(sT):
+            got = tr2.get_tracer(t)
+            err = np.abs(got[wp] - props[t][wp]) / np.maximum(np.abs(props[t][wp]), 1e-300)
+            worst = max(worst, float(err.max()))
+        tr2.close()
+        out["cpu_baseline"] = {
+            "value": cval, "unit": "cell-updates/s", "cores": nthreads, "kind": "port",
+            "sample": f"{nsteps} steps x {sT} tracers of {scfg.name} on a {scfg.ni}x{scfg.nj}"
+                      f"x{scfg.nk} window ({swet} water cells) after 1 untimed step; "
+                      f"oracle/advdiff_oracle.c with OpenMP",
+            "wall_s": cwall,
+        }
+        out["parity_vs_oracle"] = {"sample": f"{scfg.ni}x{scfg.nj}x{scfg.nk}",
+                                   "steps": 1 + nsteps, "max_rel_err": worst}
+        _log(f"cpu oracle {cval:.4e} cell-updates/s on {nthreads} threads ({nsteps} steps, "
+             f"{cwall:.1f}s); gpu vs oracle max rel err {worst:.3e}")
+    tr.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

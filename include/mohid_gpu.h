@@ -64,6 +64,7 @@ typedef struct {
     const int32_t *KFloorZ;                /* GEO KFloor (2-D) */
     const double *AreaU, *AreaV;           /* GEO areas (3-D) */
     const double *DZZ, *DWZ;               /* GEO distances (3-D) */
+    int32_t device_pointers;               /* 1: all of the above are device pointers */
 } mgpu_geometry;
 
 /* Arguments of AdvectionDiffusion (AD:1108-1198) in the reference's order.
@@ -157,6 +158,15 @@ int mgpu_transport_run(mgpu_transport *t, int32_t nsteps, int32_t use_graph);
 int mgpu_transport_tracer_ptr(mgpu_transport *t, int32_t index, double **dev_ptr);
 int mgpu_transport_stream(mgpu_transport *t, void **stream);
 int mgpu_transport_destroy(mgpu_transport *t);
+/* Per-launch timing for measurement: with enable = 1 the following eager
+ * mgpu_transport_run calls (use_graph = 0, up to max_steps steps) record a
+ * hipEvent pair around every kernel on the handle's stream;
+ * mgpu_transport_kernel_times returns the mean duration (ms) of the
+ * face-conductance kernel and of the per-tracer transport kernel, and the
+ * number of transport launches timed. */
+int mgpu_transport_timing(mgpu_transport *t, int32_t enable, int32_t max_steps);
+int mgpu_transport_kernel_times(mgpu_transport *t, double *prep_ms, double *transport_ms,
+                                int32_t *launches);
 
 /* Library version string and the last HIP error message (thread-local). */
 const char *mgpu_version(void);

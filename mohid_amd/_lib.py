@@ -42,7 +42,8 @@ class Size3D(ct.Structure):
 
 class Geometry(ct.Structure):
     _fields_ = [("DUX", _dp), ("DVY", _dp), ("DZX", _dp), ("DZY", _dp),
-                ("KFloorZ", _ip), ("AreaU", _dp), ("AreaV", _dp), ("DZZ", _dp), ("DWZ", _dp)]
+                ("KFloorZ", _ip), ("AreaU", _dp), ("AreaV", _dp), ("DZZ", _dp), ("DWZ", _dp),
+                ("device_pointers", ct.c_int32)]
 
 
 class ADArgs(ct.Structure):
@@ -83,7 +84,8 @@ EXPORTS = (
     "mgpu_ad_zero_pivots", "mgpu_thomasz",
     "mgpu_transport_create", "mgpu_transport_set_hydro", "mgpu_transport_set_tracer",
     "mgpu_transport_get_tracer", "mgpu_transport_run", "mgpu_transport_tracer_ptr",
-    "mgpu_transport_stream", "mgpu_transport_destroy",
+    "mgpu_transport_stream", "mgpu_transport_destroy", "mgpu_transport_timing",
+    "mgpu_transport_kernel_times",
     "mgpu_version", "mgpu_last_error",
 )
 
@@ -97,6 +99,14 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"HIP library not built: {LIB_PATH} (python -m mohid_amd.build)")
+    # One HIP runtime per process: torch ships its own libamdhip64 (same
+    # SONAME, different file).  Loading torch first makes this library's
+    # libamdhip64.so.7 dependency resolve to that already-loaded runtime, so
+    # device tensors and library handles share one runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ct.CDLL(LIB_PATH)
     vp = ct.c_void_p
     L.mgpu_ad_create.argtypes = [ct.POINTER(Size3D), ct.POINTER(Size3D), ct.c_int32, ct.c_int32,
@@ -117,6 +127,9 @@ def lib():
     L.mgpu_transport_tracer_ptr.argtypes = [vp, ct.c_int32, ct.POINTER(vp)]
     L.mgpu_transport_stream.argtypes = [vp, ct.POINTER(vp)]
     L.mgpu_transport_destroy.argtypes = [vp]
+    L.mgpu_transport_timing.argtypes = [vp, ct.c_int32, ct.c_int32]
+    L.mgpu_transport_kernel_times.argtypes = [vp, ct.POINTER(ct.c_double),
+                                              ct.POINTER(ct.c_double), ct.POINTER(ct.c_int32)]
     L.mgpu_version.restype = ct.c_char_p
     L.mgpu_last_error.restype = ct.c_char_p
     for name in EXPORTS:

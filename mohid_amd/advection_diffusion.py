@@ -43,6 +43,30 @@ def _addr(a, keep: list, dtype):
 
 
 def _geometry(geom: Dict[str, np.ndarray], keep: list) -> L.Geometry:
+    """mgpu_geometry from host arrays, or from device tensors when the 3-D
+    geometry is on the GPU (2-D metrics are then uploaded as well, so that
+    every pointer of the struct lives on the same side)."""
+    device = _is_device(geom["DWZ"])
+    if device:
+        import torch
+        dev = geom["DWZ"].device
+
+        def up(n, dt):
+            a = geom[n]
+            if not _is_device(a):
+                a = np.asfortranarray(a, dtype=dt)
+                # Fortran (nI, nJ) layout == C layout of the transposed array
+                a = torch.from_numpy(np.ascontiguousarray(a.T)).to(dev)
+            keep.append(a)
+            return a.data_ptr()
+        ptr = {n: up(n, np.float64) for n in ("DUX", "DVY", "DZX", "DZY", "AreaU", "AreaV",
+                                               "DZZ", "DWZ")}
+        ptr["KFloorZ"] = up("KFloorZ", np.int32)
+        dp = lambda n: ct.cast(ct.c_void_p(ptr[n]), ct.POINTER(ct.c_double))
+        return L.Geometry(dp("DUX"), dp("DVY"), dp("DZX"), dp("DZY"),
+                          ct.cast(ct.c_void_p(ptr["KFloorZ"]), ct.POINTER(ct.c_int32)),
+                          dp("AreaU"), dp("AreaV"), dp("DZZ"), dp("DWZ"), 1)
+
     def d(n):
         a = np.asfortranarray(geom[n], dtype=np.float64)
         keep.append(a)
@@ -51,7 +75,7 @@ def _geometry(geom: Dict[str, np.ndarray], keep: list) -> L.Geometry:
     keep.append(kf)
     return L.Geometry(d("DUX"), d("DVY"), d("DZX"), d("DZY"),
                       kf.ctypes.data_as(ct.POINTER(ct.c_int32)),
-                      d("AreaU"), d("AreaV"), d("DZZ"), d("DWZ"))
+                      d("AreaU"), d("AreaV"), d("DZZ"), d("DWZ"), 0)
 
 
 def _args(hydro: Dict, params: Dict, keep: list, device: bool, optimize: bool,
@@ -203,6 +227,18 @@ class Transport:
     def run(self, nsteps: int, use_graph: bool = True) -> None:
         L.check("mgpu_transport_run", L.lib().mgpu_transport_run(self._h, int(nsteps),
                                                                  int(bool(use_graph))))
+
+    def timing(self, enable: bool, max_steps: int = 64) -> None:
+        L.check("mgpu_transport_timing", L.lib().mgpu_transport_timing(self._h, int(enable),
+                                                                       int(max_steps)))
+
+    def kernel_times(self):
+        """(mean prep-kernel ms, mean transport-kernel ms, transport launches)."""
+        a, b, n = ct.c_double(), ct.c_double(), ct.c_int32()
+        L.check("mgpu_transport_kernel_times",
+                L.lib().mgpu_transport_kernel_times(self._h, ct.byref(a), ct.byref(b),
+                                                    ct.byref(n)))
+        return a.value, b.value, n.value
 
     def stream_handle(self) -> int:
         s = ct.c_void_p()

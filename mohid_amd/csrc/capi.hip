@@ -75,16 +75,17 @@ struct DevGeometry {
             !g->AreaV || !g->DZZ || !g->DWZ)
             return MGPU_ERR_BADARG;
         const size_t b2 = sizeof(double) * (size_t)d.n2(), b3 = sizeof(double) * (size_t)d.n3();
-        HIPCHK(hipMemcpyAsync(DUX, g->DUX, b2, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(DVY, g->DVY, b2, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(DZX, g->DZX, b2, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(DZY, g->DZY, b2, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(KFloor, g->KFloorZ, sizeof(int32_t) * (size_t)d.n2(),
-                              hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(AreaU, g->AreaU, b3, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(AreaV, g->AreaV, b3, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(DZZ, g->DZZ, b3, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(DWZ, g->DWZ, b3, hipMemcpyHostToDevice, s));
+        const hipMemcpyKind kd =
+            g->device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        HIPCHK(hipMemcpyAsync(DUX, g->DUX, b2, kd, s));
+        HIPCHK(hipMemcpyAsync(DVY, g->DVY, b2, kd, s));
+        HIPCHK(hipMemcpyAsync(DZX, g->DZX, b2, kd, s));
+        HIPCHK(hipMemcpyAsync(DZY, g->DZY, b2, kd, s));
+        HIPCHK(hipMemcpyAsync(KFloor, g->KFloorZ, sizeof(int32_t) * (size_t)d.n2(), kd, s));
+        HIPCHK(hipMemcpyAsync(AreaU, g->AreaU, b3, kd, s));
+        HIPCHK(hipMemcpyAsync(AreaV, g->AreaV, b3, kd, s));
+        HIPCHK(hipMemcpyAsync(DZZ, g->DZZ, b3, kd, s));
+        HIPCHK(hipMemcpyAsync(DWZ, g->DWZ, b3, kd, s));
         HIPCHK(hipStreamSynchronize(s));
         return MGPU_SUCCESS;
     }
@@ -438,7 +439,27 @@ struct mgpu_transport {
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     int graph_steps[2] = {0, 0};
+    // optional per-launch timing: event pairs around every kernel of a step
+    int timing = 0;
+    hipEvent_t *ev = nullptr;        // 2 per launch: [prep, tracer 0 .. T-1] per step
+    int ev_cap = 0, ev_used = 0;
 };
+
+// record an event of the timing pool (no-op when timing is off or the pool is full)
+static int mark(mgpu_transport *t) {
+    if (!t->timing || t->ev_used >= t->ev_cap) return MGPU_SUCCESS;
+    HIPCHK(hipEventRecord(t->ev[t->ev_used++], t->stream));
+    return MGPU_SUCCESS;
+}
+
+static void drop_events(mgpu_transport *t) {
+    if (t->ev) {
+        for (int q = 0; q < t->ev_cap; ++q) (void)hipEventDestroy(t->ev[q]);
+        delete[] t->ev;
+    }
+    t->ev = nullptr;
+    t->ev_cap = t->ev_used = 0;
+}
 
 static void drop_graphs(mgpu_transport *t) {
     for (int p = 0; p < 2; ++p) {
@@ -454,6 +475,7 @@ extern "C" int mgpu_transport_destroy(mgpu_transport *t) {
     if (!t) return MGPU_OFF_ERR;
     if (t->stream) (void)hipStreamSynchronize(t->stream);
     drop_graphs(t);
+    drop_events(t);
     t->geom.release();
     t->hydro.release();
     t->der.release();
@@ -585,14 +607,18 @@ static int enqueue_steps(mgpu_transport *t, int nsteps, int cur) {
     const SharedFields f = make_fields(t->hydro, t->der, t->geom);
     for (int st = 0; st < nsteps; ++st) {
         // first property of the step: face conductances (AD:1421-1478)
-        int rc = launch_prep_faces(d, t->hydro.visc, t->hydro.diffv, t->hydro.Qx, t->hydro.Qy,
+        int rc = mark(t);
+        if (rc) return rc;
+        rc = launch_prep_faces(d, t->hydro.visc, t->hydro.diffv, t->hydro.Qx, t->hydro.Qy,
                                    t->hydro.Qz, t->geom.DUX, t->geom.DVY, t->geom.DZX,
                                    t->geom.DZY, t->geom.AreaU, t->geom.AreaV, t->geom.DZZ,
                                    t->der.mask, t->schmidtH, t->schmidtV, t->backgroundV,
                                    t->sc.optimize ? 0 : t->nulldif, t->der.cu, t->der.cv,
                                    t->der.cw, s);
         if (rc) return rc;
+        if ((rc = mark(t))) return rc;
         for (int q = 0; q < t->opts.ntracers; ++q) {
+            if ((rc = mark(t))) return rc;
             if (t->opts.precision == MGPU_PREC_MIXED)
                 rc = launch_transport_mixed(d, t->sc, f, t->buf[cur][q], t->buf[cur ^ 1][q],
                                             t->Wscr, t->der.counters, t->opts.tile_i,
@@ -602,6 +628,7 @@ static int enqueue_steps(mgpu_transport *t, int nsteps, int cur) {
                                            t->Wscr, t->der.counters, t->opts.tile_i,
                                            t->opts.tile_j, s);
             if (rc) return rc;
+            if ((rc = mark(t))) return rc;
         }
         cur ^= 1;
     }
@@ -617,6 +644,7 @@ extern "C" int mgpu_transport_run(mgpu_transport *t, int32_t nsteps, int32_t use
         int rc = enqueue_steps(t, nsteps, t->cur);
         if (rc) return rc;
     } else {
+        if (t->timing) return MGPU_ERR_STATE;  // events are not captured into graphs
         const int p = t->cur;
         if (t->graph_steps[p] != nsteps) {
             if (t->exec[p]) (void)hipGraphExecDestroy(t->exec[p]);
@@ -642,5 +670,60 @@ extern "C" int mgpu_transport_run(mgpu_transport *t, int32_t nsteps, int32_t use
         HIPCHK(hipGraphLaunch(t->exec[p], t->stream));
     }
     t->cur ^= (nsteps & 1);
+    return MGPU_SUCCESS;
+}
+
+// Timing of individual launches (eager steps only): with `enable`, the next
+// mgpu_transport_run(nsteps, 0) records a hipEvent pair around every kernel
+// on the handle's stream; mgpu_transport_kernel_times then returns the mean
+// duration of the face-conductance kernel and of the transport kernel.
+extern "C" int mgpu_transport_timing(mgpu_transport *t, int32_t enable, int32_t max_steps) {
+    if (!t) return MGPU_OFF_ERR;
+    HIPCHK(hipStreamSynchronize(t->stream));
+    drop_events(t);
+    t->timing = 0;
+    if (!enable) return MGPU_SUCCESS;
+    if (max_steps < 1) return MGPU_ERR_BADARG;
+    const int cap = 2 * max_steps * (1 + t->opts.ntracers);
+    t->ev = new (std::nothrow) hipEvent_t[cap];
+    if (!t->ev) return MGPU_UNKNOWN;
+    for (int q = 0; q < cap; ++q) {
+        hipError_t e = hipEventCreate(&t->ev[q]);
+        if (e != hipSuccess) {
+            t->ev_cap = q;
+            drop_events(t);
+            mgpu_set_error("hipEventCreate", e);
+            return MGPU_ERR_HIP;
+        }
+    }
+    t->ev_cap = cap;
+    t->ev_used = 0;
+    t->timing = 1;
+    return MGPU_SUCCESS;
+}
+
+extern "C" int mgpu_transport_kernel_times(mgpu_transport *t, double *prep_ms,
+                                           double *transport_ms, int32_t *launches) {
+    if (!t) return MGPU_OFF_ERR;
+    if (!prep_ms || !transport_ms || !launches) return MGPU_ERR_BADARG;
+    HIPCHK(hipStreamSynchronize(t->stream));
+    const int per_step = 2 * (1 + t->opts.ntracers);
+    const int steps = t->ev_used / per_step;
+    double sp = 0.0, st = 0.0;
+    int np = 0, nt = 0;
+    for (int s = 0; s < steps; ++s) {
+        hipEvent_t *e = t->ev + (long)s * per_step;
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e[0], e[1]));
+        sp += ms; ++np;
+        for (int q = 0; q < t->opts.ntracers; ++q) {
+            HIPCHK(hipEventElapsedTime(&ms, e[2 + 2 * q], e[3 + 2 * q]));
+            st += ms; ++nt;
+        }
+    }
+    *prep_ms = np ? sp / np : 0.0;
+    *transport_ms = nt ? st / nt : 0.0;
+    *launches = nt;
+    t->ev_used = 0;
     return MGPU_SUCCESS;
 }

@@ -158,15 +158,58 @@ def _arr3(shape, dtype=np.float64, fill=0.0):
     return np.full(shape, fill, dtype=dtype, order="F")
 
 
-def make_case(cfg: CaseConfig) -> Case:
-    """Build a synthetic case (deterministic in cfg.seed)."""
+class _NumpySink:
+    """Collects k-slabs into Fortran-ordered numpy arrays spanning Size."""
+
+    def __init__(self, shape):
+        self.shape = shape
+
+    def new(self, dtype):
+        return np.zeros(self.shape, dtype=dtype, order="F")
+
+    @staticmethod
+    def put(arr, k, slab):
+        arr[:, :, k] = slab
+
+
+class _TorchSink:
+    """Collects k-slabs into device tensors of shape (nK, nJ, nI), i.e. the
+    Fortran layout of an (nI, nJ, nK) array, without holding the field on the
+    host."""
+
+    def __init__(self, shape, device):
+        import torch
+        self.torch = torch
+        self.shape = shape
+        self.device = device
+
+    def new(self, dtype):
+        tdt = {np.float64: self.torch.float64, np.int32: self.torch.int32}[dtype]
+        nI, nJ, nK = self.shape
+        return self.torch.zeros((nK, nJ, nI), dtype=tdt, device=self.device)
+
+    def put(self, arr, k, slab):
+        arr[k].copy_(self.torch.from_numpy(np.ascontiguousarray(slab.T)))
+
+
+def make_case(cfg: CaseConfig, device=None) -> Case:
+    """Build a synthetic case (deterministic in cfg.seed).
+
+    device=None: numpy arrays in Fortran order.  device='cuda': 3-D fields are
+    device tensors of shape (nK, nJ, nI) (same memory layout), 2-D geometry
+    stays in numpy; only one k-slab of each field is held on the host.
+    """
     ni, nj, nk = cfg.ni, cfg.nj, cfg.nk
     nI, nJ, nK = ni + 2, nj + 2, nk + 2
     KUB = nk
     size = (0, ni + 1, 0, nj + 1, 0, nk + 1)
     ws = (1, ni, 1, nj, 1, nk)
     ss = np.random.SeedSequence(cfg.seed)
-    r_land, r_kf, r_bat, r_psi, r_ot, r_visc, r_kz, r_tr = [np.random.default_rng(s) for s in ss.spawn(8)]
+    r_land, r_kf, r_bat, r_psi, r_ot, r_visc, r_kz, r_tr = [np.random.default_rng(s)
+                                                           for s in ss.spawn(8)]
+    r_tracers = [np.random.default_rng(s) for s in np.random.SeedSequence(
+        (cfg.seed, 77)).spawn(len(cfg.tracers))]
+    sink = _NumpySink((nI, nJ, nK)) if device is None else _TorchSink((nI, nJ, nK), device)
 
     # ---- 2-D land map and KFloor ------------------------------------------
     wet = np.ones((ni, nj), dtype=bool)
@@ -189,18 +232,13 @@ def make_case(cfg: CaseConfig) -> Case:
     WP2[1:-1, 1:-1] = wet
     KF = np.full((nI, nJ), KUB + 1, dtype=np.int32, order="F")
     KF[1:-1, 1:-1] = kf
-    kidx = np.arange(nK)[None, None, :]            # Size k index (KLB = 0)
-    water3 = WP2[:, :, None] & (kidx >= KF[:, :, None]) & (kidx <= KUB)
-    WaterPoints3D = np.asfortranarray(water3.astype(np.int32))
-    LandPoints3D = _arr3((nI, nJ, nK), np.int32, 0)
-    inner = (slice(1, -1), slice(1, -1), slice(1, KUB + 1))
-    LandPoints3D[inner] = 1 - WaterPoints3D[inner]
+    inner2 = np.zeros((nI, nJ), dtype=bool, order="F")
+    inner2[1:-1, 1:-1] = True
 
-    # ---- vertical geometry ------------------------------------------------
-    DWZ = _arr3((nI, nJ, nK))
+    # ---- vertical layer thicknesses ------------------------------------------
     if cfg.vertical == "cartesian":
-        thick = np.full((nI, nJ), cfg.dz)
         frac = None
+        h = None
     else:
         lo, hi = cfg.depth
         bat = _smooth_field(r_bat, ni, nj, cfg.modes)
@@ -210,27 +248,26 @@ def make_case(cfg: CaseConfig) -> Case:
         if cfg.vertical == "sigma":
             frac = np.full(nk, 1.0 / nk)
         else:
-            # geometric stretching, thinnest layer at the surface; the ratio
-            # makes the surface layer of the deepest column dz_min thick
+            # geometric stretching with the thinnest layer at the surface: grow
+            # the ratio until the deepest column's surface layer is dz_min
             target = cfg.dz_min / hi
             ratio = 1.0
-            for _ in range(200):  # solve frac_top(ratio) = target by bisection-free fixed point
+            for _ in range(400):
                 w = ratio ** np.arange(nk)[::-1]
                 f = w / w.sum()
                 if f[-1] <= target:
                     break
                 ratio *= 1.01
             frac = f
-        thick = None
-    for k in range(1, KUB + 1):
-        if frac is None:
-            layer = thick
-        else:
-            layer = h * frac[k - 1]
-        DWZ[:, :, k] = np.where(water3[:, :, k], layer, 0.0)
-    DZZ = _arr3((nI, nJ, nK))
-    DZZ[:, :, 1:KUB + 1] = np.where(water3[:, :, 1:KUB + 1],
-                                    (DWZ[:, :, 2:KUB + 2] + DWZ[:, :, 1:KUB + 1]) / 2.0, 0.0)
+
+    def water_k(k):
+        return WP2 & (k >= KF) & (1 <= k <= KUB)
+
+    def dwz_k(k):
+        if not (1 <= k <= KUB):
+            return np.zeros((nI, nJ))
+        layer = np.full((nI, nJ), cfg.dz) if frac is None else h * frac[k - 1]
+        return np.where(water_k(k), layer, 0.0)
 
     # ---- horizontal metrics (uniform grid) ---------------------------------
     DUX = np.full((nI, nJ), cfg.dx, order="F")
@@ -240,10 +277,7 @@ def make_case(cfg: CaseConfig) -> Case:
     DXX = DUX
     DYY = DVY
 
-    # ---- faces ----------------------------------------------------------------
-    CFU = _arr3((nI, nJ, nK), np.int32, 0)
-    CFV = _arr3((nI, nJ, nK), np.int32, 0)
-    CFW = _arr3((nI, nJ, nK), np.int32, 0)
+    # ---- 2-D face structure -----------------------------------------------------
     # U face j between (i, j-1) and (i, j), j in WorkSize
     cf2u = np.zeros((nI, nJ), dtype=bool, order="F")
     cf2u[1:-1, 2:-1] = WP2[1:-1, 1:-2] & WP2[1:-1, 2:-1]
@@ -253,45 +287,32 @@ def make_case(cfg: CaseConfig) -> Case:
     cf2v[2:-1, 1:-1] = WP2[1:-2, 1:-1] & WP2[2:-1, 1:-1]
     kfv = np.full((nI, nJ), KUB + 1, dtype=np.int32, order="F")
     kfv[1:, :] = np.maximum(KF[:-1, :], KF[1:, :])
-    krange = (kidx >= 1) & (kidx <= KUB)
-    CFU[...] = (cf2u[:, :, None] & (kidx >= kfu[:, :, None]) & krange).astype(np.int32)
-    CFV[...] = (cf2v[:, :, None] & (kidx >= kfv[:, :, None]) & krange).astype(np.int32)
+
+    def cfu_k(k):
+        return cf2u & (k >= kfu) & (1 <= k <= KUB)
+
+    def cfv_k(k):
+        return cf2v & (k >= kfv) & (1 <= k <= KUB)
+
     top_any = np.zeros((nI, nJ), dtype=bool)
-    top_any[:, :-1] |= CFU[:, :-1, KUB] > 0
-    top_any[:, :-1] |= CFU[:, 1:, KUB] > 0
-    top_any[:-1, :] |= CFV[:-1, :, KUB] > 0
-    top_any[:-1, :] |= CFV[1:, :, KUB] > 0
-    CFW[...] = (top_any[:, :, None] & (kidx >= KF[:, :, None] + 1) & krange).astype(np.int32)
-    OP = np.zeros((nI, nJ, nK), dtype=bool, order="F")
-    OP[:, :-1, :] |= CFU[:, :-1, :] > 0
-    OP[:, :-1, :] |= CFU[:, 1:, :] > 0
-    OP[:-1, :, :] |= CFV[:-1, :, :] > 0
-    OP[:-1, :, :] |= CFV[1:, :, :] > 0
-    OP[:, :, :-1] |= CFW[:, :, :-1] > 0
-    OP[:, :, :-1] |= CFW[:, :, 1:] > 0
-    OP &= krange & (np.arange(nI)[:, None, None] >= 1) & (np.arange(nI)[:, None, None] <= ni)
-    OP &= (np.arange(nJ)[None, :, None] >= 1) & (np.arange(nJ)[None, :, None] <= nj)
-    OpenPoints3D = np.asfortranarray(OP.astype(np.int32))
+    top_any[:, :-1] |= cfu_k(KUB)[:, :-1]
+    top_any[:, :-1] |= cfu_k(KUB)[:, 1:]
+    top_any[:-1, :] |= cfv_k(KUB)[:-1, :]
+    top_any[:-1, :] |= cfv_k(KUB)[1:, :]
 
-    # ---- areas and volumes -----------------------------------------------------
-    AreaU = _arr3((nI, nJ, nK))
-    AreaV = _arr3((nI, nJ, nK))
-    wU = water3[:, :-1, :] & water3[:, 1:, :]
-    duz = (DWZ[:, 1:, :] * DUX[:, :-1, None] + DWZ[:, :-1, :] * DUX[:, 1:, None]) / \
-          (DUX[:, :-1, None] + DUX[:, 1:, None])
-    AreaU[:, 1:, :] = np.where(wU, duz * DYY[:, 1:, None], 0.0)
-    wV = water3[:-1, :, :] & water3[1:, :, :]
-    dvz = (DWZ[1:, :, :] * DVY[:-1, :, None] + DWZ[:-1, :, :] * DVY[1:, :, None]) / \
-          (DVY[:-1, :, None] + DVY[1:, :, None])
-    AreaV[1:, :, :] = np.where(wV, dvz * DXX[1:, :, None], 0.0)
-    nominal = DUX * DVY * (cfg.dz if frac is None else float(np.mean(h)) / nk)
-    VolumeZ = np.where(water3, DUX[:, :, None] * DVY[:, :, None] * DWZ, nominal[:, :, None])
-    VolumeZ = np.asfortranarray(VolumeZ)
-    VolumeZOld = VolumeZ.copy(order="F")
+    def cfw_k(k):
+        return top_any & (k >= KF + 1) & (1 <= k <= KUB)
 
-    # ---- fluxes -----------------------------------------------------------------
-    # depth-integrated streamfunction on corners (ic, jc), ic = 0..ni, jc = 0..nj,
-    # zero on the domain edge and on every corner touching land
+    def open_k(k):
+        u, v = cfu_k(k), cfv_k(k)
+        op = np.zeros((nI, nJ), dtype=bool)
+        op[:, :-1] |= u[:, :-1] | u[:, 1:]
+        op[:-1, :] |= v[:-1, :] | v[1:, :]
+        op |= cfw_k(k) | cfw_k(k + 1)
+        return op & inner2 & (1 <= k <= KUB)
+
+    # ---- fluxes: depth-integrated streamfunction on corners (ic, jc),
+    # ic = 0..ni, jc = 0..nj, zero on the domain edge and on corners touching land
     ys = np.arange(ni + 1) / ni
     xs = np.arange(nj + 1) / nj
     psi = _smooth_field(r_psi, ni + 1, nj + 1, cfg.modes, xs=xs, ys=ys, vanish=True)
@@ -311,54 +332,93 @@ def make_case(cfg: CaseConfig) -> Case:
     ax *= 0.3 * peak / max(np.abs(ax).max(), 1e-30)
     ay *= 0.3 * peak / max(np.abs(ay).max(), 1e-30)
 
-    def layered(T2, A2, kface, CF):
+    def layered_k(T2, A2, kface, cf, k):
         n = np.maximum(KUB - kface + 1, 1).astype(np.float64)
-        l = kidx - kface[:, :, None]
-        mode = np.cos(np.pi * (l + 0.5) / n[:, :, None])
-        q = (T2[:, :, None] + A2[:, :, None] * mode) / n[:, :, None]
-        return np.asfortranarray(np.where(CF > 0, q, 0.0))
+        mode = np.cos(np.pi * ((k - kface) + 0.5) / n)
+        return np.where(cf, (T2 + A2 * mode) / n, 0.0)
 
-    Qx = layered(Tx, ax, kfu, CFU)
-    Qy = layered(Ty, ay, kfv, CFV)
-    with np.errstate(divide="ignore", invalid="ignore"):
-        ux = np.where(CFU > 0, np.abs(Qx) / np.where(AreaU > 0, AreaU, 1.0), 0.0).max()
-        uy = np.where(CFV > 0, np.abs(Qy) / np.where(AreaV > 0, AreaV, 1.0), 0.0).max()
-    scale = cfg.u_peak / max(ux, uy, 1e-30)
-    Qx *= scale
-    Qy *= scale
-    # continuity, bottom-up (HYD:43315-43349): Wz(k+1) = Wz(k) + in - out
-    div = np.zeros((nI, nJ, nK))
-    div[:, :-1, :] += Qx[:, :-1, :] - Qx[:, 1:, :]
-    div[:-1, :, :] += Qy[:-1, :, :] - Qy[1:, :, :]
-    div = np.where(water3, div, 0.0)
-    Qz = _arr3((nI, nJ, nK))
-    Qz[:, :, 1:] = np.cumsum(div[:, :, :-1], axis=2)
-    Qz = np.where(CFW > 0, Qz, 0.0)   # only inner water faces carry flux
-    Qz[:, :, KUB + 1] = 0.0            # closed surface (depth-integrated divergence is zero)
-    Qz = np.asfortranarray(Qz)
+    def areas_k(k):
+        dwz = dwz_k(k)
+        wk = water_k(k)
+        au = np.zeros((nI, nJ))
+        wU = wk[:, :-1] & wk[:, 1:]
+        duz = (dwz[:, 1:] * DUX[:, :-1] + dwz[:, :-1] * DUX[:, 1:]) / (DUX[:, :-1] + DUX[:, 1:])
+        au[:, 1:] = np.where(wU, duz * DYY[:, 1:], 0.0)
+        av = np.zeros((nI, nJ))
+        wV = wk[:-1, :] & wk[1:, :]
+        dvz = (dwz[1:, :] * DVY[:-1, :] + dwz[:-1, :] * DVY[1:, :]) / (DVY[:-1, :] + DVY[1:, :])
+        av[1:, :] = np.where(wV, dvz * DXX[1:, :], 0.0)
+        return au, av
 
-    # ---- diffusivities ---------------------------------------------------------
-    lo, hi = cfg.visc_h
-    Visc_H = np.asfortranarray(r_visc.uniform(lo, hi, size=(nI, nJ, nK)) if hi > lo
-                               else np.full((nI, nJ, nK), lo))
+    # pass 1: velocity scale (peak |u| = u_peak over all compute faces)
+    umax = 1e-30
+    for k in range(1, KUB + 1):
+        au, av = areas_k(k)
+        cu_, cv_ = cfu_k(k), cfv_k(k)
+        qx = layered_k(Tx, ax, kfu, cu_, k)
+        qy = layered_k(Ty, ay, kfv, cv_, k)
+        if cu_.any():
+            umax = max(umax, float((np.abs(qx[cu_]) / au[cu_]).max()))
+        if cv_.any():
+            umax = max(umax, float((np.abs(qy[cv_]) / av[cv_]).max()))
+    scale = cfg.u_peak / umax
+
+    # pass 2: every 3-D field, one k-slab at a time
+    f64, i32 = np.float64, np.int32
+    names64 = ("DWZ", "DZZ", "AreaU", "AreaV", "VolumeZ", "VolumeZOld", "Wflux_X", "Wflux_Y",
+               "Wflux_Z", "Visc_H", "Diff_V")
+    names32 = ("WaterPoints3D", "LandPoints3D", "OpenPoints3D", "ComputeFacesU3D",
+               "ComputeFacesV3D", "ComputeFacesW3D")
+    A = {n: sink.new(f64) for n in names64}
+    A.update({n: sink.new(i32) for n in names32})
+    tracers = [sink.new(f64) for _ in cfg.tracers]
+    nominal = DUX * DVY * (cfg.dz if frac is None else float(np.mean(h)) / nk)
+    vlo, vhi = cfg.visc_h
     klo, khi = cfg.kz
-    Diff_V = np.asfortranarray(np.exp(r_kz.uniform(np.log(klo), np.log(khi), size=(nI, nJ, nK))))
+    acc = np.zeros((nI, nJ))         # running continuity sum (HYD:43315-43349)
+    n_wet = 0
+    for k in range(nK):
+        wk = water_k(k)
+        n_wet += int(wk.sum())
+        dwz = dwz_k(k)
+        au, av = areas_k(k)
+        cu_, cv_, cw_ = cfu_k(k), cfv_k(k), cfw_k(k)
+        qx = layered_k(Tx, ax, kfu, cu_, k) * scale
+        qy = layered_k(Ty, ay, kfv, cv_, k) * scale
+        div = np.zeros((nI, nJ))
+        div[:, :-1] += qx[:, :-1] - qx[:, 1:]
+        div[:-1, :] += qy[:-1, :] - qy[1:, :]
+        div = np.where(wk, div, 0.0)
+        qz = np.where(cw_, acc, 0.0)      # closed surface: the face KUB+1 has no CFW
+        acc = acc + div
+        land = np.zeros((nI, nJ), dtype=i32)
+        if 1 <= k <= KUB:
+            land = np.where(inner2, 1 - wk.astype(i32), 0).astype(i32)
+        vol = np.where(wk, DUX * DVY * dwz, nominal)
+        dzz = np.where(wk, (dwz_k(k + 1) + dwz) / 2.0, 0.0)
+        visc = (r_visc.uniform(vlo, vhi, size=(nJ, nI)).T if vhi > vlo
+                else np.full((nI, nJ), vlo))
+        diffv = np.exp(r_kz.uniform(np.log(klo), np.log(khi), size=(nJ, nI))).T
+        slabs = dict(DWZ=dwz, DZZ=dzz, AreaU=au, AreaV=av, VolumeZ=vol, VolumeZOld=vol,
+                     Wflux_X=qx, Wflux_Y=qy, Wflux_Z=qz, Visc_H=visc, Diff_V=diffv,
+                     WaterPoints3D=wk.astype(i32), LandPoints3D=land,
+                     OpenPoints3D=open_k(k).astype(i32), ComputeFacesU3D=cu_.astype(i32),
+                     ComputeFacesV3D=cv_.astype(i32), ComputeFacesW3D=cw_.astype(i32))
+        for n, sl in slabs.items():
+            sink.put(A[n], k, sl)
+        for t, kind in enumerate(cfg.tracers):
+            rt = r_tracers[t]
+            if kind == "temperature":
+                v = rt.uniform(5.0, 25.0, size=(nJ, nI)).T
+            elif kind == "salinity":
+                v = rt.uniform(30.0, 36.0, size=(nJ, nI)).T
+            else:
+                v = np.exp(rt.normal(0.0, 1.0, size=(nJ, nI))).T
+            v = np.where(wk, v, C.NULL_REAL)
+            v = np.where(inner2 & (1 <= k <= KUB), v, 0.0)
+            sink.put(tracers[t], k, v)
 
-    # ---- tracers ------------------------------------------------------------------
-    tracers = []
-    for kind in cfg.tracers:
-        if kind == "temperature":
-            t = r_tr.uniform(5.0, 25.0, size=(nI, nJ, nK))
-        elif kind == "salinity":
-            t = r_tr.uniform(30.0, 36.0, size=(nI, nJ, nK))
-        else:
-            t = np.exp(r_tr.normal(0.0, 1.0, size=(nI, nJ, nK)))
-        t = np.where(water3, t, C.NULL_REAL)
-        t[0, :, :] = 0.0; t[-1, :, :] = 0.0
-        t[:, 0, :] = 0.0; t[:, -1, :] = 0.0
-        t[:, :, 0] = 0.0; t[:, :, -1] = 0.0
-        tracers.append(np.asfortranarray(t))
-
+    tvd = cfg.adv_method == C.P2_TVD
     tvd = cfg.adv_method == C.P2_TVD
     optimize = cfg.optimize
     if optimize is None:  # WP:14576-14598
@@ -375,13 +435,13 @@ def make_case(cfg: CaseConfig) -> Case:
         ImpExp_DifH=0.0, NullDif=False, Optimize=bool(optimize),
     )
     geom = dict(DUX=DUX, DVY=DVY, DZX=DZX, DZY=DZY, KFloorZ=KF,
-                AreaU=AreaU, AreaV=AreaV, DZZ=DZZ, DWZ=DWZ)
-    hydro = dict(Wflux_X=Qx, Wflux_Y=Qy, Wflux_Z=Qz, VolumeZOld=VolumeZOld,
-                 VolumeZ=VolumeZ, OpenPoints3D=OpenPoints3D, LandPoints3D=LandPoints3D,
-                 ComputeFacesU3D=CFU, ComputeFacesV3D=CFV, ComputeFacesW3D=CFW,
-                 WaterPoints3D=WaterPoints3D, Visc_H=Visc_H, Diff_V=Diff_V)
+                AreaU=A["AreaU"], AreaV=A["AreaV"], DZZ=A["DZZ"], DWZ=A["DWZ"])
+    hydro = {n: A[n] for n in ("Wflux_X", "Wflux_Y", "Wflux_Z", "VolumeZOld", "VolumeZ",
+                               "OpenPoints3D", "LandPoints3D", "ComputeFacesU3D",
+                               "ComputeFacesV3D", "ComputeFacesW3D", "WaterPoints3D",
+                               "Visc_H", "Diff_V")}
     case = Case(cfg=cfg, size=size, worksize=ws, geom=geom, hydro=hydro,
                 tracers=tracers, params=params)
-    case.n_wet = int(WaterPoints3D[1:-1, 1:-1, 1:KUB + 1].sum())
+    case.n_wet = n_wet
     case.wet_columns = int(wet.sum())
     return case

@@ -205,3 +205,40 @@ def test_rejects_like_the_reference():
     assert np.array_equal(p, case.tracers[0])
     gpu.kill()
     g2.kill()
+
+
+def test_device_resident_inputs_match_host_inputs():
+    """Fixtures built directly on the device (device-pointer geometry, hydro
+    and tracers) give the same transport result as host inputs, and the
+    drop-in's device mode updates PROP in place like the host mode."""
+    import torch
+    from mohid_amd import fixtures as fx
+    AD, Transport, _ = _ad()
+    cfg = fx.config("c3", ni=30, nj=26, nk=8, kfloor=(2, 8),
+                    tracers=("temperature", "lognormal"))
+    host = fx.make_case(cfg)
+    dev = fx.make_case(cfg, device="cuda")
+    for t in range(2):
+        got = dev.tracers[t].cpu().numpy()                       # (nK, nJ, nI)
+        assert np.array_equal(got.transpose(2, 1, 0), host.tracers[t])
+    outs = []
+    for case in (host, dev):
+        tr = Transport(case.size, case.worksize, case.geom, 2)
+        tr.set_hydro(case.hydro, case.params)
+        for t, p in enumerate(case.tracers):
+            tr.set_tracer(t, p)
+        tr.run(3, use_graph=True)
+        outs.append([tr.get_tracer(t) for t in range(2)])
+        tr.close()
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+    # drop-in, device mode: PROP is a device tensor updated in place
+    gpu = AD(dev.size, dev.worksize, dev.geom)
+    prop = dev.tracers[1].clone()
+    gpu(prop, dev.hydro, dev.params, first_property=True)
+    ref = np.array(host.tracers[1], order="F", copy=True)
+    gpu_h = AD(host.size, host.worksize, host.geom)
+    gpu_h(ref, host.hydro, host.params, first_property=True)
+    assert np.array_equal(prop.cpu().numpy().transpose(2, 1, 0), ref)
+    gpu.kill()
+    gpu_h.kill()

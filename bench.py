@@ -16,10 +16,11 @@ steps as one captured hipGraph.
 
 Prints ONE JSON line.  `value` is tracer cell-updates/s over water cells
 (T x N_wet x K / wall).  `roofline` is for the per-tracer transport kernel:
-algorithmic bytes per launch = (16 + 64/T) B x N_wet (FP64 byte model: one
-read and one write of the tracer plus eight shared FP64 fields amortised over
-T tracers) divided by the kernel's mean duration from HIP events recorded on
-the library's stream around each launch.  `cpu_baseline` times the CPU oracle
+algorithmic bytes per step = (16 + 64/T) B x T x N_wet (FP64 byte model: one
+read and one write of each tracer plus eight shared FP64 fields amortised over
+T tracers) divided by the mean duration per step of the transport kernels
+(explicit + vertical, all tracers) from HIP events recorded on the library's
+stream around each launch.  `cpu_baseline` times the CPU oracle
 (a restatement of the reference algorithm, OpenMP over the host cores) on a
 bounded sample of the same workload.
 """
@@ -44,8 +45,9 @@ def _log(msg):
 
 
 def _traffic_from_profiles(kernel_key: str, workload: str):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
-    WRITE_SIZE), as recorded in profiles/*.md (a JSON block per workload)."""
+    """HBM bytes per step of the transport kernels measured with rocprofv3
+    --pmc (FETCH_SIZE, WRITE_SIZE), as recorded in profiles/*.md (a JSON block
+    per workload)."""
     prof = os.path.join(ROOT, "profiles")
     if not os.path.isdir(prof):
         return None
@@ -75,6 +77,7 @@ def main():
     ap.add_argument("--cpu-window", type=int, default=384)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--tile", default="0,0")
+    ap.add_argument("--tracers-per-launch", type=int, default=0)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -105,7 +108,8 @@ def main():
 
     prec = _lib.MGPU_PREC_MIXED if args.precision == "mixed" else _lib.MGPU_PREC_FP64
     tile = tuple(int(x) for x in args.tile.split(","))
-    tr = Transport(case.size, case.worksize, case.geom, T, precision=prec, tile=tile)
+    tr = Transport(case.size, case.worksize, case.geom, T, precision=prec, tile=tile,
+                   tracers_per_launch=args.tracers_per_launch)
     tr.set_hydro(case.hydro, case.params)
     for t, p in enumerate(case.tracers):
         tr.set_tracer(t, p)
@@ -136,19 +140,22 @@ def main():
     _log(f"timed {args.steps} steps: {wall * 1e3:.2f} ms wall, {ev_ms:.2f} ms events, "
          f"{value:.4e} cell-updates/s")
 
-    # ---- per-kernel durations (HIP events around every launch, eager steps)
+    # ---- per-kernel durations (HIP events on the library's stream around the
+    # kernels of every step, eager steps)
     ksteps = max(2, min(args.steps, 10))
     tr.timing(True, ksteps)
     tr.run(ksteps, use_graph=False)
-    prep_ms, trans_ms, nlaunch = tr.kernel_times()
+    prep_ms, h_ms, v_ms, nsteps_t = tr.kernel_times()
     tr.timing(False)
+    trans_ms = h_ms + v_ms
+    # FP64 byte model: (16 + 64/T) B per tracer-cell update, T*N_wet updates
     per_unit = 16.0 + 64.0 / T
-    alg_bytes = per_unit * nwet
+    alg_bytes = per_unit * nwet * T
     achieved = alg_bytes / (trans_ms * 1e-3) / 1e9
     kernel_key = f"transport_{args.precision}"
     traffic = _traffic_from_profiles(kernel_key, cfg.name)
-    _log(f"transport kernel {trans_ms:.3f} ms/launch over {nlaunch} launches, "
-         f"prep {prep_ms:.3f} ms; {achieved:.1f} GB/s algorithmic")
+    _log(f"per step: prep {prep_ms:.3f} ms, explicit {h_ms:.3f} ms, vertical {v_ms:.3f} ms "
+         f"({nsteps_t} steps); transport {achieved:.1f} GB/s algorithmic")
 
     out = {
         "metric": "tracer cell-updates/s (%HBM peak), 1024^2x40x8 tracers, batched GPU step",
@@ -179,9 +186,12 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": kernel_key,
+            "kernel": "explicit_kernel + vertical_kernel (all tracers of one step)",
             "kernel_ms": trans_ms,
+            "explicit_kernel_ms": h_ms,
+            "vertical_kernel_ms": v_ms,
             "alg_bytes_per_launch": alg_bytes,
+            "alg_bytes_model": f"(16 + 64/T) B x T x N_wet, T={T}",
             "prep_kernel_ms": prep_ms,
         },
     }
@@ -214,7 +224,8 @@ def main():
         cwall = time.perf_counter() - c0
         inst.close()
         cval = sT * swet * nsteps / cwall
-        tr2 = Transport(scase.size, scase.worksize, scase.geom, sT, precision=prec, tile=tile)
+        tr2 = Transport(scase.size, scase.worksize, scase.geom, sT, precision=prec, tile=tile,
+                        tracers_per_launch=args.tracers_per_launch)
         tr2.set_hydro(scase.hydro, scase.params)
         for t, p in enumerate(scase.tracers):
             tr2.set_tracer(t, p)

@@ -141,7 +141,8 @@ typedef struct {
     int32_t ntracers;
     int32_t precision;                    /* MGPU_PREC_FP64 / MGPU_PREC_MIXED */
     int32_t docycle_method;
-    int32_t tile_i, tile_j;               /* 0: autotuned / default */
+    int32_t tile_i, tile_j;               /* column-solve block shape; 0: 64 x 4 */
+    int32_t tracers_per_launch;           /* 1..8 tracers per kernel launch; 0: 8 */
 } mgpu_transport_opts;
 
 int mgpu_transport_create(const mgpu_size3d *size, const mgpu_size3d *worksize,
@@ -154,19 +155,22 @@ int mgpu_transport_get_tracer(mgpu_transport *t, int32_t index, double *prop, in
 /* Advance all tracers nsteps steps.  use_graph = 1 captures the step loop in
  * a hipGraph (re-used while nsteps is unchanged). */
 int mgpu_transport_run(mgpu_transport *t, int32_t nsteps, int32_t use_graph);
-/* Device pointer of tracer `index` (current state) and the handle's stream. */
-int mgpu_transport_tracer_ptr(mgpu_transport *t, int32_t index, double **dev_ptr);
+/* Current state of tracer `index` on the device: the handle keeps its
+ * tracers interleaved, value (i,j,k) at dev_ptr[cell * stride] with cell the
+ * Fortran linear index of (i,j,k) in Size.  Also the handle's HIP stream. */
+int mgpu_transport_tracer_ptr(mgpu_transport *t, int32_t index, double **dev_ptr,
+                              int64_t *stride);
 int mgpu_transport_stream(mgpu_transport *t, void **stream);
 int mgpu_transport_destroy(mgpu_transport *t);
 /* Per-launch timing for measurement: with enable = 1 the following eager
- * mgpu_transport_run calls (use_graph = 0, up to max_steps steps) record a
- * hipEvent pair around every kernel on the handle's stream;
- * mgpu_transport_kernel_times returns the mean duration (ms) of the
- * face-conductance kernel and of the per-tracer transport kernel, and the
- * number of transport launches timed. */
+ * mgpu_transport_run calls (use_graph = 0, up to max_steps steps) record
+ * hipEvents on the handle's stream around the kernels of every step;
+ * mgpu_transport_kernel_times returns the mean duration (ms) per step of the
+ * face-conductance kernel, of the explicit (horizontal) kernel(s) and of the
+ * vertical/column-solve kernel(s) over all tracers, and the steps timed. */
 int mgpu_transport_timing(mgpu_transport *t, int32_t enable, int32_t max_steps);
-int mgpu_transport_kernel_times(mgpu_transport *t, double *prep_ms, double *transport_ms,
-                                int32_t *launches);
+int mgpu_transport_kernel_times(mgpu_transport *t, double *prep_ms, double *explicit_ms,
+                                double *vertical_ms, int32_t *steps);
 
 /* Library version string and the last HIP error message (thread-local). */
 const char *mgpu_version(void);

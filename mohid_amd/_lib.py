@@ -75,7 +75,8 @@ class ADArgs(ct.Structure):
 
 class TransportOpts(ct.Structure):
     _fields_ = [("ntracers", ct.c_int32), ("precision", ct.c_int32),
-                ("docycle_method", ct.c_int32), ("tile_i", ct.c_int32), ("tile_j", ct.c_int32)]
+                ("docycle_method", ct.c_int32), ("tile_i", ct.c_int32), ("tile_j", ct.c_int32),
+                ("tracers_per_launch", ct.c_int32)]
 
 
 # every symbol include/mohid_gpu.h declares
@@ -124,12 +125,14 @@ def lib():
     L.mgpu_transport_set_tracer.argtypes = [vp, ct.c_int32, vp, ct.c_int32]
     L.mgpu_transport_get_tracer.argtypes = [vp, ct.c_int32, vp, ct.c_int32]
     L.mgpu_transport_run.argtypes = [vp, ct.c_int32, ct.c_int32]
-    L.mgpu_transport_tracer_ptr.argtypes = [vp, ct.c_int32, ct.POINTER(vp)]
+    L.mgpu_transport_tracer_ptr.argtypes = [vp, ct.c_int32, ct.POINTER(vp),
+                                            ct.POINTER(ct.c_int64)]
     L.mgpu_transport_stream.argtypes = [vp, ct.POINTER(vp)]
     L.mgpu_transport_destroy.argtypes = [vp]
     L.mgpu_transport_timing.argtypes = [vp, ct.c_int32, ct.c_int32]
     L.mgpu_transport_kernel_times.argtypes = [vp, ct.POINTER(ct.c_double),
-                                              ct.POINTER(ct.c_double), ct.POINTER(ct.c_int32)]
+                                              ct.POINTER(ct.c_double), ct.POINTER(ct.c_double),
+                                              ct.POINTER(ct.c_int32)]
     L.mgpu_version.restype = ct.c_char_p
     L.mgpu_last_error.restype = ct.c_char_p
     for name in EXPORTS:

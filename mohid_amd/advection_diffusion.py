@@ -185,11 +185,11 @@ class Transport:
     ModuleWaterProperties::Advection_Diffusion_Processes, WP:14520-15169)."""
 
     def __init__(self, size, worksize, geometry: Dict, ntracers: int, precision: int = 0,
-                 docycle_method: int = 1, tile=(0, 0)):
+                 docycle_method: int = 1, tile=(0, 0), tracers_per_launch: int = 0):
         keep: list = []
         g = _geometry(geometry, keep)
         opts = L.TransportOpts(int(ntracers), int(precision), int(docycle_method),
-                               int(tile[0]), int(tile[1]))
+                               int(tile[0]), int(tile[1]), int(tracers_per_launch))
         h = ct.c_void_p()
         self._size = L.Size3D(*size)
         self._ws = L.Size3D(*worksize)
@@ -233,12 +233,13 @@ class Transport:
                                                                        int(max_steps)))
 
     def kernel_times(self):
-        """(mean prep-kernel ms, mean transport-kernel ms, transport launches)."""
-        a, b, n = ct.c_double(), ct.c_double(), ct.c_int32()
+        """Mean ms per step of (face-conductance kernel, explicit kernel(s),
+        vertical kernel(s)) and the number of steps timed."""
+        a, b, c, n = ct.c_double(), ct.c_double(), ct.c_double(), ct.c_int32()
         L.check("mgpu_transport_kernel_times",
                 L.lib().mgpu_transport_kernel_times(self._h, ct.byref(a), ct.byref(b),
-                                                    ct.byref(n)))
-        return a.value, b.value, n.value
+                                                    ct.byref(c), ct.byref(n)))
+        return a.value, b.value, c.value, n.value
 
     def stream_handle(self) -> int:
         s = ct.c_void_p()

@@ -356,7 +356,8 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
         ad->have_faces = true;
     }
     const SharedFields f = make_fields(ad->hydro, ad->der, ad->geom);
-    rc = launch_transport_fp64(d, sc, f, P, ad->out, ad->Wscr, ad->der.counters, 0, 0, s);
+    rc = launch_transport_fp64(d, sc, f, 1, 1, P, ad->Wscr, ad->out, ad->Wscr, ad->der.counters,
+                               1, 0, 0, s);
     if (rc) return rc;
     rc = launch_copy_back(d, ad->der.mask, ad->out, P, s);
     if (rc) return rc;
@@ -433,9 +434,12 @@ struct mgpu_transport {
     StepScalars sc;
     double schmidtH = 1.0, schmidtV = 1.0, backgroundV = 0.0;
     int nulldif = 0;
-    double **buf[2] = {nullptr, nullptr};
+    // tracer state, interleaved: tracer t of cell x at [x * ntracers + t]
+    double *buf[2] = {nullptr, nullptr};
     int cur = 0;
-    double *Wscr = nullptr;
+    double *scr = nullptr;           // explicit RHS, then forward-sweep W (same layout)
+    double *stage = nullptr;         // one Fortran-layout tracer (host transfers)
+    int chunk = 8;                   // tracers per launch
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     int graph_steps[2] = {0, 0};
@@ -479,14 +483,10 @@ extern "C" int mgpu_transport_destroy(mgpu_transport *t) {
     t->geom.release();
     t->hydro.release();
     t->der.release();
-    for (int p = 0; p < 2; ++p) {
-        if (t->buf[p]) {
-            for (int q = 0; q < t->opts.ntracers; ++q) dfree(t->buf[p][q]);
-            delete[] t->buf[p];
-            t->buf[p] = nullptr;
-        }
-    }
-    dfree(t->Wscr);
+    dfree(t->buf[0]);
+    dfree(t->buf[1]);
+    dfree(t->scr);
+    dfree(t->stage);
     if (t->stream) (void)hipStreamDestroy(t->stream);
     delete t;
     return MGPU_SUCCESS;
@@ -508,6 +508,11 @@ extern "C" int mgpu_transport_create(const mgpu_size3d *size, const mgpu_size3d 
     t->ws = *worksize;
     t->d = make_dims(*size, *worksize);
     t->opts = *opts;
+    if (opts->tracers_per_launch < 0 || opts->tracers_per_launch > 8) {
+        delete t;
+        return MGPU_ERR_BADARG;
+    }
+    t->chunk = opts->tracers_per_launch ? opts->tracers_per_launch : 8;
     hipError_t e = hipStreamCreate(&t->stream);
     if (e != hipSuccess) {
         mgpu_set_error("hipStreamCreate", e);
@@ -518,12 +523,11 @@ extern "C" int mgpu_transport_create(const mgpu_size3d *size, const mgpu_size3d 
     rc = t->geom.alloc(t->d);
     if (!rc) rc = t->hydro.alloc(t->d);
     if (!rc) rc = t->der.alloc(t->d);
-    if (!rc) rc = dalloc(&t->Wscr, n3);
-    for (int p = 0; p < 2 && !rc; ++p) {
-        t->buf[p] = new (std::nothrow) double *[opts->ntracers]();
-        if (!t->buf[p]) rc = MGPU_UNKNOWN;
-        for (int q = 0; q < opts->ntracers && !rc; ++q) rc = dalloc(&t->buf[p][q], n3);
-    }
+    const long nT = (long)opts->ntracers;
+    if (!rc) rc = dalloc(&t->scr, n3 * nT);
+    if (!rc) rc = dalloc(&t->buf[0], n3 * nT);
+    if (!rc) rc = dalloc(&t->buf[1], n3 * nT);
+    if (!rc) rc = dalloc(&t->stage, n3);
     if (!rc) rc = t->geom.upload(t->d, geometry, t->stream);
     if (rc) {
         mgpu_transport_destroy(t);
@@ -565,12 +569,18 @@ extern "C" int mgpu_transport_set_tracer(mgpu_transport *t, int32_t index, const
                                          int32_t dev) {
     if (!t) return MGPU_OFF_ERR;
     if (!prop || index < 0 || index >= t->opts.ntracers) return MGPU_ERR_BADARG;
-    const size_t b3 = sizeof(double) * (size_t)t->d.n3();
-    const hipMemcpyKind kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const long n3 = t->d.n3();
+    const double *src = prop;
+    if (!dev) {
+        HIPCHK(hipMemcpyAsync(t->stage, prop, sizeof(double) * (size_t)n3,
+                              hipMemcpyHostToDevice, t->stream));
+        src = t->stage;
+    }
     // both ping-pong buffers start equal, so cells a step never writes
     // (halos, land columns) stay identical in both
-    HIPCHK(hipMemcpyAsync(t->buf[0][index], prop, b3, kind, t->stream));
-    HIPCHK(hipMemcpyAsync(t->buf[1][index], prop, b3, kind, t->stream));
+    int rc = launch_interleave(n3, src, t->buf[0], t->opts.ntracers, index, 1, t->stream);
+    if (!rc) rc = launch_interleave(n3, src, t->buf[1], t->opts.ntracers, index, 1, t->stream);
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(t->stream));
     return MGPU_SUCCESS;
 }
@@ -579,17 +589,23 @@ extern "C" int mgpu_transport_get_tracer(mgpu_transport *t, int32_t index, doubl
                                          int32_t dev) {
     if (!t) return MGPU_OFF_ERR;
     if (!prop || index < 0 || index >= t->opts.ntracers) return MGPU_ERR_BADARG;
-    const size_t b3 = sizeof(double) * (size_t)t->d.n3();
-    HIPCHK(hipMemcpyAsync(prop, t->buf[t->cur][index], b3,
-                          dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, t->stream));
+    const long n3 = t->d.n3();
+    double *dst = dev ? prop : t->stage;
+    int rc = launch_interleave(n3, t->buf[t->cur], dst, t->opts.ntracers, index, 0, t->stream);
+    if (rc) return rc;
+    if (!dev)
+        HIPCHK(hipMemcpyAsync(prop, t->stage, sizeof(double) * (size_t)n3,
+                              hipMemcpyDeviceToHost, t->stream));
     HIPCHK(hipStreamSynchronize(t->stream));
     return MGPU_SUCCESS;
 }
 
-extern "C" int mgpu_transport_tracer_ptr(mgpu_transport *t, int32_t index, double **dev_ptr) {
+extern "C" int mgpu_transport_tracer_ptr(mgpu_transport *t, int32_t index, double **dev_ptr,
+                                         int64_t *stride) {
     if (!t) return MGPU_OFF_ERR;
-    if (!dev_ptr || index < 0 || index >= t->opts.ntracers) return MGPU_ERR_BADARG;
-    *dev_ptr = t->buf[t->cur][index];
+    if (!dev_ptr || !stride || index < 0 || index >= t->opts.ntracers) return MGPU_ERR_BADARG;
+    *dev_ptr = t->buf[t->cur] + index;
+    *stride = t->opts.ntracers;
     return MGPU_SUCCESS;
 }
 
@@ -617,16 +633,16 @@ static int enqueue_steps(mgpu_transport *t, int nsteps, int cur) {
                                    t->der.cw, s);
         if (rc) return rc;
         if ((rc = mark(t))) return rc;
-        for (int q = 0; q < t->opts.ntracers; ++q) {
-            if ((rc = mark(t))) return rc;
+        const int n = t->opts.ntracers;
+        for (int ph = 1; ph <= 2; ++ph) {   // explicit kernel(s), then vertical kernel(s)
             if (t->opts.precision == MGPU_PREC_MIXED)
-                rc = launch_transport_mixed(d, t->sc, f, t->buf[cur][q], t->buf[cur ^ 1][q],
-                                            t->Wscr, t->der.counters, t->opts.tile_i,
-                                            t->opts.tile_j, s);
+                rc = launch_transport_mixed(d, t->sc, f, n, n, t->buf[cur], t->scr,
+                                            t->buf[cur ^ 1], t->scr, t->der.counters, t->chunk,
+                                            t->opts.tile_i, t->opts.tile_j, s, ph);
             else
-                rc = launch_transport_fp64(d, t->sc, f, t->buf[cur][q], t->buf[cur ^ 1][q],
-                                           t->Wscr, t->der.counters, t->opts.tile_i,
-                                           t->opts.tile_j, s);
+                rc = launch_transport_fp64(d, t->sc, f, n, n, t->buf[cur], t->scr,
+                                           t->buf[cur ^ 1], t->scr, t->der.counters, t->chunk,
+                                           t->opts.tile_i, t->opts.tile_j, s, ph);
             if (rc) return rc;
             if ((rc = mark(t))) return rc;
         }
@@ -674,9 +690,10 @@ extern "C" int mgpu_transport_run(mgpu_transport *t, int32_t nsteps, int32_t use
 }
 
 // Timing of individual launches (eager steps only): with `enable`, the next
-// mgpu_transport_run(nsteps, 0) records a hipEvent pair around every kernel
-// on the handle's stream; mgpu_transport_kernel_times then returns the mean
-// duration of the face-conductance kernel and of the transport kernel.
+// mgpu_transport_run(nsteps, 0) records hipEvents on the handle's stream
+// before the face-conductance kernel and after it, after the explicit
+// kernel(s) and after the vertical kernel(s) of every step;
+// mgpu_transport_kernel_times returns their mean durations per step.
 extern "C" int mgpu_transport_timing(mgpu_transport *t, int32_t enable, int32_t max_steps) {
     if (!t) return MGPU_OFF_ERR;
     HIPCHK(hipStreamSynchronize(t->stream));
@@ -684,7 +701,7 @@ extern "C" int mgpu_transport_timing(mgpu_transport *t, int32_t enable, int32_t 
     t->timing = 0;
     if (!enable) return MGPU_SUCCESS;
     if (max_steps < 1) return MGPU_ERR_BADARG;
-    const int cap = 2 * max_steps * (1 + t->opts.ntracers);
+    const int cap = 4 * max_steps;
     t->ev = new (std::nothrow) hipEvent_t[cap];
     if (!t->ev) return MGPU_UNKNOWN;
     for (int q = 0; q < cap; ++q) {
@@ -703,27 +720,26 @@ extern "C" int mgpu_transport_timing(mgpu_transport *t, int32_t enable, int32_t 
 }
 
 extern "C" int mgpu_transport_kernel_times(mgpu_transport *t, double *prep_ms,
-                                           double *transport_ms, int32_t *launches) {
+                                           double *explicit_ms, double *vertical_ms,
+                                           int32_t *steps) {
     if (!t) return MGPU_OFF_ERR;
-    if (!prep_ms || !transport_ms || !launches) return MGPU_ERR_BADARG;
+    if (!prep_ms || !explicit_ms || !vertical_ms || !steps) return MGPU_ERR_BADARG;
     HIPCHK(hipStreamSynchronize(t->stream));
-    const int per_step = 2 * (1 + t->opts.ntracers);
-    const int steps = t->ev_used / per_step;
-    double sp = 0.0, st = 0.0;
-    int np = 0, nt = 0;
-    for (int s = 0; s < steps; ++s) {
+    const int per_step = 4;  // before prep | after prep | after explicit | after vertical
+    const int ns = t->ev_used / per_step;
+    double sp = 0.0, sh = 0.0, sv = 0.0;
+    for (int s = 0; s < ns; ++s) {
         hipEvent_t *e = t->ev + (long)s * per_step;
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, e[0], e[1]));
-        sp += ms; ++np;
-        for (int q = 0; q < t->opts.ntracers; ++q) {
-            HIPCHK(hipEventElapsedTime(&ms, e[2 + 2 * q], e[3 + 2 * q]));
-            st += ms; ++nt;
-        }
+        float a = 0.f, b = 0.f, c = 0.f;
+        HIPCHK(hipEventElapsedTime(&a, e[0], e[1]));
+        HIPCHK(hipEventElapsedTime(&b, e[1], e[2]));
+        HIPCHK(hipEventElapsedTime(&c, e[2], e[3]));
+        sp += a; sh += b; sv += c;
     }
-    *prep_ms = np ? sp / np : 0.0;
-    *transport_ms = nt ? st / nt : 0.0;
-    *launches = nt;
+    *prep_ms = ns ? sp / ns : 0.0;
+    *explicit_ms = ns ? sh / ns : 0.0;
+    *vertical_ms = ns ? sv / ns : 0.0;
+    *steps = ns;
     t->ev_used = 0;
     return MGPU_SUCCESS;
 }

@@ -86,14 +86,24 @@ int launch_prep_faces(const Dims &d, const double *visc, const double *diffv,
                       const double *DZZ, const uint8_t *mask, double schmidtH,
                       double schmidtV, double backgroundV, int nulldif,
                       double *cu, double *cv, double *cw, hipStream_t s);
+// Transport of n tracers P -> out.  Tracer t of cell index x lives at
+// [x * stride + t] of each buffer (stride 1: one tracer per array; stride n:
+// interleaved).  TI and W are scratch of the same shape and may alias
+// (kernel V reads TI(k) before writing W(k)).  Tracers go in chunks of
+// `chunk` (<= 8) per launch.  phase: 1 = explicit kernel only, 2 = vertical
+// kernel only, 3 = both.
 int launch_transport_fp64(const Dims &d, const StepScalars &sc, const SharedFields &f,
-                          const double *P, double *out, double *Wscr,
-                          unsigned long long *zero_pivots, int tile_i, int tile_j,
-                          hipStream_t s);
+                          int n, long stride, const double *P, double *TI, double *out,
+                          double *W, unsigned long long *counters, int chunk, int tile_i,
+                          int tile_j, hipStream_t s, int phase = 3);
 int launch_transport_mixed(const Dims &d, const StepScalars &sc, const SharedFields &f,
-                           const double *P, double *out, double *Wscr,
-                           unsigned long long *zero_pivots, int tile_i, int tile_j,
-                           hipStream_t s);
+                           int n, long stride, const double *P, double *TI, double *out,
+                           double *W, unsigned long long *counters, int chunk, int tile_i,
+                           int tile_j, hipStream_t s, int phase = 3);
+// Copy one tracer between a Fortran array (stride 1) and an interleaved
+// buffer (stride `stride`, slot t).  to_interleaved = 1: src Fortran -> dst.
+int launch_interleave(long n3, const double *src, double *dst, long stride, int t,
+                      int to_interleaved, hipStream_t s);
 int launch_thomasz(const Dims &d, const double *D, const double *E, const double *F,
                    const double *TI, const int32_t *wp, double *res, double *Wscr,
                    hipStream_t s);

@@ -8,17 +8,20 @@
 //                   lowest row that is not an inert land row (`kstart`);
 //   prep_faces      face conductances Kh*A/dx (U, V faces) and
 //                   Kz*DUX*DVY/DZZ (W faces), AD:2364-2675 + AD:1514-1619;
-//   transport_fp64  one lane per water column, walking k upward: the
-//                   explicit right-hand side TI (AD:3966-4021, 4368-4953,
-//                   5156-5365), the tridiagonal rows D/E/F (AD:2708-3140) and
-//                   the THOMASZ_NewType2 forward sweep (MF:4080-4100), then the
-//                   back substitution (MF:4102-4106).
+//   explicit (H)    per level, the explicit right-hand side TI of every water
+//                   cell (AD:3966-4021, 4368-4953, 5156-5365), face fluxes
+//                   computed once per face into LDS;
+//   vertical (V)    one lane per water column, walking k upward: the rows
+//                   D/E/F (AD:2708-3140), the THOMASZ_NewType2 forward sweep
+//                   (MF:4080-4100), then the back substitution (MF:4102-4106).
+// Both kernels advance up to MAXT tracers per launch, so the shared fields
+// and every tracer-independent quantity are loaded/computed once for all.
 //
 // The reference assembles TI/D/E/F by scattering face contributions into
-// whole 3-D arrays; here every lane gathers the contributions of its own cell
-// in the order the reference's loops deliver them, so nothing is stored and
-// no atomics are needed, and each cell sees the same floating-point
-// operations in the same order as in the reference.
+// whole 3-D arrays; here every cell gathers the contributions of its own row
+// in the order the reference's loops deliver them, so no atomics are needed
+// and each cell sees the same floating-point operations in the same order as
+// in the reference.
 #include "face.h"
 #include "mgpu_internal.h"
 
@@ -129,57 +132,280 @@ __global__ void prep_faces_kernel(Dims d, const double *__restrict__ visc,
 }
 
 // ---------------------------------------------------------------------------
-// Loads along a horizontal line of the stencil.
-struct Line {
-    double p[5], v[5];   // cells c-2 .. c+2
-    bool op[5];
-    uint8_t m1;          // mask of cell c+1 (its bottom-side face flag)
+// Explicit right-hand side (kernel H).  One block per (64 x 4) tile of
+// columns and one level k.  Phase 1 computes the advective flux of every
+// U and V face of the tile once per tracer (face-parallel, into LDS); phase 2
+// gathers, per water cell, the reference's contributions to TICOEF3 in the
+// order its loops deliver them:
+//   VolumeVariation (AD:3986-4000), HorizontalDiffusionXX/XX2 (AD:5178-5197,
+//   5239-5246), YY/YY2 (AD:5335-5350, 5285-5292), HorizontalAdvectionXX
+//   (+_Explicit, AD:4463-4487, 4560-4572), YY (+_Explicit, AD:4828-4853,
+//   4930-4940).
+// Tracer-independent parts of every face (Courant number, stencil spacing,
+// masks) are computed once for all tracers of the launch.
+constexpr int HX = 64, HY = 4;
+constexpr int NFU = (HY + 1) * HX;   // U faces j0 .. j0+HY of the tile
+constexpr int NFV = HY * (HX + 1);   // V faces i0 .. i0+HX of the tile
+constexpr int MAXT = 8;
+
+// The tracers of one launch: value of tracer t (0 <= t < NT) at cell index x
+// is base[x * s + t].  Separate Fortran arrays (the drop-in) have s = 1;
+// the batched path keeps its tracers interleaved (s = number of tracers) so
+// that the tracers of a cell share cache lines and each quantity is one
+// memory stream.
+struct TracerSet {
+    const double *P;         // state at the start of the step
+    double *TI;              // explicit right-hand side (kernel H -> kernel V)
+    double *out;             // new state; holds the forward-sweep G in between
+    double *W;               // forward-sweep W (may alias TI)
+    long s;                  // tracer stride
 };
 
-template <bool OPT>
-__device__ __forceinline__ double hflux(const double p[4], const double v[4], const double du[4],
-                                        const bool op[4], double Q, const StepScalars &sc,
-                                        int method, int tvd, int up2, int &bad) {
+// Flux of one horizontal face for NT tracers.  `c0` indexes the cell on the
+// face's high side, `st` is the stride along the line, `pos` its line index
+// and `lo` the lowest valid line index (cells below lo are outside Size).
+template <bool OPT, int NT>
+__device__ __forceinline__ void hface_flux(const Dims &d, const StepScalars &sc,
+                                           const SharedFields &f, const TracerSet &ts, long c0,
+                                           long st, int pos, int lo, const double *du_line,
+                                           long du_st, double Q, double (&fl)[NT], int &bad) {
+    double v[4], du[4];
+    bool op[4];
+    long cell[4];
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+        const bool in = pos - 2 + o >= lo;
+        cell[o] = c0 + (long)(o - 2) * st;
+        op[o] = in && (f.mask[cell[o]] & M_OPEN);
+        v[o] = in ? f.V[cell[o]] : 1.0;
+        du[o] = in ? du_line[(long)(o - 2) * du_st] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) fl[t] = 0.0;
+    if (!(op[1] && op[2])) return;
     if (OPT) {
-        double D, E;
-        face_superbee2(p, v, du, op, Q, sc.dt, D, E);
-        // HorizontalAdvectionXX_Explicit / YY_Explicit (AD:4560-4572, 4930-4940)
-        return D * p[1] + E * p[2];
+        // ComputeAdvection1D_TVD_SuperBee_2 (MF:10642-10699)
+        const bool up = Q > 0.0;
+        const bool nearb = up ? !op[0] : !op[3];
+        if (nearb) {
+            const double Dc = up ? Q : 0.0, Ec = up ? 0.0 : Q;
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+                fl[t] = Dc * ts.P[cell[1] * ts.s + t] + Ec * ts.P[cell[2] * ts.s + t];
+            return;
+        }
+        const double cr = up ? Q * sc.dt / v[1] : Q * sc.dt / v[2];
+        const double s1 = du[2] + du[1];
+        const double s0 = up ? du[1] + du[0] : du[2] + du[3];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const double *P = ts.P + t;
+            const double p1 = P[cell[1] * ts.s], p2 = P[cell[2] * ts.s];
+            double dc, rr;
+            if (up) {
+                const double p0 = P[cell[0] * ts.s];
+                dc = clamp_min_value((p2 - p1) / s1);
+                rr = (p1 - p0) / s0 / dc;
+            } else {
+                const double p3 = P[cell[3] * ts.s];
+                dc = clamp_min_value((p1 - p2) / s1);
+                rr = (p2 - p3) / s0 / dc;
+            }
+            const double th = 0.5 * superbee_theta(rr) * (1.0 - cr);
+            const double Dc = up ? Q * (1.0 - th) : Q * th;
+            const double Ec = up ? Q * th : Q * (1.0 - th);
+            fl[t] = Dc * p1 + Ec * p2;
+        }
     } else {
-        const Face4 c = face_general(p, v, du, op, Q, sc.dt, sc.volrelmax, method, tvd, up2, bad);
-        // AD:4465-4472 / 4830-4837
-        return c.c * p[0] + c.d * p[1] + c.e * p[2] + c.f * p[3];
+        // ComputeAdvection1D_V2 + ComputeAdvectionFace (MF:10534-10591, 10702-10894)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const double *P = ts.P + t;
+            double p[4];
+#pragma unroll
+            for (int o = 0; o < 4; ++o) p[o] = (pos - 2 + o >= lo) ? P[cell[o] * ts.s] : 0.0;
+            const Face4 c = face_general(p, v, du, op, Q, sc.dt, sc.volrelmax, sc.methodH,
+                                         sc.tvdH, sc.up2H, bad);
+            fl[t] = c.c * p[0] + c.d * p[1] + c.e * p[2] + c.f * p[3];
+        }
     }
 }
 
-// Vertical face coefficients with their explicit flux, for the face between
-// k-1 and k with stencil k-2..k+1.
-struct VFace {
+template <bool OPT, int NT>
+__global__ void __launch_bounds__(HX * HY)
+explicit_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
+                unsigned int *__restrict__ badflag) {
+    __shared__ double FU[NT][HY + 1][HX];
+    __shared__ double FV[NT][HY][HX + 1];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * HX + tx;
+    const int i0 = d.wILB + blockIdx.x * HX, j0 = d.wJLB + blockIdx.y * HY;
+    const int k = d.wKLB + blockIdx.z;
+    const int i = i0 + tx, j = j0 + ty;
+    const double dt = sc.dt;
+    int bad = 0;
+    bool active = false;
+    long x = 0;
+    if (i <= d.wIUB && j <= d.wJUB) {
+        x = d.idx(i, j, k);
+        active = (f.mask[d.idx(i, j, d.wKUB)] & M_WATER) && k >= f.kstart[d.idx2(i, j)];
+    }
+    if (!__syncthreads_or(active)) return;
+
+    if (!sc.vertical1d) {
+        for (int q = tid; q < NFU + NFV; q += HX * HY) {
+            double fl[NT];
+            int r, c;
+            bool isU = q < NFU;
+            if (isU) {
+                r = q / HX; c = q % HX;
+                const int fi = i0 + c, fj = j0 + r;   // face between (fi, fj-1) and (fi, fj)
+                bool on = fi <= d.wIUB && fj <= d.wJUB;
+                const long y = on ? d.idx(fi, fj, k) : 0;
+                on = on && (f.mask[y] & M_CFU);
+                if (on)
+                    hface_flux<OPT, NT>(d, sc, f, ts, y, d.sj, fj, d.JLB, f.DUX + d.idx2(fi, fj),
+                                        d.sj, f.Qx[y], fl, bad);
+                else {
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) fl[t] = 0.0;
+                }
+#pragma unroll
+                for (int t = 0; t < NT; ++t) FU[t][r][c] = fl[t];
+            } else {
+                const int q2 = q - NFU;
+                r = q2 / (HX + 1); c = q2 % (HX + 1);
+                const int fi = i0 + c, fj = j0 + r;   // face between (fi-1, fj) and (fi, fj)
+                bool on = !sc.xzflow && fi <= d.wIUB && fj <= d.wJUB;
+                const long y = on ? d.idx(fi, fj, k) : 0;
+                on = on && (f.mask[y] & M_CFV);
+                if (on)
+                    hface_flux<OPT, NT>(d, sc, f, ts, y, 1, fi, d.ILB, f.DVY + d.idx2(fi, fj), 1,
+                                        f.Qy[y], fl, bad);
+                else {
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) fl[t] = 0.0;
+                }
+#pragma unroll
+                for (int t = 0; t < NT; ++t) FV[t][r][c] = fl[t];
+            }
+        }
+        __syncthreads();
+    }
+    if (bad) atomicOr(badflag, 1u);
+    if (!active) return;
+
+    const uint8_t mk = f.mask[x];
+    const double vk = f.V[x];
+    const bool open = mk & M_OPEN;
+    const double vv = open ? f.Vold[x] / vk : 0.0;
+    const bool hz = !sc.vertical1d;
+    const bool cfu0 = hz && (mk & M_CFU);
+    const bool cfu1 = hz && (f.mask[x + d.sj] & M_CFU);
+    const bool cfv0 = hz && !sc.xzflow && (mk & M_CFV);
+    const bool cfv1 = hz && !sc.xzflow && (f.mask[x + 1] & M_CFV);
+    const double au0 = cfu0 ? f.cu[x] : 0.0, au1 = cfu1 ? f.cu[x + d.sj] : 0.0;
+    const double av0 = cfv0 ? f.cv[x] : 0.0, av1 = cfv1 ? f.cv[x + 1] : 0.0;
+    // tracer-independent diffusion weights of the non-optimised path
+    const double wu0 = au0 * dt / vk, wu1 = au1 * dt / vk;
+    const double wv0 = av0 * dt / vk, wv1 = av1 * dt / vk;
+    // the optimised YY2 form multiplies the conductance by dt first
+    const double bv0 = av0 * dt, bv1 = av1 * dt;
+    const long s = ts.s;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const double *P = ts.P + t;
+        const double pk = P[x * s];
+        double TI = open ? pk * vv : pk;
+        if (cfu0) {
+            const double g = pk - P[(x - d.sj) * s];
+            TI = OPT ? TI - au0 * g * dt / vk : TI - wu0 * g;
+        }
+        if (cfu1) {
+            const double g = P[(x + d.sj) * s] - pk;
+            TI = OPT ? TI + au1 * g * dt / vk : TI + wu1 * g;
+        }
+        if (cfv0) {
+            const double g = pk - P[(x - 1) * s];
+            TI = OPT ? TI - bv0 * g / vk : TI - wv0 * g;
+        }
+        if (cfv1) {
+            const double g = P[(x + 1) * s] - pk;
+            TI = OPT ? TI + bv1 * g / vk : TI + wv1 * g;
+        }
+        if (cfu0) TI = TI + FU[t][ty][tx] * dt / vk;
+        if (cfu1) TI = TI - FU[t][ty + 1][tx] * dt / vk;
+        if (cfv0) TI = TI + FV[t][ty][tx] * dt / vk;
+        if (cfv1) TI = TI - FV[t][ty][tx + 1] * dt / vk;
+        ts.TI[x * s + t] = TI;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Vertical operator and column solve (kernel V).  One lane per water column,
+// walking k from kstart to KUB for NT tracers at once: the rows D/E/F of
+// VerticalDiffusion/VerticalDiffusion2 (AD:2731-2770, 2812-2834, 2894-2906)
+// and VerticalAdvection (+_ExplicitScheme, AD:3059-3120, 3160-3203), the
+// null_real land rows (AD:1753) and the THOMASZ_NewType2 forward sweep
+// (MF:4090-4099), then the back substitution (MF:4102-4106).  The forward
+// sweep's W goes to a scratch array and G to the output, which the back
+// substitution overwrites with the solution.
+
+// Coefficients D/E (and C/F) of the W face with stencil levels q..q+3 for
+// NT tracers.  Shared values (Courant number, spacing) are per face.
+struct VCoef {
     double c, d, e, f;
 };
 
-template <bool OPT>
-__device__ __forceinline__ VFace vface(const double p[4], const double v[4], const double du[4],
-                                       const bool op[4], double Q, bool build,
-                                       const StepScalars &sc, int &bad) {
-    VFace r = {0.0, 0.0, 0.0, 0.0};
-    if (!build) return r;
+template <bool OPT, int NT>
+__device__ __forceinline__ void vface_coefs(const StepScalars &sc, const double (&p)[NT][4],
+                                            const double v[4], const double du[4],
+                                            const bool op[4], double Q, bool build,
+                                            VCoef (&co)[NT], int &bad) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) co[t] = VCoef{0.0, 0.0, 0.0, 0.0};
+    if (!build || !(op[1] && op[2])) return;
     if (OPT) {
-        face_superbee2(p, v, du, op, Q, sc.dt, r.d, r.e);
+        const bool up = Q > 0.0;
+        const bool nearb = up ? !op[0] : !op[3];
+        if (nearb) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                co[t].d = up ? Q : 0.0;
+                co[t].e = up ? 0.0 : Q;
+            }
+            return;
+        }
+        const double cr = up ? Q * sc.dt / v[1] : Q * sc.dt / v[2];
+        const double s1 = du[2] + du[1];
+        const double s0 = up ? du[1] + du[0] : du[2] + du[3];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            double dc, rr;
+            if (up) {
+                dc = clamp_min_value((p[t][2] - p[t][1]) / s1);
+                rr = (p[t][1] - p[t][0]) / s0 / dc;
+            } else {
+                dc = clamp_min_value((p[t][1] - p[t][2]) / s1);
+                rr = (p[t][2] - p[t][3]) / s0 / dc;
+            }
+            const double th = 0.5 * superbee_theta(rr) * (1.0 - cr);
+            co[t].d = up ? Q * (1.0 - th) : Q * th;
+            co[t].e = up ? Q * th : Q * (1.0 - th);
+        }
     } else {
-        const Face4 c = face_general(p, v, du, op, Q, sc.dt, sc.volrelmax, sc.methodV, sc.tvdV,
-                                     sc.up2V, bad);
-        r.c = c.c; r.d = c.d; r.e = c.e; r.f = c.f;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const Face4 c = face_general(p[t], v, du, op, Q, sc.dt, sc.volrelmax, sc.methodV,
+                                         sc.tvdV, sc.up2V, bad);
+            co[t] = VCoef{c.c, c.d, c.e, c.f};
+        }
     }
-    return r;
 }
 
-template <bool OPT>
+template <bool OPT, int NT>
 __global__ void __launch_bounds__(256)
-transport_fp64_kernel(Dims d, StepScalars sc, SharedFields f, const double *__restrict__ P,
-                      double *__restrict__ out, double *__restrict__ Wscr,
-                      unsigned long long *__restrict__ zero_pivots,
-                      unsigned int *__restrict__ badflag) {
+vertical_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
+                unsigned long long *__restrict__ zero_pivots, unsigned int *__restrict__ badflag) {
     const int i = d.wILB + blockIdx.x * blockDim.x + threadIdx.x;
     const int j = d.wJLB + blockIdx.y * blockDim.y + threadIdx.y;
     if (i > d.wIUB || j > d.wJUB) return;
@@ -188,265 +414,177 @@ transport_fp64_kernel(Dims d, StepScalars sc, SharedFields f, const double *__re
     if (!(f.mask[d.idx(i, j, KUB)] & M_WATER)) return;
     const int ks = f.kstart[d.idx2(i, j)];
     const double dt = sc.dt;
+    const long sk = d.sk;
+    const long base = d.idx(i, j, 0);   // index of level 0; level k is base + k*sk
     int bad = 0;
 
-    for (int k = d.wKLB; k < ks && k <= KUB; ++k) out[d.idx(i, j, k)] = MGPU_NULL_REAL;
-
-    // horizontal metrics of the column's lines (constant along k)
-    double dux[5], dvy[5];
+    for (int k = d.wKLB; k < ks && k <= KUB; ++k) {
 #pragma unroll
-    for (int o = 0; o < 5; ++o) {
-        const int jj = min(max(j - 2 + o, d.JLB), d.JUB);
-        const int ii = min(max(i - 2 + o, d.ILB), d.IUB);
-        dux[o] = f.DUX[d.idx2(i, jj)];
-        dvy[o] = f.DVY[d.idx2(ii, j)];
+        for (int t = 0; t < NT; ++t) ts.out[(base + k * sk) * ts.s + t] = MGPU_NULL_REAL;
     }
-    const bool columnBuild = f.mask[d.idx(i, j, KUB)] & M_OPEN;  // AD:2982/3004
+    const bool columnBuild = f.mask[base + KUB * sk] & M_OPEN;  // AD:2982/3004
+    const bool vflux = sc.vert && !sc.vertical1d && columnBuild;
 
-    // vertical window: w*[0..3] = levels k-2..k+1 of the face below row k
-    auto ld = [&](const double *a, int k) -> double {
-        return (k >= d.KLB && k <= d.KUB) ? a[d.idx(i, j, k)] : 0.0;
-    };
-    auto ldm = [&](int k) -> uint8_t {
-        return (k >= d.KLB && k <= d.KUB) ? f.mask[d.idx(i, j, k)] : (uint8_t)0;
-    };
-    double wp[5], wv[5], wdu[5];
-    uint8_t wm[5];
-    // levels ks-2 .. ks+2
+    auto inK = [&](int k) { return k >= d.KLB && k <= d.KUB; };
+    // windows over levels k-1 .. k+2
+    double wv[4], wdu[4];
+    uint8_t wm[4];
+    double wp[NT][4];
 #pragma unroll
-    for (int o = 0; o < 5; ++o) {
-        const int kk = ks - 2 + o;
-        wp[o] = ld(P, kk);
-        wv[o] = ld(f.V, kk);
-        wdu[o] = ld(f.DWZ, kk);
-        wm[o] = ldm(kk);
+    for (int o = 0; o < 4; ++o) {
+        const int kk = ks - 1 + o;
+        const bool in = inK(kk);
+        const long y = base + kk * sk;
+        wv[o] = in ? f.V[y] : 1.0;
+        wdu[o] = in ? f.DWZ[y] : 0.0;
+        wm[o] = in ? f.mask[y] : (uint8_t)0;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) wp[t][o] = in ? ts.P[y * ts.s + t] : 0.0;
     }
-    // coefficients of face ks (between ks-1 and ks)
-    VFace fb;
+    // coefficients of face ks (stencil ks-2 .. ks+1)
+    VCoef fb[NT];
     {
-        const double p4[4] = {wp[0], wp[1], wp[2], wp[3]};
-        const double v4[4] = {wv[0], wv[1], wv[2], wv[3]};
-        const double du4[4] = {wdu[0], wdu[1], wdu[2], wdu[3]};
-        const bool o4[4] = {(wm[0] & M_OPEN) != 0, (wm[1] & M_OPEN) != 0,
-                            (wm[2] & M_OPEN) != 0, (wm[3] & M_OPEN) != 0};
-        const bool build = sc.vert && !sc.vertical1d && columnBuild && ks <= KUB;
-        fb = vface<OPT>(p4, v4, du4, o4, ld(f.Qz, ks), build, sc, bad);
+        const int kk = ks - 2;
+        const bool in = inK(kk);
+        const long y = base + kk * sk;
+        double p[NT][4];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            p[t][0] = in ? ts.P[y * ts.s + t] : 0.0;
+            p[t][1] = wp[t][0]; p[t][2] = wp[t][1]; p[t][3] = wp[t][2];
+        }
+        const double v4[4] = {in ? f.V[y] : 1.0, wv[0], wv[1], wv[2]};
+        const double du4[4] = {in ? f.DWZ[y] : 0.0, wdu[0], wdu[1], wdu[2]};
+        const uint8_t m0 = in ? f.mask[y] : (uint8_t)0;
+        const bool o4[4] = {(m0 & M_OPEN) != 0, (wm[0] & M_OPEN) != 0, (wm[1] & M_OPEN) != 0,
+                            (wm[2] & M_OPEN) != 0};
+        vface_coefs<OPT, NT>(sc, p, v4, du4, o4, inK(ks) ? f.Qz[base + ks * sk] : 0.0,
+                             vflux && ks <= KUB, fb, bad);
     }
 
-    double Wm = 0.0, Gm = 0.0;
-    for (int k = ks; k <= KUB; ++k) {
-        const long x = d.idx(i, j, k);
-        const double pk = wp[2], vk = wv[2];
-        const uint8_t mk = wm[2];
-        // ---- VolumeVariation (AD:3986-4007)
-        double TI = (mk & M_OPEN) ? pk * (f.Vold[x] / vk) : pk;
-        if (!sc.vertical1d) {
-            // ---- stencil along j (XX) and i (YY) at level k
-            double lp[5], lv[5], ip[5], iv[5];
-            bool lo[5], io[5];
+    double Wm[NT], Gm[NT];
 #pragma unroll
-            for (int o = 0; o < 5; ++o) {
-                if (o == 2) {
-                    lp[o] = pk; lv[o] = vk; lo[o] = mk & M_OPEN;
-                    ip[o] = pk; iv[o] = vk; io[o] = mk & M_OPEN;
-                    continue;
-                }
-                const int jj = j - 2 + o, ii = i - 2 + o;
-                if (jj >= d.JLB && jj <= d.JUB) {
-                    const long y = d.idx(i, jj, k);
-                    lp[o] = P[y]; lv[o] = f.V[y]; lo[o] = f.mask[y] & M_OPEN;
-                } else {
-                    lp[o] = 0.0; lv[o] = 1.0; lo[o] = false;
-                }
-                if (!sc.xzflow && ii >= d.ILB && ii <= d.IUB) {
-                    const long y = d.idx(ii, j, k);
-                    ip[o] = P[y]; iv[o] = f.V[y]; io[o] = f.mask[y] & M_OPEN;
-                } else {
-                    ip[o] = 0.0; iv[o] = 1.0; io[o] = false;
-                }
-            }
-            const long xj1 = d.idx(i, j + 1, k), xi1 = d.idx(i + 1, j, k);
-            const bool cfu0 = mk & M_CFU, cfu1 = f.mask[xj1] & M_CFU;
-            const bool cfv0 = mk & M_CFV, cfv1 = !sc.xzflow && (f.mask[xi1] & M_CFV);
-            // ---- HorizontalDiffusionXX / XX2 (AD:5178-5197, 5239-5246)
-            if (cfu0) {
-                const double a = f.cu[x];
-                if (OPT) TI = TI - a * (pk - lp[1]) * dt / vk;
-                else TI = TI - a * dt / vk * (pk - lp[1]);
-            }
-            if (cfu1) {
-                const double a = f.cu[xj1];
-                if (OPT) TI = TI + a * (lp[3] - pk) * dt / vk;
-                else TI = TI + a * dt / vk * (lp[3] - pk);
-            }
-            // ---- HorizontalDiffusionYY / YY2 (AD:5335-5350, 5285-5292)
-            if (!sc.xzflow) {
-                if (cfv0) {
-                    const double a = f.cv[x];
-                    if (OPT) TI = TI - a * dt * (pk - ip[1]) / vk;
-                    else TI = TI - a * dt / vk * (pk - ip[1]);
-                }
-                if (cfv1) {
-                    const double a = f.cv[xi1];
-                    if (OPT) TI = TI + a * dt * (ip[3] - pk) / vk;
-                    else TI = TI + a * dt / vk * (ip[3] - pk);
-                }
-            }
-            // ---- HorizontalAdvectionXX (+_Explicit) (AD:4463-4487, 4560-4572)
-            if (cfu0) {
-                const double p4[4] = {lp[0], lp[1], lp[2], lp[3]};
-                const double v4[4] = {lv[0], lv[1], lv[2], lv[3]};
-                const double du4[4] = {dux[0], dux[1], dux[2], dux[3]};
-                const bool o4[4] = {lo[0], lo[1], lo[2], lo[3]};
-                const double fl = hflux<OPT>(p4, v4, du4, o4, f.Qx[x], sc, sc.methodH, sc.tvdH,
-                                             sc.up2H, bad);
-                TI = TI + fl * dt / vk;
-            }
-            if (cfu1) {
-                const double p4[4] = {lp[1], lp[2], lp[3], lp[4]};
-                const double v4[4] = {lv[1], lv[2], lv[3], lv[4]};
-                const double du4[4] = {dux[1], dux[2], dux[3], dux[4]};
-                const bool o4[4] = {lo[1], lo[2], lo[3], lo[4]};
-                const double fl = hflux<OPT>(p4, v4, du4, o4, f.Qx[xj1], sc, sc.methodH,
-                                             sc.tvdH, sc.up2H, bad);
-                TI = TI - fl * dt / vk;
-            }
-            // ---- HorizontalAdvectionYY (+_Explicit) (AD:4828-4853, 4930-4940)
-            if (!sc.xzflow) {
-                if (cfv0) {
-                    const double p4[4] = {ip[0], ip[1], ip[2], ip[3]};
-                    const double v4[4] = {iv[0], iv[1], iv[2], iv[3]};
-                    const double du4[4] = {dvy[0], dvy[1], dvy[2], dvy[3]};
-                    const bool o4[4] = {io[0], io[1], io[2], io[3]};
-                    const double fl = hflux<OPT>(p4, v4, du4, o4, f.Qy[x], sc, sc.methodH,
-                                                 sc.tvdH, sc.up2H, bad);
-                    TI = TI + fl * dt / vk;
-                }
-                if (cfv1) {
-                    const double p4[4] = {ip[1], ip[2], ip[3], ip[4]};
-                    const double v4[4] = {iv[1], iv[2], iv[3], iv[4]};
-                    const double du4[4] = {dvy[1], dvy[2], dvy[3], dvy[4]};
-                    const bool o4[4] = {io[1], io[2], io[3], io[4]};
-                    const double fl = hflux<OPT>(p4, v4, du4, o4, f.Qy[xi1], sc, sc.methodH,
-                                                 sc.tvdH, sc.up2H, bad);
-                    TI = TI - fl * dt / vk;
-                }
-            }
+    for (int t = 0; t < NT; ++t) { Wm[t] = 0.0; Gm[t] = 0.0; }
+    for (int k = ks; k <= KUB; ++k) {
+        const long x = base + k * sk;
+        const double vk = wv[1];
+        const uint8_t mk = wm[1];
+        // coefficients of the face above (k, k+1): stencil k-1 .. k+2
+        VCoef ft[NT];
+        {
+            const bool o4[4] = {(wm[0] & M_OPEN) != 0, (wm[1] & M_OPEN) != 0,
+                                (wm[2] & M_OPEN) != 0, (wm[3] & M_OPEN) != 0};
+            vface_coefs<OPT, NT>(sc, wp, wv, wdu, o4, inK(k + 1) ? f.Qz[x + sk] : 0.0,
+                                 vflux && k < KUB, ft, bad);
         }
-
-        // ---- rows: E after VolumeVariation (AD:3994-3999), D = F = 0
-        double E = 1.0, D = 0.0, F = 0.0;
+        // row values shared by all tracers
+        double E0 = 1.0;
         if (k == KUB && (mk & M_OPEN)) {
             const double dtv = dt / vk;
-            E = 1.0 + dtv * f.Qz[d.idx(i, j, k + 1)];
+            E0 = 1.0 + dtv * f.Qz[x + sk];
         }
-        // coefficients of the face above (k, k+1), stencil k-1..k+2
-        VFace ft = {0.0, 0.0, 0.0, 0.0};
-        const uint8_t mUp = wm[3];
-        const bool cfwB = mk & M_CFW, cfwT = (k < KUB) && (mUp & M_CFW);
-        if (sc.vert) {
-            {
-                const double p4[4] = {wp[1], wp[2], wp[3], wp[4]};
-                const double v4[4] = {wv[1], wv[2], wv[3], wv[4]};
-                const double du4[4] = {wdu[1], wdu[2], wdu[3], wdu[4]};
-                const bool o4[4] = {(wm[1] & M_OPEN) != 0, (wm[2] & M_OPEN) != 0,
-                                    (wm[3] & M_OPEN) != 0, (wm[4] & M_OPEN) != 0};
-                const bool build = !sc.vertical1d && columnBuild && k < KUB;
-                ft = vface<OPT>(p4, v4, du4, o4, ld(f.Qz, k + 1), build, sc, bad);
-            }
-            // ---- VerticalDiffusion (AD:2731-2770) / VerticalDiffusion2 (AD:2812-2834, 2894-2906)
+        const bool cfwB = sc.vert && (mk & M_CFW);
+        const bool cfwT = sc.vert && k < KUB && (wm[2] & M_CFW);
+        const double a2 = cfwB ? f.cw[x] * dt / vk : 0.0;
+        const double a1 = cfwT ? f.cw[x + sk] * dt / vk : 0.0;
+        const double dtk = dt / vk;
+        const bool land = mk & M_LAND;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const double pk = wp[t][1], pm = wp[t][0], pp = wp[t][2];
+            double TI = ts.TI[x * ts.s + t];
+            double E = E0, D = 0.0, F = 0.0;
             if (cfwB) {
-                const double a2 = f.cw[x] * dt / vk;
                 if (OPT) {
                     if (sc.thV > 0.0) { D = D - a2; E = E + a2; }
-                    else TI = TI - a2 * (pk - wp[1]);
+                    else TI = TI - a2 * (pk - pm);
                 } else {
                     D = D - a2 * sc.thV;
                     E = E + a2 * sc.thV;
-                    TI = TI - a2 * (pk - wp[1]) * (1.0 - sc.thV);
+                    TI = TI - a2 * (pk - pm) * (1.0 - sc.thV);
                 }
             }
             if (cfwT) {
-                const double a1 = f.cw[d.idx(i, j, k + 1)] * dt / vk;
                 if (OPT) {
                     if (sc.thV > 0.0) { E = E + a1; F = F - a1; }
-                    else TI = TI + a1 * (wp[3] - pk);
+                    else TI = TI + a1 * (pp - pk);
                 } else {
                     E = E + a1 * sc.thV;
                     F = F - a1 * sc.thV;
-                    TI = TI + a1 * (wp[3] - pk) * (1.0 - sc.thV);
+                    TI = TI + a1 * (pp - pk) * (1.0 - sc.thV);
                 }
             }
-            // ---- VerticalAdvection (AD:3059-3120) / _ExplicitScheme (AD:3160-3203)
-            if (!sc.vertical1d) {
+            if (sc.vert && !sc.vertical1d) {
                 if (sc.impAdvV) {
-                    if (cfwB) {
-                        const double dt2 = dt / vk;
-                        D = D - fb.d * dt2;
-                        E = E - fb.e * dt2;
-                    }
-                    if (cfwT) {
-                        const double dt1 = dt / vk;
-                        E = E + ft.d * dt1;
-                        F = F + ft.e * dt1;
-                    }
+                    if (cfwB) { D = D - fb[t].d * dtk; E = E - fb[t].e * dtk; }
+                    if (cfwT) { E = E + ft[t].d * dtk; F = F + ft[t].e * dtk; }
                 } else if (OPT) {
-                    if (cfwB) TI = TI + (fb.d * wp[1] + fb.e * pk) * dt / vk;
-                    if (cfwT) TI = TI - (ft.d * pk + ft.e * wp[3]) * dt / vk;
+                    if (cfwB) TI = TI + (fb[t].d * pm + fb[t].e * pk) * dt / vk;
+                    if (cfwT) TI = TI - (ft[t].d * pk + ft[t].e * pp) * dt / vk;
                 } else {
                     if (cfwB) {
-                        const double fl = fb.c * wp[0] + fb.d * wp[1] + fb.e * pk + fb.f * wp[3];
+                        const double pmm =
+                            (fb[t].c == 0.0 || k - 2 < d.KLB) ? 0.0
+                                                              : ts.P[(x - 2 * sk) * ts.s + t];
+                        const double fl = fb[t].c * pmm + fb[t].d * pm + fb[t].e * pk + fb[t].f * pp;
                         TI = TI + fl * dt / vk;
                     }
                     if (cfwT) {
-                        const double fl = ft.c * wp[1] + ft.d * pk + ft.e * wp[3] + ft.f * wp[4];
+                        const double fl = ft[t].c * pm + ft[t].d * pk + ft[t].e * pp +
+                                          ft[t].f * wp[t][3];
                         TI = TI - fl * dt / vk;
                     }
                 }
             }
+            if (land) TI = MGPU_NULL_REAL;
+            // THOMASZ_NewType2 forward sweep (MF:4090-4099)
+            const double aux = E + D * Wm[t];
+            double W = 0.0, G = 0.0;
+            if (fabs(aux) > 0.0) {
+                W = -F / aux;
+                G = (TI - D * Gm[t]) / aux;
+            } else {
+                atomicAdd(zero_pivots, 1ull);
+            }
+            ts.W[x * ts.s + t] = W;
+            ts.out[x * ts.s + t] = G;
+            Wm[t] = W;
+            Gm[t] = G;
+            fb[t] = ft[t];
         }
-        // ---- land points keep null_real (AD:1753)
-        if (mk & M_LAND) TI = MGPU_NULL_REAL;
-        // ---- THOMASZ_NewType2 forward sweep (MF:4090-4099)
-        const double aux = E + D * Wm;
-        double W, G;
-        if (fabs(aux) > 0.0) {
-            W = -F / aux;
-            G = (TI - D * Gm) / aux;
-        } else {
-            W = 0.0;
-            G = 0.0;
-            atomicAdd(zero_pivots, 1ull);
-        }
-        Wscr[x] = W;
-        out[x] = G;
-        Wm = W;
-        Gm = G;
-        // ---- slide the vertical window up one level
-        fb = ft;
+        // slide the windows up one level
+        const int kn = k + 3;
+        const bool in = inK(kn);
+        const long y = base + kn * sk;
 #pragma unroll
-        for (int o = 0; o < 4; ++o) {
-            wp[o] = wp[o + 1]; wv[o] = wv[o + 1]; wdu[o] = wdu[o + 1]; wm[o] = wm[o + 1];
+        for (int o = 0; o < 3; ++o) { wv[o] = wv[o + 1]; wdu[o] = wdu[o + 1]; wm[o] = wm[o + 1]; }
+        wv[3] = in ? f.V[y] : 1.0;
+        wdu[3] = in ? f.DWZ[y] : 0.0;
+        wm[3] = in ? f.mask[y] : (uint8_t)0;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            wp[t][0] = wp[t][1]; wp[t][1] = wp[t][2]; wp[t][2] = wp[t][3];
+            wp[t][3] = in ? ts.P[y * ts.s + t] : 0.0;
         }
-        wp[4] = ld(P, k + 3);
-        wv[4] = ld(f.V, k + 3);
-        wdu[4] = ld(f.DWZ, k + 3);
-        wm[4] = ldm(k + 3);
     }
-    // ---- ghost top row KUB+1: E = 1, D = F = 0, TI = 0 or null_real
-    double x1;
-    {
-        const uint8_t mg = f.mask[d.idx(i, j, KUB + 1)];
-        const double TIg = (mg & M_LAND) ? MGPU_NULL_REAL : 0.0;
-        const double aux = 1.0 + 0.0 * Wm;
-        x1 = (TIg - 0.0 * Gm) / aux;
-        out[d.idx(i, j, KUB + 1)] = x1;
+    // ghost top row KUB+1: E = 1, D = F = 0, TI = 0 or null_real (AD:1662-1695, 1753)
+    const long xg = base + (KUB + 1) * sk;
+    const double TIg = (f.mask[xg] & M_LAND) ? MGPU_NULL_REAL : 0.0;
+    double xs[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const double aux = 1.0 + 0.0 * Wm[t];
+        xs[t] = (TIg - 0.0 * Gm[t]) / aux;
+        ts.out[xg * ts.s + t] = xs[t];
     }
-    // ---- back substitution (MF:4102-4106)
+    // back substitution (MF:4102-4106)
     for (int k = KUB; k >= ks; --k) {
-        const long x = d.idx(i, j, k);
-        x1 = Wscr[x] * x1 + out[x];
-        out[x] = x1;
+        const long x = base + k * sk;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            xs[t] = ts.W[x * ts.s + t] * xs[t] + ts.out[x * ts.s + t];
+            ts.out[x * ts.s + t] = xs[t];
+        }
     }
     if (bad) atomicOr(badflag, 1u);
 }
@@ -527,24 +665,62 @@ int launch_prep_faces(const Dims &d, const double *visc, const double *diffv, co
     return MGPU_SUCCESS;
 }
 
+// Launch kernels H and V for `n` tracers (chunks of up to MAXT per launch).
+// phase bit 1: kernel H, bit 2: kernel V
+template <bool OPT, int NT>
+static int launch_chunk(int phase, const Dims &d, const StepScalars &sc, const SharedFields &f,
+                        const TracerSet &ts, unsigned long long *counters, int tile_i,
+                        int tile_j, hipStream_t s) {
+    const int ni = d.wIUB - d.wILB + 1, nj = d.wJUB - d.wJLB + 1, nk = d.wKUB - d.wKLB + 1;
+    unsigned int *bad = reinterpret_cast<unsigned int *>(counters + 1);
+    if (phase & 1) {
+        dim3 bh(HX, HY), gh((ni + HX - 1) / HX, (nj + HY - 1) / HY, nk);
+        hipLaunchKernelGGL((explicit_kernel<OPT, NT>), gh, bh, 0, s, d, sc, f, ts, bad);
+        HIPCHK(hipGetLastError());
+    }
+    if (phase & 2) {
+        dim3 bv(tile_i, tile_j), gv((ni + tile_i - 1) / tile_i, (nj + tile_j - 1) / tile_j);
+        hipLaunchKernelGGL((vertical_kernel<OPT, NT>), gv, bv, 0, s, d, sc, f, ts, counters,
+                           bad);
+        HIPCHK(hipGetLastError());
+    }
+    return MGPU_SUCCESS;
+}
+
+template <bool OPT>
+static int launch_n(int nt, int ph, const Dims &d, const StepScalars &sc, const SharedFields &f,
+                    const TracerSet &ts, unsigned long long *c, int ti, int tj, hipStream_t s) {
+    switch (nt) {
+        case 1: return launch_chunk<OPT, 1>(ph, d, sc, f, ts, c, ti, tj, s);
+        case 2: return launch_chunk<OPT, 2>(ph, d, sc, f, ts, c, ti, tj, s);
+        case 3: return launch_chunk<OPT, 3>(ph, d, sc, f, ts, c, ti, tj, s);
+        case 4: return launch_chunk<OPT, 4>(ph, d, sc, f, ts, c, ti, tj, s);
+        case 5: return launch_chunk<OPT, 5>(ph, d, sc, f, ts, c, ti, tj, s);
+        case 6: return launch_chunk<OPT, 6>(ph, d, sc, f, ts, c, ti, tj, s);
+        case 7: return launch_chunk<OPT, 7>(ph, d, sc, f, ts, c, ti, tj, s);
+        case 8: return launch_chunk<OPT, 8>(ph, d, sc, f, ts, c, ti, tj, s);
+        default: return MGPU_ERR_BADARG;
+    }
+}
+
 int launch_transport_fp64(const Dims &d, const StepScalars &sc, const SharedFields &f,
-                          const double *P, double *out, double *Wscr,
-                          unsigned long long *zero_pivots, int tile_i, int tile_j,
-                          hipStream_t s) {
+                          int n, long stride, const double *P, double *TI, double *out,
+                          double *W, unsigned long long *counters, int chunk, int tile_i,
+                          int tile_j, hipStream_t s, int phase) {
     if (tile_i <= 0) tile_i = 64;
     if (tile_j <= 0) tile_j = 4;
-    if (tile_i * tile_j > 256) return MGPU_ERR_BADARG;
-    const int ni = d.wIUB - d.wILB + 1, nj = d.wJUB - d.wJLB + 1;
-    dim3 b(tile_i, tile_j), g((ni + tile_i - 1) / tile_i, (nj + tile_j - 1) / tile_j);
-    // the bad-method flag shares the zero-pivot counter's allocation (word 2)
-    unsigned int *bad = reinterpret_cast<unsigned int *>(zero_pivots + 1);
-    if (sc.optimize)
-        hipLaunchKernelGGL(transport_fp64_kernel<true>, g, b, 0, s, d, sc, f, P, out, Wscr,
-                           zero_pivots, bad);
-    else
-        hipLaunchKernelGGL(transport_fp64_kernel<false>, g, b, 0, s, d, sc, f, P, out, Wscr,
-                           zero_pivots, bad);
-    HIPCHK(hipGetLastError());
+    if (tile_i * tile_j > 256 || tile_i * tile_j < 1) return MGPU_ERR_BADARG;
+    if (chunk <= 0 || chunk > MAXT) chunk = MAXT;
+    if (stride < n) return MGPU_ERR_BADARG;
+    for (int t0 = 0; t0 < n; t0 += chunk) {
+        const int nt = n - t0 < chunk ? n - t0 : chunk;
+        TracerSet ts;
+        ts.P = P + t0; ts.TI = TI + t0; ts.out = out + t0; ts.W = W + t0; ts.s = stride;
+        const int rc =
+            sc.optimize ? launch_n<true>(nt, phase, d, sc, f, ts, counters, tile_i, tile_j, s)
+                        : launch_n<false>(nt, phase, d, sc, f, ts, counters, tile_i, tile_j, s);
+        if (rc) return rc;
+    }
     return MGPU_SUCCESS;
 }
 
@@ -554,6 +730,29 @@ int launch_thomasz(const Dims &d, const double *D, const double *E, const double
     const int ni = d.wIUB - d.wILB + 1, nj = d.wJUB - d.wJLB + 1;
     dim3 b(64, 4), g((ni + 63) / 64, (nj + 3) / 4);
     hipLaunchKernelGGL(thomasz_kernel, g, b, 0, s, d, D, E, F, TI, wp, res, Wscr);
+    HIPCHK(hipGetLastError());
+    return MGPU_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// One tracer between a Fortran array and an interleaved buffer.
+namespace {
+__global__ void interleave_kernel(long n3, const double *__restrict__ src,
+                                  double *__restrict__ dst, long stride, int t, int to) {
+    for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < n3;
+         x += (long)gridDim.x * blockDim.x) {
+        if (to) dst[x * stride + t] = src[x];
+        else dst[x] = src[x * stride + t];
+    }
+}
+}  // namespace
+
+int launch_interleave(long n3, const double *src, double *dst, long stride, int t,
+                      int to_interleaved, hipStream_t s) {
+    long blocks = (n3 + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(interleave_kernel, dim3((unsigned)blocks), dim3(256), 0, s, n3, src, dst,
+                       stride, t, to_interleaved);
     HIPCHK(hipGetLastError());
     return MGPU_SUCCESS;
 }

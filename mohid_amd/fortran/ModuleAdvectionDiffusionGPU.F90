@@ -1,0 +1,282 @@
+!------------------------------------------------------------------------------
+! ModuleAdvectionDiffusionGPU
+!
+! Fortran host side of the MI355X tracer-transport library: ISO_C_BINDING
+! interfaces to the C ABI of include/mohid_gpu.h and Fortran wrappers that
+! keep the argument lists of the reference module
+!   StartAdvectionDiffusion  Software/MOHIDBase2/ModuleAdvectionDiffusion.F90:400
+!   AdvectionDiffusion       Software/MOHIDBase2/ModuleAdvectionDiffusion.F90:1108
+!   KillAdvectionDiffusion   Software/MOHIDBase2/ModuleAdvectionDiffusion.F90:5849
+!   THOMASZ_NewType2         Software/MOHIDBase1/ModuleFunctions.F90:4026
+! so a caller such as ModuleWaterProperties::Advection_Diffusion_Processes
+! (Software/MOHIDWater/ModuleWaterProperties.F90:14776) can switch its call.
+!
+! Arrays are the caller's own: real(8) (the reference's production build is
+! default-real-8, Solutions/mohid-in-linux/compile_mohid.sh:185) and integer
+! masks spanning Size, passed by the address of their first element.  STAT
+! returns 0 on success or the library's status code: MGPU_ERR_UNSUPPORTED
+! (103) asks the caller to keep its CPU path for this call.
+!------------------------------------------------------------------------------
+module ModuleAdvectionDiffusionGPU
+
+    use, intrinsic :: iso_c_binding
+    implicit none
+    private
+
+    public :: StartAdvectionDiffusionGPU
+    public :: AdvectionDiffusionGPU
+    public :: KillAdvectionDiffusionGPU
+    public :: THOMASZ_GPU
+    public :: MGPU_SUCCESS, MGPU_OFF_ERR, MGPU_ERR_OPTION, MGPU_ERR_UNSUPPORTED
+
+    integer, parameter :: MGPU_SUCCESS         = 0
+    integer, parameter :: MGPU_OFF_ERR         = 1
+    integer, parameter :: MGPU_ERR_OPTION      = 102
+    integer, parameter :: MGPU_ERR_UNSUPPORTED = 103
+
+    type, bind(c) :: t_mgpu_size3d
+        integer(c_int32_t) :: ILB, IUB, JLB, JUB, KLB, KUB
+    end type t_mgpu_size3d
+
+    type, bind(c) :: t_mgpu_geometry
+        type(c_ptr)        :: DUX, DVY, DZX, DZY
+        type(c_ptr)        :: KFloorZ
+        type(c_ptr)        :: AreaU, AreaV, DZZ, DWZ
+        integer(c_int32_t) :: device_pointers
+    end type t_mgpu_geometry
+
+    type, bind(c) :: t_mgpu_ad_args
+        real(c_double)     :: schmidt_H, SchmidtCoef_V, SchmidtBackground_V
+        integer(c_int32_t) :: AdvMethodH, TVDLimitationH, AdvMethodV, TVDLimitationV
+        integer(c_int32_t) :: Upwind2H, Upwind2V
+        real(c_double)     :: VolumeRelMax
+        integer(c_int32_t) :: AdvectionNudging, AdvectionNudgingCells
+        real(c_double)     :: DTProp
+        real(c_double)     :: ImpExp_AdvV, ImpExp_DifV, ImpExp_AdvXX, ImpExp_AdvYY, ImpExp_DifH
+        integer(c_int32_t) :: NullDif
+        type(c_ptr)        :: Wflux_X, Wflux_Y, Wflux_Z
+        type(c_ptr)        :: VolumeZOld, VolumeZ
+        type(c_ptr)        :: OpenPoints3D, LandPoints3D
+        type(c_ptr)        :: ComputeFacesU3D, ComputeFacesV3D, ComputeFacesW3D
+        type(c_ptr)        :: Visc_H, Diff_V
+        type(c_ptr)        :: WaterPoints3D
+        integer(c_int32_t) :: Optimize, FirstProperty
+        integer(c_int32_t) :: unsupported_CellFluxes, unsupported_ReferenceProp
+        integer(c_int32_t) :: unsupported_BoundaryCondition
+        integer(c_int32_t) :: unsupported_SmallDepths, unsupported_NoAdvFlux
+        integer(c_int32_t) :: unsupported_NoDifFlux, unsupported_Discharges
+        integer(c_int32_t) :: device_pointers
+    end type t_mgpu_ad_args
+
+    interface
+        integer(c_int) function mgpu_ad_create(size, worksize, docycle_method, vertical1d, &
+                                               xzflow, geometry, handle)                   &
+                       bind(c, name="mgpu_ad_create")
+            import :: c_int, c_int32_t, c_ptr, t_mgpu_size3d, t_mgpu_geometry
+            type(t_mgpu_size3d),   intent(in)  :: size, worksize
+            integer(c_int32_t),    value       :: docycle_method, vertical1d, xzflow
+            type(t_mgpu_geometry), intent(in)  :: geometry
+            type(c_ptr),           intent(out) :: handle
+        end function mgpu_ad_create
+
+        integer(c_int) function mgpu_ad_step(handle, args, prop) bind(c, name="mgpu_ad_step")
+            import :: c_int, c_ptr, t_mgpu_ad_args
+            type(c_ptr),          value      :: handle
+            type(t_mgpu_ad_args), intent(in) :: args
+            type(c_ptr),          value      :: prop
+        end function mgpu_ad_step
+
+        integer(c_int) function mgpu_ad_destroy(handle) bind(c, name="mgpu_ad_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle
+        end function mgpu_ad_destroy
+
+        integer(c_int) function mgpu_thomasz(size, worksize, D, E, F, TI, waterpoints, res, &
+                                             dev, stream) bind(c, name="mgpu_thomasz")
+            import :: c_int, c_int32_t, c_ptr, t_mgpu_size3d
+            type(t_mgpu_size3d), intent(in) :: size, worksize
+            type(c_ptr),         value      :: D, E, F, TI, waterpoints, res
+            integer(c_int32_t),  value      :: dev
+            type(c_ptr),         value      :: stream
+        end function mgpu_thomasz
+    end interface
+
+    ! instance table: integer IDs as MOHID's modules hand out (ModuleGlobalData
+    ! RegisterNewInstance), each holding one library handle
+    type(c_ptr), dimension(64), save :: Handles = c_null_ptr
+
+contains
+
+    function to_size3d(b) result(s)
+        integer, dimension(6), intent(in) :: b
+        type(t_mgpu_size3d) :: s
+        s%ILB = b(1); s%IUB = b(2); s%JLB = b(3); s%JUB = b(4); s%KLB = b(5); s%KUB = b(6)
+    end function to_size3d
+
+    function b2i(l) result(i)
+        logical, intent(in) :: l
+        integer(c_int32_t)  :: i
+        i = 0
+        if (l) i = 1
+    end function b2i
+
+    !> StartAdvectionDiffusion plus the geometry AdvectionDiffusion fetches
+    !> from HorizontalGrid/Geometry (ModuleAdvectionDiffusion.F90:1353-1401).
+    !> Size/WorkSize: ILB, IUB, JLB, JUB, KLB, KUB (T_Size3D order).
+    subroutine StartAdvectionDiffusionGPU(ID, Size, WorkSize, DUX, DVY, DZX, DZY, KFloorZ, &
+                                          AreaU, AreaV, DZZ, DWZ, Docycle_method,            &
+                                          Vertical1D, XZFlow, STAT)
+        integer,                           intent(out) :: ID
+        integer, dimension(6),             intent(in)  :: Size, WorkSize
+        real(8), dimension(:,:),   target, intent(in)  :: DUX, DVY, DZX, DZY
+        integer, dimension(:,:),   target, intent(in)  :: KFloorZ
+        real(8), dimension(:,:,:), target, intent(in)  :: AreaU, AreaV, DZZ, DWZ
+        integer,                           intent(in)  :: Docycle_method
+        logical, optional,                 intent(in)  :: Vertical1D, XZFlow
+        integer, optional,                 intent(out) :: STAT
+        type(t_mgpu_geometry) :: g
+        type(t_mgpu_size3d)   :: s, w
+        type(c_ptr)           :: h
+        integer               :: rc, n
+        logical               :: v1d, xz
+
+        v1d = .false.; if (present(Vertical1D)) v1d = Vertical1D
+        xz  = .false.; if (present(XZFlow))     xz  = XZFlow
+        g%DUX = c_loc(DUX); g%DVY = c_loc(DVY); g%DZX = c_loc(DZX); g%DZY = c_loc(DZY)
+        g%KFloorZ = c_loc(KFloorZ)
+        g%AreaU = c_loc(AreaU); g%AreaV = c_loc(AreaV); g%DZZ = c_loc(DZZ); g%DWZ = c_loc(DWZ)
+        g%device_pointers = 0
+        s = to_size3d(Size); w = to_size3d(WorkSize)
+        ID = 0
+        rc = mgpu_ad_create(s, w, int(Docycle_method, c_int32_t), b2i(v1d), b2i(xz), g, h)
+        if (rc == MGPU_SUCCESS) then
+            do n = 1, size(Handles)
+                if (.not. c_associated(Handles(n))) then
+                    Handles(n) = h
+                    ID = n
+                    exit
+                endif
+            enddo
+            if (ID == 0) then
+                rc = mgpu_ad_destroy(h)
+                rc = -99
+            endif
+        endif
+        if (present(STAT)) STAT = rc
+    end subroutine StartAdvectionDiffusionGPU
+
+    !> AdvectionDiffusion (ModuleAdvectionDiffusion.F90:1108-1198): PROP is
+    !> advanced by DTProp in place.  Arguments keep the reference's names,
+    !> order and meaning; options the GPU path does not offload are passed
+    !> as present/absent flags and make STAT = MGPU_ERR_UNSUPPORTED.
+    subroutine AdvectionDiffusionGPU(ID, PROP, schmidt_H, SchmidtCoef_V, SchmidtBackground_V, &
+                                     AdvMethodH, TVDLimitationH, AdvMethodV, TVDLimitationV,  &
+                                     Upwind2H, Upwind2V, VolumeRelMax, AdvectionNudging,      &
+                                     AdvectionNudgingCells, DTProp, ImpExp_AdvV, ImpExp_DifV, &
+                                     ImpExp_AdvXX, ImpExp_AdvYY, ImpExp_DifH, NullDif,        &
+                                     Wflux_X, Wflux_Y, Wflux_Z, VolumeZOld, VolumeZ,          &
+                                     OpenPoints3D, LandPoints3D, ComputeFacesU3D,             &
+                                     ComputeFacesV3D, ComputeFacesW3D, Visc_H, Diff_V,        &
+                                     WaterPoints3D, CellFluxes, Discharges, Optimize,         &
+                                     FirstProperty, STAT)
+        integer,                           intent(in)    :: ID
+        real(8), dimension(:,:,:), target, intent(inout) :: PROP
+        real(8),                           intent(in)    :: schmidt_H, SchmidtCoef_V
+        real(8),                           intent(in)    :: SchmidtBackground_V
+        integer,                           intent(in)    :: AdvMethodH, TVDLimitationH
+        integer,                           intent(in)    :: AdvMethodV, TVDLimitationV
+        logical,                           intent(in)    :: Upwind2H, Upwind2V
+        real(8),                           intent(in)    :: VolumeRelMax
+        logical,                           intent(in)    :: AdvectionNudging
+        integer,                           intent(in)    :: AdvectionNudgingCells
+        real(8),                           intent(in)    :: DTProp
+        real(8),                           intent(in)    :: ImpExp_AdvV, ImpExp_DifV
+        real(8),                           intent(in)    :: ImpExp_AdvXX, ImpExp_AdvYY
+        real(8),                           intent(in)    :: ImpExp_DifH
+        logical,                           intent(in)    :: NullDif
+        real(8), dimension(:,:,:), target, intent(in)    :: Wflux_X, Wflux_Y, Wflux_Z
+        real(8), dimension(:,:,:), target, intent(in)    :: VolumeZOld, VolumeZ
+        integer, dimension(:,:,:), target, intent(in)    :: OpenPoints3D, LandPoints3D
+        integer, dimension(:,:,:), target, intent(in)    :: ComputeFacesU3D, ComputeFacesV3D
+        integer, dimension(:,:,:), target, intent(in)    :: ComputeFacesW3D
+        real(8), dimension(:,:,:), target, intent(in)    :: Visc_H, Diff_V
+        integer, dimension(:,:,:), target, intent(in)    :: WaterPoints3D
+        logical, optional,                 intent(in)    :: CellFluxes, Discharges
+        logical, optional,                 intent(in)    :: Optimize, FirstProperty
+        integer, optional,                 intent(out)   :: STAT
+        type(t_mgpu_ad_args) :: a
+        integer              :: rc
+
+        if (ID < 1 .or. ID > size(Handles)) then
+            if (present(STAT)) STAT = MGPU_OFF_ERR
+            return
+        endif
+        a%schmidt_H = schmidt_H
+        a%SchmidtCoef_V = SchmidtCoef_V
+        a%SchmidtBackground_V = SchmidtBackground_V
+        a%AdvMethodH = AdvMethodH; a%TVDLimitationH = TVDLimitationH
+        a%AdvMethodV = AdvMethodV; a%TVDLimitationV = TVDLimitationV
+        a%Upwind2H = b2i(Upwind2H); a%Upwind2V = b2i(Upwind2V)
+        a%VolumeRelMax = VolumeRelMax
+        a%AdvectionNudging = b2i(AdvectionNudging)
+        a%AdvectionNudgingCells = AdvectionNudgingCells
+        a%DTProp = DTProp
+        a%ImpExp_AdvV = ImpExp_AdvV; a%ImpExp_DifV = ImpExp_DifV
+        a%ImpExp_AdvXX = ImpExp_AdvXX; a%ImpExp_AdvYY = ImpExp_AdvYY
+        a%ImpExp_DifH = ImpExp_DifH
+        a%NullDif = b2i(NullDif)
+        a%Wflux_X = c_loc(Wflux_X); a%Wflux_Y = c_loc(Wflux_Y); a%Wflux_Z = c_loc(Wflux_Z)
+        a%VolumeZOld = c_loc(VolumeZOld); a%VolumeZ = c_loc(VolumeZ)
+        a%OpenPoints3D = c_loc(OpenPoints3D); a%LandPoints3D = c_loc(LandPoints3D)
+        a%ComputeFacesU3D = c_loc(ComputeFacesU3D)
+        a%ComputeFacesV3D = c_loc(ComputeFacesV3D)
+        a%ComputeFacesW3D = c_loc(ComputeFacesW3D)
+        a%Visc_H = c_loc(Visc_H); a%Diff_V = c_loc(Diff_V)
+        a%WaterPoints3D = c_loc(WaterPoints3D)
+        a%Optimize = 0; if (present(Optimize)) a%Optimize = b2i(Optimize)
+        a%FirstProperty = 0; if (present(FirstProperty)) a%FirstProperty = b2i(FirstProperty)
+        a%unsupported_CellFluxes = 0
+        if (present(CellFluxes)) a%unsupported_CellFluxes = b2i(CellFluxes)
+        a%unsupported_ReferenceProp = 0
+        a%unsupported_BoundaryCondition = 0
+        a%unsupported_SmallDepths = 0
+        a%unsupported_NoAdvFlux = 0
+        a%unsupported_NoDifFlux = 0
+        a%unsupported_Discharges = 0
+        if (present(Discharges)) a%unsupported_Discharges = b2i(Discharges)
+        a%device_pointers = 0
+        rc = mgpu_ad_step(Handles(ID), a, c_loc(PROP))
+        if (present(STAT)) STAT = rc
+    end subroutine AdvectionDiffusionGPU
+
+    !> KillAdvectionDiffusion (ModuleAdvectionDiffusion.F90:5849).
+    subroutine KillAdvectionDiffusionGPU(ID, STAT)
+        integer,           intent(inout) :: ID
+        integer, optional, intent(out)   :: STAT
+        integer :: rc
+        rc = MGPU_OFF_ERR
+        if (ID >= 1 .and. ID <= size(Handles)) then
+            if (c_associated(Handles(ID))) then
+                rc = mgpu_ad_destroy(Handles(ID))
+                Handles(ID) = c_null_ptr
+                ID = 0
+            endif
+        endif
+        if (present(STAT)) STAT = rc
+    end subroutine KillAdvectionDiffusionGPU
+
+    !> THOMASZ_NewType2 (ModuleFunctions.F90:4026-4123) on the caller's
+    !> D/E/F/TI arrays (Size-shaped), writing the water columns of RES.
+    subroutine THOMASZ_GPU(Size, WorkSize, D, E, F, TI, WaterPoints, RES, STAT)
+        integer, dimension(6),             intent(in)    :: Size, WorkSize
+        real(8), dimension(:,:,:), target, intent(in)    :: D, E, F, TI
+        integer, dimension(:,:,:), target, intent(in)    :: WaterPoints
+        real(8), dimension(:,:,:), target, intent(inout) :: RES
+        integer, optional,                 intent(out)   :: STAT
+        integer :: rc
+        rc = mgpu_thomasz(to_size3d(Size), to_size3d(WorkSize), c_loc(D), c_loc(E), c_loc(F), &
+                          c_loc(TI), c_loc(WaterPoints), c_loc(RES), 0_c_int32_t, c_null_ptr)
+        if (present(STAT)) STAT = rc
+    end subroutine THOMASZ_GPU
+
+end module ModuleAdvectionDiffusionGPU

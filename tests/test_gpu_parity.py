@@ -133,13 +133,15 @@ def test_advection_diffusion_options(variant):
     gpu.kill()
 
 
-def test_batched_transport_matches_oracle():
+@pytest.mark.parametrize("per_launch", [0, 2])
+def test_batched_transport_matches_oracle(per_launch):
     """Device-resident multi-tracer transport (hipGraph of 5 steps + 2 eager
-    steps) against the oracle's per-tracer Optimize call pattern."""
+    steps) against the oracle's per-tracer Optimize call pattern; with 2
+    tracers per launch the 3 tracers run in two chunks."""
     _, Transport, _ = _ad()
     case = make("tvd_ragged_opt")
     T = len(case.tracers)
-    tr = Transport(case.size, case.worksize, case.geom, T)
+    tr = Transport(case.size, case.worksize, case.geom, T, tracers_per_launch=per_launch)
     tr.set_hydro(case.hydro, case.params)
     for t, p in enumerate(case.tracers):
         tr.set_tracer(t, p)

@@ -1,7 +1,7 @@
 """Benchmark of the MI355X tracer-transport step.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3]
-                    [--precision fp64|mixed] [--no-graph] [--no-cpu]
+                    [--precision fp64|fast] [--no-graph] [--no-cpu]
 
 Workload (default): 8 tracers on a 1024 x 1024 x 40 Cartesian grid with ~30 %
 land and per-column KFloor uniform in 5..40, TVD-SuperBee advection in all
@@ -71,7 +71,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c3")
-    ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed"))
+    ap.add_argument("--precision", default="fp64", choices=("fp64", "fast"))
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-window", type=int, default=384)
@@ -106,7 +106,7 @@ def main():
     _log(f"case {cfg.name}: {cfg.ni}x{cfg.nj}x{cfg.nk}, {T} tracers, {nwet} water cells "
          f"({time.time() - t0:.1f}s to build)")
 
-    prec = _lib.MGPU_PREC_MIXED if args.precision == "mixed" else _lib.MGPU_PREC_FP64
+    prec = _lib.MGPU_PREC_FAST if args.precision == "fast" else _lib.MGPU_PREC_FP64
     tile = tuple(int(x) for x in args.tile.split(","))
     tr = Transport(case.size, case.worksize, case.geom, T, precision=prec, tile=tile,
                    tracers_per_launch=args.tracers_per_launch)

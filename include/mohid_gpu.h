@@ -48,9 +48,10 @@ extern "C" {
 #define MGPU_ERR_STATE 105       /* Optimize call without a FirstProperty call this step */
 #define MGPU_ERR_NOMEM 106       /* device allocation failed */
 
-/* ---- precision of the batched transport -------------------------------- */
-#define MGPU_PREC_FP64 0         /* reference operation order, IEEE FP64 */
-#define MGPU_PREC_MIXED 1        /* FP32 sweeps + FP64 residual correction */
+/* ---- arithmetic of the batched transport -------------------------------- */
+#define MGPU_PREC_FP64 0         /* reference operation order, IEEE FP64 division */
+#define MGPU_PREC_FAST 1         /* FP64 with reciprocal-multiply division (rounding-level
+                                    differences from the reference's operation order) */
 
 /* T_Size3D (GD:2044-2052): ILB, IUB, JLB, JUB, KLB, KUB. */
 typedef struct {
@@ -139,7 +140,7 @@ typedef struct mgpu_transport mgpu_transport;
 
 typedef struct {
     int32_t ntracers;
-    int32_t precision;                    /* MGPU_PREC_FP64 / MGPU_PREC_MIXED */
+    int32_t precision;                    /* MGPU_PREC_FP64 / MGPU_PREC_FAST */
     int32_t docycle_method;
     int32_t tile_i, tile_j;               /* column-solve block shape; 0: 64 x 4 */
     int32_t tracers_per_launch;           /* 1..8 tracers per kernel launch; 0: 8 */

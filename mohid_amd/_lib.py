@@ -27,7 +27,7 @@ MGPU_ERR_MASKS = 104
 MGPU_ERR_STATE = 105
 MGPU_ERR_NOMEM = 106
 MGPU_PREC_FP64 = 0
-MGPU_PREC_MIXED = 1
+MGPU_PREC_FAST = 1
 
 STATUS_NAMES = {
     0: "SUCCESS", 1: "OFF_ERR", -99: "UNKNOWN", 100: "ERR_HIP", 101: "ERR_BADARG",

@@ -142,12 +142,16 @@ struct DevDerived {
     uint8_t *mask = nullptr;
     int32_t *kstart = nullptr;
     double *cu = nullptr, *cv = nullptr, *cw = nullptr;
+    uint8_t *cperm = nullptr;
+    uint16_t *ccnt = nullptr, *fperm = nullptr, *fcnt = nullptr;
     unsigned long long *counters = nullptr;  // [0] zero pivots, [1] bad-method flag, [2] mask flags
     int alloc(const Dims &d) {
-        const long n3 = d.n3();
+        const long n3 = d.n3(), nt = tile_count(d), nK = d.KUB - d.KLB + 1;
         int rc = 0;
         rc |= dalloc(&mask, n3); rc |= dalloc(&kstart, d.n2());
         rc |= dalloc(&cu, n3); rc |= dalloc(&cv, n3); rc |= dalloc(&cw, n3);
+        rc |= dalloc(&cperm, nt * 256); rc |= dalloc(&ccnt, nt * nK);
+        rc |= dalloc(&fperm, nt * 580); rc |= dalloc(&fcnt, nt * nK);
         rc |= dalloc(&counters, 4);
         if (rc) return MGPU_ERR_NOMEM;
         // cells the prep kernel never writes must still be finite
@@ -159,6 +163,15 @@ struct DevDerived {
     }
     void release() {
         dfree(mask); dfree(kstart); dfree(cu); dfree(cv); dfree(cw); dfree(counters);
+        dfree(cperm); dfree(ccnt); dfree(fperm); dfree(fcnt);
+    }
+    // masks -> mask bytes, kstart, tile orders (after the mask arrays are on the device)
+    int pack(const Dims &d, const DevHydro &h, const int32_t *kfloor, int docycle,
+             hipStream_t s) {
+        int rc = launch_pack_masks(d, h.open, h.land, h.cfu, h.cfv, h.cfw, h.water, kfloor,
+                                   docycle, mask, kstart, mask_flags(), s);
+        if (!rc) rc = launch_tile_sort(d, mask, kstart, cperm, ccnt, fperm, fcnt, s);
+        return rc;
     }
     unsigned int *mask_flags() { return reinterpret_cast<unsigned int *>(counters + 2); }
 };
@@ -208,6 +221,7 @@ SharedFields make_fields(const DevHydro &h, const DevDerived &dv, const DevGeome
     f.cu = dv.cu; f.cv = dv.cv; f.cw = dv.cw;
     f.DUX = g.DUX; f.DVY = g.DVY; f.DWZ = g.DWZ;
     f.mask = dv.mask; f.kstart = dv.kstart;
+    f.cperm = dv.cperm; f.ccnt = dv.ccnt; f.fperm = dv.fperm; f.fcnt = dv.fcnt;
     return f;
 }
 
@@ -338,9 +352,7 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     else HIPCHK(hipMemcpyAsync(ad->prop, prop, b3, hipMemcpyHostToDevice, s));
 
     HIPCHK(hipMemsetAsync(ad->der.counters, 0, sizeof(unsigned long long) * 4, s));
-    rc = launch_pack_masks(d, ad->hydro.open, ad->hydro.land, ad->hydro.cfu, ad->hydro.cfv,
-                           ad->hydro.cfw, ad->hydro.water, ad->geom.KFloor, ad->docycle,
-                           ad->der.mask, ad->der.kstart, ad->der.mask_flags(), s);
+    rc = ad->der.pack(d, ad->hydro, ad->geom.KFloor, ad->docycle, s);
     if (rc) return rc;
     if (first) {
         // Convert_Visc_Dif_Horizontal(_opt) + Convert_Dif_Vertical (+ the
@@ -500,7 +512,7 @@ extern "C" int mgpu_transport_create(const mgpu_size3d *size, const mgpu_size3d 
     int rc = check_dims(size, worksize);
     if (rc) return rc;
     if (opts->ntracers < 1 || opts->ntracers > 1024) return MGPU_ERR_BADARG;
-    if (opts->precision != MGPU_PREC_FP64 && opts->precision != MGPU_PREC_MIXED)
+    if (opts->precision != MGPU_PREC_FP64 && opts->precision != MGPU_PREC_FAST)
         return MGPU_ERR_BADARG;
     mgpu_transport *t = new (std::nothrow) mgpu_transport();
     if (!t) return MGPU_UNKNOWN;
@@ -552,9 +564,7 @@ extern "C" int mgpu_transport_set_hydro(mgpu_transport *t, const mgpu_ad_args *a
     rc = t->hydro.upload(t->d, a, t->stream);
     if (rc) return rc;
     HIPCHK(hipMemsetAsync(t->der.counters, 0, sizeof(unsigned long long) * 4, t->stream));
-    rc = launch_pack_masks(t->d, t->hydro.open, t->hydro.land, t->hydro.cfu, t->hydro.cfv,
-                           t->hydro.cfw, t->hydro.water, t->geom.KFloor, t->opts.docycle_method,
-                           t->der.mask, t->der.kstart, t->der.mask_flags(), t->stream);
+    rc = t->der.pack(t->d, t->hydro, t->geom.KFloor, t->opts.docycle_method, t->stream);
     if (rc) return rc;
     unsigned long long cnt[4];
     HIPCHK(hipMemcpyAsync(cnt, t->der.counters, sizeof(cnt), hipMemcpyDeviceToHost, t->stream));
@@ -635,8 +645,8 @@ static int enqueue_steps(mgpu_transport *t, int nsteps, int cur) {
         if ((rc = mark(t))) return rc;
         const int n = t->opts.ntracers;
         for (int ph = 1; ph <= 2; ++ph) {   // explicit kernel(s), then vertical kernel(s)
-            if (t->opts.precision == MGPU_PREC_MIXED)
-                rc = launch_transport_mixed(d, t->sc, f, n, n, t->buf[cur], t->scr,
+            if (t->opts.precision == MGPU_PREC_FAST)
+                rc = launch_transport_fast(d, t->sc, f, n, n, t->buf[cur], t->scr,
                                             t->buf[cur ^ 1], t->scr, t->der.counters, t->chunk,
                                             t->opts.tile_i, t->opts.tile_j, s, ph);
             else

@@ -72,6 +72,10 @@ struct SharedFields {     // device pointers shared by all tracers
     const double *DUX, *DVY, *DWZ;
     const uint8_t *mask;
     const int32_t *kstart;
+    // per 64 x 4 tile: columns sorted by kstart, faces by first computing
+    // level, and how many of each are active at every level (tile_sort)
+    const uint8_t *cperm;
+    const uint16_t *ccnt, *fperm, *fcnt;
 };
 
 // Kernel launchers (transport.hip)
@@ -79,6 +83,11 @@ int launch_pack_masks(const Dims &d, const int32_t *open, const int32_t *land,
                       const int32_t *cfu, const int32_t *cfv, const int32_t *cfw,
                       const int32_t *water, const int32_t *kfloor, int docycle,
                       uint8_t *mask, int32_t *kstart, unsigned int *flags, hipStream_t s);
+// Tile ordering for ragged columns; tile_count() tiles, 256 columns and
+// 580 faces each, counts over the Size k range.
+long tile_count(const Dims &d);
+int launch_tile_sort(const Dims &d, const uint8_t *mask, const int32_t *kstart, uint8_t *cperm,
+                     uint16_t *ccnt, uint16_t *fperm, uint16_t *fcnt, hipStream_t s);
 int launch_prep_faces(const Dims &d, const double *visc, const double *diffv,
                       const double *Qx, const double *Qy, const double *Qz,
                       const double *DUX, const double *DVY, const double *DZX,
@@ -96,10 +105,14 @@ int launch_transport_fp64(const Dims &d, const StepScalars &sc, const SharedFiel
                           int n, long stride, const double *P, double *TI, double *out,
                           double *W, unsigned long long *counters, int chunk, int tile_i,
                           int tile_j, hipStream_t s, int phase = 3);
-int launch_transport_mixed(const Dims &d, const StepScalars &sc, const SharedFields &f,
-                           int n, long stride, const double *P, double *TI, double *out,
-                           double *W, unsigned long long *counters, int chunk, int tile_i,
-                           int tile_j, hipStream_t s, int phase = 3);
+// Same transport with reciprocal-based division (fast mode): FP64 state and
+// arithmetic, divisions replaced by multiplications with reciprocals shared
+// across tracers where possible; results differ from the reference's order
+// by rounding only.
+int launch_transport_fast(const Dims &d, const StepScalars &sc, const SharedFields &f,
+                          int n, long stride, const double *P, double *TI, double *out,
+                          double *W, unsigned long long *counters, int chunk, int tile_i,
+                          int tile_j, hipStream_t s, int phase = 3);
 // Copy one tracer between a Fortran array (stride 1) and an interleaved
 // buffer (stride `stride`, slot t).  to_interleaved = 1: src Fortran -> dst.
 int launch_interleave(long n3, const double *src, double *dst, long stride, int t,

@@ -27,6 +27,17 @@
 
 namespace {
 
+// Reciprocal for the fast path: hardware estimate refined by two Newton
+// steps (relative error within a few ulp of 1/y).  y is never zero where it
+// is used (volumes, stencil spacings, clamped gradients, non-zero pivots).
+__device__ __forceinline__ double frcp(double y) {
+    double r = __builtin_amdgcn_rcp(y);
+    double e = fma(-y, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-y, r, 1.0);
+    return fma(r, e, r);
+}
+
 // ---------------------------------------------------------------------------
 // Masks.  A row k of a water column is "inert" when it is land and touches no
 // compute face: the reference gives it TI = null_real (AD:1753) and an
@@ -87,6 +98,90 @@ __global__ void kstart_kernel(Dims d, const uint8_t *__restrict__ mask,
 }
 
 // ---------------------------------------------------------------------------
+// Ragged columns.  With KFloor varying from column to column only part of a
+// tile's cells are active at a level.  Per 64 x 4 tile, the columns are sorted
+// by kstart and the tile's U/V faces by the first level at which they compute,
+// so that the active cells (faces) of level k are a prefix of the sorted list
+// and the kernels hand them to consecutive lanes (no idle lanes inside a wave,
+// waves of similar column depth in the column solve).
+constexpr int TX = 64, TY = 4, TC = TX * TY;       // tile of columns
+constexpr int TFU = (TY + 1) * TX;                 // U faces j0 .. j0+TY
+constexpr int TFV = TY * (TX + 1);                 // V faces i0 .. i0+TX
+constexpr int TF = TFU + TFV;
+constexpr uint8_t NEVER = 255;
+
+// face f of a tile: U faces r*TX+c (face between (i0+c, j0+r-1) and (i0+c, j0+r)),
+// then V faces TFU + r*(TX+1)+c (face between (i0+c-1, j0+r) and (i0+c, j0+r))
+__device__ __forceinline__ void face_pos(int f, int &fi, int &fj, bool &isU, int i0, int j0) {
+    isU = f < TFU;
+    if (isU) {
+        fi = i0 + f % TX;
+        fj = j0 + f / TX;
+    } else {
+        const int g = f - TFU;
+        fi = i0 + g % (TX + 1);
+        fj = j0 + g / (TX + 1);
+    }
+}
+
+__global__ void __launch_bounds__(TC)
+tile_sort_kernel(Dims d, const uint8_t *__restrict__ mask, const int32_t *__restrict__ kstart,
+                 uint8_t *__restrict__ cperm, uint16_t *__restrict__ ccnt,
+                 uint16_t *__restrict__ fperm, uint16_t *__restrict__ fcnt) {
+    __shared__ uint8_t ckey[TC];
+    __shared__ uint8_t fkey[TF];
+    const int tid = threadIdx.x;
+    const long tile = blockIdx.x + (long)gridDim.x * blockIdx.y;
+    const int i0 = d.wILB + blockIdx.x * TX, j0 = d.wJLB + blockIdx.y * TY;
+    const int nK = d.KUB - d.KLB + 1;
+    {
+        const int i = i0 + tid % TX, j = j0 + tid / TX;
+        uint8_t key = NEVER;
+        if (i <= d.wIUB && j <= d.wJUB && (mask[d.idx(i, j, d.wKUB)] & M_WATER))
+            key = (uint8_t)kstart[d.idx2(i, j)];
+        ckey[tid] = key;
+    }
+    for (int f = tid; f < TF; f += TC) {
+        int fi, fj;
+        bool isU;
+        face_pos(f, fi, fj, isU, i0, j0);
+        uint8_t key = NEVER;
+        if (fi <= d.wIUB && fj <= d.wJUB) {
+            const uint8_t bit = isU ? M_CFU : M_CFV;
+            for (int k = d.wKLB; k <= d.wKUB; ++k)
+                if (mask[d.idx(fi, fj, k)] & bit) { key = (uint8_t)k; break; }
+        }
+        fkey[f] = key;
+    }
+    __syncthreads();
+    {   // stable rank of the cell
+        const uint8_t key = ckey[tid];
+        int rank = 0;
+        for (int q = 0; q < TC; ++q) {
+            const uint8_t o = ckey[q];
+            rank += (o < key) || (o == key && q < tid);
+        }
+        cperm[tile * TC + rank] = (uint8_t)tid;
+    }
+    for (int f = tid; f < TF; f += TC) {
+        const uint8_t key = fkey[f];
+        int rank = 0;
+        for (int q = 0; q < TF; ++q) {
+            const uint8_t o = fkey[q];
+            rank += (o < key) || (o == key && q < f);
+        }
+        fperm[tile * TF + rank] = (uint16_t)f;
+    }
+    for (int k = tid; k < nK; k += TC) {
+        int nc = 0, nf = 0;
+        for (int q = 0; q < TC; ++q) nc += ckey[q] <= k;
+        for (int q = 0; q < TF; ++q) nf += fkey[q] <= k;
+        ccnt[tile * nK + k] = (uint16_t)nc;
+        fcnt[tile * nK + k] = (uint16_t)nf;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Face conductances.  cu(i,j,k) on the U face between (i,j-1) and (i,j):
 //   Kh = Schmidt_H*(nu_j*DUX_{j-1} + nu_{j-1}*DUX_j)/(DUX_j + DUX_{j-1})  (AD:2480-2494)
 //   cu = Kh*AreaU/DZX(i,j-1)                                              (AD:5180-5182, 1551)
@@ -143,9 +238,6 @@ __global__ void prep_faces_kernel(Dims d, const double *__restrict__ visc,
 //   4930-4940).
 // Tracer-independent parts of every face (Courant number, stencil spacing,
 // masks) are computed once for all tracers of the launch.
-constexpr int HX = 64, HY = 4;
-constexpr int NFU = (HY + 1) * HX;   // U faces j0 .. j0+HY of the tile
-constexpr int NFV = HY * (HX + 1);   // V faces i0 .. i0+HX of the tile
 constexpr int MAXT = 8;
 
 // The tracers of one launch: value of tracer t (0 <= t < NT) at cell index x
@@ -164,7 +256,7 @@ struct TracerSet {
 // Flux of one horizontal face for NT tracers.  `c0` indexes the cell on the
 // face's high side, `st` is the stride along the line, `pos` its line index
 // and `lo` the lowest valid line index (cells below lo are outside Size).
-template <bool OPT, int NT>
+template <bool OPT, int NT, bool FAST>
 __device__ __forceinline__ void hface_flux(const Dims &d, const StepScalars &sc,
                                            const SharedFields &f, const TracerSet &ts, long c0,
                                            long st, int pos, int lo, const double *du_line,
@@ -194,9 +286,11 @@ __device__ __forceinline__ void hface_flux(const Dims &d, const StepScalars &sc,
                 fl[t] = Dc * ts.P[cell[1] * ts.s + t] + Ec * ts.P[cell[2] * ts.s + t];
             return;
         }
-        const double cr = up ? Q * sc.dt / v[1] : Q * sc.dt / v[2];
+        const double vup = up ? v[1] : v[2];
+        const double cr = FAST ? Q * sc.dt * frcp(vup) : Q * sc.dt / vup;
         const double s1 = du[2] + du[1];
         const double s0 = up ? du[1] + du[0] : du[2] + du[3];
+        const double i1 = FAST ? frcp(s1) : 0.0, i0 = FAST ? frcp(s0) : 0.0;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const double *P = ts.P + t;
@@ -204,12 +298,22 @@ __device__ __forceinline__ void hface_flux(const Dims &d, const StepScalars &sc,
             double dc, rr;
             if (up) {
                 const double p0 = P[cell[0] * ts.s];
-                dc = clamp_min_value((p2 - p1) / s1);
-                rr = (p1 - p0) / s0 / dc;
+                if (FAST) {
+                    dc = clamp_min_value((p2 - p1) * i1);
+                    rr = (p1 - p0) * i0 * frcp(dc);
+                } else {
+                    dc = clamp_min_value((p2 - p1) / s1);
+                    rr = (p1 - p0) / s0 / dc;
+                }
             } else {
                 const double p3 = P[cell[3] * ts.s];
-                dc = clamp_min_value((p1 - p2) / s1);
-                rr = (p2 - p3) / s0 / dc;
+                if (FAST) {
+                    dc = clamp_min_value((p1 - p2) * i1);
+                    rr = (p2 - p3) * i0 * frcp(dc);
+                } else {
+                    dc = clamp_min_value((p1 - p2) / s1);
+                    rr = (p2 - p3) / s0 / dc;
+                }
             }
             const double th = 0.5 * superbee_theta(rr) * (1.0 - cr);
             const double Dc = up ? Q * (1.0 - th) : Q * th;
@@ -231,60 +335,43 @@ __device__ __forceinline__ void hface_flux(const Dims &d, const StepScalars &sc,
     }
 }
 
-template <bool OPT, int NT>
-__global__ void __launch_bounds__(HX * HY)
+template <bool OPT, int NT, bool FAST>
+__global__ void __launch_bounds__(TC)
 explicit_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
                 unsigned int *__restrict__ badflag) {
-    __shared__ double FU[NT][HY + 1][HX];
-    __shared__ double FV[NT][HY][HX + 1];
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * HX + tx;
-    const int i0 = d.wILB + blockIdx.x * HX, j0 = d.wJLB + blockIdx.y * HY;
+    __shared__ double FU[NT][TY + 1][TX];
+    __shared__ double FV[NT][TY][TX + 1];
+    const int tid = threadIdx.x;
+    const long tile = blockIdx.x + (long)gridDim.x * blockIdx.y;
+    const int i0 = d.wILB + blockIdx.x * TX, j0 = d.wJLB + blockIdx.y * TY;
     const int k = d.wKLB + blockIdx.z;
-    const int i = i0 + tx, j = j0 + ty;
+    const int nK = d.KUB - d.KLB + 1;
+    const int nc = f.ccnt[tile * nK + k];    // active cells of this level (sorted prefix)
+    if (nc == 0) return;
     const double dt = sc.dt;
     int bad = 0;
-    bool active = false;
-    long x = 0;
-    if (i <= d.wIUB && j <= d.wJUB) {
-        x = d.idx(i, j, k);
-        active = (f.mask[d.idx(i, j, d.wKUB)] & M_WATER) && k >= f.kstart[d.idx2(i, j)];
-    }
-    if (!__syncthreads_or(active)) return;
 
     if (!sc.vertical1d) {
-        for (int q = tid; q < NFU + NFV; q += HX * HY) {
+        const int nf = f.fcnt[tile * nK + k];
+        for (int q = tid; q < nf; q += TC) {
+            int fi, fj;
+            bool isU;
+            const int fid = f.fperm[tile * TF + q];
+            face_pos(fid, fi, fj, isU, i0, j0);
+            const long y = d.idx(fi, fj, k);
             double fl[NT];
-            int r, c;
-            bool isU = q < NFU;
             if (isU) {
-                r = q / HX; c = q % HX;
-                const int fi = i0 + c, fj = j0 + r;   // face between (fi, fj-1) and (fi, fj)
-                bool on = fi <= d.wIUB && fj <= d.wJUB;
-                const long y = on ? d.idx(fi, fj, k) : 0;
-                on = on && (f.mask[y] & M_CFU);
-                if (on)
-                    hface_flux<OPT, NT>(d, sc, f, ts, y, d.sj, fj, d.JLB, f.DUX + d.idx2(fi, fj),
+                if (f.mask[y] & M_CFU) {
+                    hface_flux<OPT, NT, FAST>(d, sc, f, ts, y, d.sj, fj, d.JLB, f.DUX + d.idx2(fi, fj),
                                         d.sj, f.Qx[y], fl, bad);
-                else {
+                    const int r = fid / TX, c = fid % TX;
 #pragma unroll
-                    for (int t = 0; t < NT; ++t) fl[t] = 0.0;
+                    for (int t = 0; t < NT; ++t) FU[t][r][c] = fl[t];
                 }
-#pragma unroll
-                for (int t = 0; t < NT; ++t) FU[t][r][c] = fl[t];
-            } else {
-                const int q2 = q - NFU;
-                r = q2 / (HX + 1); c = q2 % (HX + 1);
-                const int fi = i0 + c, fj = j0 + r;   // face between (fi-1, fj) and (fi, fj)
-                bool on = !sc.xzflow && fi <= d.wIUB && fj <= d.wJUB;
-                const long y = on ? d.idx(fi, fj, k) : 0;
-                on = on && (f.mask[y] & M_CFV);
-                if (on)
-                    hface_flux<OPT, NT>(d, sc, f, ts, y, 1, fi, d.ILB, f.DVY + d.idx2(fi, fj), 1,
-                                        f.Qy[y], fl, bad);
-                else {
-#pragma unroll
-                    for (int t = 0; t < NT; ++t) fl[t] = 0.0;
-                }
+            } else if (!sc.xzflow && (f.mask[y] & M_CFV)) {
+                hface_flux<OPT, NT, FAST>(d, sc, f, ts, y, 1, fi, d.ILB, f.DVY + d.idx2(fi, fj), 1,
+                                    f.Qy[y], fl, bad);
+                const int g = fid - TFU, r = g / (TX + 1), c = g % (TX + 1);
 #pragma unroll
                 for (int t = 0; t < NT; ++t) FV[t][r][c] = fl[t];
             }
@@ -292,22 +379,29 @@ explicit_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
         __syncthreads();
     }
     if (bad) atomicOr(badflag, 1u);
-    if (!active) return;
+    if (tid >= nc) return;
 
+    const int slot = f.cperm[tile * TC + tid];
+    const int tx = slot % TX, ty = slot / TX;
+    const long x = d.idx(i0 + tx, j0 + ty, k);
     const uint8_t mk = f.mask[x];
     const double vk = f.V[x];
     const bool open = mk & M_OPEN;
-    const double vv = open ? f.Vold[x] / vk : 0.0;
+    const double ivk = FAST ? frcp(vk) : 0.0;
+    const double dtv = dt * ivk;   // fast path: dt/V(k) once per cell
+    const double vv = open ? (FAST ? f.Vold[x] * ivk : f.Vold[x] / vk) : 0.0;
     const bool hz = !sc.vertical1d;
+    // a face flag set here is a face computed above (its sort key is <= k)
     const bool cfu0 = hz && (mk & M_CFU);
     const bool cfu1 = hz && (f.mask[x + d.sj] & M_CFU);
     const bool cfv0 = hz && !sc.xzflow && (mk & M_CFV);
     const bool cfv1 = hz && !sc.xzflow && (f.mask[x + 1] & M_CFV);
     const double au0 = cfu0 ? f.cu[x] : 0.0, au1 = cfu1 ? f.cu[x + d.sj] : 0.0;
     const double av0 = cfv0 ? f.cv[x] : 0.0, av1 = cfv1 ? f.cv[x + 1] : 0.0;
-    // tracer-independent diffusion weights of the non-optimised path
-    const double wu0 = au0 * dt / vk, wu1 = au1 * dt / vk;
-    const double wv0 = av0 * dt / vk, wv1 = av1 * dt / vk;
+    // tracer-independent diffusion weights of the non-optimised path (and of
+    // every path in the fast mode)
+    const double wu0 = FAST ? au0 * dtv : au0 * dt / vk, wu1 = FAST ? au1 * dtv : au1 * dt / vk;
+    const double wv0 = FAST ? av0 * dtv : av0 * dt / vk, wv1 = FAST ? av1 * dtv : av1 * dt / vk;
     // the optimised YY2 form multiplies the conductance by dt first
     const double bv0 = av0 * dt, bv1 = av1 * dt;
     const long s = ts.s;
@@ -316,26 +410,34 @@ explicit_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
         const double *P = ts.P + t;
         const double pk = P[x * s];
         double TI = open ? pk * vv : pk;
+        const bool ex = OPT && !FAST;   // the optimised path's own operation order
         if (cfu0) {
             const double g = pk - P[(x - d.sj) * s];
-            TI = OPT ? TI - au0 * g * dt / vk : TI - wu0 * g;
+            TI = ex ? TI - au0 * g * dt / vk : TI - wu0 * g;
         }
         if (cfu1) {
             const double g = P[(x + d.sj) * s] - pk;
-            TI = OPT ? TI + au1 * g * dt / vk : TI + wu1 * g;
+            TI = ex ? TI + au1 * g * dt / vk : TI + wu1 * g;
         }
         if (cfv0) {
             const double g = pk - P[(x - 1) * s];
-            TI = OPT ? TI - bv0 * g / vk : TI - wv0 * g;
+            TI = ex ? TI - bv0 * g / vk : TI - wv0 * g;
         }
         if (cfv1) {
             const double g = P[(x + 1) * s] - pk;
-            TI = OPT ? TI + bv1 * g / vk : TI + wv1 * g;
+            TI = ex ? TI + bv1 * g / vk : TI + wv1 * g;
         }
-        if (cfu0) TI = TI + FU[t][ty][tx] * dt / vk;
-        if (cfu1) TI = TI - FU[t][ty + 1][tx] * dt / vk;
-        if (cfv0) TI = TI + FV[t][ty][tx] * dt / vk;
-        if (cfv1) TI = TI - FV[t][ty][tx + 1] * dt / vk;
+        if (FAST) {
+            if (cfu0) TI = TI + FU[t][ty][tx] * dtv;
+            if (cfu1) TI = TI - FU[t][ty + 1][tx] * dtv;
+            if (cfv0) TI = TI + FV[t][ty][tx] * dtv;
+            if (cfv1) TI = TI - FV[t][ty][tx + 1] * dtv;
+        } else {
+            if (cfu0) TI = TI + FU[t][ty][tx] * dt / vk;
+            if (cfu1) TI = TI - FU[t][ty + 1][tx] * dt / vk;
+            if (cfv0) TI = TI + FV[t][ty][tx] * dt / vk;
+            if (cfv1) TI = TI - FV[t][ty][tx + 1] * dt / vk;
+        }
         ts.TI[x * s + t] = TI;
     }
 }
@@ -356,7 +458,7 @@ struct VCoef {
     double c, d, e, f;
 };
 
-template <bool OPT, int NT>
+template <bool OPT, int NT, bool FAST>
 __device__ __forceinline__ void vface_coefs(const StepScalars &sc, const double (&p)[NT][4],
                                             const double v[4], const double du[4],
                                             const bool op[4], double Q, bool build,
@@ -375,13 +477,23 @@ __device__ __forceinline__ void vface_coefs(const StepScalars &sc, const double 
             }
             return;
         }
-        const double cr = up ? Q * sc.dt / v[1] : Q * sc.dt / v[2];
+        const double vup = up ? v[1] : v[2];
+        const double cr = FAST ? Q * sc.dt * frcp(vup) : Q * sc.dt / vup;
         const double s1 = du[2] + du[1];
         const double s0 = up ? du[1] + du[0] : du[2] + du[3];
+        const double i1 = FAST ? frcp(s1) : 0.0, i0 = FAST ? frcp(s0) : 0.0;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             double dc, rr;
-            if (up) {
+            if (FAST) {
+                if (up) {
+                    dc = clamp_min_value((p[t][2] - p[t][1]) * i1);
+                    rr = (p[t][1] - p[t][0]) * i0 * frcp(dc);
+                } else {
+                    dc = clamp_min_value((p[t][1] - p[t][2]) * i1);
+                    rr = (p[t][2] - p[t][3]) * i0 * frcp(dc);
+                }
+            } else if (up) {
                 dc = clamp_min_value((p[t][2] - p[t][1]) / s1);
                 rr = (p[t][1] - p[t][0]) / s0 / dc;
             } else {
@@ -402,16 +514,20 @@ __device__ __forceinline__ void vface_coefs(const StepScalars &sc, const double 
     }
 }
 
-template <bool OPT, int NT>
-__global__ void __launch_bounds__(256)
+template <bool OPT, int NT, bool FAST>
+__global__ void __launch_bounds__(TC)
 vertical_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
                 unsigned long long *__restrict__ zero_pivots, unsigned int *__restrict__ badflag) {
-    const int i = d.wILB + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = d.wJLB + blockIdx.y * blockDim.y + threadIdx.y;
-    if (i > d.wIUB || j > d.wJUB) return;
+    const long tile = blockIdx.x + (long)gridDim.x * blockIdx.y;
+    const int nK = d.KUB - d.KLB + 1;
     const int KUB = d.wKUB;
-    // THOMASZ_NewType2 solves columns with WaterPoints3D(i,j,KUB) == 1 (MF:4086)
-    if (!(f.mask[d.idx(i, j, KUB)] & M_WATER)) return;
+    // THOMASZ_NewType2 solves columns with WaterPoints3D(i,j,KUB) == 1 (MF:4086):
+    // those are the columns whose sort key is <= KUB+1; lanes take them in
+    // order of kstart
+    if ((int)threadIdx.x >= f.ccnt[tile * nK + KUB + 1]) return;
+    const int slot = f.cperm[tile * TC + threadIdx.x];
+    const int i = d.wILB + blockIdx.x * TX + slot % TX;
+    const int j = d.wJLB + blockIdx.y * TY + slot / TX;
     const int ks = f.kstart[d.idx2(i, j)];
     const double dt = sc.dt;
     const long sk = d.sk;
@@ -458,7 +574,7 @@ vertical_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
         const uint8_t m0 = in ? f.mask[y] : (uint8_t)0;
         const bool o4[4] = {(m0 & M_OPEN) != 0, (wm[0] & M_OPEN) != 0, (wm[1] & M_OPEN) != 0,
                             (wm[2] & M_OPEN) != 0};
-        vface_coefs<OPT, NT>(sc, p, v4, du4, o4, inK(ks) ? f.Qz[base + ks * sk] : 0.0,
+        vface_coefs<OPT, NT, FAST>(sc, p, v4, du4, o4, inK(ks) ? f.Qz[base + ks * sk] : 0.0,
                              vflux && ks <= KUB, fb, bad);
     }
 
@@ -474,20 +590,17 @@ vertical_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
         {
             const bool o4[4] = {(wm[0] & M_OPEN) != 0, (wm[1] & M_OPEN) != 0,
                                 (wm[2] & M_OPEN) != 0, (wm[3] & M_OPEN) != 0};
-            vface_coefs<OPT, NT>(sc, wp, wv, wdu, o4, inK(k + 1) ? f.Qz[x + sk] : 0.0,
+            vface_coefs<OPT, NT, FAST>(sc, wp, wv, wdu, o4, inK(k + 1) ? f.Qz[x + sk] : 0.0,
                                  vflux && k < KUB, ft, bad);
         }
         // row values shared by all tracers
+        const double dtk = FAST ? dt * frcp(vk) : dt / vk;
         double E0 = 1.0;
-        if (k == KUB && (mk & M_OPEN)) {
-            const double dtv = dt / vk;
-            E0 = 1.0 + dtv * f.Qz[x + sk];
-        }
+        if (k == KUB && (mk & M_OPEN)) E0 = 1.0 + dtk * f.Qz[x + sk];
         const bool cfwB = sc.vert && (mk & M_CFW);
         const bool cfwT = sc.vert && k < KUB && (wm[2] & M_CFW);
-        const double a2 = cfwB ? f.cw[x] * dt / vk : 0.0;
-        const double a1 = cfwT ? f.cw[x + sk] * dt / vk : 0.0;
-        const double dtk = dt / vk;
+        const double a2 = cfwB ? (FAST ? f.cw[x] * dtk : f.cw[x] * dt / vk) : 0.0;
+        const double a1 = cfwT ? (FAST ? f.cw[x + sk] * dtk : f.cw[x + sk] * dt / vk) : 0.0;
         const bool land = mk & M_LAND;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
@@ -519,8 +632,13 @@ vertical_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
                     if (cfwB) { D = D - fb[t].d * dtk; E = E - fb[t].e * dtk; }
                     if (cfwT) { E = E + ft[t].d * dtk; F = F + ft[t].e * dtk; }
                 } else if (OPT) {
-                    if (cfwB) TI = TI + (fb[t].d * pm + fb[t].e * pk) * dt / vk;
-                    if (cfwT) TI = TI - (ft[t].d * pk + ft[t].e * pp) * dt / vk;
+                    if (FAST) {
+                        if (cfwB) TI = TI + (fb[t].d * pm + fb[t].e * pk) * dtk;
+                        if (cfwT) TI = TI - (ft[t].d * pk + ft[t].e * pp) * dtk;
+                    } else {
+                        if (cfwB) TI = TI + (fb[t].d * pm + fb[t].e * pk) * dt / vk;
+                        if (cfwT) TI = TI - (ft[t].d * pk + ft[t].e * pp) * dt / vk;
+                    }
                 } else {
                     if (cfwB) {
                         const double pmm =
@@ -541,8 +659,14 @@ vertical_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
             const double aux = E + D * Wm[t];
             double W = 0.0, G = 0.0;
             if (fabs(aux) > 0.0) {
-                W = -F / aux;
-                G = (TI - D * Gm[t]) / aux;
+                if (FAST) {
+                    const double ia = frcp(aux);
+                    W = -F * ia;
+                    G = (TI - D * Gm[t]) * ia;
+                } else {
+                    W = -F / aux;
+                    G = (TI - D * Gm[t]) / aux;
+                }
             } else {
                 atomicAdd(zero_pivots, 1ull);
             }
@@ -650,6 +774,22 @@ int launch_pack_masks(const Dims &d, const int32_t *open, const int32_t *land,
     return MGPU_SUCCESS;
 }
 
+long tile_count(const Dims &d) {
+    const long ni = d.wIUB - d.wILB + 1, nj = d.wJUB - d.wJLB + 1;
+    return ((ni + TX - 1) / TX) * ((nj + TY - 1) / TY);
+}
+
+int launch_tile_sort(const Dims &d, const uint8_t *mask, const int32_t *kstart, uint8_t *cperm,
+                     uint16_t *ccnt, uint16_t *fperm, uint16_t *fcnt, hipStream_t s) {
+    if (d.wKUB + 1 >= NEVER) return MGPU_ERR_BADARG;   // level keys are bytes
+    const int ni = d.wIUB - d.wILB + 1, nj = d.wJUB - d.wJLB + 1;
+    const dim3 tiles((ni + TX - 1) / TX, (nj + TY - 1) / TY);
+    hipLaunchKernelGGL(tile_sort_kernel, tiles, dim3(TC), 0, s, d, mask, kstart, cperm, ccnt,
+                       fperm, fcnt);
+    HIPCHK(hipGetLastError());
+    return MGPU_SUCCESS;
+}
+
 int launch_prep_faces(const Dims &d, const double *visc, const double *diffv, const double *Qx,
                       const double *Qy, const double *Qz, const double *DUX, const double *DVY,
                       const double *DZX, const double *DZY, const double *AreaU,
@@ -667,61 +807,81 @@ int launch_prep_faces(const Dims &d, const double *visc, const double *diffv, co
 
 // Launch kernels H and V for `n` tracers (chunks of up to MAXT per launch).
 // phase bit 1: kernel H, bit 2: kernel V
-template <bool OPT, int NT>
+template <bool OPT, int NT, bool FAST>
 static int launch_chunk(int phase, const Dims &d, const StepScalars &sc, const SharedFields &f,
                         const TracerSet &ts, unsigned long long *counters, int tile_i,
                         int tile_j, hipStream_t s) {
     const int ni = d.wIUB - d.wILB + 1, nj = d.wJUB - d.wJLB + 1, nk = d.wKUB - d.wKLB + 1;
     unsigned int *bad = reinterpret_cast<unsigned int *>(counters + 1);
+    (void)tile_i;
+    (void)tile_j;
+    const dim3 tiles((ni + TX - 1) / TX, (nj + TY - 1) / TY);
     if (phase & 1) {
-        dim3 bh(HX, HY), gh((ni + HX - 1) / HX, (nj + HY - 1) / HY, nk);
-        hipLaunchKernelGGL((explicit_kernel<OPT, NT>), gh, bh, 0, s, d, sc, f, ts, bad);
+        hipLaunchKernelGGL((explicit_kernel<OPT, NT, FAST>), dim3(tiles.x, tiles.y, nk), dim3(TC), 0,
+                           s, d, sc, f, ts, bad);
         HIPCHK(hipGetLastError());
     }
     if (phase & 2) {
-        dim3 bv(tile_i, tile_j), gv((ni + tile_i - 1) / tile_i, (nj + tile_j - 1) / tile_j);
-        hipLaunchKernelGGL((vertical_kernel<OPT, NT>), gv, bv, 0, s, d, sc, f, ts, counters,
-                           bad);
+        hipLaunchKernelGGL((vertical_kernel<OPT, NT, FAST>), tiles, dim3(TC), 0, s, d, sc, f, ts,
+                           counters, bad);
         HIPCHK(hipGetLastError());
     }
     return MGPU_SUCCESS;
 }
 
-template <bool OPT>
+template <bool OPT, bool FAST>
 static int launch_n(int nt, int ph, const Dims &d, const StepScalars &sc, const SharedFields &f,
                     const TracerSet &ts, unsigned long long *c, int ti, int tj, hipStream_t s) {
     switch (nt) {
-        case 1: return launch_chunk<OPT, 1>(ph, d, sc, f, ts, c, ti, tj, s);
-        case 2: return launch_chunk<OPT, 2>(ph, d, sc, f, ts, c, ti, tj, s);
-        case 3: return launch_chunk<OPT, 3>(ph, d, sc, f, ts, c, ti, tj, s);
-        case 4: return launch_chunk<OPT, 4>(ph, d, sc, f, ts, c, ti, tj, s);
-        case 5: return launch_chunk<OPT, 5>(ph, d, sc, f, ts, c, ti, tj, s);
-        case 6: return launch_chunk<OPT, 6>(ph, d, sc, f, ts, c, ti, tj, s);
-        case 7: return launch_chunk<OPT, 7>(ph, d, sc, f, ts, c, ti, tj, s);
-        case 8: return launch_chunk<OPT, 8>(ph, d, sc, f, ts, c, ti, tj, s);
+        case 1: return launch_chunk<OPT, 1, FAST>(ph, d, sc, f, ts, c, ti, tj, s);
+        case 2: return launch_chunk<OPT, 2, FAST>(ph, d, sc, f, ts, c, ti, tj, s);
+        case 3: return launch_chunk<OPT, 3, FAST>(ph, d, sc, f, ts, c, ti, tj, s);
+        case 4: return launch_chunk<OPT, 4, FAST>(ph, d, sc, f, ts, c, ti, tj, s);
+        case 5: return launch_chunk<OPT, 5, FAST>(ph, d, sc, f, ts, c, ti, tj, s);
+        case 6: return launch_chunk<OPT, 6, FAST>(ph, d, sc, f, ts, c, ti, tj, s);
+        case 7: return launch_chunk<OPT, 7, FAST>(ph, d, sc, f, ts, c, ti, tj, s);
+        case 8: return launch_chunk<OPT, 8, FAST>(ph, d, sc, f, ts, c, ti, tj, s);
         default: return MGPU_ERR_BADARG;
     }
 }
 
-int launch_transport_fp64(const Dims &d, const StepScalars &sc, const SharedFields &f,
-                          int n, long stride, const double *P, double *TI, double *out,
-                          double *W, unsigned long long *counters, int chunk, int tile_i,
-                          int tile_j, hipStream_t s, int phase) {
-    if (tile_i <= 0) tile_i = 64;
-    if (tile_j <= 0) tile_j = 4;
-    if (tile_i * tile_j > 256 || tile_i * tile_j < 1) return MGPU_ERR_BADARG;
+static int launch_transport(bool fast, const Dims &d, const StepScalars &sc,
+                            const SharedFields &f, int n, long stride, const double *P,
+                            double *TI, double *out, double *W, unsigned long long *counters,
+                            int chunk, int tile_i, int tile_j, hipStream_t s, int phase) {
     if (chunk <= 0 || chunk > MAXT) chunk = MAXT;
     if (stride < n) return MGPU_ERR_BADARG;
     for (int t0 = 0; t0 < n; t0 += chunk) {
         const int nt = n - t0 < chunk ? n - t0 : chunk;
         TracerSet ts;
         ts.P = P + t0; ts.TI = TI + t0; ts.out = out + t0; ts.W = W + t0; ts.s = stride;
-        const int rc =
-            sc.optimize ? launch_n<true>(nt, phase, d, sc, f, ts, counters, tile_i, tile_j, s)
-                        : launch_n<false>(nt, phase, d, sc, f, ts, counters, tile_i, tile_j, s);
+        int rc;
+        if (sc.optimize)
+            rc = fast ? launch_n<true, true>(nt, phase, d, sc, f, ts, counters, tile_i, tile_j, s)
+                      : launch_n<true, false>(nt, phase, d, sc, f, ts, counters, tile_i, tile_j, s);
+        else
+            rc = fast ? launch_n<false, true>(nt, phase, d, sc, f, ts, counters, tile_i, tile_j, s)
+                      : launch_n<false, false>(nt, phase, d, sc, f, ts, counters, tile_i, tile_j,
+                                               s);
         if (rc) return rc;
     }
     return MGPU_SUCCESS;
+}
+
+int launch_transport_fp64(const Dims &d, const StepScalars &sc, const SharedFields &f, int n,
+                          long stride, const double *P, double *TI, double *out, double *W,
+                          unsigned long long *counters, int chunk, int tile_i, int tile_j,
+                          hipStream_t s, int phase) {
+    return launch_transport(false, d, sc, f, n, stride, P, TI, out, W, counters, chunk, tile_i,
+                            tile_j, s, phase);
+}
+
+int launch_transport_fast(const Dims &d, const StepScalars &sc, const SharedFields &f, int n,
+                          long stride, const double *P, double *TI, double *out, double *W,
+                          unsigned long long *counters, int chunk, int tile_i, int tile_j,
+                          hipStream_t s, int phase) {
+    return launch_transport(true, d, sc, f, n, stride, P, TI, out, W, counters, chunk, tile_i,
+                            tile_j, s, phase);
 }
 
 int launch_thomasz(const Dims &d, const double *D, const double *E, const double *F,

@@ -103,7 +103,8 @@ module ModuleAdvectionDiffusionGPU
 
     ! instance table: integer IDs as MOHID's modules hand out (ModuleGlobalData
     ! RegisterNewInstance), each holding one library handle
-    type(c_ptr), dimension(64), save :: Handles = c_null_ptr
+    integer, parameter :: MAX_INSTANCES = 64
+    type(c_ptr), dimension(MAX_INSTANCES), save :: Handles = c_null_ptr
 
 contains
 
@@ -130,7 +131,7 @@ contains
         integer, dimension(6),             intent(in)  :: Size, WorkSize
         real(8), dimension(:,:),   target, intent(in)  :: DUX, DVY, DZX, DZY
         integer, dimension(:,:),   target, intent(in)  :: KFloorZ
-        real(8), dimension(:,:,:), target, intent(in)  :: AreaU, AreaV, DZZ, DWZ
+        real(8), dimension(:,:,:), contiguous, target, intent(in)  :: AreaU, AreaV, DZZ, DWZ
         integer,                           intent(in)  :: Docycle_method
         logical, optional,                 intent(in)  :: Vertical1D, XZFlow
         integer, optional,                 intent(out) :: STAT
@@ -150,7 +151,7 @@ contains
         ID = 0
         rc = mgpu_ad_create(s, w, int(Docycle_method, c_int32_t), b2i(v1d), b2i(xz), g, h)
         if (rc == MGPU_SUCCESS) then
-            do n = 1, size(Handles)
+            do n = 1, MAX_INSTANCES
                 if (.not. c_associated(Handles(n))) then
                     Handles(n) = h
                     ID = n
@@ -180,7 +181,7 @@ contains
                                      WaterPoints3D, CellFluxes, Discharges, Optimize,         &
                                      FirstProperty, STAT)
         integer,                           intent(in)    :: ID
-        real(8), dimension(:,:,:), target, intent(inout) :: PROP
+        real(8), dimension(:,:,:), contiguous, target, intent(inout) :: PROP
         real(8),                           intent(in)    :: schmidt_H, SchmidtCoef_V
         real(8),                           intent(in)    :: SchmidtBackground_V
         integer,                           intent(in)    :: AdvMethodH, TVDLimitationH
@@ -194,20 +195,20 @@ contains
         real(8),                           intent(in)    :: ImpExp_AdvXX, ImpExp_AdvYY
         real(8),                           intent(in)    :: ImpExp_DifH
         logical,                           intent(in)    :: NullDif
-        real(8), dimension(:,:,:), target, intent(in)    :: Wflux_X, Wflux_Y, Wflux_Z
-        real(8), dimension(:,:,:), target, intent(in)    :: VolumeZOld, VolumeZ
-        integer, dimension(:,:,:), target, intent(in)    :: OpenPoints3D, LandPoints3D
-        integer, dimension(:,:,:), target, intent(in)    :: ComputeFacesU3D, ComputeFacesV3D
-        integer, dimension(:,:,:), target, intent(in)    :: ComputeFacesW3D
-        real(8), dimension(:,:,:), target, intent(in)    :: Visc_H, Diff_V
-        integer, dimension(:,:,:), target, intent(in)    :: WaterPoints3D
+        real(8), dimension(:,:,:), contiguous, target, intent(in)    :: Wflux_X, Wflux_Y, Wflux_Z
+        real(8), dimension(:,:,:), contiguous, target, intent(in)    :: VolumeZOld, VolumeZ
+        integer, dimension(:,:,:), contiguous, target, intent(in)    :: OpenPoints3D, LandPoints3D
+        integer, dimension(:,:,:), contiguous, target, intent(in)    :: ComputeFacesU3D, ComputeFacesV3D
+        integer, dimension(:,:,:), contiguous, target, intent(in)    :: ComputeFacesW3D
+        real(8), dimension(:,:,:), contiguous, target, intent(in)    :: Visc_H, Diff_V
+        integer, dimension(:,:,:), contiguous, target, intent(in)    :: WaterPoints3D
         logical, optional,                 intent(in)    :: CellFluxes, Discharges
         logical, optional,                 intent(in)    :: Optimize, FirstProperty
         integer, optional,                 intent(out)   :: STAT
         type(t_mgpu_ad_args) :: a
         integer              :: rc
 
-        if (ID < 1 .or. ID > size(Handles)) then
+        if (ID < 1 .or. ID > MAX_INSTANCES) then
             if (present(STAT)) STAT = MGPU_OFF_ERR
             return
         endif
@@ -255,7 +256,7 @@ contains
         integer, optional, intent(out)   :: STAT
         integer :: rc
         rc = MGPU_OFF_ERR
-        if (ID >= 1 .and. ID <= size(Handles)) then
+        if (ID >= 1 .and. ID <= MAX_INSTANCES) then
             if (c_associated(Handles(ID))) then
                 rc = mgpu_ad_destroy(Handles(ID))
                 Handles(ID) = c_null_ptr
@@ -269,9 +270,9 @@ contains
     !> D/E/F/TI arrays (Size-shaped), writing the water columns of RES.
     subroutine THOMASZ_GPU(Size, WorkSize, D, E, F, TI, WaterPoints, RES, STAT)
         integer, dimension(6),             intent(in)    :: Size, WorkSize
-        real(8), dimension(:,:,:), target, intent(in)    :: D, E, F, TI
-        integer, dimension(:,:,:), target, intent(in)    :: WaterPoints
-        real(8), dimension(:,:,:), target, intent(inout) :: RES
+        real(8), dimension(:,:,:), contiguous, target, intent(in)    :: D, E, F, TI
+        integer, dimension(:,:,:), contiguous, target, intent(in)    :: WaterPoints
+        real(8), dimension(:,:,:), contiguous, target, intent(inout) :: RES
         integer, optional,                 intent(out)   :: STAT
         integer :: rc
         rc = mgpu_thomasz(to_size3d(Size), to_size3d(WorkSize), c_loc(D), c_loc(E), c_loc(F), &

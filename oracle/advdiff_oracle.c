@@ -73,6 +73,11 @@ struct ora_instance {
 #define I3(me, i, j, k) (((long)(i) - (me)->size.ILB) + (me)->nI * (((long)(j) - (me)->size.JLB) + (me)->nJ * ((long)(k) - (me)->size.KLB)))
 #define I2(me, i, j) (((long)(i) - (me)->size.ILB) + (me)->nI * ((long)(j) - (me)->size.JLB))
 
+/* Planted mistakes (0 = none): the invariant and closed-form tests switch
+ * one on to show that each of them is caught.  Never set outside tests. */
+static int g_mistake = 0;
+void ora_plant_mistake(int id) { g_mistake = id; }
+
 static double fmin2(double a, double b) { return a < b ? a : b; }
 static double fmax2(double a, double b) { return a > b ? a : b; }
 
@@ -81,7 +86,7 @@ static double fmax2(double a, double b) { return a > b ? a : b; }
 static void face_up1(double c[4], double q)
 {
     c[0] = c[1] = c[2] = c[3] = 0.0;
-    if (q > 0.0) c[1] = 1.0; else c[2] = 1.0;
+    if ((q > 0.0) != (g_mistake == 2)) c[1] = 1.0; else c[2] = 1.0;
 }
 
 /* MF:11056-11086 FaceConcUpSecondOrder */
@@ -346,7 +351,7 @@ static void thomasz_column(const ora_instance *me_or_null, const ora_size3d *siz
     res[X3(KUB + 1)] = G[KUB + 1];
     for (II = KLB + 1; II <= KUB + 1; II++) {
         int MM = KUB + KLB + 1 - II;
-        res[X3(MM)] = W[MM] * res[X3(MM + 1)] + G[MM];
+        res[X3(MM)] = W[g_mistake == 5 ? MM + 1 : MM] * res[X3(MM + 1)] + G[MM];
     }
 #undef X3
 }
@@ -607,7 +612,7 @@ static void compute_difv_constants(ora_instance *me, const ora_args *a)
                 long x = I3(me, i, j, k);
                 if (a->ComputeFacesW3D[x] == 1) {
                     double auxk = me->KzZ[x] * me->g.DUX[I2(me, i, j)] * me->g.DVY[I2(me, i, j)];
-                    me->DVC[x] = auxk / me->g.DZZ[I3(me, i, j, k - 1)];
+                    me->DVC[x] = auxk / me->g.DZZ[I3(me, i, j, g_mistake == 7 ? k : k - 1)];
                 }
             }
     }
@@ -626,7 +631,8 @@ static void volume_variation(ora_instance *me, const ora_args *a, const double *
             for (i = w->ILB; i <= w->IUB; i++) {
                 long x = I3(me, i, j, k);
                 if (a->OpenPoints3D[x] == 1) {
-                    me->TI[x] = P[x] * (a->VolumeZOld[x] / a->VolumeZ[x]);
+                    me->TI[x] = g_mistake == 3 ? P[x] * (a->VolumeZ[x] / a->VolumeZOld[x])
+                                               : P[x] * (a->VolumeZOld[x] / a->VolumeZ[x]);
                     if (k == w->KUB) {
                         double dtv = a->DTProp / a->VolumeZ[x];
                         me->E[x] = 1.0 + dtv * a->Wflux_Z[I3(me, i, j, k + 1)];
@@ -689,7 +695,8 @@ static void horizontal_diffusion_yy(ora_instance *me, const ora_args *a, const d
                     me->TI[x] = me->TI[x] - auxi / a->VolumeZ[x];
                 } else {
                     double auxi = me->KhY[x] * me->g.AreaV[x] / me->g.DZY[I2(me, i - 1, j)];
-                    me->TI[xm] = me->TI[xm] + auxi * dt / a->VolumeZ[xm] * (P[x] - P[xm]);
+                    me->TI[xm] = me->TI[xm] + auxi * dt / a->VolumeZ[g_mistake == 4 ? x : xm] *
+                                              (P[x] - P[xm]);
                     me->TI[x] = me->TI[x] - auxi * dt / a->VolumeZ[x] * (P[x] - P[xm]);
                 }
             }
@@ -739,7 +746,10 @@ static int horizontal_advection_xx(ora_instance *me, const ora_args *a, const do
                     f = me->xC[x] * P[I3(me, i, j - 2, k)] + me->xD[x] * P[xm] +
                         me->xE[x] * P[x] + me->xF[x] * P[I3(me, i, j + 1, k)];
                 me->TI[x] = me->TI[x] + f * dt / a->VolumeZ[x];
-                me->TI[xm] = me->TI[xm] - f * dt / a->VolumeZ[xm];
+                if (g_mistake == 1)
+                    me->TI[xm] = me->TI[xm] + f * dt / a->VolumeZ[xm];
+                else
+                    me->TI[xm] = me->TI[xm] - f * dt / a->VolumeZ[xm];
             }
     }
     return rc;
@@ -810,7 +820,7 @@ static void vertical_diffusion(ora_instance *me, const ora_args *a, const double
                 double auxk, aux1, aux2;
                 if (a->ComputeFacesW3D[x] != 1) continue;
                 auxk = me->KzZ[x] * me->g.DUX[I2(me, i, j)] * me->g.DVY[I2(me, i, j)] /
-                       me->g.DZZ[xm];
+                       me->g.DZZ[g_mistake == 6 ? x : xm];
                 aux1 = auxk * dt / a->VolumeZ[xm];
                 aux2 = auxk * dt / a->VolumeZ[x];
                 me->E[xm] = me->E[xm] + aux1 * th;

@@ -86,6 +86,17 @@ void ora_kill(ora_instance *me);
 /* Read back an internal array after a call (names: "D","E","F","TI"). */
 const double *ora_internal(const ora_instance *me, const char *name);
 
+/* Test-only: plant deliberate mistake `id` (0 = none) in the restatement,
+ * to show that the invariant and closed-form tests catch it.
+ *   1 sign of the XX advective flux on the upstream cell
+ *   2 first-order face value taken downwind
+ *   3 VolumeZOld/VolumeZ ratio inverted
+ *   4 YY diffusion update of cell i-1 divided by the volume of cell i
+ *   5 THOMASZ back substitution with W(k+1)
+ *   6 theta-scheme vertical diffusion with DZZ(k) instead of DZZ(k-1)
+ *   7 optimised vertical diffusion constant with DZZ(k) instead of DZZ(k-1) */
+void ora_plant_mistake(int id);
+
 /* THOMASZ_NewType2 (ModuleFunctions.F90:4026-4123) on explicit arrays.
  * W, G: scratch of length >= KUB+2 (per-thread VEC in the reference). */
 void ora_thomasz(const ora_size3d *size, const ora_size3d *ws,

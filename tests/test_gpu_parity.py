@@ -133,28 +133,74 @@ def test_advection_diffusion_options(variant):
     gpu.kill()
 
 
-@pytest.mark.parametrize("per_launch", [0, 2])
-def test_batched_transport_matches_oracle(per_launch):
-    """Device-resident multi-tracer transport (hipGraph of 5 steps + 2 eager
-    steps) against the oracle's per-tracer Optimize call pattern; with 2
-    tracers per launch the 3 tracers run in two chunks."""
+def _batched(case, steps_graph, steps_eager, **kw):
     _, Transport, _ = _ad()
-    case = make("tvd_ragged_opt")
     T = len(case.tracers)
-    tr = Transport(case.size, case.worksize, case.geom, T, tracers_per_launch=per_launch)
+    tr = Transport(case.size, case.worksize, case.geom, T, **kw)
     tr.set_hydro(case.hydro, case.params)
     for t, p in enumerate(case.tracers):
         tr.set_tracer(t, p)
-    tr.run(5, use_graph=True)
-    tr.run(2, use_graph=False)
+    if steps_graph:
+        tr.run(steps_graph, use_graph=True)
+    if steps_eager:
+        tr.run(steps_eager, use_graph=False)
+    out = [tr.get_tracer(t) for t in range(T)]
+    tr.close()
+    return out
+
+
+@pytest.mark.parametrize("per_launch", [0, 2])
+def test_batched_transport_chunks_match_oracle(per_launch):
+    """Device-resident multi-tracer transport (hipGraph of 5 steps + 2 eager
+    steps) against the oracle's per-tracer Optimize call pattern; with 2
+    tracers per launch the 3 tracers run in two chunks."""
+    case = make("tvd_ragged_opt")
+    got = _batched(case, 5, 2, tracers_per_launch=per_launch)
     ref = ora.run_case(case, 7, nthreads=1)
     solved = _solved_mask(case)
-    for t in range(T):
-        got = tr.get_tracer(t)
-        err = _rel_err(got, ref[t], solved)
+    for t in range(len(got)):
+        err = _rel_err(got[t], ref[t], solved)
         assert err <= RTOL, (t, err)
-        assert np.array_equal(got[~solved], case.tracers[t][~solved])
-    tr.close()
+        assert np.array_equal(got[t][~solved], case.tracers[t][~solved])
+
+
+@pytest.mark.parametrize("name", sorted(SMALL))
+def test_batched_transport_matches_oracle(name):
+    """Batched FP64 transport on every small case (optimised and plain
+    operation orders, flat, sigma, ragged, land, stiff) for 6 steps."""
+    case = make(name)
+    got = _batched(case, 4, 2)
+    ref = ora.run_case(case, 6, nthreads=1)
+    solved = _solved_mask(case)
+    bitwise = True
+    for t in range(len(got)):
+        err = _rel_err(got[t], ref[t], solved)
+        assert err <= RTOL, (name, t, err)
+        bitwise &= bool(np.array_equal(got[t], ref[t]))
+    print(f"{name}: batched bit-identical={bitwise}")
+
+
+# Fast mode: FP64 with reciprocal-multiply division.  Each replaced division
+# differs from the IEEE quotient by at most a few ulp, so after a handful of
+# steps the results stay within 1e-11 of the oracle (relative, every water
+# cell); the observed error is printed.
+RTOL_FAST = 1e-11
+
+
+@pytest.mark.parametrize("name", sorted(SMALL))
+def test_fast_transport_matches_oracle(name):
+    from mohid_amd import _lib as L
+    case = make(name)
+    got = _batched(case, 4, 2, precision=L.MGPU_PREC_FAST)
+    ref = ora.run_case(case, 6, nthreads=1)
+    solved = _solved_mask(case)
+    worst = 0.0
+    for t in range(len(got)):
+        err = _rel_err(got[t], ref[t], solved)
+        worst = max(worst, err)
+        assert err <= RTOL_FAST, (name, t, err)
+        assert np.array_equal(got[t][~solved], case.tracers[t][~solved])
+    print(f"{name}: fast max rel err {worst:.3e}")
 
 
 def test_batched_graph_replays_agree():

@@ -1,0 +1,294 @@
+"""Pins the oracle's AdvectionDiffusion restatement without the reference.
+
+ModuleAdvectionDiffusion.F90 needs the HDF5-backed grid modules and does not
+build here, so the orchestration the oracle restates (VolumeVariation, the
+horizontal diffusion and advection scatters, the vertical diffusion and
+advection rows, THOMASZ over the columns: AD:1624-1922) is pinned by what the
+mathematics fixes:
+
+* closed forms: first-order upwind transport along a channel
+  (F = Q * C_upwind on every computed face), and horizontal diffusion of a
+  quadratic profile (interior change = 2 * Kh * dt * a for C = a x^2);
+* dense solves: the implicit vertical diffusion column (theta scheme and the
+  optimised path, stretched layers) and the implicit first-order vertical
+  advection column with a moving volume (V != VolumeZOld), each assembled
+  here from the textbook finite-volume balance and solved with numpy;
+* invariants of a closed basin with divergence-free transports and
+  VolumeZOld = VolumeZ (every fixture): sum(V * C) over water cells is
+  conserved and a constant field stays constant, on every small case.
+
+``test_planted_mistakes_are_caught`` plants each of a list of plausible
+mistakes in the oracle (a flux sign, the upwind side, the volume ratio, a
+wrong cell volume, a Thomas index, the vertical distance index) and requires
+that at least one of these checks fails for each.
+"""
+import ctypes as ct
+
+import numpy as np
+import pytest
+
+from mohid_amd import constants as C
+from mohid_amd import fixtures as fx
+from oracle import oracle as ora
+
+from .cases import SMALL, make
+
+DT = 60.0
+
+
+def _lib():
+    L = ora.lib()
+    L.ora_plant_mistake.argtypes = [ct.c_int]
+    return L
+
+
+def _run(case, prop, hydro=None, steps=1, docycle=1, **params):
+    inst = ora.Instance(case, docycle_method=docycle, nthreads=2)
+    dt = params.get("DTProp", case.params["DTProp"])
+    for s in range(steps):
+        rc = inst.step(prop, first_property=True, now=(s + 1) * dt, hydro=hydro, **params)
+        assert rc == 0, rc
+    inst.close()
+    return prop
+
+
+def _still_hydro(case):
+    """The case's masks and volumes with every transport and diffusivity zero."""
+    hy = {k: np.array(v, order="F", copy=True) for k, v in case.hydro.items()}
+    for n in ("Wflux_X", "Wflux_Y", "Wflux_Z", "Visc_H", "Diff_V"):
+        hy[n][...] = 0.0
+    return hy
+
+
+UPWIND = dict(AdvMethodH=C.UPWIND_ORDER1, AdvMethodV=C.UPWIND_ORDER1, Upwind2H=False,
+              Upwind2V=False, Optimize=False, DTProp=DT)
+TVD_OPT = dict(AdvMethodH=C.P2_TVD, AdvMethodV=C.P2_TVD, TVDLimitationH=C.SUPERBEE,
+               TVDLimitationV=C.SUPERBEE, Upwind2H=True, Upwind2V=True, Optimize=True,
+               DTProp=DT)
+
+
+# ---------------------------------------------------------------------------
+# closed forms
+def _channel_upwind_err(axis, q):
+    """Relative error of one upwind step along a channel against
+    C_new = C + dt / V * (F_in - F_out), F = Q * C(upwind cell)."""
+    ni, nj = (3, 12) if axis == "x" else (12, 3)
+    case = fx.make_case(fx.config("c1", ni=ni, nj=nj, nk=2))
+    hy = _still_hydro(case)
+    name, cf = ("Wflux_X", "ComputeFacesU3D") if axis == "x" else ("Wflux_Y", "ComputeFacesV3D")
+    faces = hy[cf] == 1
+    hy[name][faces] = q
+    wp = hy["WaterPoints3D"] == 1
+    rng = np.random.default_rng(11)
+    c0 = np.where(wp, np.exp(rng.normal(size=wp.shape)), 0.0)
+    c0 = np.asfortranarray(c0)
+    got = _run(case, c0.copy(order="F"), hydro=hy, **UPWIND)
+    # face f of cell c sits between c-1 and c along the axis
+    ax = 1 if axis == "x" else 0
+    up = np.roll(c0, 1, axis=ax)               # value of cell c-1
+    flux = np.where(faces, q * (up if q > 0 else c0), 0.0)
+    flux_next = np.roll(flux, -1, axis=ax)     # face of cell c+1
+    V = hy["VolumeZ"]
+    want = c0 + DT / V * (flux - flux_next)
+    return float(np.max(np.abs(got[wp] - want[wp]) / np.abs(want[wp])))
+
+
+@pytest.mark.parametrize("axis", ["x", "y"])
+@pytest.mark.parametrize("q", [2000.0, -1500.0])
+def test_upwind_channel_closed_form(axis, q):
+    assert _channel_upwind_err(axis, q) <= 1e-13
+
+
+def _quadratic_diffusion_err(axis, optimize):
+    """Interior change of C = a x^2 under horizontal diffusion Kh is
+    2 a Kh dt on a uniform grid (second difference exact for quadratics)."""
+    ni, nj = (3, 12) if axis == "x" else (12, 3)
+    case = fx.make_case(fx.config("c1", ni=ni, nj=nj, nk=2))
+    hy = _still_hydro(case)
+    kh = 50.0
+    hy["Visc_H"][...] = kh
+    wp = hy["WaterPoints3D"] == 1
+    dx = case.cfg.dx if axis == "x" else case.cfg.dy
+    a = 1e-6                                   # C = a x^2, x in metres
+    idx = np.arange(case.shape[1] if axis == "x" else case.shape[0], dtype=np.float64)
+    prof = a * (idx * dx) ** 2
+    c0 = prof[None, :, None] if axis == "x" else prof[:, None, None]
+    c0 = np.asfortranarray(np.where(wp, np.broadcast_to(c0, wp.shape), 0.0))
+    params = dict(TVD_OPT if optimize else UPWIND, schmidt_H=1.0)
+    got = _run(case, c0.copy(order="F"), hydro=hy, **params)
+    n = nj if axis == "x" else ni
+    sl = (slice(1, ni + 1), slice(2, n), slice(1, 3)) if axis == "x" else \
+        (slice(2, n), slice(1, nj + 1), slice(1, 3))
+    d = got[sl] - c0[sl]
+    want = 2.0 * a * kh * DT
+    return float(np.max(np.abs(d - want)) / want)
+
+
+@pytest.mark.parametrize("axis", ["x", "y"])
+@pytest.mark.parametrize("optimize", [False, True])
+def test_quadratic_diffusion_closed_form(axis, optimize):
+    assert _quadratic_diffusion_err(axis, optimize) <= 1e-9
+
+
+# ---------------------------------------------------------------------------
+# dense column solves
+def _column_matrix(n):
+    return np.eye(n), np.zeros(n)
+
+
+def _vertical_diffusion_err(optimize, theta=1.0, docycle=1):
+    """Implicit (theta) vertical diffusion on stretched sigma columns against
+    the dense solve of  x_k - C_k = -dt/V_k * sum_faces g (theta x + (1-theta) C)
+    differences, g = Kz * DUX * DVY / (centre distance)."""
+    case = fx.make_case(fx.config("c4", ni=3, nj=4, nk=9, depth=(5.0, 40.0)))
+    hy = _still_hydro(case)
+    hy["Diff_V"][...] = case.hydro["Diff_V"]
+    wp = hy["WaterPoints3D"] == 1
+    rng = np.random.default_rng(3)
+    c0 = np.asfortranarray(np.where(wp, rng.uniform(1.0, 9.0, size=wp.shape), 0.0))
+    params = dict(TVD_OPT if optimize else UPWIND, ImpExp_DifV=theta, SchmidtCoef_V=1.0,
+                  SchmidtBackground_V=0.0)
+    got = _run(case, c0.copy(order="F"), hydro=hy, docycle=docycle, **params)
+    g = case.geom
+    KUB = case.worksize[5]
+    worst = 0.0
+    for i in range(1, case.worksize[1] + 1):
+        for j in range(1, case.worksize[3] + 1):
+            if not wp[i, j, KUB]:
+                continue
+            kf = int(g["KFloorZ"][i, j])
+            ks = np.arange(kf, KUB + 1)
+            n = ks.size
+            V = hy["VolumeZ"][i, j, ks]
+            L = np.zeros((n, n))
+            for m in range(1, n):                 # face between ks[m-1] and ks[m]
+                k = ks[m]
+                if hy["ComputeFacesW3D"][i, j, k] != 1:
+                    continue
+                dist = 0.5 * (g["DWZ"][i, j, k - 1] + g["DWZ"][i, j, k])
+                cond = hy["Diff_V"][i, j, k] * g["DUX"][i, j] * g["DVY"][i, j] / dist
+                L[m, m] += cond; L[m, m - 1] -= cond
+                L[m - 1, m - 1] += cond; L[m - 1, m] -= cond
+            S = DT / V[:, None] * L
+            c = c0[i, j, ks]
+            A = np.eye(n) + theta * S
+            b = c - (1.0 - theta) * (S @ c)
+            x = np.linalg.solve(A, b)
+            worst = max(worst, float(np.max(np.abs(got[i, j, ks] - x) / np.abs(x))))
+    return worst
+
+
+@pytest.mark.parametrize("optimize,theta,docycle", [
+    (False, 1.0, 1), (False, 0.5, 1), (True, 1.0, 1), (True, 1.0, 2), (False, 0.0, 1)])
+def test_vertical_diffusion_dense_solve(optimize, theta, docycle):
+    if optimize and theta != 1.0:
+        pytest.skip("the optimised path is implicit only")
+    assert _vertical_diffusion_err(optimize, theta, docycle) <= 1e-12
+
+
+def _vertical_advection_err(docycle=1):
+    """Implicit first-order vertical advection with a moving volume:
+    x_k + dt/V_k (F_{k+1} - F_k) = C_k VolumeZOld_k / V_k,
+    F_f = Qz_f * x(upwind cell of face f), V = VolumeZOld + dt (Qz_k - Qz_{k+1})."""
+    case = fx.make_case(fx.config("c1", ni=3, nj=3, nk=7))
+    hy = _still_hydro(case)
+    KUB = case.worksize[5]
+    wp = hy["WaterPoints3D"] == 1
+    cfw = hy["ComputeFacesW3D"] == 1
+    rng = np.random.default_rng(8)
+    V = hy["VolumeZ"]
+    qz = np.where(cfw, rng.uniform(-0.3, 0.3, size=V.shape) * V / DT, 0.0)
+    hy["Wflux_Z"][...] = qz
+    qz_top = np.zeros_like(qz)
+    qz_top[:, :, :-1] = qz[:, :, 1:]
+    vold = np.where(wp, V - DT * (qz - qz_top), V)
+    hy["VolumeZOld"][...] = vold
+    c0 = np.asfortranarray(np.where(wp, np.exp(rng.normal(size=wp.shape)), 0.0))
+    params = dict(UPWIND, ImpExp_AdvV=C.IMPLICIT_SCHEME)
+    got = _run(case, c0.copy(order="F"), hydro=hy, docycle=docycle, **params)
+    worst = 0.0
+    for i in range(1, 4):
+        for j in range(1, 4):
+            ks = np.arange(1, KUB + 1)
+            n = ks.size
+            A = np.eye(n)
+            b = c0[i, j, ks] * vold[i, j, ks] / V[i, j, ks]
+            for m in range(1, n):                 # face ks[m]: between m-1 and m
+                q = qz[i, j, ks[m]]
+                up = m - 1 if q > 0 else m
+                # outflow of cell m-1, inflow of cell m
+                A[m - 1, up] += DT / V[i, j, ks[m - 1]] * q
+                A[m, up] -= DT / V[i, j, ks[m]] * q
+            x = np.linalg.solve(A, b)
+            worst = max(worst, float(np.max(np.abs(got[i, j, ks] - x) / np.abs(x))))
+    return worst
+
+
+@pytest.mark.parametrize("docycle", [1, 2])
+def test_vertical_advection_dense_solve(docycle):
+    assert _vertical_advection_err(docycle) <= 1e-12
+
+
+# ---------------------------------------------------------------------------
+# closed-basin invariants on every small case
+def _invariant_errs(name, steps=3):
+    case = make(name)
+    wp = case.hydro["WaterPoints3D"] == 1
+    V = case.hydro["VolumeZ"]
+    assert np.array_equal(case.hydro["VolumeZOld"], V)
+    inst = ora.Instance(case, nthreads=2)
+    props = [np.array(t, order="F", copy=True) for t in case.tracers]
+    const = np.asfortranarray(np.where(wp, 7.0, case.tracers[0]))
+    m0 = [float(np.sum(V[wp] * p[wp])) for p in props]
+    dt = case.params["DTProp"]
+    for s in range(steps):
+        for t, p in enumerate(props):
+            assert inst.step(p, first_property=(t == 0), now=(s + 1) * dt) == 0
+    inst.close()
+    # the constant field alone, through the same option set
+    inst = ora.Instance(case, nthreads=2)
+    for s in range(steps):
+        assert inst.step(const, first_property=True, now=(s + 1) * dt) == 0
+    inst.close()
+    mass = max(abs(float(np.sum(V[wp] * p[wp])) - m) / m for p, m in zip(props, m0))
+    constancy = float(np.max(np.abs(const[wp] - 7.0)) / 7.0)
+    return mass, constancy
+
+
+@pytest.mark.parametrize("name", sorted(SMALL))
+def test_closed_basin_conserves_mass_and_constants(name):
+    mass, constancy = _invariant_errs(name)
+    assert mass <= 1e-13, mass
+    assert constancy <= 1e-12, constancy
+
+
+# ---------------------------------------------------------------------------
+def _all_checks():
+    """(name, error, tolerance) of every check above."""
+    out = []
+    for axis in ("x", "y"):
+        for q in (2000.0, -1500.0):
+            out.append((f"channel {axis} {q}", _channel_upwind_err(axis, q), 1e-13))
+        for opt in (False, True):
+            out.append((f"diffusion {axis} opt={opt}", _quadratic_diffusion_err(axis, opt), 1e-9))
+    for opt, th in ((False, 1.0), (False, 0.5), (True, 1.0)):
+        out.append((f"vdiff opt={opt} theta={th}", _vertical_diffusion_err(opt, th), 1e-12))
+    out.append(("vadv", _vertical_advection_err(), 1e-12))
+    for name in sorted(SMALL):
+        mass, constancy = _invariant_errs(name, steps=2)
+        out.append((f"mass {name}", mass, 1e-13))
+        out.append((f"constant {name}", constancy, 1e-12))
+    return out
+
+
+@pytest.mark.parametrize("mistake", [1, 2, 3, 4, 5, 6, 7])
+def test_planted_mistakes_are_caught(mistake):
+    L = _lib()
+    L.ora_plant_mistake(mistake)
+    try:
+        checks = _all_checks()
+    finally:
+        L.ora_plant_mistake(0)
+    failed = [n for n, err, tol in checks if not (err <= tol)]
+    assert failed, f"planted mistake {mistake} passed every check"
+    print(f"mistake {mistake}: caught by {failed}")

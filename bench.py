@@ -76,9 +76,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-window", type=int, default=384)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--tile", default="0,0")
+    ap.add_argument("--kernel", default="split", choices=("fused", "split"))
     ap.add_argument("--tracers-per-launch", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of 512 MiB three times (the "
+                         "reference workload for scaling FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        torch.cuda.set_device(0)
+        n = 64 * 1024 * 1024
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 8, "copies": 3}), flush=True)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the transport step's "
@@ -107,8 +121,8 @@ def main():
          f"({time.time() - t0:.1f}s to build)")
 
     prec = _lib.MGPU_PREC_FAST if args.precision == "fast" else _lib.MGPU_PREC_FP64
-    tile = tuple(int(x) for x in args.tile.split(","))
-    tr = Transport(case.size, case.worksize, case.geom, T, precision=prec, tile=tile,
+    kernel = _lib.MGPU_KERNEL_SPLIT if args.kernel == "split" else _lib.MGPU_KERNEL_FUSED
+    tr = Transport(case.size, case.worksize, case.geom, T, precision=prec, kernel=kernel,
                    tracers_per_launch=args.tracers_per_launch)
     tr.set_hydro(case.hydro, case.params)
     for t, p in enumerate(case.tracers):
@@ -224,7 +238,7 @@ def main():
         cwall = time.perf_counter() - c0
         inst.close()
         cval = sT * swet * nsteps / cwall
-        tr2 = Transport(scase.size, scase.worksize, scase.geom, sT, precision=prec, tile=tile,
+        tr2 = Transport(scase.size, scase.worksize, scase.geom, sT, precision=prec, kernel=kernel,
                         tracers_per_launch=args.tracers_per_launch)
         tr2.set_hydro(scase.hydro, scase.params)
         for t, p in enumerate(scase.tracers):

@@ -53,6 +53,11 @@ extern "C" {
 #define MGPU_PREC_FAST 1         /* FP64 with reciprocal-multiply division (rounding-level
                                     differences from the reference's operation order) */
 
+/* ---- kernel variant of the transport ------------------------------------- */
+#define MGPU_KERNEL_AUTO 0       /* the library's choice (split) */
+#define MGPU_KERNEL_FUSED 1      /* one column-tile kernel per step and tracer chunk */
+#define MGPU_KERNEL_SPLIT 2      /* explicit kernel per level, then a column-solve kernel */
+
 /* T_Size3D (GD:2044-2052): ILB, IUB, JLB, JUB, KLB, KUB. */
 typedef struct {
     int32_t ILB, IUB, JLB, JUB, KLB, KUB;
@@ -142,7 +147,7 @@ typedef struct {
     int32_t ntracers;
     int32_t precision;                    /* MGPU_PREC_FP64 / MGPU_PREC_FAST */
     int32_t docycle_method;
-    int32_t tile_i, tile_j;               /* column-solve block shape; 0: 64 x 4 */
+    int32_t kernel;                       /* MGPU_KERNEL_AUTO / _FUSED / _SPLIT */
     int32_t tracers_per_launch;           /* 1..8 tracers per kernel launch; 0: 8 */
 } mgpu_transport_opts;
 
@@ -167,8 +172,10 @@ int mgpu_transport_destroy(mgpu_transport *t);
  * mgpu_transport_run calls (use_graph = 0, up to max_steps steps) record
  * hipEvents on the handle's stream around the kernels of every step;
  * mgpu_transport_kernel_times returns the mean duration (ms) per step of the
- * face-conductance kernel, of the explicit (horizontal) kernel(s) and of the
- * vertical/column-solve kernel(s) over all tracers, and the steps timed. */
+ * face-conductance kernel, of the first transport kernel(s) (the fused
+ * column-tile kernel, or the split variant's explicit kernel) and of the
+ * split variant's column-solve kernel(s) (0 for the fused variant) over all
+ * tracers, and the steps timed. */
 int mgpu_transport_timing(mgpu_transport *t, int32_t enable, int32_t max_steps);
 int mgpu_transport_kernel_times(mgpu_transport *t, double *prep_ms, double *explicit_ms,
                                 double *vertical_ms, int32_t *steps);

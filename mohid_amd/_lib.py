@@ -28,6 +28,9 @@ MGPU_ERR_STATE = 105
 MGPU_ERR_NOMEM = 106
 MGPU_PREC_FP64 = 0
 MGPU_PREC_FAST = 1
+MGPU_KERNEL_AUTO = 0
+MGPU_KERNEL_FUSED = 1
+MGPU_KERNEL_SPLIT = 2
 
 STATUS_NAMES = {
     0: "SUCCESS", 1: "OFF_ERR", -99: "UNKNOWN", 100: "ERR_HIP", 101: "ERR_BADARG",
@@ -75,7 +78,7 @@ class ADArgs(ct.Structure):
 
 class TransportOpts(ct.Structure):
     _fields_ = [("ntracers", ct.c_int32), ("precision", ct.c_int32),
-                ("docycle_method", ct.c_int32), ("tile_i", ct.c_int32), ("tile_j", ct.c_int32),
+                ("docycle_method", ct.c_int32), ("kernel", ct.c_int32),
                 ("tracers_per_launch", ct.c_int32)]
 
 

@@ -185,11 +185,11 @@ class Transport:
     ModuleWaterProperties::Advection_Diffusion_Processes, WP:14520-15169)."""
 
     def __init__(self, size, worksize, geometry: Dict, ntracers: int, precision: int = 0,
-                 docycle_method: int = 1, tile=(0, 0), tracers_per_launch: int = 0):
+                 docycle_method: int = 1, kernel: int = 0, tracers_per_launch: int = 0):
         keep: list = []
         g = _geometry(geometry, keep)
         opts = L.TransportOpts(int(ntracers), int(precision), int(docycle_method),
-                               int(tile[0]), int(tile[1]), int(tracers_per_launch))
+                               int(kernel), int(tracers_per_launch))
         h = ct.c_void_p()
         self._size = L.Size3D(*size)
         self._ws = L.Size3D(*worksize)

@@ -267,7 +267,8 @@ struct mgpu_ad {
     DevGeometry geom;
     DevHydro hydro;
     DevDerived der;
-    double *prop = nullptr, *out = nullptr, *Wscr = nullptr;
+    double *prop = nullptr, *out = nullptr;
+    double *Wb = nullptr, *Gb = nullptr;    // forward-sweep W and G (tile-blocked)
     bool have_faces = false;
     unsigned long long last_zero_pivots = 0;
 };
@@ -299,7 +300,8 @@ extern "C" int mgpu_ad_create(const mgpu_size3d *size, const mgpu_size3d *worksi
     if (!rc) rc = ad->der.alloc(ad->d);
     if (!rc) rc = dalloc(&ad->prop, n3);
     if (!rc) rc = dalloc(&ad->out, n3);
-    if (!rc) rc = dalloc(&ad->Wscr, n3);
+    if (!rc) rc = dalloc(&ad->Wb, blocked_scratch_len(ad->d, 1));
+    if (!rc) rc = dalloc(&ad->Gb, blocked_scratch_len(ad->d, 1));
     if (!rc) rc = ad->geom.upload(ad->d, geometry, ad->stream);
     if (rc) {
         mgpu_ad_destroy(ad);
@@ -320,7 +322,7 @@ extern "C" int mgpu_ad_destroy(mgpu_ad *ad) {
     ad->geom.release();
     ad->hydro.release();
     ad->der.release();
-    dfree(ad->prop); dfree(ad->out); dfree(ad->Wscr);
+    dfree(ad->prop); dfree(ad->out); dfree(ad->Wb); dfree(ad->Gb);
     if (ad->stream) (void)hipStreamDestroy(ad->stream);
     delete ad;
     return MGPU_SUCCESS;
@@ -368,8 +370,17 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
         ad->have_faces = true;
     }
     const SharedFields f = make_fields(ad->hydro, ad->der, ad->geom);
-    rc = launch_transport_fp64(d, sc, f, 1, 1, P, ad->Wscr, ad->out, ad->Wscr, ad->der.counters,
-                               1, 0, 0, s);
+    TransportLaunch L;
+    L.variant = MGPU_KERNEL_SPLIT;
+    L.n = 1;
+    L.stride = 1;
+    L.chunk = 1;
+    L.P = P;
+    L.out = ad->out;
+    L.Wb = ad->Wb;
+    L.Gb = ad->Gb;
+    L.counters = ad->der.counters;
+    rc = launch_transport(L, d, sc, f, s);
     if (rc) return rc;
     rc = launch_copy_back(d, ad->der.mask, ad->out, P, s);
     if (rc) return rc;
@@ -449,7 +460,8 @@ struct mgpu_transport {
     // tracer state, interleaved: tracer t of cell x at [x * ntracers + t]
     double *buf[2] = {nullptr, nullptr};
     int cur = 0;
-    double *scr = nullptr;           // explicit RHS, then forward-sweep W (same layout)
+    int kernel = MGPU_KERNEL_SPLIT;
+    double *Wb = nullptr, *Gb = nullptr;  // column-solve scratch (tile-blocked)
     double *stage = nullptr;         // one Fortran-layout tracer (host transfers)
     int chunk = 8;                   // tracers per launch
     hipGraph_t graph[2] = {nullptr, nullptr};
@@ -497,7 +509,8 @@ extern "C" int mgpu_transport_destroy(mgpu_transport *t) {
     t->der.release();
     dfree(t->buf[0]);
     dfree(t->buf[1]);
-    dfree(t->scr);
+    dfree(t->Wb);
+    dfree(t->Gb);
     dfree(t->stage);
     if (t->stream) (void)hipStreamDestroy(t->stream);
     delete t;
@@ -520,11 +533,15 @@ extern "C" int mgpu_transport_create(const mgpu_size3d *size, const mgpu_size3d 
     t->ws = *worksize;
     t->d = make_dims(*size, *worksize);
     t->opts = *opts;
-    if (opts->tracers_per_launch < 0 || opts->tracers_per_launch > 8) {
+    if (opts->tracers_per_launch < 0 || opts->tracers_per_launch > 8 ||
+        (opts->kernel != MGPU_KERNEL_AUTO && opts->kernel != MGPU_KERNEL_FUSED &&
+         opts->kernel != MGPU_KERNEL_SPLIT)) {
         delete t;
         return MGPU_ERR_BADARG;
     }
     t->chunk = opts->tracers_per_launch ? opts->tracers_per_launch : 8;
+    if (t->chunk > opts->ntracers) t->chunk = opts->ntracers;
+    t->kernel = opts->kernel == MGPU_KERNEL_FUSED ? MGPU_KERNEL_FUSED : MGPU_KERNEL_SPLIT;
     hipError_t e = hipStreamCreate(&t->stream);
     if (e != hipSuccess) {
         mgpu_set_error("hipStreamCreate", e);
@@ -536,7 +553,8 @@ extern "C" int mgpu_transport_create(const mgpu_size3d *size, const mgpu_size3d 
     if (!rc) rc = t->hydro.alloc(t->d);
     if (!rc) rc = t->der.alloc(t->d);
     const long nT = (long)opts->ntracers;
-    if (!rc) rc = dalloc(&t->scr, n3 * nT);
+    if (!rc) rc = dalloc(&t->Wb, blocked_scratch_len(t->d, opts->ntracers));
+    if (!rc) rc = dalloc(&t->Gb, blocked_scratch_len(t->d, opts->ntracers));
     if (!rc) rc = dalloc(&t->buf[0], n3 * nT);
     if (!rc) rc = dalloc(&t->buf[1], n3 * nT);
     if (!rc) rc = dalloc(&t->stage, n3);
@@ -643,16 +661,20 @@ static int enqueue_steps(mgpu_transport *t, int nsteps, int cur) {
                                    t->der.cw, s);
         if (rc) return rc;
         if ((rc = mark(t))) return rc;
-        const int n = t->opts.ntracers;
-        for (int ph = 1; ph <= 2; ++ph) {   // explicit kernel(s), then vertical kernel(s)
-            if (t->opts.precision == MGPU_PREC_FAST)
-                rc = launch_transport_fast(d, t->sc, f, n, n, t->buf[cur], t->scr,
-                                            t->buf[cur ^ 1], t->scr, t->der.counters, t->chunk,
-                                            t->opts.tile_i, t->opts.tile_j, s, ph);
-            else
-                rc = launch_transport_fp64(d, t->sc, f, n, n, t->buf[cur], t->scr,
-                                           t->buf[cur ^ 1], t->scr, t->der.counters, t->chunk,
-                                           t->opts.tile_i, t->opts.tile_j, s, ph);
+        TransportLaunch L;
+        L.fast = t->opts.precision == MGPU_PREC_FAST;
+        L.variant = t->kernel;
+        L.n = t->opts.ntracers;
+        L.stride = t->opts.ntracers;
+        L.chunk = t->chunk;
+        L.P = t->buf[cur];
+        L.out = t->buf[cur ^ 1];
+        L.Wb = t->Wb;
+        L.Gb = t->Gb;
+        L.counters = t->der.counters;
+        // first kernel(s) of every chunk, then the split variant's second kernel(s)
+        for (int ph = 1; ph <= 2; ++ph) {
+            rc = launch_transport(L, d, t->sc, f, s, ph);
             if (rc) return rc;
             if ((rc = mark(t))) return rc;
         }

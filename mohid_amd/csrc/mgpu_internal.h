@@ -96,23 +96,31 @@ int launch_prep_faces(const Dims &d, const double *visc, const double *diffv,
                       double schmidtV, double backgroundV, int nulldif,
                       double *cu, double *cv, double *cw, hipStream_t s);
 // Transport of n tracers P -> out.  Tracer t of cell index x lives at
-// [x * stride + t] of each buffer (stride 1: one tracer per array; stride n:
-// interleaved).  TI and W are scratch of the same shape and may alias
-// (kernel V reads TI(k) before writing W(k)).  Tracers go in chunks of
-// `chunk` (<= 8) per launch.  phase: 1 = explicit kernel only, 2 = vertical
-// kernel only, 3 = both.
-int launch_transport_fp64(const Dims &d, const StepScalars &sc, const SharedFields &f,
-                          int n, long stride, const double *P, double *TI, double *out,
-                          double *W, unsigned long long *counters, int chunk, int tile_i,
-                          int tile_j, hipStream_t s, int phase = 3);
-// Same transport with reciprocal-based division (fast mode): FP64 state and
-// arithmetic, divisions replaced by multiplications with reciprocals shared
-// across tracers where possible; results differ from the reference's order
-// by rounding only.
-int launch_transport_fast(const Dims &d, const StepScalars &sc, const SharedFields &f,
-                          int n, long stride, const double *P, double *TI, double *out,
-                          double *W, unsigned long long *counters, int chunk, int tile_i,
-                          int tile_j, hipStream_t s, int phase = 3);
+// [x * stride + t] of P and out (stride 1: one tracer per array; stride n:
+// interleaved).  Tracers go in chunks of `chunk` (<= 8) per launch.  Both
+// variants use two tile-blocked scratch arrays Wb and Gb of
+// blocked_scratch_len(d, n) doubles each (chunk c at offset
+// blocked_scratch_len(d, first tracer of c)):
+//   variant MGPU_KERNEL_FUSED: one column-tile kernel (W in Wb, G in Gb);
+//   variant MGPU_KERNEL_SPLIT: explicit kernel (TI into Wb), then the column
+//     kernel (W over TI in Wb, G in Gb).
+// phase: bit 1 = first kernel, bit 2 = second kernel (split only).
+// fast: reciprocal-multiply division (FP64 state and arithmetic, results
+// differ from the reference's operation order by rounding only).
+struct TransportLaunch {
+    bool fast = false;
+    int variant = MGPU_KERNEL_SPLIT;
+    int n = 1;
+    long stride = 1;
+    int chunk = 8;
+    const double *P = nullptr;
+    double *out = nullptr;
+    double *Wb = nullptr, *Gb = nullptr;
+    unsigned long long *counters = nullptr; // [0] zero pivots, [1] bad-method flag
+};
+long blocked_scratch_len(const Dims &d, int nt);
+int launch_transport(const TransportLaunch &L, const Dims &d, const StepScalars &sc,
+                     const SharedFields &f, hipStream_t s, int phase = 3);
 // Copy one tracer between a Fortran array (stride 1) and an interleaved
 // buffer (stride `stride`, slot t).  to_interleaved = 1: src Fortran -> dst.
 int launch_interleave(long n3, const double *src, double *dst, long stride, int t,

@@ -124,6 +124,29 @@ __device__ __forceinline__ void face_pos(int f, int &fi, int &fj, bool &isU, int
     }
 }
 
+// Blocks are dispatched to the 8 XCDs round robin; give each XCD a contiguous
+// run of tiles so that neighbouring tiles (which share halo rows) share an L2.
+__device__ __forceinline__ long xcd_tile(long b, long n) {
+    const long q = n / 8, r = n % 8, x = b % 8, idx = b / 8;
+    return x * q + (x < r ? x : r) + idx;
+}
+
+// For a grid over (level, tile) pairs: XCD x takes a contiguous segment of
+// an order that runs over bands of ceil(ntiles/8) consecutive tiles, all
+// levels of a band, level-major inside the band.  Every XCD then sees the
+// whole range of levels (the work per level varies with KFloor) and
+// consecutive blocks of an XCD are neighbouring tiles of one level.
+__device__ __forceinline__ void xcd_level_tile(long b, long n, long ntiles, int nk, long &tile,
+                                               int &level) {
+    const long pos = xcd_tile(b, n);
+    const long G = (ntiles + 7) / 8;
+    const long band = pos / (G * nk);
+    const long rem = pos - band * G * nk;
+    const long gB = ntiles - band * G < G ? ntiles - band * G : G;
+    level = (int)(rem / gB);
+    tile = band * G + rem % gB;
+}
+
 __global__ void __launch_bounds__(TC)
 tile_sort_kernel(Dims d, const uint8_t *__restrict__ mask, const int32_t *__restrict__ kstart,
                  uint8_t *__restrict__ cperm, uint16_t *__restrict__ ccnt,
@@ -247,9 +270,7 @@ constexpr int MAXT = 8;
 // memory stream.
 struct TracerSet {
     const double *P;         // state at the start of the step
-    double *TI;              // explicit right-hand side (kernel H -> kernel V)
-    double *out;             // new state; holds the forward-sweep G in between
-    double *W;               // forward-sweep W (may alias TI)
+    double *out;             // new state
     long s;                  // tracer stride
 };
 
@@ -335,55 +356,50 @@ __device__ __forceinline__ void hface_flux(const Dims &d, const StepScalars &sc,
     }
 }
 
+// Face fluxes of one tile at level k into LDS: the faces whose first
+// computing level is <= k are the sorted prefix fperm[0 .. nf).
 template <bool OPT, int NT, bool FAST>
-__global__ void __launch_bounds__(TC)
-explicit_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
-                unsigned int *__restrict__ badflag) {
-    __shared__ double FU[NT][TY + 1][TX];
-    __shared__ double FV[NT][TY][TX + 1];
-    const int tid = threadIdx.x;
-    const long tile = blockIdx.x + (long)gridDim.x * blockIdx.y;
-    const int i0 = d.wILB + blockIdx.x * TX, j0 = d.wJLB + blockIdx.y * TY;
-    const int k = d.wKLB + blockIdx.z;
-    const int nK = d.KUB - d.KLB + 1;
-    const int nc = f.ccnt[tile * nK + k];    // active cells of this level (sorted prefix)
-    if (nc == 0) return;
-    const double dt = sc.dt;
-    int bad = 0;
-
-    if (!sc.vertical1d) {
-        const int nf = f.fcnt[tile * nK + k];
-        for (int q = tid; q < nf; q += TC) {
-            int fi, fj;
-            bool isU;
-            const int fid = f.fperm[tile * TF + q];
-            face_pos(fid, fi, fj, isU, i0, j0);
-            const long y = d.idx(fi, fj, k);
-            double fl[NT];
-            if (isU) {
-                if (f.mask[y] & M_CFU) {
-                    hface_flux<OPT, NT, FAST>(d, sc, f, ts, y, d.sj, fj, d.JLB, f.DUX + d.idx2(fi, fj),
-                                        d.sj, f.Qx[y], fl, bad);
-                    const int r = fid / TX, c = fid % TX;
+__device__ __forceinline__ void tile_faces(const Dims &d, const StepScalars &sc,
+                                           const SharedFields &f, const TracerSet &ts, long tile,
+                                           int i0, int j0, int k, int nf,
+                                           double (*FU)[TY + 1][TX], double (*FV)[TY][TX + 1],
+                                           int &bad) {
+    for (int q = threadIdx.x; q < nf; q += TC) {
+        int fi, fj;
+        bool isU;
+        const int fid = f.fperm[tile * TF + q];
+        face_pos(fid, fi, fj, isU, i0, j0);
+        const long y = d.idx(fi, fj, k);
+        double fl[NT];
+        if (isU) {
+            if (f.mask[y] & M_CFU) {
+                hface_flux<OPT, NT, FAST>(d, sc, f, ts, y, d.sj, fj, d.JLB,
+                                          f.DUX + d.idx2(fi, fj), d.sj, f.Qx[y], fl, bad);
+                const int r = fid / TX, c = fid % TX;
 #pragma unroll
-                    for (int t = 0; t < NT; ++t) FU[t][r][c] = fl[t];
-                }
-            } else if (!sc.xzflow && (f.mask[y] & M_CFV)) {
-                hface_flux<OPT, NT, FAST>(d, sc, f, ts, y, 1, fi, d.ILB, f.DVY + d.idx2(fi, fj), 1,
-                                    f.Qy[y], fl, bad);
-                const int g = fid - TFU, r = g / (TX + 1), c = g % (TX + 1);
-#pragma unroll
-                for (int t = 0; t < NT; ++t) FV[t][r][c] = fl[t];
+                for (int t = 0; t < NT; ++t) FU[t][r][c] = fl[t];
             }
+        } else if (!sc.xzflow && (f.mask[y] & M_CFV)) {
+            hface_flux<OPT, NT, FAST>(d, sc, f, ts, y, 1, fi, d.ILB, f.DVY + d.idx2(fi, fj), 1,
+                                      f.Qy[y], fl, bad);
+            const int g = fid - TFU, r = g / (TX + 1), c = g % (TX + 1);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) FV[t][r][c] = fl[t];
         }
-        __syncthreads();
     }
-    if (bad) atomicOr(badflag, 1u);
-    if (tid >= nc) return;
+}
 
-    const int slot = f.cperm[tile * TC + tid];
-    const int tx = slot % TX, ty = slot / TX;
-    const long x = d.idx(i0 + tx, j0 + ty, k);
+// Explicit right-hand side TI of cell x (tile position tx, ty) for NT tracers:
+// VolumeVariation (AD:3993-4001), HorizontalDiffusionXX/YY(2) (AD:5185-5196,
+// 5240-5246, 5286-5292, 5341-5351) and the explicit horizontal advection
+// scatters (AD:4560-4576, 4930-4946), each contribution in the order the
+// reference's loops deliver it to the cell.
+template <bool OPT, int NT, bool FAST>
+__device__ __forceinline__ void cell_rhs(const Dims &d, const StepScalars &sc,
+                                         const SharedFields &f, const TracerSet &ts, long x,
+                                         int tx, int ty, const double (*FU)[TY + 1][TX],
+                                         const double (*FV)[TY][TX + 1], double (&TIo)[NT]) {
+    const double dt = sc.dt;
     const uint8_t mk = f.mask[x];
     const double vk = f.V[x];
     const bool open = mk & M_OPEN;
@@ -438,19 +454,59 @@ explicit_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
             if (cfv0) TI = TI + FV[t][ty][tx] * dt / vk;
             if (cfv1) TI = TI - FV[t][ty][tx + 1] * dt / vk;
         }
-        ts.TI[x * s + t] = TI;
+        TIo[t] = TI;
     }
 }
 
+// Scratch of the column solve, tile-blocked: level k, tracer t, sorted lane q
+// of tile `tile` at [((tile * nK + k) * NT + t) * TC + q], so that the stores
+// and loads of a wave are contiguous 512-byte runs.
+__device__ __forceinline__ long blk(long tile, int nK, int k, int NT, int t) {
+    return ((tile * nK + k) * NT + t) * (long)TC;
+}
+
+// Kernel H of the split variant: one block per (level, tile), TI to the
+// tile-blocked scratch; blocks are placed on the XCDs by xcd_level_tile.
+template <bool OPT, int NT, bool FAST>
+__global__ void __launch_bounds__(TC)
+explicit_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts, double *__restrict__ TIb,
+                int tiles_x, long ntiles, unsigned int *__restrict__ badflag) {
+    __shared__ double FU[NT][TY + 1][TX];
+    __shared__ double FV[NT][TY][TX + 1];
+    const int tid = threadIdx.x;
+    long tile;
+    int lev;
+    xcd_level_tile(blockIdx.x, gridDim.x, ntiles, d.wKUB - d.wKLB + 1, tile, lev);
+    const int i0 = d.wILB + (int)(tile % tiles_x) * TX;
+    const int j0 = d.wJLB + (int)(tile / tiles_x) * TY;
+    const int k = d.wKLB + lev;
+    const int nK = d.KUB - d.KLB + 1;
+    const int nc = f.ccnt[tile * nK + k];    // active cells of this level (sorted prefix)
+    if (nc == 0) return;
+    int bad = 0;
+    if (!sc.vertical1d) {
+        tile_faces<OPT, NT, FAST>(d, sc, f, ts, tile, i0, j0, k, f.fcnt[tile * nK + k], FU, FV,
+                                  bad);
+        __syncthreads();
+    }
+    if (bad) atomicOr(badflag, 1u);
+    if (tid >= nc) return;
+    const int slot = f.cperm[tile * TC + tid];
+    const int tx = slot % TX, ty = slot / TX;
+    const long x = d.idx(i0 + tx, j0 + ty, k);
+    double TI[NT];
+    cell_rhs<OPT, NT, FAST>(d, sc, f, ts, x, tx, ty, FU, FV, TI);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) TIb[blk(tile, nK, k, NT, t) + tid] = TI[t];
+}
+
 // ---------------------------------------------------------------------------
-// Vertical operator and column solve (kernel V).  One lane per water column,
-// walking k from kstart to KUB for NT tracers at once: the rows D/E/F of
+// Vertical operator and column solve.  One lane per water column, walking k
+// from kstart to KUB for NT tracers at once: the rows D/E/F of
 // VerticalDiffusion/VerticalDiffusion2 (AD:2731-2770, 2812-2834, 2894-2906)
 // and VerticalAdvection (+_ExplicitScheme, AD:3059-3120, 3160-3203), the
 // null_real land rows (AD:1753) and the THOMASZ_NewType2 forward sweep
-// (MF:4090-4099), then the back substitution (MF:4102-4106).  The forward
-// sweep's W goes to a scratch array and G to the output, which the back
-// substitution overwrites with the solution.
+// (MF:4090-4099); the back substitution (MF:4102-4106) follows the walk.
 
 // Coefficients D/E (and C/F) of the W face with stencil levels q..q+3 for
 // NT tracers.  Shared values (Courant number, spacing) are per face.
@@ -514,203 +570,335 @@ __device__ __forceinline__ void vface_coefs(const StepScalars &sc, const double 
     }
 }
 
+// State of one lane's column walk: windows of V, DWZ, mask and P over levels
+// k-1 .. k+2, the coefficients of the face below the current row, and the
+// previous row's forward-sweep W and G.
+template <int NT>
+struct ColumnWalk {
+    double wv[4], wdu[4];
+    uint8_t wm[4];
+    double wp[NT][4];
+    VCoef fb[NT];
+    double Wm[NT], Gm[NT];
+};
+
+// Loads the windows at the first row ks and the coefficients of face ks.
+template <bool OPT, int NT, bool FAST>
+__device__ __forceinline__ void column_begin(const Dims &d, const StepScalars &sc,
+                                             const SharedFields &f, const TracerSet &ts,
+                                             long base, int ks, bool vflux, ColumnWalk<NT> &c,
+                                             int &bad) {
+    const long sk = d.sk;
+    auto inK = [&](int k) { return k >= d.KLB && k <= d.KUB; };
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+        const int kk = ks - 1 + o;
+        const bool in = inK(kk);
+        const long y = base + kk * sk;
+        c.wv[o] = in ? f.V[y] : 1.0;
+        c.wdu[o] = in ? f.DWZ[y] : 0.0;
+        c.wm[o] = in ? f.mask[y] : (uint8_t)0;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) c.wp[t][o] = in ? ts.P[y * ts.s + t] : 0.0;
+    }
+    const int kk = ks - 2;
+    const bool in = inK(kk);
+    const long y = base + kk * sk;
+    double p[NT][4];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        p[t][0] = in ? ts.P[y * ts.s + t] : 0.0;
+        p[t][1] = c.wp[t][0]; p[t][2] = c.wp[t][1]; p[t][3] = c.wp[t][2];
+    }
+    const double v4[4] = {in ? f.V[y] : 1.0, c.wv[0], c.wv[1], c.wv[2]};
+    const double du4[4] = {in ? f.DWZ[y] : 0.0, c.wdu[0], c.wdu[1], c.wdu[2]};
+    const uint8_t m0 = in ? f.mask[y] : (uint8_t)0;
+    const bool o4[4] = {(m0 & M_OPEN) != 0, (c.wm[0] & M_OPEN) != 0, (c.wm[1] & M_OPEN) != 0,
+                        (c.wm[2] & M_OPEN) != 0};
+    vface_coefs<OPT, NT, FAST>(sc, p, v4, du4, o4, inK(ks) ? f.Qz[base + ks * sk] : 0.0,
+                               vflux && ks <= d.wKUB, c.fb, bad);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) { c.Wm[t] = 0.0; c.Gm[t] = 0.0; }
+}
+
+// Row k of the column: assembles D/E/F and completes TI with the vertical
+// terms, then one forward-sweep step of THOMASZ_NewType2; slides the
+// windows up one level.
+template <bool OPT, int NT, bool FAST>
+__device__ __forceinline__ void column_row(const Dims &d, const StepScalars &sc,
+                                           const SharedFields &f, const TracerSet &ts, long base,
+                                           int k, bool vflux, ColumnWalk<NT> &c,
+                                           const double (&TIk)[NT], double (&Wo)[NT],
+                                           double (&Go)[NT],
+                                           unsigned long long *__restrict__ zero_pivots,
+                                           int &bad) {
+    const long sk = d.sk;
+    const int KUB = d.wKUB;
+    const double dt = sc.dt;
+    auto inK = [&](int kk) { return kk >= d.KLB && kk <= d.KUB; };
+    const long x = base + k * sk;
+    const double vk = c.wv[1];
+    const uint8_t mk = c.wm[1];
+    // coefficients of the face above (k, k+1): stencil k-1 .. k+2
+    VCoef ft[NT];
+    {
+        const bool o4[4] = {(c.wm[0] & M_OPEN) != 0, (c.wm[1] & M_OPEN) != 0,
+                            (c.wm[2] & M_OPEN) != 0, (c.wm[3] & M_OPEN) != 0};
+        vface_coefs<OPT, NT, FAST>(sc, c.wp, c.wv, c.wdu, o4, inK(k + 1) ? f.Qz[x + sk] : 0.0,
+                                   vflux && k < KUB, ft, bad);
+    }
+    // row values shared by all tracers
+    const double dtk = FAST ? dt * frcp(vk) : dt / vk;
+    double E0 = 1.0;
+    if (k == KUB && (mk & M_OPEN)) E0 = 1.0 + dtk * f.Qz[x + sk];
+    const bool cfwB = sc.vert && (mk & M_CFW);
+    const bool cfwT = sc.vert && k < KUB && (c.wm[2] & M_CFW);
+    const double a2 = cfwB ? (FAST ? f.cw[x] * dtk : f.cw[x] * dt / vk) : 0.0;
+    const double a1 = cfwT ? (FAST ? f.cw[x + sk] * dtk : f.cw[x + sk] * dt / vk) : 0.0;
+    const bool land = mk & M_LAND;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const double pk = c.wp[t][1], pm = c.wp[t][0], pp = c.wp[t][2];
+        double TI = TIk[t];
+        double E = E0, D = 0.0, F = 0.0;
+        if (cfwB) {
+            if (OPT) {
+                if (sc.thV > 0.0) { D = D - a2; E = E + a2; }
+                else TI = TI - a2 * (pk - pm);
+            } else {
+                D = D - a2 * sc.thV;
+                E = E + a2 * sc.thV;
+                TI = TI - a2 * (pk - pm) * (1.0 - sc.thV);
+            }
+        }
+        if (cfwT) {
+            if (OPT) {
+                if (sc.thV > 0.0) { E = E + a1; F = F - a1; }
+                else TI = TI + a1 * (pp - pk);
+            } else {
+                E = E + a1 * sc.thV;
+                F = F - a1 * sc.thV;
+                TI = TI + a1 * (pp - pk) * (1.0 - sc.thV);
+            }
+        }
+        if (sc.vert && !sc.vertical1d) {
+            if (sc.impAdvV) {
+                if (cfwB) { D = D - c.fb[t].d * dtk; E = E - c.fb[t].e * dtk; }
+                if (cfwT) { E = E + ft[t].d * dtk; F = F + ft[t].e * dtk; }
+            } else if (OPT) {
+                if (FAST) {
+                    if (cfwB) TI = TI + (c.fb[t].d * pm + c.fb[t].e * pk) * dtk;
+                    if (cfwT) TI = TI - (ft[t].d * pk + ft[t].e * pp) * dtk;
+                } else {
+                    if (cfwB) TI = TI + (c.fb[t].d * pm + c.fb[t].e * pk) * dt / vk;
+                    if (cfwT) TI = TI - (ft[t].d * pk + ft[t].e * pp) * dt / vk;
+                }
+            } else {
+                if (cfwB) {
+                    const double pmm =
+                        (c.fb[t].c == 0.0 || k - 2 < d.KLB) ? 0.0 : ts.P[(x - 2 * sk) * ts.s + t];
+                    const double fl =
+                        c.fb[t].c * pmm + c.fb[t].d * pm + c.fb[t].e * pk + c.fb[t].f * pp;
+                    TI = TI + fl * dt / vk;
+                }
+                if (cfwT) {
+                    const double fl = ft[t].c * pm + ft[t].d * pk + ft[t].e * pp +
+                                      ft[t].f * c.wp[t][3];
+                    TI = TI - fl * dt / vk;
+                }
+            }
+        }
+        if (land) TI = MGPU_NULL_REAL;
+        // THOMASZ_NewType2 forward sweep (MF:4090-4099)
+        const double aux = E + D * c.Wm[t];
+        double W = 0.0, G = 0.0;
+        if (fabs(aux) > 0.0) {
+            if (FAST) {
+                const double ia = frcp(aux);
+                W = -F * ia;
+                G = (TI - D * c.Gm[t]) * ia;
+            } else {
+                W = -F / aux;
+                G = (TI - D * c.Gm[t]) / aux;
+            }
+        } else {
+            atomicAdd(zero_pivots, 1ull);
+        }
+        Wo[t] = W;
+        Go[t] = G;
+        c.Wm[t] = W;
+        c.Gm[t] = G;
+        c.fb[t] = ft[t];
+    }
+    // slide the windows up one level
+    const int kn = k + 3;
+    const bool in = inK(kn);
+    const long y = base + kn * sk;
+#pragma unroll
+    for (int o = 0; o < 3; ++o) { c.wv[o] = c.wv[o + 1]; c.wdu[o] = c.wdu[o + 1]; c.wm[o] = c.wm[o + 1]; }
+    c.wv[3] = in ? f.V[y] : 1.0;
+    c.wdu[3] = in ? f.DWZ[y] : 0.0;
+    c.wm[3] = in ? f.mask[y] : (uint8_t)0;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        c.wp[t][0] = c.wp[t][1]; c.wp[t][1] = c.wp[t][2]; c.wp[t][2] = c.wp[t][3];
+        c.wp[t][3] = in ? ts.P[y * ts.s + t] : 0.0;
+    }
+}
+
+// Ghost top row KUB+1 of a solved column: E = 1, D = F = 0, TI = 0 or
+// null_real (AD:1662-1695, 1753), in the forward sweep's arithmetic.
+template <int NT>
+__device__ __forceinline__ void column_ghost(const Dims &d, const SharedFields &f,
+                                             const TracerSet &ts, long base,
+                                             const ColumnWalk<NT> &c, double (&xs)[NT]) {
+    const long xg = base + (d.wKUB + 1) * d.sk;
+    const double TIg = (f.mask[xg] & M_LAND) ? MGPU_NULL_REAL : 0.0;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const double aux = 1.0 + 0.0 * c.Wm[t];
+        xs[t] = (TIg - 0.0 * c.Gm[t]) / aux;
+        ts.out[xg * ts.s + t] = xs[t];
+    }
+}
+
+// Kernel V of the split variant: TI from the tile-blocked scratch; the
+// forward sweep's W overwrites TI in place (each lane reads TI(k) before
+// writing W(k)) and G goes to a second blocked array.
 template <bool OPT, int NT, bool FAST>
 __global__ void __launch_bounds__(TC)
-vertical_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
+vertical_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts, double *__restrict__ TWb,
+                double *__restrict__ Gb, int tiles_x,
                 unsigned long long *__restrict__ zero_pivots, unsigned int *__restrict__ badflag) {
-    const long tile = blockIdx.x + (long)gridDim.x * blockIdx.y;
+    const long tile = xcd_tile(blockIdx.x, gridDim.x);
     const int nK = d.KUB - d.KLB + 1;
     const int KUB = d.wKUB;
     // THOMASZ_NewType2 solves columns with WaterPoints3D(i,j,KUB) == 1 (MF:4086):
     // those are the columns whose sort key is <= KUB+1; lanes take them in
     // order of kstart
     if ((int)threadIdx.x >= f.ccnt[tile * nK + KUB + 1]) return;
-    const int slot = f.cperm[tile * TC + threadIdx.x];
-    const int i = d.wILB + blockIdx.x * TX + slot % TX;
-    const int j = d.wJLB + blockIdx.y * TY + slot / TX;
+    const int q = threadIdx.x;
+    const int slot = f.cperm[tile * TC + q];
+    const int i = d.wILB + (int)(tile % tiles_x) * TX + slot % TX;
+    const int j = d.wJLB + (int)(tile / tiles_x) * TY + slot / TX;
     const int ks = f.kstart[d.idx2(i, j)];
-    const double dt = sc.dt;
     const long sk = d.sk;
     const long base = d.idx(i, j, 0);   // index of level 0; level k is base + k*sk
     int bad = 0;
-
     for (int k = d.wKLB; k < ks && k <= KUB; ++k) {
 #pragma unroll
         for (int t = 0; t < NT; ++t) ts.out[(base + k * sk) * ts.s + t] = MGPU_NULL_REAL;
     }
     const bool columnBuild = f.mask[base + KUB * sk] & M_OPEN;  // AD:2982/3004
     const bool vflux = sc.vert && !sc.vertical1d && columnBuild;
-
-    auto inK = [&](int k) { return k >= d.KLB && k <= d.KUB; };
-    // windows over levels k-1 .. k+2
-    double wv[4], wdu[4];
-    uint8_t wm[4];
-    double wp[NT][4];
-#pragma unroll
-    for (int o = 0; o < 4; ++o) {
-        const int kk = ks - 1 + o;
-        const bool in = inK(kk);
-        const long y = base + kk * sk;
-        wv[o] = in ? f.V[y] : 1.0;
-        wdu[o] = in ? f.DWZ[y] : 0.0;
-        wm[o] = in ? f.mask[y] : (uint8_t)0;
-#pragma unroll
-        for (int t = 0; t < NT; ++t) wp[t][o] = in ? ts.P[y * ts.s + t] : 0.0;
-    }
-    // coefficients of face ks (stencil ks-2 .. ks+1)
-    VCoef fb[NT];
-    {
-        const int kk = ks - 2;
-        const bool in = inK(kk);
-        const long y = base + kk * sk;
-        double p[NT][4];
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            p[t][0] = in ? ts.P[y * ts.s + t] : 0.0;
-            p[t][1] = wp[t][0]; p[t][2] = wp[t][1]; p[t][3] = wp[t][2];
-        }
-        const double v4[4] = {in ? f.V[y] : 1.0, wv[0], wv[1], wv[2]};
-        const double du4[4] = {in ? f.DWZ[y] : 0.0, wdu[0], wdu[1], wdu[2]};
-        const uint8_t m0 = in ? f.mask[y] : (uint8_t)0;
-        const bool o4[4] = {(m0 & M_OPEN) != 0, (wm[0] & M_OPEN) != 0, (wm[1] & M_OPEN) != 0,
-                            (wm[2] & M_OPEN) != 0};
-        vface_coefs<OPT, NT, FAST>(sc, p, v4, du4, o4, inK(ks) ? f.Qz[base + ks * sk] : 0.0,
-                             vflux && ks <= KUB, fb, bad);
-    }
-
-    double Wm[NT], Gm[NT];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) { Wm[t] = 0.0; Gm[t] = 0.0; }
+    ColumnWalk<NT> c;
+    column_begin<OPT, NT, FAST>(d, sc, f, ts, base, ks, vflux, c, bad);
     for (int k = ks; k <= KUB; ++k) {
-        const long x = base + k * sk;
-        const double vk = wv[1];
-        const uint8_t mk = wm[1];
-        // coefficients of the face above (k, k+1): stencil k-1 .. k+2
-        VCoef ft[NT];
-        {
-            const bool o4[4] = {(wm[0] & M_OPEN) != 0, (wm[1] & M_OPEN) != 0,
-                                (wm[2] & M_OPEN) != 0, (wm[3] & M_OPEN) != 0};
-            vface_coefs<OPT, NT, FAST>(sc, wp, wv, wdu, o4, inK(k + 1) ? f.Qz[x + sk] : 0.0,
-                                 vflux && k < KUB, ft, bad);
-        }
-        // row values shared by all tracers
-        const double dtk = FAST ? dt * frcp(vk) : dt / vk;
-        double E0 = 1.0;
-        if (k == KUB && (mk & M_OPEN)) E0 = 1.0 + dtk * f.Qz[x + sk];
-        const bool cfwB = sc.vert && (mk & M_CFW);
-        const bool cfwT = sc.vert && k < KUB && (wm[2] & M_CFW);
-        const double a2 = cfwB ? (FAST ? f.cw[x] * dtk : f.cw[x] * dt / vk) : 0.0;
-        const double a1 = cfwT ? (FAST ? f.cw[x + sk] * dtk : f.cw[x + sk] * dt / vk) : 0.0;
-        const bool land = mk & M_LAND;
+        double TI[NT], W[NT], G[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) TI[t] = TWb[blk(tile, nK, k, NT, t) + q];
+        column_row<OPT, NT, FAST>(d, sc, f, ts, base, k, vflux, c, TI, W, G, zero_pivots, bad);
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const double pk = wp[t][1], pm = wp[t][0], pp = wp[t][2];
-            double TI = ts.TI[x * ts.s + t];
-            double E = E0, D = 0.0, F = 0.0;
-            if (cfwB) {
-                if (OPT) {
-                    if (sc.thV > 0.0) { D = D - a2; E = E + a2; }
-                    else TI = TI - a2 * (pk - pm);
-                } else {
-                    D = D - a2 * sc.thV;
-                    E = E + a2 * sc.thV;
-                    TI = TI - a2 * (pk - pm) * (1.0 - sc.thV);
-                }
-            }
-            if (cfwT) {
-                if (OPT) {
-                    if (sc.thV > 0.0) { E = E + a1; F = F - a1; }
-                    else TI = TI + a1 * (pp - pk);
-                } else {
-                    E = E + a1 * sc.thV;
-                    F = F - a1 * sc.thV;
-                    TI = TI + a1 * (pp - pk) * (1.0 - sc.thV);
-                }
-            }
-            if (sc.vert && !sc.vertical1d) {
-                if (sc.impAdvV) {
-                    if (cfwB) { D = D - fb[t].d * dtk; E = E - fb[t].e * dtk; }
-                    if (cfwT) { E = E + ft[t].d * dtk; F = F + ft[t].e * dtk; }
-                } else if (OPT) {
-                    if (FAST) {
-                        if (cfwB) TI = TI + (fb[t].d * pm + fb[t].e * pk) * dtk;
-                        if (cfwT) TI = TI - (ft[t].d * pk + ft[t].e * pp) * dtk;
-                    } else {
-                        if (cfwB) TI = TI + (fb[t].d * pm + fb[t].e * pk) * dt / vk;
-                        if (cfwT) TI = TI - (ft[t].d * pk + ft[t].e * pp) * dt / vk;
-                    }
-                } else {
-                    if (cfwB) {
-                        const double pmm =
-                            (fb[t].c == 0.0 || k - 2 < d.KLB) ? 0.0
-                                                              : ts.P[(x - 2 * sk) * ts.s + t];
-                        const double fl = fb[t].c * pmm + fb[t].d * pm + fb[t].e * pk + fb[t].f * pp;
-                        TI = TI + fl * dt / vk;
-                    }
-                    if (cfwT) {
-                        const double fl = ft[t].c * pm + ft[t].d * pk + ft[t].e * pp +
-                                          ft[t].f * wp[t][3];
-                        TI = TI - fl * dt / vk;
-                    }
-                }
-            }
-            if (land) TI = MGPU_NULL_REAL;
-            // THOMASZ_NewType2 forward sweep (MF:4090-4099)
-            const double aux = E + D * Wm[t];
-            double W = 0.0, G = 0.0;
-            if (fabs(aux) > 0.0) {
-                if (FAST) {
-                    const double ia = frcp(aux);
-                    W = -F * ia;
-                    G = (TI - D * Gm[t]) * ia;
-                } else {
-                    W = -F / aux;
-                    G = (TI - D * Gm[t]) / aux;
-                }
-            } else {
-                atomicAdd(zero_pivots, 1ull);
-            }
-            ts.W[x * ts.s + t] = W;
-            ts.out[x * ts.s + t] = G;
-            Wm[t] = W;
-            Gm[t] = G;
-            fb[t] = ft[t];
-        }
-        // slide the windows up one level
-        const int kn = k + 3;
-        const bool in = inK(kn);
-        const long y = base + kn * sk;
-#pragma unroll
-        for (int o = 0; o < 3; ++o) { wv[o] = wv[o + 1]; wdu[o] = wdu[o + 1]; wm[o] = wm[o + 1]; }
-        wv[3] = in ? f.V[y] : 1.0;
-        wdu[3] = in ? f.DWZ[y] : 0.0;
-        wm[3] = in ? f.mask[y] : (uint8_t)0;
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            wp[t][0] = wp[t][1]; wp[t][1] = wp[t][2]; wp[t][2] = wp[t][3];
-            wp[t][3] = in ? ts.P[y * ts.s + t] : 0.0;
+            TWb[blk(tile, nK, k, NT, t) + q] = W[t];
+            Gb[blk(tile, nK, k, NT, t) + q] = G[t];
         }
     }
-    // ghost top row KUB+1: E = 1, D = F = 0, TI = 0 or null_real (AD:1662-1695, 1753)
-    const long xg = base + (KUB + 1) * sk;
-    const double TIg = (f.mask[xg] & M_LAND) ? MGPU_NULL_REAL : 0.0;
     double xs[NT];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const double aux = 1.0 + 0.0 * Wm[t];
-        xs[t] = (TIg - 0.0 * Gm[t]) / aux;
-        ts.out[xg * ts.s + t] = xs[t];
-    }
+    column_ghost<NT>(d, f, ts, base, c, xs);
     // back substitution (MF:4102-4106)
     for (int k = KUB; k >= ks; --k) {
         const long x = base + k * sk;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            xs[t] = ts.W[x * ts.s + t] * xs[t] + ts.out[x * ts.s + t];
+            xs[t] = TWb[blk(tile, nK, k, NT, t) + q] * xs[t] + Gb[blk(tile, nK, k, NT, t) + q];
             ts.out[x * ts.s + t] = xs[t];
         }
     }
     if (bad) atomicOr(badflag, 1u);
+}
+
+// ---------------------------------------------------------------------------
+// Fused variant: one block per tile walks the levels bottom-up.  At each
+// level the block computes the tile's face fluxes into LDS, then every active
+// column builds its explicit right-hand side in registers and takes one
+// forward-sweep step; W and G go to a tile-blocked scratch
+//   level k, tracer t, lane q  ->  [((tile * nK + k) * NT + t) * TC + q]
+// so every store and load of a wave is one contiguous 512-byte run.  The
+// back substitution reads them back and writes each cell's solution once.
+
+template <bool OPT, int NT, bool FAST>
+__global__ void __launch_bounds__(TC)
+column_tile_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts, double *__restrict__ Wb,
+                   double *__restrict__ Gb, int tiles_x,
+                   unsigned long long *__restrict__ zero_pivots,
+                   unsigned int *__restrict__ badflag) {
+    __shared__ double FU[NT][TY + 1][TX];
+    __shared__ double FV[NT][TY][TX + 1];
+    const long tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int i0 = d.wILB + (int)(tile % tiles_x) * TX;
+    const int j0 = d.wJLB + (int)(tile / tiles_x) * TY;
+    const int tid = threadIdx.x;
+    const int nK = d.KUB - d.KLB + 1;
+    const int KUB = d.wKUB;
+    const uint16_t *cc = f.ccnt + tile * nK, *fc = f.fcnt + tile * nK;
+    // solved columns (WaterPoints3D(i,j,KUB) == 1, MF:4086) are the sorted
+    // prefix of keys <= KUB+1; their lanes keep the same column at every level
+    const bool act = tid < cc[KUB + 1];
+    int ks = KUB + 1, tx = 0, ty = 0;
+    long base = 0;
+    if (act) {
+        const int slot = f.cperm[tile * TC + tid];
+        tx = slot % TX;
+        ty = slot / TX;
+        base = d.idx(i0 + tx, j0 + ty, 0);
+        ks = f.kstart[d.idx2(i0 + tx, j0 + ty)];
+    }
+    const long sk = d.sk;
+    const bool vflux = act && sc.vert && !sc.vertical1d && (f.mask[base + KUB * sk] & M_OPEN);
+    int bad = 0;
+    ColumnWalk<NT> c;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) { c.Wm[t] = 0.0; c.Gm[t] = 0.0; }
+    double *Wt = Wb + blk(tile, nK, 0, NT, 0) + tid;
+    double *Gt = Gb + blk(tile, nK, 0, NT, 0) + tid;
+    for (int k = d.wKLB; k <= KUB; ++k) {
+        const int nc = cc[k];                        // uniform over the block
+        const int nf = sc.vertical1d ? 0 : fc[k];
+        if (nc == 0 && nf == 0) continue;
+        if (nf) tile_faces<OPT, NT, FAST>(d, sc, f, ts, tile, i0, j0, k, nf, FU, FV, bad);
+        __syncthreads();
+        if (tid < nc) {                              // implies act and ks <= k
+            if (k == ks) column_begin<OPT, NT, FAST>(d, sc, f, ts, base, ks, vflux, c, bad);
+            double TI[NT], W[NT], G[NT];
+            cell_rhs<OPT, NT, FAST>(d, sc, f, ts, base + k * sk, tx, ty, FU, FV, TI);
+            column_row<OPT, NT, FAST>(d, sc, f, ts, base, k, vflux, c, TI, W, G, zero_pivots,
+                                      bad);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                Wt[((long)k * NT + t) * TC] = W[t];
+                Gt[((long)k * NT + t) * TC] = G[t];
+            }
+        }
+        __syncthreads();
+    }
+    if (bad) atomicOr(badflag, 1u);
+    if (!act) return;
+    for (int k = d.wKLB; k < ks && k <= KUB; ++k) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) ts.out[(base + k * sk) * ts.s + t] = MGPU_NULL_REAL;
+    }
+    double xs[NT];
+    column_ghost<NT>(d, f, ts, base, c, xs);
+    // back substitution (MF:4102-4106)
+    for (int k = KUB; k >= ks; --k) {
+        const long x = base + k * sk;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            xs[t] = Wt[((long)k * NT + t) * TC] * xs[t] + Gt[((long)k * NT + t) * TC];
+            ts.out[x * ts.s + t] = xs[t];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -805,83 +993,89 @@ int launch_prep_faces(const Dims &d, const double *visc, const double *diffv, co
     return MGPU_SUCCESS;
 }
 
-// Launch kernels H and V for `n` tracers (chunks of up to MAXT per launch).
-// phase bit 1: kernel H, bit 2: kernel V
+long blocked_scratch_len(const Dims &d, int nt) {
+    return tile_count(d) * (long)(d.KUB - d.KLB + 1) * nt * TC;
+}
+
+// Launch the transport kernels for one chunk of NT tracers.
+// Fused variant: phase bit 1 launches the column-tile kernel.  Split
+// variant: phase bit 1 = kernel H, bit 2 = kernel V.
 template <bool OPT, int NT, bool FAST>
-static int launch_chunk(int phase, const Dims &d, const StepScalars &sc, const SharedFields &f,
-                        const TracerSet &ts, unsigned long long *counters, int tile_i,
-                        int tile_j, hipStream_t s) {
+static int launch_chunk(int variant, int phase, const Dims &d, const StepScalars &sc,
+                        const SharedFields &f, const TracerSet &ts, double *Wb, double *Gb,
+                        unsigned long long *counters, hipStream_t s) {
     const int ni = d.wIUB - d.wILB + 1, nj = d.wJUB - d.wJLB + 1, nk = d.wKUB - d.wKLB + 1;
     unsigned int *bad = reinterpret_cast<unsigned int *>(counters + 1);
-    (void)tile_i;
-    (void)tile_j;
     const dim3 tiles((ni + TX - 1) / TX, (nj + TY - 1) / TY);
+    if (variant == MGPU_KERNEL_FUSED) {
+        if (phase & 1) {
+            hipLaunchKernelGGL((column_tile_kernel<OPT, NT, FAST>), dim3(tiles.x * tiles.y),
+                               dim3(TC), 0, s, d, sc, f, ts, Wb, Gb, (int)tiles.x, counters, bad);
+            HIPCHK(hipGetLastError());
+        }
+        return MGPU_SUCCESS;
+    }
+    const long ntiles = (long)tiles.x * tiles.y;
     if (phase & 1) {
-        hipLaunchKernelGGL((explicit_kernel<OPT, NT, FAST>), dim3(tiles.x, tiles.y, nk), dim3(TC), 0,
-                           s, d, sc, f, ts, bad);
+        hipLaunchKernelGGL((explicit_kernel<OPT, NT, FAST>), dim3((unsigned)(ntiles * nk)),
+                           dim3(TC), 0, s, d, sc, f, ts, Wb, (int)tiles.x, ntiles, bad);
         HIPCHK(hipGetLastError());
     }
     if (phase & 2) {
-        hipLaunchKernelGGL((vertical_kernel<OPT, NT, FAST>), tiles, dim3(TC), 0, s, d, sc, f, ts,
-                           counters, bad);
+        hipLaunchKernelGGL((vertical_kernel<OPT, NT, FAST>), dim3((unsigned)ntiles), dim3(TC), 0,
+                           s, d, sc, f, ts, Wb, Gb, (int)tiles.x, counters, bad);
         HIPCHK(hipGetLastError());
     }
     return MGPU_SUCCESS;
 }
 
 template <bool OPT, bool FAST>
-static int launch_n(int nt, int ph, const Dims &d, const StepScalars &sc, const SharedFields &f,
-                    const TracerSet &ts, unsigned long long *c, int ti, int tj, hipStream_t s) {
+static int launch_n(int nt, int variant, int ph, const Dims &d, const StepScalars &sc,
+                    const SharedFields &f, const TracerSet &ts, double *Wb, double *Gb,
+                    unsigned long long *c, hipStream_t s) {
     switch (nt) {
-        case 1: return launch_chunk<OPT, 1, FAST>(ph, d, sc, f, ts, c, ti, tj, s);
-        case 2: return launch_chunk<OPT, 2, FAST>(ph, d, sc, f, ts, c, ti, tj, s);
-        case 3: return launch_chunk<OPT, 3, FAST>(ph, d, sc, f, ts, c, ti, tj, s);
-        case 4: return launch_chunk<OPT, 4, FAST>(ph, d, sc, f, ts, c, ti, tj, s);
-        case 5: return launch_chunk<OPT, 5, FAST>(ph, d, sc, f, ts, c, ti, tj, s);
-        case 6: return launch_chunk<OPT, 6, FAST>(ph, d, sc, f, ts, c, ti, tj, s);
-        case 7: return launch_chunk<OPT, 7, FAST>(ph, d, sc, f, ts, c, ti, tj, s);
-        case 8: return launch_chunk<OPT, 8, FAST>(ph, d, sc, f, ts, c, ti, tj, s);
+        case 1: return launch_chunk<OPT, 1, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, c, s);
+        case 2: return launch_chunk<OPT, 2, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, c, s);
+        case 3: return launch_chunk<OPT, 3, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, c, s);
+        case 4: return launch_chunk<OPT, 4, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, c, s);
+        case 5: return launch_chunk<OPT, 5, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, c, s);
+        case 6: return launch_chunk<OPT, 6, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, c, s);
+        case 7: return launch_chunk<OPT, 7, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, c, s);
+        case 8: return launch_chunk<OPT, 8, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, c, s);
         default: return MGPU_ERR_BADARG;
     }
 }
 
-static int launch_transport(bool fast, const Dims &d, const StepScalars &sc,
-                            const SharedFields &f, int n, long stride, const double *P,
-                            double *TI, double *out, double *W, unsigned long long *counters,
-                            int chunk, int tile_i, int tile_j, hipStream_t s, int phase) {
+int launch_transport(const TransportLaunch &L, const Dims &d, const StepScalars &sc,
+                     const SharedFields &f, hipStream_t s, int phase) {
+    int chunk = L.chunk;
     if (chunk <= 0 || chunk > MAXT) chunk = MAXT;
-    if (stride < n) return MGPU_ERR_BADARG;
-    for (int t0 = 0; t0 < n; t0 += chunk) {
-        const int nt = n - t0 < chunk ? n - t0 : chunk;
+    if (L.stride < L.n) return MGPU_ERR_BADARG;
+    if (L.variant != MGPU_KERNEL_FUSED && L.variant != MGPU_KERNEL_SPLIT) return MGPU_ERR_BADARG;
+    if (!L.Wb || !L.Gb || !L.P || !L.out) return MGPU_ERR_BADARG;
+    for (int t0 = 0; t0 < L.n; t0 += chunk) {
+        const int nt = L.n - t0 < chunk ? L.n - t0 : chunk;
         TracerSet ts;
-        ts.P = P + t0; ts.TI = TI + t0; ts.out = out + t0; ts.W = W + t0; ts.s = stride;
+        ts.P = L.P + t0;
+        ts.out = L.out + t0;
+        ts.s = L.stride;
+        // each chunk has its own scratch region: with the split variant all
+        // first kernels run before the second ones
+        double *Wb = L.Wb + blocked_scratch_len(d, t0), *Gb = L.Gb + blocked_scratch_len(d, t0);
         int rc;
         if (sc.optimize)
-            rc = fast ? launch_n<true, true>(nt, phase, d, sc, f, ts, counters, tile_i, tile_j, s)
-                      : launch_n<true, false>(nt, phase, d, sc, f, ts, counters, tile_i, tile_j, s);
+            rc = L.fast ? launch_n<true, true>(nt, L.variant, phase, d, sc, f, ts, Wb, Gb,
+                                               L.counters, s)
+                        : launch_n<true, false>(nt, L.variant, phase, d, sc, f, ts, Wb, Gb,
+                                                L.counters, s);
         else
-            rc = fast ? launch_n<false, true>(nt, phase, d, sc, f, ts, counters, tile_i, tile_j, s)
-                      : launch_n<false, false>(nt, phase, d, sc, f, ts, counters, tile_i, tile_j,
-                                               s);
+            rc = L.fast ? launch_n<false, true>(nt, L.variant, phase, d, sc, f, ts, Wb, Gb,
+                                                L.counters, s)
+                        : launch_n<false, false>(nt, L.variant, phase, d, sc, f, ts, Wb, Gb,
+                                                 L.counters, s);
         if (rc) return rc;
     }
     return MGPU_SUCCESS;
-}
-
-int launch_transport_fp64(const Dims &d, const StepScalars &sc, const SharedFields &f, int n,
-                          long stride, const double *P, double *TI, double *out, double *W,
-                          unsigned long long *counters, int chunk, int tile_i, int tile_j,
-                          hipStream_t s, int phase) {
-    return launch_transport(false, d, sc, f, n, stride, P, TI, out, W, counters, chunk, tile_i,
-                            tile_j, s, phase);
-}
-
-int launch_transport_fast(const Dims &d, const StepScalars &sc, const SharedFields &f, int n,
-                          long stride, const double *P, double *TI, double *out, double *W,
-                          unsigned long long *counters, int chunk, int tile_i, int tile_j,
-                          hipStream_t s, int phase) {
-    return launch_transport(true, d, sc, f, n, stride, P, TI, out, W, counters, chunk, tile_i,
-                            tile_j, s, phase);
 }
 
 int launch_thomasz(const Dims &d, const double *D, const double *E, const double *F,

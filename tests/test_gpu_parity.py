@@ -164,12 +164,16 @@ def test_batched_transport_chunks_match_oracle(per_launch):
         assert np.array_equal(got[t][~solved], case.tracers[t][~solved])
 
 
+@pytest.mark.parametrize("kernel", ["fused", "split"])
 @pytest.mark.parametrize("name", sorted(SMALL))
-def test_batched_transport_matches_oracle(name):
+def test_batched_transport_matches_oracle(name, kernel):
     """Batched FP64 transport on every small case (optimised and plain
-    operation orders, flat, sigma, ragged, land, stiff) for 6 steps."""
+    operation orders, flat, sigma, ragged, land, stiff) for 6 steps, with the
+    fused column-tile kernel and with the split explicit/column kernels."""
+    from mohid_amd import _lib as L
     case = make(name)
-    got = _batched(case, 4, 2)
+    kid = L.MGPU_KERNEL_FUSED if kernel == "fused" else L.MGPU_KERNEL_SPLIT
+    got = _batched(case, 4, 2, kernel=kid)
     ref = ora.run_case(case, 6, nthreads=1)
     solved = _solved_mask(case)
     bitwise = True
@@ -177,7 +181,7 @@ def test_batched_transport_matches_oracle(name):
         err = _rel_err(got[t], ref[t], solved)
         assert err <= RTOL, (name, t, err)
         bitwise &= bool(np.array_equal(got[t], ref[t]))
-    print(f"{name}: batched bit-identical={bitwise}")
+    print(f"{name} ({kernel}): batched bit-identical={bitwise}")
 
 
 # Fast mode: FP64 with reciprocal-multiply division.  Each replaced division

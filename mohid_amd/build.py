@@ -31,6 +31,37 @@ def sources():
     return sorted(glob.glob(os.path.join(CSRC, "*.hip")))
 
 
+FC = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
+FDIR = os.path.join(HERE, "fortran")
+FBUILD = os.path.join(FDIR, "_build")
+DRIVER = os.path.join(FDIR, "adgpu_driver")
+# the reference's production build is default-real-8 (compile_mohid.sh:185)
+FFLAGS = ["-cpp", "-O2", "-fdefault-real-8", "-fdefault-double-8"]
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """The ISO_C_BINDING module (ModuleAdvectionDiffusionGPU) and the test
+    driver program, linked against the HIP library."""
+    os.makedirs(FBUILD, exist_ok=True)
+    mod_o = os.path.join(FBUILD, "ModuleAdvectionDiffusionGPU.o")
+    drv_o = os.path.join(FBUILD, "adgpu_driver.o")
+    steps = [
+        [FC, *FFLAGS, "-c", os.path.join(FDIR, "ModuleAdvectionDiffusionGPU.F90"), "-o", mod_o,
+         "-J", FBUILD],
+        [FC, *FFLAGS, "-c", os.path.join(FDIR, "adgpu_driver.F90"), "-o", drv_o, "-J", FBUILD],
+        [FC, "-o", DRIVER + ".tmp", drv_o, mod_o, f"-L{HERE}", "-lmohid_gpu",
+         "-Wl,-rpath,$ORIGIN/.."],
+    ]
+    for cmd in steps:
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"{' '.join(cmd)} failed:\n{res.stdout}{res.stderr}")
+        if verbose and (res.stdout or res.stderr).strip():
+            print(res.stdout + res.stderr)
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
+
+
 def build(verbose: bool = False) -> str:
     tmp = OUT + ".tmp"
     cmd = [HIPCC, *FLAGS, "-o", tmp, *sources()]
@@ -47,4 +78,6 @@ def build(verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     print(build(verbose=True))
+    if os.path.exists(FC):
+        print(build_fortran(verbose=True))
     sys.exit(0)

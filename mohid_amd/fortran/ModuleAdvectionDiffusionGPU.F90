@@ -34,6 +34,10 @@ module ModuleAdvectionDiffusionGPU
     integer, parameter :: MGPU_ERR_OPTION      = 102
     integer, parameter :: MGPU_ERR_UNSUPPORTED = 103
 
+    ! AdvectionDiffusion boundary-condition ids (ModuleAdvectionDiffusion.F90:152-158)
+    integer, parameter :: NullGradient_   = 4
+    integer, parameter :: CyclicBoundary_ = 8
+
     type, bind(c) :: t_mgpu_size3d
         integer(c_int32_t) :: ILB, IUB, JLB, JUB, KLB, KUB
     end type t_mgpu_size3d
@@ -168,8 +172,13 @@ contains
 
     !> AdvectionDiffusion (ModuleAdvectionDiffusion.F90:1108-1198): PROP is
     !> advanced by DTProp in place.  Arguments keep the reference's names,
-    !> order and meaning; options the GPU path does not offload are passed
-    !> as present/absent flags and make STAT = MGPU_ERR_UNSUPPORTED.
+    !> order and meaning.  Options the GPU path does not offload make
+    !> STAT = MGPU_ERR_UNSUPPORTED and leave PROP untouched:
+    !>   ReferenceProp present (open boundary, AD:5816-5830), BoundaryCondition
+    !>   NullGradient_ or CyclicBoundary_ (post-solve passes, AD:1874-1882),
+    !>   CellFluxes, any small-depth column (AD:1579-1590), NoAdvFlux or
+    !>   NoDifFlux true (AD:1321-1330), Discharges (SetDischarges was called,
+    !>   WP:14760-14772), AdvectionNudging.
     subroutine AdvectionDiffusionGPU(ID, PROP, schmidt_H, SchmidtCoef_V, SchmidtBackground_V, &
                                      AdvMethodH, TVDLimitationH, AdvMethodV, TVDLimitationV,  &
                                      Upwind2H, Upwind2V, VolumeRelMax, AdvectionNudging,      &
@@ -178,8 +187,9 @@ contains
                                      Wflux_X, Wflux_Y, Wflux_Z, VolumeZOld, VolumeZ,          &
                                      OpenPoints3D, LandPoints3D, ComputeFacesU3D,             &
                                      ComputeFacesV3D, ComputeFacesW3D, Visc_H, Diff_V,        &
-                                     WaterPoints3D, CellFluxes, Discharges, Optimize,         &
-                                     FirstProperty, STAT)
+                                     WaterPoints3D, CellFluxes, ReferenceProp,                &
+                                     BoundaryCondition, SmallDepths, NoAdvFlux, NoDifFlux,    &
+                                     Discharges, Optimize, FirstProperty, STAT)
         integer,                           intent(in)    :: ID
         real(8), dimension(:,:,:), contiguous, target, intent(inout) :: PROP
         real(8),                           intent(in)    :: schmidt_H, SchmidtCoef_V
@@ -203,6 +213,10 @@ contains
         real(8), dimension(:,:,:), contiguous, target, intent(in)    :: Visc_H, Diff_V
         integer, dimension(:,:,:), contiguous, target, intent(in)    :: WaterPoints3D
         logical, optional,                 intent(in)    :: CellFluxes, Discharges
+        real(8), dimension(:,:,:), optional, intent(in)  :: ReferenceProp
+        integer, optional,                 intent(in)    :: BoundaryCondition
+        logical, dimension(:,:), optional, intent(in)    :: SmallDepths
+        logical, optional,                 intent(in)    :: NoAdvFlux, NoDifFlux
         logical, optional,                 intent(in)    :: Optimize, FirstProperty
         integer, optional,                 intent(out)   :: STAT
         type(t_mgpu_ad_args) :: a
@@ -239,10 +253,18 @@ contains
         a%unsupported_CellFluxes = 0
         if (present(CellFluxes)) a%unsupported_CellFluxes = b2i(CellFluxes)
         a%unsupported_ReferenceProp = 0
+        if (present(ReferenceProp)) a%unsupported_ReferenceProp = 1
         a%unsupported_BoundaryCondition = 0
+        if (present(BoundaryCondition)) then
+            if (BoundaryCondition == NullGradient_ .or. BoundaryCondition == CyclicBoundary_) &
+                a%unsupported_BoundaryCondition = 1
+        endif
         a%unsupported_SmallDepths = 0
+        if (present(SmallDepths)) a%unsupported_SmallDepths = b2i(any(SmallDepths))
         a%unsupported_NoAdvFlux = 0
+        if (present(NoAdvFlux)) a%unsupported_NoAdvFlux = b2i(NoAdvFlux)
         a%unsupported_NoDifFlux = 0
+        if (present(NoDifFlux)) a%unsupported_NoDifFlux = b2i(NoDifFlux)
         a%unsupported_Discharges = 0
         if (present(Discharges)) a%unsupported_Discharges = b2i(Discharges)
         a%device_pointers = 0

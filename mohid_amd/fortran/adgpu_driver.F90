@@ -1,0 +1,78 @@
+!------------------------------------------------------------------------------
+! adgpu_driver: a Fortran caller of ModuleAdvectionDiffusionGPU, the way
+! ModuleWaterProperties calls AdvectionDiffusion (ModuleWaterProperties.F90:
+! 4833 Start, 14776 per-property step, 27844 Kill), driven from a binary
+! input file so that the test suite can compare its result with the oracle.
+!
+!   adgpu_driver <input> <output>
+!
+! Input (stream, native byte order): Size(6), WorkSize(6), nsteps,
+! iparam(8) = AdvMethodH, TVDLimitationH, AdvMethodV, TVDLimitationV,
+!             Upwind2H, Upwind2V, NullDif, Optimize (0/1),
+! rparam(10) = schmidt_H, SchmidtCoef_V, SchmidtBackground_V, VolumeRelMax,
+!              DTProp, ImpExp_AdvV, ImpExp_DifV, ImpExp_AdvXX, ImpExp_AdvYY,
+!              ImpExp_DifH,
+! then the arrays over Size in Fortran order: DUX, DVY, DZX, DZY (2-D real),
+! KFloorZ (2-D int), AreaU, AreaV, DZZ, DWZ, Wflux_X, Wflux_Y, Wflux_Z,
+! VolumeZOld, VolumeZ (3-D real), OpenPoints3D, LandPoints3D,
+! ComputeFacesU3D, ComputeFacesV3D, ComputeFacesW3D (3-D int), Visc_H, Diff_V
+! (3-D real), WaterPoints3D (3-D int), PROP (3-D real).
+! Output: PROP after nsteps steps.  Exit status 0 on success, else the STAT.
+!------------------------------------------------------------------------------
+program adgpu_driver
+
+    use ModuleAdvectionDiffusionGPU
+    implicit none
+
+    character(len=1024) :: fin, fout
+    integer :: u, s, ID, STAT, nsteps
+    integer :: sz(6), ws(6), ipar(8)
+    real(8) :: rpar(10)
+    real(8), allocatable, dimension(:,:)   :: DUX, DVY, DZX, DZY
+    integer, allocatable, dimension(:,:)   :: KFloorZ
+    real(8), allocatable, dimension(:,:,:) :: AreaU, AreaV, DZZ, DWZ, Qx, Qy, Qz, VolOld, Vol
+    real(8), allocatable, dimension(:,:,:) :: Visc, Diff, PROP
+    integer, allocatable, dimension(:,:,:) :: OP, LP, CFU, CFV, CFW, WP
+
+    call get_command_argument(1, fin)
+    call get_command_argument(2, fout)
+    open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+    read(u) sz, ws, nsteps, ipar, rpar
+    allocate(DUX(sz(1):sz(2), sz(3):sz(4)), DVY(sz(1):sz(2), sz(3):sz(4)))
+    allocate(DZX(sz(1):sz(2), sz(3):sz(4)), DZY(sz(1):sz(2), sz(3):sz(4)))
+    allocate(KFloorZ(sz(1):sz(2), sz(3):sz(4)))
+    allocate(AreaU(sz(1):sz(2), sz(3):sz(4), sz(5):sz(6)), mold=0.0_8)
+    allocate(AreaV, DZZ, DWZ, Qx, Qy, Qz, VolOld, Vol, Visc, Diff, PROP, mold=AreaU)
+    allocate(OP(sz(1):sz(2), sz(3):sz(4), sz(5):sz(6)))
+    allocate(LP, CFU, CFV, CFW, WP, mold=OP)
+    read(u) DUX, DVY, DZX, DZY, KFloorZ, AreaU, AreaV, DZZ, DWZ, Qx, Qy, Qz, VolOld, Vol
+    read(u) OP, LP, CFU, CFV, CFW, Visc, Diff, WP, PROP
+    close(u)
+
+    call StartAdvectionDiffusionGPU(ID, sz, ws, DUX, DVY, DZX, DZY, KFloorZ,           &
+                                    AreaU, AreaV, DZZ, DWZ, 1, STAT = STAT)
+    if (STAT /= MGPU_SUCCESS) then
+        write(*, *) 'StartAdvectionDiffusionGPU STAT =', STAT
+        error stop 2
+    endif
+    do s = 1, nsteps
+        call AdvectionDiffusionGPU(ID, PROP, rpar(1), rpar(2), rpar(3),                  &
+                                   ipar(1), ipar(2), ipar(3), ipar(4),                   &
+                                   ipar(5) /= 0, ipar(6) /= 0, rpar(4), .false., 0,       &
+                                   rpar(5), rpar(6), rpar(7), rpar(8), rpar(9), rpar(10), &
+                                   ipar(7) /= 0, Qx, Qy, Qz, VolOld, Vol, OP, LP,         &
+                                   CFU, CFV, CFW, Visc, Diff, WP,                         &
+                                   Optimize = ipar(8) /= 0, FirstProperty = .true.,      &
+                                   STAT = STAT)
+        if (STAT /= MGPU_SUCCESS) then
+            write(*, *) 'AdvectionDiffusionGPU step', s, 'STAT =', STAT
+            error stop 3
+        endif
+    enddo
+    call KillAdvectionDiffusionGPU(ID, STAT = STAT)
+
+    open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+    write(u) PROP
+    close(u)
+
+end program adgpu_driver

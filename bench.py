@@ -76,7 +76,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-window", type=int, default=384)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--kernel", default="split", choices=("fused", "split"))
+    ap.add_argument("--kernel", default="lanes", choices=("lanes", "split", "fused"))
     ap.add_argument("--tracers-per-launch", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 512 MiB three times (the "
@@ -121,7 +121,8 @@ def main():
          f"({time.time() - t0:.1f}s to build)")
 
     prec = _lib.MGPU_PREC_FAST if args.precision == "fast" else _lib.MGPU_PREC_FP64
-    kernel = _lib.MGPU_KERNEL_SPLIT if args.kernel == "split" else _lib.MGPU_KERNEL_FUSED
+    kernel = {"lanes": _lib.MGPU_KERNEL_LANES, "split": _lib.MGPU_KERNEL_SPLIT,
+              "fused": _lib.MGPU_KERNEL_FUSED}[args.kernel]
     tr = Transport(case.size, case.worksize, case.geom, T, precision=prec, kernel=kernel,
                    tracers_per_launch=args.tracers_per_launch)
     tr.set_hydro(case.hydro, case.params)

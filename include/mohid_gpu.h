@@ -54,9 +54,10 @@ extern "C" {
                                     differences from the reference's operation order) */
 
 /* ---- kernel variant of the transport ------------------------------------- */
-#define MGPU_KERNEL_AUTO 0       /* the library's choice (split) */
+#define MGPU_KERNEL_AUTO 0       /* the library's choice (lanes) */
 #define MGPU_KERNEL_FUSED 1      /* one column-tile kernel per step and tracer chunk */
-#define MGPU_KERNEL_SPLIT 2      /* explicit kernel per level, then a column-solve kernel */
+#define MGPU_KERNEL_SPLIT 2      /* tile kernels: explicit per level, then a column solve */
+#define MGPU_KERNEL_LANES 3      /* one lane per (cell, tracer), then per (column, tracer) */
 
 /* T_Size3D (GD:2044-2052): ILB, IUB, JLB, JUB, KLB, KUB. */
 typedef struct {
@@ -147,7 +148,7 @@ typedef struct {
     int32_t ntracers;
     int32_t precision;                    /* MGPU_PREC_FP64 / MGPU_PREC_FAST */
     int32_t docycle_method;
-    int32_t kernel;                       /* MGPU_KERNEL_AUTO / _FUSED / _SPLIT */
+    int32_t kernel;                       /* MGPU_KERNEL_AUTO / _FUSED / _SPLIT / _LANES */
     int32_t tracers_per_launch;           /* 1..8 tracers per kernel launch; 0: 8 */
 } mgpu_transport_opts;
 

@@ -31,6 +31,7 @@ MGPU_PREC_FAST = 1
 MGPU_KERNEL_AUTO = 0
 MGPU_KERNEL_FUSED = 1
 MGPU_KERNEL_SPLIT = 2
+MGPU_KERNEL_LANES = 3
 
 STATUS_NAMES = {
     0: "SUCCESS", 1: "OFF_ERR", -99: "UNKNOWN", 100: "ERR_HIP", 101: "ERR_BADARG",

@@ -144,10 +144,18 @@ struct DevDerived {
     double *cu = nullptr, *cv = nullptr, *cw = nullptr;
     uint8_t *cperm = nullptr;
     uint16_t *ccnt = nullptr, *fperm = nullptr, *fcnt = nullptr;
+    // compact lists of solved rows and columns (lane variant)
+    uint32_t *rowcnt = nullptr, *cells = nullptr, *cols = nullptr;
+    unsigned long long *totals = nullptr;
+    long ncells = 0, ncols = 0;
     unsigned long long *counters = nullptr;  // [0] zero pivots, [1] bad-method flag, [2] mask flags
     int alloc(const Dims &d) {
         const long n3 = d.n3(), nt = tile_count(d), nK = d.KUB - d.KLB + 1;
+        const long nIw = d.wIUB - d.wILB + 1, nJw = d.wJUB - d.wJLB + 1,
+                   nKw = d.wKUB - d.wKLB + 1;
         int rc = 0;
+        rc |= dalloc(&rowcnt, nJw * nKw); rc |= dalloc(&cells, nIw * nJw * nKw);
+        rc |= dalloc(&cols, nIw * nJw); rc |= dalloc(&totals, 2);
         rc |= dalloc(&mask, n3); rc |= dalloc(&kstart, d.n2());
         rc |= dalloc(&cu, n3); rc |= dalloc(&cv, n3); rc |= dalloc(&cw, n3);
         rc |= dalloc(&cperm, nt * 256); rc |= dalloc(&ccnt, nt * nK);
@@ -164,6 +172,7 @@ struct DevDerived {
     void release() {
         dfree(mask); dfree(kstart); dfree(cu); dfree(cv); dfree(cw); dfree(counters);
         dfree(cperm); dfree(ccnt); dfree(fperm); dfree(fcnt);
+        dfree(rowcnt); dfree(cells); dfree(cols); dfree(totals);
     }
     // masks -> mask bytes, kstart, tile orders (after the mask arrays are on the device)
     int pack(const Dims &d, const DevHydro &h, const int32_t *kfloor, int docycle,
@@ -171,7 +180,22 @@ struct DevDerived {
         int rc = launch_pack_masks(d, h.open, h.land, h.cfu, h.cfv, h.cfw, h.water, kfloor,
                                    docycle, mask, kstart, mask_flags(), s);
         if (!rc) rc = launch_tile_sort(d, mask, kstart, cperm, ccnt, fperm, fcnt, s);
-        return rc;
+        if (!rc) rc = launch_build_lists(d, mask, kstart, rowcnt, cells, cols, totals, s);
+        if (rc) return rc;
+        unsigned long long n[2];
+        HIPCHK(hipMemcpyAsync(n, totals, sizeof(n), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        ncells = (long)n[0];
+        ncols = (long)n[1];
+        return MGPU_SUCCESS;
+    }
+    SolveLists lists() const {
+        SolveLists l;
+        l.cells = cells;
+        l.cols = cols;
+        l.ncells = ncells;
+        l.ncols = ncols;
+        return l;
     }
     unsigned int *mask_flags() { return reinterpret_cast<unsigned int *>(counters + 2); }
 };
@@ -300,8 +324,9 @@ extern "C" int mgpu_ad_create(const mgpu_size3d *size, const mgpu_size3d *worksi
     if (!rc) rc = ad->der.alloc(ad->d);
     if (!rc) rc = dalloc(&ad->prop, n3);
     if (!rc) rc = dalloc(&ad->out, n3);
-    if (!rc) rc = dalloc(&ad->Wb, blocked_scratch_len(ad->d, 1));
-    if (!rc) rc = dalloc(&ad->Gb, blocked_scratch_len(ad->d, 1));
+    const long scr = n3 > blocked_scratch_len(ad->d, 1) ? n3 : blocked_scratch_len(ad->d, 1);
+    if (!rc) rc = dalloc(&ad->Wb, scr);
+    if (!rc) rc = dalloc(&ad->Gb, scr);
     if (!rc) rc = ad->geom.upload(ad->d, geometry, ad->stream);
     if (rc) {
         mgpu_ad_destroy(ad);
@@ -371,7 +396,8 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     }
     const SharedFields f = make_fields(ad->hydro, ad->der, ad->geom);
     TransportLaunch L;
-    L.variant = MGPU_KERNEL_SPLIT;
+    L.variant = MGPU_KERNEL_LANES;
+    L.lists = ad->der.lists();
     L.n = 1;
     L.stride = 1;
     L.chunk = 1;
@@ -460,7 +486,7 @@ struct mgpu_transport {
     // tracer state, interleaved: tracer t of cell x at [x * ntracers + t]
     double *buf[2] = {nullptr, nullptr};
     int cur = 0;
-    int kernel = MGPU_KERNEL_SPLIT;
+    int kernel = MGPU_KERNEL_LANES;
     double *Wb = nullptr, *Gb = nullptr;  // column-solve scratch (tile-blocked)
     double *stage = nullptr;         // one Fortran-layout tracer (host transfers)
     int chunk = 8;                   // tracers per launch
@@ -535,13 +561,13 @@ extern "C" int mgpu_transport_create(const mgpu_size3d *size, const mgpu_size3d 
     t->opts = *opts;
     if (opts->tracers_per_launch < 0 || opts->tracers_per_launch > 8 ||
         (opts->kernel != MGPU_KERNEL_AUTO && opts->kernel != MGPU_KERNEL_FUSED &&
-         opts->kernel != MGPU_KERNEL_SPLIT)) {
+         opts->kernel != MGPU_KERNEL_SPLIT && opts->kernel != MGPU_KERNEL_LANES)) {
         delete t;
         return MGPU_ERR_BADARG;
     }
     t->chunk = opts->tracers_per_launch ? opts->tracers_per_launch : 8;
     if (t->chunk > opts->ntracers) t->chunk = opts->ntracers;
-    t->kernel = opts->kernel == MGPU_KERNEL_FUSED ? MGPU_KERNEL_FUSED : MGPU_KERNEL_SPLIT;
+    t->kernel = opts->kernel == MGPU_KERNEL_AUTO ? MGPU_KERNEL_LANES : opts->kernel;
     hipError_t e = hipStreamCreate(&t->stream);
     if (e != hipSuccess) {
         mgpu_set_error("hipStreamCreate", e);
@@ -553,8 +579,11 @@ extern "C" int mgpu_transport_create(const mgpu_size3d *size, const mgpu_size3d 
     if (!rc) rc = t->hydro.alloc(t->d);
     if (!rc) rc = t->der.alloc(t->d);
     const long nT = (long)opts->ntracers;
-    if (!rc) rc = dalloc(&t->Wb, blocked_scratch_len(t->d, opts->ntracers));
-    if (!rc) rc = dalloc(&t->Gb, blocked_scratch_len(t->d, opts->ntracers));
+    // scratch: tile-blocked (split, fused) or laid out like the state (lanes)
+    long scr = blocked_scratch_len(t->d, opts->ntracers);
+    if (t->kernel == MGPU_KERNEL_LANES) scr = n3 * nT;
+    if (!rc) rc = dalloc(&t->Wb, scr);
+    if (!rc) rc = dalloc(&t->Gb, scr);
     if (!rc) rc = dalloc(&t->buf[0], n3 * nT);
     if (!rc) rc = dalloc(&t->buf[1], n3 * nT);
     if (!rc) rc = dalloc(&t->stage, n3);
@@ -671,8 +700,9 @@ static int enqueue_steps(mgpu_transport *t, int nsteps, int cur) {
         L.out = t->buf[cur ^ 1];
         L.Wb = t->Wb;
         L.Gb = t->Gb;
+        L.lists = t->der.lists();
         L.counters = t->der.counters;
-        // first kernel(s) of every chunk, then the split variant's second kernel(s)
+        // first kernel(s) of every chunk, then the second kernel(s)
         for (int ph = 1; ph <= 2; ++ph) {
             rc = launch_transport(L, d, t->sc, f, s, ph);
             if (rc) return rc;

@@ -104,9 +104,26 @@ int launch_prep_faces(const Dims &d, const double *visc, const double *diffv,
 //   variant MGPU_KERNEL_FUSED: one column-tile kernel (W in Wb, G in Gb);
 //   variant MGPU_KERNEL_SPLIT: explicit kernel (TI into Wb), then the column
 //     kernel (W over TI in Wb, G in Gb).
+//   variant MGPU_KERNEL_LANES: lane-per-(cell, tracer) explicit kernel and
+//     lane-per-(column, tracer) column kernel over `lists`; Wb and Gb are
+//     laid out like the state (n * Size doubles each, tracer t of cell x at
+//     [x * stride + t]).
 // phase: bit 1 = first kernel, bit 2 = second kernel (split only).
 // fast: reciprocal-multiply division (FP64 state and arithmetic, results
 // differ from the reference's operation order by rounding only).
+// Compact lists of the lane variant (device arrays, host counts).
+struct SolveLists {
+    const uint32_t *cells = nullptr;   // Size index of every solved row, memory order
+    const uint32_t *cols = nullptr;    // Size index (level 0) of every solved column
+    long ncells = 0, ncols = 0;
+};
+// Builds the lists from the packed masks and kstart; totals[0] / totals[1]
+// receive the counts (device memory; the caller copies them back).  rowcnt
+// holds (WorkSize J extent) x (WorkSize K extent) counters.
+int launch_build_lists(const Dims &d, const uint8_t *mask, const int32_t *kstart,
+                       uint32_t *rowcnt, uint32_t *cells, uint32_t *cols,
+                       unsigned long long *totals, hipStream_t s);
+
 struct TransportLaunch {
     bool fast = false;
     int variant = MGPU_KERNEL_SPLIT;
@@ -116,6 +133,7 @@ struct TransportLaunch {
     const double *P = nullptr;
     double *out = nullptr;
     double *Wb = nullptr, *Gb = nullptr;
+    SolveLists lists;                       // lane variant
     unsigned long long *counters = nullptr; // [0] zero pivots, [1] bad-method flag
 };
 long blocked_scratch_len(const Dims &d, int nt);

@@ -940,6 +940,255 @@ __global__ void thomasz_kernel(Dims d, const double *__restrict__ D,
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Lane variant.  One lane per (cell, tracer) builds the explicit right-hand
+// side, one lane per (column, tracer) solves the column; lanes run over
+// compact lists of the cells and columns the reference solves, in memory
+// order.  A lane's registers hold one tracer's state only, so many waves are
+// resident to hide memory latency, and the NT lanes of a cell read and write
+// the cell's NT interleaved tracer values as one contiguous run.  The
+// explicit fluxes of a face shared by two cells are computed by both (each
+// time with the same operations); there is no LDS and no barrier.
+
+// Decoded position of a Size-linear cell index.
+struct CellPos {
+    int i, j, k;
+};
+__device__ __forceinline__ CellPos cell_pos(const Dims &d, long x) {
+    CellPos c;
+    const long kk = x / d.sk, r = x - kk * d.sk;
+    const long jj = r / d.sj;
+    c.k = d.KLB + (int)kk;
+    c.j = d.JLB + (int)jj;
+    c.i = d.ILB + (int)(r - jj * d.sj);
+    return c;
+}
+
+// Explicit right-hand side of cell x for one tracer (ts points at it):
+// the contributions of cell_rhs, in the same order, with the four face
+// fluxes of the cell computed here.
+template <bool OPT, bool FAST>
+__device__ __forceinline__ double cell_rhs_lane(const Dims &d, const StepScalars &sc,
+                                                const SharedFields &f, const TracerSet &ts,
+                                                long x, int i, int j, int &bad) {
+    const double dt = sc.dt;
+    const uint8_t mk = f.mask[x];
+    const double vk = f.V[x];
+    const bool open = mk & M_OPEN;
+    const double ivk = FAST ? frcp(vk) : 0.0;
+    const double dtv = dt * ivk;
+    const double vv = open ? (FAST ? f.Vold[x] * ivk : f.Vold[x] / vk) : 0.0;
+    const bool hz = !sc.vertical1d;
+    const bool cfu0 = hz && (mk & M_CFU);
+    const bool cfu1 = hz && (f.mask[x + d.sj] & M_CFU);
+    const bool cfv0 = hz && !sc.xzflow && (mk & M_CFV);
+    const bool cfv1 = hz && !sc.xzflow && (f.mask[x + 1] & M_CFV);
+    const double au0 = cfu0 ? f.cu[x] : 0.0, au1 = cfu1 ? f.cu[x + d.sj] : 0.0;
+    const double av0 = cfv0 ? f.cv[x] : 0.0, av1 = cfv1 ? f.cv[x + 1] : 0.0;
+    const double wu0 = FAST ? au0 * dtv : au0 * dt / vk, wu1 = FAST ? au1 * dtv : au1 * dt / vk;
+    const double wv0 = FAST ? av0 * dtv : av0 * dt / vk, wv1 = FAST ? av1 * dtv : av1 * dt / vk;
+    const double bv0 = av0 * dt, bv1 = av1 * dt;
+    const long s = ts.s;
+    const double *P = ts.P;
+    // the four advective face fluxes (MF:10534-10894, 10642-10699)
+    double fu0 = 0.0, fu1 = 0.0, fv0 = 0.0, fv1 = 0.0;
+    {
+        double fl[1];
+        if (cfu0) {
+            hface_flux<OPT, 1, FAST>(d, sc, f, ts, x, d.sj, j, d.JLB, f.DUX + d.idx2(i, j), d.sj,
+                                     f.Qx[x], fl, bad);
+            fu0 = fl[0];
+        }
+        if (cfu1) {
+            hface_flux<OPT, 1, FAST>(d, sc, f, ts, x + d.sj, d.sj, j + 1, d.JLB,
+                                     f.DUX + d.idx2(i, j + 1), d.sj, f.Qx[x + d.sj], fl, bad);
+            fu1 = fl[0];
+        }
+        if (cfv0) {
+            hface_flux<OPT, 1, FAST>(d, sc, f, ts, x, 1, i, d.ILB, f.DVY + d.idx2(i, j), 1,
+                                     f.Qy[x], fl, bad);
+            fv0 = fl[0];
+        }
+        if (cfv1) {
+            hface_flux<OPT, 1, FAST>(d, sc, f, ts, x + 1, 1, i + 1, d.ILB,
+                                     f.DVY + d.idx2(i + 1, j), 1, f.Qy[x + 1], fl, bad);
+            fv1 = fl[0];
+        }
+    }
+    const double pk = P[x * s];
+    double TI = open ? pk * vv : pk;
+    const bool ex = OPT && !FAST;
+    if (cfu0) {
+        const double g = pk - P[(x - d.sj) * s];
+        TI = ex ? TI - au0 * g * dt / vk : TI - wu0 * g;
+    }
+    if (cfu1) {
+        const double g = P[(x + d.sj) * s] - pk;
+        TI = ex ? TI + au1 * g * dt / vk : TI + wu1 * g;
+    }
+    if (cfv0) {
+        const double g = pk - P[(x - 1) * s];
+        TI = ex ? TI - bv0 * g / vk : TI - wv0 * g;
+    }
+    if (cfv1) {
+        const double g = P[(x + 1) * s] - pk;
+        TI = ex ? TI + bv1 * g / vk : TI + wv1 * g;
+    }
+    if (FAST) {
+        if (cfu0) TI = TI + fu0 * dtv;
+        if (cfu1) TI = TI - fu1 * dtv;
+        if (cfv0) TI = TI + fv0 * dtv;
+        if (cfv1) TI = TI - fv1 * dtv;
+    } else {
+        if (cfu0) TI = TI + fu0 * dt / vk;
+        if (cfu1) TI = TI - fu1 * dt / vk;
+        if (cfv0) TI = TI + fv0 * dt / vk;
+        if (cfv1) TI = TI - fv1 * dt / vk;
+    }
+    return TI;
+}
+
+// Explicit kernel of the lane variant: lane (n, t) = cell cells[n], tracer
+// t of the launch; TI goes to TW (laid out like the tracer state).
+template <bool OPT, int NT, bool FAST>
+__global__ void __launch_bounds__(256)
+explicit_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
+                     const uint32_t *__restrict__ cells, long ncells, double *__restrict__ TW,
+                     unsigned int *__restrict__ badflag) {
+    const long g = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    const long n = g / NT;
+    if (n >= ncells) return;
+    const int t = (int)(g - n * NT);
+    const long x = cells[n];
+    const CellPos c = cell_pos(d, x);
+    const TracerSet t1{ts.P + t, ts.out + t, ts.s};
+    int bad = 0;
+    const double TI = cell_rhs_lane<OPT, FAST>(d, sc, f, t1, x, c.i, c.j, bad);
+    TW[x * ts.s + t] = TI;
+    if (bad) atomicOr(badflag, 1u);
+}
+
+// Column kernel of the lane variant: lane (n, t) solves column cols[n] for
+// tracer t: rows D/E/F, forward sweep (W over TI in TW, G in Gs), back
+// substitution (MF:4080-4106).
+template <bool OPT, int NT, bool FAST>
+__global__ void __launch_bounds__(256)
+column_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
+                   const uint32_t *__restrict__ cols, long ncols, double *__restrict__ TW,
+                   double *__restrict__ Gs, unsigned long long *__restrict__ zero_pivots,
+                   unsigned int *__restrict__ badflag) {
+    const long g = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    const long n = g / NT;
+    if (n >= ncols) return;
+    const int t = (int)(g - n * NT);
+    const long base = cols[n];                 // Size index of the column's level 0
+    const int KUB = d.wKUB;
+    const long sk = d.sk, s = ts.s;
+    const int ks = f.kstart[base];             // idx2(i, j) == idx(i, j, 0): KLB is 0
+    const TracerSet t1{ts.P + t, ts.out + t, s};
+    double *tw = TW + t, *gs = Gs + t;
+    int bad = 0;
+    for (int k = d.wKLB; k < ks && k <= KUB; ++k) t1.out[(base + k * sk) * s] = MGPU_NULL_REAL;
+    const bool columnBuild = f.mask[base + KUB * sk] & M_OPEN;  // AD:2982/3004
+    const bool vflux = sc.vert && !sc.vertical1d && columnBuild;
+    ColumnWalk<1> c;
+    column_begin<OPT, 1, FAST>(d, sc, f, t1, base, ks, vflux, c, bad);
+    for (int k = ks; k <= KUB; ++k) {
+        const long x = base + k * sk;
+        double TI[1] = {tw[x * s]}, W[1], G[1];
+        column_row<OPT, 1, FAST>(d, sc, f, t1, base, k, vflux, c, TI, W, G, zero_pivots, bad);
+        tw[x * s] = W[0];
+        gs[x * s] = G[0];
+    }
+    double xs[1];
+    column_ghost<1>(d, f, t1, base, c, xs);
+    for (int k = KUB; k >= ks; --k) {
+        const long x = base + k * sk;
+        xs[0] = tw[x * s] * xs[0] + gs[x * s];
+        t1.out[x * s] = xs[0];
+    }
+    if (bad) atomicOr(badflag, 1u);
+}
+
+// ---------------------------------------------------------------------------
+// Compact lists.  cells: every (i, j, k) the reference solves and gives an
+// explicit right-hand side -- rows kstart..KUB of the columns with
+// WaterPoints3D(i,j,KUB) == 1 -- in memory order; cols: those columns.
+// One block per row (j, k) of the WorkSize plane: counts, then (after an
+// exclusive scan of the counts) the Size-linear indices in order.
+__device__ __forceinline__ bool list_member(const Dims &d, const uint8_t *mask,
+                                            const int32_t *kstart, int i, int j, int k,
+                                            bool columns) {
+    if (!(mask[d.idx(i, j, d.wKUB)] & M_WATER)) return false;
+    return columns || k >= kstart[d.idx2(i, j)];
+}
+
+__global__ void __launch_bounds__(256)
+list_count_kernel(Dims d, const uint8_t *__restrict__ mask, const int32_t *__restrict__ kstart,
+                  int columns, uint32_t *__restrict__ rowcnt) {
+    __shared__ uint32_t part[4];
+    const int j = d.wJLB + blockIdx.x, k = d.wKLB + blockIdx.y;
+    uint32_t cnt = 0;
+    for (int i = d.wILB + threadIdx.x; i <= d.wIUB; i += 256)
+        cnt += list_member(d, mask, kstart, i, j, k, columns) ? 1u : 0u;
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_down(cnt, o, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        rowcnt[blockIdx.y * (long)gridDim.x + blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// In-place exclusive scan of n counts by one block of 1024 threads; total in
+// *total.
+__global__ void __launch_bounds__(1024)
+list_scan_kernel(uint32_t *__restrict__ a, long n, unsigned long long *__restrict__ total) {
+    __shared__ unsigned long long sums[1024];
+    const int tid = threadIdx.x;
+    const long per = (n + 1023) / 1024;
+    const long lo = tid * per, hi = lo + per < n ? lo + per : n;
+    unsigned long long s = 0;
+    for (long q = lo; q < hi; ++q) s += a[q];
+    sums[tid] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {       // inclusive Hillis-Steele scan
+        const unsigned long long v = tid >= o ? sums[tid - o] : 0ull;
+        __syncthreads();
+        sums[tid] += v;
+        __syncthreads();
+    }
+    unsigned long long run = tid ? sums[tid - 1] : 0ull;
+    for (long q = lo; q < hi; ++q) {
+        const uint32_t c = a[q];
+        a[q] = (uint32_t)run;
+        run += c;
+    }
+    if (tid == 1023) *total = sums[1023];
+}
+
+__global__ void __launch_bounds__(256)
+list_write_kernel(Dims d, const uint8_t *__restrict__ mask, const int32_t *__restrict__ kstart,
+                  int columns, const uint32_t *__restrict__ rowoff, uint32_t *__restrict__ out) {
+    __shared__ uint32_t wsum[4];
+    const int j = d.wJLB + blockIdx.x, k = d.wKLB + blockIdx.y;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t off = rowoff[blockIdx.y * (long)gridDim.x + blockIdx.x];
+    for (int i0 = d.wILB; i0 <= d.wIUB; i0 += 256) {
+        const int i = i0 + threadIdx.x;
+        const bool m = i <= d.wIUB && list_member(d, mask, kstart, i, j, k, columns);
+        const unsigned long long b = __ballot(m);
+        const uint32_t before = (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wave] = (uint32_t)__popcll(b);
+        __syncthreads();
+        uint32_t woff = 0;
+        for (int w = 0; w < wave; ++w) woff += wsum[w];
+        const uint32_t chunk = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (m) out[off + woff + before] = (uint32_t)(columns ? d.idx2(i, j) : d.idx(i, j, k));
+        __syncthreads();
+        off += chunk;
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -993,6 +1242,30 @@ int launch_prep_faces(const Dims &d, const double *visc, const double *diffv, co
     return MGPU_SUCCESS;
 }
 
+int launch_build_lists(const Dims &d, const uint8_t *mask, const int32_t *kstart,
+                       uint32_t *rowcnt, uint32_t *cells, uint32_t *cols,
+                       unsigned long long *totals, hipStream_t s) {
+    const int nj = d.wJUB - d.wJLB + 1, nk = d.wKUB - d.wKLB + 1;
+    // cells: rows (j, k); columns: rows j
+    hipLaunchKernelGGL(list_count_kernel, dim3(nj, nk), dim3(256), 0, s, d, mask, kstart, 0,
+                       rowcnt);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(list_scan_kernel, dim3(1), dim3(1024), 0, s, rowcnt, (long)nj * nk, totals);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(list_write_kernel, dim3(nj, nk), dim3(256), 0, s, d, mask, kstart, 0,
+                       rowcnt, cells);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(list_count_kernel, dim3(nj, 1), dim3(256), 0, s, d, mask, kstart, 1,
+                       rowcnt);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(list_scan_kernel, dim3(1), dim3(1024), 0, s, rowcnt, (long)nj, totals + 1);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(list_write_kernel, dim3(nj, 1), dim3(256), 0, s, d, mask, kstart, 1,
+                       rowcnt, cols);
+    HIPCHK(hipGetLastError());
+    return MGPU_SUCCESS;
+}
+
 long blocked_scratch_len(const Dims &d, int nt) {
     return tile_count(d) * (long)(d.KUB - d.KLB + 1) * nt * TC;
 }
@@ -1003,10 +1276,31 @@ long blocked_scratch_len(const Dims &d, int nt) {
 template <bool OPT, int NT, bool FAST>
 static int launch_chunk(int variant, int phase, const Dims &d, const StepScalars &sc,
                         const SharedFields &f, const TracerSet &ts, double *Wb, double *Gb,
-                        unsigned long long *counters, hipStream_t s) {
+                        const SolveLists &lists, unsigned long long *counters, hipStream_t s) {
     const int ni = d.wIUB - d.wILB + 1, nj = d.wJUB - d.wJLB + 1, nk = d.wKUB - d.wKLB + 1;
     unsigned int *bad = reinterpret_cast<unsigned int *>(counters + 1);
     const dim3 tiles((ni + TX - 1) / TX, (nj + TY - 1) / TY);
+    if (variant == MGPU_KERNEL_LANES) {
+        if (phase & 1) {
+            const long nb = (lists.ncells * NT + 255) / 256;
+            if (nb > 0) {
+                hipLaunchKernelGGL((explicit_lane_kernel<OPT, NT, FAST>), dim3((unsigned)nb),
+                                   dim3(256), 0, s, d, sc, f, ts, lists.cells, lists.ncells, Wb,
+                                   bad);
+                HIPCHK(hipGetLastError());
+            }
+        }
+        if (phase & 2) {
+            const long nb = (lists.ncols * NT + 255) / 256;
+            if (nb > 0) {
+                hipLaunchKernelGGL((column_lane_kernel<OPT, NT, FAST>), dim3((unsigned)nb),
+                                   dim3(256), 0, s, d, sc, f, ts, lists.cols, lists.ncols, Wb, Gb,
+                                   counters, bad);
+                HIPCHK(hipGetLastError());
+            }
+        }
+        return MGPU_SUCCESS;
+    }
     if (variant == MGPU_KERNEL_FUSED) {
         if (phase & 1) {
             hipLaunchKernelGGL((column_tile_kernel<OPT, NT, FAST>), dim3(tiles.x * tiles.y),
@@ -1032,16 +1326,16 @@ static int launch_chunk(int variant, int phase, const Dims &d, const StepScalars
 template <bool OPT, bool FAST>
 static int launch_n(int nt, int variant, int ph, const Dims &d, const StepScalars &sc,
                     const SharedFields &f, const TracerSet &ts, double *Wb, double *Gb,
-                    unsigned long long *c, hipStream_t s) {
+                    const SolveLists &lists, unsigned long long *c, hipStream_t s) {
     switch (nt) {
-        case 1: return launch_chunk<OPT, 1, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, c, s);
-        case 2: return launch_chunk<OPT, 2, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, c, s);
-        case 3: return launch_chunk<OPT, 3, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, c, s);
-        case 4: return launch_chunk<OPT, 4, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, c, s);
-        case 5: return launch_chunk<OPT, 5, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, c, s);
-        case 6: return launch_chunk<OPT, 6, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, c, s);
-        case 7: return launch_chunk<OPT, 7, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, c, s);
-        case 8: return launch_chunk<OPT, 8, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, c, s);
+        case 1: return launch_chunk<OPT, 1, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, c, s);
+        case 2: return launch_chunk<OPT, 2, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, c, s);
+        case 3: return launch_chunk<OPT, 3, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, c, s);
+        case 4: return launch_chunk<OPT, 4, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, c, s);
+        case 5: return launch_chunk<OPT, 5, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, c, s);
+        case 6: return launch_chunk<OPT, 6, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, c, s);
+        case 7: return launch_chunk<OPT, 7, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, c, s);
+        case 8: return launch_chunk<OPT, 8, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, c, s);
         default: return MGPU_ERR_BADARG;
     }
 }
@@ -1051,7 +1345,10 @@ int launch_transport(const TransportLaunch &L, const Dims &d, const StepScalars 
     int chunk = L.chunk;
     if (chunk <= 0 || chunk > MAXT) chunk = MAXT;
     if (L.stride < L.n) return MGPU_ERR_BADARG;
-    if (L.variant != MGPU_KERNEL_FUSED && L.variant != MGPU_KERNEL_SPLIT) return MGPU_ERR_BADARG;
+    if (L.variant != MGPU_KERNEL_FUSED && L.variant != MGPU_KERNEL_SPLIT &&
+        L.variant != MGPU_KERNEL_LANES)
+        return MGPU_ERR_BADARG;
+    if (L.variant == MGPU_KERNEL_LANES && (!L.lists.cells || !L.lists.cols)) return MGPU_ERR_BADARG;
     if (!L.Wb || !L.Gb || !L.P || !L.out) return MGPU_ERR_BADARG;
     for (int t0 = 0; t0 < L.n; t0 += chunk) {
         const int nt = L.n - t0 < chunk ? L.n - t0 : chunk;
@@ -1060,19 +1357,22 @@ int launch_transport(const TransportLaunch &L, const Dims &d, const StepScalars 
         ts.out = L.out + t0;
         ts.s = L.stride;
         // each chunk has its own scratch region: with the split variant all
-        // first kernels run before the second ones
-        double *Wb = L.Wb + blocked_scratch_len(d, t0), *Gb = L.Gb + blocked_scratch_len(d, t0);
+        // first kernels run before the second ones.  The lane variant's
+        // scratch is laid out like the state (tracer t at [x * stride + t]).
+        const bool lanes = L.variant == MGPU_KERNEL_LANES;
+        double *Wb = L.Wb + (lanes ? t0 : blocked_scratch_len(d, t0));
+        double *Gb = L.Gb + (lanes ? t0 : blocked_scratch_len(d, t0));
         int rc;
         if (sc.optimize)
             rc = L.fast ? launch_n<true, true>(nt, L.variant, phase, d, sc, f, ts, Wb, Gb,
-                                               L.counters, s)
+                                               L.lists, L.counters, s)
                         : launch_n<true, false>(nt, L.variant, phase, d, sc, f, ts, Wb, Gb,
-                                                L.counters, s);
+                                                L.lists, L.counters, s);
         else
             rc = L.fast ? launch_n<false, true>(nt, L.variant, phase, d, sc, f, ts, Wb, Gb,
-                                                L.counters, s)
+                                                L.lists, L.counters, s)
                         : launch_n<false, false>(nt, L.variant, phase, d, sc, f, ts, Wb, Gb,
-                                                 L.counters, s);
+                                                 L.lists, L.counters, s);
         if (rc) return rc;
     }
     return MGPU_SUCCESS;

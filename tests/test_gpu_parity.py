@@ -164,15 +164,17 @@ def test_batched_transport_chunks_match_oracle(per_launch):
         assert np.array_equal(got[t][~solved], case.tracers[t][~solved])
 
 
-@pytest.mark.parametrize("kernel", ["fused", "split"])
+@pytest.mark.parametrize("kernel", ["lanes", "split", "fused"])
 @pytest.mark.parametrize("name", sorted(SMALL))
 def test_batched_transport_matches_oracle(name, kernel):
     """Batched FP64 transport on every small case (optimised and plain
     operation orders, flat, sigma, ragged, land, stiff) for 6 steps, with the
-    fused column-tile kernel and with the split explicit/column kernels."""
+    lane-per-(cell, tracer) kernels, the tile kernels and the fused
+    column-tile kernel."""
     from mohid_amd import _lib as L
     case = make(name)
-    kid = L.MGPU_KERNEL_FUSED if kernel == "fused" else L.MGPU_KERNEL_SPLIT
+    kid = {"lanes": L.MGPU_KERNEL_LANES, "split": L.MGPU_KERNEL_SPLIT,
+           "fused": L.MGPU_KERNEL_FUSED}[kernel]
     got = _batched(case, 4, 2, kernel=kid)
     ref = ora.run_case(case, 6, nthreads=1)
     solved = _solved_mask(case)

@@ -955,23 +955,36 @@ __global__ void thomasz_kernel(Dims d, const double *__restrict__ D,
 struct CellPos {
     int i, j, k;
 };
-__device__ __forceinline__ CellPos cell_pos(const Dims &d, long x) {
+__device__ __forceinline__ CellPos cell_pos(const Dims &d, uint32_t x) {
+    // list entries are < 2^32 (launch_build_lists checks the Size)
     CellPos c;
-    const long kk = x / d.sk, r = x - kk * d.sk;
-    const long jj = r / d.sj;
+    const uint32_t sk = (uint32_t)d.sk, sj = (uint32_t)d.sj;
+    const uint32_t kk = x / sk, r = x - kk * sk;
+    const uint32_t jj = r / sj;
     c.k = d.KLB + (int)kk;
     c.j = d.JLB + (int)jj;
-    c.i = d.ILB + (int)(r - jj * d.sj);
+    c.i = d.ILB + (int)(r - jj * sj);
     return c;
 }
 
-// Explicit right-hand side of cell x for one tracer (ts points at it):
-// the contributions of cell_rhs, in the same order, with the four face
-// fluxes of the cell computed here.
-template <bool OPT, bool FAST>
-__device__ __forceinline__ double cell_rhs_lane(const Dims &d, const StepScalars &sc,
-                                                const SharedFields &f, const TracerSet &ts,
-                                                long x, int i, int j, int &bad) {
+// Smallest value of v over the active lanes of the wave.
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+// Explicit right-hand side of cell x for TPL tracers (ts points at the
+// first): the contributions of cell_rhs, in the same order, with the four
+// face fluxes of the cell computed here.
+template <bool OPT, int TPL, bool FAST>
+__device__ __forceinline__ void cell_rhs_lane(const Dims &d, const StepScalars &sc,
+                                              const SharedFields &f, const TracerSet &ts,
+                                              long x, int i, int j, double (&TIo)[TPL],
+                                              int &bad) {
     const double dt = sc.dt;
     const uint8_t mk = f.mask[x];
     const double vk = f.V[x];
@@ -990,123 +1003,140 @@ __device__ __forceinline__ double cell_rhs_lane(const Dims &d, const StepScalars
     const double wv0 = FAST ? av0 * dtv : av0 * dt / vk, wv1 = FAST ? av1 * dtv : av1 * dt / vk;
     const double bv0 = av0 * dt, bv1 = av1 * dt;
     const long s = ts.s;
-    const double *P = ts.P;
     // the four advective face fluxes (MF:10534-10894, 10642-10699)
-    double fu0 = 0.0, fu1 = 0.0, fv0 = 0.0, fv1 = 0.0;
-    {
-        double fl[1];
+    double fu0[TPL], fu1[TPL], fv0[TPL], fv1[TPL];
+    if (cfu0)
+        hface_flux<OPT, TPL, FAST>(d, sc, f, ts, x, d.sj, j, d.JLB, f.DUX + d.idx2(i, j), d.sj,
+                                   f.Qx[x], fu0, bad);
+    if (cfu1)
+        hface_flux<OPT, TPL, FAST>(d, sc, f, ts, x + d.sj, d.sj, j + 1, d.JLB,
+                                   f.DUX + d.idx2(i, j + 1), d.sj, f.Qx[x + d.sj], fu1, bad);
+    if (cfv0)
+        hface_flux<OPT, TPL, FAST>(d, sc, f, ts, x, 1, i, d.ILB, f.DVY + d.idx2(i, j), 1,
+                                   f.Qy[x], fv0, bad);
+    if (cfv1)
+        hface_flux<OPT, TPL, FAST>(d, sc, f, ts, x + 1, 1, i + 1, d.ILB,
+                                   f.DVY + d.idx2(i + 1, j), 1, f.Qy[x + 1], fv1, bad);
+    const bool ex = OPT && !FAST;
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+        const double *P = ts.P + t;
+        const double pk = P[x * s];
+        double TI = open ? pk * vv : pk;
         if (cfu0) {
-            hface_flux<OPT, 1, FAST>(d, sc, f, ts, x, d.sj, j, d.JLB, f.DUX + d.idx2(i, j), d.sj,
-                                     f.Qx[x], fl, bad);
-            fu0 = fl[0];
+            const double g = pk - P[(x - d.sj) * s];
+            TI = ex ? TI - au0 * g * dt / vk : TI - wu0 * g;
         }
         if (cfu1) {
-            hface_flux<OPT, 1, FAST>(d, sc, f, ts, x + d.sj, d.sj, j + 1, d.JLB,
-                                     f.DUX + d.idx2(i, j + 1), d.sj, f.Qx[x + d.sj], fl, bad);
-            fu1 = fl[0];
+            const double g = P[(x + d.sj) * s] - pk;
+            TI = ex ? TI + au1 * g * dt / vk : TI + wu1 * g;
         }
         if (cfv0) {
-            hface_flux<OPT, 1, FAST>(d, sc, f, ts, x, 1, i, d.ILB, f.DVY + d.idx2(i, j), 1,
-                                     f.Qy[x], fl, bad);
-            fv0 = fl[0];
+            const double g = pk - P[(x - 1) * s];
+            TI = ex ? TI - bv0 * g / vk : TI - wv0 * g;
         }
         if (cfv1) {
-            hface_flux<OPT, 1, FAST>(d, sc, f, ts, x + 1, 1, i + 1, d.ILB,
-                                     f.DVY + d.idx2(i + 1, j), 1, f.Qy[x + 1], fl, bad);
-            fv1 = fl[0];
+            const double g = P[(x + 1) * s] - pk;
+            TI = ex ? TI + bv1 * g / vk : TI + wv1 * g;
         }
+        if (FAST) {
+            if (cfu0) TI = TI + fu0[t] * dtv;
+            if (cfu1) TI = TI - fu1[t] * dtv;
+            if (cfv0) TI = TI + fv0[t] * dtv;
+            if (cfv1) TI = TI - fv1[t] * dtv;
+        } else {
+            if (cfu0) TI = TI + fu0[t] * dt / vk;
+            if (cfu1) TI = TI - fu1[t] * dt / vk;
+            if (cfv0) TI = TI + fv0[t] * dt / vk;
+            if (cfv1) TI = TI - fv1[t] * dt / vk;
+        }
+        TIo[t] = TI;
     }
-    const double pk = P[x * s];
-    double TI = open ? pk * vv : pk;
-    const bool ex = OPT && !FAST;
-    if (cfu0) {
-        const double g = pk - P[(x - d.sj) * s];
-        TI = ex ? TI - au0 * g * dt / vk : TI - wu0 * g;
-    }
-    if (cfu1) {
-        const double g = P[(x + d.sj) * s] - pk;
-        TI = ex ? TI + au1 * g * dt / vk : TI + wu1 * g;
-    }
-    if (cfv0) {
-        const double g = pk - P[(x - 1) * s];
-        TI = ex ? TI - bv0 * g / vk : TI - wv0 * g;
-    }
-    if (cfv1) {
-        const double g = P[(x + 1) * s] - pk;
-        TI = ex ? TI + bv1 * g / vk : TI + wv1 * g;
-    }
-    if (FAST) {
-        if (cfu0) TI = TI + fu0 * dtv;
-        if (cfu1) TI = TI - fu1 * dtv;
-        if (cfv0) TI = TI + fv0 * dtv;
-        if (cfv1) TI = TI - fv1 * dtv;
-    } else {
-        if (cfu0) TI = TI + fu0 * dt / vk;
-        if (cfu1) TI = TI - fu1 * dt / vk;
-        if (cfv0) TI = TI + fv0 * dt / vk;
-        if (cfv1) TI = TI - fv1 * dt / vk;
-    }
-    return TI;
 }
 
-// Explicit kernel of the lane variant: lane (n, t) = cell cells[n], tracer
-// t of the launch; TI goes to TW (laid out like the tracer state).
-template <bool OPT, int NT, bool FAST>
+// Explicit kernel of the lane variant: NT / TPL lanes per cell; lane
+// (n, u) = cell cells[n], tracers u*TPL .. u*TPL+TPL-1 of the launch; TI
+// goes to TW (laid out like the tracer state).
+template <bool OPT, int NT, int TPL, bool FAST>
 __global__ void __launch_bounds__(256)
 explicit_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
                      const uint32_t *__restrict__ cells, long ncells, double *__restrict__ TW,
                      unsigned int *__restrict__ badflag) {
+    constexpr int LPC = NT / TPL;   // lanes per cell
     const long g = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    const long n = g / NT;
+    const long n = g / LPC;
     if (n >= ncells) return;
-    const int t = (int)(g - n * NT);
-    const long x = cells[n];
-    const CellPos c = cell_pos(d, x);
-    const TracerSet t1{ts.P + t, ts.out + t, ts.s};
+    const int t0 = (int)(g - n * LPC) * TPL;
+    const uint32_t xc = cells[n];
+    const long x = xc;
+    const CellPos c = cell_pos(d, xc);
+    const TracerSet tl{ts.P + t0, ts.out + t0, ts.s};
     int bad = 0;
-    const double TI = cell_rhs_lane<OPT, FAST>(d, sc, f, t1, x, c.i, c.j, bad);
-    TW[x * ts.s + t] = TI;
+    double TI[TPL];
+    cell_rhs_lane<OPT, TPL, FAST>(d, sc, f, tl, x, c.i, c.j, TI, bad);
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) TW[x * ts.s + t0 + t] = TI[t];
     if (bad) atomicOr(badflag, 1u);
 }
 
-// Column kernel of the lane variant: lane (n, t) solves column cols[n] for
-// tracer t: rows D/E/F, forward sweep (W over TI in TW, G in Gs), back
-// substitution (MF:4080-4106).
-template <bool OPT, int NT, bool FAST>
+// Column kernel of the lane variant: NT / TPL lanes per column; lane (n, u)
+// solves column cols[n] for tracers u*TPL .. u*TPL+TPL-1: rows D/E/F,
+// forward sweep (W over TI in TW, G in Gs), back substitution
+// (MF:4080-4106).
+template <bool OPT, int NT, int TPL, bool FAST>
 __global__ void __launch_bounds__(256)
 column_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
                    const uint32_t *__restrict__ cols, long ncols, double *__restrict__ TW,
                    double *__restrict__ Gs, unsigned long long *__restrict__ zero_pivots,
                    unsigned int *__restrict__ badflag) {
+    constexpr int LPC = NT / TPL;
     const long g = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    const long n = g / NT;
-    if (n >= ncols) return;
-    const int t = (int)(g - n * NT);
-    const long base = cols[n];                 // Size index of the column's level 0
+    const long n = g / LPC;
+    const bool act = n < ncols;
+    const int t = (int)(g - n * LPC) * TPL;
     const int KUB = d.wKUB;
+    const long base = act ? (long)cols[n] : 0;  // Size index of the column's level 0
     const long sk = d.sk, s = ts.s;
-    const int ks = f.kstart[base];             // idx2(i, j) == idx(i, j, 0): KLB is 0
+    const int ks = act ? f.kstart[base] : KUB + 1;   // idx2(i,j) == idx(i,j,0): KLB is 0
+    // the wave walks the levels together, from its lowest first row: at every
+    // step all its lanes touch one level, so the NT lanes of neighbouring
+    // columns access one contiguous run
+    const int kw = wave_min(ks);
+    if (!act) return;
     const TracerSet t1{ts.P + t, ts.out + t, s};
     double *tw = TW + t, *gs = Gs + t;
     int bad = 0;
-    for (int k = d.wKLB; k < ks && k <= KUB; ++k) t1.out[(base + k * sk) * s] = MGPU_NULL_REAL;
+    for (int k = d.wKLB; k < ks && k <= KUB; ++k) {
+#pragma unroll
+        for (int u = 0; u < TPL; ++u) t1.out[(base + k * sk) * s + u] = MGPU_NULL_REAL;
+    }
     const bool columnBuild = f.mask[base + KUB * sk] & M_OPEN;  // AD:2982/3004
     const bool vflux = sc.vert && !sc.vertical1d && columnBuild;
-    ColumnWalk<1> c;
-    column_begin<OPT, 1, FAST>(d, sc, f, t1, base, ks, vflux, c, bad);
-    for (int k = ks; k <= KUB; ++k) {
+    ColumnWalk<TPL> c;
+    column_begin<OPT, TPL, FAST>(d, sc, f, t1, base, ks, vflux, c, bad);
+    for (int k = kw; k <= KUB; ++k) {
+        if (k < ks) continue;
         const long x = base + k * sk;
-        double TI[1] = {tw[x * s]}, W[1], G[1];
-        column_row<OPT, 1, FAST>(d, sc, f, t1, base, k, vflux, c, TI, W, G, zero_pivots, bad);
-        tw[x * s] = W[0];
-        gs[x * s] = G[0];
+        double TI[TPL], W[TPL], G[TPL];
+#pragma unroll
+        for (int u = 0; u < TPL; ++u) TI[u] = tw[x * s + u];
+        column_row<OPT, TPL, FAST>(d, sc, f, t1, base, k, vflux, c, TI, W, G, zero_pivots, bad);
+#pragma unroll
+        for (int u = 0; u < TPL; ++u) {
+            tw[x * s + u] = W[u];
+            gs[x * s + u] = G[u];
+        }
     }
-    double xs[1];
-    column_ghost<1>(d, f, t1, base, c, xs);
-    for (int k = KUB; k >= ks; --k) {
+    double xs[TPL];
+    column_ghost<TPL>(d, f, t1, base, c, xs);
+    for (int k = KUB; k >= kw; --k) {
+        if (k < ks) continue;
         const long x = base + k * sk;
-        xs[0] = tw[x * s] * xs[0] + gs[x * s];
-        t1.out[x * s] = xs[0];
+#pragma unroll
+        for (int u = 0; u < TPL; ++u) {
+            xs[u] = tw[x * s + u] * xs[u] + gs[x * s + u];
+            t1.out[x * s + u] = xs[u];
+        }
     }
     if (bad) atomicOr(badflag, 1u);
 }
@@ -1245,6 +1275,7 @@ int launch_prep_faces(const Dims &d, const double *visc, const double *diffv, co
 int launch_build_lists(const Dims &d, const uint8_t *mask, const int32_t *kstart,
                        uint32_t *rowcnt, uint32_t *cells, uint32_t *cols,
                        unsigned long long *totals, hipStream_t s) {
+    if (d.n3() >= (1L << 32)) return MGPU_ERR_UNSUPPORTED;   // list entries are 32-bit
     const int nj = d.wJUB - d.wJLB + 1, nk = d.wKUB - d.wKLB + 1;
     // cells: rows (j, k); columns: rows j
     hipLaunchKernelGGL(list_count_kernel, dim3(nj, nk), dim3(256), 0, s, d, mask, kstart, 0,
@@ -1281,19 +1312,24 @@ static int launch_chunk(int variant, int phase, const Dims &d, const StepScalars
     unsigned int *bad = reinterpret_cast<unsigned int *>(counters + 1);
     const dim3 tiles((ni + TX - 1) / TX, (nj + TY - 1) / TY);
     if (variant == MGPU_KERNEL_LANES) {
+        // tracers per lane: the explicit kernel shares a cell's
+        // tracer-independent work among TPL tracers; the column kernel keeps
+        // one column walk per lane and tracer pair
+        constexpr int TPE = NT % 4 == 0 ? 4 : (NT % 2 == 0 ? 2 : 1);
+        constexpr int TPC = NT % 2 == 0 ? 2 : 1;
         if (phase & 1) {
-            const long nb = (lists.ncells * NT + 255) / 256;
+            const long nb = (lists.ncells * (NT / TPE) + 255) / 256;
             if (nb > 0) {
-                hipLaunchKernelGGL((explicit_lane_kernel<OPT, NT, FAST>), dim3((unsigned)nb),
-                                   dim3(256), 0, s, d, sc, f, ts, lists.cells, lists.ncells, Wb,
-                                   bad);
+                hipLaunchKernelGGL((explicit_lane_kernel<OPT, NT, TPE, FAST>),
+                                   dim3((unsigned)nb), dim3(256), 0, s, d, sc, f, ts,
+                                   lists.cells, lists.ncells, Wb, bad);
                 HIPCHK(hipGetLastError());
             }
         }
         if (phase & 2) {
-            const long nb = (lists.ncols * NT + 255) / 256;
+            const long nb = (lists.ncols * (NT / TPC) + 255) / 256;
             if (nb > 0) {
-                hipLaunchKernelGGL((column_lane_kernel<OPT, NT, FAST>), dim3((unsigned)nb),
+                hipLaunchKernelGGL((column_lane_kernel<OPT, NT, TPC, FAST>), dim3((unsigned)nb),
                                    dim3(256), 0, s, d, sc, f, ts, lists.cols, lists.ncols, Wb, Gb,
                                    counters, bad);
                 HIPCHK(hipGetLastError());

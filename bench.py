@@ -78,6 +78,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--kernel", default="lanes", choices=("lanes", "split", "fused"))
     ap.add_argument("--tracers-per-launch", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 512 MiB three times (the "
                          "reference workload for scaling FETCH_SIZE/WRITE_SIZE counters)")
@@ -129,6 +130,18 @@ def main():
     for t, p in enumerate(case.tracers):
         tr.set_tracer(t, p)
     use_graph = not args.no_graph
+
+    # ---- launch shape from the autotuner (lane variant), before any timing;
+    # tuning advances the state, so the tracers are reloaded afterwards
+    tuned = None
+    if args.kernel == "lanes" and not args.no_autotune:
+        from mohid_amd.autotune import autotune
+        t0 = time.time()
+        tuned = autotune(tr, steps=2)
+        for t, p in enumerate(case.tracers):
+            tr.set_tracer(t, p)
+        _log(f"autotuned launch shape (tracers/launch, explicit tracers/lane, column "
+             f"tracers/lane) = {tuned['shape']} in {time.time() - t0:.1f}s")
 
     # ---- warm-up
     if args.warmup > 0:
@@ -192,6 +205,8 @@ def main():
             "n_wet_cells": nwet,
             "precision": args.precision,
             "graph": use_graph,
+            "kernel": args.kernel,
+            "launch_shape": list(tuned["shape"]) if tuned else None,
             "optimize_path": bool(case.params["Optimize"]),
         },
         "roofline": {

@@ -150,6 +150,8 @@ typedef struct {
     int32_t docycle_method;
     int32_t kernel;                       /* MGPU_KERNEL_AUTO / _FUSED / _SPLIT / _LANES */
     int32_t tracers_per_launch;           /* 1..8 tracers per kernel launch; 0: 8 */
+    int32_t explicit_tracers_per_lane;    /* lanes: 1, 2, 4 or 8 (dividing the launch); 0: 4 */
+    int32_t column_tracers_per_lane;      /* lanes: 1, 2, 4 or 8 (dividing the launch); 0: 2 */
 } mgpu_transport_opts;
 
 int mgpu_transport_create(const mgpu_size3d *size, const mgpu_size3d *worksize,
@@ -168,6 +170,13 @@ int mgpu_transport_run(mgpu_transport *t, int32_t nsteps, int32_t use_graph);
 int mgpu_transport_tracer_ptr(mgpu_transport *t, int32_t index, double **dev_ptr,
                               int64_t *stride);
 int mgpu_transport_stream(mgpu_transport *t, void **stream);
+/* Launch shape: tracers per launch and, for the lane variant, tracers per
+ * lane of the explicit and column kernels (as in mgpu_transport_opts).  For
+ * the autotuner; results do not depend on the shape.  The tile variants keep
+ * the tracers per launch they were created with (MGPU_ERR_STATE otherwise). */
+int mgpu_transport_set_lanes(mgpu_transport *t, int32_t tracers_per_launch,
+                             int32_t explicit_tracers_per_lane,
+                             int32_t column_tracers_per_lane);
 int mgpu_transport_destroy(mgpu_transport *t);
 /* Per-launch timing for measurement: with enable = 1 the following eager
  * mgpu_transport_run calls (use_graph = 0, up to max_steps steps) record

@@ -185,11 +185,13 @@ class Transport:
     ModuleWaterProperties::Advection_Diffusion_Processes, WP:14520-15169)."""
 
     def __init__(self, size, worksize, geometry: Dict, ntracers: int, precision: int = 0,
-                 docycle_method: int = 1, kernel: int = 0, tracers_per_launch: int = 0):
+                 docycle_method: int = 1, kernel: int = 0, tracers_per_launch: int = 0,
+                 explicit_tracers_per_lane: int = 0, column_tracers_per_lane: int = 0):
         keep: list = []
         g = _geometry(geometry, keep)
         opts = L.TransportOpts(int(ntracers), int(precision), int(docycle_method),
-                               int(kernel), int(tracers_per_launch))
+                               int(kernel), int(tracers_per_launch),
+                               int(explicit_tracers_per_lane), int(column_tracers_per_lane))
         h = ct.c_void_p()
         self._size = L.Size3D(*size)
         self._ws = L.Size3D(*worksize)
@@ -240,6 +242,15 @@ class Transport:
                 L.lib().mgpu_transport_kernel_times(self._h, ct.byref(a), ct.byref(b),
                                                     ct.byref(c), ct.byref(n)))
         return a.value, b.value, c.value, n.value
+
+    def set_lanes(self, tracers_per_launch: int = 0, explicit_tracers_per_lane: int = 0,
+                  column_tracers_per_lane: int = 0) -> None:
+        """Launch shape (tracers per launch, tracers per lane of the lane
+        variant's explicit and column kernels); results do not depend on it."""
+        L.check("mgpu_transport_set_lanes",
+                L.lib().mgpu_transport_set_lanes(self._h, int(tracers_per_launch),
+                                                 int(explicit_tracers_per_lane),
+                                                 int(column_tracers_per_lane)))
 
     def stream_handle(self) -> int:
         s = ct.c_void_p()

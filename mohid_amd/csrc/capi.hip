@@ -559,6 +559,11 @@ extern "C" int mgpu_transport_create(const mgpu_size3d *size, const mgpu_size3d 
     t->ws = *worksize;
     t->d = make_dims(*size, *worksize);
     t->opts = *opts;
+    auto tpl_ok = [](int v) { return v == 0 || v == 1 || v == 2 || v == 4 || v == 8; };
+    if (!tpl_ok(opts->explicit_tracers_per_lane) || !tpl_ok(opts->column_tracers_per_lane)) {
+        delete t;
+        return MGPU_ERR_BADARG;
+    }
     if (opts->tracers_per_launch < 0 || opts->tracers_per_launch > 8 ||
         (opts->kernel != MGPU_KERNEL_AUTO && opts->kernel != MGPU_KERNEL_FUSED &&
          opts->kernel != MGPU_KERNEL_SPLIT && opts->kernel != MGPU_KERNEL_LANES)) {
@@ -666,6 +671,27 @@ extern "C" int mgpu_transport_tracer_ptr(mgpu_transport *t, int32_t index, doubl
     return MGPU_SUCCESS;
 }
 
+extern "C" int mgpu_transport_set_lanes(mgpu_transport *t, int32_t tracers_per_launch,
+                                        int32_t explicit_tracers_per_lane,
+                                        int32_t column_tracers_per_lane) {
+    if (!t) return MGPU_OFF_ERR;
+    auto tpl_ok = [](int v) { return v == 0 || v == 1 || v == 2 || v == 4 || v == 8; };
+    if (tracers_per_launch < 0 || tracers_per_launch > 8 || !tpl_ok(explicit_tracers_per_lane) ||
+        !tpl_ok(column_tracers_per_lane))
+        return MGPU_ERR_BADARG;
+    // the tile variants' scratch is sized for the chunk chosen at creation
+    if (t->kernel != MGPU_KERNEL_LANES && tracers_per_launch != t->opts.tracers_per_launch)
+        return MGPU_ERR_STATE;
+    HIPCHK(hipStreamSynchronize(t->stream));
+    t->opts.tracers_per_launch = tracers_per_launch;
+    t->opts.explicit_tracers_per_lane = explicit_tracers_per_lane;
+    t->opts.column_tracers_per_lane = column_tracers_per_lane;
+    t->chunk = tracers_per_launch ? tracers_per_launch : 8;
+    if (t->chunk > t->opts.ntracers) t->chunk = t->opts.ntracers;
+    drop_graphs(t);  // captured launches bake the shape in
+    return MGPU_SUCCESS;
+}
+
 extern "C" int mgpu_transport_stream(mgpu_transport *t, void **stream) {
     if (!t) return MGPU_OFF_ERR;
     if (!stream) return MGPU_ERR_BADARG;
@@ -701,6 +727,8 @@ static int enqueue_steps(mgpu_transport *t, int nsteps, int cur) {
         L.Wb = t->Wb;
         L.Gb = t->Gb;
         L.lists = t->der.lists();
+        L.lanes.explicit_tpl = t->opts.explicit_tracers_per_lane;
+        L.lanes.column_tpl = t->opts.column_tracers_per_lane;
         L.counters = t->der.counters;
         // first kernel(s) of every chunk, then the second kernel(s)
         for (int ph = 1; ph <= 2; ++ph) {

@@ -124,6 +124,11 @@ int launch_build_lists(const Dims &d, const uint8_t *mask, const int32_t *kstart
                        uint32_t *rowcnt, uint32_t *cells, uint32_t *cols,
                        unsigned long long *totals, hipStream_t s);
 
+// Tracers per lane of the lane variant's kernels (0: library default).
+struct LaneShape {
+    int explicit_tpl = 0, column_tpl = 0;
+};
+
 struct TransportLaunch {
     bool fast = false;
     int variant = MGPU_KERNEL_SPLIT;
@@ -134,6 +139,7 @@ struct TransportLaunch {
     double *out = nullptr;
     double *Wb = nullptr, *Gb = nullptr;
     SolveLists lists;                       // lane variant
+    LaneShape lanes;                        // lane variant
     unsigned long long *counters = nullptr; // [0] zero pivots, [1] bad-method flag
 };
 long blocked_scratch_len(const Dims &d, int nt);

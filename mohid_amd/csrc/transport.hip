@@ -1301,39 +1301,89 @@ long blocked_scratch_len(const Dims &d, int nt) {
     return tile_count(d) * (long)(d.KUB - d.KLB + 1) * nt * TC;
 }
 
+// Lane-variant launches with TPL tracers per lane (1, 2, 4 or 8 dividing NT;
+// 0 picks the default: 4 in the explicit kernel, 2 in the column walk).
+template <bool OPT, int NT, int TPL, bool FAST>
+static int launch_explicit_tpl(const Dims &d, const StepScalars &sc, const SharedFields &f,
+                               const TracerSet &ts, const SolveLists &l, double *TW,
+                               unsigned int *bad, hipStream_t s) {
+    const long nb = (l.ncells * (NT / TPL) + 255) / 256;
+    if (nb <= 0) return MGPU_SUCCESS;
+    hipLaunchKernelGGL((explicit_lane_kernel<OPT, NT, TPL, FAST>), dim3((unsigned)nb), dim3(256),
+                       0, s, d, sc, f, ts, l.cells, l.ncells, TW, bad);
+    HIPCHK(hipGetLastError());
+    return MGPU_SUCCESS;
+}
+
+template <bool OPT, int NT, int TPL, bool FAST>
+static int launch_column_tpl(const Dims &d, const StepScalars &sc, const SharedFields &f,
+                             const TracerSet &ts, const SolveLists &l, double *TW, double *Gs,
+                             unsigned long long *counters, unsigned int *bad, hipStream_t s) {
+    const long nb = (l.ncols * (NT / TPL) + 255) / 256;
+    if (nb <= 0) return MGPU_SUCCESS;
+    hipLaunchKernelGGL((column_lane_kernel<OPT, NT, TPL, FAST>), dim3((unsigned)nb), dim3(256), 0,
+                       s, d, sc, f, ts, l.cols, l.ncols, TW, Gs, counters, bad);
+    HIPCHK(hipGetLastError());
+    return MGPU_SUCCESS;
+}
+
+// The largest of {want, then 4, 2, 1} that divides NT.
+template <int NT>
+static int lane_tpl(int want, int dflt) {
+    if (want <= 0) want = dflt;
+    for (int t : {want, 8, 4, 2, 1})
+        if (t <= want && t <= NT && NT % t == 0) return t;
+    return 1;
+}
+
+template <bool OPT, int NT, bool FAST>
+static int launch_explicit_lanes(int want, const Dims &d, const StepScalars &sc,
+                                 const SharedFields &f, const TracerSet &ts, const SolveLists &l,
+                                 double *TW, unsigned int *bad, hipStream_t s) {
+    switch (lane_tpl<NT>(want, 4)) {
+        case 8: if constexpr (NT % 8 == 0) return launch_explicit_tpl<OPT, NT, 8, FAST>(d, sc, f, ts, l, TW, bad, s); break;
+        case 4: if constexpr (NT % 4 == 0) return launch_explicit_tpl<OPT, NT, 4, FAST>(d, sc, f, ts, l, TW, bad, s); break;
+        case 2: if constexpr (NT % 2 == 0) return launch_explicit_tpl<OPT, NT, 2, FAST>(d, sc, f, ts, l, TW, bad, s); break;
+        default: break;
+    }
+    return launch_explicit_tpl<OPT, NT, 1, FAST>(d, sc, f, ts, l, TW, bad, s);
+}
+
+template <bool OPT, int NT, bool FAST>
+static int launch_column_lanes(int want, const Dims &d, const StepScalars &sc,
+                               const SharedFields &f, const TracerSet &ts, const SolveLists &l,
+                               double *TW, double *Gs, unsigned long long *counters,
+                               unsigned int *bad, hipStream_t s) {
+    switch (lane_tpl<NT>(want, 2)) {
+        case 8: if constexpr (NT % 8 == 0) return launch_column_tpl<OPT, NT, 8, FAST>(d, sc, f, ts, l, TW, Gs, counters, bad, s); break;
+        case 4: if constexpr (NT % 4 == 0) return launch_column_tpl<OPT, NT, 4, FAST>(d, sc, f, ts, l, TW, Gs, counters, bad, s); break;
+        case 2: if constexpr (NT % 2 == 0) return launch_column_tpl<OPT, NT, 2, FAST>(d, sc, f, ts, l, TW, Gs, counters, bad, s); break;
+        default: break;
+    }
+    return launch_column_tpl<OPT, NT, 1, FAST>(d, sc, f, ts, l, TW, Gs, counters, bad, s);
+}
+
 // Launch the transport kernels for one chunk of NT tracers.
 // Fused variant: phase bit 1 launches the column-tile kernel.  Split
 // variant: phase bit 1 = kernel H, bit 2 = kernel V.
 template <bool OPT, int NT, bool FAST>
 static int launch_chunk(int variant, int phase, const Dims &d, const StepScalars &sc,
                         const SharedFields &f, const TracerSet &ts, double *Wb, double *Gb,
-                        const SolveLists &lists, unsigned long long *counters, hipStream_t s) {
+                        const SolveLists &lists, const LaneShape &lanes,
+                        unsigned long long *counters, hipStream_t s) {
     const int ni = d.wIUB - d.wILB + 1, nj = d.wJUB - d.wJLB + 1, nk = d.wKUB - d.wKLB + 1;
     unsigned int *bad = reinterpret_cast<unsigned int *>(counters + 1);
     const dim3 tiles((ni + TX - 1) / TX, (nj + TY - 1) / TY);
     if (variant == MGPU_KERNEL_LANES) {
-        // tracers per lane: the explicit kernel shares a cell's
-        // tracer-independent work among TPL tracers; the column kernel keeps
-        // one column walk per lane and tracer pair
-        constexpr int TPE = NT % 4 == 0 ? 4 : (NT % 2 == 0 ? 2 : 1);
-        constexpr int TPC = NT % 2 == 0 ? 2 : 1;
         if (phase & 1) {
-            const long nb = (lists.ncells * (NT / TPE) + 255) / 256;
-            if (nb > 0) {
-                hipLaunchKernelGGL((explicit_lane_kernel<OPT, NT, TPE, FAST>),
-                                   dim3((unsigned)nb), dim3(256), 0, s, d, sc, f, ts,
-                                   lists.cells, lists.ncells, Wb, bad);
-                HIPCHK(hipGetLastError());
-            }
+            const int rc = launch_explicit_lanes<OPT, NT, FAST>(lanes.explicit_tpl, d, sc, f, ts,
+                                                                lists, Wb, bad, s);
+            if (rc) return rc;
         }
         if (phase & 2) {
-            const long nb = (lists.ncols * (NT / TPC) + 255) / 256;
-            if (nb > 0) {
-                hipLaunchKernelGGL((column_lane_kernel<OPT, NT, TPC, FAST>), dim3((unsigned)nb),
-                                   dim3(256), 0, s, d, sc, f, ts, lists.cols, lists.ncols, Wb, Gb,
-                                   counters, bad);
-                HIPCHK(hipGetLastError());
-            }
+            const int rc = launch_column_lanes<OPT, NT, FAST>(lanes.column_tpl, d, sc, f, ts,
+                                                              lists, Wb, Gb, counters, bad, s);
+            if (rc) return rc;
         }
         return MGPU_SUCCESS;
     }
@@ -1362,16 +1412,17 @@ static int launch_chunk(int variant, int phase, const Dims &d, const StepScalars
 template <bool OPT, bool FAST>
 static int launch_n(int nt, int variant, int ph, const Dims &d, const StepScalars &sc,
                     const SharedFields &f, const TracerSet &ts, double *Wb, double *Gb,
-                    const SolveLists &lists, unsigned long long *c, hipStream_t s) {
+                    const SolveLists &lists, const LaneShape &lanes, unsigned long long *c,
+                    hipStream_t s) {
     switch (nt) {
-        case 1: return launch_chunk<OPT, 1, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, c, s);
-        case 2: return launch_chunk<OPT, 2, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, c, s);
-        case 3: return launch_chunk<OPT, 3, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, c, s);
-        case 4: return launch_chunk<OPT, 4, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, c, s);
-        case 5: return launch_chunk<OPT, 5, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, c, s);
-        case 6: return launch_chunk<OPT, 6, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, c, s);
-        case 7: return launch_chunk<OPT, 7, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, c, s);
-        case 8: return launch_chunk<OPT, 8, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, c, s);
+        case 1: return launch_chunk<OPT, 1, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, lanes, c, s);
+        case 2: return launch_chunk<OPT, 2, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, lanes, c, s);
+        case 3: return launch_chunk<OPT, 3, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, lanes, c, s);
+        case 4: return launch_chunk<OPT, 4, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, lanes, c, s);
+        case 5: return launch_chunk<OPT, 5, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, lanes, c, s);
+        case 6: return launch_chunk<OPT, 6, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, lanes, c, s);
+        case 7: return launch_chunk<OPT, 7, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, lanes, c, s);
+        case 8: return launch_chunk<OPT, 8, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, lanes, c, s);
         default: return MGPU_ERR_BADARG;
     }
 }
@@ -1401,14 +1452,14 @@ int launch_transport(const TransportLaunch &L, const Dims &d, const StepScalars 
         int rc;
         if (sc.optimize)
             rc = L.fast ? launch_n<true, true>(nt, L.variant, phase, d, sc, f, ts, Wb, Gb,
-                                               L.lists, L.counters, s)
+                                               L.lists, L.lanes, L.counters, s)
                         : launch_n<true, false>(nt, L.variant, phase, d, sc, f, ts, Wb, Gb,
-                                                L.lists, L.counters, s);
+                                                L.lists, L.lanes, L.counters, s);
         else
             rc = L.fast ? launch_n<false, true>(nt, L.variant, phase, d, sc, f, ts, Wb, Gb,
-                                                L.lists, L.counters, s)
+                                                L.lists, L.lanes, L.counters, s)
                         : launch_n<false, false>(nt, L.variant, phase, d, sc, f, ts, Wb, Gb,
-                                                 L.lists, L.counters, s);
+                                                 L.lists, L.lanes, L.counters, s);
         if (rc) return rc;
     }
     return MGPU_SUCCESS;

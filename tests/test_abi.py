@@ -86,3 +86,16 @@ def test_fortran_binding_compiles(tmp_path):
         capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stderr[-2000:]
     assert (tmp_path / "adgpu.o").exists()
+
+
+def test_autotune_candidates_are_valid_shapes():
+    from mohid_amd.autotune import candidates
+    for T in range(1, 13):
+        c = candidates(T)
+        assert c, T
+        for pl, e, col in c:
+            chunk = min(pl or 8, T)
+            assert chunk % e == 0 and chunk % col == 0 and e <= chunk and col <= chunk
+            assert pl == 0 or pl < T
+        assert len(set(c)) == len(c)
+    assert (0, 4, 2) in candidates(8)

@@ -296,3 +296,27 @@ def test_device_resident_inputs_match_host_inputs():
     assert np.array_equal(prop.cpu().numpy().transpose(2, 1, 0), ref)
     gpu.kill()
     gpu_h.kill()
+
+
+def test_every_launch_shape_gives_identical_results():
+    """The lane variant's launch shapes (tracers per launch, tracers per lane
+    in the explicit and column kernels) change speed, never results: every
+    candidate the autotuner can pick reproduces the oracle bit for bit."""
+    from mohid_amd.autotune import autotune, candidates
+    _, Transport, _ = _ad()
+    case = make("tvd_ragged_opt", tracers=("temperature", "salinity") + ("lognormal",) * 6)
+    T = len(case.tracers)
+    ref = ora.run_case(case, 2, nthreads=1)
+    tr = Transport(case.size, case.worksize, case.geom, T)
+    tr.set_hydro(case.hydro, case.params)
+    for shape in candidates(T):
+        tr.set_lanes(*shape)
+        for t, p in enumerate(case.tracers):
+            tr.set_tracer(t, p)
+        tr.run(1, use_graph=True)
+        tr.run(1, use_graph=False)
+        for t in range(T):
+            assert np.array_equal(tr.get_tracer(t), ref[t]), (shape, t)
+    res = autotune(tr, steps=1)
+    assert tuple(res["shape"]) in candidates(T)
+    tr.close()

@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import glob
 import os
+import re
 import subprocess
 import sys
 
@@ -24,7 +25,28 @@ FLAGS = [
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
+    # per-kernel register/LDS/scratch report, checked below
+    "-Rpass-analysis=kernel-resource-usage",
 ]
+
+
+def check_resources(text: str) -> None:
+    """Fail the build if a kernel spills to scratch memory: every kernel here
+    is sized to keep its state in registers, and a spill (for instance an
+    array the compiler could not keep in registers) turns into hidden HBM
+    traffic."""
+    name = None
+    bad = []
+    for line in text.splitlines():
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            name = m.group(1)
+            continue
+        m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", line)
+        if m and int(m.group(1)) > 0:
+            bad.append(f"{name}: {m.group(1)} bytes/lane")
+    if bad:
+        raise RuntimeError("kernels spill to scratch memory:\n  " + "\n  ".join(bad))
 
 
 def sources():
@@ -67,9 +89,10 @@ def build(verbose: bool = False) -> str:
     cmd = [HIPCC, *FLAGS, "-o", tmp, *sources()]
     res = subprocess.run(cmd, capture_output=True, text=True)
     out = "\n".join(l for l in (res.stdout + res.stderr).splitlines()
-                    if "packed-fp32-ops" not in l and l.strip())
+                    if ("error" in l or "warning" in l) and "packed-fp32-ops" not in l)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed ({res.returncode}):\n{' '.join(cmd)}\n{out}")
+    check_resources(res.stderr)
     if verbose and out:
         print(out)
     os.replace(tmp, OUT)

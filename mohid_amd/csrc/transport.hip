@@ -276,18 +276,20 @@ struct TracerSet {
 
 // Flux of one horizontal face for NT tracers.  `c0` indexes the cell on the
 // face's high side, `st` is the stride along the line, `pos` its line index
-// and `lo` the lowest valid line index (cells below lo are outside Size).
+// and lo..hi the valid line indices (cells outside are outside Size).
 template <bool OPT, int NT, bool FAST>
 __device__ __forceinline__ void hface_flux(const Dims &d, const StepScalars &sc,
                                            const SharedFields &f, const TracerSet &ts, long c0,
-                                           long st, int pos, int lo, const double *du_line,
-                                           long du_st, double Q, double (&fl)[NT], int &bad) {
+                                           long st, int pos, int lo, int hi,
+                                           const double *du_line, long du_st, double Q,
+                                           double (&fl)[NT], int &bad) {
     double v[4], du[4];
-    bool op[4];
+    bool op[4], inl[4];
     long cell[4];
 #pragma unroll
     for (int o = 0; o < 4; ++o) {
-        const bool in = pos - 2 + o >= lo;
+        const bool in = pos - 2 + o >= lo && pos - 2 + o <= hi;
+        inl[o] = in;
         cell[o] = c0 + (long)(o - 2) * st;
         op[o] = in && (f.mask[cell[o]] & M_OPEN);
         v[o] = in ? f.V[cell[o]] : 1.0;
@@ -297,48 +299,38 @@ __device__ __forceinline__ void hface_flux(const Dims &d, const StepScalars &sc,
     for (int t = 0; t < NT; ++t) fl[t] = 0.0;
     if (!(op[1] && op[2])) return;
     if (OPT) {
-        // ComputeAdvection1D_TVD_SuperBee_2 (MF:10642-10699)
+        // ComputeAdvection1D_TVD_SuperBee_2 (MF:10642-10699).  Both flow
+        // directions run one code path on selected operands (same operations
+        // in the same order as each branch of the reference), so a wave with
+        // mixed flow signs does not execute the face twice; the near-boundary
+        // case selects the first-order coefficients at the end (the far
+        // upstream value is then unused; a cell outside Size is not read).
         const bool up = Q > 0.0;
         const bool nearb = up ? !op[0] : !op[3];
-        if (nearb) {
-            const double Dc = up ? Q : 0.0, Ec = up ? 0.0 : Q;
-#pragma unroll
-            for (int t = 0; t < NT; ++t)
-                fl[t] = Dc * ts.P[cell[1] * ts.s + t] + Ec * ts.P[cell[2] * ts.s + t];
-            return;
-        }
         const double vup = up ? v[1] : v[2];
         const double cr = FAST ? Q * sc.dt * frcp(vup) : Q * sc.dt / vup;
         const double s1 = du[2] + du[1];
         const double s0 = up ? du[1] + du[0] : du[2] + du[3];
         const double i1 = FAST ? frcp(s1) : 0.0, i0 = FAST ? frcp(s0) : 0.0;
+        const long c0m2 = cell[0], c0m1 = cell[1], c0p0 = cell[2], c0p1 = cell[3];
+        const long cfar = up ? (inl[0] ? c0m2 : c0m1) : (inl[3] ? c0p1 : c0p0);
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const double *P = ts.P + t;
             const double p1 = P[cell[1] * ts.s], p2 = P[cell[2] * ts.s];
+            const double pf = P[cfar * ts.s];
+            const double a = up ? p2 : p1, b = up ? p1 : p2;   // downstream, upstream
             double dc, rr;
-            if (up) {
-                const double p0 = P[cell[0] * ts.s];
-                if (FAST) {
-                    dc = clamp_min_value((p2 - p1) * i1);
-                    rr = (p1 - p0) * i0 * frcp(dc);
-                } else {
-                    dc = clamp_min_value((p2 - p1) / s1);
-                    rr = (p1 - p0) / s0 / dc;
-                }
+            if (FAST) {
+                dc = clamp_min_value((a - b) * i1);
+                rr = (b - pf) * i0 * frcp(dc);
             } else {
-                const double p3 = P[cell[3] * ts.s];
-                if (FAST) {
-                    dc = clamp_min_value((p1 - p2) * i1);
-                    rr = (p2 - p3) * i0 * frcp(dc);
-                } else {
-                    dc = clamp_min_value((p1 - p2) / s1);
-                    rr = (p2 - p3) / s0 / dc;
-                }
+                dc = clamp_min_value((a - b) / s1);
+                rr = (b - pf) / s0 / dc;
             }
             const double th = 0.5 * superbee_theta(rr) * (1.0 - cr);
-            const double Dc = up ? Q * (1.0 - th) : Q * th;
-            const double Ec = up ? Q * th : Q * (1.0 - th);
+            const double Dc = nearb ? (up ? Q : 0.0) : (up ? Q * (1.0 - th) : Q * th);
+            const double Ec = nearb ? (up ? 0.0 : Q) : (up ? Q * th : Q * (1.0 - th));
             fl[t] = Dc * p1 + Ec * p2;
         }
     } else {
@@ -348,7 +340,7 @@ __device__ __forceinline__ void hface_flux(const Dims &d, const StepScalars &sc,
             const double *P = ts.P + t;
             double p[4];
 #pragma unroll
-            for (int o = 0; o < 4; ++o) p[o] = (pos - 2 + o >= lo) ? P[cell[o] * ts.s] : 0.0;
+            for (int o = 0; o < 4; ++o) p[o] = inl[o] ? P[cell[o] * ts.s] : 0.0;
             const Face4 c = face_general(p, v, du, op, Q, sc.dt, sc.volrelmax, sc.methodH,
                                          sc.tvdH, sc.up2H, bad);
             fl[t] = c.c * p[0] + c.d * p[1] + c.e * p[2] + c.f * p[3];
@@ -373,15 +365,15 @@ __device__ __forceinline__ void tile_faces(const Dims &d, const StepScalars &sc,
         double fl[NT];
         if (isU) {
             if (f.mask[y] & M_CFU) {
-                hface_flux<OPT, NT, FAST>(d, sc, f, ts, y, d.sj, fj, d.JLB,
+                hface_flux<OPT, NT, FAST>(d, sc, f, ts, y, d.sj, fj, d.JLB, d.JUB,
                                           f.DUX + d.idx2(fi, fj), d.sj, f.Qx[y], fl, bad);
                 const int r = fid / TX, c = fid % TX;
 #pragma unroll
                 for (int t = 0; t < NT; ++t) FU[t][r][c] = fl[t];
             }
         } else if (!sc.xzflow && (f.mask[y] & M_CFV)) {
-            hface_flux<OPT, NT, FAST>(d, sc, f, ts, y, 1, fi, d.ILB, f.DVY + d.idx2(fi, fj), 1,
-                                      f.Qy[y], fl, bad);
+            hface_flux<OPT, NT, FAST>(d, sc, f, ts, y, 1, fi, d.ILB, d.IUB,
+                                      f.DVY + d.idx2(fi, fj), 1, f.Qy[y], fl, bad);
             const int g = fid - TFU, r = g / (TX + 1), c = g % (TX + 1);
 #pragma unroll
             for (int t = 0; t < NT; ++t) FV[t][r][c] = fl[t];
@@ -523,16 +515,9 @@ __device__ __forceinline__ void vface_coefs(const StepScalars &sc, const double 
     for (int t = 0; t < NT; ++t) co[t] = VCoef{0.0, 0.0, 0.0, 0.0};
     if (!build || !(op[1] && op[2])) return;
     if (OPT) {
+        // one code path for both flow directions (as hface_flux)
         const bool up = Q > 0.0;
         const bool nearb = up ? !op[0] : !op[3];
-        if (nearb) {
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                co[t].d = up ? Q : 0.0;
-                co[t].e = up ? 0.0 : Q;
-            }
-            return;
-        }
         const double vup = up ? v[1] : v[2];
         const double cr = FAST ? Q * sc.dt * frcp(vup) : Q * sc.dt / vup;
         const double s1 = du[2] + du[1];
@@ -540,25 +525,23 @@ __device__ __forceinline__ void vface_coefs(const StepScalars &sc, const double 
         const double i1 = FAST ? frcp(s1) : 0.0, i0 = FAST ? frcp(s0) : 0.0;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
+            // scalar copies: a select between elements of the window array
+            // would otherwise become an indexed access that keeps the window
+            // in scratch memory
+            const double q0 = p[t][0], q1 = p[t][1], q2 = p[t][2], q3 = p[t][3];
+            const double a = up ? q2 : q1, b = up ? q1 : q2;
+            const double pf = up ? q0 : q3;
             double dc, rr;
             if (FAST) {
-                if (up) {
-                    dc = clamp_min_value((p[t][2] - p[t][1]) * i1);
-                    rr = (p[t][1] - p[t][0]) * i0 * frcp(dc);
-                } else {
-                    dc = clamp_min_value((p[t][1] - p[t][2]) * i1);
-                    rr = (p[t][2] - p[t][3]) * i0 * frcp(dc);
-                }
-            } else if (up) {
-                dc = clamp_min_value((p[t][2] - p[t][1]) / s1);
-                rr = (p[t][1] - p[t][0]) / s0 / dc;
+                dc = clamp_min_value((a - b) * i1);
+                rr = (b - pf) * i0 * frcp(dc);
             } else {
-                dc = clamp_min_value((p[t][1] - p[t][2]) / s1);
-                rr = (p[t][2] - p[t][3]) / s0 / dc;
+                dc = clamp_min_value((a - b) / s1);
+                rr = (b - pf) / s0 / dc;
             }
             const double th = 0.5 * superbee_theta(rr) * (1.0 - cr);
-            co[t].d = up ? Q * (1.0 - th) : Q * th;
-            co[t].e = up ? Q * th : Q * (1.0 - th);
+            co[t].d = nearb ? (up ? Q : 0.0) : (up ? Q * (1.0 - th) : Q * th);
+            co[t].e = nearb ? (up ? 0.0 : Q) : (up ? Q * th : Q * (1.0 - th));
         }
     } else {
 #pragma unroll
@@ -951,6 +934,41 @@ __global__ void thomasz_kernel(Dims d, const double *__restrict__ D,
 // explicit fluxes of a face shared by two cells are computed by both (each
 // time with the same operations); there is no LDS and no barrier.
 
+// A lane's TPL consecutive tracer values: 16-byte vector accesses when the
+// group is 16-byte aligned, so that the lanes of one instruction cover whole
+// memory sectors (element-wise stores of interleaved tracers leave partial
+// sectors that the memory system writes back more than once).
+template <int TPL>
+__device__ __forceinline__ void load_tpl(const double *p, double (&v)[TPL]) {
+    if constexpr (TPL % 2 == 0) {
+        if ((reinterpret_cast<uintptr_t>(p) & 15) == 0) {
+#pragma unroll
+            for (int u = 0; u < TPL / 2; ++u) {
+                const double2 w = reinterpret_cast<const double2 *>(p)[u];
+                v[2 * u] = w.x;
+                v[2 * u + 1] = w.y;
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < TPL; ++u) v[u] = p[u];
+}
+
+template <int TPL>
+__device__ __forceinline__ void store_tpl(double *p, const double (&v)[TPL]) {
+    if constexpr (TPL % 2 == 0) {
+        if ((reinterpret_cast<uintptr_t>(p) & 15) == 0) {
+#pragma unroll
+            for (int u = 0; u < TPL / 2; ++u)
+                reinterpret_cast<double2 *>(p)[u] = make_double2(v[2 * u], v[2 * u + 1]);
+            return;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < TPL; ++u) p[u] = v[u];
+}
+
 // Decoded position of a Size-linear cell index.
 struct CellPos {
     int i, j, k;
@@ -1006,16 +1024,16 @@ __device__ __forceinline__ void cell_rhs_lane(const Dims &d, const StepScalars &
     // the four advective face fluxes (MF:10534-10894, 10642-10699)
     double fu0[TPL], fu1[TPL], fv0[TPL], fv1[TPL];
     if (cfu0)
-        hface_flux<OPT, TPL, FAST>(d, sc, f, ts, x, d.sj, j, d.JLB, f.DUX + d.idx2(i, j), d.sj,
-                                   f.Qx[x], fu0, bad);
+        hface_flux<OPT, TPL, FAST>(d, sc, f, ts, x, d.sj, j, d.JLB, d.JUB, f.DUX + d.idx2(i, j),
+                                   d.sj, f.Qx[x], fu0, bad);
     if (cfu1)
-        hface_flux<OPT, TPL, FAST>(d, sc, f, ts, x + d.sj, d.sj, j + 1, d.JLB,
+        hface_flux<OPT, TPL, FAST>(d, sc, f, ts, x + d.sj, d.sj, j + 1, d.JLB, d.JUB,
                                    f.DUX + d.idx2(i, j + 1), d.sj, f.Qx[x + d.sj], fu1, bad);
     if (cfv0)
-        hface_flux<OPT, TPL, FAST>(d, sc, f, ts, x, 1, i, d.ILB, f.DVY + d.idx2(i, j), 1,
+        hface_flux<OPT, TPL, FAST>(d, sc, f, ts, x, 1, i, d.ILB, d.IUB, f.DVY + d.idx2(i, j), 1,
                                    f.Qy[x], fv0, bad);
     if (cfv1)
-        hface_flux<OPT, TPL, FAST>(d, sc, f, ts, x + 1, 1, i + 1, d.ILB,
+        hface_flux<OPT, TPL, FAST>(d, sc, f, ts, x + 1, 1, i + 1, d.ILB, d.IUB,
                                    f.DVY + d.idx2(i + 1, j), 1, f.Qy[x + 1], fv1, bad);
     const bool ex = OPT && !FAST;
 #pragma unroll
@@ -1074,8 +1092,7 @@ explicit_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
     int bad = 0;
     double TI[TPL];
     cell_rhs_lane<OPT, TPL, FAST>(d, sc, f, tl, x, c.i, c.j, TI, bad);
-#pragma unroll
-    for (int t = 0; t < TPL; ++t) TW[x * ts.s + t0 + t] = TI[t];
+    store_tpl<TPL>(TW + x * ts.s + t0, TI);
     if (bad) atomicOr(badflag, 1u);
 }
 
@@ -1106,9 +1123,11 @@ column_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
     const TracerSet t1{ts.P + t, ts.out + t, s};
     double *tw = TW + t, *gs = Gs + t;
     int bad = 0;
-    for (int k = d.wKLB; k < ks && k <= KUB; ++k) {
+    {
+        double nr[TPL];
 #pragma unroll
-        for (int u = 0; u < TPL; ++u) t1.out[(base + k * sk) * s + u] = MGPU_NULL_REAL;
+        for (int u = 0; u < TPL; ++u) nr[u] = MGPU_NULL_REAL;
+        for (int k = d.wKLB; k < ks && k <= KUB; ++k) store_tpl<TPL>(t1.out + (base + k * sk) * s, nr);
     }
     const bool columnBuild = f.mask[base + KUB * sk] & M_OPEN;  // AD:2982/3004
     const bool vflux = sc.vert && !sc.vertical1d && columnBuild;
@@ -1118,25 +1137,22 @@ column_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
         if (k < ks) continue;
         const long x = base + k * sk;
         double TI[TPL], W[TPL], G[TPL];
-#pragma unroll
-        for (int u = 0; u < TPL; ++u) TI[u] = tw[x * s + u];
+        load_tpl<TPL>(tw + x * s, TI);
         column_row<OPT, TPL, FAST>(d, sc, f, t1, base, k, vflux, c, TI, W, G, zero_pivots, bad);
-#pragma unroll
-        for (int u = 0; u < TPL; ++u) {
-            tw[x * s + u] = W[u];
-            gs[x * s + u] = G[u];
-        }
+        store_tpl<TPL>(tw + x * s, W);
+        store_tpl<TPL>(gs + x * s, G);
     }
     double xs[TPL];
     column_ghost<TPL>(d, f, t1, base, c, xs);
     for (int k = KUB; k >= kw; --k) {
         if (k < ks) continue;
         const long x = base + k * sk;
+        double W[TPL], G[TPL];
+        load_tpl<TPL>(tw + x * s, W);
+        load_tpl<TPL>(gs + x * s, G);
 #pragma unroll
-        for (int u = 0; u < TPL; ++u) {
-            xs[u] = tw[x * s + u] * xs[u] + gs[x * s + u];
-            t1.out[x * s + u] = xs[u];
-        }
+        for (int u = 0; u < TPL; ++u) xs[u] = W[u] * xs[u] + G[u];
+        store_tpl<TPL>(t1.out + x * s, xs);
     }
     if (bad) atomicOr(badflag, 1u);
 }

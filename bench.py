@@ -79,6 +79,11 @@ def main():
     ap.add_argument("--kernel", default="lanes", choices=("lanes", "split", "fused"))
     ap.add_argument("--tracers-per-launch", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--suite", action="store_true",
+                    help="instead of the headline line, run the comparison suite: plain FP64 "
+                         "drop-in vs batched on 512x512x50, FAST vs FP64 per-tracer error and "
+                         "mass drift on the stiff 512x512x50 case, the autotuned "
+                         "2048x2048x60 run; one JSON line per measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 512 MiB three times (the "
                          "reference workload for scaling FETCH_SIZE/WRITE_SIZE counters)")
@@ -99,6 +104,9 @@ def main():
         print("bench.py: this build measures one GPU only (the transport step's "
               "single-GPU kernel); run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.suite:
+        suite(args)
+        return
 
     import numpy as np
     import torch
@@ -280,6 +288,123 @@ def main():
              f"{cwall:.1f}s); gpu vs oracle max rel err {worst:.3e}")
     tr.close()
     print(json.dumps(out), flush=True)
+
+
+def _timed(fn, stream, k):
+    """Wall seconds of k calls of fn(), with the device idle before and after."""
+    import torch
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    for _ in range(k):
+        fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - w0
+
+
+def _batched_rate(case, precision, steps, chunk=None, tune=True):
+    """Batched transport of all the case's tracers: autotuned shape, steps
+    replayed as captured graphs of `chunk` steps; returns (updates/s, shape)."""
+    import torch
+    from mohid_amd.advection_diffusion import Transport
+    from mohid_amd.autotune import autotune
+    T = len(case.tracers)
+    tr = Transport(case.size, case.worksize, case.geom, T, precision=precision)
+    tr.set_hydro(case.hydro, case.params)
+    for t, p in enumerate(case.tracers):
+        tr.set_tracer(t, p)
+    shape = autotune(tr, steps=2)["shape"] if tune else None
+    for t, p in enumerate(case.tracers):
+        tr.set_tracer(t, p)
+    chunk = chunk or steps
+    tr.run(chunk, use_graph=True)                   # capture + warm-up
+    wall = _timed(lambda: tr.run(chunk, use_graph=True), None, steps // chunk)
+    tr.close()
+    torch.cuda.empty_cache()
+    return T * case.n_wet * (steps // chunk) * chunk / wall, shape
+
+
+def suite(args):
+    """Comparison measurements beside the headline (one JSON line each)."""
+    import numpy as np
+    import torch
+    from mohid_amd import _lib
+    from mohid_amd import fixtures as fx
+    from mohid_amd.advection_diffusion import AdvectionDiffusion, Transport
+
+    torch.cuda.set_device(0)
+    _lib.lib()
+
+    def emit(rec):
+        print(json.dumps(rec), flush=True)
+
+    # 1. plain FP64 drop-in vs batched, one tracer, 512 x 512 x 50 sigma, TVD
+    case = fx.make_case(fx.config("c2"), device="cuda")
+    ad = AdvectionDiffusion(case.size, case.worksize, case.geom)
+    prop = case.tracers[0].clone()
+    k = 20
+    for _ in range(2):
+        ad(prop, case.hydro, case.params)
+    wall = _timed(lambda: ad(prop, case.hydro, case.params), None, k)
+    ad.kill()
+    drop = case.n_wet * k / wall
+    emit({"measure": "plain FP64 drop-in (per-call AdvectionDiffusion, device arrays)",
+          "workload": "512x512x50 sigma, 1 tracer, TVD-SuperBee, dt 30 s",
+          "value": drop, "unit": "cell-updates/s", "ms_per_call": 1e3 * wall / k})
+    b64, shp = _batched_rate(case, _lib.MGPU_PREC_FP64, 20)
+    emit({"measure": "batched FP64, same workload", "value": b64, "unit": "cell-updates/s",
+          "launch_shape": shp, "vs_plain_dropin": b64 / drop})
+    del case
+    torch.cuda.empty_cache()
+
+    # 2. the headline shape: batched FP64 vs FAST, 8 tracers, 1024 x 1024 x 40
+    case = fx.make_case(fx.config("c3"), device="cuda")
+    r64, s64 = _batched_rate(case, _lib.MGPU_PREC_FP64, 20)
+    rf, sf = _batched_rate(case, _lib.MGPU_PREC_FAST, 20)
+    emit({"measure": "batched FP64 vs FAST, 1024x1024x40 x 8 tracers",
+          "fp64": r64, "fast": rf, "unit": "cell-updates/s", "fast_over_fp64": rf / r64,
+          "launch_shapes": {"fp64": s64, "fast": sf}})
+    del case
+    torch.cuda.empty_cache()
+
+    # 3. FAST vs FP64 error per tracer and mass drift, stiff 512 x 512 x 50
+    # (8 tracers, thin surface layers, Kz up to 0.1, dt 120 s); the FP64 GPU
+    # path is bit-identical to the oracle, so it stands in for it at full size
+    case = fx.make_case(fx.config("c4"), device="cuda")
+    steps = 50
+    outs = {}
+    for name, prec in (("fp64", _lib.MGPU_PREC_FP64), ("fast", _lib.MGPU_PREC_FAST)):
+        tr = Transport(case.size, case.worksize, case.geom, len(case.tracers), precision=prec)
+        tr.set_hydro(case.hydro, case.params)
+        for t, p in enumerate(case.tracers):
+            tr.set_tracer(t, p)
+        tr.run(steps, use_graph=True)
+        outs[name] = [tr.get_tracer(t, out=torch.empty_like(case.tracers[t]))
+                      for t in range(len(case.tracers))]
+        tr.close()
+    wp = case.hydro["WaterPoints3D"] == 1
+    V = case.hydro["VolumeZ"]
+    per = []
+    for t in range(len(case.tracers)):
+        a, b, c0 = outs["fast"][t][wp], outs["fp64"][t][wp], case.tracers[t][wp]
+        rel = (a - b).abs() / b.abs().clamp_min(1e-300)
+        m0 = float((V[wp] * c0).sum())
+        per.append({"tracer": case.cfg.tracers[t], "max_rel": float(rel.max()),
+                    "rms_rel": float(rel.pow(2).mean().sqrt()),
+                    "mass_drift_fp64": abs(float((V[wp] * b).sum()) - m0) / abs(m0),
+                    "mass_drift_fast": abs(float((V[wp] * a).sum()) - m0) / abs(m0)})
+    emit({"measure": "FAST vs FP64 after 50 steps, stiff 512x512x50 sigma (dz to 0.25 m, "
+                     "Kz to 0.1, dt 120 s), 8 tracers", "per_tracer": per})
+    del case, outs
+    torch.cuda.empty_cache()
+
+    # 4. large run: 2048 x 2048 x 60 sigma, 4 tracers, graphs of 50 steps
+    case = fx.make_case(fx.config("c5"), device="cuda")
+    r64, s64 = _batched_rate(case, _lib.MGPU_PREC_FP64, 100, chunk=50)
+    rf, sf = _batched_rate(case, _lib.MGPU_PREC_FAST, 100, chunk=50)
+    emit({"measure": "batched, 2048x2048x60 sigma x 4 tracers, 25 % land, 2 graphs of 50 steps",
+          "n_wet_cells": case.n_wet, "fp64": r64, "fast": rf, "unit": "cell-updates/s",
+          "alg_GBps_fp64": r64 * (16 + 64 / 4) / 1e9, "alg_GBps_fast": rf * (16 + 64 / 4) / 1e9,
+          "launch_shapes": {"fp64": s64, "fast": sf}})
 
 
 if __name__ == "__main__":

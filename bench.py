@@ -188,7 +188,7 @@ def main():
     per_unit = 16.0 + 64.0 / T
     alg_bytes = per_unit * nwet * T
     achieved = alg_bytes / (trans_ms * 1e-3) / 1e9
-    kernel_key = f"transport_{args.precision}"
+    kernel_key = f"transport_{args.kernel}_{args.precision}"
     traffic = _traffic_from_profiles(kernel_key, cfg.name)
     _log(f"per step: prep {prep_ms:.3f} ms, explicit {h_ms:.3f} ms, vertical {v_ms:.3f} ms "
          f"({nsteps_t} steps); transport {achieved:.1f} GB/s algorithmic")
@@ -224,7 +224,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "explicit_kernel + vertical_kernel (all tracers of one step)",
+            "kernel": ("explicit_lane_kernel + column_lane_kernel" if args.kernel == "lanes"
+                       else "explicit_kernel + vertical_kernel") + " (all tracers of one step)",
             "kernel_ms": trans_ms,
             "explicit_kernel_ms": h_ms,
             "vertical_kernel_ms": v_ms,
@@ -356,12 +357,30 @@ def suite(args):
     del case
     torch.cuda.empty_cache()
 
-    # 2. the headline shape: batched FP64 vs FAST, 8 tracers, 1024 x 1024 x 40
+    # 2. the headline shape: plain FP64 drop-in (one AdvectionDiffusion call per
+    # tracer and step, FirstProperty on the first: WP:14776-14822) vs batched
+    # FP64 and FAST, 8 tracers, 1024 x 1024 x 40
     case = fx.make_case(fx.config("c3"), device="cuda")
+    ad = AdvectionDiffusion(case.size, case.worksize, case.geom)
+    props = [t.clone() for t in case.tracers]
+
+    def dropin_step():
+        for t, p in enumerate(props):
+            ad(p, case.hydro, case.params, first_property=(t == 0))
+
+    dropin_step()
+    k = 5
+    wall = _timed(dropin_step, None, k)
+    ad.kill()
+    del props
+    drop3 = len(case.tracers) * case.n_wet * k / wall
+    emit({"measure": "plain FP64 drop-in, 1024x1024x40 x 8 tracers (8 calls per step)",
+          "value": drop3, "unit": "cell-updates/s", "ms_per_step": 1e3 * wall / k})
     r64, s64 = _batched_rate(case, _lib.MGPU_PREC_FP64, 20)
     rf, sf = _batched_rate(case, _lib.MGPU_PREC_FAST, 20)
     emit({"measure": "batched FP64 vs FAST, 1024x1024x40 x 8 tracers",
           "fp64": r64, "fast": rf, "unit": "cell-updates/s", "fast_over_fp64": rf / r64,
+          "fp64_over_plain_dropin": r64 / drop3, "fast_over_plain_dropin": rf / drop3,
           "launch_shapes": {"fp64": s64, "fast": sf}})
     del case
     torch.cuda.empty_cache()

@@ -176,10 +176,11 @@ struct DevDerived {
     }
     // masks -> mask bytes, kstart, tile orders (after the mask arrays are on the device)
     int pack(const Dims &d, const DevHydro &h, const int32_t *kfloor, int docycle,
-             hipStream_t s) {
+             hipStream_t s, bool tiles) {
         int rc = launch_pack_masks(d, h.open, h.land, h.cfu, h.cfv, h.cfw, h.water, kfloor,
                                    docycle, mask, kstart, mask_flags(), s);
-        if (!rc) rc = launch_tile_sort(d, mask, kstart, cperm, ccnt, fperm, fcnt, s);
+        // tile metadata only for the tile variants (level keys are bytes there)
+        if (!rc && tiles) rc = launch_tile_sort(d, mask, kstart, cperm, ccnt, fperm, fcnt, s);
         if (!rc) rc = launch_build_lists(d, mask, kstart, rowcnt, cells, cols, totals, s);
         if (rc) return rc;
         unsigned long long n[2];
@@ -379,7 +380,7 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     else HIPCHK(hipMemcpyAsync(ad->prop, prop, b3, hipMemcpyHostToDevice, s));
 
     HIPCHK(hipMemsetAsync(ad->der.counters, 0, sizeof(unsigned long long) * 4, s));
-    rc = ad->der.pack(d, ad->hydro, ad->geom.KFloor, ad->docycle, s);
+    rc = ad->der.pack(d, ad->hydro, ad->geom.KFloor, ad->docycle, s, false);
     if (rc) return rc;
     if (first) {
         // Convert_Visc_Dif_Horizontal(_opt) + Convert_Dif_Vertical (+ the
@@ -616,7 +617,8 @@ extern "C" int mgpu_transport_set_hydro(mgpu_transport *t, const mgpu_ad_args *a
     rc = t->hydro.upload(t->d, a, t->stream);
     if (rc) return rc;
     HIPCHK(hipMemsetAsync(t->der.counters, 0, sizeof(unsigned long long) * 4, t->stream));
-    rc = t->der.pack(t->d, t->hydro, t->geom.KFloor, t->opts.docycle_method, t->stream);
+    rc = t->der.pack(t->d, t->hydro, t->geom.KFloor, t->opts.docycle_method, t->stream,
+                     t->kernel != MGPU_KERNEL_LANES);
     if (rc) return rc;
     unsigned long long cnt[4];
     HIPCHK(hipMemcpyAsync(cnt, t->der.counters, sizeof(cnt), hipMemcpyDeviceToHost, t->stream));

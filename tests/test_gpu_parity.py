@@ -320,3 +320,30 @@ def test_every_launch_shape_gives_identical_results():
     res = autotune(tr, steps=1)
     assert tuple(res["shape"]) in candidates(T)
     tr.close()
+
+
+@pytest.mark.parametrize("shape", [
+    dict(base="c3", ni=9, nj=7, nk=5, land_frac=1.0),                 # no water at all
+    dict(base="c1", ni=1, nj=2, nk=6),                                # one pair of columns
+    dict(base="c2", ni=12, nj=10, nk=1, depth=(5.0, 30.0)),           # one layer (2-D)
+    dict(base="c3", ni=70, nj=9, nk=11, kfloor=(1, 11), land_frac=0.5,
+         tracers=("temperature", "salinity", "lognormal")),           # ragged, two i-tiles
+])
+def test_edge_shapes_match_oracle(shape):
+    """Degenerate and edge domains through the drop-in and the batched path."""
+    from mohid_amd import fixtures as fx
+    AD, Transport, _ = _ad()
+    shape = dict(shape)
+    case = fx.make_case(fx.config(shape.pop("base"), **shape))
+    T = len(case.tracers)
+    ref = ora.run_case(case, 3, nthreads=1)
+    got = _batched(case, 2, 1)
+    gpu = AD(case.size, case.worksize, case.geom)
+    drop = [np.array(t, order="F", copy=True) for t in case.tracers]
+    for s in range(3):
+        for t in range(T):
+            gpu(drop[t], case.hydro, case.params, first_property=(t == 0))
+    gpu.kill()
+    for t in range(T):
+        assert np.array_equal(got[t], ref[t]), t
+        assert np.array_equal(drop[t], ref[t]), t

@@ -371,6 +371,56 @@ void ora_thomasz(const ora_size3d *size, const ora_size3d *ws,
         }
 }
 
+/* MF:3667-3740 THOMAS_3D_NewType, dispatching on (di, dj) to
+ * THOMAS_3D_i0_j1_NewType (MF:3756-3810, lines along j) or
+ * THOMAS_3D_i1_j0_NewType (MF:3814-3870, lines along i).  Every line is
+ * independent, so the reference's OpenMP split over K does not matter. */
+int ora_thomas3d(const ora_size3d *size, int IJmin, int IJmax, int JImin, int JImax,
+                 int Kmin, int Kmax, int di, int dj, const double *D, const double *E,
+                 const double *F, const double *TI, double *ans, double *W, double *G,
+                 int32_t *where)
+{
+    long nI = size->IUB - size->ILB + 1, nJ = size->JUB - size->JLB + 1;
+    long sIJ, sJI;
+    int lbIJ, lbJI, K, IJ;
+    if (di == 0 && dj == 1) {
+        sIJ = 1; sJI = nI; lbIJ = size->ILB; lbJI = size->JLB;
+    } else if (di == 1 && dj == 0) {
+        sIJ = nI; sJI = 1; lbIJ = size->JLB; lbJI = size->ILB;
+    } else {
+        return ORA_ERR_OPTION;
+    }
+    for (K = Kmin; K <= Kmax; K++)
+        for (IJ = IJmin; IJ <= IJmax; IJ++) {
+            long base = nI * nJ * ((long)K - size->KLB) + ((long)IJ - lbIJ) * sIJ;
+#define XJ(ji) (base + ((long)(ji) - lbJI) * sJI)
+            int JI, II;
+            W[JImin] = -F[XJ(JImin)] / E[XJ(JImin)];
+            G[JImin] = TI[XJ(JImin)] / E[XJ(JImin)];
+            for (JI = JImin + 1; JI <= JImax + 1; JI++) {
+                double aux = E[XJ(JI)] + D[XJ(JI)] * W[JI - 1];
+                if (fabs(aux) > 0.0) {
+                    W[JI] = -F[XJ(JI)] / aux;
+                    G[JI] = (TI[XJ(JI)] - D[XJ(JI)] * G[JI - 1]) / aux;
+                } else {
+                    if (where) {
+                        where[0] = di ? JI : IJ;
+                        where[1] = di ? IJ : JI;
+                        where[2] = K;
+                    }
+                    return ORA_ERR_PIVOT;
+                }
+            }
+            ans[XJ(JImax + 1)] = G[JImax + 1];
+            for (II = JImin + 1; II <= JImax + 1; II++) {
+                int MM = JImax + JImin + 1 - II;
+                ans[XJ(MM)] = W[MM] * ans[XJ(MM + 1)] + G[MM];
+            }
+#undef XJ
+        }
+    return ORA_SUCCESS;
+}
+
 /* ------------------------------------------------------------------------ */
 static double *alloc_fill(long n, double v)
 {
@@ -731,7 +781,29 @@ static int horizontal_advection_xx(ora_instance *me, const ora_args *a, const do
             }
         }
     }
-    if (a->ImpExp_AdvXX != EXPLICIT) return rc ? rc : ORA_ERR_UNSUPPORTED;
+    if (rc) return rc;
+    if (a->ImpExp_AdvXX == IMPLICIT) {
+        /* AD:4490-4525: the faces' D/E coefficients go into the rows of the
+         * two cells (tridiagonal along j); each k plane is independent */
+#pragma omp parallel for schedule(static)
+        for (k = w->KLB; k <= w->KUB; k++) {
+            int j, i;
+            for (j = w->JLB; j <= w->JUB; j++)
+                for (i = w->ILB; i <= w->IUB; i++) {
+                    long x = I3(me, i, j, k), xm = I3(me, i, j - 1, k);
+                    double dt2, dt1;
+                    if (a->ComputeFacesU3D[x] != 1) continue;
+                    dt2 = dt / a->VolumeZ[x];
+                    dt1 = dt / a->VolumeZ[g_mistake == 9 ? x : xm];
+                    me->D[x] = me->D[x] - me->xD[x] * dt2;
+                    me->E[x] = me->E[x] - me->xE[x] * dt2;
+                    me->E[xm] = me->E[xm] + me->xD[x] * dt1;
+                    me->F[xm] = me->F[xm] + (g_mistake == 8 ? me->xD[x] : me->xE[x]) * dt1;
+                }
+        }
+        return 0;
+    }
+    if (a->ImpExp_AdvXX != EXPLICIT) return ORA_ERR_OPTION;   /* AD:4527 ERR01 */
 #pragma omp parallel for schedule(static)
     for (k = w->KLB; k <= w->KUB; k++) {
         int j, i;
@@ -781,7 +853,30 @@ static int horizontal_advection_yy(ora_instance *me, const ora_args *a, const do
             }
         }
     }
-    if (a->ImpExp_AdvYY != EXPLICIT) return rc ? rc : ORA_ERR_UNSUPPORTED;
+    if (rc) return rc;
+    if (a->ImpExp_AdvYY == IMPLICIT) {
+        /* AD:4858-4885, tridiagonal along i */
+        for (k = w->KLB; k <= w->KUB; k++) {
+            int j;
+#pragma omp parallel for schedule(static)
+            for (j = w->JLB; j <= w->JUB; j++) {
+                int i;
+                for (i = w->ILB; i <= w->IUB; i++) {
+                    long x = I3(me, i, j, k), xm = I3(me, i - 1, j, k);
+                    double dt2, dt1;
+                    if (a->ComputeFacesV3D[x] != 1) continue;
+                    dt2 = dt / a->VolumeZ[x];
+                    dt1 = dt / a->VolumeZ[xm];
+                    me->D[x] = me->D[x] - me->yD[x] * dt2;
+                    me->E[x] = me->E[x] - me->yE[x] * dt2;
+                    me->E[xm] = me->E[xm] + me->yD[x] * dt1;
+                    me->F[xm] = me->F[xm] + me->yE[x] * dt1;
+                }
+            }
+        }
+        return 0;
+    }
+    if (a->ImpExp_AdvYY != EXPLICIT) return ORA_ERR_OPTION;   /* AD:4887 ERR01 */
     for (k = w->KLB; k <= w->KUB; k++) {
         int j;
 #pragma omp parallel for schedule(static)
@@ -1042,12 +1137,13 @@ static int vertical_advection(ora_instance *me, const ora_args *a, const double 
     return rc;
 }
 
-/* AD:1624-1922 AdvectionDiffusionIteration (explicit-horizontal branch) */
+/* AD:1624-1922 AdvectionDiffusionIteration */
 static int iteration(ora_instance *me, const ora_args *a, double *P)
 {
     const ora_size3d *w = &me->ws;
     long n3 = me->n3;
     int rc = 0;
+    const int impXX = a->ImpExp_AdvXX == IMPLICIT, impYY = a->ImpExp_AdvYY == IMPLICIT;
 
     if (a->Optimize) {
         if (a->FirstProperty) {
@@ -1086,6 +1182,24 @@ static int iteration(ora_instance *me, const ora_args *a, double *P)
             rc = horizontal_advection_yy(me, a, P);
             if (rc) return rc;
         }
+        /* AD:4167-4256: in 3-D the implicit direction is solved at once into
+         * PROP, and the vertical step starts again from the identity rows with
+         * TI = PROP */
+        if ((impXX || impYY) && w->KUB > 1) {
+            rc = impXX ? ora_thomas3d(&me->size, w->ILB, w->IUB, w->JLB, w->JUB, w->KLB,
+                                      w->KUB, 0, 1, me->D, me->E, me->F, me->TI, P,
+                                      me->vecW, me->vecG, NULL)
+                       : ora_thomas3d(&me->size, w->JLB, w->JUB, w->ILB, w->IUB, w->KLB,
+                                      w->KUB, 1, 0, me->D, me->E, me->F, me->TI, P,
+                                      me->vecW, me->vecG, NULL);
+            if (rc) return rc;
+            if (g_mistake != 10) {
+                fill(me->D, n3, 0.0);
+                fill(me->E, n3, 1.0);
+                fill(me->F, n3, 0.0);
+            }
+            memcpy(me->TI, P, sizeof(double) * (size_t)n3);
+        }
     }
     if (w->KUB > 1) {
         if (a->Optimize) vertical_diffusion2(me, a, P);
@@ -1102,6 +1216,14 @@ static int iteration(ora_instance *me, const ora_args *a, double *P)
         for (q = 0; q < n3; q++)
             if (a->LandPoints3D[q] == 1) me->TI[q] = NULL_REAL;
     }
+    /* AD:1758-1843: one layer with implicit horizontal advection solves the
+     * lines instead of the columns */
+    if (w->KUB == 1 && impXX)
+        return ora_thomas3d(&me->size, w->ILB, w->IUB, w->JLB, w->JUB, w->KLB, w->KUB, 0, 1,
+                            me->D, me->E, me->F, me->TI, P, me->vecW, me->vecG, NULL);
+    if (w->KUB == 1 && impYY)
+        return ora_thomas3d(&me->size, w->JLB, w->JUB, w->ILB, w->IUB, w->KLB, w->KUB, 1, 0,
+                            me->D, me->E, me->F, me->TI, P, me->vecW, me->vecG, NULL);
     /* AD:1858-1867 THOMASZ_NewType2 over WorkSize, OpenMP over j */
     {
         long nI = me->nI, nJ = me->nJ;
@@ -1143,8 +1265,10 @@ int ora_advection_diffusion(ora_instance *me, const ora_args *a, double *prop)
     /* AD:1340-1349 */
     if (a->ImpExp_DifH != 0.0) return ORA_ERR_OPTION;
     if (a->ImpExp_AdvXX == IMPLICIT && a->ImpExp_AdvYY == IMPLICIT) return ORA_ERR_OPTION;
-    /* implicit horizontal advection needs THOMAS_3D: not restated */
-    if (a->ImpExp_AdvXX != EXPLICIT || a->ImpExp_AdvYY != EXPLICIT) return ORA_ERR_UNSUPPORTED;
+    /* HorizontalAdvectionXX/YY stop on any other weight (AD:4527, 4887) */
+    if ((a->ImpExp_AdvXX != EXPLICIT && a->ImpExp_AdvXX != IMPLICIT) ||
+        (a->ImpExp_AdvYY != EXPLICIT && a->ImpExp_AdvYY != IMPLICIT))
+        return ORA_ERR_OPTION;
 
     aa = *a;
     optimize = a->Optimize && !me->vertical1d;   /* AD:1419 */

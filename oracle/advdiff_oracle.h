@@ -41,7 +41,8 @@ enum {
     ORA_ERR_OPTION = 200,      /* AdvectionDiffusion ERR02/ERR03/ERR100/ERR200 */
     ORA_ERR_METHOD = 201,      /* "This method is not valid to compute Advection1D" */
     ORA_ERR_UNSUPPORTED = 202, /* branch not restated (discharges, open boundary, ...) */
-    ORA_ERR_BADARG = 203
+    ORA_ERR_BADARG = 203,
+    ORA_ERR_PIVOT = 204        /* zero pivot in THOMAS_3D: the reference stops (MF ERR10) */
 };
 
 /* Geometry the reference fetches inside AdvectionDiffusion
@@ -94,7 +95,11 @@ const double *ora_internal(const ora_instance *me, const char *name);
  *   4 YY diffusion update of cell i-1 divided by the volume of cell i
  *   5 THOMASZ back substitution with W(k+1)
  *   6 theta-scheme vertical diffusion with DZZ(k) instead of DZZ(k-1)
- *   7 optimised vertical diffusion constant with DZZ(k) instead of DZZ(k-1) */
+ *   7 optimised vertical diffusion constant with DZZ(k) instead of DZZ(k-1)
+ *   8 implicit XX advection: F row of cell j-1 built from the face's D flux
+ *   9 implicit XX advection: cell j-1 rows scaled by the volume of cell j
+ *  10 implicit horizontal advection in 3-D: rows not reset to the identity
+ *     before the vertical step */
 void ora_plant_mistake(int id);
 
 /* THOMASZ_NewType2 (ModuleFunctions.F90:4026-4123) on explicit arrays.
@@ -103,6 +108,18 @@ void ora_thomasz(const ora_size3d *size, const ora_size3d *ws,
                  const double *D, const double *E, const double *F,
                  const double *TI, const int32_t *waterpoints, double *res,
                  double *W, double *G);
+
+/* THOMAS_3D_NewType (ModuleFunctions.F90:3667-3740) on explicit arrays:
+ * di=0, dj=1 solves lines along j (IJ = i, JI = j, MF:3756-3810); di=1,
+ * dj=0 lines along i (IJ = j, JI = i, MF:3814-3870).  Rows JImin..JImax+1
+ * of every line IJmin..IJmax, Kmin..Kmax go to ans.  W, G: scratch indexed
+ * by JI, length >= JImax+2.  Returns ORA_ERR_PIVOT at a zero pivot (the
+ * reference stops there) with its (i, j, k) in where[] when where is not
+ * NULL, ORA_ERR_OPTION for another (di, dj) (ERR01). */
+int ora_thomas3d(const ora_size3d *size, int IJmin, int IJmax, int JImin, int JImax,
+                 int Kmin, int Kmax, int di, int dj, const double *D, const double *E,
+                 const double *F, const double *TI, double *ans, double *W, double *G,
+                 int32_t *where);
 
 /* ComputeAdvectionFace (ModuleFunctions.F90:10702-10894) for one face;
  * returns ORA_ERR_METHOD where the reference stops. */

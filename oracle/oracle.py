@@ -21,6 +21,9 @@ REF_PATH = os.path.join(HERE, "_ref", "libmohid_mf_ref.so")
 _dp = ct.POINTER(ct.c_double)
 _ip = ct.POINTER(ct.c_int32)
 
+# status codes of advdiff_oracle.h
+OK, ERR_OPTION, ERR_METHOD, ERR_UNSUPPORTED, ERR_BADARG, ERR_PIVOT = 0, 200, 201, 202, 203, 204
+
 
 class Size3D(ct.Structure):
     _fields_ = [(n, ct.c_int32) for n in ("ILB", "IUB", "JLB", "JUB", "KLB", "KUB")]
@@ -76,6 +79,10 @@ def lib():
         L.ora_internal.restype = _dp
         L.ora_thomasz.argtypes = [ct.POINTER(Size3D), ct.POINTER(Size3D), _dp, _dp, _dp, _dp,
                                   _ip, _dp, _dp, _dp]
+        L.ora_thomas3d.argtypes = [ct.POINTER(Size3D), ct.c_int, ct.c_int, ct.c_int, ct.c_int,
+                                   ct.c_int, ct.c_int, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp,
+                                   _dp, _dp, _dp, _ip]
+        L.ora_thomas3d.restype = ct.c_int
         L.ora_advection1d_v2.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_double, _dp, _dp,
                                          _dp, _dp, _ip, _dp, _dp, _dp, _dp, ct.c_int, ct.c_int,
                                          ct.c_double, ct.c_int]
@@ -102,6 +109,8 @@ def ref():
         R = ct.CDLL(REF_PATH)
         R.ref_thomasz.argtypes = [_ip, _ip, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp, _ip, _dp,
                                   _dp, _dp]
+        R.ref_thomas3d.argtypes = [_ip, _ip, ct.c_int, ct.c_int, ct.c_int, ct.c_int, _dp, _dp,
+                                   _dp, _dp, _dp, _dp, _dp]
         R.ref_advection1d_v2.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_double, _dp, _dp,
                                          _dp, _dp, _ip, _dp, _dp, _dp, _dp, ct.c_int, ct.c_int,
                                          ct.c_double, ct.c_int]

@@ -12,7 +12,7 @@
 !------------------------------------------------------------------------------
 module ref_mf_bind
     use iso_c_binding
-    use ModuleFunctions, only : THOMASZ_NewType2, ComputeAdvection1D_V2,       &
+    use ModuleFunctions, only : THOMASZ_NewType2, THOMAS_3D, ComputeAdvection1D_V2, &
                                 ComputeAdvection1D_TVD_SuperBee_2,              &
                                 ComputeAdvectionFace, T_THOMAS, T_D_E_F, T_VECGW
     implicit none
@@ -64,6 +64,50 @@ contains
         deallocate(th%COEF3)
         deallocate(th)
     end subroutine ref_thomasz
+
+    !> THOMAS_3D (the T_THOMAS form, THOMAS_3D_NewType, ModuleFunctions.F90:
+    !> 3667-3870) on caller arrays that span Size, column-major.  lim =
+    !> IJmin, IJmax, JImin, JImax, Kmin, Kmax; vecw/vecg as for ref_thomasz.
+    subroutine ref_thomas3d(sz, lim, di, dj, lo, hi, D, E, F, TI, RES, vecw, vecg) &
+               bind(c, name='ref_thomas3d')
+        integer(c_int), intent(in)        :: sz(6), lim(6)
+        integer(c_int), value             :: di, dj, lo, hi
+        type(c_ptr), value                :: D, E, F, TI, RES, vecw, vecg
+        real,    pointer :: fD(:,:,:), fE(:,:,:), fF(:,:,:), fTI(:,:,:), fR(:,:,:)
+        real,    pointer :: bD(:,:,:), bE(:,:,:), bF(:,:,:), bTI(:,:,:), bR(:,:,:)
+        real,    pointer :: fw(:), fg(:)
+        type(T_THOMAS), pointer :: th
+        integer :: shp(3)
+
+        shp = [sz(2)-sz(1)+1, sz(4)-sz(3)+1, sz(6)-sz(5)+1]
+        call c_f_pointer(D,   fD,  shp); bD (sz(1):, sz(3):, sz(5):) => fD
+        call c_f_pointer(E,   fE,  shp); bE (sz(1):, sz(3):, sz(5):) => fE
+        call c_f_pointer(F,   fF,  shp); bF (sz(1):, sz(3):, sz(5):) => fF
+        call c_f_pointer(TI,  fTI, shp); bTI(sz(1):, sz(3):, sz(5):) => fTI
+        call c_f_pointer(RES, fR,  shp); bR (sz(1):, sz(3):, sz(5):) => fR
+        call c_f_pointer(vecw, fw, [hi-lo+1])
+        call c_f_pointer(vecg, fg, [hi-lo+1])
+
+        allocate(th)
+        allocate(th%COEF3)
+        th%COEF3%D => bD
+        th%COEF3%E => bE
+        th%COEF3%F => bF
+        th%TI      => bTI
+        allocate(th%VEC(1))
+        allocate(th%VEC(1)%W(lo:hi), th%VEC(1)%G(lo:hi))
+        th%VEC(1)%W(lo:hi) = fw
+        th%VEC(1)%G(lo:hi) = fg
+
+        call THOMAS_3D(lim(1), lim(2), lim(3), lim(4), lim(5), lim(6), di, dj, th, bR)
+
+        fw = th%VEC(1)%W(lo:hi)
+        fg = th%VEC(1)%G(lo:hi)
+        deallocate(th%VEC(1)%W, th%VEC(1)%G)
+        deallocate(th%VEC)
+        deallocate(th%COEF3)
+        deallocate(th)
+    end subroutine ref_thomas3d
 
     !> ComputeAdvection1D_V2 (ModuleFunctions.F90:10534-10591) on 1-based lines of length n.
     subroutine ref_advection1d_v2(n, ilb, iub, dt, du, prop, q, v, cp, C, D, E, F,  &

@@ -15,7 +15,12 @@ mathematics fixes:
   here from the textbook finite-volume balance and solved with numpy;
 * invariants of a closed basin with divergence-free transports and
   VolumeZOld = VolumeZ (every fixture): sum(V * C) over water cells is
-  conserved and a constant field stays constant, on every small case.
+  conserved and a constant field stays constant, on every small case;
+* implicit horizontal advection (the THOMAS_3D path, AD:1758-1843,
+  4167-4256, 4490-4525, 4858-4885): dense line solves of implicit upwind
+  transport at a Courant number of 3, the 3-D
+  fractional step (line solves, then the dense vertical diffusion solve of
+  their result), and closed-basin mass conservation on every small case.
 
 ``test_planted_mistakes_are_caught`` plants each of a list of plausible
 mistakes in the oracle (a flux sign, the upwind side, the volume ratio, a
@@ -230,12 +235,128 @@ def test_vertical_advection_dense_solve(docycle):
 
 
 # ---------------------------------------------------------------------------
+# implicit horizontal advection (THOMAS_3D lines)
+def _line_solves(c0, hy, axis, q, dt):
+    """Dense solve, line by line along `axis`, of implicit first-order upwind
+    transport  x_c + dt / V_c (F_out - F_in) = C_c VolumeZOld_c / V_c  with
+    F_f = q * x(upwind cell of face f) on the faces that compute."""
+    ax = 1 if axis == "x" else 0
+    cf = hy["ComputeFacesU3D" if axis == "x" else "ComputeFacesV3D"] == 1
+    V, vold = hy["VolumeZ"], hy["VolumeZOld"]
+    out = np.array(c0, copy=True)
+    ni, nj, nk = c0.shape
+    lines = [(a, k) for a in range(c0.shape[1 - ax]) for k in range(nk)]
+    for a, k in lines:
+        def at(arr, c):
+            return arr[a, c, k] if ax == 1 else arr[c, a, k]
+        n = c0.shape[ax]
+        A = np.eye(n)
+        b = np.array([at(c0, c) * at(vold, c) / at(V, c) for c in range(n)])
+        for c in range(1, n):                  # face between c-1 and c
+            if not at(cf, c):
+                continue
+            up = c - 1 if q > 0 else c
+            A[c - 1, up] += dt / at(V, c - 1) * q
+            A[c, up] -= dt / at(V, c) * q
+        x = np.linalg.solve(A, b)
+        for c in range(n):
+            if ax == 1:
+                out[a, c, k] = x[c]
+            else:
+                out[c, a, k] = x[c]
+    return out
+
+
+def _implicit_channel_err(axis, sign, nk):
+    """Implicit first-order upwind along a channel at a Courant number of 3
+    against the dense line solves.  With one layer the line solve is the
+    whole step (AD:1758-1843); with two, the vertical step that follows has
+    no vertical transport or diffusion and keeps its result."""
+    ni, nj = (3, 12) if axis == "x" else (12, 3)
+    case = fx.make_case(fx.config("c1", ni=ni, nj=nj, nk=nk))
+    hy = _still_hydro(case)
+    V = hy["VolumeZ"]
+    wp = hy["WaterPoints3D"] == 1
+    q = sign * 3.0 * float(V[wp].min()) / DT
+    name, cf = ("Wflux_X", "ComputeFacesU3D") if axis == "x" else ("Wflux_Y", "ComputeFacesV3D")
+    hy[name][hy[cf] == 1] = q
+    rng = np.random.default_rng(17)
+    c0 = np.asfortranarray(np.where(wp, np.exp(rng.normal(size=wp.shape)), 0.0))
+    params = dict(UPWIND, ImpExp_AdvXX=1.0 if axis == "x" else 0.0,
+                  ImpExp_AdvYY=1.0 if axis == "y" else 0.0)
+    got = _run(case, c0.copy(order="F"), hydro=hy, **params)
+    want = _line_solves(c0, hy, axis, q, DT)
+    return float(np.max(np.abs(got[wp] - want[wp]) / np.abs(want[wp])))
+
+
+@pytest.mark.parametrize("axis", ["x", "y"])
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+@pytest.mark.parametrize("nk", [1, 2])
+def test_implicit_channel_dense_solve(axis, sign, nk):
+    assert _implicit_channel_err(axis, sign, nk) <= 1e-12
+
+
+def _implicit_fractional_err(axis):
+    """The 3-D implicit step is a fractional step (AD:4167-4256): the line
+    solves of the horizontal transport first, then the implicit vertical
+    diffusion of their result, each a dense solve here (stretched layers,
+    moving channel flow in every layer)."""
+    ni, nj = (3, 9) if axis == "x" else (9, 3)
+    case = fx.make_case(fx.config("c4", ni=ni, nj=nj, nk=6, depth=(5.0, 40.0)))
+    hy = _still_hydro(case)
+    hy["Diff_V"][...] = case.hydro["Diff_V"]
+    V = hy["VolumeZ"]
+    wp = hy["WaterPoints3D"] == 1
+    q = 2.0 * float(V[wp].min()) / DT
+    name, cf = ("Wflux_X", "ComputeFacesU3D") if axis == "x" else ("Wflux_Y", "ComputeFacesV3D")
+    hy[name][hy[cf] == 1] = q
+    rng = np.random.default_rng(23)
+    c0 = np.asfortranarray(np.where(wp, rng.uniform(1.0, 9.0, size=wp.shape), 0.0))
+    params = dict(UPWIND, ImpExp_DifV=1.0, SchmidtCoef_V=1.0, SchmidtBackground_V=0.0,
+                  ImpExp_AdvXX=1.0 if axis == "x" else 0.0,
+                  ImpExp_AdvYY=1.0 if axis == "y" else 0.0)
+    got = _run(case, c0.copy(order="F"), hydro=hy, **params)
+    star = _line_solves(c0, hy, axis, q, DT)
+    g = case.geom
+    KUB = case.worksize[5]
+    worst = 0.0
+    for i in range(1, case.worksize[1] + 1):
+        for j in range(1, case.worksize[3] + 1):
+            if not wp[i, j, KUB]:
+                continue
+            ks = np.arange(int(g["KFloorZ"][i, j]), KUB + 1)
+            n = ks.size
+            Vc = V[i, j, ks]
+            L = np.zeros((n, n))
+            for m in range(1, n):
+                k = ks[m]
+                if hy["ComputeFacesW3D"][i, j, k] != 1:
+                    continue
+                dist = 0.5 * (g["DWZ"][i, j, k - 1] + g["DWZ"][i, j, k])
+                cond = hy["Diff_V"][i, j, k] * g["DUX"][i, j] * g["DVY"][i, j] / dist
+                L[m, m] += cond; L[m, m - 1] -= cond
+                L[m - 1, m - 1] += cond; L[m - 1, m] -= cond
+            x = np.linalg.solve(np.eye(n) + DT / Vc[:, None] * L, star[i, j, ks])
+            worst = max(worst, float(np.max(np.abs(got[i, j, ks] - x) / np.abs(x))))
+    return worst
+
+
+@pytest.mark.parametrize("axis", ["x", "y"])
+def test_implicit_fractional_step_dense_solve(axis):
+    assert _implicit_fractional_err(axis) <= 1e-12
+
+
+# ---------------------------------------------------------------------------
 # closed-basin invariants on every small case
-def _invariant_errs(name, steps=3):
+def _invariant_errs(name, steps=3, implicit=None):
     case = make(name)
     wp = case.hydro["WaterPoints3D"] == 1
     V = case.hydro["VolumeZ"]
     assert np.array_equal(case.hydro["VolumeZOld"], V)
+    imp = {} if implicit is None else dict(ImpExp_AdvXX=float(implicit == "x"),
+                                           ImpExp_AdvYY=float(implicit == "y"))
+    if imp and case.params["AdvMethodH"] in (C.UPWIND_ORDER2, C.UPWIND_ORDER3):
+        imp["AdvMethodH"] = C.UPWIND_ORDER1       # AD:1231-1239
     inst = ora.Instance(case, nthreads=2)
     props = [np.array(t, order="F", copy=True) for t in case.tracers]
     const = np.asfortranarray(np.where(wp, 7.0, case.tracers[0]))
@@ -243,12 +364,12 @@ def _invariant_errs(name, steps=3):
     dt = case.params["DTProp"]
     for s in range(steps):
         for t, p in enumerate(props):
-            assert inst.step(p, first_property=(t == 0), now=(s + 1) * dt) == 0
+            assert inst.step(p, first_property=(t == 0), now=(s + 1) * dt, **imp) == 0
     inst.close()
     # the constant field alone, through the same option set
     inst = ora.Instance(case, nthreads=2)
     for s in range(steps):
-        assert inst.step(const, first_property=True, now=(s + 1) * dt) == 0
+        assert inst.step(const, first_property=True, now=(s + 1) * dt, **imp) == 0
     inst.close()
     mass = max(abs(float(np.sum(V[wp] * p[wp])) - m) / m for p, m in zip(props, m0))
     constancy = float(np.max(np.abs(const[wp] - 7.0)) / 7.0)
@@ -260,6 +381,16 @@ def test_closed_basin_conserves_mass_and_constants(name):
     mass, constancy = _invariant_errs(name)
     assert mass <= 1e-13, mass
     assert constancy <= 1e-12, constancy
+
+
+@pytest.mark.parametrize("implicit", ["x", "y"])
+@pytest.mark.parametrize("name", sorted(SMALL))
+def test_closed_basin_conserves_mass_implicit(name, implicit):
+    """Implicit horizontal advection keeps sum(V * C) (each face enters the
+    rows of its two cells with opposite signs, volume-weighted).  A constant
+    is not kept exactly: the fractional step splits the divergence."""
+    mass, _ = _invariant_errs(name, implicit=implicit)
+    assert mass <= 1e-13, mass
 
 
 # ---------------------------------------------------------------------------
@@ -274,14 +405,23 @@ def _all_checks():
     for opt, th in ((False, 1.0), (False, 0.5), (True, 1.0)):
         out.append((f"vdiff opt={opt} theta={th}", _vertical_diffusion_err(opt, th), 1e-12))
     out.append(("vadv", _vertical_advection_err(), 1e-12))
+    for axis in ("x", "y"):
+        for sign in (1.0, -1.0):
+            for nk in (1, 2):
+                out.append((f"implicit channel {axis} {sign} nk={nk}",
+                            _implicit_channel_err(axis, sign, nk), 1e-12))
+        out.append((f"implicit fractional {axis}", _implicit_fractional_err(axis), 1e-12))
     for name in sorted(SMALL):
         mass, constancy = _invariant_errs(name, steps=2)
         out.append((f"mass {name}", mass, 1e-13))
         out.append((f"constant {name}", constancy, 1e-12))
+        for imp in ("x", "y"):
+            mass, _ = _invariant_errs(name, steps=2, implicit=imp)
+            out.append((f"mass {name} implicit {imp}", mass, 1e-13))
     return out
 
 
-@pytest.mark.parametrize("mistake", [1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("mistake", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
 def test_planted_mistakes_are_caught(mistake):
     L = _lib()
     L.ora_plant_mistake(mistake)

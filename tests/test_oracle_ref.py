@@ -6,6 +6,8 @@ inputs and require bit-identical results:
 
 * THOMASZ_NewType2 (ModuleFunctions.F90:4026-4123), including the silent
   zero-pivot skip (MF:4092-4098) with identical VEC scratch contents;
+* THOMAS_3D (its T_THOMAS form, MF:3667-3870) along j and along i, with
+  identical VEC scratch contents afterwards;
 * ComputeAdvectionFace (MF:10702-10894) for every method, limiter, flow sign
   and NearBoundary/Upwind2 combination the reference accepts;
 * ComputeAdvection1D_V2 (MF:10534-10591) and
@@ -70,6 +72,94 @@ def test_thomasz_bitwise(seed, zero_pivot):
                           P(r_ref), P(w_ref), P(g_ref))
     assert np.array_equal(r_ora, r_ref, equal_nan=True)
     assert np.array_equal(w_ora, w_ref) and np.array_equal(g_ora, g_ref)
+
+
+@pytest.mark.parametrize("seed,di,dj", [(11, 0, 1), (12, 1, 0), (13, 0, 1), (14, 1, 0)])
+def test_thomas3d_bitwise(seed, di, dj):
+    rng = np.random.default_rng(seed)
+    ni, nj, nk = 9, 6, 4
+    size, ws, D, E, F, TI, _ = _thomas_inputs(rng, ni, nj, nk)
+    # the reference's calls: lines over the WorkSize, rows JImin..JImax+1
+    # (AD:4194, 4233), and a sub-range to show the bounds are honoured
+    if di == 0:
+        lim = (1, ni, 1, nj, 1, nk) if seed < 13 else (2, ni - 1, 2, nj - 2, 2, nk)
+    else:
+        lim = (1, nj, 1, ni, 1, nk) if seed < 13 else (2, nj - 1, 1, ni - 3, 1, nk - 1)
+    lo, hi = 0, max(ni, nj, nk) + 1
+    r_ora = np.asfortranarray(rng.normal(size=D.shape))
+    r_ref = r_ora.copy(order="F")
+    w0, g0 = rng.normal(size=hi - lo + 1), rng.normal(size=hi - lo + 1)
+    w_ora, g_ora, w_ref, g_ref = w0.copy(), g0.copy(), w0.copy(), g0.copy()
+    S = ora.Size3D(*size)
+    rc = ora.lib().ora_thomas3d(ct.byref(S), *lim, di, dj, P(D), P(E), P(F), P(TI), P(r_ora),
+                                P(w_ora), P(g_ora), None)
+    assert rc == 0
+    sz = np.array(size, dtype=np.int32)
+    lz = np.array(lim, dtype=np.int32)
+    ora.ref().ref_thomas3d(P(sz, ip), P(lz, ip), di, dj, lo, hi, P(D), P(E), P(F), P(TI),
+                           P(r_ref), P(w_ref), P(g_ref))
+    assert np.array_equal(r_ora, r_ref)
+    assert np.array_equal(w_ora, w_ref) and np.array_equal(g_ora, g_ref)
+    # rows outside the lines are left alone
+    touched = np.zeros(D.shape, dtype=bool)
+    a0, a1, b0, b1, k0, k1 = lim
+    if di == 0:
+        touched[a0:a1 + 1, b0:b1 + 2, k0:k1 + 1] = True
+    else:
+        touched[b0:b1 + 2, a0:a1 + 1, k0:k1 + 1] = True
+    r0 = np.asfortranarray(np.random.default_rng(seed).normal(size=D.shape))
+    assert not np.array_equal(r_ora[touched], r0[touched])
+
+
+@pytest.mark.parametrize("di,dj", [(0, 1), (1, 0)])
+def test_thomas3d_matches_dense_solve(di, dj):
+    """Each line of THOMAS_3D equals a dense solve (numpy) of its tridiagonal
+    system, rows JImin..JImax+1 with D of the first and F of the last row
+    dropped."""
+    rng = np.random.default_rng(21 + di)
+    ni, nj, nk = 8, 7, 3
+    size, ws, D, E, F, TI, _ = _thomas_inputs(rng, ni, nj, nk)
+    lim = (1, ni, 1, nj, 1, nk) if di == 0 else (1, nj, 1, ni, 1, nk)
+    res = np.asfortranarray(np.zeros(D.shape))
+    hi = max(ni, nj, nk) + 1
+    w, g = np.zeros(hi + 1), np.zeros(hi + 1)
+    S = ora.Size3D(*size)
+    assert ora.lib().ora_thomas3d(ct.byref(S), *lim, di, dj, P(D), P(E), P(F), P(TI), P(res),
+                                  P(w), P(g), None) == 0
+    for k in range(1, nk + 1):
+        for ij in range(lim[0], lim[1] + 1):
+            rows = range(lim[2], lim[3] + 2)
+            sel = (lambda r: (ij, r, k)) if di == 0 else (lambda r: (r, ij, k))
+            n = len(rows)
+            A = np.zeros((n, n))
+            b = np.zeros(n)
+            for m, r in enumerate(rows):
+                A[m, m] = E[sel(r)]
+                if m > 0:
+                    A[m, m - 1] = D[sel(r)]
+                if m < n - 1:
+                    A[m, m + 1] = F[sel(r)]
+                b[m] = TI[sel(r)]
+            x = np.linalg.solve(A, b)
+            got = np.array([res[sel(r)] for r in rows])
+            np.testing.assert_allclose(got, x, rtol=1e-11, atol=1e-12)
+
+
+def test_thomas3d_zero_pivot_reported():
+    """A zero pivot is reported with its cell (the reference stops, MF:3788)."""
+    rng = np.random.default_rng(5)
+    ni, nj, nk = 4, 5, 2
+    size, ws, D, E, F, TI, _ = _thomas_inputs(rng, ni, nj, nk)
+    F[2, 2, 1] = 0.0        # W(2) = 0 on line (i=2, k=1), so AUX(3) = E(3)
+    E[2, 3, 1] = 0.0
+    res = np.asfortranarray(np.zeros(D.shape))
+    w, g = np.zeros(8), np.zeros(8)
+    where = np.zeros(3, dtype=np.int32)
+    S = ora.Size3D(*size)
+    rc = ora.lib().ora_thomas3d(ct.byref(S), 1, ni, 1, nj, 1, nk, 0, 1, P(D), P(E), P(F),
+                                P(TI), P(res), P(w), P(g), P(where, ip))
+    assert rc == ora.ERR_PIVOT
+    assert tuple(where) == (2, 3, 1)
 
 
 def test_thomasz_matches_dense_solve():

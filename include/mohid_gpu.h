@@ -2,8 +2,9 @@
  * mohid_gpu.h -- C ABI of the MI355X (gfx950) tracer-transport library.
  *
  * Drop-in for MOHID Water's tracer-transport step:
- *   ModuleAdvectionDiffusion's explicit horizontal advection/diffusion and the
- *   implicit vertical advection-diffusion solved per water column by
+ *   ModuleAdvectionDiffusion's horizontal advection/diffusion (explicit, or
+ *   implicit along i or j through ModuleFunctions' THOMAS_3D line solves)
+ *   and the implicit vertical advection-diffusion solved per water column by
  *   ModuleFunctions' THOMASZ_NewType2.
  * Reference files (relative to the reference root):
  *   AD = Software/MOHIDBase2/ModuleAdvectionDiffusion.F90
@@ -47,6 +48,8 @@ extern "C" {
 #define MGPU_ERR_MASKS 104       /* masks inconsistent with KFloor under DOCYCLE_METHOD 1 */
 #define MGPU_ERR_STATE 105       /* Optimize call without a FirstProperty call this step */
 #define MGPU_ERR_NOMEM 106       /* device allocation failed */
+#define MGPU_ERR_PIVOT 107       /* zero pivot in a THOMAS_3D line solve, where the reference
+                                    stops (MF:3788-3791, 3846-3849); PROP is not written */
 
 /* ---- arithmetic of the batched transport -------------------------------- */
 #define MGPU_PREC_FP64 0         /* reference operation order, IEEE FP64 division */
@@ -136,12 +139,30 @@ int mgpu_thomasz(const mgpu_size3d *size, const mgpu_size3d *worksize,
                  const double *TI, const int32_t *waterpoints, double *res,
                  int32_t dev, void *stream);
 
+/* THOMAS_3D (MF:455-458, its T_THOMAS form THOMAS_3D_NewType, MF:3667-3870).
+ * di = 0, dj = 1: for every i in IJmin..IJmax and k in Kmin..Kmax, solve
+ * along j  D(j) x(j-1) + E(j) x(j) + F(j) x(j+1) = TI(j),  j = JImin..JImax+1
+ * (THOMAS_3D_i0_j1_NewType; D of the first row and F of the last are not
+ * used); di = 1, dj = 0: the same along i for every j in IJmin..IJmax
+ * (THOMAS_3D_i1_j0_NewType).  Arrays span Size; x goes to res.  Returns
+ * MGPU_ERR_PIVOT at a zero pivot (res is then undefined), MGPU_ERR_OPTION
+ * for another (di, dj) (ERR01).  dev = 1: device pointers, work on `stream`
+ * (a hipStream_t, may be null); the call waits for it to read the pivot
+ * check. */
+int mgpu_thomas3d(const mgpu_size3d *size, int32_t IJmin, int32_t IJmax, int32_t JImin,
+                  int32_t JImax, int32_t Kmin, int32_t Kmax, int32_t di, int32_t dj,
+                  const double *D, const double *E, const double *F, const double *TI,
+                  double *res, int32_t dev, void *stream);
+
 /* ---- batched, device-resident transport ----------------------------------
  * The multi-tracer loop of ModuleWaterProperties::Advection_Diffusion_Processes
  * (WP:14520-15169) for T tracers that share one set of fluxes, volumes,
  * diffusivities and masks, kept in HBM across steps.  One step advances
  * every tracer by DTProp with the reference's per-tracer semantics
- * (Optimize/FirstProperty as WP:14576-14598 decides). */
+ * (Optimize/FirstProperty as WP:14576-14598 decides).  With implicit
+ * horizontal advection (ImpExp_AdvXX or ImpExp_AdvYY = 1 in the args) the
+ * first step solves along that direction and the direction alternates from
+ * step to step, as WP:14681-14712 does (lane variant only). */
 typedef struct mgpu_transport mgpu_transport;
 
 typedef struct {
@@ -178,6 +199,11 @@ int mgpu_transport_set_lanes(mgpu_transport *t, int32_t tracers_per_launch,
                              int32_t explicit_tracers_per_lane,
                              int32_t column_tracers_per_lane);
 int mgpu_transport_destroy(mgpu_transport *t);
+/* Waits for the handle's work; MGPU_ERR_OPTION if a face method or limiter
+ * the reference stops on was met since the last set_hydro, MGPU_ERR_PIVOT
+ * if an implicit line solve met a zero pivot (the tracers are then
+ * undefined), else MGPU_SUCCESS. */
+int mgpu_transport_status(mgpu_transport *t);
 /* Per-launch timing for measurement: with enable = 1 the following eager
  * mgpu_transport_run calls (use_graph = 0, up to max_steps steps) record
  * hipEvents on the handle's stream around the kernels of every step;

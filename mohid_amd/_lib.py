@@ -26,6 +26,7 @@ MGPU_ERR_UNSUPPORTED = 103
 MGPU_ERR_MASKS = 104
 MGPU_ERR_STATE = 105
 MGPU_ERR_NOMEM = 106
+MGPU_ERR_PIVOT = 107
 MGPU_PREC_FP64 = 0
 MGPU_PREC_FAST = 1
 MGPU_KERNEL_AUTO = 0
@@ -36,7 +37,7 @@ MGPU_KERNEL_LANES = 3
 STATUS_NAMES = {
     0: "SUCCESS", 1: "OFF_ERR", -99: "UNKNOWN", 100: "ERR_HIP", 101: "ERR_BADARG",
     102: "ERR_OPTION", 103: "ERR_UNSUPPORTED", 104: "ERR_MASKS", 105: "ERR_STATE",
-    106: "ERR_NOMEM",
+    106: "ERR_NOMEM", 107: "ERR_PIVOT",
 }
 
 
@@ -87,7 +88,7 @@ class TransportOpts(ct.Structure):
 # every symbol include/mohid_gpu.h declares
 EXPORTS = (
     "mgpu_ad_create", "mgpu_ad_set_geometry", "mgpu_ad_step", "mgpu_ad_destroy",
-    "mgpu_ad_zero_pivots", "mgpu_thomasz",
+    "mgpu_ad_zero_pivots", "mgpu_thomasz", "mgpu_thomas3d", "mgpu_transport_status",
     "mgpu_transport_create", "mgpu_transport_set_hydro", "mgpu_transport_set_tracer",
     "mgpu_transport_get_tracer", "mgpu_transport_run", "mgpu_transport_tracer_ptr",
     "mgpu_transport_set_lanes",
@@ -124,6 +125,10 @@ def lib():
     L.mgpu_ad_zero_pivots.argtypes = [vp, ct.POINTER(ct.c_uint64)]
     L.mgpu_thomasz.argtypes = [ct.POINTER(Size3D), ct.POINTER(Size3D), vp, vp, vp, vp, vp, vp,
                                ct.c_int32, vp]
+    i32 = ct.c_int32
+    L.mgpu_thomas3d.argtypes = [ct.POINTER(Size3D), i32, i32, i32, i32, i32, i32, i32, i32, vp,
+                                vp, vp, vp, vp, i32, vp]
+    L.mgpu_transport_status.argtypes = [vp]
     L.mgpu_transport_create.argtypes = [ct.POINTER(Size3D), ct.POINTER(Size3D),
                                         ct.POINTER(TransportOpts), ct.POINTER(Geometry),
                                         ct.POINTER(vp)]

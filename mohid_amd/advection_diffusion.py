@@ -179,6 +179,23 @@ def thomasz(size, worksize, D, E, F, TI, waterpoints, res) -> None:
                                                      int(dev), None))
 
 
+def thomas3d(size, IJmin, IJmax, JImin, JImax, Kmin, Kmax, di, dj, D, E, F, TI, res) -> None:
+    """THOMAS_3D (MF:3667-3870): the tridiagonal lines along j (di=0, dj=1)
+    or i (di=1, dj=0), rows JImin..JImax+1, into ``res`` (in place).  Raises
+    MgpuError with MGPU_ERR_PIVOT at a zero pivot, where the reference stops."""
+    keep: list = []
+    dev = _is_device(res)
+    addrs = [_addr(a, keep, np.float64) for a in (D, E, F, TI)]
+    if not dev and not (isinstance(res, np.ndarray) and res.flags.f_contiguous
+                        and res.dtype == np.float64):
+        raise TypeError("RES must be a Fortran-ordered float64 array")
+    r = ct.c_void_p(res.data_ptr() if dev else res.ctypes.data)
+    s = L.Size3D(*size)
+    L.check("THOMAS_3D", L.lib().mgpu_thomas3d(ct.byref(s), int(IJmin), int(IJmax), int(JImin),
+                                               int(JImax), int(Kmin), int(Kmax), int(di),
+                                               int(dj), *addrs, r, int(dev), None))
+
+
 class Transport:
     """Batched, device-resident transport of several tracers that share one
     hydrodynamic state (the tracer loop of
@@ -229,6 +246,11 @@ class Transport:
     def run(self, nsteps: int, use_graph: bool = True) -> None:
         L.check("mgpu_transport_run", L.lib().mgpu_transport_run(self._h, int(nsteps),
                                                                  int(bool(use_graph))))
+
+    def status(self) -> None:
+        """Waits for the handle's work; raises where the reference would have
+        stopped (a rejected face method, a zero pivot of a line solve)."""
+        L.check("mgpu_transport_status", L.lib().mgpu_transport_status(self._h))
 
     def timing(self, enable: bool, max_steps: int = 64) -> None:
         L.check("mgpu_transport_timing", L.lib().mgpu_transport_timing(self._h, int(enable),

@@ -21,7 +21,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MGPU_ARCH", "gfx950")
 
 FLAGS = [
-    "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-shared",
+    "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
@@ -34,17 +34,25 @@ def check_resources(text: str) -> None:
     """Fail the build if a kernel spills to scratch memory: every kernel here
     is sized to keep its state in registers, and a spill (for instance an
     array the compiler could not keep in registers) turns into hidden HBM
-    traffic."""
+    traffic.  SGPR spills go to VGPR lanes (no memory traffic); a kernel with
+    them also reports the register scavenger's reserved emergency slot as
+    scratch, which is accepted while no VGPR spills."""
     name = None
-    bad = []
+    info = {}
     for line in text.splitlines():
         m = re.search(r"Function Name: (\S+)", line)
         if m:
             name = m.group(1)
+            info[name] = {}
             continue
-        m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", line)
-        if m and int(m.group(1)) > 0:
-            bad.append(f"{name}: {m.group(1)} bytes/lane")
+        m = re.search(r"(ScratchSize \[bytes/lane\]|SGPRs Spill|VGPRs Spill): (\d+)", line)
+        if m and name:
+            info[name][m.group(1).split()[0]] = int(m.group(2))
+    bad = []
+    for name, r in info.items():
+        scratch, sspill, vspill = r.get("ScratchSize", 0), r.get("SGPRs", 0), r.get("VGPRs", 0)
+        if vspill > 0 or (scratch > 0 and sspill == 0):
+            bad.append(f"{name}: {scratch} bytes/lane scratch, {vspill} VGPRs spilled")
     if bad:
         raise RuntimeError("kernels spill to scratch memory:\n  " + "\n  ".join(bad))
 
@@ -84,17 +92,50 @@ def build_fortran(verbose: bool = False) -> str:
     return DRIVER
 
 
-def build(verbose: bool = False) -> str:
+OBJ = os.path.join(HERE, "_build")
+
+
+def _filtered(text: str) -> str:
+    return "\n".join(l for l in text.splitlines()
+                     if ("error" in l or "warning" in l) and "packed-fp32-ops" not in l)
+
+
+def build(verbose: bool = False, jobs: int = 0) -> str:
+    """Compile every HIP source to an object (in parallel: the transport
+    kernels are split over several translation units), check the kernels'
+    resource use, and link the shared library."""
+    os.makedirs(OBJ, exist_ok=True)
+    srcs = sources()
+    jobs = jobs or min(len(srcs), max(1, os.cpu_count() or 1), 16)
+    objs = [os.path.join(OBJ, os.path.basename(f)[:-4] + ".o") for f in srcs]
+    procs = []
+    pending = list(zip(srcs, objs))
+    stderr = []
+    while pending or procs:
+        while pending and len(procs) < jobs:
+            src, obj = pending.pop(0)
+            cmd = [HIPCC, *FLAGS, "-c", "-o", obj, src]
+            procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                                stderr=subprocess.PIPE, text=True)))
+        cmd, p = procs.pop(0)
+        out, err = p.communicate()
+        if p.returncode != 0:
+            for _, q in procs:
+                q.kill()
+            raise RuntimeError(f"hipcc failed ({p.returncode}):\n{' '.join(cmd)}\n"
+                               f"{_filtered(out + err)}")
+        stderr.append(err)
+    check_resources("\n".join(stderr))
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, *sources()]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     res = subprocess.run(cmd, capture_output=True, text=True)
-    out = "\n".join(l for l in (res.stdout + res.stderr).splitlines()
-                    if ("error" in l or "warning" in l) and "packed-fp32-ops" not in l)
     if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{' '.join(cmd)}\n{out}")
-    check_resources(res.stderr)
-    if verbose and out:
-        print(out)
+        raise RuntimeError(f"link failed ({res.returncode}):\n{' '.join(cmd)}\n"
+                           f"{_filtered(res.stdout + res.stderr)}")
+    if verbose:
+        text = _filtered("\n".join(stderr))
+        if text:
+            print(text)
     os.replace(tmp, OUT)
     return OUT
 

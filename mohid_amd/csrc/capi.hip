@@ -14,9 +14,13 @@ namespace {
 thread_local char g_err[512] = "";
 
 // copy the solved cells (water columns, rows KLB..KUB+1) of `src` into `dst`;
-// the reference writes exactly these cells of PROP (MF:4086-4107)
+// the reference writes exactly these cells of PROP (MF:4086-4107).  Nothing
+// is copied when *stop is set (a THOMAS_3D zero pivot, where the reference
+// stops).
 __global__ void copy_back_kernel(Dims d, const uint8_t *__restrict__ mask,
-                                 const double *__restrict__ src, double *__restrict__ dst) {
+                                 const double *__restrict__ src, double *__restrict__ dst,
+                                 const unsigned long long *__restrict__ stop) {
+    if (*stop) return;
     const int i = d.wILB + blockIdx.x * blockDim.x + threadIdx.x;
     const int j = d.wJLB + blockIdx.y * blockDim.y + threadIdx.y;
     if (i > d.wIUB || j > d.wJUB) return;
@@ -28,10 +32,10 @@ __global__ void copy_back_kernel(Dims d, const uint8_t *__restrict__ mask,
 }
 
 int launch_copy_back(const Dims &d, const uint8_t *mask, const double *src, double *dst,
-                     hipStream_t s) {
+                     const unsigned long long *stop, hipStream_t s) {
     const int ni = d.wIUB - d.wILB + 1, nj = d.wJUB - d.wJLB + 1;
     dim3 b(64, 4), g((ni + 63) / 64, (nj + 3) / 4);
-    hipLaunchKernelGGL(copy_back_kernel, g, b, 0, s, d, mask, src, dst);
+    hipLaunchKernelGGL(copy_back_kernel, g, b, 0, s, d, mask, src, dst, stop);
     HIPCHK(hipGetLastError());
     return MGPU_SUCCESS;
 }
@@ -148,7 +152,8 @@ struct DevDerived {
     uint32_t *rowcnt = nullptr, *cells = nullptr, *cols = nullptr;
     unsigned long long *totals = nullptr;
     long ncells = 0, ncols = 0;
-    unsigned long long *counters = nullptr;  // [0] zero pivots, [1] bad-method flag, [2] mask flags
+    // [0] THOMASZ zero pivots, [1] bad-method flag, [2] mask flags, [3] THOMAS_3D zero pivots
+    unsigned long long *counters = nullptr;
     int alloc(const Dims &d) {
         const long n3 = d.n3(), nt = tile_count(d), nK = d.KUB - d.KLB + 1;
         const long nIw = d.wIUB - d.wILB + 1, nJw = d.wJUB - d.wJLB + 1,
@@ -212,8 +217,9 @@ int check_options(const mgpu_ad_args *a) {
     if (a->ImpExp_DifH != 0.0) return MGPU_ERR_OPTION;      // ERR02
     if (impXX && impYY) return MGPU_ERR_OPTION;            // ERR03
     if (a->ImpExp_AdvV != 0.0 && a->ImpExp_AdvV != 1.0) return MGPU_ERR_OPTION;  // VerticalAdvection ERR01
-    // implicit horizontal advection goes through THOMAS_3D (AD:4167-4255): CPU path
-    if (a->ImpExp_AdvXX != 0.0 || a->ImpExp_AdvYY != 0.0) return MGPU_ERR_UNSUPPORTED;
+    // HorizontalAdvectionXX/YY ERR01 (AD:4527, 4887)
+    if ((a->ImpExp_AdvXX != 0.0 && !impXX) || (a->ImpExp_AdvYY != 0.0 && !impYY))
+        return MGPU_ERR_OPTION;
     if (a->AdvectionNudging || a->unsupported_CellFluxes || a->unsupported_ReferenceProp ||
         a->unsupported_BoundaryCondition || a->unsupported_SmallDepths ||
         a->unsupported_NoAdvFlux || a->unsupported_NoDifFlux || a->unsupported_Discharges)
@@ -237,6 +243,12 @@ StepScalars make_scalars(const Dims &d, const mgpu_ad_args *a, int vertical1d, i
     sc.vertical1d = vertical1d; sc.xzflow = xzflow;
     sc.impAdvV = a->ImpExp_AdvV == 1.0;
     sc.vert = d.wKUB > 1;                              // AD:1722
+    // implicit horizontal advection: the one-layer line solve runs in every
+    // case (AD:1758-1843); the 3-D one belongs to HorizontalAdvection, which
+    // Vertical1D skips (AD:1704, 4167)
+    sc.impH = a->ImpExp_AdvXX == 1.0 ? 0 : (a->ImpExp_AdvYY == 1.0 ? 1 : -1);
+    if (sc.vert && vertical1d) sc.impH = -1;
+    sc.hreset = sc.impH >= 0 && sc.vert;
     return sc;
 }
 
@@ -293,7 +305,11 @@ struct mgpu_ad {
     DevHydro hydro;
     DevDerived der;
     double *prop = nullptr, *out = nullptr;
-    double *Wb = nullptr, *Gb = nullptr;    // forward-sweep W and G (tile-blocked)
+    double *Wb = nullptr, *Gb = nullptr;    // forward-sweep W and G
+    // TI the reference keeps between calls where no call of a step rewrites
+    // it (the ghost rows: kept by the optimised path, AD:1660-1666), and
+    // PROP after implicit line solves
+    double *tig = nullptr, *mid = nullptr;
     bool have_faces = false;
     unsigned long long last_zero_pivots = 0;
 };
@@ -328,6 +344,8 @@ extern "C" int mgpu_ad_create(const mgpu_size3d *size, const mgpu_size3d *worksi
     const long scr = n3 > blocked_scratch_len(ad->d, 1) ? n3 : blocked_scratch_len(ad->d, 1);
     if (!rc) rc = dalloc(&ad->Wb, scr);
     if (!rc) rc = dalloc(&ad->Gb, scr);
+    if (!rc) rc = dalloc(&ad->tig, n3);
+    if (!rc) rc = dalloc(&ad->mid, n3);
     if (!rc) rc = ad->geom.upload(ad->d, geometry, ad->stream);
     if (rc) {
         mgpu_ad_destroy(ad);
@@ -349,6 +367,7 @@ extern "C" int mgpu_ad_destroy(mgpu_ad *ad) {
     ad->hydro.release();
     ad->der.release();
     dfree(ad->prop); dfree(ad->out); dfree(ad->Wb); dfree(ad->Gb);
+    dfree(ad->tig); dfree(ad->mid);
     if (ad->stream) (void)hipStreamDestroy(ad->stream);
     delete ad;
     return MGPU_SUCCESS;
@@ -396,6 +415,8 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
         ad->have_faces = true;
     }
     const SharedFields f = make_fields(ad->hydro, ad->der, ad->geom);
+    // the step's own initialisation of TI (AD:1660-1671) reaches the ghost rows
+    if (first) HIPCHK(hipMemsetAsync(ad->tig, 0, b3, s));
     TransportLaunch L;
     L.variant = MGPU_KERNEL_LANES;
     L.lists = ad->der.lists();
@@ -407,15 +428,37 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     L.Wb = ad->Wb;
     L.Gb = ad->Gb;
     L.counters = ad->der.counters;
-    rc = launch_transport(L, d, sc, f, s);
+    L.tig = ad->tig;
+    const unsigned long long *stop = ad->der.counters + 3;
+    if (sc.hreset) {
+        // line solves into mid (a copy of PROP, so that it is PROP after
+        // THOMAS_3D everywhere), then TI = PROP over Size (AD:4245-4248)
+        // before the column solve
+        L.mid = ad->mid;
+        HIPCHK(hipMemcpyAsync(ad->mid, P, b3, hipMemcpyDeviceToDevice, s));
+        rc = launch_transport(L, d, sc, f, s, 1);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(ad->tig, ad->mid, b3, hipMemcpyDeviceToDevice, s));
+        rc = launch_transport(L, d, sc, f, s, 2);
+    } else {
+        rc = launch_transport(L, d, sc, f, s);
+    }
     if (rc) return rc;
-    rc = launch_copy_back(d, ad->der.mask, ad->out, P, s);
+    // TI = null_real on land (AD:1753), kept for the next call
+    rc = launch_land_fill(d, ad->der.mask, ad->tig, s);
+    if (!rc && sc.impH >= 0) rc = launch_copy_lines(d, sc.impH, ad->out, P, stop, s);
+    if (!rc && (sc.impH < 0 || sc.hreset))
+        rc = launch_copy_back(d, ad->der.mask, ad->out, P, stop, s);
     if (rc) return rc;
-    if (!a->device_pointers) HIPCHK(hipMemcpyAsync(prop, ad->prop, b3, hipMemcpyDeviceToHost, s));
     unsigned long long cnt[4];
     HIPCHK(hipMemcpyAsync(cnt, ad->der.counters, sizeof(cnt), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     ad->last_zero_pivots = cnt[0];
+    if (cnt[3]) return MGPU_ERR_PIVOT;            // THOMAS_3D ERR10: PROP not written
+    if (!a->device_pointers) {
+        HIPCHK(hipMemcpyAsync(prop, ad->prop, b3, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
     if (cnt[1]) return MGPU_ERR_OPTION;          // method/limiter the reference stops on
     const unsigned int mflags = (unsigned int)(cnt[2] & 0xffffffffu);
     if (mflags) return MGPU_ERR_MASKS;
@@ -471,6 +514,66 @@ extern "C" int mgpu_thomasz(const mgpu_size3d *size, const mgpu_size3d *worksize
 }
 
 // ---------------------------------------------------------------------------
+extern "C" int mgpu_thomas3d(const mgpu_size3d *size, int32_t IJmin, int32_t IJmax,
+                             int32_t JImin, int32_t JImax, int32_t Kmin, int32_t Kmax,
+                             int32_t di, int32_t dj, const double *D, const double *E,
+                             const double *F, const double *TI, double *res, int32_t dev,
+                             void *stream) {
+    if (!size || !D || !E || !F || !TI || !res) return MGPU_ERR_BADARG;
+    if (!((di == 0 && dj == 1) || (di == 1 && dj == 0))) return MGPU_ERR_OPTION;  // MF ERR01
+    const mgpu_size3d &z = *size;
+    if (z.IUB < z.ILB || z.JUB < z.JLB || z.KUB < z.KLB) return MGPU_ERR_BADARG;
+    // lines IJ (i for di = 0, j for di = 1), rows JImin..JImax+1, levels K
+    const int ijl = di == 0 ? z.ILB : z.JLB, iju = di == 0 ? z.IUB : z.JUB;
+    const int jil = di == 0 ? z.JLB : z.ILB, jiu = di == 0 ? z.JUB : z.IUB;
+    if (IJmin > IJmax || JImin > JImax + 1 || Kmin > Kmax) return MGPU_SUCCESS;
+    if (IJmin < ijl || IJmax > iju || JImin < jil || JImax + 1 > jiu || Kmin < z.KLB ||
+        Kmax > z.KUB)
+        return MGPU_ERR_BADARG;
+    const long n3 = (long)(z.IUB - z.ILB + 1) * (z.JUB - z.JLB + 1) * (z.KUB - z.KLB + 1);
+    const size_t b3 = sizeof(double) * (size_t)n3;
+    hipStream_t s = (hipStream_t)stream;
+    int rc = 0;
+    double *W = nullptr;
+    unsigned long long *piv = nullptr;
+    if ((rc = dalloc(&W, n3))) return rc;
+    if ((rc = dalloc(&piv, 1))) { dfree(W); return rc; }
+    double *dD = nullptr, *dE = nullptr, *dF = nullptr, *dT = nullptr, *dR = nullptr;
+    if (!dev) {
+        rc |= dalloc(&dD, n3); rc |= dalloc(&dE, n3); rc |= dalloc(&dF, n3);
+        rc |= dalloc(&dT, n3); rc |= dalloc(&dR, n3);
+    }
+    unsigned long long npiv = 0;
+    if (!rc) {
+        hipError_t e = hipMemsetAsync(piv, 0, sizeof(*piv), s);
+        if (!dev) {
+            if (e == hipSuccess) e = hipMemcpyAsync(dD, D, b3, hipMemcpyHostToDevice, s);
+            if (e == hipSuccess) e = hipMemcpyAsync(dE, E, b3, hipMemcpyHostToDevice, s);
+            if (e == hipSuccess) e = hipMemcpyAsync(dF, F, b3, hipMemcpyHostToDevice, s);
+            if (e == hipSuccess) e = hipMemcpyAsync(dT, TI, b3, hipMemcpyHostToDevice, s);
+            if (e == hipSuccess) e = hipMemcpyAsync(dR, res, b3, hipMemcpyHostToDevice, s);
+        }
+        if (e != hipSuccess) { mgpu_set_error("hipMemcpyAsync", e); rc = MGPU_ERR_HIP; }
+        if (!rc)
+            rc = dev ? launch_thomas3d(z, IJmin, IJmax, JImin, JImax, Kmin, Kmax, di, D, E, F, TI,
+                                       res, W, piv, s)
+                     : launch_thomas3d(z, IJmin, IJmax, JImin, JImax, Kmin, Kmax, di, dD, dE, dF,
+                                       dT, dR, W, piv, s);
+        if (!rc) {
+            e = hipMemcpyAsync(&npiv, piv, sizeof(npiv), hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess && !dev) e = hipMemcpyAsync(res, dR, b3, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) { mgpu_set_error("hipMemcpyAsync", e); rc = MGPU_ERR_HIP; }
+        }
+    } else {
+        rc = MGPU_ERR_NOMEM;
+    }
+    dfree(dD); dfree(dE); dfree(dF); dfree(dT); dfree(dR); dfree(W); dfree(piv);
+    if (!rc && npiv) rc = MGPU_ERR_PIVOT;
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
 // Batched device-resident transport
 struct mgpu_transport {
     mgpu_size3d size, ws;
@@ -491,9 +594,15 @@ struct mgpu_transport {
     double *Wb = nullptr, *Gb = nullptr;  // column-solve scratch (tile-blocked)
     double *stage = nullptr;         // one Fortran-layout tracer (host transfers)
     int chunk = 8;                   // tracers per launch
+    // implicit horizontal advection: direction of the next step (-1: none;
+    // 0: along j, 1: along i), alternating every step as WP:14681-14712
+    // does, and PROP after the line solves (interleaved like the state)
+    int impdir = -1;
+    double *mid = nullptr;
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     int graph_steps[2] = {0, 0};
+    int graph_dir[2] = {-1, -1};
     // optional per-launch timing: event pairs around every kernel of a step
     int timing = 0;
     hipEvent_t *ev = nullptr;        // 2 per launch: [prep, tracer 0 .. T-1] per step
@@ -523,6 +632,7 @@ static void drop_graphs(mgpu_transport *t) {
         t->exec[p] = nullptr;
         t->graph[p] = nullptr;
         t->graph_steps[p] = 0;
+        t->graph_dir[p] = -1;
     }
 }
 
@@ -539,6 +649,7 @@ extern "C" int mgpu_transport_destroy(mgpu_transport *t) {
     dfree(t->Wb);
     dfree(t->Gb);
     dfree(t->stage);
+    dfree(t->mid);
     if (t->stream) (void)hipStreamDestroy(t->stream);
     delete t;
     return MGPU_SUCCESS;
@@ -608,7 +719,17 @@ extern "C" int mgpu_transport_set_hydro(mgpu_transport *t, const mgpu_ad_args *a
     int rc = check_options(a);
     if (rc) return rc;
     const int vertical1d = 0, xzflow = 0;
-    t->sc = make_scalars(t->d, a, vertical1d, xzflow);
+    const StepScalars sc = make_scalars(t->d, a, vertical1d, xzflow);
+    if (sc.impH >= 0) {
+        // the line solves run on the lane variant's state-shaped scratch
+        if (t->kernel != MGPU_KERNEL_LANES) return MGPU_ERR_UNSUPPORTED;
+        if (sc.vert && !t->mid) {
+            HIPCHK(hipStreamSynchronize(t->stream));
+            if ((rc = dalloc(&t->mid, t->d.n3() * (long)t->opts.ntracers))) return rc;
+        }
+    }
+    t->sc = sc;
+    t->impdir = sc.impH;
     // WP:14576-14598 picks the optimised path for >= 2 TVD/SuperBee tracers
     t->schmidtH = a->schmidt_H;
     t->schmidtV = a->SchmidtCoef_V;
@@ -701,12 +822,19 @@ extern "C" int mgpu_transport_stream(mgpu_transport *t, void **stream) {
     return MGPU_SUCCESS;
 }
 
-// Enqueue `nsteps` steps starting from buffer parity `cur`.
-static int enqueue_steps(mgpu_transport *t, int nsteps, int cur) {
+// Enqueue `nsteps` steps starting from buffer parity `cur` (and implicit
+// direction `dir`).
+static int enqueue_steps(mgpu_transport *t, int nsteps, int cur, int dir) {
     const Dims &d = t->d;
     hipStream_t s = t->stream;
     const SharedFields f = make_fields(t->hydro, t->der, t->geom);
     for (int st = 0; st < nsteps; ++st) {
+        StepScalars sc = t->sc;
+        if (dir >= 0) {
+            sc.impH = dir;
+            sc.hreset = sc.vert;
+            dir ^= 1;
+        }
         // first property of the step: face conductances (AD:1421-1478)
         int rc = mark(t);
         if (rc) return rc;
@@ -732,9 +860,10 @@ static int enqueue_steps(mgpu_transport *t, int nsteps, int cur) {
         L.lanes.explicit_tpl = t->opts.explicit_tracers_per_lane;
         L.lanes.column_tpl = t->opts.column_tracers_per_lane;
         L.counters = t->der.counters;
+        L.mid = t->mid;
         // first kernel(s) of every chunk, then the second kernel(s)
         for (int ph = 1; ph <= 2; ++ph) {
-            rc = launch_transport(L, d, t->sc, f, s, ph);
+            rc = launch_transport(L, d, sc, f, s, ph);
             if (rc) return rc;
             if ((rc = mark(t))) return rc;
         }
@@ -749,18 +878,18 @@ extern "C" int mgpu_transport_run(mgpu_transport *t, int32_t nsteps, int32_t use
     if (!t->have_hydro) return MGPU_ERR_STATE;
     if (nsteps == 0) return MGPU_SUCCESS;
     if (!use_graph) {
-        int rc = enqueue_steps(t, nsteps, t->cur);
+        int rc = enqueue_steps(t, nsteps, t->cur, t->impdir);
         if (rc) return rc;
     } else {
         if (t->timing) return MGPU_ERR_STATE;  // events are not captured into graphs
         const int p = t->cur;
-        if (t->graph_steps[p] != nsteps) {
+        if (t->graph_steps[p] != nsteps || t->graph_dir[p] != t->impdir) {
             if (t->exec[p]) (void)hipGraphExecDestroy(t->exec[p]);
             if (t->graph[p]) (void)hipGraphDestroy(t->graph[p]);
             t->exec[p] = nullptr;
             t->graph[p] = nullptr;
             HIPCHK(hipStreamBeginCapture(t->stream, hipStreamCaptureModeThreadLocal));
-            int rc = enqueue_steps(t, nsteps, p);
+            int rc = enqueue_steps(t, nsteps, p, t->impdir);
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(t->stream, &g);
             if (rc) {
@@ -774,10 +903,26 @@ extern "C" int mgpu_transport_run(mgpu_transport *t, int32_t nsteps, int32_t use
             t->graph[p] = g;
             HIPCHK(hipGraphInstantiate(&t->exec[p], g, nullptr, nullptr, 0));
             t->graph_steps[p] = nsteps;
+            t->graph_dir[p] = t->impdir;
         }
         HIPCHK(hipGraphLaunch(t->exec[p], t->stream));
     }
     t->cur ^= (nsteps & 1);
+    if (t->impdir >= 0) t->impdir ^= (nsteps & 1);
+    return MGPU_SUCCESS;
+}
+
+// Waits for the handle's work and reports what the reference would have
+// stopped on since the last mgpu_transport_set_hydro: a face method or
+// limiter it rejects (MGPU_ERR_OPTION) or a zero pivot of an implicit
+// horizontal line solve (MGPU_ERR_PIVOT; the tracers are then undefined).
+extern "C" int mgpu_transport_status(mgpu_transport *t) {
+    if (!t) return MGPU_OFF_ERR;
+    unsigned long long cnt[4];
+    HIPCHK(hipMemcpyAsync(cnt, t->der.counters, sizeof(cnt), hipMemcpyDeviceToHost, t->stream));
+    HIPCHK(hipStreamSynchronize(t->stream));
+    if (cnt[3]) return MGPU_ERR_PIVOT;
+    if (cnt[1]) return MGPU_ERR_OPTION;
     return MGPU_SUCCESS;
 }
 

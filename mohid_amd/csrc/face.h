@@ -36,7 +36,7 @@ __device__ __forceinline__ double superbee_theta(double r) {
 
 // ComputeAdvectionFace for every method/limiter; `bad` is set where the
 // reference would stop ("This method is not valid to compute Advection1D").
-__device__ inline Face4 face_general(const double p[4], const double v[4], const double du[4],
+__device__ __forceinline__ Face4 face_general(const double p[4], const double v[4], const double du[4],
                                      const bool op[4], double Q, double dt, double volrelmax,
                                      int method, int tvd, int up2, int &bad) {
     Face4 r = {0.0, 0.0, 0.0, 0.0};

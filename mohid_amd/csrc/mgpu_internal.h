@@ -64,6 +64,12 @@ struct StepScalars {
     int vertical1d, xzflow;
     int impAdvV;         // ImpExp_AdvV == ImplicitScheme
     int vert;            // WorkSize%KUB > 1
+    // implicit horizontal advection (AD:4167-4256, 1758-1843): -1 none,
+    // 0 lines along j (ImpExp_AdvXX), 1 lines along i (ImpExp_AdvYY)
+    int impH;
+    // 3-D implicit horizontal advection: the vertical step restarts from the
+    // identity rows with TI = PROP after the line solves (AD:4245-4248)
+    int hreset;
 };
 
 struct SharedFields {     // device pointers shared by all tracers
@@ -140,15 +146,76 @@ struct TransportLaunch {
     double *Wb = nullptr, *Gb = nullptr;
     SolveLists lists;                       // lane variant
     LaneShape lanes;                        // lane variant
-    unsigned long long *counters = nullptr; // [0] zero pivots, [1] bad-method flag
+    unsigned long long *counters = nullptr; // [0] zero pivots, [1] bad-method flag,
+                                            // [2] mask flags, [3] THOMAS_3D zero pivots
+    // implicit horizontal advection (lane variant only)
+    double *mid = nullptr;       // PROP after the line solves, laid out like P (3-D)
+    // TI left at the ghost rows by earlier calls (stride-1 per-tracer state of
+    // the drop-in, Size-shaped); null: every call of the step behaves as one
+    // of a run of calls with the same options (the batched transport)
+    const double *tig = nullptr;
 };
 long blocked_scratch_len(const Dims &d, int nt);
+// The tracers of one launch: value of tracer t (0 <= t < NT) at cell index x
+// is base[x * s + t].  Separate Fortran arrays (the drop-in) have s = 1;
+// the batched path keeps its tracers interleaved (s = number of tracers) so
+// that the tracers of a cell share cache lines and each quantity is one
+// memory stream.
+struct TracerSet {
+    const double *P;         // state at the start of the step
+    double *out;             // new state
+    long s;                  // tracer stride
+    // TI of the columns' ghost top row KUB+1 where it is not land (laid out
+    // like P); null: 0, the value the step's own initialisation leaves
+    const double *G0 = nullptr;
+};
+
+// Implicit horizontal advection state of one chunk (see TransportLaunch).
+struct ChunkImp {
+    double *mid = nullptr;        // PROP after the line solves, chunk's first tracer
+    const double *tig = nullptr;  // drop-in ghost-row record (stride 1) or null
+    int t_base = 0;               // index of the chunk's first tracer in the run
+};
+
+// The transport kernels for one chunk of nt tracers (optimised path OPT,
+// fast mode FAST, nt in 4*HALF+1 .. 4*HALF+4).  Each specialisation is
+// compiled in a translation unit of its own (transport_inst_*.hip), so that
+// the build compiles them in parallel.
+#define MGPU_CHUNK_SET_PARAMS                                                           \
+    int nt, int variant, int ph, const Dims &d, const StepScalars &sc,                  \
+        const SharedFields &f, const TracerSet &ts, double *Wb, double *Gb,             \
+        const SolveLists &lists, const LaneShape &lanes, unsigned long long *c,         \
+        const ChunkImp &imp, hipStream_t s
+template <bool OPT, bool FAST, int HALF>
+int launch_chunk_set(MGPU_CHUNK_SET_PARAMS);
+template <> int launch_chunk_set<false, false, 0>(MGPU_CHUNK_SET_PARAMS);
+template <> int launch_chunk_set<false, false, 1>(MGPU_CHUNK_SET_PARAMS);
+template <> int launch_chunk_set<false, true, 0>(MGPU_CHUNK_SET_PARAMS);
+template <> int launch_chunk_set<false, true, 1>(MGPU_CHUNK_SET_PARAMS);
+template <> int launch_chunk_set<true, false, 0>(MGPU_CHUNK_SET_PARAMS);
+template <> int launch_chunk_set<true, false, 1>(MGPU_CHUNK_SET_PARAMS);
+template <> int launch_chunk_set<true, true, 0>(MGPU_CHUNK_SET_PARAMS);
+template <> int launch_chunk_set<true, true, 1>(MGPU_CHUNK_SET_PARAMS);
+
 int launch_transport(const TransportLaunch &L, const Dims &d, const StepScalars &sc,
                      const SharedFields &f, hipStream_t s, int phase = 3);
 // Copy one tracer between a Fortran array (stride 1) and an interleaved
 // buffer (stride `stride`, slot t).  to_interleaved = 1: src Fortran -> dst.
 int launch_interleave(long n3, const double *src, double *dst, long stride, int t,
                       int to_interleaved, hipStream_t s);
+// Copies the cells of the implicit-advection lines (WorkSize lines along j
+// for axis 0, along i for axis 1, with their ghost end row) from src to dst,
+// unless *stop is non-zero.
+int launch_copy_lines(const Dims &d, int axis, const double *src, double *dst,
+                      const unsigned long long *stop, hipStream_t s);
+// dst[x] = null_real where the mask has M_LAND, over Size.
+int launch_land_fill(const Dims &d, const uint8_t *mask, double *dst, hipStream_t s);
+// THOMAS_3D lines (di 0: along j, IJ = i; di 1: along i, IJ = j) on arrays
+// spanning sz; bounds already checked.  pivots counts lines with a zero pivot.
+int launch_thomas3d(const mgpu_size3d &sz, int IJmin, int IJmax, int JImin, int JImax, int Kmin,
+                    int Kmax, int di, const double *D, const double *E, const double *F,
+                    const double *TI, double *res, double *Wscr, unsigned long long *pivots,
+                    hipStream_t s);
 int launch_thomasz(const Dims &d, const double *D, const double *E, const double *F,
                    const double *TI, const int32_t *wp, double *res, double *Wscr,
                    hipStream_t s);

@@ -1,1241 +1,10 @@
-// Tracer-transport kernels for gfx950.
+// Tracer-transport library: launchers of the kernels in transport_kernels.h
+// (the per-chunk transport kernels are instantiated in transport_inst_*.hip),
+// and the small kernels of the C ABI's utilities.
 //
-// One step of ModuleAdvectionDiffusion's AdvectionDiffusion for one tracer
-// (reference: Software/MOHIDBase2/ModuleAdvectionDiffusion.F90 = AD,
-// Software/MOHIDBase1/ModuleFunctions.F90 = MF) is done by
-//
-//   pack_masks      six int32 masks -> one byte per cell, plus per column the
-//                   lowest row that is not an inert land row (`kstart`);
-//   prep_faces      face conductances Kh*A/dx (U, V faces) and
-//                   Kz*DUX*DVY/DZZ (W faces), AD:2364-2675 + AD:1514-1619;
-//   explicit (H)    per level, the explicit right-hand side TI of every water
-//                   cell (AD:3966-4021, 4368-4953, 5156-5365), face fluxes
-//                   computed once per face into LDS;
-//   vertical (V)    one lane per water column, walking k upward: the rows
-//                   D/E/F (AD:2708-3140), the THOMASZ_NewType2 forward sweep
-//                   (MF:4080-4100), then the back substitution (MF:4102-4106).
-// Both kernels advance up to MAXT tracers per launch, so the shared fields
-// and every tracer-independent quantity are loaded/computed once for all.
-//
-// The reference assembles TI/D/E/F by scattering face contributions into
-// whole 3-D arrays; here every cell gathers the contributions of its own row
-// in the order the reference's loops deliver them, so no atomics are needed
-// and each cell sees the same floating-point operations in the same order as
-// in the reference.
-#include "face.h"
-#include "mgpu_internal.h"
-
-namespace {
-
-// Reciprocal for the fast path: hardware estimate refined by two Newton
-// steps (relative error within a few ulp of 1/y).  y is never zero where it
-// is used (volumes, stencil spacings, clamped gradients, non-zero pivots).
-__device__ __forceinline__ double frcp(double y) {
-    double r = __builtin_amdgcn_rcp(y);
-    double e = fma(-y, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-y, r, 1.0);
-    return fma(r, e, r);
-}
-
-// ---------------------------------------------------------------------------
-// Masks.  A row k of a water column is "inert" when it is land and touches no
-// compute face: the reference gives it TI = null_real (AD:1753) and an
-// identity row, so its solution is null_real and it does not couple to the
-// rows above.  kstart is the first row above the inert bottom rows.
-__global__ void pack_masks_kernel(Dims d, const int32_t *__restrict__ open,
-                                  const int32_t *__restrict__ land,
-                                  const int32_t *__restrict__ cfu,
-                                  const int32_t *__restrict__ cfv,
-                                  const int32_t *__restrict__ cfw,
-                                  const int32_t *__restrict__ water,
-                                  uint8_t *__restrict__ mask, long n) {
-    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n;
-         q += (long)gridDim.x * blockDim.x) {
-        uint8_t m = 0;
-        if (open[q] == 1) m |= M_OPEN;
-        if (land[q] == 1) m |= M_LAND;
-        if (cfu[q] == 1) m |= M_CFU;
-        if (cfv[q] == 1) m |= M_CFV;
-        if (cfw[q] == 1) m |= M_CFW;
-        if (water[q] == 1) m |= M_WATER;
-        mask[q] = m;
-    }
-}
-
-__global__ void kstart_kernel(Dims d, const uint8_t *__restrict__ mask,
-                              const int32_t *__restrict__ kfloor, int docycle,
-                              int32_t *__restrict__ kstart, unsigned int *__restrict__ flags) {
-    const int i = d.wILB + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = d.wJLB + blockIdx.y * blockDim.y + threadIdx.y;
-    if (i > d.wIUB || j > d.wJUB) return;
-    int ks = d.wKUB + 1;
-    for (int k = d.wKLB; k <= d.wKUB; ++k) {
-        const uint8_t m = mask[d.idx(i, j, k)];
-        const bool faces = (m & (M_CFU | M_CFV | M_CFW)) ||
-                           (mask[d.idx(i, j + 1, k)] & M_CFU) ||
-                           (mask[d.idx(i + 1, j, k)] & M_CFV) ||
-                           (mask[d.idx(i, j, k + 1)] & M_CFW);
-        const bool inert = (m & M_LAND) && !(m & M_OPEN) && !faces;
-        if (!inert) { ks = k; break; }
-    }
-    kstart[d.idx2(i, j)] = ks;
-    // A W face at the bottom level would use never-computed coefficients in
-    // the reference (Convert_Dif_Vertical starts at KLB+1, AD:2392).
-    if (mask[d.idx(i, j, d.wKLB)] & M_CFW) atomicOr(flags, 1u);
-    // DOCYCLE_METHOD 1 walks KFloor+1..KUB for columns whose surface W face
-    // computes (AD:2809-2835, 3059-3080); the per-face masks used here are the
-    // same walk when the masks agree with KFloor -- check that they do.
-    if (docycle == 1) {
-        const bool top = mask[d.idx(i, j, d.wKUB)] & M_CFW;
-        const int kb = kfloor[d.idx2(i, j)];
-        for (int k = d.wKLB; k <= d.wKUB; ++k) {
-            const bool want = top && k >= kb + 1;
-            const bool has = mask[d.idx(i, j, k)] & M_CFW;
-            if (want != has) { atomicOr(flags, 2u); break; }
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Ragged columns.  With KFloor varying from column to column only part of a
-// tile's cells are active at a level.  Per 64 x 4 tile, the columns are sorted
-// by kstart and the tile's U/V faces by the first level at which they compute,
-// so that the active cells (faces) of level k are a prefix of the sorted list
-// and the kernels hand them to consecutive lanes (no idle lanes inside a wave,
-// waves of similar column depth in the column solve).
-constexpr int TX = 64, TY = 4, TC = TX * TY;       // tile of columns
-constexpr int TFU = (TY + 1) * TX;                 // U faces j0 .. j0+TY
-constexpr int TFV = TY * (TX + 1);                 // V faces i0 .. i0+TX
-constexpr int TF = TFU + TFV;
-constexpr uint8_t NEVER = 255;
-
-// face f of a tile: U faces r*TX+c (face between (i0+c, j0+r-1) and (i0+c, j0+r)),
-// then V faces TFU + r*(TX+1)+c (face between (i0+c-1, j0+r) and (i0+c, j0+r))
-__device__ __forceinline__ void face_pos(int f, int &fi, int &fj, bool &isU, int i0, int j0) {
-    isU = f < TFU;
-    if (isU) {
-        fi = i0 + f % TX;
-        fj = j0 + f / TX;
-    } else {
-        const int g = f - TFU;
-        fi = i0 + g % (TX + 1);
-        fj = j0 + g / (TX + 1);
-    }
-}
-
-// Blocks are dispatched to the 8 XCDs round robin; give each XCD a contiguous
-// run of tiles so that neighbouring tiles (which share halo rows) share an L2.
-__device__ __forceinline__ long xcd_tile(long b, long n) {
-    const long q = n / 8, r = n % 8, x = b % 8, idx = b / 8;
-    return x * q + (x < r ? x : r) + idx;
-}
-
-// For a grid over (level, tile) pairs: XCD x takes a contiguous segment of
-// an order that runs over bands of ceil(ntiles/8) consecutive tiles, all
-// levels of a band, level-major inside the band.  Every XCD then sees the
-// whole range of levels (the work per level varies with KFloor) and
-// consecutive blocks of an XCD are neighbouring tiles of one level.
-__device__ __forceinline__ void xcd_level_tile(long b, long n, long ntiles, int nk, long &tile,
-                                               int &level) {
-    const long pos = xcd_tile(b, n);
-    const long G = (ntiles + 7) / 8;
-    const long band = pos / (G * nk);
-    const long rem = pos - band * G * nk;
-    const long gB = ntiles - band * G < G ? ntiles - band * G : G;
-    level = (int)(rem / gB);
-    tile = band * G + rem % gB;
-}
-
-__global__ void __launch_bounds__(TC)
-tile_sort_kernel(Dims d, const uint8_t *__restrict__ mask, const int32_t *__restrict__ kstart,
-                 uint8_t *__restrict__ cperm, uint16_t *__restrict__ ccnt,
-                 uint16_t *__restrict__ fperm, uint16_t *__restrict__ fcnt) {
-    __shared__ uint8_t ckey[TC];
-    __shared__ uint8_t fkey[TF];
-    const int tid = threadIdx.x;
-    const long tile = blockIdx.x + (long)gridDim.x * blockIdx.y;
-    const int i0 = d.wILB + blockIdx.x * TX, j0 = d.wJLB + blockIdx.y * TY;
-    const int nK = d.KUB - d.KLB + 1;
-    {
-        const int i = i0 + tid % TX, j = j0 + tid / TX;
-        uint8_t key = NEVER;
-        if (i <= d.wIUB && j <= d.wJUB && (mask[d.idx(i, j, d.wKUB)] & M_WATER))
-            key = (uint8_t)kstart[d.idx2(i, j)];
-        ckey[tid] = key;
-    }
-    for (int f = tid; f < TF; f += TC) {
-        int fi, fj;
-        bool isU;
-        face_pos(f, fi, fj, isU, i0, j0);
-        uint8_t key = NEVER;
-        if (fi <= d.wIUB && fj <= d.wJUB) {
-            const uint8_t bit = isU ? M_CFU : M_CFV;
-            for (int k = d.wKLB; k <= d.wKUB; ++k)
-                if (mask[d.idx(fi, fj, k)] & bit) { key = (uint8_t)k; break; }
-        }
-        fkey[f] = key;
-    }
-    __syncthreads();
-    {   // stable rank of the cell
-        const uint8_t key = ckey[tid];
-        int rank = 0;
-        for (int q = 0; q < TC; ++q) {
-            const uint8_t o = ckey[q];
-            rank += (o < key) || (o == key && q < tid);
-        }
-        cperm[tile * TC + rank] = (uint8_t)tid;
-    }
-    for (int f = tid; f < TF; f += TC) {
-        const uint8_t key = fkey[f];
-        int rank = 0;
-        for (int q = 0; q < TF; ++q) {
-            const uint8_t o = fkey[q];
-            rank += (o < key) || (o == key && q < f);
-        }
-        fperm[tile * TF + rank] = (uint16_t)f;
-    }
-    for (int k = tid; k < nK; k += TC) {
-        int nc = 0, nf = 0;
-        for (int q = 0; q < TC; ++q) nc += ckey[q] <= k;
-        for (int q = 0; q < TF; ++q) nf += fkey[q] <= k;
-        ccnt[tile * nK + k] = (uint16_t)nc;
-        fcnt[tile * nK + k] = (uint16_t)nf;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Face conductances.  cu(i,j,k) on the U face between (i,j-1) and (i,j):
-//   Kh = Schmidt_H*(nu_j*DUX_{j-1} + nu_{j-1}*DUX_j)/(DUX_j + DUX_{j-1})  (AD:2480-2494)
-//   cu = Kh*AreaU/DZX(i,j-1)                                              (AD:5180-5182, 1551)
-// cv likewise on V faces (AD:2510-2520, 5337-5339, 1535-1536); on W faces
-//   Kz = SchmidtCoef_V*Diff_V + SchmidtBackground_V                      (AD:2396-2399)
-//   cw = Kz*DUX*DVY/DZZ(k-1)                                              (AD:2735-2738, 1583-1589)
-// The Optimize path's Diff_H_Const/Diff_V_Const are the same numbers.
-__global__ void prep_faces_kernel(Dims d, const double *__restrict__ visc,
-                                  const double *__restrict__ diffv,
-                                  const double *__restrict__ Qx, const double *__restrict__ Qy,
-                                  const double *__restrict__ Qz,
-                                  const double *__restrict__ DUX, const double *__restrict__ DVY,
-                                  const double *__restrict__ DZX, const double *__restrict__ DZY,
-                                  const double *__restrict__ AreaU,
-                                  const double *__restrict__ AreaV,
-                                  const double *__restrict__ DZZ,
-                                  const uint8_t *__restrict__ mask, double sH, double sV,
-                                  double bV, int nulldif, double *__restrict__ cu,
-                                  double *__restrict__ cv, double *__restrict__ cw) {
-    const int i = d.wILB + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = d.wJLB + blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = d.wKLB + blockIdx.z;
-    if (i > d.wIUB || j > d.wJUB) return;
-    const long x = d.idx(i, j, k);
-    const uint8_t m = mask[x];
-    if (m & M_CFU) {
-        const double dux0 = DUX[d.idx2(i, j)], dux1 = DUX[d.idx2(i, j - 1)];
-        double kh = sH * (visc[x] * dux1 + visc[d.idx(i, j - 1, k)] * dux0) / (dux0 + dux1);
-        if (nulldif && Qx[x] == 0.0) kh = 0.0;
-        cu[x] = kh * AreaU[x] / DZX[d.idx2(i, j - 1)];
-    }
-    if (m & M_CFV) {
-        const double dvy0 = DVY[d.idx2(i, j)], dvy1 = DVY[d.idx2(i - 1, j)];
-        double kh = sH * (visc[x] * dvy1 + visc[d.idx(i - 1, j, k)] * dvy0) / (dvy0 + dvy1);
-        if (nulldif && Qy[x] == 0.0) kh = 0.0;
-        cv[x] = kh * AreaV[x] / DZY[d.idx2(i - 1, j)];
-    }
-    if ((m & M_CFW) && k > d.wKLB) {
-        double kz = sV * diffv[x] + bV;
-        if (nulldif && Qz[x] == 0.0) kz = 0.0;
-        cw[x] = kz * DUX[d.idx2(i, j)] * DVY[d.idx2(i, j)] / DZZ[d.idx(i, j, k - 1)];
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Explicit right-hand side (kernel H).  One block per (64 x 4) tile of
-// columns and one level k.  Phase 1 computes the advective flux of every
-// U and V face of the tile once per tracer (face-parallel, into LDS); phase 2
-// gathers, per water cell, the reference's contributions to TICOEF3 in the
-// order its loops deliver them:
-//   VolumeVariation (AD:3986-4000), HorizontalDiffusionXX/XX2 (AD:5178-5197,
-//   5239-5246), YY/YY2 (AD:5335-5350, 5285-5292), HorizontalAdvectionXX
-//   (+_Explicit, AD:4463-4487, 4560-4572), YY (+_Explicit, AD:4828-4853,
-//   4930-4940).
-// Tracer-independent parts of every face (Courant number, stencil spacing,
-// masks) are computed once for all tracers of the launch.
-constexpr int MAXT = 8;
-
-// The tracers of one launch: value of tracer t (0 <= t < NT) at cell index x
-// is base[x * s + t].  Separate Fortran arrays (the drop-in) have s = 1;
-// the batched path keeps its tracers interleaved (s = number of tracers) so
-// that the tracers of a cell share cache lines and each quantity is one
-// memory stream.
-struct TracerSet {
-    const double *P;         // state at the start of the step
-    double *out;             // new state
-    long s;                  // tracer stride
-};
-
-// Flux of one horizontal face for NT tracers.  `c0` indexes the cell on the
-// face's high side, `st` is the stride along the line, `pos` its line index
-// and lo..hi the valid line indices (cells outside are outside Size).
-template <bool OPT, int NT, bool FAST>
-__device__ __forceinline__ void hface_flux(const Dims &d, const StepScalars &sc,
-                                           const SharedFields &f, const TracerSet &ts, long c0,
-                                           long st, int pos, int lo, int hi,
-                                           const double *du_line, long du_st, double Q,
-                                           double (&fl)[NT], int &bad) {
-    double v[4], du[4];
-    bool op[4], inl[4];
-    long cell[4];
-#pragma unroll
-    for (int o = 0; o < 4; ++o) {
-        const bool in = pos - 2 + o >= lo && pos - 2 + o <= hi;
-        inl[o] = in;
-        cell[o] = c0 + (long)(o - 2) * st;
-        op[o] = in && (f.mask[cell[o]] & M_OPEN);
-        v[o] = in ? f.V[cell[o]] : 1.0;
-        du[o] = in ? du_line[(long)(o - 2) * du_st] : 0.0;
-    }
-#pragma unroll
-    for (int t = 0; t < NT; ++t) fl[t] = 0.0;
-    if (!(op[1] && op[2])) return;
-    if (OPT) {
-        // ComputeAdvection1D_TVD_SuperBee_2 (MF:10642-10699).  Both flow
-        // directions run one code path on selected operands (same operations
-        // in the same order as each branch of the reference), so a wave with
-        // mixed flow signs does not execute the face twice; the near-boundary
-        // case selects the first-order coefficients at the end (the far
-        // upstream value is then unused; a cell outside Size is not read).
-        const bool up = Q > 0.0;
-        const bool nearb = up ? !op[0] : !op[3];
-        const double vup = up ? v[1] : v[2];
-        const double cr = FAST ? Q * sc.dt * frcp(vup) : Q * sc.dt / vup;
-        const double s1 = du[2] + du[1];
-        const double s0 = up ? du[1] + du[0] : du[2] + du[3];
-        const double i1 = FAST ? frcp(s1) : 0.0, i0 = FAST ? frcp(s0) : 0.0;
-        const long c0m2 = cell[0], c0m1 = cell[1], c0p0 = cell[2], c0p1 = cell[3];
-        const long cfar = up ? (inl[0] ? c0m2 : c0m1) : (inl[3] ? c0p1 : c0p0);
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const double *P = ts.P + t;
-            const double p1 = P[cell[1] * ts.s], p2 = P[cell[2] * ts.s];
-            const double pf = P[cfar * ts.s];
-            const double a = up ? p2 : p1, b = up ? p1 : p2;   // downstream, upstream
-            double dc, rr;
-            if (FAST) {
-                dc = clamp_min_value((a - b) * i1);
-                rr = (b - pf) * i0 * frcp(dc);
-            } else {
-                dc = clamp_min_value((a - b) / s1);
-                rr = (b - pf) / s0 / dc;
-            }
-            const double th = 0.5 * superbee_theta(rr) * (1.0 - cr);
-            const double Dc = nearb ? (up ? Q : 0.0) : (up ? Q * (1.0 - th) : Q * th);
-            const double Ec = nearb ? (up ? 0.0 : Q) : (up ? Q * th : Q * (1.0 - th));
-            fl[t] = Dc * p1 + Ec * p2;
-        }
-    } else {
-        // ComputeAdvection1D_V2 + ComputeAdvectionFace (MF:10534-10591, 10702-10894)
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const double *P = ts.P + t;
-            double p[4];
-#pragma unroll
-            for (int o = 0; o < 4; ++o) p[o] = inl[o] ? P[cell[o] * ts.s] : 0.0;
-            const Face4 c = face_general(p, v, du, op, Q, sc.dt, sc.volrelmax, sc.methodH,
-                                         sc.tvdH, sc.up2H, bad);
-            fl[t] = c.c * p[0] + c.d * p[1] + c.e * p[2] + c.f * p[3];
-        }
-    }
-}
-
-// Face fluxes of one tile at level k into LDS: the faces whose first
-// computing level is <= k are the sorted prefix fperm[0 .. nf).
-template <bool OPT, int NT, bool FAST>
-__device__ __forceinline__ void tile_faces(const Dims &d, const StepScalars &sc,
-                                           const SharedFields &f, const TracerSet &ts, long tile,
-                                           int i0, int j0, int k, int nf,
-                                           double (*FU)[TY + 1][TX], double (*FV)[TY][TX + 1],
-                                           int &bad) {
-    for (int q = threadIdx.x; q < nf; q += TC) {
-        int fi, fj;
-        bool isU;
-        const int fid = f.fperm[tile * TF + q];
-        face_pos(fid, fi, fj, isU, i0, j0);
-        const long y = d.idx(fi, fj, k);
-        double fl[NT];
-        if (isU) {
-            if (f.mask[y] & M_CFU) {
-                hface_flux<OPT, NT, FAST>(d, sc, f, ts, y, d.sj, fj, d.JLB, d.JUB,
-                                          f.DUX + d.idx2(fi, fj), d.sj, f.Qx[y], fl, bad);
-                const int r = fid / TX, c = fid % TX;
-#pragma unroll
-                for (int t = 0; t < NT; ++t) FU[t][r][c] = fl[t];
-            }
-        } else if (!sc.xzflow && (f.mask[y] & M_CFV)) {
-            hface_flux<OPT, NT, FAST>(d, sc, f, ts, y, 1, fi, d.ILB, d.IUB,
-                                      f.DVY + d.idx2(fi, fj), 1, f.Qy[y], fl, bad);
-            const int g = fid - TFU, r = g / (TX + 1), c = g % (TX + 1);
-#pragma unroll
-            for (int t = 0; t < NT; ++t) FV[t][r][c] = fl[t];
-        }
-    }
-}
-
-// Explicit right-hand side TI of cell x (tile position tx, ty) for NT tracers:
-// VolumeVariation (AD:3993-4001), HorizontalDiffusionXX/YY(2) (AD:5185-5196,
-// 5240-5246, 5286-5292, 5341-5351) and the explicit horizontal advection
-// scatters (AD:4560-4576, 4930-4946), each contribution in the order the
-// reference's loops deliver it to the cell.
-template <bool OPT, int NT, bool FAST>
-__device__ __forceinline__ void cell_rhs(const Dims &d, const StepScalars &sc,
-                                         const SharedFields &f, const TracerSet &ts, long x,
-                                         int tx, int ty, const double (*FU)[TY + 1][TX],
-                                         const double (*FV)[TY][TX + 1], double (&TIo)[NT]) {
-    const double dt = sc.dt;
-    const uint8_t mk = f.mask[x];
-    const double vk = f.V[x];
-    const bool open = mk & M_OPEN;
-    const double ivk = FAST ? frcp(vk) : 0.0;
-    const double dtv = dt * ivk;   // fast path: dt/V(k) once per cell
-    const double vv = open ? (FAST ? f.Vold[x] * ivk : f.Vold[x] / vk) : 0.0;
-    const bool hz = !sc.vertical1d;
-    // a face flag set here is a face computed above (its sort key is <= k)
-    const bool cfu0 = hz && (mk & M_CFU);
-    const bool cfu1 = hz && (f.mask[x + d.sj] & M_CFU);
-    const bool cfv0 = hz && !sc.xzflow && (mk & M_CFV);
-    const bool cfv1 = hz && !sc.xzflow && (f.mask[x + 1] & M_CFV);
-    const double au0 = cfu0 ? f.cu[x] : 0.0, au1 = cfu1 ? f.cu[x + d.sj] : 0.0;
-    const double av0 = cfv0 ? f.cv[x] : 0.0, av1 = cfv1 ? f.cv[x + 1] : 0.0;
-    // tracer-independent diffusion weights of the non-optimised path (and of
-    // every path in the fast mode)
-    const double wu0 = FAST ? au0 * dtv : au0 * dt / vk, wu1 = FAST ? au1 * dtv : au1 * dt / vk;
-    const double wv0 = FAST ? av0 * dtv : av0 * dt / vk, wv1 = FAST ? av1 * dtv : av1 * dt / vk;
-    // the optimised YY2 form multiplies the conductance by dt first
-    const double bv0 = av0 * dt, bv1 = av1 * dt;
-    const long s = ts.s;
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const double *P = ts.P + t;
-        const double pk = P[x * s];
-        double TI = open ? pk * vv : pk;
-        const bool ex = OPT && !FAST;   // the optimised path's own operation order
-        if (cfu0) {
-            const double g = pk - P[(x - d.sj) * s];
-            TI = ex ? TI - au0 * g * dt / vk : TI - wu0 * g;
-        }
-        if (cfu1) {
-            const double g = P[(x + d.sj) * s] - pk;
-            TI = ex ? TI + au1 * g * dt / vk : TI + wu1 * g;
-        }
-        if (cfv0) {
-            const double g = pk - P[(x - 1) * s];
-            TI = ex ? TI - bv0 * g / vk : TI - wv0 * g;
-        }
-        if (cfv1) {
-            const double g = P[(x + 1) * s] - pk;
-            TI = ex ? TI + bv1 * g / vk : TI + wv1 * g;
-        }
-        if (FAST) {
-            if (cfu0) TI = TI + FU[t][ty][tx] * dtv;
-            if (cfu1) TI = TI - FU[t][ty + 1][tx] * dtv;
-            if (cfv0) TI = TI + FV[t][ty][tx] * dtv;
-            if (cfv1) TI = TI - FV[t][ty][tx + 1] * dtv;
-        } else {
-            if (cfu0) TI = TI + FU[t][ty][tx] * dt / vk;
-            if (cfu1) TI = TI - FU[t][ty + 1][tx] * dt / vk;
-            if (cfv0) TI = TI + FV[t][ty][tx] * dt / vk;
-            if (cfv1) TI = TI - FV[t][ty][tx + 1] * dt / vk;
-        }
-        TIo[t] = TI;
-    }
-}
-
-// Scratch of the column solve, tile-blocked: level k, tracer t, sorted lane q
-// of tile `tile` at [((tile * nK + k) * NT + t) * TC + q], so that the stores
-// and loads of a wave are contiguous 512-byte runs.
-__device__ __forceinline__ long blk(long tile, int nK, int k, int NT, int t) {
-    return ((tile * nK + k) * NT + t) * (long)TC;
-}
-
-// Kernel H of the split variant: one block per (level, tile), TI to the
-// tile-blocked scratch; blocks are placed on the XCDs by xcd_level_tile.
-template <bool OPT, int NT, bool FAST>
-__global__ void __launch_bounds__(TC)
-explicit_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts, double *__restrict__ TIb,
-                int tiles_x, long ntiles, unsigned int *__restrict__ badflag) {
-    __shared__ double FU[NT][TY + 1][TX];
-    __shared__ double FV[NT][TY][TX + 1];
-    const int tid = threadIdx.x;
-    long tile;
-    int lev;
-    xcd_level_tile(blockIdx.x, gridDim.x, ntiles, d.wKUB - d.wKLB + 1, tile, lev);
-    const int i0 = d.wILB + (int)(tile % tiles_x) * TX;
-    const int j0 = d.wJLB + (int)(tile / tiles_x) * TY;
-    const int k = d.wKLB + lev;
-    const int nK = d.KUB - d.KLB + 1;
-    const int nc = f.ccnt[tile * nK + k];    // active cells of this level (sorted prefix)
-    if (nc == 0) return;
-    int bad = 0;
-    if (!sc.vertical1d) {
-        tile_faces<OPT, NT, FAST>(d, sc, f, ts, tile, i0, j0, k, f.fcnt[tile * nK + k], FU, FV,
-                                  bad);
-        __syncthreads();
-    }
-    if (bad) atomicOr(badflag, 1u);
-    if (tid >= nc) return;
-    const int slot = f.cperm[tile * TC + tid];
-    const int tx = slot % TX, ty = slot / TX;
-    const long x = d.idx(i0 + tx, j0 + ty, k);
-    double TI[NT];
-    cell_rhs<OPT, NT, FAST>(d, sc, f, ts, x, tx, ty, FU, FV, TI);
-#pragma unroll
-    for (int t = 0; t < NT; ++t) TIb[blk(tile, nK, k, NT, t) + tid] = TI[t];
-}
-
-// ---------------------------------------------------------------------------
-// Vertical operator and column solve.  One lane per water column, walking k
-// from kstart to KUB for NT tracers at once: the rows D/E/F of
-// VerticalDiffusion/VerticalDiffusion2 (AD:2731-2770, 2812-2834, 2894-2906)
-// and VerticalAdvection (+_ExplicitScheme, AD:3059-3120, 3160-3203), the
-// null_real land rows (AD:1753) and the THOMASZ_NewType2 forward sweep
-// (MF:4090-4099); the back substitution (MF:4102-4106) follows the walk.
-
-// Coefficients D/E (and C/F) of the W face with stencil levels q..q+3 for
-// NT tracers.  Shared values (Courant number, spacing) are per face.
-struct VCoef {
-    double c, d, e, f;
-};
-
-template <bool OPT, int NT, bool FAST>
-__device__ __forceinline__ void vface_coefs(const StepScalars &sc, const double (&p)[NT][4],
-                                            const double v[4], const double du[4],
-                                            const bool op[4], double Q, bool build,
-                                            VCoef (&co)[NT], int &bad) {
-#pragma unroll
-    for (int t = 0; t < NT; ++t) co[t] = VCoef{0.0, 0.0, 0.0, 0.0};
-    if (!build || !(op[1] && op[2])) return;
-    if (OPT) {
-        // one code path for both flow directions (as hface_flux)
-        const bool up = Q > 0.0;
-        const bool nearb = up ? !op[0] : !op[3];
-        const double vup = up ? v[1] : v[2];
-        const double cr = FAST ? Q * sc.dt * frcp(vup) : Q * sc.dt / vup;
-        const double s1 = du[2] + du[1];
-        const double s0 = up ? du[1] + du[0] : du[2] + du[3];
-        const double i1 = FAST ? frcp(s1) : 0.0, i0 = FAST ? frcp(s0) : 0.0;
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            // scalar copies: a select between elements of the window array
-            // would otherwise become an indexed access that keeps the window
-            // in scratch memory
-            const double q0 = p[t][0], q1 = p[t][1], q2 = p[t][2], q3 = p[t][3];
-            const double a = up ? q2 : q1, b = up ? q1 : q2;
-            const double pf = up ? q0 : q3;
-            double dc, rr;
-            if (FAST) {
-                dc = clamp_min_value((a - b) * i1);
-                rr = (b - pf) * i0 * frcp(dc);
-            } else {
-                dc = clamp_min_value((a - b) / s1);
-                rr = (b - pf) / s0 / dc;
-            }
-            const double th = 0.5 * superbee_theta(rr) * (1.0 - cr);
-            co[t].d = nearb ? (up ? Q : 0.0) : (up ? Q * (1.0 - th) : Q * th);
-            co[t].e = nearb ? (up ? 0.0 : Q) : (up ? Q * th : Q * (1.0 - th));
-        }
-    } else {
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const Face4 c = face_general(p[t], v, du, op, Q, sc.dt, sc.volrelmax, sc.methodV,
-                                         sc.tvdV, sc.up2V, bad);
-            co[t] = VCoef{c.c, c.d, c.e, c.f};
-        }
-    }
-}
-
-// State of one lane's column walk: windows of V, DWZ, mask and P over levels
-// k-1 .. k+2, the coefficients of the face below the current row, and the
-// previous row's forward-sweep W and G.
-template <int NT>
-struct ColumnWalk {
-    double wv[4], wdu[4];
-    uint8_t wm[4];
-    double wp[NT][4];
-    VCoef fb[NT];
-    double Wm[NT], Gm[NT];
-};
-
-// Loads the windows at the first row ks and the coefficients of face ks.
-template <bool OPT, int NT, bool FAST>
-__device__ __forceinline__ void column_begin(const Dims &d, const StepScalars &sc,
-                                             const SharedFields &f, const TracerSet &ts,
-                                             long base, int ks, bool vflux, ColumnWalk<NT> &c,
-                                             int &bad) {
-    const long sk = d.sk;
-    auto inK = [&](int k) { return k >= d.KLB && k <= d.KUB; };
-#pragma unroll
-    for (int o = 0; o < 4; ++o) {
-        const int kk = ks - 1 + o;
-        const bool in = inK(kk);
-        const long y = base + kk * sk;
-        c.wv[o] = in ? f.V[y] : 1.0;
-        c.wdu[o] = in ? f.DWZ[y] : 0.0;
-        c.wm[o] = in ? f.mask[y] : (uint8_t)0;
-#pragma unroll
-        for (int t = 0; t < NT; ++t) c.wp[t][o] = in ? ts.P[y * ts.s + t] : 0.0;
-    }
-    const int kk = ks - 2;
-    const bool in = inK(kk);
-    const long y = base + kk * sk;
-    double p[NT][4];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        p[t][0] = in ? ts.P[y * ts.s + t] : 0.0;
-        p[t][1] = c.wp[t][0]; p[t][2] = c.wp[t][1]; p[t][3] = c.wp[t][2];
-    }
-    const double v4[4] = {in ? f.V[y] : 1.0, c.wv[0], c.wv[1], c.wv[2]};
-    const double du4[4] = {in ? f.DWZ[y] : 0.0, c.wdu[0], c.wdu[1], c.wdu[2]};
-    const uint8_t m0 = in ? f.mask[y] : (uint8_t)0;
-    const bool o4[4] = {(m0 & M_OPEN) != 0, (c.wm[0] & M_OPEN) != 0, (c.wm[1] & M_OPEN) != 0,
-                        (c.wm[2] & M_OPEN) != 0};
-    vface_coefs<OPT, NT, FAST>(sc, p, v4, du4, o4, inK(ks) ? f.Qz[base + ks * sk] : 0.0,
-                               vflux && ks <= d.wKUB, c.fb, bad);
-#pragma unroll
-    for (int t = 0; t < NT; ++t) { c.Wm[t] = 0.0; c.Gm[t] = 0.0; }
-}
-
-// Row k of the column: assembles D/E/F and completes TI with the vertical
-// terms, then one forward-sweep step of THOMASZ_NewType2; slides the
-// windows up one level.
-template <bool OPT, int NT, bool FAST>
-__device__ __forceinline__ void column_row(const Dims &d, const StepScalars &sc,
-                                           const SharedFields &f, const TracerSet &ts, long base,
-                                           int k, bool vflux, ColumnWalk<NT> &c,
-                                           const double (&TIk)[NT], double (&Wo)[NT],
-                                           double (&Go)[NT],
-                                           unsigned long long *__restrict__ zero_pivots,
-                                           int &bad) {
-    const long sk = d.sk;
-    const int KUB = d.wKUB;
-    const double dt = sc.dt;
-    auto inK = [&](int kk) { return kk >= d.KLB && kk <= d.KUB; };
-    const long x = base + k * sk;
-    const double vk = c.wv[1];
-    const uint8_t mk = c.wm[1];
-    // coefficients of the face above (k, k+1): stencil k-1 .. k+2
-    VCoef ft[NT];
-    {
-        const bool o4[4] = {(c.wm[0] & M_OPEN) != 0, (c.wm[1] & M_OPEN) != 0,
-                            (c.wm[2] & M_OPEN) != 0, (c.wm[3] & M_OPEN) != 0};
-        vface_coefs<OPT, NT, FAST>(sc, c.wp, c.wv, c.wdu, o4, inK(k + 1) ? f.Qz[x + sk] : 0.0,
-                                   vflux && k < KUB, ft, bad);
-    }
-    // row values shared by all tracers
-    const double dtk = FAST ? dt * frcp(vk) : dt / vk;
-    double E0 = 1.0;
-    if (k == KUB && (mk & M_OPEN)) E0 = 1.0 + dtk * f.Qz[x + sk];
-    const bool cfwB = sc.vert && (mk & M_CFW);
-    const bool cfwT = sc.vert && k < KUB && (c.wm[2] & M_CFW);
-    const double a2 = cfwB ? (FAST ? f.cw[x] * dtk : f.cw[x] * dt / vk) : 0.0;
-    const double a1 = cfwT ? (FAST ? f.cw[x + sk] * dtk : f.cw[x + sk] * dt / vk) : 0.0;
-    const bool land = mk & M_LAND;
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const double pk = c.wp[t][1], pm = c.wp[t][0], pp = c.wp[t][2];
-        double TI = TIk[t];
-        double E = E0, D = 0.0, F = 0.0;
-        if (cfwB) {
-            if (OPT) {
-                if (sc.thV > 0.0) { D = D - a2; E = E + a2; }
-                else TI = TI - a2 * (pk - pm);
-            } else {
-                D = D - a2 * sc.thV;
-                E = E + a2 * sc.thV;
-                TI = TI - a2 * (pk - pm) * (1.0 - sc.thV);
-            }
-        }
-        if (cfwT) {
-            if (OPT) {
-                if (sc.thV > 0.0) { E = E + a1; F = F - a1; }
-                else TI = TI + a1 * (pp - pk);
-            } else {
-                E = E + a1 * sc.thV;
-                F = F - a1 * sc.thV;
-                TI = TI + a1 * (pp - pk) * (1.0 - sc.thV);
-            }
-        }
-        if (sc.vert && !sc.vertical1d) {
-            if (sc.impAdvV) {
-                if (cfwB) { D = D - c.fb[t].d * dtk; E = E - c.fb[t].e * dtk; }
-                if (cfwT) { E = E + ft[t].d * dtk; F = F + ft[t].e * dtk; }
-            } else if (OPT) {
-                if (FAST) {
-                    if (cfwB) TI = TI + (c.fb[t].d * pm + c.fb[t].e * pk) * dtk;
-                    if (cfwT) TI = TI - (ft[t].d * pk + ft[t].e * pp) * dtk;
-                } else {
-                    if (cfwB) TI = TI + (c.fb[t].d * pm + c.fb[t].e * pk) * dt / vk;
-                    if (cfwT) TI = TI - (ft[t].d * pk + ft[t].e * pp) * dt / vk;
-                }
-            } else {
-                if (cfwB) {
-                    const double pmm =
-                        (c.fb[t].c == 0.0 || k - 2 < d.KLB) ? 0.0 : ts.P[(x - 2 * sk) * ts.s + t];
-                    const double fl =
-                        c.fb[t].c * pmm + c.fb[t].d * pm + c.fb[t].e * pk + c.fb[t].f * pp;
-                    TI = TI + fl * dt / vk;
-                }
-                if (cfwT) {
-                    const double fl = ft[t].c * pm + ft[t].d * pk + ft[t].e * pp +
-                                      ft[t].f * c.wp[t][3];
-                    TI = TI - fl * dt / vk;
-                }
-            }
-        }
-        if (land) TI = MGPU_NULL_REAL;
-        // THOMASZ_NewType2 forward sweep (MF:4090-4099)
-        const double aux = E + D * c.Wm[t];
-        double W = 0.0, G = 0.0;
-        if (fabs(aux) > 0.0) {
-            if (FAST) {
-                const double ia = frcp(aux);
-                W = -F * ia;
-                G = (TI - D * c.Gm[t]) * ia;
-            } else {
-                W = -F / aux;
-                G = (TI - D * c.Gm[t]) / aux;
-            }
-        } else {
-            atomicAdd(zero_pivots, 1ull);
-        }
-        Wo[t] = W;
-        Go[t] = G;
-        c.Wm[t] = W;
-        c.Gm[t] = G;
-        c.fb[t] = ft[t];
-    }
-    // slide the windows up one level
-    const int kn = k + 3;
-    const bool in = inK(kn);
-    const long y = base + kn * sk;
-#pragma unroll
-    for (int o = 0; o < 3; ++o) { c.wv[o] = c.wv[o + 1]; c.wdu[o] = c.wdu[o + 1]; c.wm[o] = c.wm[o + 1]; }
-    c.wv[3] = in ? f.V[y] : 1.0;
-    c.wdu[3] = in ? f.DWZ[y] : 0.0;
-    c.wm[3] = in ? f.mask[y] : (uint8_t)0;
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        c.wp[t][0] = c.wp[t][1]; c.wp[t][1] = c.wp[t][2]; c.wp[t][2] = c.wp[t][3];
-        c.wp[t][3] = in ? ts.P[y * ts.s + t] : 0.0;
-    }
-}
-
-// Ghost top row KUB+1 of a solved column: E = 1, D = F = 0, TI = 0 or
-// null_real (AD:1662-1695, 1753), in the forward sweep's arithmetic.
-template <int NT>
-__device__ __forceinline__ void column_ghost(const Dims &d, const SharedFields &f,
-                                             const TracerSet &ts, long base,
-                                             const ColumnWalk<NT> &c, double (&xs)[NT]) {
-    const long xg = base + (d.wKUB + 1) * d.sk;
-    const double TIg = (f.mask[xg] & M_LAND) ? MGPU_NULL_REAL : 0.0;
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const double aux = 1.0 + 0.0 * c.Wm[t];
-        xs[t] = (TIg - 0.0 * c.Gm[t]) / aux;
-        ts.out[xg * ts.s + t] = xs[t];
-    }
-}
-
-// Kernel V of the split variant: TI from the tile-blocked scratch; the
-// forward sweep's W overwrites TI in place (each lane reads TI(k) before
-// writing W(k)) and G goes to a second blocked array.
-template <bool OPT, int NT, bool FAST>
-__global__ void __launch_bounds__(TC)
-vertical_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts, double *__restrict__ TWb,
-                double *__restrict__ Gb, int tiles_x,
-                unsigned long long *__restrict__ zero_pivots, unsigned int *__restrict__ badflag) {
-    const long tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int nK = d.KUB - d.KLB + 1;
-    const int KUB = d.wKUB;
-    // THOMASZ_NewType2 solves columns with WaterPoints3D(i,j,KUB) == 1 (MF:4086):
-    // those are the columns whose sort key is <= KUB+1; lanes take them in
-    // order of kstart
-    if ((int)threadIdx.x >= f.ccnt[tile * nK + KUB + 1]) return;
-    const int q = threadIdx.x;
-    const int slot = f.cperm[tile * TC + q];
-    const int i = d.wILB + (int)(tile % tiles_x) * TX + slot % TX;
-    const int j = d.wJLB + (int)(tile / tiles_x) * TY + slot / TX;
-    const int ks = f.kstart[d.idx2(i, j)];
-    const long sk = d.sk;
-    const long base = d.idx(i, j, 0);   // index of level 0; level k is base + k*sk
-    int bad = 0;
-    for (int k = d.wKLB; k < ks && k <= KUB; ++k) {
-#pragma unroll
-        for (int t = 0; t < NT; ++t) ts.out[(base + k * sk) * ts.s + t] = MGPU_NULL_REAL;
-    }
-    const bool columnBuild = f.mask[base + KUB * sk] & M_OPEN;  // AD:2982/3004
-    const bool vflux = sc.vert && !sc.vertical1d && columnBuild;
-    ColumnWalk<NT> c;
-    column_begin<OPT, NT, FAST>(d, sc, f, ts, base, ks, vflux, c, bad);
-    for (int k = ks; k <= KUB; ++k) {
-        double TI[NT], W[NT], G[NT];
-#pragma unroll
-        for (int t = 0; t < NT; ++t) TI[t] = TWb[blk(tile, nK, k, NT, t) + q];
-        column_row<OPT, NT, FAST>(d, sc, f, ts, base, k, vflux, c, TI, W, G, zero_pivots, bad);
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            TWb[blk(tile, nK, k, NT, t) + q] = W[t];
-            Gb[blk(tile, nK, k, NT, t) + q] = G[t];
-        }
-    }
-    double xs[NT];
-    column_ghost<NT>(d, f, ts, base, c, xs);
-    // back substitution (MF:4102-4106)
-    for (int k = KUB; k >= ks; --k) {
-        const long x = base + k * sk;
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            xs[t] = TWb[blk(tile, nK, k, NT, t) + q] * xs[t] + Gb[blk(tile, nK, k, NT, t) + q];
-            ts.out[x * ts.s + t] = xs[t];
-        }
-    }
-    if (bad) atomicOr(badflag, 1u);
-}
-
-// ---------------------------------------------------------------------------
-// Fused variant: one block per tile walks the levels bottom-up.  At each
-// level the block computes the tile's face fluxes into LDS, then every active
-// column builds its explicit right-hand side in registers and takes one
-// forward-sweep step; W and G go to a tile-blocked scratch
-//   level k, tracer t, lane q  ->  [((tile * nK + k) * NT + t) * TC + q]
-// so every store and load of a wave is one contiguous 512-byte run.  The
-// back substitution reads them back and writes each cell's solution once.
-
-template <bool OPT, int NT, bool FAST>
-__global__ void __launch_bounds__(TC)
-column_tile_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts, double *__restrict__ Wb,
-                   double *__restrict__ Gb, int tiles_x,
-                   unsigned long long *__restrict__ zero_pivots,
-                   unsigned int *__restrict__ badflag) {
-    __shared__ double FU[NT][TY + 1][TX];
-    __shared__ double FV[NT][TY][TX + 1];
-    const long tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int i0 = d.wILB + (int)(tile % tiles_x) * TX;
-    const int j0 = d.wJLB + (int)(tile / tiles_x) * TY;
-    const int tid = threadIdx.x;
-    const int nK = d.KUB - d.KLB + 1;
-    const int KUB = d.wKUB;
-    const uint16_t *cc = f.ccnt + tile * nK, *fc = f.fcnt + tile * nK;
-    // solved columns (WaterPoints3D(i,j,KUB) == 1, MF:4086) are the sorted
-    // prefix of keys <= KUB+1; their lanes keep the same column at every level
-    const bool act = tid < cc[KUB + 1];
-    int ks = KUB + 1, tx = 0, ty = 0;
-    long base = 0;
-    if (act) {
-        const int slot = f.cperm[tile * TC + tid];
-        tx = slot % TX;
-        ty = slot / TX;
-        base = d.idx(i0 + tx, j0 + ty, 0);
-        ks = f.kstart[d.idx2(i0 + tx, j0 + ty)];
-    }
-    const long sk = d.sk;
-    const bool vflux = act && sc.vert && !sc.vertical1d && (f.mask[base + KUB * sk] & M_OPEN);
-    int bad = 0;
-    ColumnWalk<NT> c;
-#pragma unroll
-    for (int t = 0; t < NT; ++t) { c.Wm[t] = 0.0; c.Gm[t] = 0.0; }
-    double *Wt = Wb + blk(tile, nK, 0, NT, 0) + tid;
-    double *Gt = Gb + blk(tile, nK, 0, NT, 0) + tid;
-    for (int k = d.wKLB; k <= KUB; ++k) {
-        const int nc = cc[k];                        // uniform over the block
-        const int nf = sc.vertical1d ? 0 : fc[k];
-        if (nc == 0 && nf == 0) continue;
-        if (nf) tile_faces<OPT, NT, FAST>(d, sc, f, ts, tile, i0, j0, k, nf, FU, FV, bad);
-        __syncthreads();
-        if (tid < nc) {                              // implies act and ks <= k
-            if (k == ks) column_begin<OPT, NT, FAST>(d, sc, f, ts, base, ks, vflux, c, bad);
-            double TI[NT], W[NT], G[NT];
-            cell_rhs<OPT, NT, FAST>(d, sc, f, ts, base + k * sk, tx, ty, FU, FV, TI);
-            column_row<OPT, NT, FAST>(d, sc, f, ts, base, k, vflux, c, TI, W, G, zero_pivots,
-                                      bad);
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                Wt[((long)k * NT + t) * TC] = W[t];
-                Gt[((long)k * NT + t) * TC] = G[t];
-            }
-        }
-        __syncthreads();
-    }
-    if (bad) atomicOr(badflag, 1u);
-    if (!act) return;
-    for (int k = d.wKLB; k < ks && k <= KUB; ++k) {
-#pragma unroll
-        for (int t = 0; t < NT; ++t) ts.out[(base + k * sk) * ts.s + t] = MGPU_NULL_REAL;
-    }
-    double xs[NT];
-    column_ghost<NT>(d, f, ts, base, c, xs);
-    // back substitution (MF:4102-4106)
-    for (int k = KUB; k >= ks; --k) {
-        const long x = base + k * sk;
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            xs[t] = Wt[((long)k * NT + t) * TC] * xs[t] + Gt[((long)k * NT + t) * TC];
-            ts.out[x * ts.s + t] = xs[t];
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Stand-alone THOMASZ_NewType2 (MF:4026-4123) on caller-held D/E/F/TI.
-__global__ void thomasz_kernel(Dims d, const double *__restrict__ D,
-                               const double *__restrict__ E, const double *__restrict__ F,
-                               const double *__restrict__ TI, const int32_t *__restrict__ wp,
-                               double *__restrict__ res, double *__restrict__ Wscr) {
-    const int i = d.wILB + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = d.wJLB + blockIdx.y * blockDim.y + threadIdx.y;
-    if (i > d.wIUB || j > d.wJUB) return;
-    const int KLB = d.wKLB, KUB = d.wKUB;
-    if (wp[d.idx(i, j, KUB)] != 1) return;
-    // row 1 (MF:4087-4088 index the first row with the literal 1)
-    long x = d.idx(i, j, 1);
-    double Wm = -F[x] / E[x];
-    double Gm = TI[x] / E[x];
-    Wscr[x] = Wm;
-    res[x] = Gm;
-    for (int k = KLB + 1; k <= KUB + 1; ++k) {
-        x = d.idx(i, j, k);
-        const double Dk = D[x];
-        const double aux = E[x] + Dk * Wm;
-        double W = 0.0, G = 0.0;
-        if (fabs(aux) > 0.0) {
-            W = -F[x] / aux;
-            G = (TI[x] - Dk * Gm) / aux;
-        }
-        Wscr[x] = W;
-        res[x] = G;
-        Wm = W;
-        Gm = G;
-    }
-    double x1 = Gm;
-    for (int k = KUB; k >= KLB; --k) {
-        x = d.idx(i, j, k);
-        x1 = Wscr[x] * x1 + res[x];
-        res[x] = x1;
-    }
-}
-
-
-// ---------------------------------------------------------------------------
-// Lane variant.  One lane per (cell, tracer) builds the explicit right-hand
-// side, one lane per (column, tracer) solves the column; lanes run over
-// compact lists of the cells and columns the reference solves, in memory
-// order.  A lane's registers hold one tracer's state only, so many waves are
-// resident to hide memory latency, and the NT lanes of a cell read and write
-// the cell's NT interleaved tracer values as one contiguous run.  The
-// explicit fluxes of a face shared by two cells are computed by both (each
-// time with the same operations); there is no LDS and no barrier.
-
-// A lane's TPL consecutive tracer values: 16-byte vector accesses when the
-// group is 16-byte aligned, so that the lanes of one instruction cover whole
-// memory sectors (element-wise stores of interleaved tracers leave partial
-// sectors that the memory system writes back more than once).
-template <int TPL>
-__device__ __forceinline__ void load_tpl(const double *p, double (&v)[TPL]) {
-    if constexpr (TPL % 2 == 0) {
-        if ((reinterpret_cast<uintptr_t>(p) & 15) == 0) {
-#pragma unroll
-            for (int u = 0; u < TPL / 2; ++u) {
-                const double2 w = reinterpret_cast<const double2 *>(p)[u];
-                v[2 * u] = w.x;
-                v[2 * u + 1] = w.y;
-            }
-            return;
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < TPL; ++u) v[u] = p[u];
-}
-
-template <int TPL>
-__device__ __forceinline__ void store_tpl(double *p, const double (&v)[TPL]) {
-    if constexpr (TPL % 2 == 0) {
-        if ((reinterpret_cast<uintptr_t>(p) & 15) == 0) {
-#pragma unroll
-            for (int u = 0; u < TPL / 2; ++u)
-                reinterpret_cast<double2 *>(p)[u] = make_double2(v[2 * u], v[2 * u + 1]);
-            return;
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < TPL; ++u) p[u] = v[u];
-}
-
-// Decoded position of a Size-linear cell index.
-struct CellPos {
-    int i, j, k;
-};
-__device__ __forceinline__ CellPos cell_pos(const Dims &d, uint32_t x) {
-    // list entries are < 2^32 (launch_build_lists checks the Size)
-    CellPos c;
-    const uint32_t sk = (uint32_t)d.sk, sj = (uint32_t)d.sj;
-    const uint32_t kk = x / sk, r = x - kk * sk;
-    const uint32_t jj = r / sj;
-    c.k = d.KLB + (int)kk;
-    c.j = d.JLB + (int)jj;
-    c.i = d.ILB + (int)(r - jj * sj);
-    return c;
-}
-
-// Smallest value of v over the active lanes of the wave.
-__device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const int w = __shfl_xor(v, o, 64);
-        v = w < v ? w : v;
-    }
-    return v;
-}
-
-// Explicit right-hand side of cell x for TPL tracers (ts points at the
-// first): the contributions of cell_rhs, in the same order, with the four
-// face fluxes of the cell computed here.
-template <bool OPT, int TPL, bool FAST>
-__device__ __forceinline__ void cell_rhs_lane(const Dims &d, const StepScalars &sc,
-                                              const SharedFields &f, const TracerSet &ts,
-                                              long x, int i, int j, double (&TIo)[TPL],
-                                              int &bad) {
-    const double dt = sc.dt;
-    const uint8_t mk = f.mask[x];
-    const double vk = f.V[x];
-    const bool open = mk & M_OPEN;
-    const double ivk = FAST ? frcp(vk) : 0.0;
-    const double dtv = dt * ivk;
-    const double vv = open ? (FAST ? f.Vold[x] * ivk : f.Vold[x] / vk) : 0.0;
-    const bool hz = !sc.vertical1d;
-    const bool cfu0 = hz && (mk & M_CFU);
-    const bool cfu1 = hz && (f.mask[x + d.sj] & M_CFU);
-    const bool cfv0 = hz && !sc.xzflow && (mk & M_CFV);
-    const bool cfv1 = hz && !sc.xzflow && (f.mask[x + 1] & M_CFV);
-    const double au0 = cfu0 ? f.cu[x] : 0.0, au1 = cfu1 ? f.cu[x + d.sj] : 0.0;
-    const double av0 = cfv0 ? f.cv[x] : 0.0, av1 = cfv1 ? f.cv[x + 1] : 0.0;
-    const double wu0 = FAST ? au0 * dtv : au0 * dt / vk, wu1 = FAST ? au1 * dtv : au1 * dt / vk;
-    const double wv0 = FAST ? av0 * dtv : av0 * dt / vk, wv1 = FAST ? av1 * dtv : av1 * dt / vk;
-    const double bv0 = av0 * dt, bv1 = av1 * dt;
-    const long s = ts.s;
-    // the four advective face fluxes (MF:10534-10894, 10642-10699)
-    double fu0[TPL], fu1[TPL], fv0[TPL], fv1[TPL];
-    if (cfu0)
-        hface_flux<OPT, TPL, FAST>(d, sc, f, ts, x, d.sj, j, d.JLB, d.JUB, f.DUX + d.idx2(i, j),
-                                   d.sj, f.Qx[x], fu0, bad);
-    if (cfu1)
-        hface_flux<OPT, TPL, FAST>(d, sc, f, ts, x + d.sj, d.sj, j + 1, d.JLB, d.JUB,
-                                   f.DUX + d.idx2(i, j + 1), d.sj, f.Qx[x + d.sj], fu1, bad);
-    if (cfv0)
-        hface_flux<OPT, TPL, FAST>(d, sc, f, ts, x, 1, i, d.ILB, d.IUB, f.DVY + d.idx2(i, j), 1,
-                                   f.Qy[x], fv0, bad);
-    if (cfv1)
-        hface_flux<OPT, TPL, FAST>(d, sc, f, ts, x + 1, 1, i + 1, d.ILB, d.IUB,
-                                   f.DVY + d.idx2(i + 1, j), 1, f.Qy[x + 1], fv1, bad);
-    const bool ex = OPT && !FAST;
-#pragma unroll
-    for (int t = 0; t < TPL; ++t) {
-        const double *P = ts.P + t;
-        const double pk = P[x * s];
-        double TI = open ? pk * vv : pk;
-        if (cfu0) {
-            const double g = pk - P[(x - d.sj) * s];
-            TI = ex ? TI - au0 * g * dt / vk : TI - wu0 * g;
-        }
-        if (cfu1) {
-            const double g = P[(x + d.sj) * s] - pk;
-            TI = ex ? TI + au1 * g * dt / vk : TI + wu1 * g;
-        }
-        if (cfv0) {
-            const double g = pk - P[(x - 1) * s];
-            TI = ex ? TI - bv0 * g / vk : TI - wv0 * g;
-        }
-        if (cfv1) {
-            const double g = P[(x + 1) * s] - pk;
-            TI = ex ? TI + bv1 * g / vk : TI + wv1 * g;
-        }
-        if (FAST) {
-            if (cfu0) TI = TI + fu0[t] * dtv;
-            if (cfu1) TI = TI - fu1[t] * dtv;
-            if (cfv0) TI = TI + fv0[t] * dtv;
-            if (cfv1) TI = TI - fv1[t] * dtv;
-        } else {
-            if (cfu0) TI = TI + fu0[t] * dt / vk;
-            if (cfu1) TI = TI - fu1[t] * dt / vk;
-            if (cfv0) TI = TI + fv0[t] * dt / vk;
-            if (cfv1) TI = TI - fv1[t] * dt / vk;
-        }
-        TIo[t] = TI;
-    }
-}
-
-// Explicit kernel of the lane variant: NT / TPL lanes per cell; lane
-// (n, u) = cell cells[n], tracers u*TPL .. u*TPL+TPL-1 of the launch; TI
-// goes to TW (laid out like the tracer state).
-template <bool OPT, int NT, int TPL, bool FAST>
-__global__ void __launch_bounds__(256)
-explicit_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
-                     const uint32_t *__restrict__ cells, long ncells, double *__restrict__ TW,
-                     unsigned int *__restrict__ badflag) {
-    constexpr int LPC = NT / TPL;   // lanes per cell
-    const long g = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    const long n = g / LPC;
-    if (n >= ncells) return;
-    const int t0 = (int)(g - n * LPC) * TPL;
-    const uint32_t xc = cells[n];
-    const long x = xc;
-    const CellPos c = cell_pos(d, xc);
-    const TracerSet tl{ts.P + t0, ts.out + t0, ts.s};
-    int bad = 0;
-    double TI[TPL];
-    cell_rhs_lane<OPT, TPL, FAST>(d, sc, f, tl, x, c.i, c.j, TI, bad);
-    store_tpl<TPL>(TW + x * ts.s + t0, TI);
-    if (bad) atomicOr(badflag, 1u);
-}
-
-// Column kernel of the lane variant: NT / TPL lanes per column; lane (n, u)
-// solves column cols[n] for tracers u*TPL .. u*TPL+TPL-1: rows D/E/F,
-// forward sweep (W over TI in TW, G in Gs), back substitution
-// (MF:4080-4106).
-template <bool OPT, int NT, int TPL, bool FAST>
-__global__ void __launch_bounds__(256)
-column_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
-                   const uint32_t *__restrict__ cols, long ncols, double *__restrict__ TW,
-                   double *__restrict__ Gs, unsigned long long *__restrict__ zero_pivots,
-                   unsigned int *__restrict__ badflag) {
-    constexpr int LPC = NT / TPL;
-    const long g = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    const long n = g / LPC;
-    const bool act = n < ncols;
-    const int t = (int)(g - n * LPC) * TPL;
-    const int KUB = d.wKUB;
-    const long base = act ? (long)cols[n] : 0;  // Size index of the column's level 0
-    const long sk = d.sk, s = ts.s;
-    const int ks = act ? f.kstart[base] : KUB + 1;   // idx2(i,j) == idx(i,j,0): KLB is 0
-    // the wave walks the levels together, from its lowest first row: at every
-    // step all its lanes touch one level, so the NT lanes of neighbouring
-    // columns access one contiguous run
-    const int kw = wave_min(ks);
-    if (!act) return;
-    const TracerSet t1{ts.P + t, ts.out + t, s};
-    double *tw = TW + t, *gs = Gs + t;
-    int bad = 0;
-    {
-        double nr[TPL];
-#pragma unroll
-        for (int u = 0; u < TPL; ++u) nr[u] = MGPU_NULL_REAL;
-        for (int k = d.wKLB; k < ks && k <= KUB; ++k) store_tpl<TPL>(t1.out + (base + k * sk) * s, nr);
-    }
-    const bool columnBuild = f.mask[base + KUB * sk] & M_OPEN;  // AD:2982/3004
-    const bool vflux = sc.vert && !sc.vertical1d && columnBuild;
-    ColumnWalk<TPL> c;
-    column_begin<OPT, TPL, FAST>(d, sc, f, t1, base, ks, vflux, c, bad);
-    for (int k = kw; k <= KUB; ++k) {
-        if (k < ks) continue;
-        const long x = base + k * sk;
-        double TI[TPL], W[TPL], G[TPL];
-        load_tpl<TPL>(tw + x * s, TI);
-        column_row<OPT, TPL, FAST>(d, sc, f, t1, base, k, vflux, c, TI, W, G, zero_pivots, bad);
-        store_tpl<TPL>(tw + x * s, W);
-        store_tpl<TPL>(gs + x * s, G);
-    }
-    double xs[TPL];
-    column_ghost<TPL>(d, f, t1, base, c, xs);
-    for (int k = KUB; k >= kw; --k) {
-        if (k < ks) continue;
-        const long x = base + k * sk;
-        double W[TPL], G[TPL];
-        load_tpl<TPL>(tw + x * s, W);
-        load_tpl<TPL>(gs + x * s, G);
-#pragma unroll
-        for (int u = 0; u < TPL; ++u) xs[u] = W[u] * xs[u] + G[u];
-        store_tpl<TPL>(t1.out + x * s, xs);
-    }
-    if (bad) atomicOr(badflag, 1u);
-}
-
-// ---------------------------------------------------------------------------
-// Compact lists.  cells: every (i, j, k) the reference solves and gives an
-// explicit right-hand side -- rows kstart..KUB of the columns with
-// WaterPoints3D(i,j,KUB) == 1 -- in memory order; cols: those columns.
-// One block per row (j, k) of the WorkSize plane: counts, then (after an
-// exclusive scan of the counts) the Size-linear indices in order.
-__device__ __forceinline__ bool list_member(const Dims &d, const uint8_t *mask,
-                                            const int32_t *kstart, int i, int j, int k,
-                                            bool columns) {
-    if (!(mask[d.idx(i, j, d.wKUB)] & M_WATER)) return false;
-    return columns || k >= kstart[d.idx2(i, j)];
-}
-
-__global__ void __launch_bounds__(256)
-list_count_kernel(Dims d, const uint8_t *__restrict__ mask, const int32_t *__restrict__ kstart,
-                  int columns, uint32_t *__restrict__ rowcnt) {
-    __shared__ uint32_t part[4];
-    const int j = d.wJLB + blockIdx.x, k = d.wKLB + blockIdx.y;
-    uint32_t cnt = 0;
-    for (int i = d.wILB + threadIdx.x; i <= d.wIUB; i += 256)
-        cnt += list_member(d, mask, kstart, i, j, k, columns) ? 1u : 0u;
-    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_down(cnt, o, 64);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = cnt;
-    __syncthreads();
-    if (threadIdx.x == 0)
-        rowcnt[blockIdx.y * (long)gridDim.x + blockIdx.x] = part[0] + part[1] + part[2] + part[3];
-}
-
-// In-place exclusive scan of n counts by one block of 1024 threads; total in
-// *total.
-__global__ void __launch_bounds__(1024)
-list_scan_kernel(uint32_t *__restrict__ a, long n, unsigned long long *__restrict__ total) {
-    __shared__ unsigned long long sums[1024];
-    const int tid = threadIdx.x;
-    const long per = (n + 1023) / 1024;
-    const long lo = tid * per, hi = lo + per < n ? lo + per : n;
-    unsigned long long s = 0;
-    for (long q = lo; q < hi; ++q) s += a[q];
-    sums[tid] = s;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {       // inclusive Hillis-Steele scan
-        const unsigned long long v = tid >= o ? sums[tid - o] : 0ull;
-        __syncthreads();
-        sums[tid] += v;
-        __syncthreads();
-    }
-    unsigned long long run = tid ? sums[tid - 1] : 0ull;
-    for (long q = lo; q < hi; ++q) {
-        const uint32_t c = a[q];
-        a[q] = (uint32_t)run;
-        run += c;
-    }
-    if (tid == 1023) *total = sums[1023];
-}
-
-__global__ void __launch_bounds__(256)
-list_write_kernel(Dims d, const uint8_t *__restrict__ mask, const int32_t *__restrict__ kstart,
-                  int columns, const uint32_t *__restrict__ rowoff, uint32_t *__restrict__ out) {
-    __shared__ uint32_t wsum[4];
-    const int j = d.wJLB + blockIdx.x, k = d.wKLB + blockIdx.y;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t off = rowoff[blockIdx.y * (long)gridDim.x + blockIdx.x];
-    for (int i0 = d.wILB; i0 <= d.wIUB; i0 += 256) {
-        const int i = i0 + threadIdx.x;
-        const bool m = i <= d.wIUB && list_member(d, mask, kstart, i, j, k, columns);
-        const unsigned long long b = __ballot(m);
-        const uint32_t before = (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
-        if (lane == 0) wsum[wave] = (uint32_t)__popcll(b);
-        __syncthreads();
-        uint32_t woff = 0;
-        for (int w = 0; w < wave; ++w) woff += wsum[w];
-        const uint32_t chunk = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-        if (m) out[off + woff + before] = (uint32_t)(columns ? d.idx2(i, j) : d.idx(i, j, k));
-        __syncthreads();
-        off += chunk;
-    }
-}
-
-}  // namespace
+// Reference files (root-relative): AD = Software/MOHIDBase2/ModuleAdvectionDiffusion.F90,
+// MF = Software/MOHIDBase1/ModuleFunctions.F90.
+#include "transport_kernels.h"
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -1317,131 +86,6 @@ long blocked_scratch_len(const Dims &d, int nt) {
     return tile_count(d) * (long)(d.KUB - d.KLB + 1) * nt * TC;
 }
 
-// Lane-variant launches with TPL tracers per lane (1, 2, 4 or 8 dividing NT;
-// 0 picks the default: 4 in the explicit kernel, 2 in the column walk).
-template <bool OPT, int NT, int TPL, bool FAST>
-static int launch_explicit_tpl(const Dims &d, const StepScalars &sc, const SharedFields &f,
-                               const TracerSet &ts, const SolveLists &l, double *TW,
-                               unsigned int *bad, hipStream_t s) {
-    const long nb = (l.ncells * (NT / TPL) + 255) / 256;
-    if (nb <= 0) return MGPU_SUCCESS;
-    hipLaunchKernelGGL((explicit_lane_kernel<OPT, NT, TPL, FAST>), dim3((unsigned)nb), dim3(256),
-                       0, s, d, sc, f, ts, l.cells, l.ncells, TW, bad);
-    HIPCHK(hipGetLastError());
-    return MGPU_SUCCESS;
-}
-
-template <bool OPT, int NT, int TPL, bool FAST>
-static int launch_column_tpl(const Dims &d, const StepScalars &sc, const SharedFields &f,
-                             const TracerSet &ts, const SolveLists &l, double *TW, double *Gs,
-                             unsigned long long *counters, unsigned int *bad, hipStream_t s) {
-    const long nb = (l.ncols * (NT / TPL) + 255) / 256;
-    if (nb <= 0) return MGPU_SUCCESS;
-    hipLaunchKernelGGL((column_lane_kernel<OPT, NT, TPL, FAST>), dim3((unsigned)nb), dim3(256), 0,
-                       s, d, sc, f, ts, l.cols, l.ncols, TW, Gs, counters, bad);
-    HIPCHK(hipGetLastError());
-    return MGPU_SUCCESS;
-}
-
-// The largest of {want, then 4, 2, 1} that divides NT.
-template <int NT>
-static int lane_tpl(int want, int dflt) {
-    if (want <= 0) want = dflt;
-    for (int t : {want, 8, 4, 2, 1})
-        if (t <= want && t <= NT && NT % t == 0) return t;
-    return 1;
-}
-
-template <bool OPT, int NT, bool FAST>
-static int launch_explicit_lanes(int want, const Dims &d, const StepScalars &sc,
-                                 const SharedFields &f, const TracerSet &ts, const SolveLists &l,
-                                 double *TW, unsigned int *bad, hipStream_t s) {
-    switch (lane_tpl<NT>(want, 4)) {
-        case 8: if constexpr (NT % 8 == 0) return launch_explicit_tpl<OPT, NT, 8, FAST>(d, sc, f, ts, l, TW, bad, s); break;
-        case 4: if constexpr (NT % 4 == 0) return launch_explicit_tpl<OPT, NT, 4, FAST>(d, sc, f, ts, l, TW, bad, s); break;
-        case 2: if constexpr (NT % 2 == 0) return launch_explicit_tpl<OPT, NT, 2, FAST>(d, sc, f, ts, l, TW, bad, s); break;
-        default: break;
-    }
-    return launch_explicit_tpl<OPT, NT, 1, FAST>(d, sc, f, ts, l, TW, bad, s);
-}
-
-template <bool OPT, int NT, bool FAST>
-static int launch_column_lanes(int want, const Dims &d, const StepScalars &sc,
-                               const SharedFields &f, const TracerSet &ts, const SolveLists &l,
-                               double *TW, double *Gs, unsigned long long *counters,
-                               unsigned int *bad, hipStream_t s) {
-    switch (lane_tpl<NT>(want, 2)) {
-        case 8: if constexpr (NT % 8 == 0) return launch_column_tpl<OPT, NT, 8, FAST>(d, sc, f, ts, l, TW, Gs, counters, bad, s); break;
-        case 4: if constexpr (NT % 4 == 0) return launch_column_tpl<OPT, NT, 4, FAST>(d, sc, f, ts, l, TW, Gs, counters, bad, s); break;
-        case 2: if constexpr (NT % 2 == 0) return launch_column_tpl<OPT, NT, 2, FAST>(d, sc, f, ts, l, TW, Gs, counters, bad, s); break;
-        default: break;
-    }
-    return launch_column_tpl<OPT, NT, 1, FAST>(d, sc, f, ts, l, TW, Gs, counters, bad, s);
-}
-
-// Launch the transport kernels for one chunk of NT tracers.
-// Fused variant: phase bit 1 launches the column-tile kernel.  Split
-// variant: phase bit 1 = kernel H, bit 2 = kernel V.
-template <bool OPT, int NT, bool FAST>
-static int launch_chunk(int variant, int phase, const Dims &d, const StepScalars &sc,
-                        const SharedFields &f, const TracerSet &ts, double *Wb, double *Gb,
-                        const SolveLists &lists, const LaneShape &lanes,
-                        unsigned long long *counters, hipStream_t s) {
-    const int ni = d.wIUB - d.wILB + 1, nj = d.wJUB - d.wJLB + 1, nk = d.wKUB - d.wKLB + 1;
-    unsigned int *bad = reinterpret_cast<unsigned int *>(counters + 1);
-    const dim3 tiles((ni + TX - 1) / TX, (nj + TY - 1) / TY);
-    if (variant == MGPU_KERNEL_LANES) {
-        if (phase & 1) {
-            const int rc = launch_explicit_lanes<OPT, NT, FAST>(lanes.explicit_tpl, d, sc, f, ts,
-                                                                lists, Wb, bad, s);
-            if (rc) return rc;
-        }
-        if (phase & 2) {
-            const int rc = launch_column_lanes<OPT, NT, FAST>(lanes.column_tpl, d, sc, f, ts,
-                                                              lists, Wb, Gb, counters, bad, s);
-            if (rc) return rc;
-        }
-        return MGPU_SUCCESS;
-    }
-    if (variant == MGPU_KERNEL_FUSED) {
-        if (phase & 1) {
-            hipLaunchKernelGGL((column_tile_kernel<OPT, NT, FAST>), dim3(tiles.x * tiles.y),
-                               dim3(TC), 0, s, d, sc, f, ts, Wb, Gb, (int)tiles.x, counters, bad);
-            HIPCHK(hipGetLastError());
-        }
-        return MGPU_SUCCESS;
-    }
-    const long ntiles = (long)tiles.x * tiles.y;
-    if (phase & 1) {
-        hipLaunchKernelGGL((explicit_kernel<OPT, NT, FAST>), dim3((unsigned)(ntiles * nk)),
-                           dim3(TC), 0, s, d, sc, f, ts, Wb, (int)tiles.x, ntiles, bad);
-        HIPCHK(hipGetLastError());
-    }
-    if (phase & 2) {
-        hipLaunchKernelGGL((vertical_kernel<OPT, NT, FAST>), dim3((unsigned)ntiles), dim3(TC), 0,
-                           s, d, sc, f, ts, Wb, Gb, (int)tiles.x, counters, bad);
-        HIPCHK(hipGetLastError());
-    }
-    return MGPU_SUCCESS;
-}
-
-template <bool OPT, bool FAST>
-static int launch_n(int nt, int variant, int ph, const Dims &d, const StepScalars &sc,
-                    const SharedFields &f, const TracerSet &ts, double *Wb, double *Gb,
-                    const SolveLists &lists, const LaneShape &lanes, unsigned long long *c,
-                    hipStream_t s) {
-    switch (nt) {
-        case 1: return launch_chunk<OPT, 1, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, lanes, c, s);
-        case 2: return launch_chunk<OPT, 2, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, lanes, c, s);
-        case 3: return launch_chunk<OPT, 3, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, lanes, c, s);
-        case 4: return launch_chunk<OPT, 4, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, lanes, c, s);
-        case 5: return launch_chunk<OPT, 5, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, lanes, c, s);
-        case 6: return launch_chunk<OPT, 6, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, lanes, c, s);
-        case 7: return launch_chunk<OPT, 7, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, lanes, c, s);
-        case 8: return launch_chunk<OPT, 8, FAST>(variant, ph, d, sc, f, ts, Wb, Gb, lists, lanes, c, s);
-        default: return MGPU_ERR_BADARG;
-    }
-}
 
 int launch_transport(const TransportLaunch &L, const Dims &d, const StepScalars &sc,
                      const SharedFields &f, hipStream_t s, int phase) {
@@ -1453,31 +97,109 @@ int launch_transport(const TransportLaunch &L, const Dims &d, const StepScalars 
         return MGPU_ERR_BADARG;
     if (L.variant == MGPU_KERNEL_LANES && (!L.lists.cells || !L.lists.cols)) return MGPU_ERR_BADARG;
     if (!L.Wb || !L.Gb || !L.P || !L.out) return MGPU_ERR_BADARG;
+    if (sc.impH >= 0) {
+        // the line solves are built on the lane variant's state-shaped scratch
+        if (L.variant != MGPU_KERNEL_LANES) return MGPU_ERR_UNSUPPORTED;
+        if (sc.vert && !L.mid) return MGPU_ERR_BADARG;
+    }
+    if (L.tig && L.n != 1) return MGPU_ERR_BADARG;   // a per-tracer record
     for (int t0 = 0; t0 < L.n; t0 += chunk) {
         const int nt = L.n - t0 < chunk ? L.n - t0 : chunk;
         TracerSet ts;
         ts.P = L.P + t0;
         ts.out = L.out + t0;
         ts.s = L.stride;
+        ts.G0 = L.tig;
+        ChunkImp imp;
+        imp.mid = L.mid ? L.mid + t0 : nullptr;
+        imp.tig = L.tig;
+        imp.t_base = t0;
         // each chunk has its own scratch region: with the split variant all
         // first kernels run before the second ones.  The lane variant's
         // scratch is laid out like the state (tracer t at [x * stride + t]).
         const bool lanes = L.variant == MGPU_KERNEL_LANES;
         double *Wb = L.Wb + (lanes ? t0 : blocked_scratch_len(d, t0));
         double *Gb = L.Gb + (lanes ? t0 : blocked_scratch_len(d, t0));
+        const bool big = nt > 4;
+#define MGPU_ARGS nt, L.variant, phase, d, sc, f, ts, Wb, Gb, L.lists, L.lanes, L.counters, imp, s
         int rc;
         if (sc.optimize)
-            rc = L.fast ? launch_n<true, true>(nt, L.variant, phase, d, sc, f, ts, Wb, Gb,
-                                               L.lists, L.lanes, L.counters, s)
-                        : launch_n<true, false>(nt, L.variant, phase, d, sc, f, ts, Wb, Gb,
-                                                L.lists, L.lanes, L.counters, s);
+            rc = L.fast ? (big ? launch_chunk_set<true, true, 1>(MGPU_ARGS)
+                               : launch_chunk_set<true, true, 0>(MGPU_ARGS))
+                        : (big ? launch_chunk_set<true, false, 1>(MGPU_ARGS)
+                               : launch_chunk_set<true, false, 0>(MGPU_ARGS));
         else
-            rc = L.fast ? launch_n<false, true>(nt, L.variant, phase, d, sc, f, ts, Wb, Gb,
-                                                L.lists, L.lanes, L.counters, s)
-                        : launch_n<false, false>(nt, L.variant, phase, d, sc, f, ts, Wb, Gb,
-                                                 L.lists, L.lanes, L.counters, s);
+            rc = L.fast ? (big ? launch_chunk_set<false, true, 1>(MGPU_ARGS)
+                               : launch_chunk_set<false, true, 0>(MGPU_ARGS))
+                        : (big ? launch_chunk_set<false, false, 1>(MGPU_ARGS)
+                               : launch_chunk_set<false, false, 0>(MGPU_ARGS));
+#undef MGPU_ARGS
         if (rc) return rc;
     }
+    return MGPU_SUCCESS;
+}
+
+namespace {
+// THOMAS_3D_i0_j1_NewType / _i1_j0_NewType (MF:3756-3870) on explicit
+// arrays: one lane per (IJ, K) line; the line's rows are sJI apart from
+// row JImin at base0 + IJ * sIJ + K * sK.  W goes to Wscr, G to res, which
+// the back substitution then overwrites with the solution.
+__global__ void thomas3d_kernel(long base0, long sIJ, long sJI, long sK, int nIJ, long nl,
+                                int nrow, const double *__restrict__ D,
+                                const double *__restrict__ E, const double *__restrict__ F,
+                                const double *__restrict__ TI, double *__restrict__ res,
+                                double *__restrict__ Wscr, unsigned long long *__restrict__ pivots) {
+    const long line = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (line >= nl) return;
+    const long b = base0 + (line % nIJ) * sIJ + (line / nIJ) * sK;
+    double Wm = -F[b] / E[b];
+    double Gm = TI[b] / E[b];
+    Wscr[b] = Wm;
+    res[b] = Gm;
+    bool pivot = false;
+    for (int r = 1; r < nrow; ++r) {
+        const long x = b + r * sJI;
+        const double Dr = D[x];
+        const double aux = E[x] + Dr * Wm;
+        double W = 0.0, G = 0.0;
+        if (fabs(aux) > 0.0) {
+            W = -F[x] / aux;
+            G = (TI[x] - Dr * Gm) / aux;
+        } else {
+            pivot = true;
+        }
+        Wscr[x] = W;
+        res[x] = G;
+        Wm = W;
+        Gm = G;
+    }
+    double xs = Gm;
+    for (int r = nrow - 2; r >= 0; --r) {
+        const long x = b + r * sJI;
+        xs = Wscr[x] * xs + res[x];
+        res[x] = xs;
+    }
+    if (pivot) atomicAdd(pivots, 1ull);
+}
+}  // namespace
+
+int launch_thomas3d(const mgpu_size3d &sz, int IJmin, int IJmax, int JImin, int JImax, int Kmin,
+                    int Kmax, int di, const double *D, const double *E, const double *F,
+                    const double *TI, double *res, double *Wscr, unsigned long long *pivots,
+                    hipStream_t s) {
+    const long nI = sz.IUB - sz.ILB + 1, nJ = sz.JUB - sz.JLB + 1;
+    const long sj = nI, sk = nI * nJ;
+    const long sIJ = di == 0 ? 1 : sj, sJI = di == 0 ? sj : 1;
+    const long base0 = di == 0
+        ? (IJmin - sz.ILB) + (JImin - sz.JLB) * sj + (Kmin - sz.KLB) * sk
+        : (JImin - sz.ILB) + (IJmin - sz.JLB) * sj + (Kmin - sz.KLB) * sk;
+    const int nIJ = IJmax - IJmin + 1;
+    const long nl = (long)nIJ * (Kmax - Kmin + 1);
+    const int nrow = JImax - JImin + 2;
+    if (nl <= 0 || nrow <= 0) return MGPU_SUCCESS;
+    hipLaunchKernelGGL(thomas3d_kernel, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, s,
+                       base0, sIJ, sJI, sk, nIJ, nl, nrow, D, E, F, TI, res, Wscr, pivots);
+    HIPCHK(hipGetLastError());
     return MGPU_SUCCESS;
 }
 
@@ -1503,6 +225,51 @@ __global__ void interleave_kernel(long n3, const double *__restrict__ src,
     }
 }
 }  // namespace
+
+namespace {
+__global__ void copy_lines_kernel(Dims d, int axis, const double *__restrict__ src,
+                                  double *__restrict__ dst,
+                                  const unsigned long long *__restrict__ stop) {
+    if (*stop) return;
+    const long ni = d.wIUB - d.wILB + 1 + (axis == 1 ? 1 : 0);
+    const long nj = d.wJUB - d.wJLB + 1 + (axis == 0 ? 1 : 0);
+    const long n = ni * nj * (d.wKUB - d.wKLB + 1);
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n;
+         q += (long)gridDim.x * blockDim.x) {
+        const long r = q / ni;
+        const long x = d.idx(d.wILB + (int)(q - r * ni), d.wJLB + (int)(r % nj),
+                             d.wKLB + (int)(r / nj));
+        dst[x] = src[x];
+    }
+}
+
+__global__ void land_fill_kernel(long n3, const uint8_t *__restrict__ mask,
+                                 double *__restrict__ dst) {
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n3;
+         q += (long)gridDim.x * blockDim.x)
+        if (mask[q] & M_LAND) dst[q] = MGPU_NULL_REAL;
+}
+}  // namespace
+
+int launch_copy_lines(const Dims &d, int axis, const double *src, double *dst,
+                      const unsigned long long *stop, hipStream_t s) {
+    const long n = (long)(d.wIUB - d.wILB + 2) * (d.wJUB - d.wJLB + 2) * (d.wKUB - d.wKLB + 1);
+    long blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(copy_lines_kernel, dim3((unsigned)blocks), dim3(256), 0, s, d, axis, src,
+                       dst, stop);
+    HIPCHK(hipGetLastError());
+    return MGPU_SUCCESS;
+}
+
+int launch_land_fill(const Dims &d, const uint8_t *mask, double *dst, hipStream_t s) {
+    const long n3 = d.n3();
+    long blocks = (n3 + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(land_fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s, n3, mask, dst);
+    HIPCHK(hipGetLastError());
+    return MGPU_SUCCESS;
+}
 
 int launch_interleave(long n3, const double *src, double *dst, long stride, int t,
                       int to_interleaved, hipStream_t s) {

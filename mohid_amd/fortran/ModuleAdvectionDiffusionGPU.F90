@@ -8,6 +8,7 @@
 !   AdvectionDiffusion       Software/MOHIDBase2/ModuleAdvectionDiffusion.F90:1108
 !   KillAdvectionDiffusion   Software/MOHIDBase2/ModuleAdvectionDiffusion.F90:5849
 !   THOMASZ_NewType2         Software/MOHIDBase1/ModuleFunctions.F90:4026
+!   THOMAS_3D                Software/MOHIDBase1/ModuleFunctions.F90:455, 3667
 ! so a caller such as ModuleWaterProperties::Advection_Diffusion_Processes
 ! (Software/MOHIDWater/ModuleWaterProperties.F90:14776) can switch its call.
 !
@@ -27,12 +28,15 @@ module ModuleAdvectionDiffusionGPU
     public :: AdvectionDiffusionGPU
     public :: KillAdvectionDiffusionGPU
     public :: THOMASZ_GPU
+    public :: THOMAS_3D_GPU
     public :: MGPU_SUCCESS, MGPU_OFF_ERR, MGPU_ERR_OPTION, MGPU_ERR_UNSUPPORTED
+    public :: MGPU_ERR_PIVOT
 
     integer, parameter :: MGPU_SUCCESS         = 0
     integer, parameter :: MGPU_OFF_ERR         = 1
     integer, parameter :: MGPU_ERR_OPTION      = 102
     integer, parameter :: MGPU_ERR_UNSUPPORTED = 103
+    integer, parameter :: MGPU_ERR_PIVOT       = 107
 
     ! AdvectionDiffusion boundary-condition ids (ModuleAdvectionDiffusion.F90:152-158)
     integer, parameter :: NullGradient_   = 4
@@ -103,6 +107,17 @@ module ModuleAdvectionDiffusionGPU
             integer(c_int32_t),  value      :: dev
             type(c_ptr),         value      :: stream
         end function mgpu_thomasz
+
+        integer(c_int) function mgpu_thomas3d(size, IJmin, IJmax, JImin, JImax, Kmin, Kmax, &
+                                              di, dj, D, E, F, TI, res, dev, stream)       &
+                                              bind(c, name="mgpu_thomas3d")
+            import :: c_int, c_int32_t, c_ptr, t_mgpu_size3d
+            type(t_mgpu_size3d), intent(in) :: size
+            integer(c_int32_t),  value      :: IJmin, IJmax, JImin, JImax, Kmin, Kmax, di, dj
+            type(c_ptr),         value      :: D, E, F, TI, res
+            integer(c_int32_t),  value      :: dev
+            type(c_ptr),         value      :: stream
+        end function mgpu_thomas3d
     end interface
 
     ! instance table: integer IDs as MOHID's modules hand out (ModuleGlobalData
@@ -301,5 +316,26 @@ contains
                           c_loc(TI), c_loc(WaterPoints), c_loc(RES), 0_c_int32_t, c_null_ptr)
         if (present(STAT)) STAT = rc
     end subroutine THOMASZ_GPU
+
+    !> THOMAS_3D (ModuleFunctions.F90:3667-3870, the T_THOMAS form) with the
+    !> reference's bounds and direction arguments on the caller's D/E/F/TI
+    !> arrays (Size-shaped), writing the lines' rows JImin..JImax+1 of RES.
+    !> STAT = MGPU_ERR_PIVOT where the reference stops on a zero pivot.
+    subroutine THOMAS_3D_GPU(Size, IJmin, IJmax, JImin, JImax, Kmin, Kmax, di, dj,   &
+                             D, E, F, TI, RES, STAT)
+        integer, dimension(6),             intent(in)    :: Size
+        integer,                           intent(in)    :: IJmin, IJmax, JImin, JImax
+        integer,                           intent(in)    :: Kmin, Kmax, di, dj
+        real(8), dimension(:,:,:), contiguous, target, intent(in)    :: D, E, F, TI
+        real(8), dimension(:,:,:), contiguous, target, intent(inout) :: RES
+        integer, optional,                 intent(out)   :: STAT
+        integer :: rc
+        rc = mgpu_thomas3d(to_size3d(Size), int(IJmin, c_int32_t), int(IJmax, c_int32_t),  &
+                           int(JImin, c_int32_t), int(JImax, c_int32_t),                   &
+                           int(Kmin, c_int32_t), int(Kmax, c_int32_t),                     &
+                           int(di, c_int32_t), int(dj, c_int32_t), c_loc(D), c_loc(E),     &
+                           c_loc(F), c_loc(TI), c_loc(RES), 0_c_int32_t, c_null_ptr)
+        if (present(STAT)) STAT = rc
+    end subroutine THOMAS_3D_GPU
 
 end module ModuleAdvectionDiffusionGPU

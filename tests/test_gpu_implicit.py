@@ -1,0 +1,286 @@
+"""Implicit horizontal advection on the GPU (THOMAS_3D line solves) against
+the oracle, through the C ABI.
+
+The reference solves the implicit direction line by line (THOMAS_3D_NewType,
+ModuleFunctions.F90:3667-3870) after assembling the rows from the face
+coefficients (ModuleAdvectionDiffusion.F90:4490-4525, 4858-4885); in 3-D the
+vertical step then restarts from the line solution (AD:4167-4256), with one
+layer the line solve is the step (AD:1758-1843).  ModuleWaterProperties
+alternates the implicit direction every step (WP:14681-14712); the tests do
+the same.  Bar: |gpu - oracle| <= 1e-12 |oracle| on every cell (the FP64
+path keeps the reference's operation order; bit-identity is recorded).
+"""
+import ctypes as ct
+
+import numpy as np
+import pytest
+
+from mohid_amd import constants as C
+from mohid_amd import fixtures as fx
+from oracle import oracle as ora
+
+from .cases import SMALL, make
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+RTOL_FAST = 1e-11
+
+
+def _mods():
+    from mohid_amd import _lib as L
+    from mohid_amd.advection_diffusion import AdvectionDiffusion, Transport, thomas3d
+    return L, AdvectionDiffusion, Transport, thomas3d
+
+
+def _rel_err(a, b):
+    den = np.maximum(np.abs(b), 1e-300)
+    return float(np.max(np.abs(a - b) / den))
+
+
+def _implicit_params(case, **extra):
+    p = dict(case.params)
+    if p["AdvMethodH"] in (C.UPWIND_ORDER2, C.UPWIND_ORDER3):
+        p["AdvMethodH"] = C.UPWIND_ORDER1            # AD:1231-1239
+    p.update(extra)
+    return p
+
+
+def _dirs(step, start):
+    """(ImpExp_AdvXX, ImpExp_AdvYY) of a step, alternating as WP:14681-14712."""
+    xx = (step + start) % 2 == 0
+    return dict(ImpExp_AdvXX=1.0 if xx else 0.0, ImpExp_AdvYY=0.0 if xx else 1.0)
+
+
+@pytest.mark.parametrize("di,dj,sub", [(0, 1, False), (1, 0, False), (0, 1, True), (1, 0, True)])
+def test_thomas3d_gpu_matches_oracle(di, dj, sub):
+    """mgpu_thomas3d against the oracle's THOMAS_3D (pinned to the
+    reference's own build) on random diagonally dominant lines."""
+    L, _, _, thomas3d = _mods()
+    rng = np.random.default_rng(31 + di + 2 * sub)
+    ni, nj, nk = 37, 29, 6
+    size = (0, ni + 1, 0, nj + 1, 0, nk + 1)
+    shape = (ni + 2, nj + 2, nk + 2)
+    D = np.asfortranarray(-rng.uniform(0, 1, shape))
+    F = np.asfortranarray(-rng.uniform(0, 1, shape))
+    E = np.asfortranarray(1.0 + rng.uniform(0, 1, shape) - D - F)
+    TI = np.asfortranarray(rng.normal(0, 3, shape))
+    if di == 0:
+        lim = (1, ni, 1, nj, 1, nk) if not sub else (3, ni - 4, 2, nj - 5, 2, nk - 1)
+    else:
+        lim = (1, nj, 1, ni, 1, nk) if not sub else (2, nj - 3, 4, ni - 2, 1, nk - 2)
+    res_gpu = np.asfortranarray(rng.normal(size=shape))
+    res_ora = res_gpu.copy(order="F")
+    thomas3d(size, *lim, di, dj, D, E, F, TI, res_gpu)
+    w, g = np.zeros(max(ni, nj) + 3), np.zeros(max(ni, nj) + 3)
+    dp = ct.POINTER(ct.c_double)
+    rc = ora.lib().ora_thomas3d(ct.byref(ora.Size3D(*size)), *lim, di, dj,
+                                D.ctypes.data_as(dp), E.ctypes.data_as(dp), F.ctypes.data_as(dp),
+                                TI.ctypes.data_as(dp), res_ora.ctypes.data_as(dp),
+                                w.ctypes.data_as(dp), g.ctypes.data_as(dp), None)
+    assert rc == 0
+    assert np.array_equal(res_gpu, res_ora)
+
+
+def test_thomas3d_gpu_rejects():
+    L, _, _, thomas3d = _mods()
+    ni, nj, nk = 5, 4, 3
+    size = (0, ni + 1, 0, nj + 1, 0, nk + 1)
+    shape = (ni + 2, nj + 2, nk + 2)
+    D = np.zeros(shape, order="F")
+    F = np.zeros(shape, order="F")
+    E = np.ones(shape, order="F")
+    TI = np.ones(shape, order="F")
+    res = np.zeros(shape, order="F")
+    E[2, 3, 1] = 0.0                     # zero pivot on line (i=2, k=1), row j=3
+    with pytest.raises(L.MgpuError) as ei:
+        thomas3d(size, 1, ni, 1, nj, 1, nk, 0, 1, D, E, F, TI, res)
+    assert ei.value.code == L.MGPU_ERR_PIVOT
+    E[2, 3, 1] = 1.0
+    with pytest.raises(L.MgpuError) as ei:               # MF ERR01
+        thomas3d(size, 1, ni, 1, nj, 1, nk, 1, 1, D, E, F, TI, res)
+    assert ei.value.code == L.MGPU_ERR_OPTION
+    with pytest.raises(L.MgpuError) as ei:               # rows past Size
+        thomas3d(size, 1, ni, 1, nj + 1, 1, nk, 0, 1, D, E, F, TI, res)
+    assert ei.value.code == L.MGPU_ERR_BADARG
+
+
+def _drop_in_vs_oracle(case, steps, params_of, vertical1d=False, tracers=None):
+    """Per-tracer calls (WP's call pattern) through the drop-in and the
+    oracle; params_of(step, tracer) gives each call's options.  Returns the
+    worst relative error and whether every call was bit-identical."""
+    _, AD, _, _ = _mods()
+    gpu = AD(case.size, case.worksize, case.geom, vertical1d=vertical1d)
+    inst = ora.Instance(case, nthreads=1, vertical1d=vertical1d)
+    idx = range(len(case.tracers)) if tracers is None else tracers
+    p_gpu = [np.array(case.tracers[t], order="F", copy=True) for t in idx]
+    p_ora = [np.array(case.tracers[t], order="F", copy=True) for t in idx]
+    worst, bitwise = 0.0, True
+    for s in range(steps):
+        for t in range(len(p_gpu)):
+            prm = params_of(s, t)
+            first = t == 0
+            now = (s + 1) * prm["DTProp"]
+            rc = inst.step(p_ora[t], first_property=first, now=now, **prm)
+            assert rc == 0, (s, t, rc)
+            gpu(p_gpu[t], case.hydro, prm, first_property=first)
+            err = _rel_err(p_gpu[t], p_ora[t])
+            worst = max(worst, err)
+            assert err <= RTOL, (s, t, err)
+            bitwise &= bool(np.array_equal(p_gpu[t], p_ora[t]))
+    gpu.kill()
+    inst.close()
+    return worst, bitwise
+
+
+@pytest.mark.parametrize("start", [0, 1])
+@pytest.mark.parametrize("name", sorted(SMALL))
+def test_implicit_drop_in_matches_oracle(name, start):
+    """Every small case, 6 steps, implicit direction alternating from X
+    (start 0) or Y (start 1), every tracer per step: every cell of PROP
+    (the lines' ghost ends and land included) against the oracle."""
+    case = make(name)
+    base = _implicit_params(case)
+    worst, bitwise = _drop_in_vs_oracle(case, 6, lambda s, t: dict(base, **_dirs(s, start)))
+    print(f"{name} start={start}: max rel err {worst:.3e} bit-identical={bitwise}")
+
+
+@pytest.mark.parametrize("optimize", [False, True])
+def test_implicit_mixed_directions_within_a_step(optimize):
+    """Tracers of one step with different implicit directions and explicit
+    ones: with the optimised path the reference keeps TI between calls, so
+    ghost rows carry the previous tracer's values into the next call."""
+    case = make("tvd_ragged_opt")
+    base = _implicit_params(case, Optimize=optimize)
+    plans = [dict(ImpExp_AdvXX=1.0, ImpExp_AdvYY=0.0), dict(ImpExp_AdvXX=0.0, ImpExp_AdvYY=1.0),
+             dict(ImpExp_AdvXX=0.0, ImpExp_AdvYY=0.0)]
+    worst, bitwise = _drop_in_vs_oracle(
+        case, 4, lambda s, t: dict(base, **plans[(t + s) % 3]))
+    print(f"mixed optimize={optimize}: max rel err {worst:.3e} bit-identical={bitwise}")
+
+
+@pytest.mark.parametrize("optimize", [False, True])
+@pytest.mark.parametrize("start", [0, 1])
+def test_implicit_one_layer(start, optimize):
+    """One layer (WorkSize KUB = 1): the line solve replaces THOMASZ
+    (AD:1758-1843), after the land fill."""
+    cfg = fx.config("c3", ni=21, nj=17, nk=1, land_frac=0.3,
+                    tracers=("temperature", "lognormal", "salinity"))
+    case = fx.make_case(cfg)
+    base = _implicit_params(case, Optimize=optimize)
+    worst, bitwise = _drop_in_vs_oracle(case, 4, lambda s, t: dict(base, **_dirs(s, start)))
+    print(f"one layer start={start} optimize={optimize}: {worst:.3e} bit-identical={bitwise}")
+
+
+@pytest.mark.parametrize("nk", [1, 5])
+def test_implicit_vertical1d(nk):
+    """Vertical1D instances: no horizontal terms; in 3-D no line solve
+    (HorizontalAdvection is skipped, AD:1704), with one layer the line solve
+    still runs (AD:1758)."""
+    cfg = fx.config("c2", ni=9, nj=8, nk=nk)
+    case = fx.make_case(cfg)
+    base = _implicit_params(case)
+    worst, _ = _drop_in_vs_oracle(case, 3, lambda s, t: dict(base, **_dirs(s, 0)),
+                                  vertical1d=True)
+    print(f"vertical1d nk={nk}: {worst:.3e}")
+
+
+def _oracle_alternating(case, steps, start, params):
+    inst = ora.Instance(case, nthreads=1)
+    props = [np.array(t, order="F", copy=True) for t in case.tracers]
+    for s in range(steps):
+        for t, p in enumerate(props):
+            rc = inst.step(p, first_property=t == 0, now=(s + 1) * params["DTProp"],
+                           **dict(params, **_dirs(s, start)))
+            assert rc == 0
+    inst.close()
+    return props
+
+
+@pytest.mark.parametrize("start", [0, 1])
+@pytest.mark.parametrize("name", sorted(SMALL))
+def test_implicit_batched_matches_oracle(name, start):
+    """Batched transport with implicit horizontal advection: 4 graph steps
+    then 3 eager steps (the direction alternates across both), every tracer
+    against the oracle's per-tracer calls."""
+    L, _, Transport, _ = _mods()
+    case = make(name)
+    params = _implicit_params(case, **_dirs(0, start))
+    T = len(case.tracers)
+    tr = Transport(case.size, case.worksize, case.geom, T)
+    tr.set_hydro(case.hydro, params)
+    for t, p in enumerate(case.tracers):
+        tr.set_tracer(t, p)
+    tr.run(4, use_graph=True)
+    tr.run(3, use_graph=False)
+    tr.status()
+    got = [tr.get_tracer(t) for t in range(T)]
+    tr.close()
+    ref = _oracle_alternating(case, 7, start, params)
+    worst, bitwise = 0.0, True
+    for t in range(T):
+        err = _rel_err(got[t], ref[t])
+        worst = max(worst, err)
+        assert err <= RTOL, (name, t, err)
+        bitwise &= bool(np.array_equal(got[t], ref[t]))
+    print(f"{name} start={start}: batched {worst:.3e} bit-identical={bitwise}")
+
+
+@pytest.mark.parametrize("shape", [(0, 0, 0), (1, 2, 4), (2, 1, 2)])
+def test_implicit_batched_launch_shapes(shape):
+    """Tracers per launch / per lane do not change the implicit results."""
+    L, _, Transport, _ = _mods()
+    case = make("tvd_land_sigma", tracers=("temperature", "lognormal", "salinity", "lognormal"))
+    params = _implicit_params(case, **_dirs(0, 0))
+    outs = []
+    for shp in ((0, 0, 0), shape):
+        tr = Transport(case.size, case.worksize, case.geom, len(case.tracers))
+        tr.set_lanes(*shp)
+        tr.set_hydro(case.hydro, params)
+        for t, p in enumerate(case.tracers):
+            tr.set_tracer(t, p)
+        tr.run(3, use_graph=True)
+        outs.append([tr.get_tracer(t) for t in range(len(case.tracers))])
+        tr.close()
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
+def test_implicit_fast_mode():
+    L, _, Transport, _ = _mods()
+    case = make("tvd_ragged_opt")
+    params = _implicit_params(case, **_dirs(0, 0))
+    T = len(case.tracers)
+    tr = Transport(case.size, case.worksize, case.geom, T, precision=L.MGPU_PREC_FAST)
+    tr.set_hydro(case.hydro, params)
+    for t, p in enumerate(case.tracers):
+        tr.set_tracer(t, p)
+    tr.run(6, use_graph=True)
+    got = [tr.get_tracer(t) for t in range(T)]
+    tr.close()
+    ref = _oracle_alternating(case, 6, 0, params)
+    worst = max(_rel_err(g, r) for g, r in zip(got, ref))
+    print(f"implicit fast: max rel err {worst:.3e}")
+    assert worst <= RTOL_FAST
+
+
+def test_implicit_rejects():
+    L, AD, Transport, _ = _mods()
+    case = make("upwind_flat")
+    gpu = AD(case.size, case.worksize, case.geom)
+    p = np.array(case.tracers[0], order="F", copy=True)
+    for ov in (dict(ImpExp_AdvXX=0.5), dict(ImpExp_AdvYY=0.25)):      # AD:4527/4887 ERR01
+        with pytest.raises(L.MgpuError) as ei:
+            gpu(p, case.hydro, dict(case.params, **ov))
+        assert ei.value.code == L.MGPU_ERR_OPTION
+    with pytest.raises(L.MgpuError) as ei:                          # ERR100
+        gpu(p, case.hydro, dict(case.params, ImpExp_AdvXX=1.0, AdvMethodH=C.UPWIND_ORDER2))
+    assert ei.value.code == L.MGPU_ERR_OPTION
+    assert np.array_equal(p, case.tracers[0])
+    gpu.kill()
+    # the tile variants do not run the line solves
+    tr = Transport(case.size, case.worksize, case.geom, 1, kernel=L.MGPU_KERNEL_SPLIT)
+    with pytest.raises(L.MgpuError) as ei:
+        tr.set_hydro(case.hydro, _implicit_params(case, ImpExp_AdvXX=1.0))
+    assert ei.value.code == L.MGPU_ERR_UNSUPPORTED
+    tr.close()

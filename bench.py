@@ -79,6 +79,9 @@ def main():
     ap.add_argument("--kernel", default="lanes", choices=("lanes", "split", "fused"))
     ap.add_argument("--tracers-per-launch", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--implicit", default="none", choices=("none", "x", "y"),
+                    help="implicit horizontal advection (THOMAS_3D line solves), first "
+                         "direction; it alternates every step")
     ap.add_argument("--suite", action="store_true",
                     help="instead of the headline line, run the comparison suite: plain FP64 "
                          "drop-in vs batched on 512x512x50, FAST vs FP64 per-tracer error and "
@@ -132,9 +135,10 @@ def main():
     prec = _lib.MGPU_PREC_FAST if args.precision == "fast" else _lib.MGPU_PREC_FP64
     kernel = {"lanes": _lib.MGPU_KERNEL_LANES, "split": _lib.MGPU_KERNEL_SPLIT,
               "fused": _lib.MGPU_KERNEL_FUSED}[args.kernel]
+    params = _with_implicit(case.params, args.implicit)
     tr = Transport(case.size, case.worksize, case.geom, T, precision=prec, kernel=kernel,
                    tracers_per_launch=args.tracers_per_launch)
-    tr.set_hydro(case.hydro, case.params)
+    tr.set_hydro(case.hydro, params)
     for t, p in enumerate(case.tracers):
         tr.set_tracer(t, p)
     use_graph = not args.no_graph
@@ -189,6 +193,8 @@ def main():
     alg_bytes = per_unit * nwet * T
     achieved = alg_bytes / (trans_ms * 1e-3) / 1e9
     kernel_key = f"transport_{args.kernel}_{args.precision}"
+    if args.implicit != "none":
+        kernel_key += "_implicit"
     traffic = _traffic_from_profiles(kernel_key, cfg.name)
     _log(f"per step: prep {prep_ms:.3f} ms, explicit {h_ms:.3f} ms, vertical {v_ms:.3f} ms "
          f"({nsteps_t} steps); transport {achieved:.1f} GB/s algorithmic")
@@ -216,6 +222,7 @@ def main():
             "kernel": args.kernel,
             "launch_shape": list(tuned["shape"]) if tuned else None,
             "optimize_path": bool(case.params["Optimize"]),
+            "implicit_horizontal": args.implicit,
         },
         "roofline": {
             "bound": "hbm",
@@ -224,7 +231,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": ("explicit_lane_kernel + column_lane_kernel" if args.kernel == "lanes"
+            "kernel": ("line_lane_kernel + column_lane_kernel" if args.implicit != "none"
+                       else "explicit_lane_kernel + column_lane_kernel" if args.kernel == "lanes"
                        else "explicit_kernel + vertical_kernel") + " (all tracers of one step)",
             "kernel_ms": trans_ms,
             "explicit_kernel_ms": h_ms,
@@ -251,7 +259,8 @@ def main():
 
         def ostep(s):
             for t, p in enumerate(props):
-                rc = inst.step(p, first_property=(t == 0), now=(s + 1) * dt)
+                rc = inst.step(p, first_property=(t == 0), now=(s + 1) * dt,
+                               **_implicit_dirs(args.implicit, s))
                 assert rc == 0, rc
 
         ostep(0)                      # untimed first step
@@ -265,7 +274,7 @@ def main():
         cval = sT * swet * nsteps / cwall
         tr2 = Transport(scase.size, scase.worksize, scase.geom, sT, precision=prec, kernel=kernel,
                         tracers_per_launch=args.tracers_per_launch)
-        tr2.set_hydro(scase.hydro, scase.params)
+        tr2.set_hydro(scase.hydro, _with_implicit(scase.params, args.implicit))
         for t, p in enumerate(scase.tracers):
             tr2.set_tracer(t, p)
         tr2.run(1 + nsteps, use_graph=False)
@@ -291,6 +300,19 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def _implicit_dirs(first, step):
+    """ImpExp_AdvXX/YY of a step when the implicit direction starts at
+    `first` ("x" or "y") and alternates every step (WP:14681-14712)."""
+    if first == "none":
+        return {}
+    xx = (step % 2 == 0) == (first == "x")
+    return {"ImpExp_AdvXX": 1.0 if xx else 0.0, "ImpExp_AdvYY": 0.0 if xx else 1.0}
+
+
+def _with_implicit(params, first):
+    return dict(params, **_implicit_dirs(first, 0))
+
+
 def _timed(fn, stream, k):
     """Wall seconds of k calls of fn(), with the device idle before and after."""
     import torch
@@ -302,7 +324,7 @@ def _timed(fn, stream, k):
     return time.perf_counter() - w0
 
 
-def _batched_rate(case, precision, steps, chunk=None, tune=True):
+def _batched_rate(case, precision, steps, chunk=None, tune=True, implicit="none"):
     """Batched transport of all the case's tracers: autotuned shape, steps
     replayed as captured graphs of `chunk` steps; returns (updates/s, shape)."""
     import torch
@@ -310,7 +332,7 @@ def _batched_rate(case, precision, steps, chunk=None, tune=True):
     from mohid_amd.autotune import autotune
     T = len(case.tracers)
     tr = Transport(case.size, case.worksize, case.geom, T, precision=precision)
-    tr.set_hydro(case.hydro, case.params)
+    tr.set_hydro(case.hydro, _with_implicit(case.params, implicit))
     for t, p in enumerate(case.tracers):
         tr.set_tracer(t, p)
     shape = autotune(tr, steps=2)["shape"] if tune else None
@@ -378,6 +400,7 @@ def suite(args):
           "value": drop3, "unit": "cell-updates/s", "ms_per_step": 1e3 * wall / k})
     r64, s64 = _batched_rate(case, _lib.MGPU_PREC_FP64, 20)
     rf, sf = _batched_rate(case, _lib.MGPU_PREC_FAST, 20)
+    r64_headline = r64
     emit({"measure": "batched FP64 vs FAST, 1024x1024x40 x 8 tracers",
           "fp64": r64, "fast": rf, "unit": "cell-updates/s", "fast_over_fp64": rf / r64,
           "fp64_over_plain_dropin": r64 / drop3, "fast_over_plain_dropin": rf / drop3,
@@ -416,7 +439,51 @@ def suite(args):
     del case, outs
     torch.cuda.empty_cache()
 
-    # 4. large run: 2048 x 2048 x 60 sigma, 4 tracers, graphs of 50 steps
+    # 4. implicit horizontal advection (THOMAS_3D line solves, direction
+    # alternating every step) on the headline shape: batched FP64 and FAST,
+    # per-kernel times, and the oracle on a bounded window of the same case
+    case = fx.make_case(fx.config("c3"), device="cuda")
+    T = len(case.tracers)
+    i64, si = _batched_rate(case, _lib.MGPU_PREC_FP64, 20, implicit="x")
+    ifa, sfa = _batched_rate(case, _lib.MGPU_PREC_FAST, 20, implicit="x")
+    tr = Transport(case.size, case.worksize, case.geom, T)
+    tr.set_hydro(case.hydro, _with_implicit(case.params, "x"))
+    for t, p in enumerate(case.tracers):
+        tr.set_tracer(t, p)
+    tr.run(2, use_graph=False)
+    tr.timing(True, 4)
+    tr.run(4, use_graph=False)
+    prep_ms, line_ms, col_ms, _ = tr.kernel_times()
+    tr.timing(False)
+    tr.close()
+    del case
+    torch.cuda.empty_cache()
+    emit({"measure": "implicit horizontal advection (alternating x/y), 1024x1024x40 x 8 tracers",
+          "fp64": i64, "fast": ifa, "unit": "cell-updates/s",
+          "explicit_fp64_same_workload": r64_headline, "implicit_over_explicit": i64 / r64_headline,
+          "line_kernel_ms": line_ms, "column_kernel_ms": col_ms, "prep_kernel_ms": prep_ms,
+          "alg_GBps_fp64": i64 * (16 + 64 / T) / 1e9,
+          "launch_shapes": {"fp64": si, "fast": sfa}})
+    from oracle import oracle as ora
+    scase = fx.make_case(fx.config("c3", ni=256, nj=256))
+    inst = ora.Instance(scase, nthreads=min(16, os.cpu_count() or 1))
+    props = [np.array(t, order="F", copy=True) for t in scase.tracers]
+    dt = scase.params["DTProp"]
+    c0 = time.perf_counter()
+    ns = 0
+    while ns < 2 or (time.perf_counter() - c0 < 10.0 and ns < 40):
+        for t, p in enumerate(props):
+            assert inst.step(p, first_property=(t == 0), now=(ns + 1) * dt,
+                             **dict(scase.params, **_implicit_dirs("x", ns))) == 0
+        ns += 1
+    cw = time.perf_counter() - c0
+    inst.close()
+    emit({"measure": "oracle (CPU), implicit horizontal advection, 256x256x40 window of the "
+                     "same case x 8 tracers", "value": len(scase.tracers) * scase.n_wet * ns / cw,
+          "unit": "cell-updates/s", "threads": min(16, os.cpu_count() or 1), "steps": ns})
+    del scase, props
+
+    # 5. large run: 2048 x 2048 x 60 sigma, 4 tracers, graphs of 50 steps
     case = fx.make_case(fx.config("c5"), device="cuda")
     r64, s64 = _batched_rate(case, _lib.MGPU_PREC_FP64, 100, chunk=50)
     rf, sf = _batched_rate(case, _lib.MGPU_PREC_FAST, 100, chunk=50)

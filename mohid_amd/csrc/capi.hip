@@ -149,9 +149,9 @@ struct DevDerived {
     uint8_t *cperm = nullptr;
     uint16_t *ccnt = nullptr, *fperm = nullptr, *fcnt = nullptr;
     // compact lists of solved rows and columns (lane variant)
-    uint32_t *rowcnt = nullptr, *cells = nullptr, *cols = nullptr;
+    uint32_t *rowcnt = nullptr, *cells = nullptr, *cols = nullptr, *hrows = nullptr;
     unsigned long long *totals = nullptr;
-    long ncells = 0, ncols = 0;
+    long ncells = 0, ncols = 0, nhrows = 0;
     // [0] THOMASZ zero pivots, [1] bad-method flag, [2] mask flags, [3] THOMAS_3D zero pivots
     unsigned long long *counters = nullptr;
     int alloc(const Dims &d) {
@@ -160,7 +160,7 @@ struct DevDerived {
                    nKw = d.wKUB - d.wKLB + 1;
         int rc = 0;
         rc |= dalloc(&rowcnt, nJw * nKw); rc |= dalloc(&cells, nIw * nJw * nKw);
-        rc |= dalloc(&cols, nIw * nJw); rc |= dalloc(&totals, 2);
+        rc |= dalloc(&cols, nIw * nJw); rc |= dalloc(&totals, 3);
         rc |= dalloc(&mask, n3); rc |= dalloc(&kstart, d.n2());
         rc |= dalloc(&cu, n3); rc |= dalloc(&cv, n3); rc |= dalloc(&cw, n3);
         rc |= dalloc(&cperm, nt * 256); rc |= dalloc(&ccnt, nt * nK);
@@ -177,22 +177,32 @@ struct DevDerived {
     void release() {
         dfree(mask); dfree(kstart); dfree(cu); dfree(cv); dfree(cw); dfree(counters);
         dfree(cperm); dfree(ccnt); dfree(fperm); dfree(fcnt);
-        dfree(rowcnt); dfree(cells); dfree(cols); dfree(totals);
+        dfree(rowcnt); dfree(cells); dfree(cols); dfree(totals); dfree(hrows);
     }
     // masks -> mask bytes, kstart, tile orders (after the mask arrays are on the device)
+    // hlines: also the list of implicit line rows (allocated on first use)
     int pack(const Dims &d, const DevHydro &h, const int32_t *kfloor, int docycle,
-             hipStream_t s, bool tiles) {
+             hipStream_t s, bool tiles, bool hlines = false) {
+        if (hlines && !hrows) {
+            const long nw = (long)(d.wIUB - d.wILB + 1) * (d.wJUB - d.wJLB + 1) *
+                            (d.wKUB - d.wKLB + 1);
+            HIPCHK(hipStreamSynchronize(s));
+            if (dalloc(&hrows, nw)) return MGPU_ERR_NOMEM;
+        }
         int rc = launch_pack_masks(d, h.open, h.land, h.cfu, h.cfv, h.cfw, h.water, kfloor,
                                    docycle, mask, kstart, mask_flags(), s);
         // tile metadata only for the tile variants (level keys are bytes there)
         if (!rc && tiles) rc = launch_tile_sort(d, mask, kstart, cperm, ccnt, fperm, fcnt, s);
-        if (!rc) rc = launch_build_lists(d, mask, kstart, rowcnt, cells, cols, totals, s);
+        if (!rc)
+            rc = launch_build_lists(d, mask, kstart, rowcnt, cells, cols, hlines ? hrows : nullptr,
+                                    totals, s);
         if (rc) return rc;
-        unsigned long long n[2];
+        unsigned long long n[3];
         HIPCHK(hipMemcpyAsync(n, totals, sizeof(n), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         ncells = (long)n[0];
         ncols = (long)n[1];
+        nhrows = hlines ? (long)n[2] : 0;
         return MGPU_SUCCESS;
     }
     SolveLists lists() const {
@@ -201,6 +211,8 @@ struct DevDerived {
         l.cols = cols;
         l.ncells = ncells;
         l.ncols = ncols;
+        l.hrows = hrows;
+        l.nhrows = nhrows;
         return l;
     }
     unsigned int *mask_flags() { return reinterpret_cast<unsigned int *>(counters + 2); }
@@ -310,6 +322,7 @@ struct mgpu_ad {
     // it (the ghost rows: kept by the optimised path, AD:1660-1666), and
     // PROP after implicit line solves
     double *tig = nullptr, *mid = nullptr;
+    double *Rb = nullptr;                   // implicit line row records {D, E, F, TI}
     bool have_faces = false;
     unsigned long long last_zero_pivots = 0;
 };
@@ -346,6 +359,7 @@ extern "C" int mgpu_ad_create(const mgpu_size3d *size, const mgpu_size3d *worksi
     if (!rc) rc = dalloc(&ad->Gb, scr);
     if (!rc) rc = dalloc(&ad->tig, n3);
     if (!rc) rc = dalloc(&ad->mid, n3);
+    if (!rc) rc = dalloc(&ad->Rb, 4 * n3);
     if (!rc) rc = ad->geom.upload(ad->d, geometry, ad->stream);
     if (rc) {
         mgpu_ad_destroy(ad);
@@ -367,7 +381,7 @@ extern "C" int mgpu_ad_destroy(mgpu_ad *ad) {
     ad->hydro.release();
     ad->der.release();
     dfree(ad->prop); dfree(ad->out); dfree(ad->Wb); dfree(ad->Gb);
-    dfree(ad->tig); dfree(ad->mid);
+    dfree(ad->tig); dfree(ad->mid); dfree(ad->Rb);
     if (ad->stream) (void)hipStreamDestroy(ad->stream);
     delete ad;
     return MGPU_SUCCESS;
@@ -399,7 +413,7 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     else HIPCHK(hipMemcpyAsync(ad->prop, prop, b3, hipMemcpyHostToDevice, s));
 
     HIPCHK(hipMemsetAsync(ad->der.counters, 0, sizeof(unsigned long long) * 4, s));
-    rc = ad->der.pack(d, ad->hydro, ad->geom.KFloor, ad->docycle, s, false);
+    rc = ad->der.pack(d, ad->hydro, ad->geom.KFloor, ad->docycle, s, false, sc.impH >= 0);
     if (rc) return rc;
     if (first) {
         // Convert_Visc_Dif_Horizontal(_opt) + Convert_Dif_Vertical (+ the
@@ -429,6 +443,7 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     L.Gb = ad->Gb;
     L.counters = ad->der.counters;
     L.tig = ad->tig;
+    L.Rb = ad->Rb;
     const unsigned long long *stop = ad->der.counters + 3;
     if (sc.hreset) {
         // line solves into mid (a copy of PROP, so that it is PROP after
@@ -436,11 +451,14 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
         // before the column solve
         L.mid = ad->mid;
         HIPCHK(hipMemcpyAsync(ad->mid, P, b3, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(ad->out, P, b3, hipMemcpyDeviceToDevice, s));
         rc = launch_transport(L, d, sc, f, s, 1);
         if (rc) return rc;
         HIPCHK(hipMemcpyAsync(ad->tig, ad->mid, b3, hipMemcpyDeviceToDevice, s));
         rc = launch_transport(L, d, sc, f, s, 2);
     } else {
+        // the line kernel leaves cells whose solution is PROP itself unwritten
+        if (sc.impH >= 0) HIPCHK(hipMemcpyAsync(ad->out, P, b3, hipMemcpyDeviceToDevice, s));
         rc = launch_transport(L, d, sc, f, s);
     }
     if (rc) return rc;
@@ -598,7 +616,7 @@ struct mgpu_transport {
     // 0: along j, 1: along i), alternating every step as WP:14681-14712
     // does, and PROP after the line solves (interleaved like the state)
     int impdir = -1;
-    double *mid = nullptr;
+    double *mid = nullptr, *Rb = nullptr;
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     int graph_steps[2] = {0, 0};
@@ -650,6 +668,7 @@ extern "C" int mgpu_transport_destroy(mgpu_transport *t) {
     dfree(t->Gb);
     dfree(t->stage);
     dfree(t->mid);
+    dfree(t->Rb);
     if (t->stream) (void)hipStreamDestroy(t->stream);
     delete t;
     return MGPU_SUCCESS;
@@ -723,10 +742,10 @@ extern "C" int mgpu_transport_set_hydro(mgpu_transport *t, const mgpu_ad_args *a
     if (sc.impH >= 0) {
         // the line solves run on the lane variant's state-shaped scratch
         if (t->kernel != MGPU_KERNEL_LANES) return MGPU_ERR_UNSUPPORTED;
-        if (sc.vert && !t->mid) {
-            HIPCHK(hipStreamSynchronize(t->stream));
-            if ((rc = dalloc(&t->mid, t->d.n3() * (long)t->opts.ntracers))) return rc;
-        }
+        const long ns = t->d.n3() * (long)t->opts.ntracers;
+        HIPCHK(hipStreamSynchronize(t->stream));
+        if (sc.vert && !t->mid && (rc = dalloc(&t->mid, ns))) return rc;
+        if (!t->Rb && (rc = dalloc(&t->Rb, 4 * ns))) return rc;
     }
     t->sc = sc;
     t->impdir = sc.impH;
@@ -739,12 +758,18 @@ extern "C" int mgpu_transport_set_hydro(mgpu_transport *t, const mgpu_ad_args *a
     if (rc) return rc;
     HIPCHK(hipMemsetAsync(t->der.counters, 0, sizeof(unsigned long long) * 4, t->stream));
     rc = t->der.pack(t->d, t->hydro, t->geom.KFloor, t->opts.docycle_method, t->stream,
-                     t->kernel != MGPU_KERNEL_LANES);
+                     t->kernel != MGPU_KERNEL_LANES, sc.impH >= 0);
     if (rc) return rc;
     unsigned long long cnt[4];
     HIPCHK(hipMemcpyAsync(cnt, t->der.counters, sizeof(cnt), hipMemcpyDeviceToHost, t->stream));
     HIPCHK(hipStreamSynchronize(t->stream));
     if (cnt[2] & 0xffffffffu) return MGPU_ERR_MASKS;
+    // cells a step does not write (not solved under the new masks) keep their
+    // latest value in both ping-pong buffers
+    HIPCHK(hipMemcpyAsync(t->buf[t->cur ^ 1], t->buf[t->cur],
+                          sizeof(double) * (size_t)t->d.n3() * (size_t)t->opts.ntracers,
+                          hipMemcpyDeviceToDevice, t->stream));
+    HIPCHK(hipStreamSynchronize(t->stream));
     drop_graphs(t);  // step scalars are baked into captured launches
     t->have_hydro = true;
     return MGPU_SUCCESS;
@@ -861,6 +886,7 @@ static int enqueue_steps(mgpu_transport *t, int nsteps, int cur, int dir) {
         L.lanes.column_tpl = t->opts.column_tracers_per_lane;
         L.counters = t->der.counters;
         L.mid = t->mid;
+        L.Rb = t->Rb;
         // first kernel(s) of every chunk, then the second kernel(s)
         for (int ph = 1; ph <= 2; ++ph) {
             rc = launch_transport(L, d, sc, f, s, ph);

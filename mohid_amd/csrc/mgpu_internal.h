@@ -22,6 +22,8 @@ enum : uint8_t {
     M_CFV = 8,     // ComputeFacesV3D (face between i-1 and i)
     M_CFW = 16,    // ComputeFacesW3D (face between k-1 and k)
     M_WATER = 32,  // WaterPoints3D
+    M_HID = 64,    // not open and no U/V face on any side: an identity row of
+                   // every horizontal line solve (D = F = 0, E = 1, TI = PROP)
 };
 
 // Extents of a Size-shaped Fortran array and its WorkSize.
@@ -121,13 +123,15 @@ int launch_prep_faces(const Dims &d, const double *visc, const double *diffv,
 struct SolveLists {
     const uint32_t *cells = nullptr;   // Size index of every solved row, memory order
     const uint32_t *cols = nullptr;    // Size index (level 0) of every solved column
-    long ncells = 0, ncols = 0;
+    const uint32_t *hrows = nullptr;   // WorkSize cells without M_HID (implicit line rows)
+    long ncells = 0, ncols = 0, nhrows = 0;
 };
-// Builds the lists from the packed masks and kstart; totals[0] / totals[1]
-// receive the counts (device memory; the caller copies them back).  rowcnt
-// holds (WorkSize J extent) x (WorkSize K extent) counters.
+// Builds the lists from the packed masks and kstart; totals[0..2] receive
+// the counts (device memory; the caller copies them back); hrows may be null
+// (not built).  rowcnt holds (WorkSize J extent) x (WorkSize K extent)
+// counters.
 int launch_build_lists(const Dims &d, const uint8_t *mask, const int32_t *kstart,
-                       uint32_t *rowcnt, uint32_t *cells, uint32_t *cols,
+                       uint32_t *rowcnt, uint32_t *cells, uint32_t *cols, uint32_t *hrows,
                        unsigned long long *totals, hipStream_t s);
 
 // Tracers per lane of the lane variant's kernels (0: library default).
@@ -150,6 +154,7 @@ struct TransportLaunch {
                                             // [2] mask flags, [3] THOMAS_3D zero pivots
     // implicit horizontal advection (lane variant only)
     double *mid = nullptr;       // PROP after the line solves, laid out like P (3-D)
+    double *Rb = nullptr;        // line row records {D, E, F, TI}: 4 doubles per value of P
     // TI left at the ghost rows by earlier calls (stride-1 per-tracer state of
     // the drop-in, Size-shaped); null: every call of the step behaves as one
     // of a run of calls with the same options (the batched transport)
@@ -172,6 +177,9 @@ struct TracerSet {
 
 // Implicit horizontal advection state of one chunk (see TransportLaunch).
 struct ChunkImp {
+    double *Rb = nullptr;         // line row records {D, E, F, TI}, chunk's first tracer
+    const uint32_t *hrows = nullptr;   // cells with a record (SolveLists::hrows)
+    long nhrows = 0;
     double *mid = nullptr;        // PROP after the line solves, chunk's first tracer
     const double *tig = nullptr;  // drop-in ghost-row record (stride 1) or null
     int t_base = 0;               // index of the chunk's first tracer in the run

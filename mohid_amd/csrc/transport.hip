@@ -58,7 +58,7 @@ int launch_prep_faces(const Dims &d, const double *visc, const double *diffv, co
 }
 
 int launch_build_lists(const Dims &d, const uint8_t *mask, const int32_t *kstart,
-                       uint32_t *rowcnt, uint32_t *cells, uint32_t *cols,
+                       uint32_t *rowcnt, uint32_t *cells, uint32_t *cols, uint32_t *hrows,
                        unsigned long long *totals, hipStream_t s) {
     if (d.n3() >= (1L << 32)) return MGPU_ERR_UNSUPPORTED;   // list entries are 32-bit
     const int nj = d.wJUB - d.wJLB + 1, nk = d.wKUB - d.wKLB + 1;
@@ -79,6 +79,17 @@ int launch_build_lists(const Dims &d, const uint8_t *mask, const int32_t *kstart
     hipLaunchKernelGGL(list_write_kernel, dim3(nj, 1), dim3(256), 0, s, d, mask, kstart, 1,
                        rowcnt, cols);
     HIPCHK(hipGetLastError());
+    if (hrows) {
+        hipLaunchKernelGGL(list_count_kernel, dim3(nj, nk), dim3(256), 0, s, d, mask, kstart, 2,
+                           rowcnt);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(list_scan_kernel, dim3(1), dim3(1024), 0, s, rowcnt, (long)nj * nk,
+                           totals + 2);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(list_write_kernel, dim3(nj, nk), dim3(256), 0, s, d, mask, kstart, 2,
+                           rowcnt, hrows);
+        HIPCHK(hipGetLastError());
+    }
     return MGPU_SUCCESS;
 }
 
@@ -100,7 +111,7 @@ int launch_transport(const TransportLaunch &L, const Dims &d, const StepScalars 
     if (sc.impH >= 0) {
         // the line solves are built on the lane variant's state-shaped scratch
         if (L.variant != MGPU_KERNEL_LANES) return MGPU_ERR_UNSUPPORTED;
-        if (sc.vert && !L.mid) return MGPU_ERR_BADARG;
+        if ((sc.vert && !L.mid) || !L.Rb || !L.lists.hrows) return MGPU_ERR_BADARG;
     }
     if (L.tig && L.n != 1) return MGPU_ERR_BADARG;   // a per-tracer record
     for (int t0 = 0; t0 < L.n; t0 += chunk) {
@@ -112,6 +123,9 @@ int launch_transport(const TransportLaunch &L, const Dims &d, const StepScalars 
         ts.G0 = L.tig;
         ChunkImp imp;
         imp.mid = L.mid ? L.mid + t0 : nullptr;
+        imp.Rb = L.Rb ? L.Rb + 4L * t0 : nullptr;
+        imp.hrows = L.lists.hrows;
+        imp.nhrows = L.lists.nhrows;
         imp.tig = L.tig;
         imp.t_base = t0;
         // each chunk has its own scratch region: with the split variant all

@@ -68,6 +68,10 @@ __global__ void pack_masks_kernel(Dims d, const int32_t *__restrict__ open,
         if (inws && cfv[q] == 1) m |= M_CFV;
         if (cfw[q] == 1) m |= M_CFW;
         if (water[q] == 1) m |= M_WATER;
+        // faces above along j (i, j+1) and i (i+1, j), inside the WorkSize
+        const bool up_j = j + 1 <= d.wJUB && inws && cfu[q + d.sj] == 1;
+        const bool up_i = i + 1 <= d.wIUB && inws && cfv[q + 1] == 1;
+        if (!(m & (M_OPEN | M_CFU | M_CFV)) && !up_j && !up_i) m |= M_HID;
         mask[q] = m;
     }
 }
@@ -1220,172 +1224,251 @@ column_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
 }
 
 // ---------------------------------------------------------------------------
-// Implicit horizontal advection: one lane per (line, TPL tracers) walks a
-// WorkSize line along j (AX 0: ImpExp_AdvXX, THOMAS_3D_i0_j1_NewType,
-// MF:3756-3810, lines (i, k)) or along i (AX 1: ImpExp_AdvYY,
-// THOMAS_3D_i1_j0_NewType, MF:3814-3870, lines (j, k)), building each row
-// (cell_row_lane) and running the forward sweep (W, G to scratch laid out
-// like the state), then the ghost end row and the back substitution.
-// Neighbouring lanes of AX 0 take neighbouring i, so every row access of a
-// wave is one contiguous run.
+// Implicit horizontal advection, in two kernels.
 //
-// Rows lo..hi are WorkSize cells; the end row hi+1 is the ghost cell: D = F
-// = 0, E = 1, and TI what earlier work left there: null_real on land after
-// the land fill when the line solve is the last step (one layer,
-// AD:1753-1800); otherwise 0 for the first call of a step, and for later
-// calls of a step with the optimised path (which keeps TI between calls)
-// the previous call's value -- null_real on land (tig == null: every call
-// of the step with these options), or tig's value (the drop-in's record).
+// Rows: one lane per (listed cell, TPL tracers) builds the cell's
+// tridiagonal row along the implicit direction (cell_row_lane: AX 0 along j,
+// ImpExp_AdvXX; AX 1 along i, ImpExp_AdvYY) into a 32-byte record
+// {D, E, F, TI} per (cell, tracer), records laid out like the state.  With
+// one layer (the line solve is the step's solve, AD:1753-1800) TI is
+// null_real on land, as the land fill before it.  The list holds the cells
+// without M_HID; the others get no record: their row is D = F = 0, E = 1,
+// TI = PROP (null_real on land with one layer) -- no open point, no face,
+// the VolumeVariation diagonal 1 -- and the line kernel builds it from PROP.
+template <bool OPT, int NT, int TPL, bool FAST, int AX>
+__global__ void __launch_bounds__(256)
+line_rows_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
+                 const uint32_t *__restrict__ rows, long nrows, double4 *__restrict__ R,
+                 unsigned int *__restrict__ badflag) {
+    constexpr int LPC = NT / TPL;
+    const long g = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    const long n = g / LPC;
+    if (n >= nrows) return;
+    const int t = (int)(g - n * LPC) * TPL;
+    const uint32_t xr = rows[n];
+    const long x = xr, s = ts.s;
+    const CellPos c = cell_pos(d, xr);
+    const TracerSet tl{ts.P + t, ts.out + t, s, nullptr};
+    int bad = 0;
+    double TI[TPL], D[TPL], E[TPL], F[TPL];
+    cell_row_lane<OPT, TPL, FAST, AX>(d, sc, f, tl, x, c.i, c.j, c.k, TI, D, E, F, bad);
+    const bool nul = !sc.vert && (f.mask[x] & M_LAND);
+#pragma unroll
+    for (int u = 0; u < TPL; ++u)
+        R[x * s + t + u] = make_double4(D[u], E[u], F[u], nul ? MGPU_NULL_REAL : TI[u]);
+    if (bad) atomicOr(badflag, 1u);
+}
+
+__device__ __forceinline__ void line_put(double *md, double *out, long y, bool solved, double v) {
+    if (md) md[y] = v;
+    if (!md || !solved) out[y] = v;
+}
+
+// Lines: one lane per (line, tracer) solves a WorkSize line along j (AX 0,
+// lines (i, k): THOMAS_3D_i0_j1_NewType, MF:3756-3810) or along i (AX 1,
+// lines (j, k): THOMAS_3D_i1_j0_NewType, MF:3814-3870) from the row
+// records: the forward sweep (W and G over the record's first half, each
+// record read before it is written), the ghost end row, the back
+// substitution.  LROWS consecutive records are loaded before they are swept,
+// so each lane keeps several independent loads in flight.  Neighbouring
+// lanes take neighbouring tracers, then (AX 0) neighbouring i, so every
+// record access of a wave is one contiguous run.
+//
+// The end row hi+1 is the ghost cell: D = F = 0, E = 1, and TI what earlier
+// work left there: null_real on land after the land fill when the line solve
+// is the step's solve (one layer); otherwise 0 for the first call of a step,
+// and for later calls of a step with the optimised path (which keeps TI
+// between calls) the previous call's value -- null_real on land (tig ==
+// null: every call of the step with these options), or tig's value (the
+// drop-in's record).
 //
 // In 3-D the solution goes to `mid` (PROP after the line solves, laid out
 // like P); lanes of the bottom and top levels also copy P's rows KLB-1 and
 // KUB+1 there, which the column walk reads.  Cells the column solve does not
 // overwrite (columns it does not solve, the ghost row) take the line
-// solution in `out` as well.  With one layer (no column solve) every line
-// cell goes to `out`.
-template <int TPL>
-__device__ __forceinline__ void line_put(double *md, double *out, long x, long s, bool solved,
-                                         const double (&v)[TPL]) {
-    if (md) store_tpl<TPL>(md + x * s, v);
-    if (!md || !solved) store_tpl<TPL>(out + x * s, v);
-}
+// solution in `out` as well.  With one layer every line cell goes to `out`.
+constexpr int LROWS = 4;
 
-template <bool OPT, int NT, int TPL, bool FAST, int AX>
+template <int NT, bool FAST, int AX>
 __global__ void __launch_bounds__(256)
-line_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts, double *__restrict__ mid,
-                 const double *__restrict__ tig, int t_base, double *__restrict__ Ws,
-                 double *__restrict__ Gs, unsigned long long *__restrict__ pivots,
-                 unsigned int *__restrict__ badflag) {
-    constexpr int LPC = NT / TPL;
+line_thomas_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts, double4 *__restrict__ R,
+                   double *__restrict__ mid, const double *__restrict__ tig, int t_base,
+                   unsigned long long *__restrict__ pivots) {
     const long g = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    const long n = g / LPC;
+    const long n = g / NT;
     const int na = AX == 0 ? d.wIUB - d.wILB + 1 : d.wJUB - d.wJLB + 1;
     const long nl = (long)na * (d.wKUB - d.wKLB + 1);
     if (n >= nl) return;
-    const int t = (int)(g - n * LPC) * TPL;
+    const int t = (int)(g - n * NT);
     const int a = (AX == 0 ? d.wILB : d.wJLB) + (int)(n % na);
     const int k = d.wKLB + (int)(n / na);
     const int lo = AX == 0 ? d.wJLB : d.wILB, hi = AX == 0 ? d.wJUB : d.wIUB;
-    const long st = AX == 0 ? d.sj : 1;
-    const long x0 = AX == 0 ? d.idx(a, lo, k) : d.idx(lo, a, k);
     const long s = ts.s;
-    const TracerSet tl{ts.P + t, ts.out + t, s, nullptr};
-    double *ws = Ws + t, *gs = Gs + t;
-    double *md = mid ? mid + t : nullptr;
-    const bool last = !sc.vert;           // one layer: the line solve is the step's solve
-    const long top = (long)(d.wKUB - k) * d.sk;   // offset of the column's KUB cell
-    int bad = 0;
+    const long ss = (AX == 0 ? d.sj : 1) * s;                                 // row stride
+    const long x0 = (AX == 0 ? d.idx(a, lo, k) : d.idx(lo, a, k)) * s + t;   // row lo, slot t
+    const bool last = !sc.vert;
+    const bool edge = mid && (k == d.wKLB || k == d.wKUB);
+    const long ez = (k == d.wKLB ? -d.sk : d.sk) * s;
     bool pivot = false;
-    double Wm[TPL], Gm[TPL];
-    for (int r = lo; r <= hi; ++r) {
-        const long x = x0 + (long)(r - lo) * st;
-        const int i = AX == 0 ? a : r, j = AX == 0 ? r : a;
-        double TI[TPL], D[TPL], E[TPL], F[TPL];
-        cell_row_lane<OPT, TPL, FAST, AX>(d, sc, f, tl, x, i, j, k, TI, D, E, F, bad);
-        if (last && (f.mask[x] & M_LAND)) {
+    double Wm = 0.0, Gm = 0.0;
+    const int nrow = hi - lo + 1;
+    // An identity row's W and G are -0 and TI exactly when the row before it
+    // left finite W, G and TI is not zero (aux = 1 + 0*W = 1, G = TI - 0*G);
+    // such rows are not stored, the back substitution recomputes them.  From
+    // the first identity row where that does not hold (rows >= keep) every
+    // row is stored.
+    int keep = nrow;
+    for (int r0 = 0; r0 < nrow; r0 += LROWS) {
+        double4 q4[LROWS];
+        bool idr[LROWS];
 #pragma unroll
-            for (int u = 0; u < TPL; ++u) TI[u] = MGPU_NULL_REAL;
+        for (int q = 0; q < LROWS; ++q) {
+            idr[q] = false;
+            if (r0 + q < nrow) {
+                const long y = x0 + (long)(r0 + q) * ss;
+                const int r = lo + r0 + q;
+                const long xc = AX == 0 ? d.idx(a, r, k) : d.idx(r, a, k);
+                const uint8_t m = f.mask[xc];
+                idr[q] = m & M_HID;
+                if (idr[q])
+                    q4[q] = make_double4(0.0, 1.0, 0.0,
+                                         (last && (m & M_LAND)) ? MGPU_NULL_REAL : ts.P[y]);
+                else
+                    q4[q] = R[y];
+            }
         }
-        double W[TPL], G[TPL];
 #pragma unroll
-        for (int u = 0; u < TPL; ++u) {
-            if (r == lo) {
-                if (FAST) {
-                    const double ie = frcp(E[u]);
-                    W[u] = -F[u] * ie;
-                    G[u] = TI[u] * ie;
-                } else {
-                    W[u] = -F[u] / E[u];
-                    G[u] = TI[u] / E[u];
-                }
-            } else {
-                const double aux = E[u] + D[u] * Wm[u];
-                if (fabs(aux) > 0.0) {
+        for (int q = 0; q < LROWS; ++q) {
+            if (r0 + q < nrow) {
+                const double D = q4[q].x, E = q4[q].y, F = q4[q].z, TI = q4[q].w;
+                const bool plain = idr[q] && keep == nrow && isfinite(Wm) && isfinite(Gm) &&
+                                   TI != 0.0 && isfinite(TI);
+                if (idr[q] && !plain && keep == nrow) keep = r0 + q;
+                double W, G;
+                if (plain) {
+                    // -F/aux and (TI - D*G)/aux with D = F = 0, aux = 1
+                    W = -0.0;
+                    G = TI;
+                } else if (r0 + q == 0) {
                     if (FAST) {
-                        const double ia = frcp(aux);
-                        W[u] = -F[u] * ia;
-                        G[u] = (TI[u] - D[u] * Gm[u]) * ia;
+                        const double ie = frcp(E);
+                        W = -F * ie;
+                        G = TI * ie;
                     } else {
-                        W[u] = -F[u] / aux;
-                        G[u] = (TI[u] - D[u] * Gm[u]) / aux;
+                        W = -F / E;
+                        G = TI / E;
                     }
                 } else {
-                    W[u] = 0.0;
-                    G[u] = 0.0;
-                    pivot = true;
+                    const double aux = E + D * Wm;
+                    if (fabs(aux) > 0.0) {
+                        if (FAST) {
+                            const double ia = frcp(aux);
+                            W = -F * ia;
+                            G = (TI - D * Gm) * ia;
+                        } else {
+                            W = -F / aux;
+                            G = (TI - D * Gm) / aux;
+                        }
+                    } else {
+                        W = 0.0;
+                        G = 0.0;
+                        pivot = true;
+                    }
                 }
-            }
-            Wm[u] = W[u];
-            Gm[u] = G[u];
-        }
-        store_tpl<TPL>(ws + x * s, W);
-        store_tpl<TPL>(gs + x * s, G);
-        if (md) {
-            double cp[TPL];
-            if (k == d.wKLB) {
-                load_tpl<TPL>(tl.P + (x - d.sk) * s, cp);
-                store_tpl<TPL>(md + (x - d.sk) * s, cp);
-            }
-            if (k == d.wKUB) {
-                load_tpl<TPL>(tl.P + (x + d.sk) * s, cp);
-                store_tpl<TPL>(md + (x + d.sk) * s, cp);
+                Wm = W;
+                Gm = G;
+                const long y = x0 + (long)(r0 + q) * ss;
+                if (!plain) reinterpret_cast<double2 *>(R + y)[0] = make_double2(W, G);
+                if (edge) mid[y + ez] = ts.P[y + ez];   // rows KLB-1 / KUB+1 for the column walk
             }
         }
     }
     // ghost end row
-    const int rg = hi + 1;
-    const long xg = x0 + (long)(rg - lo) * st;
+    const long xg = AX == 0 ? d.idx(a, hi + 1, k) : d.idx(hi + 1, a, k);
     const bool landg = f.mask[xg] & M_LAND;
-    double xs[TPL];
-#pragma unroll
-    for (int u = 0; u < TPL; ++u) {
-        double TIg;
-        if (tig)
-            TIg = (last && landg) ? MGPU_NULL_REAL : tig[xg];
-        else
-            TIg = (landg && (last || (sc.optimize && t_base + t + u > 0))) ? MGPU_NULL_REAL : 0.0;
-        const double aux = 1.0 + 0.0 * Wm[u];
-        double G = 0.0;
-        if (fabs(aux) > 0.0) G = FAST ? (TIg - 0.0 * Gm[u]) * frcp(aux) : (TIg - 0.0 * Gm[u]) / aux;
+    double TIg;
+    if (tig)
+        TIg = (last && landg) ? MGPU_NULL_REAL : tig[xg];
+    else
+        TIg = (landg && (last || (sc.optimize && t_base + t > 0))) ? MGPU_NULL_REAL : 0.0;
+    double xs = 0.0;
+    {
+        const double aux = 1.0 + 0.0 * Wm;
+        if (fabs(aux) > 0.0) xs = FAST ? (TIg - 0.0 * Gm) * frcp(aux) : (TIg - 0.0 * Gm) / aux;
         else pivot = true;
-        xs[u] = G;
     }
-    line_put<TPL>(md, tl.out, xg, s, false, xs);
-    for (int r = hi; r >= lo; --r) {
-        const long x = x0 + (long)(r - lo) * st;
-        double W[TPL], G[TPL];
-        load_tpl<TPL>(ws + x * s, W);
-        load_tpl<TPL>(gs + x * s, G);
+    double *md = mid ? mid + t : nullptr;
+    double *out = ts.out + t;
+    line_put(md, out, xg * s, false, xs);
+    // back substitution, LROWS records loaded at a time.  A recomputed
+    // identity row below a finite value comes out as TI exactly; where TI is
+    // PROP, `out` already holds it (the caller keeps out equal to P on such
+    // cells) and `mid` is only needed where the column walk reads it
+    // (solved columns, from kstart-2 up).
+    const long top = (long)(d.wKUB - k) * d.sk;   // offset of a cell's column top (KUB)
+    for (int r1 = nrow - 1; r1 >= 0; r1 -= LROWS) {
+        double2 wg[LROWS];
+        uint8_t mq[LROWS];
 #pragma unroll
-        for (int u = 0; u < TPL; ++u) xs[u] = W[u] * xs[u] + G[u];
-        // water columns are overwritten by the column solve (MF:4086)
-        line_put<TPL>(md, tl.out, x, s, (f.mask[x + top] & M_WATER) != 0, xs);
+        for (int q = 0; q < LROWS; ++q) {
+            if (r1 - q >= 0) {
+                const int r = lo + r1 - q;
+                const long xc = AX == 0 ? d.idx(a, r, k) : d.idx(r, a, k);
+                const long y = x0 + (long)(r1 - q) * ss;
+                mq[q] = f.mask[xc];
+                if ((mq[q] & M_HID) && r1 - q < keep)
+                    wg[q] = make_double2(-0.0, (last && (mq[q] & M_LAND)) ? MGPU_NULL_REAL
+                                                                           : ts.P[y]);
+                else
+                    wg[q] = reinterpret_cast<const double2 *>(R + y)[0];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < LROWS; ++q) {
+            if (r1 - q >= 0) {
+                const int r = lo + r1 - q;
+                const long xc = AX == 0 ? d.idx(a, r, k) : d.idx(r, a, k);
+                const bool same = (mq[q] & M_HID) && r1 - q < keep && isfinite(xs) &&
+                                  !(last && (mq[q] & M_LAND));
+                xs = wg[q].x * xs + wg[q].y;
+                // water columns are overwritten by the column solve (MF:4086)
+                const bool solved = (f.mask[xc + top] & M_WATER) != 0;
+                if (!same) {
+                    line_put(md, out, xc * s, solved, xs);
+                } else if (md && solved) {
+                    const int ks = f.kstart[d.idx2(AX == 0 ? a : r, AX == 0 ? r : a)];
+                    if (k >= ks - 2) md[xc * s] = xs;
+                }
+            }
+        }
     }
     if (pivot) atomicAdd(pivots, 1ull);
-    if (bad) atomicOr(badflag, 1u);
 }
 
 // ---------------------------------------------------------------------------
-// Compact lists.  cells: every (i, j, k) the reference solves and gives an
-// explicit right-hand side -- rows kstart..KUB of the columns with
-// WaterPoints3D(i,j,KUB) == 1 -- in memory order; cols: those columns.
-// One block per row (j, k) of the WorkSize plane: counts, then (after an
-// exclusive scan of the counts) the Size-linear indices in order.
+// Compact lists.  cells (kind 0): every (i, j, k) the reference solves and
+// gives an explicit right-hand side -- rows kstart..KUB of the columns with
+// WaterPoints3D(i,j,KUB) == 1 -- in memory order; cols (kind 1): those
+// columns; hrows (kind 2): the WorkSize cells whose horizontal line-solve
+// row is not the identity (no M_HID).  One block per row (j, k) of the
+// WorkSize plane: counts, then (after an exclusive scan of the counts) the
+// Size-linear indices in order.
 __device__ __forceinline__ bool list_member(const Dims &d, const uint8_t *mask,
                                             const int32_t *kstart, int i, int j, int k,
-                                            bool columns) {
+                                            int kind) {
+    if (kind == 2) return !(mask[d.idx(i, j, k)] & M_HID);
     if (!(mask[d.idx(i, j, d.wKUB)] & M_WATER)) return false;
-    return columns || k >= kstart[d.idx2(i, j)];
+    return kind == 1 || k >= kstart[d.idx2(i, j)];
 }
 
 __global__ void __launch_bounds__(256)
 list_count_kernel(Dims d, const uint8_t *__restrict__ mask, const int32_t *__restrict__ kstart,
-                  int columns, uint32_t *__restrict__ rowcnt) {
+                  int kind, uint32_t *__restrict__ rowcnt) {
     __shared__ uint32_t part[4];
     const int j = d.wJLB + blockIdx.x, k = d.wKLB + blockIdx.y;
     uint32_t cnt = 0;
     for (int i = d.wILB + threadIdx.x; i <= d.wIUB; i += 256)
-        cnt += list_member(d, mask, kstart, i, j, k, columns) ? 1u : 0u;
+        cnt += list_member(d, mask, kstart, i, j, k, kind) ? 1u : 0u;
     for (int o = 32; o > 0; o >>= 1) cnt += __shfl_down(cnt, o, 64);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = cnt;
     __syncthreads();
@@ -1422,14 +1505,14 @@ list_scan_kernel(uint32_t *__restrict__ a, long n, unsigned long long *__restric
 
 __global__ void __launch_bounds__(256)
 list_write_kernel(Dims d, const uint8_t *__restrict__ mask, const int32_t *__restrict__ kstart,
-                  int columns, const uint32_t *__restrict__ rowoff, uint32_t *__restrict__ out) {
+                  int kind, const uint32_t *__restrict__ rowoff, uint32_t *__restrict__ out) {
     __shared__ uint32_t wsum[4];
     const int j = d.wJLB + blockIdx.x, k = d.wKLB + blockIdx.y;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t off = rowoff[blockIdx.y * (long)gridDim.x + blockIdx.x];
     for (int i0 = d.wILB; i0 <= d.wIUB; i0 += 256) {
         const int i = i0 + threadIdx.x;
-        const bool m = i <= d.wIUB && list_member(d, mask, kstart, i, j, k, columns);
+        const bool m = i <= d.wIUB && list_member(d, mask, kstart, i, j, k, kind);
         const unsigned long long b = __ballot(m);
         const uint32_t before = (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
         if (lane == 0) wsum[wave] = (uint32_t)__popcll(b);
@@ -1437,7 +1520,7 @@ list_write_kernel(Dims d, const uint8_t *__restrict__ mask, const int32_t *__res
         uint32_t woff = 0;
         for (int w = 0; w < wave; ++w) woff += wsum[w];
         const uint32_t chunk = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-        if (m) out[off + woff + before] = (uint32_t)(columns ? d.idx2(i, j) : d.idx(i, j, k));
+        if (m) out[off + woff + before] = (uint32_t)(kind == 1 ? d.idx2(i, j) : d.idx(i, j, k));
         __syncthreads();
         off += chunk;
     }
@@ -1509,38 +1592,47 @@ static int launch_column_lanes(int want, const Dims &d, const StepScalars &sc,
 
 template <bool OPT, int NT, int TPL, bool FAST>
 static int launch_line_tpl(int axis, const Dims &d, const StepScalars &sc, const SharedFields &f,
-                           const TracerSet &ts, double *mid, const double *tig, int t_base,
-                           double *Ws, double *Gs, unsigned long long *pivots,
-                           unsigned int *bad, hipStream_t s) {
+                           const TracerSet &ts, const ChunkImp &imp, double *Ws, double *Gs,
+                           unsigned long long *pivots, unsigned int *bad, hipStream_t s) {
+    (void)Ws; (void)Gs;
     const long na = axis == 0 ? d.wIUB - d.wILB + 1 : d.wJUB - d.wJLB + 1;
     const long nl = na * (d.wKUB - d.wKLB + 1);
-    const long nb = (nl * (NT / TPL) + 255) / 256;
-    if (nb <= 0) return MGPU_SUCCESS;
-    if (axis == 0)
-        hipLaunchKernelGGL((line_lane_kernel<OPT, NT, TPL, FAST, 0>), dim3((unsigned)nb),
-                           dim3(256), 0, s, d, sc, f, ts, mid, tig, t_base, Ws, Gs, pivots, bad);
-    else
-        hipLaunchKernelGGL((line_lane_kernel<OPT, NT, TPL, FAST, 1>), dim3((unsigned)nb),
-                           dim3(256), 0, s, d, sc, f, ts, mid, tig, t_base, Ws, Gs, pivots, bad);
+    const long nb1 = (imp.nhrows * (NT / TPL) + 255) / 256, nb2 = (nl * NT + 255) / 256;
+    if (nb2 <= 0) return MGPU_SUCCESS;
+    double *mid = sc.vert ? imp.mid : nullptr;
+    double4 *R = reinterpret_cast<double4 *>(imp.Rb);
+    if (axis == 0) {
+        if (nb1 > 0)
+            hipLaunchKernelGGL((line_rows_kernel<OPT, NT, TPL, FAST, 0>), dim3((unsigned)nb1),
+                               dim3(256), 0, s, d, sc, f, ts, imp.hrows, imp.nhrows, R, bad);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL((line_thomas_kernel<NT, FAST, 0>), dim3((unsigned)nb2), dim3(256), 0,
+                           s, d, sc, f, ts, R, mid, imp.tig, imp.t_base, pivots);
+    } else {
+        if (nb1 > 0)
+            hipLaunchKernelGGL((line_rows_kernel<OPT, NT, TPL, FAST, 1>), dim3((unsigned)nb1),
+                               dim3(256), 0, s, d, sc, f, ts, imp.hrows, imp.nhrows, R, bad);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL((line_thomas_kernel<NT, FAST, 1>), dim3((unsigned)nb2), dim3(256), 0,
+                           s, d, sc, f, ts, R, mid, imp.tig, imp.t_base, pivots);
+    }
     HIPCHK(hipGetLastError());
     return MGPU_SUCCESS;
 }
 
 template <bool OPT, int NT, bool FAST>
 static int launch_line_lanes(int want, int axis, const Dims &d, const StepScalars &sc,
-                             const SharedFields &f, const TracerSet &ts, double *mid,
-                             const double *tig, int t_base, double *Ws, double *Gs,
-                             unsigned long long *pivots, unsigned int *bad, hipStream_t s) {
-    // at most 4 tracers per lane: a line lane carries a whole row's faces
-    switch (lane_tpl<NT>(want > 4 ? 4 : want, 2)) {
-        case 4: if constexpr (NT % 4 == 0) return launch_line_tpl<OPT, NT, 4, FAST>(axis, d, sc, f, ts, mid, tig, t_base, Ws, Gs, pivots, bad, s); break;
-        case 2: if constexpr (NT % 2 == 0) return launch_line_tpl<OPT, NT, 2, FAST>(axis, d, sc, f, ts, mid, tig, t_base, Ws, Gs, pivots, bad, s); break;
+                             const SharedFields &f, const TracerSet &ts, const ChunkImp &imp,
+                             double *Ws, double *Gs, unsigned long long *pivots,
+                             unsigned int *bad, hipStream_t s) {
+    // at most 4 tracers per lane: a row lane carries a whole row's faces
+    switch (lane_tpl<NT>(want > 4 ? 4 : want, 4)) {
+        case 4: if constexpr (NT % 4 == 0) return launch_line_tpl<OPT, NT, 4, FAST>(axis, d, sc, f, ts, imp, Ws, Gs, pivots, bad, s); break;
+        case 2: if constexpr (NT % 2 == 0) return launch_line_tpl<OPT, NT, 2, FAST>(axis, d, sc, f, ts, imp, Ws, Gs, pivots, bad, s); break;
         default: break;
     }
-    return launch_line_tpl<OPT, NT, 1, FAST>(axis, d, sc, f, ts, mid, tig, t_base, Ws, Gs, pivots,
-                                             bad, s);
+    return launch_line_tpl<OPT, NT, 1, FAST>(axis, d, sc, f, ts, imp, Ws, Gs, pivots, bad, s);
 }
-
 
 // Launch the transport kernels for one chunk of NT tracers.
 // Fused variant: phase bit 1 launches the column-tile kernel.  Split
@@ -1557,9 +1649,9 @@ static int launch_chunk(int variant, int phase, const Dims &d, const StepScalars
         // implicit horizontal advection: line solves, then (3-D) the column
         // solve restarting from their result
         if (phase & 1) {
-            const int rc = launch_line_lanes<OPT, NT, FAST>(
-                lanes.column_tpl, sc.impH, d, sc, f, ts, sc.vert ? imp.mid : nullptr, imp.tig,
-                imp.t_base, Wb, Gb, counters + 3, bad, s);
+            const int rc = launch_line_lanes<OPT, NT, FAST>(lanes.explicit_tpl, sc.impH, d, sc,
+                                                            f, ts, imp, Wb, Gb, counters + 3, bad,
+                                                            s);
             if (rc) return rc;
         }
         if ((phase & 2) && sc.vert) {

@@ -231,7 +231,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": ("line_lane_kernel + column_lane_kernel" if args.implicit != "none"
+            "kernel": ("line_rows_kernel + line_thomas_kernel + column_lane_kernel"
+                       if args.implicit != "none"
                        else "explicit_lane_kernel + column_lane_kernel" if args.kernel == "lanes"
                        else "explicit_kernel + vertical_kernel") + " (all tracers of one step)",
             "kernel_ms": trans_ms,

@@ -100,8 +100,13 @@ typedef struct {
     /* present-and-set optional arguments the GPU path does not offload */
     int32_t unsupported_CellFluxes, unsupported_ReferenceProp;
     int32_t unsupported_BoundaryCondition; /* NullGradient_(4) / CyclicBoundary_(8) post-processing */
-    int32_t unsupported_SmallDepths, unsupported_NoAdvFlux, unsupported_NoDifFlux;
+    int32_t unsupported_NoAdvFlux;         /* its YY branch zeroes the XX coefficients, AD:4804-4812 */
     int32_t unsupported_Discharges;
+    /* offloaded optional arguments (AD:1287-1335); 0 / null when absent */
+    int32_t NoDifFlux;                     /* zero diffusion across the NoFlux faces */
+    const int32_t *NoFluxU, *NoFluxV, *NoFluxW;   /* 3-D, 1 = flagged face; needed with NoDifFlux */
+    const int32_t *SmallDepths;            /* 2-D (Size i,j), 1 = small-depth column: no
+                                              vertical diffusion (AD:2680-2699) */
     int32_t device_pointers;               /* 1: every array above is a device pointer */
 } mgpu_ad_args;
 

@@ -71,9 +71,10 @@ class ADArgs(ct.Structure):
         ("Visc_H", _dp), ("Diff_V", _dp), ("WaterPoints3D", _ip),
         ("Optimize", ct.c_int32), ("FirstProperty", ct.c_int32),
         ("unsupported_CellFluxes", ct.c_int32), ("unsupported_ReferenceProp", ct.c_int32),
-        ("unsupported_BoundaryCondition", ct.c_int32), ("unsupported_SmallDepths", ct.c_int32),
-        ("unsupported_NoAdvFlux", ct.c_int32), ("unsupported_NoDifFlux", ct.c_int32),
+        ("unsupported_BoundaryCondition", ct.c_int32), ("unsupported_NoAdvFlux", ct.c_int32),
         ("unsupported_Discharges", ct.c_int32),
+        ("NoDifFlux", ct.c_int32), ("NoFluxU", _ip), ("NoFluxV", _ip), ("NoFluxW", _ip),
+        ("SmallDepths", _ip),
         ("device_pointers", ct.c_int32),
     ]
 

@@ -88,6 +88,10 @@ def _args(hydro: Dict, params: Dict, keep: list, device: bool, optimize: bool,
         v = _addr(hydro[n], keep, np.int32)
         return ct.cast(v, ct.POINTER(ct.c_int32))
 
+    def opt_ip(n):
+        # optional arrays (NoFluxU/V/W, SmallDepths): null when absent
+        return ip(n) if hydro.get(n) is not None else ct.POINTER(ct.c_int32)()
+
     p = params
     return L.ADArgs(
         float(p["schmidt_H"]), float(p["SchmidtCoef_V"]), float(p["SchmidtBackground_V"]),
@@ -102,9 +106,10 @@ def _args(hydro: Dict, params: Dict, keep: list, device: bool, optimize: bool,
         ip("ComputeFacesW3D"), dp("Visc_H"), dp("Diff_V"), ip("WaterPoints3D"),
         int(bool(optimize)), int(bool(first_property)),
         int(bool(p.get("CellFluxes", False))), int(bool(p.get("ReferenceProp", False))),
-        int(p.get("BoundaryCondition", 1) in (4, 8)), int(bool(p.get("SmallDepths", False))),
-        int(bool(p.get("NoAdvFlux", False))), int(bool(p.get("NoDifFlux", False))),
-        int(bool(p.get("Discharges", False))), int(bool(device)))
+        int(p.get("BoundaryCondition", 1) in (4, 8)), int(bool(p.get("NoAdvFlux", False))),
+        int(bool(p.get("Discharges", False))), int(bool(p.get("NoDifFlux", False))),
+        opt_ip("NoFluxU"), opt_ip("NoFluxV"), opt_ip("NoFluxW"), opt_ip("SmallDepths"),
+        int(bool(device)))
 
 
 class AdvectionDiffusion:

@@ -105,6 +105,10 @@ struct DevHydro {
     double *visc = nullptr, *diffv = nullptr;
     int32_t *open = nullptr, *land = nullptr, *cfu = nullptr, *cfv = nullptr, *cfw = nullptr,
             *water = nullptr;
+    // optional inputs, allocated on first use; have_* tells whether this
+    // step's call passed them
+    int32_t *nfu = nullptr, *nfv = nullptr, *nfw = nullptr, *small = nullptr;
+    bool have_nf = false, have_small = false;
 
     int alloc(const Dims &d) {
         const long n3 = d.n3();
@@ -133,11 +137,32 @@ struct DevHydro {
         HIPCHK(hipMemcpyAsync(cfv, a->ComputeFacesV3D, bi, kind, s));
         HIPCHK(hipMemcpyAsync(cfw, a->ComputeFacesW3D, bi, kind, s));
         HIPCHK(hipMemcpyAsync(water, a->WaterPoints3D, bi, kind, s));
+        have_nf = a->NoDifFlux != 0;
+        if (have_nf) {
+            if (!nfu) {
+                HIPCHK(hipStreamSynchronize(s));
+                if (dalloc(&nfu, d.n3()) || dalloc(&nfv, d.n3()) || dalloc(&nfw, d.n3()))
+                    return MGPU_ERR_NOMEM;
+            }
+            HIPCHK(hipMemcpyAsync(nfu, a->NoFluxU, bi, kind, s));
+            HIPCHK(hipMemcpyAsync(nfv, a->NoFluxV, bi, kind, s));
+            HIPCHK(hipMemcpyAsync(nfw, a->NoFluxW, bi, kind, s));
+        }
+        have_small = a->SmallDepths != nullptr;
+        if (have_small) {
+            if (!small) {
+                HIPCHK(hipStreamSynchronize(s));
+                if (dalloc(&small, d.n2())) return MGPU_ERR_NOMEM;
+            }
+            HIPCHK(hipMemcpyAsync(small, a->SmallDepths, sizeof(int32_t) * (size_t)d.n2(), kind,
+                                  s));
+        }
         return MGPU_SUCCESS;
     }
     void release() {
         dfree(Qx); dfree(Qy); dfree(Qz); dfree(V); dfree(Vold); dfree(visc); dfree(diffv);
         dfree(open); dfree(land); dfree(cfu); dfree(cfv); dfree(cfw); dfree(water);
+        dfree(nfu); dfree(nfv); dfree(nfw); dfree(small);
     }
 };
 
@@ -189,8 +214,9 @@ struct DevDerived {
             HIPCHK(hipStreamSynchronize(s));
             if (dalloc(&hrows, nw)) return MGPU_ERR_NOMEM;
         }
-        int rc = launch_pack_masks(d, h.open, h.land, h.cfu, h.cfv, h.cfw, h.water, kfloor,
-                                   docycle, mask, kstart, mask_flags(), s);
+        int rc = launch_pack_masks(d, h.open, h.land, h.cfu, h.cfv, h.cfw, h.water,
+                                   h.have_small ? h.small : nullptr, kfloor, docycle, mask,
+                                   kstart, mask_flags(), s);
         // tile metadata only for the tile variants (level keys are bytes there)
         if (!rc && tiles) rc = launch_tile_sort(d, mask, kstart, cperm, ccnt, fperm, fcnt, s);
         if (!rc)
@@ -233,9 +259,10 @@ int check_options(const mgpu_ad_args *a) {
     if ((a->ImpExp_AdvXX != 0.0 && !impXX) || (a->ImpExp_AdvYY != 0.0 && !impYY))
         return MGPU_ERR_OPTION;
     if (a->AdvectionNudging || a->unsupported_CellFluxes || a->unsupported_ReferenceProp ||
-        a->unsupported_BoundaryCondition || a->unsupported_SmallDepths ||
-        a->unsupported_NoAdvFlux || a->unsupported_NoDifFlux || a->unsupported_Discharges)
+        a->unsupported_BoundaryCondition || a->unsupported_NoAdvFlux ||
+        a->unsupported_Discharges)
         return MGPU_ERR_UNSUPPORTED;
+    if (a->NoDifFlux && (!a->NoFluxU || !a->NoFluxV || !a->NoFluxW)) return MGPU_ERR_BADARG;
     if (!a->Wflux_X || !a->Wflux_Y || !a->Wflux_Z || !a->VolumeZOld || !a->VolumeZ ||
         !a->OpenPoints3D || !a->LandPoints3D || !a->ComputeFacesU3D || !a->ComputeFacesV3D ||
         !a->ComputeFacesW3D || !a->Visc_H || !a->Diff_V || !a->WaterPoints3D)
@@ -324,6 +351,7 @@ struct mgpu_ad {
     double *tig = nullptr, *mid = nullptr;
     double *Rb = nullptr;                   // implicit line row records {D, E, F, TI}
     bool have_faces = false;
+    double last_dt = 0.0, last_schmidtH = 0.0;   // Set_Internal_State's memory (AD:5746)
     unsigned long long last_zero_pivots = 0;
 };
 
@@ -415,18 +443,33 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     HIPCHK(hipMemsetAsync(ad->der.counters, 0, sizeof(unsigned long long) * 4, s));
     rc = ad->der.pack(d, ad->hydro, ad->geom.KFloor, ad->docycle, s, false, sc.impH >= 0);
     if (rc) return rc;
-    if (first) {
+    {
         // Convert_Visc_Dif_Horizontal(_opt) + Convert_Dif_Vertical (+ the
-        // Optimize path's constants, AD:1421-1478); NullDif only on the
-        // non-optimised path (the _opt conversion ignores it, AD:2598-2675)
+        // Optimize path's constants, AD:1421-1478); NullDif and NoDifFlux only
+        // on the non-optimised path (the _opt conversion and VerticalDiffusion2
+        // ignore them, AD:2598-2675, 2788-2937).  The optimised path makes
+        // them at FirstProperty.  The non-optimised path makes the W faces'
+        // on every call (NoFluxW applies in VerticalDiffusion, AD:2741) and
+        // keeps the U/V faces' while Set_Internal_State would (AD:5746-5835:
+        // same model time -- a later property of the step --, same DTProp
+        // and Schmidt number, no NullDif), NoDifFlux zeros included.
+        const bool hfresh = !ad->have_faces || a->FirstProperty || a->DTProp != ad->last_dt ||
+                            a->schmidt_H != ad->last_schmidtH || a->NullDif;
+        const int do_h = sc.optimize ? first : hfresh;
+        const int do_v = sc.optimize ? first : 1;
+        const bool nf = !sc.optimize && a->NoDifFlux;
         rc = launch_prep_faces(d, ad->hydro.visc, ad->hydro.diffv, ad->hydro.Qx, ad->hydro.Qy,
                                ad->hydro.Qz, ad->geom.DUX, ad->geom.DVY, ad->geom.DZX,
                                ad->geom.DZY, ad->geom.AreaU, ad->geom.AreaV, ad->geom.DZZ,
                                ad->der.mask, a->schmidt_H, a->SchmidtCoef_V,
                                a->SchmidtBackground_V, sc.optimize ? 0 : a->NullDif,
-                               ad->der.cu, ad->der.cv, ad->der.cw, s);
+                               nf ? ad->hydro.nfu : nullptr, nf ? ad->hydro.nfv : nullptr,
+                               nf ? ad->hydro.nfw : nullptr, do_h, do_v, ad->der.cu, ad->der.cv,
+                               ad->der.cw, s);
         if (rc) return rc;
-        ad->have_faces = true;
+        if (do_h) ad->have_faces = true;
+        ad->last_dt = a->DTProp;
+        ad->last_schmidtH = a->schmidt_H;
     }
     const SharedFields f = make_fields(ad->hydro, ad->der, ad->geom);
     // the step's own initialisation of TI (AD:1660-1671) reaches the ghost rows
@@ -863,11 +906,14 @@ static int enqueue_steps(mgpu_transport *t, int nsteps, int cur, int dir) {
         // first property of the step: face conductances (AD:1421-1478)
         int rc = mark(t);
         if (rc) return rc;
+        const bool nf = !t->sc.optimize && t->hydro.have_nf;
         rc = launch_prep_faces(d, t->hydro.visc, t->hydro.diffv, t->hydro.Qx, t->hydro.Qy,
                                    t->hydro.Qz, t->geom.DUX, t->geom.DVY, t->geom.DZX,
                                    t->geom.DZY, t->geom.AreaU, t->geom.AreaV, t->geom.DZZ,
                                    t->der.mask, t->schmidtH, t->schmidtV, t->backgroundV,
-                                   t->sc.optimize ? 0 : t->nulldif, t->der.cu, t->der.cv,
+                                   t->sc.optimize ? 0 : t->nulldif,
+                                   nf ? t->hydro.nfu : nullptr, nf ? t->hydro.nfv : nullptr,
+                                   nf ? t->hydro.nfw : nullptr, 1, 1, t->der.cu, t->der.cv,
                                    t->der.cw, s);
         if (rc) return rc;
         if ((rc = mark(t))) return rc;

@@ -24,6 +24,7 @@ enum : uint8_t {
     M_WATER = 32,  // WaterPoints3D
     M_HID = 64,    // not open and no U/V face on any side: an identity row of
                    // every horizontal line solve (D = F = 0, E = 1, TI = PROP)
+    M_SDV = 128,   // cell of a small-depth column: no vertical diffusion
 };
 
 // Extents of a Size-shaped Fortran array and its WorkSize.
@@ -87,22 +88,30 @@ struct SharedFields {     // device pointers shared by all tracers
 };
 
 // Kernel launchers (transport.hip)
+// small: the 2-D small-depth flags (M_SDV on every cell of a flagged
+// column) or null.
 int launch_pack_masks(const Dims &d, const int32_t *open, const int32_t *land,
                       const int32_t *cfu, const int32_t *cfv, const int32_t *cfw,
-                      const int32_t *water, const int32_t *kfloor, int docycle,
-                      uint8_t *mask, int32_t *kstart, unsigned int *flags, hipStream_t s);
+                      const int32_t *water, const int32_t *small, const int32_t *kfloor,
+                      int docycle, uint8_t *mask, int32_t *kstart, unsigned int *flags,
+                      hipStream_t s);
 // Tile ordering for ragged columns; tile_count() tiles, 256 columns and
 // 580 faces each, counts over the Size k range.
 long tile_count(const Dims &d);
 int launch_tile_sort(const Dims &d, const uint8_t *mask, const int32_t *kstart, uint8_t *cperm,
                      uint16_t *ccnt, uint16_t *fperm, uint16_t *fcnt, hipStream_t s);
+// Face conductances: U and V faces (cu, cv) when do_h, W faces (cw) when
+// do_v.  nfu/nfv/nfw (NoFluxU/V/W, or null) zero the diffusion across
+// flagged faces (NoDifFlux on the non-optimised path, AD:2496-2531,
+// 2741-2745).
 int launch_prep_faces(const Dims &d, const double *visc, const double *diffv,
                       const double *Qx, const double *Qy, const double *Qz,
                       const double *DUX, const double *DVY, const double *DZX,
                       const double *DZY, const double *AreaU, const double *AreaV,
                       const double *DZZ, const uint8_t *mask, double schmidtH,
                       double schmidtV, double backgroundV, int nulldif,
-                      double *cu, double *cv, double *cw, hipStream_t s);
+                      const int32_t *nfu, const int32_t *nfv, const int32_t *nfw, int do_h,
+                      int do_v, double *cu, double *cv, double *cw, hipStream_t s);
 // Transport of n tracers P -> out.  Tracer t of cell index x lives at
 // [x * stride + t] of P and out (stride 1: one tracer per array; stride n:
 // interleaved).  Tracers go in chunks of `chunk` (<= 8) per launch.  Both

@@ -10,14 +10,15 @@
 // launchers
 int launch_pack_masks(const Dims &d, const int32_t *open, const int32_t *land,
                       const int32_t *cfu, const int32_t *cfv, const int32_t *cfw,
-                      const int32_t *water, const int32_t *kfloor, int docycle, uint8_t *mask,
-                      int32_t *kstart, unsigned int *flags, hipStream_t s) {
+                      const int32_t *water, const int32_t *small, const int32_t *kfloor,
+                      int docycle, uint8_t *mask, int32_t *kstart, unsigned int *flags,
+                      hipStream_t s) {
     const long n = d.n3();
     const int threads = 256;
     long blocks = (n + threads - 1) / threads;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(pack_masks_kernel, dim3((unsigned)blocks), dim3(threads), 0, s, d, open,
-                       land, cfu, cfv, cfw, water, mask, n);
+                       land, cfu, cfv, cfw, water, small, mask, n);
     HIPCHK(hipGetLastError());
     const int ni = d.wIUB - d.wILB + 1, nj = d.wJUB - d.wJLB + 1;
     dim3 b(64, 4), g((ni + 63) / 64, (nj + 3) / 4);
@@ -47,12 +48,14 @@ int launch_prep_faces(const Dims &d, const double *visc, const double *diffv, co
                       const double *DZX, const double *DZY, const double *AreaU,
                       const double *AreaV, const double *DZZ, const uint8_t *mask,
                       double schmidtH, double schmidtV, double backgroundV, int nulldif,
-                      double *cu, double *cv, double *cw, hipStream_t s) {
+                      const int32_t *nfu, const int32_t *nfv, const int32_t *nfw, int do_h,
+                      int do_v, double *cu, double *cv, double *cw, hipStream_t s) {
     const int ni = d.wIUB - d.wILB + 1, nj = d.wJUB - d.wJLB + 1, nk = d.wKUB - d.wKLB + 1;
     dim3 b(64, 4), g((ni + 63) / 64, (nj + 3) / 4, nk);
+    if (!do_h && !do_v) return MGPU_SUCCESS;
     hipLaunchKernelGGL(prep_faces_kernel, g, b, 0, s, d, visc, diffv, Qx, Qy, Qz, DUX, DVY, DZX,
-                       DZY, AreaU, AreaV, DZZ, mask, schmidtH, schmidtV, backgroundV, nulldif, cu,
-                       cv, cw);
+                       DZY, AreaU, AreaV, DZZ, mask, schmidtH, schmidtV, backgroundV, nulldif,
+                       nfu, nfv, nfw, do_h, do_v, cu, cv, cw);
     HIPCHK(hipGetLastError());
     return MGPU_SUCCESS;
 }

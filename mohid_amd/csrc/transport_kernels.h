@@ -52,6 +52,7 @@ __global__ void pack_masks_kernel(Dims d, const int32_t *__restrict__ open,
                                   const int32_t *__restrict__ cfv,
                                   const int32_t *__restrict__ cfw,
                                   const int32_t *__restrict__ water,
+                                  const int32_t *__restrict__ small,
                                   uint8_t *__restrict__ mask, long n) {
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n;
          q += (long)gridDim.x * blockDim.x) {
@@ -72,6 +73,7 @@ __global__ void pack_masks_kernel(Dims d, const int32_t *__restrict__ open,
         const bool up_j = j + 1 <= d.wJUB && inws && cfu[q + d.sj] == 1;
         const bool up_i = i + 1 <= d.wIUB && inws && cfv[q + 1] == 1;
         if (!(m & (M_OPEN | M_CFU | M_CFV)) && !up_j && !up_i) m |= M_HID;
+        if (small && small[r] != 0) m |= M_SDV;   // r: the (i, j) index in Size
         mask[q] = m;
     }
 }
@@ -235,30 +237,36 @@ __global__ void prep_faces_kernel(Dims d, const double *__restrict__ visc,
                                   const double *__restrict__ AreaV,
                                   const double *__restrict__ DZZ,
                                   const uint8_t *__restrict__ mask, double sH, double sV,
-                                  double bV, int nulldif, double *__restrict__ cu,
-                                  double *__restrict__ cv, double *__restrict__ cw) {
+                                  double bV, int nulldif, const int32_t *__restrict__ nfu,
+                                  const int32_t *__restrict__ nfv,
+                                  const int32_t *__restrict__ nfw, int do_h, int do_v,
+                                  double *__restrict__ cu, double *__restrict__ cv,
+                                  double *__restrict__ cw) {
     const int i = d.wILB + blockIdx.x * blockDim.x + threadIdx.x;
     const int j = d.wJLB + blockIdx.y * blockDim.y + threadIdx.y;
     const int k = d.wKLB + blockIdx.z;
     if (i > d.wIUB || j > d.wJUB) return;
     const long x = d.idx(i, j, k);
     const uint8_t m = mask[x];
-    if (m & M_CFU) {
+    if (do_h && (m & M_CFU)) {
         const double dux0 = DUX[d.idx2(i, j)], dux1 = DUX[d.idx2(i, j - 1)];
         double kh = sH * (visc[x] * dux1 + visc[d.idx(i, j - 1, k)] * dux0) / (dux0 + dux1);
+        if (nfu && nfu[x] == 1) kh = 0.0;
         if (nulldif && Qx[x] == 0.0) kh = 0.0;
         cu[x] = kh * AreaU[x] / DZX[d.idx2(i, j - 1)];
     }
-    if (m & M_CFV) {
+    if (do_h && (m & M_CFV)) {
         const double dvy0 = DVY[d.idx2(i, j)], dvy1 = DVY[d.idx2(i - 1, j)];
         double kh = sH * (visc[x] * dvy1 + visc[d.idx(i - 1, j, k)] * dvy0) / (dvy0 + dvy1);
+        if (nfv && nfv[x] == 1) kh = 0.0;
         if (nulldif && Qy[x] == 0.0) kh = 0.0;
         cv[x] = kh * AreaV[x] / DZY[d.idx2(i - 1, j)];
     }
-    if ((m & M_CFW) && k > d.wKLB) {
+    if (do_v && (m & M_CFW) && k > d.wKLB) {
         double kz = sV * diffv[x] + bV;
         if (nulldif && Qz[x] == 0.0) kz = 0.0;
         cw[x] = kz * DUX[d.idx2(i, j)] * DVY[d.idx2(i, j)] / DZZ[d.idx(i, j, k - 1)];
+        if (nfw && nfw[x] == 1) cw[x] = 0.0;   // AuxK = 0 (AD:2741-2745)
     }
 }
 
@@ -658,12 +666,13 @@ __device__ __forceinline__ void column_row(const Dims &d, const StepScalars &sc,
     const double a2 = cfwB ? (FAST ? f.cw[x] * dtk : f.cw[x] * dt / vk) : 0.0;
     const double a1 = cfwT ? (FAST ? f.cw[x + sk] * dtk : f.cw[x + sk] * dt / vk) : 0.0;
     const bool land = mk & M_LAND;
+    const bool vdif = !(mk & M_SDV);   // SmallDepthCell (AD:2733, 2813-2907)
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const double pk = c.wp[t][1], pm = c.wp[t][0], pp = c.wp[t][2];
         double TI = TIk[t];
         double E = E0, D = 0.0, F = 0.0;
-        if (cfwB) {
+        if (cfwB && vdif) {
             if (OPT) {
                 if (sc.thV > 0.0) { D = D - a2; E = E + a2; }
                 else TI = TI - a2 * (pk - pm);
@@ -673,7 +682,7 @@ __device__ __forceinline__ void column_row(const Dims &d, const StepScalars &sc,
                 TI = TI - a2 * (pk - pm) * (1.0 - sc.thV);
             }
         }
-        if (cfwT) {
+        if (cfwT && vdif) {
             if (OPT) {
                 if (sc.thV > 0.0) { E = E + a1; F = F - a1; }
                 else TI = TI + a1 * (pp - pk);

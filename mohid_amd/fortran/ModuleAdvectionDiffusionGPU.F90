@@ -70,9 +70,11 @@ module ModuleAdvectionDiffusionGPU
         type(c_ptr)        :: WaterPoints3D
         integer(c_int32_t) :: Optimize, FirstProperty
         integer(c_int32_t) :: unsupported_CellFluxes, unsupported_ReferenceProp
-        integer(c_int32_t) :: unsupported_BoundaryCondition
-        integer(c_int32_t) :: unsupported_SmallDepths, unsupported_NoAdvFlux
-        integer(c_int32_t) :: unsupported_NoDifFlux, unsupported_Discharges
+        integer(c_int32_t) :: unsupported_BoundaryCondition, unsupported_NoAdvFlux
+        integer(c_int32_t) :: unsupported_Discharges
+        integer(c_int32_t) :: NoDifFlux
+        type(c_ptr)        :: NoFluxU, NoFluxV, NoFluxW
+        type(c_ptr)        :: SmallDepths
         integer(c_int32_t) :: device_pointers
     end type t_mgpu_ad_args
 
@@ -191,9 +193,12 @@ contains
     !> STAT = MGPU_ERR_UNSUPPORTED and leave PROP untouched:
     !>   ReferenceProp present (open boundary, AD:5816-5830), BoundaryCondition
     !>   NullGradient_ or CyclicBoundary_ (post-solve passes, AD:1874-1882),
-    !>   CellFluxes, any small-depth column (AD:1579-1590), NoAdvFlux or
-    !>   NoDifFlux true (AD:1321-1330), Discharges (SetDischarges was called,
-    !>   WP:14760-14772), AdvectionNudging.
+    !>   CellFluxes, NoAdvFlux true (AD:1321-1325), Discharges (SetDischarges
+    !>   was called, WP:14760-14772), AdvectionNudging.
+    !> SmallDepths and NoDifFlux (with NoFluxU/V/W) are offloaded.
+    !> FirstProperty is .true. on the first call of every step (WP:14600,
+    !> 14821-14826): the library keeps, like the reference, coefficients of
+    !> the previous call of the same step.
     subroutine AdvectionDiffusionGPU(ID, PROP, schmidt_H, SchmidtCoef_V, SchmidtBackground_V, &
                                      AdvMethodH, TVDLimitationH, AdvMethodV, TVDLimitationV,  &
                                      Upwind2H, Upwind2V, VolumeRelMax, AdvectionNudging,      &
@@ -204,6 +209,7 @@ contains
                                      ComputeFacesV3D, ComputeFacesW3D, Visc_H, Diff_V,        &
                                      WaterPoints3D, CellFluxes, ReferenceProp,                &
                                      BoundaryCondition, SmallDepths, NoAdvFlux, NoDifFlux,    &
+                                     NoFluxU, NoFluxV, NoFluxW,                               &
                                      Discharges, Optimize, FirstProperty, STAT)
         integer,                           intent(in)    :: ID
         real(8), dimension(:,:,:), contiguous, target, intent(inout) :: PROP
@@ -232,10 +238,13 @@ contains
         integer, optional,                 intent(in)    :: BoundaryCondition
         logical, dimension(:,:), optional, intent(in)    :: SmallDepths
         logical, optional,                 intent(in)    :: NoAdvFlux, NoDifFlux
+        integer, dimension(:,:,:), optional, contiguous, target, intent(in) :: NoFluxU, NoFluxV
+        integer, dimension(:,:,:), optional, contiguous, target, intent(in) :: NoFluxW
         logical, optional,                 intent(in)    :: Optimize, FirstProperty
         integer, optional,                 intent(out)   :: STAT
         type(t_mgpu_ad_args) :: a
         integer              :: rc
+        integer, allocatable, target :: small(:,:)
 
         if (ID < 1 .or. ID > MAX_INSTANCES) then
             if (present(STAT)) STAT = MGPU_OFF_ERR
@@ -274,12 +283,20 @@ contains
             if (BoundaryCondition == NullGradient_ .or. BoundaryCondition == CyclicBoundary_) &
                 a%unsupported_BoundaryCondition = 1
         endif
-        a%unsupported_SmallDepths = 0
-        if (present(SmallDepths)) a%unsupported_SmallDepths = b2i(any(SmallDepths))
         a%unsupported_NoAdvFlux = 0
         if (present(NoAdvFlux)) a%unsupported_NoAdvFlux = b2i(NoAdvFlux)
-        a%unsupported_NoDifFlux = 0
-        if (present(NoDifFlux)) a%unsupported_NoDifFlux = b2i(NoDifFlux)
+        a%NoDifFlux = 0
+        if (present(NoDifFlux)) a%NoDifFlux = b2i(NoDifFlux)
+        a%NoFluxU = c_null_ptr; a%NoFluxV = c_null_ptr; a%NoFluxW = c_null_ptr
+        if (present(NoFluxU)) a%NoFluxU = c_loc(NoFluxU)
+        if (present(NoFluxV)) a%NoFluxV = c_loc(NoFluxV)
+        if (present(NoFluxW)) a%NoFluxW = c_loc(NoFluxW)
+        a%SmallDepths = c_null_ptr
+        if (present(SmallDepths)) then
+            allocate(small(size(SmallDepths, 1), size(SmallDepths, 2)))
+            small = merge(1, 0, SmallDepths)
+            a%SmallDepths = c_loc(small)
+        endif
         a%unsupported_Discharges = 0
         if (present(Discharges)) a%unsupported_Discharges = b2i(Discharges)
         a%device_pointers = 0

@@ -564,8 +564,10 @@ static void convert_dif_vertical(ora_instance *me, const ora_args *a)
 }
 
 /* AD:2453-2595 Convert_Visc_Dif_Horizontal (and the _opt variant,
- * AD:2598-2675, which is the same without NoDifFlux/NullDif). */
-static void convert_visc_dif_horizontal(ora_instance *me, const ora_args *a, int nulldif)
+ * AD:2598-2675, which is the same without NoDifFlux/NullDif: nodif and
+ * nulldif are 0 for it). */
+static void convert_visc_dif_horizontal(ora_instance *me, const ora_args *a, int nulldif,
+                                        int nodif)
 {
     const ora_size3d *w = &me->ws;
     const double *DUX = me->g.DUX, *DVY = me->g.DVY, *nu = a->Visc_H;
@@ -582,6 +584,7 @@ static void convert_visc_dif_horizontal(ora_instance *me, const ora_args *a, int
                                  (nu[x] * DUX[I2(me, i, j - 1)] +
                                   nu[I3(me, i, j - 1, k)] * DUX[I2(me, i, j)]) /
                                  (DUX[I2(me, i, j)] + DUX[I2(me, i, j - 1)]);
+                    if (nodif && a->NoFluxU[x] == 1) me->KhX[x] = 0.0;   /* AD:2496-2500 */
                 }
             }
         }
@@ -598,6 +601,8 @@ static void convert_visc_dif_horizontal(ora_instance *me, const ora_args *a, int
                                  (nu[x] * DVY[I2(me, i - 1, j)] +
                                   nu[I3(me, i - 1, j, k)] * DVY[I2(me, i, j)]) /
                                  (DVY[I2(me, i, j)] + DVY[I2(me, i - 1, j)]);
+                    if (nodif && g_mistake != 12 && a->NoFluxV[x] == 1)   /* AD:2527-2531 */
+                        me->KhY[x] = 0.0;
                 }
             }
         }
@@ -649,7 +654,14 @@ static void compute_difh_constants(ora_instance *me, const ora_args *a)
     }
 }
 
-/* AD:1563-1619 Compute_DifV_Constants (SmallDepths absent) */
+static int small_depth(const ora_instance *me, const ora_args *a, int i, int j)
+{
+    /* AD:2680-2699 SmallDepthCell */
+    return a->SmallDepths ? a->SmallDepths[I2(me, i, j)] != 0 : 0;
+}
+
+/* AD:1563-1619 Compute_DifV_Constants: small-depth columns keep their
+ * previous constants (AD:1579-1593) */
 static void compute_difv_constants(ora_instance *me, const ora_args *a)
 {
     const ora_size3d *w = &me->ws;
@@ -660,7 +672,7 @@ static void compute_difv_constants(ora_instance *me, const ora_args *a)
         for (j = w->JLB; j <= w->JUB; j++)
             for (i = w->ILB; i <= w->IUB; i++) {
                 long x = I3(me, i, j, k);
-                if (a->ComputeFacesW3D[x] == 1) {
+                if (a->ComputeFacesW3D[x] == 1 && !small_depth(me, a, i, j)) {
                     double auxk = me->KzZ[x] * me->g.DUX[I2(me, i, j)] * me->g.DVY[I2(me, i, j)];
                     me->DVC[x] = auxk / me->g.DZZ[I3(me, i, j, g_mistake == 7 ? k : k - 1)];
                 }
@@ -913,9 +925,10 @@ static void vertical_diffusion(ora_instance *me, const ora_args *a, const double
             for (i = w->ILB; i <= w->IUB; i++) {
                 long x = I3(me, i, j, k), xm = I3(me, i, j, k - 1);
                 double auxk, aux1, aux2;
-                if (a->ComputeFacesW3D[x] != 1) continue;
+                if (a->ComputeFacesW3D[x] != 1 || small_depth(me, a, i, j)) continue;
                 auxk = me->KzZ[x] * me->g.DUX[I2(me, i, j)] * me->g.DVY[I2(me, i, j)] /
                        me->g.DZZ[g_mistake == 6 ? x : xm];
+                if (a->NoDifFlux && a->NoFluxW[x] == 1) auxk = 0.0;   /* AD:2741-2745 */
                 aux1 = auxk * dt / a->VolumeZ[xm];
                 aux2 = auxk * dt / a->VolumeZ[x];
                 me->E[xm] = me->E[xm] + aux1 * th;
@@ -944,6 +957,7 @@ static void vertical_diffusion2(ora_instance *me, const ora_args *a, const doubl
                 int kb;
                 double vb, pb = 0.0;
                 if (a->ComputeFacesW3D[I3(me, i, j, w->KUB)] != 1) continue;
+                if (g_mistake != 11 && small_depth(me, a, i, j)) continue;
                 kb = me->g.KFloorZ[I2(me, i, j)];
                 vb = a->VolumeZ[I3(me, i, j, kb)];
                 if (!implicit) pb = P[I3(me, i, j, kb)];
@@ -978,6 +992,7 @@ static void vertical_diffusion2(ora_instance *me, const ora_args *a, const doubl
                     long x = I3(me, i, j, k), xm = I3(me, i, j, k - 1);
                     double auxk, aux1, aux2;
                     if (a->ComputeFacesW3D[x] != 1) continue;
+                    if (g_mistake != 11 && small_depth(me, a, i, j)) continue;
                     auxk = me->DVC[x] * dt;
                     aux1 = auxk / a->VolumeZ[xm];
                     aux2 = auxk / a->VolumeZ[x];
@@ -1284,11 +1299,12 @@ int ora_advection_diffusion(ora_instance *me, const ora_args *a, double *prop)
     if (me->st_vdif) {
         if (!optimize || a->FirstProperty) convert_dif_vertical(me, &aa);
     }
+    if (a->NoDifFlux && (!a->NoFluxU || !a->NoFluxV || !a->NoFluxW)) return ORA_ERR_BADARG;
     if (me->st_hdif) {
         if (optimize) {
-            if (a->FirstProperty) convert_visc_dif_horizontal(me, &aa, 0);
+            if (a->FirstProperty) convert_visc_dif_horizontal(me, &aa, 0, 0);
         } else {
-            convert_visc_dif_horizontal(me, &aa, a->NullDif);
+            convert_visc_dif_horizontal(me, &aa, a->NullDif, a->NoDifFlux);
         }
     }
     if (me->st_vadv) {

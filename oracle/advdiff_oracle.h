@@ -72,6 +72,10 @@ typedef struct {
     const double *Visc_H, *Diff_V;
     int32_t Optimize, FirstProperty;
     double Now;                    /* model time of this call (ModuleTime) */
+    /* optional arguments (AD:1287-1335); NULL / 0 when absent */
+    int32_t NoDifFlux;
+    const int32_t *NoFluxU, *NoFluxV, *NoFluxW;   /* 3-D, read with NoDifFlux */
+    const int32_t *SmallDepths;                   /* 2-D (Size i, j), logical as 0/1 */
 } ora_args;
 
 typedef struct ora_instance ora_instance;
@@ -99,7 +103,9 @@ const double *ora_internal(const ora_instance *me, const char *name);
  *   8 implicit XX advection: F row of cell j-1 built from the face's D flux
  *   9 implicit XX advection: cell j-1 rows scaled by the volume of cell j
  *  10 implicit horizontal advection in 3-D: rows not reset to the identity
- *     before the vertical step */
+ *     before the vertical step
+ *  11 small-depth columns keep their vertical diffusion in VerticalDiffusion2
+ *  12 NoDifFlux ignores the V faces */
 void ora_plant_mistake(int id);
 
 /* THOMASZ_NewType2 (ModuleFunctions.F90:4026-4123) on explicit arrays.

@@ -52,6 +52,8 @@ class Args(ct.Structure):
         ("ComputeFacesU3D", _ip), ("ComputeFacesV3D", _ip), ("ComputeFacesW3D", _ip),
         ("WaterPoints3D", _ip), ("Visc_H", _dp), ("Diff_V", _dp),
         ("Optimize", ct.c_int32), ("FirstProperty", ct.c_int32), ("Now", ct.c_double),
+        ("NoDifFlux", ct.c_int32), ("NoFluxU", _ip), ("NoFluxV", _ip), ("NoFluxW", _ip),
+        ("SmallDepths", _ip),
     ]
 
 
@@ -172,6 +174,9 @@ class Instance:
             keep.append(a)
             return _ptr(a, _ip)
 
+        def opt_i(n):
+            return i(n) if hy.get(n) is not None else None
+
         args = Args(p["schmidt_H"], p["SchmidtCoef_V"], p["SchmidtBackground_V"],
                     p["AdvMethodH"], p["TVDLimitationH"], p["AdvMethodV"], p["TVDLimitationV"],
                     int(p["Upwind2H"]), int(p["Upwind2V"]), p["VolumeRelMax"], p["DTProp"],
@@ -181,7 +186,8 @@ class Instance:
                     i("OpenPoints3D"), i("LandPoints3D"), i("ComputeFacesU3D"),
                     i("ComputeFacesV3D"), i("ComputeFacesW3D"), i("WaterPoints3D"),
                     d("Visc_H"), d("Diff_V"), int(p["Optimize"]), int(first_property),
-                    self.now if now is None else now)
+                    self.now if now is None else now, int(bool(p.get("NoDifFlux", False))),
+                    opt_i("NoFluxU"), opt_i("NoFluxV"), opt_i("NoFluxW"), opt_i("SmallDepths"))
         assert prop.flags.f_contiguous and prop.dtype == np.float64
         return lib().ora_advection_diffusion(self.h, ct.byref(args), _ptr(prop, _dp))
 

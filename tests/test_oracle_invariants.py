@@ -16,6 +16,11 @@ mathematics fixes:
 * invariants of a closed basin with divergence-free transports and
   VolumeZOld = VolumeZ (every fixture): sum(V * C) over water cells is
   conserved and a constant field stays constant, on every small case;
+* NoDifFlux faces and small-depth columns: every face flagged is, bit for
+  bit, zero diffusivity; a no-flux wall keeps a field that is constant on
+  each side constant (1e-13); small-depth columns keep their values under
+  vertical diffusion alone, the others match the step without small depths,
+  and small depths everywhere are zero vertical diffusivity;
 * implicit horizontal advection (the THOMAS_3D path, AD:1758-1843,
   4167-4256, 4490-4525, 4858-4885): dense line solves of implicit upwind
   transport at a Courant number of 3, the 3-D
@@ -347,6 +352,102 @@ def test_implicit_fractional_step_dense_solve(axis):
 
 
 # ---------------------------------------------------------------------------
+# NoDifFlux faces and small-depth columns (AD:1287-1335, 2496-2531, 2680-2745,
+# 2813-2907)
+def _nodif_all_err():
+    """NoDifFlux with every face flagged gives, bit for bit, the step with
+    zero diffusivities (the coefficients are zeroed where they are made)."""
+    case = make("tvd_land_sigma")
+    params = dict(case.params, Optimize=False, SchmidtBackground_V=0.0)
+    a = np.array(case.tracers[0], order="F", copy=True)
+    hy = {k: np.array(v, order="F", copy=True) for k, v in case.hydro.items()}
+    ones = np.ones(a.shape, dtype=np.int32, order="F")
+    hy.update(NoFluxU=ones, NoFluxV=ones, NoFluxW=ones)
+    _run(case, a, hydro=hy, NoDifFlux=True, **params)
+    b = np.array(case.tracers[0], order="F", copy=True)
+    hz = {k: np.array(v, order="F", copy=True) for k, v in case.hydro.items()}
+    hz["Visc_H"][...] = 0.0
+    hz["Diff_V"][...] = 0.0
+    _run(case, b, hydro=hz, **params)
+    return float(np.max(np.abs(a - b)))
+
+
+def _nodif_wall_err(axis):
+    """Diffusion only, a NoDifFlux wall across the domain (U faces at one j,
+    V faces at one i, or W faces at one level): a field that is constant on
+    each side of the wall stays constant on both sides (to the rounding of
+    the column solve)."""
+    case = fx.make_case(fx.config("c2", ni=9, nj=10, nk=6))
+    hy = _still_hydro(case)
+    hy["Visc_H"][...] = 80.0
+    hy["Diff_V"][...] = case.hydro["Diff_V"]
+    wp = hy["WaterPoints3D"] == 1
+    nf = {n: np.zeros(wp.shape, dtype=np.int32, order="F") for n in ("NoFluxU", "NoFluxV",
+                                                                     "NoFluxW")}
+    c0 = np.zeros(wp.shape)
+    if axis == "x":
+        nf["NoFluxU"][:, 5, :] = 1
+        c0[:, 5:, :] = 4.0
+    elif axis == "y":
+        nf["NoFluxV"][5, :, :] = 1
+        c0[5:, :, :] = 4.0
+    else:
+        nf["NoFluxW"][:, :, 4] = 1
+        c0[:, :, 4:] = 4.0
+    c0 = np.asfortranarray(np.where(wp, c0 + 1.0, 0.0))
+    hy.update(nf)
+    params = dict(UPWIND, ImpExp_DifV=1.0, SchmidtCoef_V=1.0, SchmidtBackground_V=0.0,
+                  schmidt_H=1.0)
+    got = _run(case, c0.copy(order="F"), hydro=hy, NoDifFlux=True, **params)
+    return float(np.max(np.abs(got[wp] - c0[wp]) / c0[wp]))
+
+
+def _small_depth_err(optimize, docycle):
+    """Vertical diffusion only (still water), small depths on a checkerboard
+    of columns: flagged columns keep their values exactly, the others equal
+    the step run without small depths; and small depths on every column
+    give the step with zero vertical diffusivity."""
+    case = fx.make_case(fx.config("c4", ni=6, nj=5, nk=7, depth=(5.0, 40.0)))
+    hy = _still_hydro(case)
+    hy["Diff_V"][...] = case.hydro["Diff_V"]
+    wp = hy["WaterPoints3D"] == 1
+    rng = np.random.default_rng(13)
+    c0 = np.asfortranarray(np.where(wp, rng.uniform(1.0, 9.0, size=wp.shape), 0.0))
+    params = dict(TVD_OPT if optimize else UPWIND, ImpExp_DifV=1.0, SchmidtCoef_V=1.0,
+                  SchmidtBackground_V=0.0)
+    ii, jj = np.meshgrid(np.arange(wp.shape[0]), np.arange(wp.shape[1]), indexing="ij")
+    small = np.asfortranarray(((ii + jj) % 2 == 0).astype(np.int32))
+    got = _run(case, c0.copy(order="F"), hydro=dict(hy, SmallDepths=small), docycle=docycle,
+               **params)
+    free = _run(case, c0.copy(order="F"), hydro=hy, docycle=docycle, **params)
+    flagged = wp & (small[:, :, None] == 1)
+    err = float(np.max(np.abs(got[flagged] - c0[flagged]))) if flagged.any() else 0.0
+    err = max(err, float(np.max(np.abs(got[wp & ~flagged] - free[wp & ~flagged]))))
+    allsmall = np.ones_like(small)
+    a = _run(case, c0.copy(order="F"), hydro=dict(hy, SmallDepths=allsmall), docycle=docycle,
+             **params)
+    hz = dict(hy)
+    hz["Diff_V"] = np.zeros_like(hy["Diff_V"])
+    b = _run(case, c0.copy(order="F"), hydro=hz, docycle=docycle, **params)
+    return max(err, float(np.max(np.abs(a - b))))
+
+
+def test_nodifflux_everywhere_is_zero_diffusivity():
+    assert _nodif_all_err() == 0.0
+
+
+@pytest.mark.parametrize("axis", ["x", "y", "z"])
+def test_nodifflux_wall_keeps_both_sides(axis):
+    # the column solve of a constant keeps it to rounding (a few ulp)
+    assert _nodif_wall_err(axis) <= 1e-13
+
+
+@pytest.mark.parametrize("optimize,docycle", [(False, 1), (True, 1), (True, 2)])
+def test_small_depths_switch_off_vertical_diffusion(optimize, docycle):
+    assert _small_depth_err(optimize, docycle) == 0.0
+
+
+# ---------------------------------------------------------------------------
 # closed-basin invariants on every small case
 def _invariant_errs(name, steps=3, implicit=None):
     case = make(name)
@@ -411,6 +512,11 @@ def _all_checks():
                 out.append((f"implicit channel {axis} {sign} nk={nk}",
                             _implicit_channel_err(axis, sign, nk), 1e-12))
         out.append((f"implicit fractional {axis}", _implicit_fractional_err(axis), 1e-12))
+    out.append(("nodif everywhere", _nodif_all_err(), 0.0))
+    for axis in ("x", "y", "z"):
+        out.append((f"nodif wall {axis}", _nodif_wall_err(axis), 1e-13))
+    for opt, dc in ((False, 1), (True, 1), (True, 2)):
+        out.append((f"small depths opt={opt} docycle={dc}", _small_depth_err(opt, dc), 0.0))
     for name in sorted(SMALL):
         mass, constancy = _invariant_errs(name, steps=2)
         out.append((f"mass {name}", mass, 1e-13))
@@ -421,7 +527,7 @@ def _all_checks():
     return out
 
 
-@pytest.mark.parametrize("mistake", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
+@pytest.mark.parametrize("mistake", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12])
 def test_planted_mistakes_are_caught(mistake):
     L = _lib()
     L.ora_plant_mistake(mistake)

@@ -46,6 +46,15 @@
 #define PDM 5
 #define EXPLICIT 0.0
 #define IMPLICIT 1.0
+#define DISCH_UNIFORM 5          /* GD:1006 DischUniform_ */
+#define FILL_INT (-9999999)      /* GD:131 FillValueInt */
+
+/* SetDischarges arrays (AD:270-284) */
+typedef struct {
+    int on, number;
+    const double *flow, *conc, *concMF;
+    const int32_t *I, *J, *K, *Kmin, *Kmax, *vert, *ignore, *ncells, *bypass;
+} ora_disch;
 
 struct ora_instance {
     ora_size3d size, ws;
@@ -68,6 +77,7 @@ struct ora_instance {
     int ext_AdvMethodH, ext_TVDH, ext_AdvMethodV, ext_TVDV;
     double ext_SchmidtCoef_V, ext_SchmidtBackground_V, ext_Schmidt_H;
     int first_call;
+    ora_disch disch;
 };
 
 #define I3(me, i, j, k) (((long)(i) - (me)->size.ILB) + (me)->nI * (((long)(j) - (me)->size.JLB) + (me)->nJ * ((long)(k) - (me)->size.KLB)))
@@ -709,6 +719,78 @@ static void volume_variation(ora_instance *me, const ora_args *a, const double *
     }
 }
 
+/* AD:4025-4128 Discharges */
+static void discharges(ora_instance *me, const ora_args *a, const double *P)
+{
+    const ora_disch *q = &me->disch;
+    int n = 0, dis;
+    for (dis = 0; dis < q->number; dis++) {
+        int nc;
+        if (q->ignore[dis]) continue;
+        for (nc = 0; nc < q->ncells[dis]; nc++) {
+            int i, j, kd, kmin, kmax, k;
+            double wc = 0.0;
+            n++;
+            i = q->I[n - 1]; j = q->J[n - 1]; kd = q->K[n - 1];
+            kmin = q->Kmin[n - 1]; kmax = q->Kmax[n - 1];
+            if (q->vert[dis] == DISCH_UNIFORM) {
+                if (kmin == FILL_INT) kmin = me->g.KFloorZ[I2(me, i, j)];
+                if (kmax == FILL_INT) kmax = me->ws.KUB;
+            } else {
+                kmin = kd;
+                kmax = kd;
+            }
+            for (k = (g_mistake == 13 ? me->g.KFloorZ[I2(me, i, j)] : kmin); k <= kmax; k++)
+                wc = wc + me->g.DWZ[I3(me, i, j, k)];
+            for (k = kmin; k <= kmax; k++) {
+                long x = I3(me, i, j, k);
+                double dtv = a->DTProp / a->VolumeZ[x];
+                double flow = q->vert[dis] == DISCH_UNIFORM
+                                  ? q->flow[n - 1] * me->g.DWZ[x] / wc : q->flow[n - 1];
+                if (a->OpenPoints3D[x] == 1) {
+                    if (q->bypass[dis]) {
+                        me->TI[x] = me->TI[x] + flow * dtv * q->conc[n - 1];
+                    } else if (flow > 0.0) {
+                        me->TI[x] = me->TI[x] + flow * dtv * q->conc[n - 1];
+                    } else {
+                        me->E[x] = me->E[x] - flow * dtv * q->concMF[n - 1];
+                    }
+                } else if (flow > 0.0) {
+                    /* a water point that is not open: no dilution by VolumeVariation */
+                    me->TI[x] = P[x] * a->VolumeZOld[x] / a->VolumeZ[x] +
+                                flow * dtv * q->conc[n - 1];
+                }
+            }
+        }
+    }
+}
+
+int ora_set_discharges(ora_instance *me, int number, const double *flow, const double *conc,
+                       const double *concMF, const int32_t *I, const int32_t *J,
+                       const int32_t *K, const int32_t *Kmin, const int32_t *Kmax,
+                       const int32_t *vert, const int32_t *ignore, const int32_t *ncells,
+                       const int32_t *bypass)
+{
+    ora_disch *q;
+    if (!me) return ORA_OFF_ERR;
+    if (number < 0 || (number > 0 && (!flow || !conc || !concMF || !I || !J || !K || !Kmin ||
+                                      !Kmax || !vert || !ignore || !ncells || !bypass)))
+        return ORA_ERR_BADARG;
+    q = &me->disch;
+    q->on = 1; q->number = number;
+    q->flow = flow; q->conc = conc; q->concMF = concMF;
+    q->I = I; q->J = J; q->K = K; q->Kmin = Kmin; q->Kmax = Kmax;
+    q->vert = vert; q->ignore = ignore; q->ncells = ncells; q->bypass = bypass;
+    return ORA_SUCCESS;
+}
+
+int ora_unset_discharges(ora_instance *me)
+{
+    if (!me) return ORA_OFF_ERR;
+    memset(&me->disch, 0, sizeof(me->disch));
+    return ORA_SUCCESS;
+}
+
 /* AD:5156-5214 HorizontalDiffusionXX / AD:5216-5265 HorizontalDiffusionXX2 */
 static void horizontal_diffusion_xx(ora_instance *me, const ora_args *a, const double *P)
 {
@@ -1188,6 +1270,7 @@ static int iteration(ora_instance *me, const ora_args *a, double *P)
     }
 
     volume_variation(me, a, P);
+    if (me->disch.on) discharges(me, a, P);      /* AD:1702 */
     if (!me->vertical1d) {
         horizontal_diffusion_xx(me, a, P);
         if (!me->xzflow) horizontal_diffusion_yy(me, a, P);

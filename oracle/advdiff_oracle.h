@@ -88,6 +88,20 @@ int ora_start(const ora_size3d *size, const ora_size3d *worksize,
 int ora_advection_diffusion(ora_instance *me, const ora_args *args, double *prop);
 /* KillAdvectionDiffusion (ModuleAdvectionDiffusion.F90:5849). */
 void ora_kill(ora_instance *me);
+/* SetDischarges (ModuleAdvectionDiffusion.F90:977-1030): the discharges of
+ * the following calls, until ora_unset_discharges (AD:1034-1080).  The
+ * arrays are kept by reference, as the reference keeps its pointers, and use
+ * the reference's 1-based numbering shifted to 0: per discharge cell n (the
+ * cells of the discharges that are not ignored, in order) flow, conc,
+ * concMF, I, J, K, Kmin, Kmax; per discharge vert (DischUniform_ = 5 spreads
+ * the flow over Kmin..Kmax), ignore, ncells, bypass. */
+int ora_set_discharges(ora_instance *me, int number, const double *flow, const double *conc,
+                       const double *concMF, const int32_t *I, const int32_t *J,
+                       const int32_t *K, const int32_t *Kmin, const int32_t *Kmax,
+                       const int32_t *vert, const int32_t *ignore, const int32_t *ncells,
+                       const int32_t *bypass);
+int ora_unset_discharges(ora_instance *me);
+
 /* Read back an internal array after a call (names: "D","E","F","TI"). */
 const double *ora_internal(const ora_instance *me, const char *name);
 
@@ -105,7 +119,9 @@ const double *ora_internal(const ora_instance *me, const char *name);
  *  10 implicit horizontal advection in 3-D: rows not reset to the identity
  *     before the vertical step
  *  11 small-depth columns keep their vertical diffusion in VerticalDiffusion2
- *  12 NoDifFlux ignores the V faces */
+ *  12 NoDifFlux ignores the V faces
+ *  13 a uniform discharge spread by the cell's thickness over the column's
+ *     total depth instead of the discharge cells' */
 void ora_plant_mistake(int id);
 
 /* THOMASZ_NewType2 (ModuleFunctions.F90:4026-4123) on explicit arrays.

@@ -81,6 +81,11 @@ def lib():
         L.ora_internal.restype = _dp
         L.ora_thomasz.argtypes = [ct.POINTER(Size3D), ct.POINTER(Size3D), _dp, _dp, _dp, _dp,
                                   _ip, _dp, _dp, _dp]
+        L.ora_set_discharges.argtypes = [ct.c_void_p, ct.c_int, _dp, _dp, _dp, _ip, _ip, _ip,
+                                         _ip, _ip, _ip, _ip, _ip, _ip]
+        L.ora_set_discharges.restype = ct.c_int
+        L.ora_unset_discharges.argtypes = [ct.c_void_p]
+        L.ora_unset_discharges.restype = ct.c_int
         L.ora_thomas3d.argtypes = [ct.POINTER(Size3D), ct.c_int, ct.c_int, ct.c_int, ct.c_int,
                                    ct.c_int, ct.c_int, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp,
                                    _dp, _dp, _dp, _ip]
@@ -190,6 +195,24 @@ class Instance:
                     opt_i("NoFluxU"), opt_i("NoFluxV"), opt_i("NoFluxW"), opt_i("SmallDepths"))
         assert prop.flags.f_contiguous and prop.dtype == np.float64
         return lib().ora_advection_diffusion(self.h, ct.byref(args), _ptr(prop, _dp))
+
+    def set_discharges(self, d) -> int:
+        """SetDischarges (AD:977-1030) from a dict of arrays: per discharge
+        cell flow, conc, concMF, I, J, K, Kmin, Kmax; per discharge vert,
+        ignore, ncells, bypass (see ora_set_discharges)."""
+        f = lambda n: np.ascontiguousarray(d[n], dtype=np.float64)
+        i = lambda n: np.ascontiguousarray(d[n], dtype=np.int32)
+        arrs = [f("flow"), f("conc"), f("concMF")] + [i(n) for n in ("I", "J", "K", "Kmin",
+                                                                      "Kmax", "vert", "ignore",
+                                                                      "ncells", "bypass")]
+        self._disch = arrs                      # kept alive while set
+        ptrs = [_ptr(a, _dp) for a in arrs[:3]] + [_ptr(a, _ip) for a in arrs[3:]]
+        return lib().ora_set_discharges(self.h, len(arrs[8]), *ptrs)
+
+    def unset_discharges(self) -> int:
+        rc = lib().ora_unset_discharges(self.h)
+        self._disch = None
+        return rc
 
     def internal(self, name: str) -> np.ndarray:
         ptr = lib().ora_internal(self.h, name.encode())

@@ -21,6 +21,11 @@ mathematics fixes:
   each side constant (1e-13); small-depth columns keep their values under
   vertical diffusion alone, the others match the step without small depths,
   and small depths everywhere are zero vertical diffusivity;
+* discharges (AD:4025-4128): the mass a step gains from positive
+  discharges is sum(flow * dt * conc), for point discharges and for
+  uniform ones spread over the column or over Kmin..Kmax; a withdrawal is
+  the implicit sink C' = C / (1 + |flow| dt MF / V); an ignored discharge
+  changes nothing;
 * implicit horizontal advection (the THOMAS_3D path, AD:1758-1843,
   4167-4256, 4490-4525, 4858-4885): dense line solves of implicit upwind
   transport at a Courant number of 3, the 3-D
@@ -432,6 +437,121 @@ def _small_depth_err(optimize, docycle):
     return max(err, float(np.max(np.abs(a - b))))
 
 
+def _discharge_set(case, rng, kinds, flows, conc, bypass=None, ignore=None, mf=None,
+                   kmin=None, kmax=None):
+    """Discharges at random open surface-column cells: kinds[d] 'point' (at a
+    level of the column) or 'uniform' (spread over kmin..kmax, FillValue =
+    KFloor / KUB)."""
+    KUB = case.worksize[5]
+    wp = case.hydro["WaterPoints3D"][:, :, KUB] == 1
+    ii, jj = np.nonzero(wp)
+    pick = rng.choice(len(ii), size=len(kinds), replace=False)
+    d = {n: [] for n in ("flow", "conc", "concMF", "I", "J", "K", "Kmin", "Kmax")}
+    ign = ignore or [0] * len(kinds)
+    for q, kind in enumerate(kinds):
+        i, j = int(ii[pick[q]]), int(jj[pick[q]])
+        kf = int(case.geom["KFloorZ"][i, j])
+        if ign[q]:
+            continue                                  # its cells are not listed (AD:4042)
+        d["flow"].append(flows[q]); d["conc"].append(conc[q])
+        d["concMF"].append(1.0 if mf is None else mf[q])
+        d["I"].append(i); d["J"].append(j); d["K"].append(int(rng.integers(kf, KUB + 1)))
+        d["Kmin"].append(-9999999 if kmin is None else max(kf, KUB + 1 - kmin))
+        d["Kmax"].append(-9999999 if kmax is None else KUB)
+    d["vert"] = [5 if k == "uniform" else 1 for k in kinds]
+    d["ignore"] = ign
+    d["ncells"] = [1] * len(kinds)
+    d["bypass"] = bypass or [0] * len(kinds)
+    return d
+
+
+def _discharge_budget_err(kind, kmin=None):
+    """Closed basin, vertical diffusion, positive discharges: the mass gained
+    in one step is sum(flow * dt * conc) (point discharges; a uniform one
+    spreads its flow over the cells of Kmin..Kmax in proportion to their
+    thickness, which sums to the flow)."""
+    case = fx.make_case(fx.config("c4", ni=8, nj=7, nk=6, depth=(5.0, 40.0)))
+    hy = _still_hydro(case)
+    hy["Diff_V"][...] = case.hydro["Diff_V"]
+    wp = hy["WaterPoints3D"] == 1
+    V = hy["VolumeZ"]
+    rng = np.random.default_rng(31)
+    c0 = np.asfortranarray(np.where(wp, rng.uniform(1.0, 3.0, size=wp.shape), 0.0))
+    flows, conc = [3.0, 1.5, 0.75], [10.0, 4.0, 2.0]
+    d = _discharge_set(case, rng, [kind] * 3, flows, conc, bypass=[0, 1, 0], kmin=kmin,
+                       kmax=None if kmin is None else 0)
+    inst = ora.Instance(case, nthreads=2)
+    assert inst.set_discharges(d) == 0
+    got = c0.copy(order="F")
+    params = dict(UPWIND, ImpExp_DifV=1.0, SchmidtCoef_V=1.0, SchmidtBackground_V=0.0)
+    assert inst.step(got, first_property=True, now=DT, hydro=hy, **params) == 0
+    inst.close()
+    m0 = float(np.sum(V[wp] * c0[wp]))
+    gain = float(np.sum(V[wp] * got[wp])) - m0
+    want = sum(f * DT * c for f, c in zip(flows, conc))
+    # relative to the basin's mass: the difference of two sums of that size
+    # carries their rounding (the gain itself is about 2e-5 of it)
+    return abs(gain - want) / m0
+
+
+def _withdrawal_err():
+    """A withdrawal (negative flow, no bypass) in still water is the
+    implicit sink  V (C' - C) / dt = flow * MF * C'  at its cell."""
+    case = fx.make_case(fx.config("c1", ni=5, nj=5, nk=3))
+    hy = _still_hydro(case)
+    wp = hy["WaterPoints3D"] == 1
+    V = hy["VolumeZ"]
+    rng = np.random.default_rng(5)
+    c0 = np.asfortranarray(np.where(wp, rng.uniform(1.0, 3.0, size=wp.shape), 0.0))
+    d = _discharge_set(case, rng, ["point"], [-0.8], [7.0], mf=[1.5])
+    inst = ora.Instance(case, nthreads=2)
+    assert inst.set_discharges(d) == 0
+    got = c0.copy(order="F")
+    assert inst.step(got, first_property=True, now=DT, hydro=hy, **UPWIND) == 0
+    inst.close()
+    i, j, k = d["I"][0], d["J"][0], d["K"][0]
+    want = c0[i, j, k] / (1.0 + 0.8 * DT * 1.5 / V[i, j, k])
+    others = np.ones(wp.shape, dtype=bool)
+    others[i, j, k] = False
+    err = abs(got[i, j, k] - want) / want
+    return max(err, float(np.max(np.abs(got[others & wp] - c0[others & wp]))))
+
+
+def _ignored_discharge_err():
+    """An ignored discharge (its cells not listed, AD:4042) changes nothing."""
+    case = make("tvd_land_sigma")
+    rng = np.random.default_rng(8)
+    a = _discharge_set(case, rng, ["point", "uniform", "point"], [2.0, 1.0, -0.5],
+                       [5.0, 3.0, 1.0], ignore=[0, 1, 0])
+    b = {k: list(v) for k, v in a.items()}
+    b["vert"] = [a["vert"][0], a["vert"][2]]
+    b["ignore"] = [0, 0]
+    b["ncells"] = [1, 1]
+    b["bypass"] = [a["bypass"][0], a["bypass"][2]]
+    outs = []
+    for dd in (a, b):
+        inst = ora.Instance(case, nthreads=2)
+        assert inst.set_discharges(dd) == 0
+        p = np.array(case.tracers[0], order="F", copy=True)
+        assert inst.step(p, first_property=True, now=DT) == 0
+        inst.close()
+        outs.append(p)
+    return float(np.max(np.abs(outs[0] - outs[1])))
+
+
+@pytest.mark.parametrize("kind,kmin", [("point", None), ("uniform", None), ("uniform", 3)])
+def test_discharge_mass_budget(kind, kmin):
+    assert _discharge_budget_err(kind, kmin) <= 1e-14
+
+
+def test_withdrawal_closed_form():
+    assert _withdrawal_err() <= 1e-14
+
+
+def test_ignored_discharge_changes_nothing():
+    assert _ignored_discharge_err() == 0.0
+
+
 def test_nodifflux_everywhere_is_zero_diffusivity():
     assert _nodif_all_err() == 0.0
 
@@ -513,6 +633,11 @@ def _all_checks():
                             _implicit_channel_err(axis, sign, nk), 1e-12))
         out.append((f"implicit fractional {axis}", _implicit_fractional_err(axis), 1e-12))
     out.append(("nodif everywhere", _nodif_all_err(), 0.0))
+    for kind, kmin in (("point", None), ("uniform", None), ("uniform", 3)):
+        out.append((f"discharge budget {kind} kmin={kmin}", _discharge_budget_err(kind, kmin),
+                    1e-14))
+    out.append(("withdrawal", _withdrawal_err(), 1e-14))
+    out.append(("ignored discharge", _ignored_discharge_err(), 0.0))
     for axis in ("x", "y", "z"):
         out.append((f"nodif wall {axis}", _nodif_wall_err(axis), 1e-13))
     for opt, dc in ((False, 1), (True, 1), (True, 2)):
@@ -527,7 +652,7 @@ def _all_checks():
     return out
 
 
-@pytest.mark.parametrize("mistake", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12])
+@pytest.mark.parametrize("mistake", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13])
 def test_planted_mistakes_are_caught(mistake):
     L = _lib()
     L.ora_plant_mistake(mistake)

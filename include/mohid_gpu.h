@@ -101,7 +101,6 @@ typedef struct {
     int32_t unsupported_CellFluxes, unsupported_ReferenceProp;
     int32_t unsupported_BoundaryCondition; /* NullGradient_(4) / CyclicBoundary_(8) post-processing */
     int32_t unsupported_NoAdvFlux;         /* its YY branch zeroes the XX coefficients, AD:4804-4812 */
-    int32_t unsupported_Discharges;
     /* offloaded optional arguments (AD:1287-1335); 0 / null when absent */
     int32_t NoDifFlux;                     /* zero diffusion across the NoFlux faces */
     const int32_t *NoFluxU, *NoFluxV, *NoFluxW;   /* 3-D, 1 = flagged face; needed with NoDifFlux */
@@ -125,6 +124,23 @@ int mgpu_ad_set_geometry(mgpu_ad *ad, const mgpu_geometry *geometry);
  * Host mode copies the inputs in and PROP out per call (PCIe-bound: use the
  * device mode or the batched transport for performance). */
 int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *args, double *prop);
+
+/* SetDischarges (AD:978-1034) / UnSetDischarges (AD:1040-1086): the
+ * discharges of the following mgpu_ad_step calls, in the reference's
+ * argument order.  Per discharge cell n (the cells of the discharges that
+ * are not ignored, in order): DischFlow, DischConc, DischConcMF, DischI/J/K,
+ * DischKmin/Kmax (FillValueInt -9999999 = the column's KFloor / KUB for a
+ * uniform discharge); per discharge: DischVert (DischUniform_ = 5 spreads
+ * the flow over Kmin..Kmax), IgnoreDisch, DischnCells, ByPass (0/1).  Host
+ * arrays, copied: WaterProperties sets them before every call (WP:14763,
+ * 14830).  Cells outside the WorkSize give MGPU_ERR_BADARG. */
+int mgpu_ad_set_discharges(mgpu_ad *ad, const double *DischFlow, const double *DischConc,
+                           const int32_t *DischI, const int32_t *DischJ, const int32_t *DischK,
+                           const int32_t *DischKmin, const int32_t *DischKmax,
+                           const int32_t *DischVert, int32_t DischNumber,
+                           const int32_t *IgnoreDisch, const int32_t *DischnCells,
+                           const int32_t *ByPass, const double *DischConcMF);
+int mgpu_ad_unset_discharges(mgpu_ad *ad);
 
 /* KillAdvectionDiffusion (AD:5849-6066). */
 int mgpu_ad_destroy(mgpu_ad *ad);

@@ -72,7 +72,6 @@ class ADArgs(ct.Structure):
         ("Optimize", ct.c_int32), ("FirstProperty", ct.c_int32),
         ("unsupported_CellFluxes", ct.c_int32), ("unsupported_ReferenceProp", ct.c_int32),
         ("unsupported_BoundaryCondition", ct.c_int32), ("unsupported_NoAdvFlux", ct.c_int32),
-        ("unsupported_Discharges", ct.c_int32),
         ("NoDifFlux", ct.c_int32), ("NoFluxU", _ip), ("NoFluxV", _ip), ("NoFluxW", _ip),
         ("SmallDepths", _ip),
         ("device_pointers", ct.c_int32),
@@ -89,7 +88,8 @@ class TransportOpts(ct.Structure):
 # every symbol include/mohid_gpu.h declares
 EXPORTS = (
     "mgpu_ad_create", "mgpu_ad_set_geometry", "mgpu_ad_step", "mgpu_ad_destroy",
-    "mgpu_ad_zero_pivots", "mgpu_thomasz", "mgpu_thomas3d", "mgpu_transport_status",
+    "mgpu_ad_zero_pivots", "mgpu_ad_set_discharges", "mgpu_ad_unset_discharges",
+    "mgpu_thomasz", "mgpu_thomas3d", "mgpu_transport_status",
     "mgpu_transport_create", "mgpu_transport_set_hydro", "mgpu_transport_set_tracer",
     "mgpu_transport_get_tracer", "mgpu_transport_run", "mgpu_transport_tracer_ptr",
     "mgpu_transport_set_lanes",
@@ -127,6 +127,8 @@ def lib():
     L.mgpu_thomasz.argtypes = [ct.POINTER(Size3D), ct.POINTER(Size3D), vp, vp, vp, vp, vp, vp,
                                ct.c_int32, vp]
     i32 = ct.c_int32
+    L.mgpu_ad_set_discharges.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp]
+    L.mgpu_ad_unset_discharges.argtypes = [vp]
     L.mgpu_thomas3d.argtypes = [ct.POINTER(Size3D), i32, i32, i32, i32, i32, i32, i32, i32, vp,
                                 vp, vp, vp, vp, i32, vp]
     L.mgpu_transport_status.argtypes = [vp]

@@ -107,7 +107,7 @@ def _args(hydro: Dict, params: Dict, keep: list, device: bool, optimize: bool,
         int(bool(optimize)), int(bool(first_property)),
         int(bool(p.get("CellFluxes", False))), int(bool(p.get("ReferenceProp", False))),
         int(p.get("BoundaryCondition", 1) in (4, 8)), int(bool(p.get("NoAdvFlux", False))),
-        int(bool(p.get("Discharges", False))), int(bool(p.get("NoDifFlux", False))),
+        int(bool(p.get("NoDifFlux", False))),
         opt_ip("NoFluxU"), opt_ip("NoFluxV"), opt_ip("NoFluxW"), opt_ip("SmallDepths"),
         int(bool(device)))
 
@@ -150,6 +150,23 @@ class AdvectionDiffusion:
                 raise TypeError("PROP must be a Fortran-ordered float64 array")
             addr = ct.c_void_p(prop.ctypes.data)
         L.check("AdvectionDiffusion", L.lib().mgpu_ad_step(self._h, ct.byref(args), addr))
+
+    def set_discharges(self, d: Dict) -> None:
+        """SetDischarges (AD:978-1034) from a dict of arrays: per discharge
+        cell flow, conc, concMF, I, J, K, Kmin, Kmax; per discharge vert,
+        ignore, ncells, bypass (see mgpu_ad_set_discharges)."""
+        f = lambda n: np.ascontiguousarray(d[n], dtype=np.float64)
+        i = lambda n: np.ascontiguousarray(d[n], dtype=np.int32)
+        a = {n: f(n) for n in ("flow", "conc", "concMF")}
+        a.update({n: i(n) for n in ("I", "J", "K", "Kmin", "Kmax", "vert", "ignore", "ncells",
+                                    "bypass")})
+        p = {n: ct.c_void_p(v.ctypes.data) for n, v in a.items()}
+        L.check("SetDischarges", L.lib().mgpu_ad_set_discharges(
+            self._h, p["flow"], p["conc"], p["I"], p["J"], p["K"], p["Kmin"], p["Kmax"],
+            p["vert"], len(a["vert"]), p["ignore"], p["ncells"], p["bypass"], p["concMF"]))
+
+    def unset_discharges(self) -> None:
+        L.check("UnSetDischarges", L.lib().mgpu_ad_unset_discharges(self._h))
 
     def zero_pivots(self) -> int:
         c = ct.c_uint64()

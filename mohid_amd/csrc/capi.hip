@@ -5,7 +5,10 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <new>
+#include <utility>
+#include <vector>
 
 #include "mgpu_internal.h"
 
@@ -207,7 +210,7 @@ struct DevDerived {
     // masks -> mask bytes, kstart, tile orders (after the mask arrays are on the device)
     // hlines: also the list of implicit line rows (allocated on first use)
     int pack(const Dims &d, const DevHydro &h, const int32_t *kfloor, int docycle,
-             hipStream_t s, bool tiles, bool hlines = false) {
+             hipStream_t s, bool tiles, bool hlines = false, const int32_t *dptr = nullptr) {
         if (hlines && !hrows) {
             const long nw = (long)(d.wIUB - d.wILB + 1) * (d.wJUB - d.wJLB + 1) *
                             (d.wKUB - d.wKLB + 1);
@@ -215,8 +218,8 @@ struct DevDerived {
             if (dalloc(&hrows, nw)) return MGPU_ERR_NOMEM;
         }
         int rc = launch_pack_masks(d, h.open, h.land, h.cfu, h.cfv, h.cfw, h.water,
-                                   h.have_small ? h.small : nullptr, kfloor, docycle, mask,
-                                   kstart, mask_flags(), s);
+                                   h.have_small ? h.small : nullptr, dptr, kfloor, docycle,
+                                   mask, kstart, mask_flags(), s);
         // tile metadata only for the tile variants (level keys are bytes there)
         if (!rc && tiles) rc = launch_tile_sort(d, mask, kstart, cperm, ccnt, fperm, fcnt, s);
         if (!rc)
@@ -259,8 +262,7 @@ int check_options(const mgpu_ad_args *a) {
     if ((a->ImpExp_AdvXX != 0.0 && !impXX) || (a->ImpExp_AdvYY != 0.0 && !impYY))
         return MGPU_ERR_OPTION;
     if (a->AdvectionNudging || a->unsupported_CellFluxes || a->unsupported_ReferenceProp ||
-        a->unsupported_BoundaryCondition || a->unsupported_NoAdvFlux ||
-        a->unsupported_Discharges)
+        a->unsupported_BoundaryCondition || a->unsupported_NoAdvFlux)
         return MGPU_ERR_UNSUPPORTED;
     if (a->NoDifFlux && (!a->NoFluxU || !a->NoFluxV || !a->NoFluxW)) return MGPU_ERR_BADARG;
     if (!a->Wflux_X || !a->Wflux_Y || !a->Wflux_Z || !a->VolumeZOld || !a->VolumeZ ||
@@ -290,6 +292,42 @@ StepScalars make_scalars(const Dims &d, const mgpu_ad_args *a, int vertical1d, i
     sc.hreset = sc.impH >= 0 && sc.vert;
     return sc;
 }
+
+// Discharges of a drop-in handle (SetDischarges, AD:978-1034): the
+// reference's discharge cells expanded to one contribution per level,
+// numbered in its loop order (AD:4040-4123), and a per-cell index into the
+// cell-sorted list.  Only the cells of the current set are non-negative in
+// `ptr`; a new set clears the previous cells first.
+struct DevDischarges {
+    bool on = false;
+    int ncontrib = 0, ncell = 0, nptr = 0;   // contributions, discharge cells, distinct cells
+    int cap_c = 0, cap_q = 0;
+    int32_t *ptr = nullptr;                  // n3
+    uint32_t *cell = nullptr;                // cell-sorted: Size index
+    int32_t *id = nullptr;                   // cell-sorted: contribution number
+    uint32_t *ucell = nullptr;               // distinct cells
+    int32_t *ufirst = nullptr;               // their first entry in the sorted list
+    uint32_t *ccell = nullptr;               // per contribution: Size index
+    double *flow = nullptr, *conc = nullptr, *mf = nullptr;
+    uint8_t *bypass = nullptr;
+    int32_t *cbase = nullptr, *cnt = nullptr;   // per discharge cell
+    uint8_t *uniform = nullptr;
+    double *dflow = nullptr;
+
+    void release() {
+        dfree(ptr); dfree(cell); dfree(id); dfree(ucell); dfree(ufirst); dfree(ccell);
+        dfree(flow); dfree(conc); dfree(mf); dfree(bypass); dfree(cbase); dfree(cnt);
+        dfree(uniform); dfree(dflow);
+        cap_c = cap_q = 0;
+    }
+    DischargeFields fields() const {
+        DischargeFields q;
+        if (!on) return q;
+        q.ptr = ptr; q.cell = cell; q.id = id; q.flow = flow; q.conc = conc; q.mf = mf;
+        q.bypass = bypass; q.n = ncontrib;
+        return q;
+    }
+};
 
 SharedFields make_fields(const DevHydro &h, const DevDerived &dv, const DevGeometry &g) {
     SharedFields f;
@@ -351,6 +389,7 @@ struct mgpu_ad {
     double *tig = nullptr, *mid = nullptr;
     double *Rb = nullptr;                   // implicit line row records {D, E, F, TI}
     bool have_faces = false;
+    DevDischarges disch;
     double last_dt = 0.0, last_schmidtH = 0.0;   // Set_Internal_State's memory (AD:5746)
     unsigned long long last_zero_pivots = 0;
 };
@@ -410,6 +449,7 @@ extern "C" int mgpu_ad_destroy(mgpu_ad *ad) {
     ad->der.release();
     dfree(ad->prop); dfree(ad->out); dfree(ad->Wb); dfree(ad->Gb);
     dfree(ad->tig); dfree(ad->mid); dfree(ad->Rb);
+    ad->disch.release();
     if (ad->stream) (void)hipStreamDestroy(ad->stream);
     delete ad;
     return MGPU_SUCCESS;
@@ -419,6 +459,154 @@ extern "C" int mgpu_ad_zero_pivots(const mgpu_ad *ad, uint64_t *count) {
     if (!ad) return MGPU_OFF_ERR;
     if (!count) return MGPU_ERR_BADARG;
     *count = ad->last_zero_pivots;
+    return MGPU_SUCCESS;
+}
+
+namespace {
+template <typename T>
+int upload(T *dst, const std::vector<T> &v, hipStream_t s) {
+    if (v.empty()) return MGPU_SUCCESS;
+    HIPCHK(hipMemcpyAsync(dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s));
+    return MGPU_SUCCESS;
+}
+
+// clears the per-cell index of the current set (the cells it marked)
+int disch_clear(mgpu_ad *ad) {
+    DevDischarges &q = ad->disch;
+    int rc = MGPU_SUCCESS;
+    if (q.on && q.nptr > 0) rc = launch_disch_scatter(q.nptr, q.ucell, nullptr, q.ptr, ad->stream);
+    q.on = false;
+    q.nptr = 0;
+    return rc;
+}
+}  // namespace
+
+extern "C" int mgpu_ad_set_discharges(mgpu_ad *ad, const double *DischFlow,
+                                      const double *DischConc, const int32_t *DischI,
+                                      const int32_t *DischJ, const int32_t *DischK,
+                                      const int32_t *DischKmin, const int32_t *DischKmax,
+                                      const int32_t *DischVert, int32_t DischNumber,
+                                      const int32_t *IgnoreDisch, const int32_t *DischnCells,
+                                      const int32_t *ByPass, const double *DischConcMF) {
+    if (!ad) return MGPU_OFF_ERR;
+    if (DischNumber < 0) return MGPU_ERR_BADARG;
+    if (DischNumber > 0 && (!DischVert || !IgnoreDisch || !DischnCells || !ByPass))
+        return MGPU_ERR_BADARG;
+    const Dims &d = ad->d;
+    hipStream_t s = ad->stream;
+    // KFloorZ for the uniform discharges' default bottom (AD:4056)
+    std::vector<int32_t> kfloor((size_t)d.n2());
+    HIPCHK(hipMemcpyAsync(kfloor.data(), ad->geom.KFloor, sizeof(int32_t) * kfloor.size(),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::vector<uint32_t> ccell;
+    std::vector<double> conc, mf, dflow;
+    std::vector<uint8_t> bypass, uniform;
+    std::vector<int32_t> cbase, cnt;
+    int n = 0;
+    for (int dis = 0; dis < DischNumber; ++dis) {
+        if (IgnoreDisch[dis]) continue;                  // AD:4042
+        if (DischnCells[dis] < 0) return MGPU_ERR_BADARG;
+        if (DischnCells[dis] > 0 && (!DischFlow || !DischConc || !DischConcMF || !DischI ||
+                                     !DischJ || !DischK || !DischKmin || !DischKmax))
+            return MGPU_ERR_BADARG;
+        for (int nc = 0; nc < DischnCells[dis]; ++nc, ++n) {
+            const int i = DischI[n], j = DischJ[n];
+            if (i < d.wILB || i > d.wIUB || j < d.wJLB || j > d.wJUB) return MGPU_ERR_BADARG;
+            const bool uni = DischVert[dis] == 5;        // DischUniform_ (GD:1006)
+            int kmin = DischKmin[n], kmax = DischKmax[n];
+            if (uni) {
+                if (kmin == -9999999) kmin = kfloor[(size_t)d.idx2(i, j)];   // FillValueInt
+                if (kmax == -9999999) kmax = d.wKUB;
+            } else {
+                kmin = kmax = DischK[n];
+            }
+            if (kmin <= kmax && (kmin < d.wKLB || kmax > d.wKUB)) return MGPU_ERR_BADARG;
+            cbase.push_back((int32_t)ccell.size());
+            cnt.push_back(kmax >= kmin ? kmax - kmin + 1 : 0);
+            uniform.push_back(uni ? 1 : 0);
+            dflow.push_back(DischFlow[n]);
+            for (int k = kmin; k <= kmax; ++k) {
+                ccell.push_back((uint32_t)d.idx(i, j, k));
+                conc.push_back(DischConc[n]);
+                mf.push_back(DischConcMF[n]);
+                bypass.push_back(ByPass[dis] ? 1 : 0);
+            }
+        }
+    }
+    const int nc = (int)ccell.size(), nq = (int)cbase.size();
+    // cell-sorted list (stable: contributions of a cell keep their order)
+    std::vector<std::pair<uint32_t, int32_t>> order((size_t)nc);
+    for (int c = 0; c < nc; ++c) order[(size_t)c] = {ccell[(size_t)c], c};
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::pair<uint32_t, int32_t> &a,
+                        const std::pair<uint32_t, int32_t> &b) { return a.first < b.first; });
+    std::vector<uint32_t> cell((size_t)nc), ucell;
+    std::vector<int32_t> id((size_t)nc), ufirst;
+    for (int m = 0; m < nc; ++m) {
+        cell[(size_t)m] = order[(size_t)m].first;
+        id[(size_t)m] = order[(size_t)m].second;
+        if (m == 0 || cell[(size_t)m] != cell[(size_t)m - 1]) {
+            ucell.push_back(cell[(size_t)m]);
+            ufirst.push_back(m);
+        }
+    }
+    DevDischarges &q = ad->disch;
+    int rc = disch_clear(ad);
+    if (rc) return rc;
+    if (!q.ptr) {
+        if ((rc = dalloc(&q.ptr, d.n3()))) return rc;
+        HIPCHK(hipMemsetAsync(q.ptr, 0xff, sizeof(int32_t) * (size_t)d.n3(), s));   // all -1
+    }
+    if (nc > q.cap_c) {
+        int cap = q.cap_c;
+        dfree(q.cell); dfree(q.id); dfree(q.ucell); dfree(q.ufirst); dfree(q.ccell);
+        dfree(q.flow); dfree(q.conc); dfree(q.mf); dfree(q.bypass);
+        cap = nc < 64 ? 64 : nc;
+        rc = dalloc(&q.cell, cap);
+        if (!rc) rc = dalloc(&q.id, cap);
+        if (!rc) rc = dalloc(&q.ucell, cap);
+        if (!rc) rc = dalloc(&q.ufirst, cap);
+        if (!rc) rc = dalloc(&q.ccell, cap);
+        if (!rc) rc = dalloc(&q.flow, cap);
+        if (!rc) rc = dalloc(&q.conc, cap);
+        if (!rc) rc = dalloc(&q.mf, cap);
+        if (!rc) rc = dalloc(&q.bypass, cap);
+        if (rc) { q.cap_c = 0; return rc; }
+        q.cap_c = cap;
+    }
+    if (nq > q.cap_q) {
+        dfree(q.cbase); dfree(q.cnt); dfree(q.uniform); dfree(q.dflow);
+        const int cap = nq < 64 ? 64 : nq;
+        rc = dalloc(&q.cbase, cap);
+        if (!rc) rc = dalloc(&q.cnt, cap);
+        if (!rc) rc = dalloc(&q.uniform, cap);
+        if (!rc) rc = dalloc(&q.dflow, cap);
+        if (rc) { q.cap_q = 0; return rc; }
+        q.cap_q = cap;
+    }
+    if ((rc = upload(q.cell, cell, s)) || (rc = upload(q.id, id, s)) ||
+        (rc = upload(q.ucell, ucell, s)) || (rc = upload(q.ufirst, ufirst, s)) ||
+        (rc = upload(q.ccell, ccell, s)) || (rc = upload(q.conc, conc, s)) ||
+        (rc = upload(q.mf, mf, s)) || (rc = upload(q.bypass, bypass, s)) ||
+        (rc = upload(q.cbase, cbase, s)) || (rc = upload(q.cnt, cnt, s)) ||
+        (rc = upload(q.uniform, uniform, s)) || (rc = upload(q.dflow, dflow, s)))
+        return rc;
+    rc = launch_disch_scatter((int)ucell.size(), q.ucell, q.ufirst, q.ptr, s);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(s));   // the host vectors go out of scope
+    q.ncontrib = nc;
+    q.ncell = nq;
+    q.nptr = (int)ucell.size();
+    q.on = true;
+    return MGPU_SUCCESS;
+}
+
+extern "C" int mgpu_ad_unset_discharges(mgpu_ad *ad) {
+    if (!ad) return MGPU_OFF_ERR;
+    int rc = disch_clear(ad);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(ad->stream));
     return MGPU_SUCCESS;
 }
 
@@ -441,7 +629,15 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     else HIPCHK(hipMemcpyAsync(ad->prop, prop, b3, hipMemcpyHostToDevice, s));
 
     HIPCHK(hipMemsetAsync(ad->der.counters, 0, sizeof(unsigned long long) * 4, s));
-    rc = ad->der.pack(d, ad->hydro, ad->geom.KFloor, ad->docycle, s, false, sc.impH >= 0);
+    if (ad->disch.on) {
+        // this call's Flow of every discharge contribution (AD:4064-4082)
+        rc = launch_disch_flow(ad->disch.ncell, ad->disch.cbase, ad->disch.cnt,
+                               ad->disch.uniform, ad->disch.dflow, ad->disch.ccell, ad->geom.DWZ,
+                               ad->disch.flow, s);
+        if (rc) return rc;
+    }
+    rc = ad->der.pack(d, ad->hydro, ad->geom.KFloor, ad->docycle, s, false, sc.impH >= 0,
+                      ad->disch.on ? ad->disch.ptr : nullptr);
     if (rc) return rc;
     {
         // Convert_Visc_Dif_Horizontal(_opt) + Convert_Dif_Vertical (+ the
@@ -471,7 +667,8 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
         ad->last_dt = a->DTProp;
         ad->last_schmidtH = a->schmidt_H;
     }
-    const SharedFields f = make_fields(ad->hydro, ad->der, ad->geom);
+    SharedFields f = make_fields(ad->hydro, ad->der, ad->geom);
+    f.disch = ad->disch.fields();
     // the step's own initialisation of TI (AD:1660-1671) reaches the ghost rows
     if (first) HIPCHK(hipMemsetAsync(ad->tig, 0, b3, s));
     TransportLaunch L;

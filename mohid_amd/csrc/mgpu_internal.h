@@ -75,6 +75,20 @@ struct StepScalars {
     int hreset;
 };
 
+// Discharges of a drop-in call (SetDischarges, AD:978-1034), one entry per
+// (discharge cell, level) "contribution", numbered in the reference's loop
+// order (AD:4040-4123).  ptr == null: none.
+struct DischargeFields {
+    const int32_t *ptr = nullptr;     // per Size cell: first entry of the cell in the
+                                      // cell-sorted list, or -1
+    const uint32_t *cell = nullptr;   // cell-sorted list: Size index
+    const int32_t *id = nullptr;      // cell-sorted list: contribution number
+    const double *flow = nullptr;     // per contribution: Flow (per call, disch_flow kernel)
+    const double *conc = nullptr, *mf = nullptr;   // per contribution: DischConc, DischConcMF
+    const uint8_t *bypass = nullptr;  // per contribution: ByPass
+    int n = 0;                        // contributions
+};
+
 struct SharedFields {     // device pointers shared by all tracers
     const double *Qx, *Qy, *Qz, *V, *Vold;
     const double *cu, *cv, *cw;     // face conductances (prep kernel)
@@ -85,16 +99,28 @@ struct SharedFields {     // device pointers shared by all tracers
     // level, and how many of each are active at every level (tile_sort)
     const uint8_t *cperm;
     const uint16_t *ccnt, *fperm, *fcnt;
+    DischargeFields disch;
 };
 
 // Kernel launchers (transport.hip)
 // small: the 2-D small-depth flags (M_SDV on every cell of a flagged
-// column) or null.
+// column) or null; dptr: the discharge cell index (a discharge cell is never
+// an identity row) or null.
 int launch_pack_masks(const Dims &d, const int32_t *open, const int32_t *land,
                       const int32_t *cfu, const int32_t *cfv, const int32_t *cfw,
-                      const int32_t *water, const int32_t *small, const int32_t *kfloor,
-                      int docycle, uint8_t *mask, int32_t *kstart, unsigned int *flags,
-                      hipStream_t s);
+                      const int32_t *water, const int32_t *small, const int32_t *dptr,
+                      const int32_t *kfloor, int docycle, uint8_t *mask, int32_t *kstart,
+                      unsigned int *flags, hipStream_t s);
+// Discharges: Flow of every contribution for this call (uniform discharges
+// spread over their cells by thickness, AD:4064-4082).  Per discharge cell
+// q: first contribution cbase[q], count cnt[q], uniform flag, DischFlow;
+// ccell: the Size index of every contribution.
+int launch_disch_flow(int ncell, const int32_t *cbase, const int32_t *cnt, const uint8_t *uniform,
+                      const double *dflow, const uint32_t *ccell, const double *DWZ,
+                      double *flow, hipStream_t s);
+// ptr[cells[m]] = vals ? vals[m] : -1 for m < n.
+int launch_disch_scatter(int n, const uint32_t *cells, const int32_t *vals, int32_t *ptr,
+                         hipStream_t s);
 // Tile ordering for ragged columns; tile_count() tiles, 256 columns and
 // 580 faces each, counts over the Size k range.
 long tile_count(const Dims &d);

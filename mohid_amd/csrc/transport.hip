@@ -10,15 +10,15 @@
 // launchers
 int launch_pack_masks(const Dims &d, const int32_t *open, const int32_t *land,
                       const int32_t *cfu, const int32_t *cfv, const int32_t *cfw,
-                      const int32_t *water, const int32_t *small, const int32_t *kfloor,
-                      int docycle, uint8_t *mask, int32_t *kstart, unsigned int *flags,
-                      hipStream_t s) {
+                      const int32_t *water, const int32_t *small, const int32_t *dptr,
+                      const int32_t *kfloor, int docycle, uint8_t *mask, int32_t *kstart,
+                      unsigned int *flags, hipStream_t s) {
     const long n = d.n3();
     const int threads = 256;
     long blocks = (n + threads - 1) / threads;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(pack_masks_kernel, dim3((unsigned)blocks), dim3(threads), 0, s, d, open,
-                       land, cfu, cfv, cfw, water, small, mask, n);
+                       land, cfu, cfv, cfw, water, small, dptr, mask, n);
     HIPCHK(hipGetLastError());
     const int ni = d.wIUB - d.wILB + 1, nj = d.wJUB - d.wJLB + 1;
     dim3 b(64, 4), g((ni + 63) / 64, (nj + 3) / 4);
@@ -244,6 +244,31 @@ __global__ void interleave_kernel(long n3, const double *__restrict__ src,
 }  // namespace
 
 namespace {
+__global__ void disch_flow_kernel(int ncell, const int32_t *__restrict__ cbase,
+                                  const int32_t *__restrict__ cnt,
+                                  const uint8_t *__restrict__ uniform,
+                                  const double *__restrict__ dflow,
+                                  const uint32_t *__restrict__ ccell,
+                                  const double *__restrict__ DWZ, double *__restrict__ flow) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ncell) return;
+    const int b = cbase[q], nk = cnt[q];
+    if (!uniform[q]) {
+        for (int c = b; c < b + nk; ++c) flow[c] = dflow[q];
+        return;
+    }
+    double wc = 0.0;                      // WaterColumn, summed over kmin..kmax
+    for (int c = b; c < b + nk; ++c) wc = wc + DWZ[ccell[c]];
+    for (int c = b; c < b + nk; ++c) flow[c] = dflow[q] * DWZ[ccell[c]] / wc;
+}
+
+__global__ void disch_scatter_kernel(int n, const uint32_t *__restrict__ cells,
+                                     const int32_t *__restrict__ vals, int32_t *__restrict__ ptr) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    ptr[cells[m]] = vals ? vals[m] : -1;
+}
+
 __global__ void copy_lines_kernel(Dims d, int axis, const double *__restrict__ src,
                                   double *__restrict__ dst,
                                   const unsigned long long *__restrict__ stop) {
@@ -267,6 +292,25 @@ __global__ void land_fill_kernel(long n3, const uint8_t *__restrict__ mask,
         if (mask[q] & M_LAND) dst[q] = MGPU_NULL_REAL;
 }
 }  // namespace
+
+int launch_disch_flow(int ncell, const int32_t *cbase, const int32_t *cnt, const uint8_t *uniform,
+                      const double *dflow, const uint32_t *ccell, const double *DWZ,
+                      double *flow, hipStream_t s) {
+    if (ncell <= 0) return MGPU_SUCCESS;
+    hipLaunchKernelGGL(disch_flow_kernel, dim3((ncell + 63) / 64), dim3(64), 0, s, ncell, cbase,
+                       cnt, uniform, dflow, ccell, DWZ, flow);
+    HIPCHK(hipGetLastError());
+    return MGPU_SUCCESS;
+}
+
+int launch_disch_scatter(int n, const uint32_t *cells, const int32_t *vals, int32_t *ptr,
+                         hipStream_t s) {
+    if (n <= 0) return MGPU_SUCCESS;
+    hipLaunchKernelGGL(disch_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, cells,
+                       vals, ptr);
+    HIPCHK(hipGetLastError());
+    return MGPU_SUCCESS;
+}
 
 int launch_copy_lines(const Dims &d, int axis, const double *src, double *dst,
                       const unsigned long long *stop, hipStream_t s) {

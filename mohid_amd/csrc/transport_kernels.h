@@ -53,6 +53,7 @@ __global__ void pack_masks_kernel(Dims d, const int32_t *__restrict__ open,
                                   const int32_t *__restrict__ cfw,
                                   const int32_t *__restrict__ water,
                                   const int32_t *__restrict__ small,
+                                  const int32_t *__restrict__ dptr,
                                   uint8_t *__restrict__ mask, long n) {
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n;
          q += (long)gridDim.x * blockDim.x) {
@@ -72,7 +73,8 @@ __global__ void pack_masks_kernel(Dims d, const int32_t *__restrict__ open,
         // faces above along j (i, j+1) and i (i+1, j), inside the WorkSize
         const bool up_j = j + 1 <= d.wJUB && inws && cfu[q + d.sj] == 1;
         const bool up_i = i + 1 <= d.wIUB && inws && cfv[q + 1] == 1;
-        if (!(m & (M_OPEN | M_CFU | M_CFV)) && !up_j && !up_i) m |= M_HID;
+        if (!(m & (M_OPEN | M_CFU | M_CFV)) && !up_j && !up_i && !(dptr && dptr[q] >= 0))
+            m |= M_HID;
         if (small && small[r] != 0) m |= M_SDV;   // r: the (i, j) index in Size
         mask[q] = m;
     }
@@ -284,6 +286,37 @@ __global__ void prep_faces_kernel(Dims d, const double *__restrict__ visc,
 // masks) are computed once for all tracers of the launch.
 constexpr int MAXT = 8;
 
+
+// Discharges at cell x (AD:4085-4119), in the reference's order: into TI
+// right after VolumeVariation, and the withdrawals' diagonal into E.  The
+// drop-in sets them for its one tracer.
+__device__ __forceinline__ void disch_ti(const SharedFields &f, long x, bool open, double pk,
+                                         double dt, double vk, double &TI) {
+    const DischargeFields &q = f.disch;
+    if (!q.ptr) return;
+    for (int m = q.ptr[x]; m >= 0 && m < q.n && q.cell[m] == (uint32_t)x; ++m) {
+        const int c = q.id[m];
+        const double dtv = dt / vk;
+        const double flow = q.flow[c];
+        if (open) {
+            if (q.bypass[c] || flow > 0.0) TI = TI + flow * dtv * q.conc[c];
+        } else if (flow > 0.0) {
+            // a water point that is not open: no dilution (AD:4104-4117)
+            TI = pk * f.Vold[x] / vk + flow * dtv * q.conc[c];
+        }
+    }
+}
+
+__device__ __forceinline__ void disch_e(const SharedFields &f, long x, bool open, double dt,
+                                        double vk, double &E) {
+    const DischargeFields &q = f.disch;
+    if (!q.ptr || !open) return;
+    for (int m = q.ptr[x]; m >= 0 && m < q.n && q.cell[m] == (uint32_t)x; ++m) {
+        const int c = q.id[m];
+        const double flow = q.flow[c];
+        if (!q.bypass[c] && !(flow > 0.0)) E = E - flow * (dt / vk) * q.mf[c];
+    }
+}
 
 // Flux of one horizontal face for NT tracers.  `c0` indexes the cell on the
 // face's high side, `st` is the stride along the line, `pos` its line index
@@ -661,6 +694,7 @@ __device__ __forceinline__ void column_row(const Dims &d, const StepScalars &sc,
     const double dtk = FAST ? dt * frcp(vk) : dt / vk;
     double E0 = 1.0;
     if (k == KUB && (mk & M_OPEN) && !sc.hreset) E0 = 1.0 + dtk * f.Qz[x + sk];
+    if (!sc.hreset) disch_e(f, x, mk & M_OPEN, dt, vk, E0);   // AD:4100-4102
     const bool cfwB = sc.vert && (mk & M_CFW);
     const bool cfwT = sc.vert && k < KUB && (c.wm[2] & M_CFW);
     const double a2 = cfwB ? (FAST ? f.cw[x] * dtk : f.cw[x] * dt / vk) : 0.0;
@@ -1087,12 +1121,14 @@ __device__ __forceinline__ void cell_row_lane(const Dims &d, const StepScalars &
     if constexpr (IMP >= 0) {
         dt2 = FAST ? dt * frcp(vk) : dt / vk;
         if (k == d.wKUB && open) E0 = 1.0 + dt2 * f.Qz[x + d.sk];
+        disch_e(f, x, open, dt, vk, E0);
     }
 #pragma unroll
     for (int t = 0; t < TPL; ++t) {
         const double *P = ts.P + t;
         const double pk = P[x * s];
         double TI = open ? pk * vv : pk;
+        disch_ti(f, x, open, pk, dt, vk, TI);
         if (cfu0) {
             const double g = pk - P[(x - d.sj) * s];
             TI = ex ? TI - au0 * g * dt / vk : TI - wu0 * g;

@@ -9,6 +9,8 @@
 !   KillAdvectionDiffusion   Software/MOHIDBase2/ModuleAdvectionDiffusion.F90:5849
 !   THOMASZ_NewType2         Software/MOHIDBase1/ModuleFunctions.F90:4026
 !   THOMAS_3D                Software/MOHIDBase1/ModuleFunctions.F90:455, 3667
+!   SetDischarges            Software/MOHIDBase2/ModuleAdvectionDiffusion.F90:978
+!   UnSetDischarges          Software/MOHIDBase2/ModuleAdvectionDiffusion.F90:1040
 ! so a caller such as ModuleWaterProperties::Advection_Diffusion_Processes
 ! (Software/MOHIDWater/ModuleWaterProperties.F90:14776) can switch its call.
 !
@@ -29,6 +31,7 @@ module ModuleAdvectionDiffusionGPU
     public :: KillAdvectionDiffusionGPU
     public :: THOMASZ_GPU
     public :: THOMAS_3D_GPU
+    public :: SetDischargesGPU, UnSetDischargesGPU
     public :: MGPU_SUCCESS, MGPU_OFF_ERR, MGPU_ERR_OPTION, MGPU_ERR_UNSUPPORTED
     public :: MGPU_ERR_PIVOT
 
@@ -71,7 +74,6 @@ module ModuleAdvectionDiffusionGPU
         integer(c_int32_t) :: Optimize, FirstProperty
         integer(c_int32_t) :: unsupported_CellFluxes, unsupported_ReferenceProp
         integer(c_int32_t) :: unsupported_BoundaryCondition, unsupported_NoAdvFlux
-        integer(c_int32_t) :: unsupported_Discharges
         integer(c_int32_t) :: NoDifFlux
         type(c_ptr)        :: NoFluxU, NoFluxV, NoFluxW
         type(c_ptr)        :: SmallDepths
@@ -120,6 +122,23 @@ module ModuleAdvectionDiffusionGPU
             integer(c_int32_t),  value      :: dev
             type(c_ptr),         value      :: stream
         end function mgpu_thomas3d
+
+        integer(c_int) function mgpu_ad_set_discharges(handle, DischFlow, DischConc, DischI,   &
+                DischJ, DischK, DischKmin, DischKmax, DischVert, DischNumber, IgnoreDisch,    &
+                DischnCells, ByPass, DischConcMF) bind(c, name="mgpu_ad_set_discharges")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr),        value :: handle
+            type(c_ptr),        value :: DischFlow, DischConc, DischI, DischJ, DischK
+            type(c_ptr),        value :: DischKmin, DischKmax, DischVert
+            integer(c_int32_t), value :: DischNumber
+            type(c_ptr),        value :: IgnoreDisch, DischnCells, ByPass, DischConcMF
+        end function mgpu_ad_set_discharges
+
+        integer(c_int) function mgpu_ad_unset_discharges(handle)                               &
+                bind(c, name="mgpu_ad_unset_discharges")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle
+        end function mgpu_ad_unset_discharges
     end interface
 
     ! instance table: integer IDs as MOHID's modules hand out (ModuleGlobalData
@@ -193,9 +212,9 @@ contains
     !> STAT = MGPU_ERR_UNSUPPORTED and leave PROP untouched:
     !>   ReferenceProp present (open boundary, AD:5816-5830), BoundaryCondition
     !>   NullGradient_ or CyclicBoundary_ (post-solve passes, AD:1874-1882),
-    !>   CellFluxes, NoAdvFlux true (AD:1321-1325), Discharges (SetDischarges
-    !>   was called, WP:14760-14772), AdvectionNudging.
-    !> SmallDepths and NoDifFlux (with NoFluxU/V/W) are offloaded.
+    !>   CellFluxes, NoAdvFlux true (AD:1321-1325), AdvectionNudging.
+    !> SmallDepths, NoDifFlux (with NoFluxU/V/W) and the discharges of
+    !> SetDischargesGPU are offloaded.
     !> FirstProperty is .true. on the first call of every step (WP:14600,
     !> 14821-14826): the library keeps, like the reference, coefficients of
     !> the previous call of the same step.
@@ -210,7 +229,7 @@ contains
                                      WaterPoints3D, CellFluxes, ReferenceProp,                &
                                      BoundaryCondition, SmallDepths, NoAdvFlux, NoDifFlux,    &
                                      NoFluxU, NoFluxV, NoFluxW,                               &
-                                     Discharges, Optimize, FirstProperty, STAT)
+                                     Optimize, FirstProperty, STAT)
         integer,                           intent(in)    :: ID
         real(8), dimension(:,:,:), contiguous, target, intent(inout) :: PROP
         real(8),                           intent(in)    :: schmidt_H, SchmidtCoef_V
@@ -233,7 +252,7 @@ contains
         integer, dimension(:,:,:), contiguous, target, intent(in)    :: ComputeFacesW3D
         real(8), dimension(:,:,:), contiguous, target, intent(in)    :: Visc_H, Diff_V
         integer, dimension(:,:,:), contiguous, target, intent(in)    :: WaterPoints3D
-        logical, optional,                 intent(in)    :: CellFluxes, Discharges
+        logical, optional,                 intent(in)    :: CellFluxes
         real(8), dimension(:,:,:), optional, intent(in)  :: ReferenceProp
         integer, optional,                 intent(in)    :: BoundaryCondition
         logical, dimension(:,:), optional, intent(in)    :: SmallDepths
@@ -297,12 +316,53 @@ contains
             small = merge(1, 0, SmallDepths)
             a%SmallDepths = c_loc(small)
         endif
-        a%unsupported_Discharges = 0
-        if (present(Discharges)) a%unsupported_Discharges = b2i(Discharges)
         a%device_pointers = 0
         rc = mgpu_ad_step(Handles(ID), a, c_loc(PROP))
         if (present(STAT)) STAT = rc
     end subroutine AdvectionDiffusionGPU
+
+    !> SetDischarges (ModuleAdvectionDiffusion.F90:978-1034) with the reference's
+    !> arguments: the discharges of the following AdvectionDiffusionGPU calls
+    !> (the arrays are copied; WaterProperties sets them before every call,
+    !> WP:14763).
+    subroutine SetDischargesGPU(ID, DischFlow, DischConc, DischI, DischJ, DischK,        &
+                                DischKmin, DischKmax, DischVert, DischNumber,             &
+                                IgnoreDisch, DischnCells, ByPass, DischConcMF, STAT)
+        integer,                        intent(in)  :: ID
+        real(8), dimension(:), target,  intent(in)  :: DischFlow, DischConc, DischConcMF
+        integer, dimension(:), target,  intent(in)  :: DischI, DischJ, DischK
+        integer, dimension(:), target,  intent(in)  :: DischKmin, DischKmax, DischVert
+        integer,                        intent(in)  :: DischNumber
+        logical, dimension(:),          intent(in)  :: IgnoreDisch, ByPass
+        integer, dimension(:), target,  intent(in)  :: DischnCells
+        integer, optional,              intent(out) :: STAT
+        integer, allocatable, target :: ign(:), byp(:)
+        integer :: rc
+        if (ID < 1 .or. ID > MAX_INSTANCES) then
+            if (present(STAT)) STAT = MGPU_OFF_ERR
+            return
+        endif
+        allocate(ign(max(1, size(IgnoreDisch))), byp(max(1, size(ByPass))))
+        ign = 0; byp = 0
+        ign(1:size(IgnoreDisch)) = merge(1, 0, IgnoreDisch)
+        byp(1:size(ByPass)) = merge(1, 0, ByPass)
+        rc = mgpu_ad_set_discharges(Handles(ID), c_loc(DischFlow), c_loc(DischConc),          &
+                                    c_loc(DischI), c_loc(DischJ), c_loc(DischK),              &
+                                    c_loc(DischKmin), c_loc(DischKmax), c_loc(DischVert),     &
+                                    int(DischNumber, c_int32_t), c_loc(ign),                  &
+                                    c_loc(DischnCells), c_loc(byp), c_loc(DischConcMF))
+        if (present(STAT)) STAT = rc
+    end subroutine SetDischargesGPU
+
+    !> UnSetDischarges (ModuleAdvectionDiffusion.F90:1040-1086).
+    subroutine UnSetDischargesGPU(ID, STAT)
+        integer,           intent(in)  :: ID
+        integer, optional, intent(out) :: STAT
+        integer :: rc
+        rc = MGPU_OFF_ERR
+        if (ID >= 1 .and. ID <= MAX_INSTANCES) rc = mgpu_ad_unset_discharges(Handles(ID))
+        if (present(STAT)) STAT = rc
+    end subroutine UnSetDischargesGPU
 
     !> KillAdvectionDiffusion (ModuleAdvectionDiffusion.F90:5849).
     subroutine KillAdvectionDiffusionGPU(ID, STAT)

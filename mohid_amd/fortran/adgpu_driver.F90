@@ -16,7 +16,13 @@
 ! KFloorZ (2-D int), AreaU, AreaV, DZZ, DWZ, Wflux_X, Wflux_Y, Wflux_Z,
 ! VolumeZOld, VolumeZ (3-D real), OpenPoints3D, LandPoints3D,
 ! ComputeFacesU3D, ComputeFacesV3D, ComputeFacesW3D (3-D int), Visc_H, Diff_V
-! (3-D real), WaterPoints3D (3-D int), PROP (3-D real).
+! (3-D real), WaterPoints3D (3-D int), PROP (3-D real), then the
+! discharges: DischNumber, the number of discharge cells nc, and when
+! DischNumber > 0: DischFlow, DischConc, DischConcMF (nc real), DischI,
+! DischJ, DischK, DischKmin, DischKmax (nc int), DischVert, IgnoreDisch,
+! DischnCells, ByPass (DischNumber int, 0/1 for the logicals).  The
+! discharges are set before every step and unset after it, as
+! ModuleWaterProperties does around each call (WP:14763, 14830).
 ! Output: PROP after nsteps steps.  Exit status 0 on success, else the STAT.
 !------------------------------------------------------------------------------
 program adgpu_driver
@@ -33,6 +39,9 @@ program adgpu_driver
     real(8), allocatable, dimension(:,:,:) :: AreaU, AreaV, DZZ, DWZ, Qx, Qy, Qz, VolOld, Vol
     real(8), allocatable, dimension(:,:,:) :: Visc, Diff, PROP
     integer, allocatable, dimension(:,:,:) :: OP, LP, CFU, CFV, CFW, WP
+    integer :: ndis, ncell
+    real(8), allocatable, dimension(:) :: DFlow, DConc, DConcMF
+    integer, allocatable, dimension(:) :: DI, DJ, DK, DKmin, DKmax, DVert, DIgn, DnC, DByp
 
     call get_command_argument(1, fin)
     call get_command_argument(2, fout)
@@ -47,6 +56,15 @@ program adgpu_driver
     allocate(LP, CFU, CFV, CFW, WP, mold=OP)
     read(u) DUX, DVY, DZX, DZY, KFloorZ, AreaU, AreaV, DZZ, DWZ, Qx, Qy, Qz, VolOld, Vol
     read(u) OP, LP, CFU, CFV, CFW, Visc, Diff, WP, PROP
+    read(u) ndis, ncell
+    allocate(DFlow(max(1, ncell)), DConc(max(1, ncell)), DConcMF(max(1, ncell)))
+    allocate(DI(max(1, ncell)), DJ, DK, DKmin, DKmax, mold=DI)
+    allocate(DVert(max(1, ndis)), DIgn, DnC, DByp, mold=DVert)
+    if (ndis > 0) then
+        read(u) DFlow(1:ncell), DConc(1:ncell), DConcMF(1:ncell)
+        read(u) DI(1:ncell), DJ(1:ncell), DK(1:ncell), DKmin(1:ncell), DKmax(1:ncell)
+        read(u) DVert(1:ndis), DIgn(1:ndis), DnC(1:ndis), DByp(1:ndis)
+    endif
     close(u)
 
     call StartAdvectionDiffusionGPU(ID, sz, ws, DUX, DVY, DZX, DZY, KFloorZ,           &
@@ -56,6 +74,15 @@ program adgpu_driver
         error stop 2
     endif
     do s = 1, nsteps
+        if (ndis > 0) then
+            call SetDischargesGPU(ID, DFlow, DConc, DI, DJ, DK, DKmin, DKmax, DVert, ndis,  &
+                                  DIgn(1:ndis) /= 0, DnC, DByp(1:ndis) /= 0, DConcMF,      &
+                                  STAT = STAT)
+            if (STAT /= MGPU_SUCCESS) then
+                write(*, *) 'SetDischargesGPU STAT =', STAT
+                error stop 4
+            endif
+        endif
         call AdvectionDiffusionGPU(ID, PROP, rpar(1), rpar(2), rpar(3),                  &
                                    ipar(1), ipar(2), ipar(3), ipar(4),                   &
                                    ipar(5) /= 0, ipar(6) /= 0, rpar(4), .false., 0,       &
@@ -67,6 +94,10 @@ program adgpu_driver
         if (STAT /= MGPU_SUCCESS) then
             write(*, *) 'AdvectionDiffusionGPU step', s, 'STAT =', STAT
             error stop 3
+        endif
+        if (ndis > 0) then
+            call UnSetDischargesGPU(ID, STAT = STAT)
+            if (STAT /= MGPU_SUCCESS) error stop 5
         endif
     enddo
     call KillAdvectionDiffusionGPU(ID, STAT = STAT)

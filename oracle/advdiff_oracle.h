@@ -88,8 +88,8 @@ int ora_start(const ora_size3d *size, const ora_size3d *worksize,
 int ora_advection_diffusion(ora_instance *me, const ora_args *args, double *prop);
 /* KillAdvectionDiffusion (ModuleAdvectionDiffusion.F90:5849). */
 void ora_kill(ora_instance *me);
-/* SetDischarges (ModuleAdvectionDiffusion.F90:977-1030): the discharges of
- * the following calls, until ora_unset_discharges (AD:1034-1080).  The
+/* SetDischarges (ModuleAdvectionDiffusion.F90:978-1034): the discharges of
+ * the following calls, until ora_unset_discharges (AD:1040-1086).  The
  * arrays are kept by reference, as the reference keeps its pointers, and use
  * the reference's 1-based numbering shifted to 0: per discharge cell n (the
  * cells of the discharges that are not ignored, in order) flow, conc,

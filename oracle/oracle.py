@@ -197,7 +197,7 @@ class Instance:
         return lib().ora_advection_diffusion(self.h, ct.byref(args), _ptr(prop, _dp))
 
     def set_discharges(self, d) -> int:
-        """SetDischarges (AD:977-1030) from a dict of arrays: per discharge
+        """SetDischarges (AD:978-1034) from a dict of arrays: per discharge
         cell flow, conc, concMF, I, J, K, Kmin, Kmax; per discharge vert,
         ignore, ncells, bypass (see ora_set_discharges)."""
         f = lambda n: np.ascontiguousarray(d[n], dtype=np.float64)

@@ -1,6 +1,7 @@
 """The Fortran ISO_C_BINDING module on the GPU: a Fortran program
 (mohid_amd/fortran/adgpu_driver.F90) calls StartAdvectionDiffusionGPU,
-AdvectionDiffusionGPU and KillAdvectionDiffusionGPU with the argument lists of
+AdvectionDiffusionGPU, SetDischargesGPU/UnSetDischargesGPU and
+KillAdvectionDiffusionGPU with the argument lists of
 the reference's ModuleAdvectionDiffusion, on arrays read from a file; its
 result must match the oracle to 1e-12 (relative) on every solved cell."""
 import os
@@ -13,11 +14,12 @@ from mohid_amd import build as B
 from oracle import oracle as ora
 
 from .cases import make
+from .test_gpu_options import _discharges
 
 pytestmark = pytest.mark.gpu
 
 
-def _write_input(path, case, params, steps):
+def _write_input(path, case, params, steps, disch=None):
     ipar = np.array([params["AdvMethodH"], params["TVDLimitationH"], params["AdvMethodV"],
                      params["TVDLimitationV"], int(params["Upwind2H"]), int(params["Upwind2V"]),
                      int(params["NullDif"]), int(params["Optimize"])], dtype=np.int32)
@@ -50,11 +52,21 @@ def _write_input(path, case, params, steps):
             put(h[n], np.float64)
         put(h["WaterPoints3D"], np.int32)
         put(case.tracers[0], np.float64)
+        if disch is None:
+            np.array([0, 0], dtype=np.int32).tofile(f)
+            return
+        np.array([len(disch["vert"]), len(disch["flow"])], dtype=np.int32).tofile(f)
+        for n in ("flow", "conc", "concMF"):
+            put(disch[n], np.float64)
+        for n in ("I", "J", "K", "Kmin", "Kmax", "vert", "ignore", "ncells", "bypass"):
+            put(disch[n], np.int32)
 
 
-@pytest.mark.parametrize("name,implicit", [("tvd_sigma", None), ("upwind_land_ragged", None),
-                                           ("tvd_sigma", "x"), ("upwind_land_ragged", "y")])
-def test_fortran_driver_matches_oracle(name, implicit, tmp_path):
+@pytest.mark.parametrize("name,implicit,discharges", [
+    ("tvd_sigma", None, False), ("upwind_land_ragged", None, False), ("tvd_sigma", "x", False),
+    ("upwind_land_ragged", "y", False), ("tvd_land_sigma", None, True),
+    ("tvd_land_sigma", "x", True)])
+def test_fortran_driver_matches_oracle(name, implicit, discharges, tmp_path):
     if not os.path.exists(B.DRIVER):
         pytest.fail(f"Fortran driver not built: {B.DRIVER} (python -m mohid_amd.build)")
     case = make(name)
@@ -64,8 +76,11 @@ def test_fortran_driver_matches_oracle(name, implicit, tmp_path):
         params["ImpExp_AdvXX"] = float(implicit == "x")
         params["ImpExp_AdvYY"] = float(implicit == "y")
     steps = 4
+    # SetDischargesGPU / UnSetDischargesGPU around every call (IgnoreDisch and
+    # ByPass go through the Fortran logical arrays)
+    disch = _discharges(case, np.random.default_rng(3), 1)[0] if discharges else None
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
-    _write_input(fin, case, params, steps)
+    _write_input(fin, case, params, steps, disch)
     print(f"fortran driver: {name}, {steps} steps", flush=True)
     res = subprocess.run([B.DRIVER, str(fin), str(fout)], capture_output=True, text=True,
                          timeout=120)
@@ -74,9 +89,13 @@ def test_fortran_driver_matches_oracle(name, implicit, tmp_path):
     ref = np.array(case.tracers[0], order="F", copy=True)
     inst = ora.Instance(case, nthreads=1)
     for s in range(steps):
+        if disch is not None:
+            assert inst.set_discharges(disch) == 0
         assert inst.step(ref, first_property=True, now=(s + 1) * params["DTProp"],
                          ImpExp_AdvXX=params["ImpExp_AdvXX"],
                          ImpExp_AdvYY=params["ImpExp_AdvYY"]) == 0
+        if disch is not None:
+            assert inst.unset_discharges() == 0
     inst.close()
     wp = case.hydro["WaterPoints3D"][:, :, case.worksize[5]] == 1
     solved = np.zeros(case.shape, dtype=bool)
@@ -85,7 +104,7 @@ def test_fortran_driver_matches_oracle(name, implicit, tmp_path):
         solved[...] = True          # the line solves also write the lines' other cells
     den = np.maximum(np.abs(ref[solved]), 1e-300)
     err = float(np.max(np.abs(got[solved] - ref[solved]) / den))
-    print(f"{name} implicit={implicit}: fortran driver max rel err {err:.3e}, "
+    print(f"{name} implicit={implicit} discharges={discharges}: fortran driver max rel err {err:.3e}, "
           f"bit-identical={bool(np.array_equal(got, ref))}")
     assert err <= 1e-12
     assert np.array_equal(got[~solved], ref[~solved])

@@ -1,4 +1,5 @@
-"""NoDifFlux faces and small-depth columns on the GPU against the oracle.
+"""NoDifFlux faces, small-depth columns and discharges on the GPU against the
+oracle.
 
 NoDifFlux zeroes the horizontal diffusion coefficient of the flagged U/V
 faces (ModuleAdvectionDiffusion.F90:2496-2531, non-optimised path) and the
@@ -7,8 +8,10 @@ SmallDepths switches off vertical diffusion in flagged columns (AD:2680-2699,
 2733, 2813-2907).  The reference keeps the horizontal coefficients of the
 previous call of the same step (Set_Internal_State, AD:5746-5835), so a
 property without NoDifFlux that follows one with it in the same step sees
-that property's zeros; the GPU keeps them the same way.  Bar: |gpu - oracle|
-<= 1e-12 |oracle| on every cell.
+that property's zeros; the GPU keeps them the same way.  Discharges
+(SetDischarges/UnSetDischarges, AD:978-1034, 4032-4120) are set before each
+property's call with that property's concentrations and unset after it, as
+WP:14763-14830 does.  Bar: |gpu - oracle| <= 1e-12 |oracle| on every cell.
 """
 import numpy as np
 import pytest
@@ -140,4 +143,130 @@ def test_noadvflux_stays_on_the_cpu():
         gpu(p, case.hydro, dict(case.params, NoDifFlux=True))
     assert ei.value.code == L.MGPU_ERR_BADARG
     assert np.array_equal(p, case.tracers[0])
+    gpu.kill()
+
+
+def _discharges(case, rng, T, non_open=None):
+    """Five discharges: point, uniform over the column, uniform over an
+    explicit range (bypass), a withdrawal, and an ignored one; two of them
+    share a cell.  Concentrations per tracer (DischConc is per property)."""
+    KUB = case.worksize[5]
+    wp = case.hydro["WaterPoints3D"][:, :, KUB] == 1
+    ii, jj = np.nonzero(wp)
+    pick = rng.choice(len(ii), size=4, replace=False)
+    cells = [(int(ii[p]), int(jj[p])) for p in pick]
+    cells.insert(2, cells[0])                       # the third shares the first's column
+    kinds = ["point", "uniform", "range", "withdraw", "point"]
+    ignore = [0, 0, 0, 0, 1]
+    d = {n: [] for n in ("flow", "I", "J", "K", "Kmin", "Kmax")}
+    conc = [[] for _ in range(T)]
+    mf = [[] for _ in range(T)]
+    for q, (kind, (i, j)) in enumerate(zip(kinds, cells)):
+        if ignore[q]:
+            continue
+        kf = int(case.geom["KFloorZ"][i, j])
+        d["I"].append(i); d["J"].append(j)
+        d["K"].append(KUB if kind == "point" else int(rng.integers(kf, KUB + 1)))
+        d["Kmin"].append(max(kf, KUB - 2) if kind == "range" else -9999999)
+        d["Kmax"].append(KUB if kind == "range" else -9999999)
+        d["flow"].append(-0.6 if kind == "withdraw" else float(rng.uniform(0.5, 3.0)))
+        for t in range(T):
+            conc[t].append(float(rng.uniform(1.0, 20.0)))
+            mf[t].append(float(rng.uniform(0.5, 1.5)))
+    if non_open is not None:
+        i, j, k = non_open
+        d["I"].append(i); d["J"].append(j); d["K"].append(k)
+        d["Kmin"].append(-9999999); d["Kmax"].append(-9999999)
+        d["flow"].append(1.25)
+        for t in range(T):
+            conc[t].append(9.0)
+            mf[t].append(1.0)
+        kinds.append("point"); ignore.append(0)
+    d["vert"] = [5 if k in ("uniform", "range") else 1 for k in kinds]
+    d["ignore"] = ignore
+    d["ncells"] = [1] * len(kinds)
+    d["bypass"] = [1 if k == "range" else 0 for k in kinds]
+    return [dict(d, conc=conc[t], concMF=mf[t]) for t in range(T)]
+
+
+def _dry_column(case, i, j):
+    """The case's hydro with column (i, j) dry: water points that are not open,
+    with no computed face on any side (AD:4104-4117 treats a discharge into
+    such a cell without dilution)."""
+    hy = {n: np.array(v, order="F", copy=True) for n, v in case.hydro.items()}
+    hy["OpenPoints3D"][i, j, :] = 0
+    for cf, q, cells in (("ComputeFacesU3D", "Wflux_X", ((i, j), (i, j + 1))),
+                         ("ComputeFacesV3D", "Wflux_Y", ((i, j), (i + 1, j))),
+                         ("ComputeFacesW3D", "Wflux_Z", ((i, j),))):
+        for c in cells:
+            hy[cf][c[0], c[1], :] = 0
+            hy[q][c[0], c[1], :] = 0.0
+    return hy
+
+
+@pytest.mark.parametrize("name,optimize,implicit", [
+    ("tvd_land_sigma", False, None), ("tvd_ragged_opt", True, None),
+    ("upwind_land_ragged", False, None), ("tvd_ragged_opt", True, "x"),
+    ("tvd_land_sigma", False, "y")])
+def test_discharges_drop_in(name, optimize, implicit):
+    L, AD, _ = _mods()
+    case = make(name)
+    T = len(case.tracers)
+    rng = np.random.default_rng(17)
+    KUB = case.worksize[5]
+    hy = case.hydro
+    non_open = None
+    if implicit is None:
+        wp = case.hydro["WaterPoints3D"][:, :, KUB] == 1
+        ii, jj = np.nonzero(wp)
+        i, j = int(ii[len(ii) // 2]), int(jj[len(jj) // 2])
+        non_open = (i, j, KUB)
+        hy = _dry_column(case, i, j)
+    sets = _discharges(case, rng, T, non_open=non_open)
+    params = dict(case.params, Optimize=optimize)
+    if params["AdvMethodH"] in (C.UPWIND_ORDER2, C.UPWIND_ORDER3) and implicit:
+        params["AdvMethodH"] = C.UPWIND_ORDER1
+    gpu = AD(case.size, case.worksize, case.geom)
+    inst = ora.Instance(case, nthreads=1)
+    pg = [np.array(p, order="F", copy=True) for p in case.tracers]
+    po = [np.array(p, order="F", copy=True) for p in case.tracers]
+    worst, bitwise = 0.0, True
+    for s in range(4):
+        prm = dict(params)
+        if implicit:
+            xx = (s % 2 == 0) == (implicit == "x")
+            prm.update(ImpExp_AdvXX=float(xx), ImpExp_AdvYY=float(not xx))
+        for t in range(T):
+            # WP's pattern: set before each property's call, unset after (WP:14763, 14830)
+            assert inst.set_discharges(sets[t]) == 0
+            assert inst.step(po[t], first_property=t == 0, now=(s + 1) * prm["DTProp"],
+                             hydro=hy, **prm) == 0
+            assert inst.unset_discharges() == 0
+            gpu.set_discharges(sets[t])
+            gpu(pg[t], hy, prm, first_property=t == 0)
+            gpu.unset_discharges()
+            err = _rel_err(pg[t], po[t])
+            worst = max(worst, err)
+            assert err <= RTOL, (s, t, err)
+            bitwise &= bool(np.array_equal(pg[t], po[t]))
+    # after unset, a call has no discharges
+    q = np.array(po[0], order="F", copy=True)
+    assert inst.step(q, first_property=True, now=9 * params["DTProp"], hydro=hy, **params) == 0
+    gpu(pg[0], hy, params, first_property=True)
+    assert _rel_err(pg[0], q) <= RTOL
+    gpu.kill()
+    inst.close()
+    print(f"{name} optimize={optimize} implicit={implicit} discharges: {worst:.3e} "
+          f"bit-identical={bitwise}")
+
+
+def test_discharges_reject_bad_cells():
+    L, AD, _ = _mods()
+    case = make("upwind_flat")
+    gpu = AD(case.size, case.worksize, case.geom)
+    d = dict(flow=[1.0], conc=[1.0], concMF=[1.0], I=[0], J=[1], K=[1], Kmin=[-9999999],
+             Kmax=[-9999999], vert=[1], ignore=[0], ncells=[1], bypass=[0])
+    with pytest.raises(L.MgpuError) as ei:          # I outside the WorkSize
+        gpu.set_discharges(d)
+    assert ei.value.code == L.MGPU_ERR_BADARG
     gpu.kill()

@@ -32,3 +32,12 @@ IMPLICIT_SCHEME = 1.0
 SUCCESS_ = 0
 OFF_ERR_ = 1
 UNKNOWN_ = -99
+
+# ModuleAdvectionDiffusion.F90:152-158 boundary conditions
+MASS_CONSERVATION = 1
+IMPOSED_VALUE = 2
+NULL_GRADIENT = 4
+SUB_MODEL = 5
+ORLANSKI = 6
+MASS_CONSERV_NULL_GRAD = 7
+CYCLIC = 8

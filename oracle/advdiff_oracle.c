@@ -48,6 +48,14 @@
 #define IMPLICIT 1.0
 #define DISCH_UNIFORM 5          /* GD:1006 DischUniform_ */
 #define FILL_INT (-9999999)      /* GD:131 FillValueInt */
+/* AD:152-158 boundary condition ids */
+#define BC_MASSCONSERVATION 1
+#define BC_IMPOSEDVALUE 2
+#define BC_NULLGRADIENT 4
+#define BC_SUBMODEL 5
+#define BC_ORLANSKI 6
+#define BC_MASSCONSERVNULLGRAD 7
+#define BC_CYCLIC 8
 
 /* SetDischarges arrays (AD:270-284) */
 typedef struct {
@@ -1235,6 +1243,177 @@ static int vertical_advection(ora_instance *me, const ora_args *a, const double 
 }
 
 /* AD:1624-1922 AdvectionDiffusionIteration */
+/* ------------------------------------------------------------------------ */
+/* Open boundary.  The water flux into a boundary cell, FluxAtOpenBoundary
+ * (AD:5697-5731): the net face flows minus the volume change per second. */
+static double flux_at_open_boundary(const ora_instance *me, const ora_args *a, int i, int j,
+                                    int k)
+{
+    const long q = I3(me, i, j, k), qu = I3(me, i, j + 1, k), qv = I3(me, i + 1, j, k);
+    const long qw = I3(me, i, j, k + 1);
+    return a->Wflux_X[q] * a->ComputeFacesU3D[q] - a->Wflux_X[qu] * a->ComputeFacesU3D[qu] +
+           a->Wflux_Y[q] * a->ComputeFacesV3D[q] - a->Wflux_Y[qv] * a->ComputeFacesV3D[qv] +
+           a->Wflux_Z[q] * a->ComputeFacesW3D[q] - a->Wflux_Z[qw] * a->ComputeFacesW3D[qw] -
+           (a->VolumeZ[q] - a->VolumeZOld[q]) / a->DTProp;
+}
+
+/* NullGradProp (AD:2071-2110): the mean of the neighbours across computed
+ * U/V faces; returns 0 (not found) when no face computes. */
+static int null_grad_prop(const ora_instance *me, const ora_args *a, const double *P, int i,
+                          int j, int k, double *out)
+{
+    const int32_t *cu = a->ComputeFacesU3D, *cv = a->ComputeFacesV3D;
+    const long q = I3(me, i, j, k);
+    const long qip = I3(me, i + 1, j, k), qim = I3(me, i - 1, j, k);
+    const long qjp = I3(me, i, j + 1, k), qjm = I3(me, i, j - 1, k);
+    const int aux = cv[qip] + cv[q] + cu[qjp] + cu[q];
+    if (aux <= 0) return 0;
+    *out = (P[qip] * cv[qip] + P[qim] * cv[q] + P[qjp] * cu[qjp] + P[qjm] * cu[q]) / (double)aux;
+    return 1;
+}
+
+/* OpenBoundaryCondition (AD:5369-5665) without Orlanski: the rows of the
+ * open boundary cells, the last change before the solve.  P is PROP at this
+ * point of the step (after the line solves in 3-D implicit horizontal
+ * advection). */
+static void open_boundary_condition(ora_instance *me, const ora_args *a, const double *P)
+{
+    const ora_size3d *w = &me->ws;
+    const int bc = a->BoundaryCondition;
+    const int32_t *bp = a->BoundaryPoints2D, *op = a->OpenPoints3D;
+    const double *ref = a->ReferenceProp;
+    const double tdec = 1.0 / (1.0 + a->DecayTime / a->DTProp);
+    int i, j, k;
+    for (j = w->JLB; j <= w->JUB; j++)
+        for (i = w->ILB; i <= w->IUB; i++) {
+            if (bp[I2(me, i, j)] != 1) continue;
+            const int kb = abs(me->g.KFloorZ[I2(me, i, j)]);
+            for (k = kb; k <= w->KUB; k++) {
+                const long q = I3(me, i, j, k);
+                if (op[q] != 1) continue;
+                const double dtv = a->DTProp / a->VolumeZ[q];
+                double ext = 0.0;
+                if (bc == BC_IMPOSEDVALUE || bc == BC_SUBMODEL) {
+                    /* interior neighbours: open and not boundary (AD:5456-5488) */
+                    const long qip = I3(me, i + 1, j, k), qim = I3(me, i - 1, j, k);
+                    const long qjp = I3(me, i, j + 1, k), qjm = I3(me, i, j - 1, k);
+                    const double A1 = (double)(op[qip] * (1 - bp[I2(me, i + 1, j)]));
+                    const double A2 = (double)(op[qim] * (1 - bp[I2(me, i - 1, j)]));
+                    const double A3 = (double)(op[qjp] * (1 - bp[I2(me, i, j + 1)]));
+                    const double A4 = (double)(op[qjm] * (1 - bp[I2(me, i, j - 1)]));
+                    const double At = A1 + A2 + A3 + A4;
+                    if (At > 0.0) {
+                        const double in = (P[qip] * A1 + P[qim] * A2 + P[qjp] * A3 + P[qjm] * A4) / At;
+                        ext = in * (1.0 - tdec) + ref[q] * tdec;
+                    } else {
+                        ext = ref[q];
+                    }
+                }
+                if (bc == BC_MASSCONSERVATION || bc == BC_MASSCONSERVNULLGRAD) {
+                    double wf;
+                    if (bc == BC_MASSCONSERVATION) ext = P[q] * (1.0 - tdec) + ref[q] * tdec;
+                    wf = flux_at_open_boundary(me, a, i, j, k);
+                    if ((wf < 0.0) != (g_mistake == 15)) {          /* water flowing in */
+                        if (bc == BC_MASSCONSERVATION) {
+                            me->TI[q] = me->TI[q] - wf * ext * dtv;
+                        } else {
+                            double v;
+                            me->TI[q] = null_grad_prop(me, a, P, i, j, k, &v) ? v : P[q];
+                            me->E[q] = 1.0;
+                            me->D[q] = 0.0;
+                            me->F[q] = 0.0;
+                        }
+                    } else {
+                        me->E[q] = me->E[q] + wf * dtv;
+                    }
+                } else if (bc == BC_IMPOSEDVALUE || bc == BC_SUBMODEL) {
+                    me->TI[q] = ext;
+                    me->D[q] = 0.0;
+                    me->E[q] = 1.0;
+                    me->F[q] = 0.0;
+                } else {                                   /* NullGradient_, CyclicBoundary_ */
+                    me->TI[q] = P[q];
+                    me->D[q] = 0.0;
+                    me->E[q] = 1.0;
+                    me->F[q] = 0.0;
+                }
+            }
+        }
+}
+
+/* ImposeNullGradient (AD:1926-1987): every boundary cell takes the mean of
+ * its face neighbours, swept in place over j then i, so a cell sees the
+ * values its boundary neighbours at i-1 and j-1 have just taken. */
+static void impose_null_gradient(ora_instance *me, const ora_args *a, double *P)
+{
+    const ora_size3d *w = &me->ws;
+    const double *src = P;
+    double *old = NULL;
+    int i, j, k;
+    if (g_mistake == 14) {
+        old = (double *)malloc(sizeof(double) * (size_t)me->n3);
+        memcpy(old, P, sizeof(double) * (size_t)me->n3);
+        src = old;
+    }
+    for (j = w->JLB; j <= w->JUB; j++)
+        for (i = w->ILB; i <= w->IUB; i++) {
+            if (a->BoundaryPoints2D[I2(me, i, j)] != 1) continue;
+            for (k = abs(me->g.KFloorZ[I2(me, i, j)]); k <= w->KUB; k++) {
+                double v;
+                if (null_grad_prop(me, a, src, i, j, k, &v)) P[I3(me, i, j, k)] = v;
+            }
+        }
+    free(old);
+}
+
+/* Prop_CyclicBoundary (AD:2116-2224): boundary columns take the reference
+ * field, then rows (columns) whose both ends are boundary points copy the
+ * values just inside the opposite end. */
+static int cyclic_boundary(ora_instance *me, const ora_args *a, double *P)
+{
+    const ora_size3d *w = &me->ws;
+    const int32_t *bp = a->BoundaryPoints2D, *kf = me->g.KFloorZ;
+    const int sh = g_mistake == 16 ? 1 : 0;
+    int i, j, k;
+    /* the copies read levels from KFloorZ up: below Size they have no cell */
+    for (i = w->ILB + 1; i <= w->IUB - 1; i++)
+        if (bp[I2(me, i, w->JLB)] == 1 && bp[I2(me, i, w->JUB)] == 1 &&
+            (kf[I2(me, i, w->JUB - 1)] < me->size.KLB || kf[I2(me, i, w->JLB + 1)] < me->size.KLB))
+            return ORA_ERR_BADARG;
+    for (j = w->JLB + 1; j <= w->JUB - 1; j++)
+        if (bp[I2(me, w->ILB, j)] == 1 && bp[I2(me, w->IUB, j)] == 1 &&
+            (kf[I2(me, w->IUB - 1, j)] < me->size.KLB || kf[I2(me, w->ILB + 1, j)] < me->size.KLB))
+            return ORA_ERR_BADARG;
+    for (j = w->JLB; j <= w->JUB; j++)
+        for (i = w->ILB; i <= w->IUB; i++)
+            if (bp[I2(me, i, j)] == 1)
+                for (k = w->KLB; k <= w->KUB; k++)
+                    P[I3(me, i, j, k)] = a->ReferenceProp[I3(me, i, j, k)];
+    for (i = w->ILB + 1; i <= w->IUB - 1; i++) {
+        if (!(bp[I2(me, i, w->JLB)] == 1 && bp[I2(me, i, w->JUB)] == 1)) continue;
+        for (k = kf[I2(me, i, w->JUB - 1)]; k <= w->KUB; k++)
+            P[I3(me, i, w->JLB, k)] = P[I3(me, i, w->JUB - 1 + sh, k)];
+        for (k = kf[I2(me, i, w->JLB + 1)]; k <= w->KUB; k++)
+            P[I3(me, i, w->JUB, k)] = P[I3(me, i, w->JLB + 1, k)];
+    }
+    for (j = w->JLB + 1; j <= w->JUB - 1; j++) {
+        if (!(bp[I2(me, w->ILB, j)] == 1 && bp[I2(me, w->IUB, j)] == 1)) continue;
+        for (k = kf[I2(me, w->IUB - 1, j)]; k <= w->KUB; k++)
+            P[I3(me, w->ILB, j, k)] = P[I3(me, w->IUB - 1, j, k)];
+        for (k = kf[I2(me, w->ILB + 1, j)]; k <= w->KUB; k++)
+            P[I3(me, w->IUB, j, k)] = P[I3(me, w->ILB + 1, j, k)];
+    }
+    return 0;
+}
+
+/* AD:1874-1880, after the solve */
+static int boundary_post(ora_instance *me, const ora_args *a, double *P)
+{
+    if (a->BoundaryCondition == BC_NULLGRADIENT) impose_null_gradient(me, a, P);
+    else if (a->BoundaryCondition == BC_CYCLIC) return cyclic_boundary(me, a, P);
+    return 0;
+}
+
 static int iteration(ora_instance *me, const ora_args *a, double *P)
 {
     const ora_size3d *w = &me->ws;
@@ -1307,6 +1486,8 @@ static int iteration(ora_instance *me, const ora_args *a, double *P)
             if (rc) return rc;
         }
     }
+    /* AD:1745-1746: the open boundary rows, the last change before the solve */
+    if (a->ReferenceProp) open_boundary_condition(me, a, P);
     /* AD:1753: TI = null_real on land points */
     {
         long q;
@@ -1316,12 +1497,13 @@ static int iteration(ora_instance *me, const ora_args *a, double *P)
     }
     /* AD:1758-1843: one layer with implicit horizontal advection solves the
      * lines instead of the columns */
-    if (w->KUB == 1 && impXX)
-        return ora_thomas3d(&me->size, w->ILB, w->IUB, w->JLB, w->JUB, w->KLB, w->KUB, 0, 1,
-                            me->D, me->E, me->F, me->TI, P, me->vecW, me->vecG, NULL);
-    if (w->KUB == 1 && impYY)
-        return ora_thomas3d(&me->size, w->JLB, w->JUB, w->ILB, w->IUB, w->KLB, w->KUB, 1, 0,
-                            me->D, me->E, me->F, me->TI, P, me->vecW, me->vecG, NULL);
+    if (w->KUB == 1 && (impXX || impYY)) {
+        rc = impXX ? ora_thomas3d(&me->size, w->ILB, w->IUB, w->JLB, w->JUB, w->KLB, w->KUB,
+                                  0, 1, me->D, me->E, me->F, me->TI, P, me->vecW, me->vecG, NULL)
+                   : ora_thomas3d(&me->size, w->JLB, w->JUB, w->ILB, w->IUB, w->KLB, w->KUB,
+                                  1, 0, me->D, me->E, me->F, me->TI, P, me->vecW, me->vecG, NULL);
+        return rc ? rc : boundary_post(me, a, P);
+    }
     /* AD:1858-1867 THOMASZ_NewType2 over WorkSize, OpenMP over j */
     {
         long nI = me->nI, nJ = me->nJ;
@@ -1344,7 +1526,7 @@ static int iteration(ora_instance *me, const ora_args *a, double *P)
                                    me->F, me->TI, P, W, G);
         }
     }
-    return 0;
+    return boundary_post(me, a, P);
 }
 
 /* AD:1108-1509 AdvectionDiffusion */
@@ -1383,6 +1565,20 @@ int ora_advection_diffusion(ora_instance *me, const ora_args *a, double *prop)
         if (!optimize || a->FirstProperty) convert_dif_vertical(me, &aa);
     }
     if (a->NoDifFlux && (!a->NoFluxU || !a->NoFluxV || !a->NoFluxW)) return ORA_ERR_BADARG;
+    /* Set_Internal_State ERR01 (AD:5816-5829): an open boundary needs a
+     * known condition; Orlanski's radiation is not restated */
+    if (a->ReferenceProp) {
+        const int bc = a->BoundaryCondition;
+        if (bc != BC_MASSCONSERVATION && bc != BC_IMPOSEDVALUE && bc != BC_SUBMODEL &&
+            bc != BC_ORLANSKI && bc != BC_NULLGRADIENT && bc != BC_CYCLIC &&
+            bc != BC_MASSCONSERVNULLGRAD)
+            return ORA_ERR_OPTION;
+        if (bc == BC_ORLANSKI) return ORA_ERR_UNSUPPORTED;
+    }
+    if ((a->ReferenceProp || a->BoundaryCondition == BC_NULLGRADIENT ||
+         a->BoundaryCondition == BC_CYCLIC) && !a->BoundaryPoints2D)
+        return ORA_ERR_BADARG;
+    if (a->BoundaryCondition == BC_CYCLIC && !a->ReferenceProp) return ORA_ERR_BADARG;
     if (me->st_hdif) {
         if (optimize) {
             if (a->FirstProperty) convert_visc_dif_horizontal(me, &aa, 0, 0);

@@ -76,6 +76,12 @@ typedef struct {
     int32_t NoDifFlux;
     const int32_t *NoFluxU, *NoFluxV, *NoFluxW;   /* 3-D, read with NoDifFlux */
     const int32_t *SmallDepths;                   /* 2-D (Size i, j), logical as 0/1 */
+    /* open boundary (AD:5369-5742, on when ReferenceProp is given, AD:5816)
+     * and the post-solve conditions (AD:1874-1880) */
+    const double *ReferenceProp;                  /* 3-D */
+    int32_t BoundaryCondition;                    /* AD:152-158 ids; 0 when absent */
+    double DecayTime;
+    const int32_t *BoundaryPoints2D;              /* 2-D (Size i, j) */
 } ora_args;
 
 typedef struct ora_instance ora_instance;
@@ -121,7 +127,12 @@ const double *ora_internal(const ora_instance *me, const char *name);
  *  11 small-depth columns keep their vertical diffusion in VerticalDiffusion2
  *  12 NoDifFlux ignores the V faces
  *  13 a uniform discharge spread by the cell's thickness over the column's
- *     total depth instead of the discharge cells' */
+ *     total depth instead of the discharge cells'
+ *  14 ImposeNullGradient reads the boundary neighbours' values from before
+ *     the sweep (a Jacobi update instead of the reference's in-place sweep)
+ *  15 open boundary inflow taken with the sign of an outflow
+ *  16 cyclic boundary copies the row next to the boundary instead of the
+ *     one before the opposite boundary */
 void ora_plant_mistake(int id);
 
 /* THOMASZ_NewType2 (ModuleFunctions.F90:4026-4123) on explicit arrays.

@@ -54,6 +54,8 @@ class Args(ct.Structure):
         ("Optimize", ct.c_int32), ("FirstProperty", ct.c_int32), ("Now", ct.c_double),
         ("NoDifFlux", ct.c_int32), ("NoFluxU", _ip), ("NoFluxV", _ip), ("NoFluxW", _ip),
         ("SmallDepths", _ip),
+        ("ReferenceProp", _dp), ("BoundaryCondition", ct.c_int32), ("DecayTime", ct.c_double),
+        ("BoundaryPoints2D", _ip),
     ]
 
 
@@ -182,6 +184,12 @@ class Instance:
         def opt_i(n):
             return i(n) if hy.get(n) is not None else None
 
+        ref = p.get("ReferenceProp")
+        if ref is not None:
+            ref = np.asfortranarray(ref, dtype=np.float64)
+            keep.append(ref)
+            ref = _ptr(ref, _dp)
+
         args = Args(p["schmidt_H"], p["SchmidtCoef_V"], p["SchmidtBackground_V"],
                     p["AdvMethodH"], p["TVDLimitationH"], p["AdvMethodV"], p["TVDLimitationV"],
                     int(p["Upwind2H"]), int(p["Upwind2V"]), p["VolumeRelMax"], p["DTProp"],
@@ -192,7 +200,9 @@ class Instance:
                     i("ComputeFacesV3D"), i("ComputeFacesW3D"), i("WaterPoints3D"),
                     d("Visc_H"), d("Diff_V"), int(p["Optimize"]), int(first_property),
                     self.now if now is None else now, int(bool(p.get("NoDifFlux", False))),
-                    opt_i("NoFluxU"), opt_i("NoFluxV"), opt_i("NoFluxW"), opt_i("SmallDepths"))
+                    opt_i("NoFluxU"), opt_i("NoFluxV"), opt_i("NoFluxW"), opt_i("SmallDepths"),
+                    ref, int(p.get("BoundaryCondition", 0)),
+                    float(p.get("DecayTime", -9.9e15)), opt_i("BoundaryPoints2D"))
         assert prop.flags.f_contiguous and prop.dtype == np.float64
         return lib().ora_advection_diffusion(self.h, ct.byref(args), _ptr(prop, _dp))
 

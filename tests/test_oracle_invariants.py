@@ -32,6 +32,19 @@ mathematics fixes:
   fractional step (line solves, then the dense vertical diffusion solve of
   their result), and closed-basin mass conservation on every small case.
 
+* open boundaries (AD:5369-5742) and the post-solve conditions
+  (AD:1874-1880, 1926-2224): an imposed value with no decay time makes
+  every open boundary cell the reference value and leaves every other cell
+  as the step without boundaries left it; with an infinite decay time a
+  boundary cell takes the mean of its open interior neighbours (a constant
+  there); mass conservation along a channel gains exactly the inflow
+  Q dt C_ref and loses the implicit outflow Q dt C_east; the null-gradient
+  sweep equals the triangular solve of its equations (each boundary cell
+  the mean of its face neighbours, the ones swept before it at their new
+  values); the cyclic condition makes both ends of every cyclic row the
+  values just inside the opposite end, and the other boundary cells the
+  reference.
+
 ``test_planted_mistakes_are_caught`` plants each of a list of plausible
 mistakes in the oracle (a flux sign, the upwind side, the volume ratio, a
 wrong cell volume, a Thomas index, the vertical distance index) and requires
@@ -615,6 +628,217 @@ def test_closed_basin_conserves_mass_implicit(name, implicit):
 
 
 # ---------------------------------------------------------------------------
+# open boundaries
+def _boundary_points(case, interior=0, seed=0):
+    """BoundaryPoints2D as ModuleHorizontalMap builds it (HM:545-611): the
+    water points on the WorkSize's edges, plus `interior` random interior
+    water points (next to an exterior cell in the reference)."""
+    ws = case.worksize
+    KUB = ws[5]
+    wp = case.hydro["WaterPoints3D"][:, :, KUB] == 1
+    edge = np.zeros_like(wp)
+    edge[ws[0], ws[2]:ws[3] + 1] = True
+    edge[ws[1], ws[2]:ws[3] + 1] = True
+    edge[ws[0]:ws[1] + 1, ws[2]] = True
+    edge[ws[0]:ws[1] + 1, ws[3]] = True
+    bp = edge & wp
+    if interior:
+        rng = np.random.default_rng(seed)
+        ii, jj = np.nonzero(wp & ~edge)
+        pick = rng.choice(len(ii), size=interior, replace=False)
+        bp[ii[pick], jj[pick]] = True
+    return np.asfortranarray(bp.astype(np.int32))
+
+
+def _boundary_run(case, c0, bp, hydro=None, **params):
+    hy = dict(case.hydro if hydro is None else hydro, BoundaryPoints2D=bp)
+    return _run(case, c0.copy(order="F"), hydro=hy, **params)
+
+
+def _bnd_cells(case, bp):
+    """Open boundary cells: boundary columns, levels |KFloorZ|..KUB, open."""
+    KUB = case.worksize[5]
+    kf = np.abs(case.geom["KFloorZ"])
+    k = np.arange(case.shape[2])[None, None, :]
+    return ((bp[:, :, None] == 1) & (k >= kf[:, :, None]) & (k <= KUB) &
+            (case.hydro["OpenPoints3D"] == 1))
+
+
+def _imposed_value_err(name="tvd_land_sigma"):
+    """ImposedValue with DecayTime = 0: open boundary cells become the
+    reference value exactly; every other cell is the step without boundary."""
+    case = make(name)
+    bp = _boundary_points(case, interior=3, seed=2)
+    rng = np.random.default_rng(4)
+    ref = np.asfortranarray(rng.uniform(1.0, 9.0, size=case.shape))
+    c0 = np.array(case.tracers[0], order="F", copy=True)
+    plain = _run(case, c0.copy(order="F"))
+    got = _boundary_run(case, c0, bp, ReferenceProp=ref, BoundaryCondition=C.IMPOSED_VALUE,
+                        DecayTime=0.0)
+    b = _bnd_cells(case, bp)
+    other = ~(bp[:, :, None] == 1) | ~b
+    other &= ~b
+    err = float(np.max(np.abs(got[b] - ref[b])))
+    return max(err, float(np.max(np.abs(got[other] - plain[other]))))
+
+
+def _imposed_interior_mean_err():
+    """ImposedValue with an infinite decay time: a boundary cell with open
+    interior neighbours takes their mean (1.25 everywhere inside), one
+    without takes the reference value."""
+    case = make("upwind_land_ragged")
+    hy = _still_hydro(case)
+    bp = _boundary_points(case)
+    b = _bnd_cells(case, bp)
+    rng = np.random.default_rng(6)
+    ref = np.asfortranarray(rng.uniform(3.0, 9.0, size=case.shape))
+    c0 = np.asfortranarray(np.where(bp[:, :, None] == 1,
+                                    rng.uniform(3.0, 9.0, size=case.shape), 1.25))
+    got = _boundary_run(case, c0, bp, hydro=hy, ReferenceProp=ref,
+                        BoundaryCondition=C.IMPOSED_VALUE, DecayTime=1e300)
+    op = case.hydro["OpenPoints3D"] == 1
+    inner = op & (bp[:, :, None] == 0)
+    nb = np.zeros_like(b)
+    nb[1:-1, 1:-1] = (inner[2:, 1:-1] | inner[:-2, 1:-1] | inner[1:-1, 2:] | inner[1:-1, :-2])
+    want = np.where(nb, 1.25, ref)
+    assert b.any() and (b & nb).any() and (b & ~nb).any()
+    return float(np.max(np.abs(got[b] - want[b])))
+
+
+def _mass_conservation_channel_err(sign):
+    """MassConservation along a channel with a steady flow: the upstream
+    boundary gains Q dt C_ref per cell (an explicit inflow), the downstream
+    one loses Q dt C_new (an implicit outflow); sum(V C) changes by exactly
+    their difference."""
+    case = fx.make_case(fx.config("c1", ni=3, nj=12, nk=2))
+    hy = _still_hydro(case)
+    q = 1500.0 * sign
+    faces = hy["ComputeFacesU3D"] == 1
+    hy["Wflux_X"][faces] = q
+    ws = case.worksize
+    wp = hy["WaterPoints3D"] == 1
+    bp = np.zeros(case.shape[:2], dtype=np.int32, order="F")
+    bp[ws[0]:ws[1] + 1, ws[2]] = 1
+    bp[ws[0]:ws[1] + 1, ws[3]] = 1
+    bp = np.asfortranarray(bp * wp[:, :, ws[5]])
+    rng = np.random.default_rng(9)
+    c0 = np.asfortranarray(np.where(wp, rng.uniform(1.0, 3.0, size=wp.shape), 0.0))
+    R = 7.5
+    ref = np.asfortranarray(np.full(case.shape, R))
+    got = _boundary_run(case, c0, bp, hydro=hy, **UPWIND, ReferenceProp=ref,
+                        BoundaryCondition=C.MASS_CONSERVATION, DecayTime=0.0)
+    V = hy["VolumeZ"]
+    b = _bnd_cells(case, bp)
+    jin, jout = (ws[2], ws[3]) if q > 0 else (ws[3], ws[2])
+    cin = b.copy(); cin[:, [j for j in range(case.shape[1]) if j != jin], :] = False
+    cout = b.copy(); cout[:, [j for j in range(case.shape[1]) if j != jout], :] = False
+    gain = float(np.sum(V[wp] * got[wp]) - np.sum(V[wp] * c0[wp]))
+    want = abs(q) * DT * (R * cin.sum() - float(np.sum(got[cout])))
+    return abs(gain - want) / float(np.sum(V[wp] * c0[wp]))
+
+
+def _null_gradient_err(name, interior):
+    """NullGradient after the solve against the triangular system of its
+    sweep: x_b Aux_b = sum over face neighbours n of P_n, where P_n is the
+    neighbour's new value x_n when n is a boundary cell swept before b
+    (j outer, i inner) and its solved value otherwise."""
+    case = make(name)
+    bp = _boundary_points(case, interior=interior, seed=5)
+    c0 = np.array(case.tracers[0], order="F", copy=True)
+    plain = _run(case, c0.copy(order="F"))
+    got = _boundary_run(case, c0, bp, BoundaryCondition=C.NULL_GRADIENT)
+    hy = case.hydro
+    cu, cv = hy["ComputeFacesU3D"], hy["ComputeFacesV3D"]
+    ws = case.worksize
+    KUB = ws[5]
+    kf = np.abs(case.geom["KFloorZ"])
+    cols = [(i, j) for j in range(ws[2], ws[3] + 1) for i in range(ws[0], ws[1] + 1)
+            if bp[i, j] == 1]
+    order = {c: n for n, c in enumerate(cols)}
+    err = 0.0
+    for k in range(1, KUB + 1):
+        rows = [(i, j) for (i, j) in cols if k >= kf[i, j] and
+                cv[i + 1, j, k] + cv[i, j, k] + cu[i, j + 1, k] + cu[i, j, k] > 0]
+        idx = {c: n for n, c in enumerate(rows)}
+        n = len(rows)
+        if not n:
+            continue
+        A = np.zeros((n, n))
+        rhs = np.zeros(n)
+        for r, (i, j) in enumerate(rows):
+            A[r, r] = cv[i + 1, j, k] + cv[i, j, k] + cu[i, j + 1, k] + cu[i, j, k]
+            for (ni, nj), w in (((i + 1, j), cv[i + 1, j, k]), ((i - 1, j), cv[i, j, k]),
+                                ((i, j + 1), cu[i, j + 1, k]), ((i, j - 1), cu[i, j, k])):
+                if not w:
+                    continue
+                if (ni, nj) in idx and order[(ni, nj)] < order[(i, j)]:
+                    A[r, idx[(ni, nj)]] -= w
+                else:
+                    rhs[r] += w * plain[ni, nj, k]
+        x = np.linalg.solve(A, rhs)
+        for r, (i, j) in enumerate(rows):
+            err = max(err, abs(got[i, j, k] - x[r]) / abs(x[r]))
+    touched = np.zeros(case.shape, dtype=bool)
+    for (i, j) in cols:
+        touched[i, j, :] = True
+    return max(err, float(np.max(np.abs(got[~touched] - plain[~touched]))))
+
+
+def _cyclic_err(mistake_free=True):
+    """CyclicBoundary: both ends of a row (column) whose ends are boundary
+    points take the values just inside the opposite end; the other boundary
+    columns the reference; the rest the step without boundary."""
+    case = make("upwind_flat")
+    bp = _boundary_points(case)
+    ws = case.worksize
+    rng = np.random.default_rng(12)
+    ref = np.asfortranarray(rng.uniform(1.0, 9.0, size=case.shape))
+    c0 = np.array(case.tracers[0], order="F", copy=True)
+    plain = _run(case, c0.copy(order="F"))
+    got = _boundary_run(case, c0, bp, ReferenceProp=ref, BoundaryCondition=C.CYCLIC,
+                        DecayTime=0.0)
+    kf = case.geom["KFloorZ"]
+    KUB = ws[5]
+    want = plain.copy(order="F")
+    want[bp == 1, ws[4]:KUB + 1] = ref[bp == 1, ws[4]:KUB + 1]
+    for i in range(ws[0] + 1, ws[1]):
+        if bp[i, ws[2]] == 1 and bp[i, ws[3]] == 1:
+            for k in range(kf[i, ws[3] - 1], KUB + 1):
+                want[i, ws[2], k] = plain[i, ws[3] - 1, k]
+            for k in range(kf[i, ws[2] + 1], KUB + 1):
+                want[i, ws[3], k] = plain[i, ws[2] + 1, k]
+    for j in range(ws[2] + 1, ws[3]):
+        if bp[ws[0], j] == 1 and bp[ws[1], j] == 1:
+            for k in range(kf[ws[1] - 1, j], KUB + 1):
+                want[ws[0], j, k] = plain[ws[1] - 1, j, k]
+            for k in range(kf[ws[0] + 1, j], KUB + 1):
+                want[ws[1], j, k] = plain[ws[0] + 1, j, k]
+    return float(np.max(np.abs(got - want)))
+
+
+def test_imposed_value_closed_form():
+    assert _imposed_value_err() == 0.0
+
+
+def test_imposed_value_interior_mean():
+    assert _imposed_interior_mean_err() == 0.0
+
+
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_mass_conservation_boundary_budget(sign):
+    assert _mass_conservation_channel_err(sign) <= 1e-14
+
+
+@pytest.mark.parametrize("name,interior", [("tvd_land_sigma", 0), ("upwind_land_ragged", 4)])
+def test_null_gradient_sweep_triangular_solve(name, interior):
+    assert _null_gradient_err(name, interior) <= 1e-14
+
+
+def test_cyclic_boundary_copies():
+    assert _cyclic_err() == 0.0
+
+
+# ---------------------------------------------------------------------------
 def _all_checks():
     """(name, error, tolerance) of every check above."""
     out = []
@@ -642,6 +866,14 @@ def _all_checks():
         out.append((f"nodif wall {axis}", _nodif_wall_err(axis), 1e-13))
     for opt, dc in ((False, 1), (True, 1), (True, 2)):
         out.append((f"small depths opt={opt} docycle={dc}", _small_depth_err(opt, dc), 0.0))
+    out.append(("imposed value", _imposed_value_err(), 0.0))
+    out.append(("imposed interior mean", _imposed_interior_mean_err(), 0.0))
+    for sign in (1.0, -1.0):
+        out.append((f"mass conservation boundary {sign}", _mass_conservation_channel_err(sign),
+                    1e-14))
+    for name, interior in (("tvd_land_sigma", 0), ("upwind_land_ragged", 4)):
+        out.append((f"null gradient {name}", _null_gradient_err(name, interior), 1e-14))
+    out.append(("cyclic", _cyclic_err(), 0.0))
     for name in sorted(SMALL):
         mass, constancy = _invariant_errs(name, steps=2)
         out.append((f"mass {name}", mass, 1e-13))
@@ -652,7 +884,7 @@ def _all_checks():
     return out
 
 
-@pytest.mark.parametrize("mistake", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13])
+@pytest.mark.parametrize("mistake", list(range(1, 17)))
 def test_planted_mistakes_are_caught(mistake):
     L = _lib()
     L.ora_plant_mistake(mistake)

@@ -80,7 +80,8 @@ typedef struct {
 /* Arguments of AdvectionDiffusion (AD:1108-1198) in the reference's order.
  * Options the GPU path does not implement are listed as `unsupported_*`:
  * a non-zero value makes mgpu_ad_step return MGPU_ERR_UNSUPPORTED so the
- * caller falls back to the Fortran path. */
+ * caller falls back to the Fortran path.  The same holds for the Orlanski_
+ * open boundary condition. */
 typedef struct {
     double schmidt_H, SchmidtCoef_V, SchmidtBackground_V;
     int32_t AdvMethodH, TVDLimitationH, AdvMethodV, TVDLimitationV;
@@ -98,9 +99,17 @@ typedef struct {
     const int32_t *WaterPoints3D;
     int32_t Optimize, FirstProperty;
     /* present-and-set optional arguments the GPU path does not offload */
-    int32_t unsupported_CellFluxes, unsupported_ReferenceProp;
-    int32_t unsupported_BoundaryCondition; /* NullGradient_(4) / CyclicBoundary_(8) post-processing */
+    int32_t unsupported_CellFluxes;
     int32_t unsupported_NoAdvFlux;         /* its YY branch zeroes the XX coefficients, AD:4804-4812 */
+    /* open boundary (OpenBoundaryCondition, AD:5369-5665: on when
+     * ReferenceProp is given, AD:5816) and the conditions applied after the
+     * solve (AD:1874-1880: NullGradient_ 4, CyclicBoundary_ 8) */
+    const double *ReferenceProp;           /* 3-D, or null */
+    int32_t BoundaryCondition;             /* AD:152-158 ids; 0 when absent */
+    double DecayTime;
+    const int32_t *BoundaryPoints2D;       /* 2-D (Size i, j), HorizontalMap's GetBoundaries
+                                              (AD:1373); needed with ReferenceProp or
+                                              BoundaryCondition 4 / 8 */
     /* offloaded optional arguments (AD:1287-1335); 0 / null when absent */
     int32_t NoDifFlux;                     /* zero diffusion across the NoFlux faces */
     const int32_t *NoFluxU, *NoFluxV, *NoFluxW;   /* 3-D, 1 = flagged face; needed with NoDifFlux */

@@ -70,8 +70,9 @@ class ADArgs(ct.Structure):
         ("ComputeFacesU3D", _ip), ("ComputeFacesV3D", _ip), ("ComputeFacesW3D", _ip),
         ("Visc_H", _dp), ("Diff_V", _dp), ("WaterPoints3D", _ip),
         ("Optimize", ct.c_int32), ("FirstProperty", ct.c_int32),
-        ("unsupported_CellFluxes", ct.c_int32), ("unsupported_ReferenceProp", ct.c_int32),
-        ("unsupported_BoundaryCondition", ct.c_int32), ("unsupported_NoAdvFlux", ct.c_int32),
+        ("unsupported_CellFluxes", ct.c_int32), ("unsupported_NoAdvFlux", ct.c_int32),
+        ("ReferenceProp", _dp), ("BoundaryCondition", ct.c_int32), ("DecayTime", ct.c_double),
+        ("BoundaryPoints2D", _ip),
         ("NoDifFlux", ct.c_int32), ("NoFluxU", _ip), ("NoFluxV", _ip), ("NoFluxW", _ip),
         ("SmallDepths", _ip),
         ("device_pointers", ct.c_int32),

@@ -19,6 +19,7 @@ from typing import Dict, Optional, Sequence
 import numpy as np
 
 from . import _lib as L
+from . import constants as C
 
 __all__ = ["AdvectionDiffusion", "thomasz", "Transport"]
 
@@ -89,10 +90,14 @@ def _args(hydro: Dict, params: Dict, keep: list, device: bool, optimize: bool,
         return ct.cast(v, ct.POINTER(ct.c_int32))
 
     def opt_ip(n):
-        # optional arrays (NoFluxU/V/W, SmallDepths): null when absent
+        # optional arrays (NoFluxU/V/W, SmallDepths, BoundaryPoints2D): null when absent
         return ip(n) if hydro.get(n) is not None else ct.POINTER(ct.c_int32)()
 
     p = params
+    # ReferenceProp is per property (WP passes its assimilation field)
+    ref = ct.POINTER(ct.c_double)()
+    if p.get("ReferenceProp") is not None:
+        ref = ct.cast(_addr(p["ReferenceProp"], keep, np.float64), ct.POINTER(ct.c_double))
     return L.ADArgs(
         float(p["schmidt_H"]), float(p["SchmidtCoef_V"]), float(p["SchmidtBackground_V"]),
         int(p["AdvMethodH"]), int(p["TVDLimitationH"]), int(p["AdvMethodV"]),
@@ -105,8 +110,9 @@ def _args(hydro: Dict, params: Dict, keep: list, device: bool, optimize: bool,
         ip("OpenPoints3D"), ip("LandPoints3D"), ip("ComputeFacesU3D"), ip("ComputeFacesV3D"),
         ip("ComputeFacesW3D"), dp("Visc_H"), dp("Diff_V"), ip("WaterPoints3D"),
         int(bool(optimize)), int(bool(first_property)),
-        int(bool(p.get("CellFluxes", False))), int(bool(p.get("ReferenceProp", False))),
-        int(p.get("BoundaryCondition", 1) in (4, 8)), int(bool(p.get("NoAdvFlux", False))),
+        int(bool(p.get("CellFluxes", False))), int(bool(p.get("NoAdvFlux", False))),
+        ref, int(p.get("BoundaryCondition", 0)), float(p.get("DecayTime", C.NULL_REAL)),
+        opt_ip("BoundaryPoints2D"),
         int(bool(p.get("NoDifFlux", False))),
         opt_ip("NoFluxU"), opt_ip("NoFluxV"), opt_ip("NoFluxW"), opt_ip("SmallDepths"),
         int(bool(device)))

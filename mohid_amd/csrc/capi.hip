@@ -112,6 +112,9 @@ struct DevHydro {
     // step's call passed them
     int32_t *nfu = nullptr, *nfv = nullptr, *nfw = nullptr, *small = nullptr;
     bool have_nf = false, have_small = false;
+    double *ref = nullptr;        // ReferenceProp
+    int32_t *bp = nullptr;        // BoundaryPoints2D
+    bool have_ref = false, have_bp = false;
 
     int alloc(const Dims &d) {
         const long n3 = d.n3();
@@ -160,13 +163,50 @@ struct DevHydro {
             HIPCHK(hipMemcpyAsync(small, a->SmallDepths, sizeof(int32_t) * (size_t)d.n2(), kind,
                                   s));
         }
+        have_ref = a->ReferenceProp != nullptr;
+        if (have_ref) {
+            if (!ref) {
+                HIPCHK(hipStreamSynchronize(s));
+                if (dalloc(&ref, d.n3())) return MGPU_ERR_NOMEM;
+            }
+            HIPCHK(hipMemcpyAsync(ref, a->ReferenceProp, b, kind, s));
+        }
+        have_bp = a->BoundaryPoints2D != nullptr;
+        if (have_bp) {
+            if (!bp) {
+                HIPCHK(hipStreamSynchronize(s));
+                if (dalloc(&bp, d.n2())) return MGPU_ERR_NOMEM;
+            }
+            HIPCHK(hipMemcpyAsync(bp, a->BoundaryPoints2D, sizeof(int32_t) * (size_t)d.n2(), kind,
+                                  s));
+        }
         return MGPU_SUCCESS;
     }
     void release() {
         dfree(Qx); dfree(Qy); dfree(Qz); dfree(V); dfree(Vold); dfree(visc); dfree(diffv);
         dfree(open); dfree(land); dfree(cfu); dfree(cfv); dfree(cfw); dfree(water);
-        dfree(nfu); dfree(nfv); dfree(nfw); dfree(small);
+        dfree(nfu); dfree(nfv); dfree(nfw); dfree(small); dfree(ref); dfree(bp);
     }
+};
+
+// The boundary columns of a call with a post-solve condition, in the
+// reference's sweep order (launch_bnd_list), allocated on first use.
+struct DevBoundary {
+    uint32_t *list = nullptr;
+    int32_t *meta = nullptr, *pos = nullptr;
+    unsigned long long *total = nullptr;
+    unsigned int *flag = nullptr;
+    int alloc(const Dims &d) {
+        if (list) return MGPU_SUCCESS;
+        const long nw = (long)(d.wIUB - d.wILB + 1) * (d.wJUB - d.wJLB + 1);
+        int rc = dalloc(&list, nw);
+        if (!rc) rc = dalloc(&meta, 3 * nw);
+        if (!rc) rc = dalloc(&pos, d.n2());
+        if (!rc) rc = dalloc(&total, 1);
+        if (!rc) rc = dalloc(&flag, 1);
+        return rc ? MGPU_ERR_NOMEM : MGPU_SUCCESS;
+    }
+    void release() { dfree(list); dfree(meta); dfree(pos); dfree(total); dfree(flag); }
 };
 
 // Derived per-step fields: packed masks, kstart, face conductances, counters.
@@ -261,9 +301,18 @@ int check_options(const mgpu_ad_args *a) {
     // HorizontalAdvectionXX/YY ERR01 (AD:4527, 4887)
     if ((a->ImpExp_AdvXX != 0.0 && !impXX) || (a->ImpExp_AdvYY != 0.0 && !impYY))
         return MGPU_ERR_OPTION;
-    if (a->AdvectionNudging || a->unsupported_CellFluxes || a->unsupported_ReferenceProp ||
-        a->unsupported_BoundaryCondition || a->unsupported_NoAdvFlux)
+    if (a->AdvectionNudging || a->unsupported_CellFluxes || a->unsupported_NoAdvFlux)
         return MGPU_ERR_UNSUPPORTED;
+    // Set_Internal_State ERR01 (AD:5816-5829): an open boundary needs a known
+    // condition; Orlanski's radiation (OrlanskiCelerity2D) is not offloaded
+    const int bc = a->BoundaryCondition;
+    if (a->ReferenceProp) {
+        if (bc != 1 && bc != 2 && bc != 4 && bc != 5 && bc != 6 && bc != 7 && bc != 8)
+            return MGPU_ERR_OPTION;
+        if (bc == 6) return MGPU_ERR_UNSUPPORTED;
+    }
+    if ((a->ReferenceProp || bc == 4 || bc == 8) && !a->BoundaryPoints2D) return MGPU_ERR_BADARG;
+    if (bc == 8 && !a->ReferenceProp) return MGPU_ERR_BADARG;
     if (a->NoDifFlux && (!a->NoFluxU || !a->NoFluxV || !a->NoFluxW)) return MGPU_ERR_BADARG;
     if (!a->Wflux_X || !a->Wflux_Y || !a->Wflux_Z || !a->VolumeZOld || !a->VolumeZ ||
         !a->OpenPoints3D || !a->LandPoints3D || !a->ComputeFacesU3D || !a->ComputeFacesV3D ||
@@ -390,6 +439,7 @@ struct mgpu_ad {
     double *Rb = nullptr;                   // implicit line row records {D, E, F, TI}
     bool have_faces = false;
     DevDischarges disch;
+    DevBoundary bnd;
     double last_dt = 0.0, last_schmidtH = 0.0;   // Set_Internal_State's memory (AD:5746)
     unsigned long long last_zero_pivots = 0;
 };
@@ -450,6 +500,7 @@ extern "C" int mgpu_ad_destroy(mgpu_ad *ad) {
     dfree(ad->prop); dfree(ad->out); dfree(ad->Wb); dfree(ad->Gb);
     dfree(ad->tig); dfree(ad->mid); dfree(ad->Rb);
     ad->disch.release();
+    ad->bnd.release();
     if (ad->stream) (void)hipStreamDestroy(ad->stream);
     delete ad;
     return MGPU_SUCCESS;
@@ -621,6 +672,11 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     const bool first = !sc.optimize || a->FirstProperty;
     if (!first && !ad->have_faces) return MGPU_ERR_STATE;
     const size_t b3 = sizeof(double) * (size_t)d.n3();
+    // one layer with implicit horizontal advection solves the open boundary
+    // rows along the lines (AD:1745-1843): not offloaded
+    if (a->ReferenceProp && sc.impH >= 0 && !sc.vert) return MGPU_ERR_UNSUPPORTED;
+    const bool post = a->BoundaryCondition == 4 || a->BoundaryCondition == 8;
+    if (post && (rc = ad->bnd.alloc(d))) return rc;
 
     rc = ad->hydro.upload(d, a, s);
     if (rc) return rc;
@@ -669,6 +725,14 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     }
     SharedFields f = make_fields(ad->hydro, ad->der, ad->geom);
     f.disch = ad->disch.fields();
+    if (a->ReferenceProp) {
+        const DevHydro &h = ad->hydro;
+        f.obc.bp = h.bp; f.obc.ref = h.ref;
+        f.obc.open = h.open; f.obc.cfu = h.cfu; f.obc.cfv = h.cfv; f.obc.cfw = h.cfw;
+        f.obc.kfloor = ad->geom.KFloor;
+        f.obc.bc = a->BoundaryCondition;
+        f.obc.tdec = 1.0 / (1.0 + a->DecayTime / a->DTProp);    // AD:5420
+    }
     // the step's own initialisation of TI (AD:1660-1671) reaches the ghost rows
     if (first) HIPCHK(hipMemsetAsync(ad->tig, 0, b3, s));
     TransportLaunch L;
@@ -708,11 +772,29 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     if (!rc && (sc.impH < 0 || sc.hreset))
         rc = launch_copy_back(d, ad->der.mask, ad->out, P, stop, s);
     if (rc) return rc;
+    // AD:1874-1880, on the solved PROP
+    unsigned int bflag = 0;
+    if (post) {
+        HIPCHK(hipMemsetAsync(ad->bnd.flag, 0, sizeof(unsigned int), s));
+        if (a->BoundaryCondition == 4) {
+            rc = launch_bnd_list(d, ad->hydro.bp, ad->geom.KFloor, ad->der.rowcnt, ad->bnd.list,
+                                 ad->bnd.meta, ad->bnd.pos, ad->bnd.total, s);
+            if (!rc)
+                rc = launch_null_gradient(d, ad->bnd.list, ad->bnd.meta, ad->bnd.total,
+                                          ad->hydro.cfu, ad->hydro.cfv, P, stop, s);
+        } else {
+            rc = launch_cyclic(d, ad->hydro.bp, ad->hydro.ref, ad->geom.KFloor, P, ad->bnd.flag,
+                               stop, s);
+        }
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(&bflag, ad->bnd.flag, sizeof(bflag), hipMemcpyDeviceToHost, s));
+    }
     unsigned long long cnt[4];
     HIPCHK(hipMemcpyAsync(cnt, ad->der.counters, sizeof(cnt), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     ad->last_zero_pivots = cnt[0];
     if (cnt[3]) return MGPU_ERR_PIVOT;            // THOMAS_3D ERR10: PROP not written
+    if (bflag) return MGPU_ERR_BADARG;            // a cyclic copy from below the Size
     if (!a->device_pointers) {
         HIPCHK(hipMemcpyAsync(prop, ad->prop, b3, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));

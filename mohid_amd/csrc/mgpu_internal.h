@@ -89,6 +89,20 @@ struct DischargeFields {
     int n = 0;                        // contributions
 };
 
+// Open boundary of a drop-in call (OpenBoundaryCondition, AD:5369-5665):
+// the rows of the open cells of boundary columns take the condition `bc`
+// (AD:152-158 ids, Orlanski excluded).  bp == null: no open boundary.  The
+// raw int32 masks are the caller's (the face masks of mask are cut to the
+// WorkSize; FluxAtOpenBoundary reads the faces beyond it).
+struct ObcFields {
+    const int32_t *bp = nullptr;      // BoundaryPoints2D (Size i, j)
+    const double *ref = nullptr;      // ReferenceProp
+    const int32_t *open = nullptr, *cfu = nullptr, *cfv = nullptr, *cfw = nullptr;
+    const int32_t *kfloor = nullptr;  // KFloorZ
+    int bc = 0;
+    double tdec = 0.0;                // 1 / (1 + DecayTime / DTProp), AD:5420
+};
+
 struct SharedFields {     // device pointers shared by all tracers
     const double *Qx, *Qy, *Qz, *V, *Vold;
     const double *cu, *cv, *cw;     // face conductances (prep kernel)
@@ -100,6 +114,7 @@ struct SharedFields {     // device pointers shared by all tracers
     const uint8_t *cperm;
     const uint16_t *ccnt, *fperm, *fcnt;
     DischargeFields disch;
+    ObcFields obc;
 };
 
 // Kernel launchers (transport.hip)
@@ -118,6 +133,24 @@ int launch_pack_masks(const Dims &d, const int32_t *open, const int32_t *land,
 int launch_disch_flow(int ncell, const int32_t *cbase, const int32_t *cnt, const uint8_t *uniform,
                       const double *dflow, const uint32_t *ccell, const double *DWZ,
                       double *flow, hipStream_t s);
+// Boundary cells after the solve (AD:1874-1880).  The boundary columns of
+// the WorkSize in the reference's sweep order (j outer, i inner): list
+// (Size (i, j) index), per entry the distance back to its boundary
+// neighbour at j-1 (0: none) and |KFloorZ|; pos (n2 int32) is scratch.
+// rowcnt: one uint32 per WorkSize j; *total: the entry count.
+int launch_bnd_list(const Dims &d, const int32_t *bp, const int32_t *kfloor, uint32_t *rowcnt,
+                    uint32_t *list, int32_t *meta, int32_t *pos, unsigned long long *total,
+                    hipStream_t s);
+// ImposeNullGradient (AD:1926-1987) in place on P over the listed columns.
+int launch_null_gradient(const Dims &d, const uint32_t *list, const int32_t *meta,
+                         const unsigned long long *total, const int32_t *cfu,
+                         const int32_t *cfv, double *P, const unsigned long long *stop,
+                         hipStream_t s);
+// Prop_CyclicBoundary (AD:2116-2224) in place on P; a copy that would start
+// below the Size sets *flag.  Both do nothing when *stop is set.
+int launch_cyclic(const Dims &d, const int32_t *bp, const double *ref, const int32_t *kfloor,
+                  double *P, unsigned int *flag, const unsigned long long *stop,
+                  hipStream_t s);
 // ptr[cells[m]] = vals ? vals[m] : -1 for m < n.
 int launch_disch_scatter(int n, const uint32_t *cells, const int32_t *vals, int32_t *ptr,
                          hipStream_t s);

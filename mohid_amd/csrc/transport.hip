@@ -117,6 +117,7 @@ int launch_transport(const TransportLaunch &L, const Dims &d, const StepScalars 
         if ((sc.vert && !L.mid) || !L.Rb || !L.lists.hrows) return MGPU_ERR_BADARG;
     }
     if (L.tig && L.n != 1) return MGPU_ERR_BADARG;   // a per-tracer record
+    if (f.obc.bp && (L.variant != MGPU_KERNEL_LANES || L.n != 1)) return MGPU_ERR_UNSUPPORTED;
     for (int t0 = 0; t0 < L.n; t0 += chunk) {
         const int nt = L.n - t0 < chunk ? L.n - t0 : chunk;
         TracerSet ts;
@@ -309,6 +310,222 @@ int launch_disch_scatter(int n, const uint32_t *cells, const int32_t *vals, int3
     hipLaunchKernelGGL(disch_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, cells,
                        vals, ptr);
     HIPCHK(hipGetLastError());
+    return MGPU_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// Boundary conditions after the solve (AD:1874-1880).
+namespace {
+__global__ void __launch_bounds__(256)
+bnd_count_kernel(Dims d, const int32_t *__restrict__ bp, uint32_t *__restrict__ rowcnt) {
+    __shared__ uint32_t part[4];
+    const int j = d.wJLB + blockIdx.x;
+    uint32_t cnt = 0;
+    for (int i = d.wILB + threadIdx.x; i <= d.wIUB; i += 256) cnt += bp[d.idx2(i, j)] == 1;
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_down(cnt, o, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) rowcnt[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// Row j's boundary columns in i order at rowoff[j]: list[e] = (i, j) index,
+// pos[(i, j)] = e.
+__global__ void __launch_bounds__(256)
+bnd_write_kernel(Dims d, const int32_t *__restrict__ bp, const uint32_t *__restrict__ rowoff,
+                 uint32_t *__restrict__ list, int32_t *__restrict__ pos) {
+    __shared__ uint32_t wsum[4];
+    const int j = d.wJLB + blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t off = rowoff[blockIdx.x];
+    for (int i0 = d.wILB; i0 <= d.wIUB; i0 += 256) {
+        const int i = i0 + threadIdx.x;
+        const bool m = i <= d.wIUB && bp[d.idx2(i, j)] == 1;
+        const unsigned long long b = __ballot(m);
+        const uint32_t before = (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wave] = (uint32_t)__popcll(b);
+        __syncthreads();
+        uint32_t woff = 0;
+        for (int w = 0; w < wave; ++w) woff += wsum[w];
+        const uint32_t chunk = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (m) {
+            const uint32_t e = off + woff + before;
+            list[e] = (uint32_t)d.idx2(i, j);
+            pos[d.idx2(i, j)] = (int32_t)e;
+        }
+        __syncthreads();
+        off += chunk;
+    }
+}
+
+// Per entry: 1 when (i-1, j) is a boundary column (then the entry just
+// before), the distance back to (i, j-1) when it is one (else 0), |KFloorZ|.
+__global__ void __launch_bounds__(256)
+bnd_meta_kernel(Dims d, const int32_t *__restrict__ bp, const int32_t *__restrict__ kfloor,
+                const uint32_t *__restrict__ list, const int32_t *__restrict__ pos,
+                const unsigned long long *__restrict__ total, int32_t *__restrict__ meta) {
+    const long n = (long)*total;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+         e += (long)gridDim.x * blockDim.x) {
+        const long r = list[e];
+        const int i = d.ILB + (int)(r % d.sj), j = d.JLB + (int)(r / d.sj);
+        meta[3 * e] = (i > d.wILB && bp[r - 1] == 1) ? 1 : 0;
+        meta[3 * e + 1] = (j > d.wJLB && bp[r - d.sj] == 1) ? (int32_t)(e - pos[r - d.sj]) : 0;
+        meta[3 * e + 2] = abs(kfloor[r]);
+    }
+}
+
+// ImposeNullGradient (AD:1926-1987), NullGradProp (AD:2071-2110): one lane
+// per level walks the boundary columns in the reference's order, so every
+// cell sees its boundary neighbours at i-1 and j-1 with the values they
+// have just taken.  The inputs of entry e+NG_PF are loaded once entry e is
+// stored; a neighbour fewer than NG_PF entries back is taken from the
+// lane's last values instead (the load may predate its store).
+constexpr int NG_PF = 4;
+
+struct NgSlot {
+    long r;
+    int di, dj, kb;
+    int cvp, cv, cup, cu;
+    double pip, pim, pjp, pjm;
+};
+
+__device__ __forceinline__ void ng_load(const Dims &d, const uint32_t *__restrict__ list,
+                                        const int32_t *__restrict__ meta, long e, long n, int k,
+                                        bool on, const int32_t *__restrict__ cfu,
+                                        const int32_t *__restrict__ cfv, const double *P,
+                                        NgSlot &v) {
+    v.r = 0; v.di = 0; v.dj = 0; v.kb = d.wKUB + 1;
+    v.cvp = v.cv = v.cup = v.cu = 0;
+    v.pip = v.pim = v.pjp = v.pjm = 0.0;
+    if (e >= n) return;
+    v.r = list[e];
+    v.di = meta[3 * e]; v.dj = meta[3 * e + 1]; v.kb = meta[3 * e + 2];
+    if (!on || k < v.kb) return;
+    const long x = v.r + (long)(k - d.KLB) * d.sk;
+    v.cvp = cfv[x + 1]; v.cv = cfv[x]; v.cup = cfu[x + d.sj]; v.cu = cfu[x];
+    v.pip = P[x + 1]; v.pim = P[x - 1]; v.pjp = P[x + d.sj]; v.pjm = P[x - d.sj];
+}
+
+__global__ void __launch_bounds__(64)
+null_gradient_kernel(Dims d, const uint32_t *__restrict__ list, const int32_t *__restrict__ meta,
+                     const unsigned long long *__restrict__ total,
+                     const int32_t *__restrict__ cfu, const int32_t *__restrict__ cfv,
+                     double *P, const unsigned long long *__restrict__ stop) {
+    if (*stop) return;
+    const int k = d.wKLB + blockIdx.x * 64 + threadIdx.x;
+    const bool on = k <= d.wKUB;
+    const long n = (long)*total;
+    NgSlot q[NG_PF];
+#pragma unroll
+    for (int u = 0; u < NG_PF; ++u) ng_load(d, list, meta, u, n, k, on, cfu, cfv, P, q[u]);
+    double v1 = 0.0, v2 = 0.0, v3 = 0.0;   // this lane's last values (entries e-1..e-3)
+    bool u1 = false, u2 = false, u3 = false;
+    for (long e = 0; e < n; ++e) {
+        const NgSlot &c = q[0];
+        double val = 0.0;
+        bool upd = false;
+        if (on && k >= c.kb) {
+            const int aux = c.cvp + c.cv + c.cup + c.cu;
+            if (aux > 0) {
+                const double xim = (c.di == 1 && u1) ? v1 : c.pim;
+                double xjm = c.pjm;
+                if (c.dj == 1 && u1) xjm = v1;
+                else if (c.dj == 2 && u2) xjm = v2;
+                else if (c.dj == 3 && u3) xjm = v3;
+                val = (c.pip * c.cvp + xim * c.cv + c.pjp * c.cup + xjm * c.cu) / (double)aux;
+                P[c.r + (long)(k - d.KLB) * d.sk] = val;
+                upd = true;
+            }
+        }
+        v3 = v2; u3 = u2; v2 = v1; u2 = u1; v1 = val; u1 = upd;
+#pragma unroll
+        for (int u = 0; u < NG_PF - 1; ++u) q[u] = q[u + 1];
+        ng_load(d, list, meta, e + NG_PF, n, k, on, cfu, cfv, P, q[NG_PF - 1]);
+    }
+}
+
+// Prop_CyclicBoundary (AD:2116-2224), first loop: boundary columns take the
+// reference field over the WorkSize levels.
+__global__ void cyclic_ref_kernel(Dims d, const int32_t *__restrict__ bp,
+                                  const double *__restrict__ ref, double *__restrict__ P,
+                                  const unsigned long long *__restrict__ stop) {
+    if (*stop) return;
+    const int i = d.wILB + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = d.wJLB + blockIdx.y;
+    if (i > d.wIUB || bp[d.idx2(i, j)] != 1) return;
+    for (int k = d.wKLB; k <= d.wKUB; ++k) P[d.idx(i, j, k)] = ref[d.idx(i, j, k)];
+}
+
+// The copies: one thread per row i (ILB+1..IUB-1) whose both ends are
+// boundary points, then one per column j (JLB+1..JUB-1).  A row thread
+// writes columns JLB and JUB and reads JLB+1 and JUB-1 of its row; a column
+// thread writes rows ILB and IUB and reads ILB+1 and IUB-1 of columns
+// strictly inside: the two sets never meet.
+__global__ void cyclic_copy_kernel(Dims d, const int32_t *__restrict__ bp,
+                                   const int32_t *__restrict__ kfloor, double *__restrict__ P,
+                                   unsigned int *__restrict__ flag,
+                                   const unsigned long long *__restrict__ stop) {
+    if (*stop) return;
+    const int ni = max(d.wIUB - d.wILB - 1, 0), nj = max(d.wJUB - d.wJLB - 1, 0);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ni) {
+        const int i = d.wILB + 1 + t;
+        if (bp[d.idx2(i, d.wJLB)] != 1 || bp[d.idx2(i, d.wJUB)] != 1) return;
+        const int k1 = kfloor[d.idx2(i, d.wJUB - 1)], k2 = kfloor[d.idx2(i, d.wJLB + 1)];
+        if (k1 < d.KLB || k2 < d.KLB) { atomicOr(flag, 1u); return; }
+        for (int k = k1; k <= d.wKUB; ++k) P[d.idx(i, d.wJLB, k)] = P[d.idx(i, d.wJUB - 1, k)];
+        for (int k = k2; k <= d.wKUB; ++k) P[d.idx(i, d.wJUB, k)] = P[d.idx(i, d.wJLB + 1, k)];
+    } else if (t < ni + nj) {
+        const int j = d.wJLB + 1 + (t - ni);
+        if (bp[d.idx2(d.wILB, j)] != 1 || bp[d.idx2(d.wIUB, j)] != 1) return;
+        const int k1 = kfloor[d.idx2(d.wIUB - 1, j)], k2 = kfloor[d.idx2(d.wILB + 1, j)];
+        if (k1 < d.KLB || k2 < d.KLB) { atomicOr(flag, 1u); return; }
+        for (int k = k1; k <= d.wKUB; ++k) P[d.idx(d.wILB, j, k)] = P[d.idx(d.wIUB - 1, j, k)];
+        for (int k = k2; k <= d.wKUB; ++k) P[d.idx(d.wIUB, j, k)] = P[d.idx(d.wILB + 1, j, k)];
+    }
+}
+}  // namespace
+
+int launch_bnd_list(const Dims &d, const int32_t *bp, const int32_t *kfloor, uint32_t *rowcnt,
+                    uint32_t *list, int32_t *meta, int32_t *pos, unsigned long long *total,
+                    hipStream_t s) {
+    const int nj = d.wJUB - d.wJLB + 1;
+    hipLaunchKernelGGL(bnd_count_kernel, dim3(nj), dim3(256), 0, s, d, bp, rowcnt);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(list_scan_kernel, dim3(1), dim3(1024), 0, s, rowcnt, (long)nj, total);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(bnd_write_kernel, dim3(nj), dim3(256), 0, s, d, bp, rowcnt, list, pos);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(bnd_meta_kernel, dim3(64), dim3(256), 0, s, d, bp, kfloor, list, pos,
+                       total, meta);
+    HIPCHK(hipGetLastError());
+    return MGPU_SUCCESS;
+}
+
+int launch_null_gradient(const Dims &d, const uint32_t *list, const int32_t *meta,
+                         const unsigned long long *total, const int32_t *cfu,
+                         const int32_t *cfv, double *P, const unsigned long long *stop,
+                         hipStream_t s) {
+    const int nk = d.wKUB - d.wKLB + 1;
+    hipLaunchKernelGGL(null_gradient_kernel, dim3((nk + 63) / 64), dim3(64), 0, s, d, list,
+                       meta, total, cfu, cfv, P, stop);
+    HIPCHK(hipGetLastError());
+    return MGPU_SUCCESS;
+}
+
+int launch_cyclic(const Dims &d, const int32_t *bp, const double *ref, const int32_t *kfloor,
+                  double *P, unsigned int *flag, const unsigned long long *stop,
+                  hipStream_t s) {
+    const int ni = d.wIUB - d.wILB + 1, nj = d.wJUB - d.wJLB + 1;
+    hipLaunchKernelGGL(cyclic_ref_kernel, dim3((ni + 255) / 256, nj), dim3(256), 0, s, d, bp,
+                       ref, P, stop);
+    HIPCHK(hipGetLastError());
+    const int nt = (ni - 2 > 0 ? ni - 2 : 0) + (nj - 2 > 0 ? nj - 2 : 0);
+    if (nt > 0) {
+        hipLaunchKernelGGL(cyclic_copy_kernel, dim3((nt + 255) / 256), dim3(256), 0, s, d, bp,
+                           kfloor, P, flag, stop);
+        HIPCHK(hipGetLastError());
+    }
     return MGPU_SUCCESS;
 }
 

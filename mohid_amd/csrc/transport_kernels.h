@@ -664,10 +664,69 @@ __device__ __forceinline__ void column_begin(const Dims &d, const StepScalars &s
     for (int t = 0; t < NT; ++t) { c.Wm[t] = 0.0; c.Gm[t] = 0.0; }
 }
 
+// OpenBoundaryCondition (AD:5369-5665, Orlanski excluded) on the row of
+// open cell x of boundary column r: the last change to the row before the
+// solve.  P is PROP at this point of the step (its stride s).
+__device__ __forceinline__ void obc_row(const Dims &d, const StepScalars &sc,
+                                        const SharedFields &f, const double *P, long s, long x,
+                                        long r, double vk, double &TI, double &D, double &E,
+                                        double &F) {
+    const ObcFields &o = f.obc;
+    const int bc = o.bc;
+    const long sj = d.sj, sk = d.sk;
+    const double dtv = sc.dt / vk;                    // DTPropDouble / VolumeZ
+    double ext = 0.0;
+    if (bc == 2 || bc == 5) {                         // ImposedValue_, SubModel_
+        // open neighbours that are not boundary points (AD:5456-5474)
+        const double A1 = (double)(o.open[x + 1] * (1 - o.bp[r + 1]));
+        const double A2 = (double)(o.open[x - 1] * (1 - o.bp[r - 1]));
+        const double A3 = (double)(o.open[x + sj] * (1 - o.bp[r + sj]));
+        const double A4 = (double)(o.open[x - sj] * (1 - o.bp[r - sj]));
+        const double At = A1 + A2 + A3 + A4;
+        if (At > 0.0) {
+            const double in = (P[(x + 1) * s] * A1 + P[(x - 1) * s] * A2 +
+                               P[(x + sj) * s] * A3 + P[(x - sj) * s] * A4) / At;
+            ext = in * (1.0 - o.tdec) + o.ref[x] * o.tdec;
+        } else {
+            ext = o.ref[x];
+        }
+    }
+    if (bc == 1 || bc == 7) {                         // MassConservation_, MassConservNullGrad_
+        if (bc == 1) ext = P[x * s] * (1.0 - o.tdec) + o.ref[x] * o.tdec;
+        // FluxAtOpenBoundary (AD:5716-5727)
+        const double wf = f.Qx[x] * o.cfu[x] - f.Qx[x + sj] * o.cfu[x + sj] +
+                          f.Qy[x] * o.cfv[x] - f.Qy[x + 1] * o.cfv[x + 1] +
+                          f.Qz[x] * o.cfw[x] - f.Qz[x + sk] * o.cfw[x + sk] -
+                          (f.V[x] - f.Vold[x]) / sc.dt;
+        if (wf < 0.0) {                               // water flowing in
+            if (bc == 1) {
+                TI = TI - wf * ext * dtv;
+            } else {
+                // NullGradProp (AD:2071-2110)
+                const int aux = o.cfv[x + 1] + o.cfv[x] + o.cfu[x + sj] + o.cfu[x];
+                if (aux > 0)
+                    TI = (P[(x + 1) * s] * o.cfv[x + 1] + P[(x - 1) * s] * o.cfv[x] +
+                          P[(x + sj) * s] * o.cfu[x + sj] + P[(x - sj) * s] * o.cfu[x]) /
+                         (double)aux;
+                else
+                    TI = P[x * s];
+                E = 1.0; D = 0.0; F = 0.0;
+            }
+        } else {
+            E = E + wf * dtv;
+        }
+    } else if (bc == 2 || bc == 5) {
+        TI = ext; D = 0.0; E = 1.0; F = 0.0;
+    } else {                                          // NullGradient_, CyclicBoundary_
+        TI = P[x * s]; D = 0.0; E = 1.0; F = 0.0;
+    }
+}
+
 // Row k of the column: assembles D/E/F and completes TI with the vertical
 // terms, then one forward-sweep step of THOMASZ_NewType2; slides the
-// windows up one level.
-template <bool OPT, int NT, bool FAST>
+// windows up one level.  With OBC the open cells of boundary columns take
+// the open boundary rows.
+template <bool OPT, int NT, bool FAST, bool OBC = false>
 __device__ __forceinline__ void column_row(const Dims &d, const StepScalars &sc,
                                            const SharedFields &f, const TracerSet &ts, long base,
                                            int k, bool vflux, ColumnWalk<NT> &c,
@@ -752,6 +811,12 @@ __device__ __forceinline__ void column_row(const Dims &d, const StepScalars &sc,
                     TI = TI - fl * dt / vk;
                 }
             }
+        }
+        if constexpr (OBC) {
+            // boundary column (base is its (i, j) index: KLB is 0), levels
+            // |KFloorZ|..KUB (AD:5437-5444)
+            if (f.obc.bp[base] == 1 && (mk & M_OPEN) && k >= abs(f.obc.kfloor[base]))
+                obc_row(d, sc, f, ts.P + t, ts.s, x, base, vk, TI, D, E, F);
         }
         if (land) TI = MGPU_NULL_REAL;
         // THOMASZ_NewType2 forward sweep (MF:4090-4099)
@@ -1210,7 +1275,7 @@ explicit_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
 // solves column cols[n] for tracers u*TPL .. u*TPL+TPL-1: rows D/E/F,
 // forward sweep (W over TI in TW, G in Gs), back substitution
 // (MF:4080-4106).
-template <bool OPT, int NT, int TPL, bool FAST>
+template <bool OPT, int NT, int TPL, bool FAST, bool OBC = false>
 __global__ void __launch_bounds__(256)
 column_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
                    const uint32_t *__restrict__ cols, long ncols, double *__restrict__ TW,
@@ -1249,7 +1314,8 @@ column_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
         double TI[TPL], W[TPL], G[TPL];
         // after implicit line solves the rows restart from TI = PROP (AD:4248)
         load_tpl<TPL>((sc.hreset ? t1.P : tw) + x * s, TI);
-        column_row<OPT, TPL, FAST>(d, sc, f, t1, base, k, vflux, c, TI, W, G, zero_pivots, bad);
+        column_row<OPT, TPL, FAST, OBC>(d, sc, f, t1, base, k, vflux, c, TI, W, G, zero_pivots,
+                                        bad);
         store_tpl<TPL>(tw + x * s, W);
         store_tpl<TPL>(gs + x * s, G);
     }
@@ -1593,6 +1659,17 @@ static int launch_column_tpl(const Dims &d, const StepScalars &sc, const SharedF
                              unsigned long long *counters, unsigned int *bad, hipStream_t s) {
     const long nb = (l.ncols * (NT / TPL) + 255) / 256;
     if (nb <= 0) return MGPU_SUCCESS;
+    if (f.obc.bp) {
+        // open boundaries are a per-property option: the drop-in's one tracer
+        if constexpr (NT == 1 && TPL == 1) {
+            hipLaunchKernelGGL((column_lane_kernel<OPT, 1, 1, FAST, true>), dim3((unsigned)nb),
+                               dim3(256), 0, s, d, sc, f, ts, l.cols, l.ncols, TW, Gs, counters,
+                               bad);
+            HIPCHK(hipGetLastError());
+            return MGPU_SUCCESS;
+        }
+        return MGPU_ERR_UNSUPPORTED;
+    }
     hipLaunchKernelGGL((column_lane_kernel<OPT, NT, TPL, FAST>), dim3((unsigned)nb), dim3(256), 0,
                        s, d, sc, f, ts, l.cols, l.ncols, TW, Gs, counters, bad);
     HIPCHK(hipGetLastError());

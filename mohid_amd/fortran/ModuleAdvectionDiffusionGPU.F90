@@ -72,8 +72,11 @@ module ModuleAdvectionDiffusionGPU
         type(c_ptr)        :: Visc_H, Diff_V
         type(c_ptr)        :: WaterPoints3D
         integer(c_int32_t) :: Optimize, FirstProperty
-        integer(c_int32_t) :: unsupported_CellFluxes, unsupported_ReferenceProp
-        integer(c_int32_t) :: unsupported_BoundaryCondition, unsupported_NoAdvFlux
+        integer(c_int32_t) :: unsupported_CellFluxes, unsupported_NoAdvFlux
+        type(c_ptr)        :: ReferenceProp
+        integer(c_int32_t) :: BoundaryCondition
+        real(c_double)     :: DecayTime
+        type(c_ptr)        :: BoundaryPoints2D
         integer(c_int32_t) :: NoDifFlux
         type(c_ptr)        :: NoFluxU, NoFluxV, NoFluxW
         type(c_ptr)        :: SmallDepths
@@ -210,11 +213,15 @@ contains
     !> advanced by DTProp in place.  Arguments keep the reference's names,
     !> order and meaning.  Options the GPU path does not offload make
     !> STAT = MGPU_ERR_UNSUPPORTED and leave PROP untouched:
-    !>   ReferenceProp present (open boundary, AD:5816-5830), BoundaryCondition
-    !>   NullGradient_ or CyclicBoundary_ (post-solve passes, AD:1874-1882),
-    !>   CellFluxes, NoAdvFlux true (AD:1321-1325), AdvectionNudging.
-    !> SmallDepths, NoDifFlux (with NoFluxU/V/W) and the discharges of
-    !> SetDischargesGPU are offloaded.
+    !>   CellFluxes, NoAdvFlux true (AD:1321-1325), AdvectionNudging, the
+    !>   Orlanski_ open boundary, and an open boundary with one layer and
+    !>   implicit horizontal advection.
+    !> SmallDepths, NoDifFlux (with NoFluxU/V/W), the discharges of
+    !> SetDischargesGPU, the open boundary (ReferenceProp, BoundaryCondition,
+    !> DecayTime, AD:5369-5665) and the NullGradient_ / CyclicBoundary_ passes
+    !> after the solve (AD:1874-1880) are offloaded.  BoundaryPoints2D is the
+    !> array the reference fetches from HorizontalMap (GetBoundaries, AD:1373);
+    !> it is needed with ReferenceProp or those two conditions.
     !> FirstProperty is .true. on the first call of every step (WP:14600,
     !> 14821-14826): the library keeps, like the reference, coefficients of
     !> the previous call of the same step.
@@ -227,7 +234,8 @@ contains
                                      OpenPoints3D, LandPoints3D, ComputeFacesU3D,             &
                                      ComputeFacesV3D, ComputeFacesW3D, Visc_H, Diff_V,        &
                                      WaterPoints3D, CellFluxes, ReferenceProp,                &
-                                     BoundaryCondition, SmallDepths, NoAdvFlux, NoDifFlux,    &
+                                     BoundaryCondition, DecayTime, BoundaryPoints2D,          &
+                                     SmallDepths, NoAdvFlux, NoDifFlux,                       &
                                      NoFluxU, NoFluxV, NoFluxW,                               &
                                      Optimize, FirstProperty, STAT)
         integer,                           intent(in)    :: ID
@@ -253,8 +261,10 @@ contains
         real(8), dimension(:,:,:), contiguous, target, intent(in)    :: Visc_H, Diff_V
         integer, dimension(:,:,:), contiguous, target, intent(in)    :: WaterPoints3D
         logical, optional,                 intent(in)    :: CellFluxes
-        real(8), dimension(:,:,:), optional, intent(in)  :: ReferenceProp
+        real(8), dimension(:,:,:), optional, contiguous, target, intent(in) :: ReferenceProp
         integer, optional,                 intent(in)    :: BoundaryCondition
+        real(8), optional,                 intent(in)    :: DecayTime
+        integer, dimension(:,:), optional, contiguous, target, intent(in) :: BoundaryPoints2D
         logical, dimension(:,:), optional, intent(in)    :: SmallDepths
         logical, optional,                 intent(in)    :: NoAdvFlux, NoDifFlux
         integer, dimension(:,:,:), optional, contiguous, target, intent(in) :: NoFluxU, NoFluxV
@@ -295,13 +305,14 @@ contains
         a%FirstProperty = 0; if (present(FirstProperty)) a%FirstProperty = b2i(FirstProperty)
         a%unsupported_CellFluxes = 0
         if (present(CellFluxes)) a%unsupported_CellFluxes = b2i(CellFluxes)
-        a%unsupported_ReferenceProp = 0
-        if (present(ReferenceProp)) a%unsupported_ReferenceProp = 1
-        a%unsupported_BoundaryCondition = 0
-        if (present(BoundaryCondition)) then
-            if (BoundaryCondition == NullGradient_ .or. BoundaryCondition == CyclicBoundary_) &
-                a%unsupported_BoundaryCondition = 1
-        endif
+        a%ReferenceProp = c_null_ptr
+        if (present(ReferenceProp)) a%ReferenceProp = c_loc(ReferenceProp)
+        a%BoundaryCondition = 0
+        if (present(BoundaryCondition)) a%BoundaryCondition = BoundaryCondition
+        a%DecayTime = -9.9e15_8
+        if (present(DecayTime)) a%DecayTime = DecayTime
+        a%BoundaryPoints2D = c_null_ptr
+        if (present(BoundaryPoints2D)) a%BoundaryPoints2D = c_loc(BoundaryPoints2D)
         a%unsupported_NoAdvFlux = 0
         if (present(NoAdvFlux)) a%unsupported_NoAdvFlux = b2i(NoAdvFlux)
         a%NoDifFlux = 0

@@ -87,6 +87,10 @@ def main():
                          "drop-in vs batched on 512x512x50, FAST vs FP64 per-tracer error and "
                          "mass drift on the stiff 512x512x50 case, the autotuned "
                          "2048x2048x60 run; one JSON line per measurement")
+    ap.add_argument("--boundaries", action="store_true",
+                    help="instead of the headline line, time the drop-in on 1024x1024x40 "
+                         "(one tracer, device arrays) without and with each offloaded "
+                         "boundary condition; one JSON line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 512 MiB three times (the "
                          "reference workload for scaling FETCH_SIZE/WRITE_SIZE counters)")
@@ -107,6 +111,9 @@ def main():
         print("bench.py: this build measures one GPU only (the transport step's "
               "single-GPU kernel); run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.boundaries:
+        boundary_costs(args)
+        return
     if args.suite:
         suite(args)
         return
@@ -345,6 +352,65 @@ def _batched_rate(case, precision, steps, chunk=None, tune=True, implicit="none"
     tr.close()
     torch.cuda.empty_cache()
     return T * case.n_wet * (steps // chunk) * chunk / wall, shape
+
+
+def boundary_costs(args):
+    """Per-call time of the drop-in with each boundary condition on the
+    headline grid: the water points on the domain's edges are boundary
+    points (as HorizontalMap makes them), a reference field for the open
+    boundary, DecayTime 1 h."""
+    import torch
+    from mohid_amd import _lib
+    from mohid_amd import constants as C
+    from mohid_amd import fixtures as fx
+    from mohid_amd.advection_diffusion import AdvectionDiffusion
+
+    torch.cuda.set_device(0)
+    _lib.lib()
+    case = fx.make_case(fx.config("c3"), device="cuda")
+    ws = case.worksize
+    # device fields are (k, j, i) tensors: Fortran (i, j, k) memory order
+    wp = case.hydro["WaterPoints3D"][ws[5]] == 1
+    edge = torch.zeros_like(wp)
+    edge[ws[2]:ws[3] + 1, ws[0]] = True
+    edge[ws[2]:ws[3] + 1, ws[1]] = True
+    edge[ws[2], ws[0]:ws[1] + 1] = True
+    edge[ws[3], ws[0]:ws[1] + 1] = True
+    bp = (edge & wp).to(torch.int32).contiguous()
+    hy = dict(case.hydro, BoundaryPoints2D=bp)
+    ref = torch.full_like(case.tracers[0], 2.0)
+    ad = AdvectionDiffusion(case.size, case.worksize, case.geom)
+    prop = case.tracers[0].clone()
+    out = {"measure": "drop-in per call with boundary conditions, 1024x1024x40, 1 tracer, "
+                      "device arrays (median of 10 calls)", "boundary_columns": int(bp.sum()),
+           "unit": "ms_per_call"}
+    k = 10
+
+    def median_ms(fn):
+        # per-call wall times (each call ends in a stream synchronisation);
+        # the median keeps a host-side stall from one call out of the figure
+        ts = []
+        for _ in range(k):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return 1e3 * sorted(ts)[k // 2]
+
+    for name, extra in (("none", {}),
+                        ("MassConservation", dict(ReferenceProp=ref,
+                                                  BoundaryCondition=C.MASS_CONSERVATION)),
+                        ("ImposedValue", dict(ReferenceProp=ref,
+                                              BoundaryCondition=C.IMPOSED_VALUE)),
+                        ("NullGradient", dict(ReferenceProp=ref,
+                                              BoundaryCondition=C.NULL_GRADIENT)),
+                        ("Cyclic", dict(ReferenceProp=ref, BoundaryCondition=C.CYCLIC))):
+        prm = dict(case.params, DecayTime=3600.0, **extra)
+        for _ in range(2):
+            ad(prop, hy, prm)
+        out[name] = median_ms(lambda: ad(prop, hy, prm))
+        print(f"boundaries: {name} {out[name]:.3f} ms per call", file=sys.stderr, flush=True)
+    ad.kill()
+    print(json.dumps(out), flush=True)
 
 
 def suite(args):

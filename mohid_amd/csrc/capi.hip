@@ -200,7 +200,7 @@ struct DevBoundary {
         if (list) return MGPU_SUCCESS;
         const long nw = (long)(d.wIUB - d.wILB + 1) * (d.wJUB - d.wJLB + 1);
         int rc = dalloc(&list, nw);
-        if (!rc) rc = dalloc(&meta, 3 * nw);
+        if (!rc) rc = dalloc(&meta, 4 * nw);   // int4 per entry
         if (!rc) rc = dalloc(&pos, d.n2());
         if (!rc) rc = dalloc(&total, 1);
         if (!rc) rc = dalloc(&flag, 1);

@@ -135,8 +135,9 @@ int launch_disch_flow(int ncell, const int32_t *cbase, const int32_t *cnt, const
                       double *flow, hipStream_t s);
 // Boundary cells after the solve (AD:1874-1880).  The boundary columns of
 // the WorkSize in the reference's sweep order (j outer, i inner): list
-// (Size (i, j) index), per entry the distance back to its boundary
-// neighbour at j-1 (0: none) and |KFloorZ|; pos (n2 int32) is scratch.
+// (Size (i, j) index); meta: 4 int32 per entry ((i, j) index, 1 when the
+// previous entry is (i-1, j), the distance back to (i, j-1) or 0, |KFloorZ|);
+// pos (n2 int32) is scratch.
 // rowcnt: one uint32 per WorkSize j; *total: the entry count.
 int launch_bnd_list(const Dims &d, const int32_t *bp, const int32_t *kfloor, uint32_t *rowcnt,
                     uint32_t *list, int32_t *meta, int32_t *pos, unsigned long long *total,

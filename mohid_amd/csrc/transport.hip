@@ -357,90 +357,108 @@ bnd_write_kernel(Dims d, const int32_t *__restrict__ bp, const uint32_t *__restr
     }
 }
 
-// Per entry: 1 when (i-1, j) is a boundary column (then the entry just
-// before), the distance back to (i, j-1) when it is one (else 0), |KFloorZ|.
+// Per entry {(i, j) index, 1 when (i-1, j) is a boundary column (then the
+// entry just before), the distance back to (i, j-1) when it is one (else 0),
+// |KFloorZ|}.
 __global__ void __launch_bounds__(256)
 bnd_meta_kernel(Dims d, const int32_t *__restrict__ bp, const int32_t *__restrict__ kfloor,
                 const uint32_t *__restrict__ list, const int32_t *__restrict__ pos,
-                const unsigned long long *__restrict__ total, int32_t *__restrict__ meta) {
+                const unsigned long long *__restrict__ total, int4 *__restrict__ meta) {
     const long n = (long)*total;
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
          e += (long)gridDim.x * blockDim.x) {
         const long r = list[e];
         const int i = d.ILB + (int)(r % d.sj), j = d.JLB + (int)(r / d.sj);
-        meta[3 * e] = (i > d.wILB && bp[r - 1] == 1) ? 1 : 0;
-        meta[3 * e + 1] = (j > d.wJLB && bp[r - d.sj] == 1) ? (int32_t)(e - pos[r - d.sj]) : 0;
-        meta[3 * e + 2] = abs(kfloor[r]);
+        int4 m;
+        m.x = (int)r;
+        m.y = (i > d.wILB && bp[r - 1] == 1) ? 1 : 0;
+        m.z = (j > d.wJLB && bp[r - d.sj] == 1) ? (int)(e - pos[r - d.sj]) : 0;
+        m.w = abs(kfloor[r]);
+        meta[e] = m;
     }
 }
 
 // ImposeNullGradient (AD:1926-1987), NullGradProp (AD:2071-2110): one lane
 // per level walks the boundary columns in the reference's order, so every
 // cell sees its boundary neighbours at i-1 and j-1 with the values they
-// have just taken.  The inputs of entry e+NG_PF are loaded once entry e is
-// stored; a neighbour fewer than NG_PF entries back is taken from the
-// lane's last values instead (the load may predate its store).
-constexpr int NG_PF = 4;
+// have just taken.  Chunks of NG_CH entries: the block's four waves load the
+// chunk's inputs (neighbour values and face masks, every level) into LDS,
+// then wave 0 sweeps the chunk.  A neighbour swept earlier in the same chunk
+// was loaded before its store, so its new value comes from the sweep itself
+// (registers for the last two entries, LDS beyond); one swept in an earlier
+// chunk was stored before the barrier that precedes the chunk's loads.
+constexpr int NG_CH = 32;
 
-struct NgSlot {
-    long r;
-    int di, dj, kb;
-    int cvp, cv, cup, cu;
-    double pip, pim, pjp, pjm;
-};
-
-__device__ __forceinline__ void ng_load(const Dims &d, const uint32_t *__restrict__ list,
-                                        const int32_t *__restrict__ meta, long e, long n, int k,
-                                        bool on, const int32_t *__restrict__ cfu,
-                                        const int32_t *__restrict__ cfv, const double *P,
-                                        NgSlot &v) {
-    v.r = 0; v.di = 0; v.dj = 0; v.kb = d.wKUB + 1;
-    v.cvp = v.cv = v.cup = v.cu = 0;
-    v.pip = v.pim = v.pjp = v.pjm = 0.0;
-    if (e >= n) return;
-    v.r = list[e];
-    v.di = meta[3 * e]; v.dj = meta[3 * e + 1]; v.kb = meta[3 * e + 2];
-    if (!on || k < v.kb) return;
-    const long x = v.r + (long)(k - d.KLB) * d.sk;
-    v.cvp = cfv[x + 1]; v.cv = cfv[x]; v.cup = cfu[x + d.sj]; v.cu = cfu[x];
-    v.pip = P[x + 1]; v.pim = P[x - 1]; v.pjp = P[x + d.sj]; v.pjm = P[x - d.sj];
-}
-
-__global__ void __launch_bounds__(64)
-null_gradient_kernel(Dims d, const uint32_t *__restrict__ list, const int32_t *__restrict__ meta,
+__global__ void __launch_bounds__(256)
+null_gradient_kernel(Dims d, const int4 *__restrict__ meta,
                      const unsigned long long *__restrict__ total,
                      const int32_t *__restrict__ cfu, const int32_t *__restrict__ cfv,
                      double *P, const unsigned long long *__restrict__ stop) {
+    __shared__ double pv[NG_CH][4][64];        // P at i+1, i-1, j+1, j-1
+    __shared__ int32_t mk[NG_CH][4][64];       // faces V(i+1), V(i), U(j+1), U(j)
+    __shared__ double nv[NG_CH][64];           // the sweep's values
+    __shared__ uint8_t nu[NG_CH][64];          // ... and whether they were stored
     if (*stop) return;
-    const int k = d.wKLB + blockIdx.x * 64 + threadIdx.x;
-    const bool on = k <= d.wKUB;
     const long n = (long)*total;
-    NgSlot q[NG_PF];
+    if (n <= 0) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int k = d.wKLB + blockIdx.x * 64 + lane;
+    const bool on = k <= d.wKUB;
+    // lanes beyond KUB read a valid level and store nothing
+    const long ko = (long)((on ? k : d.wKUB) - d.KLB) * d.sk;
+    double v1 = 0.0, v2 = 0.0;       // wave 0: this lane's last two values
+    bool u1 = false, u2 = false;
+    for (long c0 = 0; c0 < n; c0 += NG_CH) {
 #pragma unroll
-    for (int u = 0; u < NG_PF; ++u) ng_load(d, list, meta, u, n, k, on, cfu, cfv, P, q[u]);
-    double v1 = 0.0, v2 = 0.0, v3 = 0.0;   // this lane's last values (entries e-1..e-3)
-    bool u1 = false, u2 = false, u3 = false;
-    for (long e = 0; e < n; ++e) {
-        const NgSlot &c = q[0];
-        double val = 0.0;
-        bool upd = false;
-        if (on && k >= c.kb) {
-            const int aux = c.cvp + c.cv + c.cup + c.cu;
-            if (aux > 0) {
-                const double xim = (c.di == 1 && u1) ? v1 : c.pim;
-                double xjm = c.pjm;
-                if (c.dj == 1 && u1) xjm = v1;
-                else if (c.dj == 2 && u2) xjm = v2;
-                else if (c.dj == 3 && u3) xjm = v3;
-                val = (c.pip * c.cvp + xim * c.cv + c.pjp * c.cup + xjm * c.cu) / (double)aux;
-                P[c.r + (long)(k - d.KLB) * d.sk] = val;
-                upd = true;
+        for (int t = 0; t < NG_CH / 4; ++t) {
+            const int u = wave + 4 * t;
+            const long e = c0 + u < n ? c0 + u : n - 1;
+            const long x = (long)meta[e].x + ko;
+            pv[u][0][lane] = P[x + 1];
+            pv[u][1][lane] = P[x - 1];
+            pv[u][2][lane] = P[x + d.sj];
+            pv[u][3][lane] = P[x - d.sj];
+            mk[u][0][lane] = cfv[x + 1];
+            mk[u][1][lane] = cfv[x];
+            mk[u][2][lane] = cfu[x + d.sj];
+            mk[u][3][lane] = cfu[x];
+        }
+        // the chunk's metadata, entry u in lane u, read out with readlane: no
+        // global load in the sweep (on this chip a load would wait for the
+        // sweep's own stores as well)
+        int4 mc = make_int4(0, 0, 0, 0);
+        if (wave == 0) mc = meta[c0 + lane < n ? c0 + lane : n - 1];
+        __syncthreads();
+        if (wave == 0) {
+            const int cnt = n - c0 < NG_CH ? (int)(n - c0) : NG_CH;
+#pragma unroll 4
+            for (int u = 0; u < cnt; ++u) {
+                const int4 m = make_int4(__builtin_amdgcn_readlane(mc.x, u),
+                                         __builtin_amdgcn_readlane(mc.y, u),
+                                         __builtin_amdgcn_readlane(mc.z, u),
+                                         __builtin_amdgcn_readlane(mc.w, u));
+                const int cvp = mk[u][0][lane], cv = mk[u][1][lane];
+                const int cup = mk[u][2][lane], cu = mk[u][3][lane];
+                const int aux = cvp + cv + cup + cu;
+                const bool upd = on && k >= m.w && aux > 0;
+                // (i-1, j) is the previous entry when it is a boundary column
+                const double xim = (m.y == 1 && u1) ? v1 : pv[u][1][lane];
+                double xjm = pv[u][3][lane];
+                const int dj = m.z;
+                if (dj >= 1 && dj <= u) {
+                    if (dj == 1) xjm = u1 ? v1 : xjm;
+                    else if (dj == 2) xjm = u2 ? v2 : xjm;
+                    else if (nu[u - dj][lane]) xjm = nv[u - dj][lane];
+                }
+                const double val = (pv[u][0][lane] * cvp + xim * cv + pv[u][2][lane] * cup +
+                                     xjm * cu) / (double)(aux > 0 ? aux : 1);
+                if (upd) P[(long)m.x + ko] = val;
+                nv[u][lane] = val;
+                nu[u][lane] = upd ? 1 : 0;
+                v2 = v1; u2 = u1; v1 = val; u1 = upd;
             }
         }
-        v3 = v2; u3 = u2; v2 = v1; u2 = u1; v1 = val; u1 = upd;
-#pragma unroll
-        for (int u = 0; u < NG_PF - 1; ++u) q[u] = q[u + 1];
-        ng_load(d, list, meta, e + NG_PF, n, k, on, cfu, cfv, P, q[NG_PF - 1]);
+        __syncthreads();
     }
 }
 
@@ -497,7 +515,7 @@ int launch_bnd_list(const Dims &d, const int32_t *bp, const int32_t *kfloor, uin
     hipLaunchKernelGGL(bnd_write_kernel, dim3(nj), dim3(256), 0, s, d, bp, rowcnt, list, pos);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(bnd_meta_kernel, dim3(64), dim3(256), 0, s, d, bp, kfloor, list, pos,
-                       total, meta);
+                       total, reinterpret_cast<int4 *>(meta));
     HIPCHK(hipGetLastError());
     return MGPU_SUCCESS;
 }
@@ -507,8 +525,9 @@ int launch_null_gradient(const Dims &d, const uint32_t *list, const int32_t *met
                          const int32_t *cfv, double *P, const unsigned long long *stop,
                          hipStream_t s) {
     const int nk = d.wKUB - d.wKLB + 1;
-    hipLaunchKernelGGL(null_gradient_kernel, dim3((nk + 63) / 64), dim3(64), 0, s, d, list,
-                       meta, total, cfu, cfv, P, stop);
+    (void)list;
+    hipLaunchKernelGGL(null_gradient_kernel, dim3((nk + 63) / 64), dim3(256), 0, s, d,
+                       reinterpret_cast<const int4 *>(meta), total, cfu, cfv, P, stop);
     HIPCHK(hipGetLastError());
     return MGPU_SUCCESS;
 }

@@ -1244,6 +1244,101 @@ static int vertical_advection(ora_instance *me, const ora_args *a, const double 
 
 /* AD:1624-1922 AdvectionDiffusionIteration */
 /* ------------------------------------------------------------------------ */
+/* MF:4129-4490 OrlanskiCelerity2D, the path AdvectionDiffusion takes: the
+ * radiation scheme of Marchesiello et al. (2000) with the interior values
+ * extrapolated to the boundary by the weights 0.0546875, -0.2578125,
+ * 0.6015625 (and 0.6015625 on the boundary value across it), relaxation
+ * times of 300 days both ways, implicit in the normal direction. */
+double ora_orlanski2d(const ora_size3d *size, double *newf, double *oldf, const double *ref,
+                      const int32_t *cp, int imin, int imax, int jmin, int jmax, int di, int dj,
+                      int i, int j, int k, int eastnorth, double limitmax, double dt,
+                      double flowvelx)
+{
+    const long nI = size->IUB - size->ILB + 1, nJ = size->JUB - size->JLB + 1;
+#define OX(a, b) (((long)(a) - size->ILB) + nI * (((long)(b) - size->JLB) + nJ * ((long)k - size->KLB)))
+    const int s = eastnorth ? -1 : 1;     /* towards the interior */
+    double i1n, i1o, i2n, i3n, i6o, i7o, b3o = 0.0, b32o = 0.0, b33o = 0.0;
+    double b4o = 0.0, b42o = 0.0, b43o = 0.0, b5o;
+    int nosw = 0, none = 0;
+    double tv, svx, svy, auxy, ss, wcx, wcy, refv, trin, trout, trelax, adjx, auxint, auxb;
+    i1n = newf[OX(i + s * di, j + s * dj)];
+    i1o = oldf[OX(i + s * di, j + s * dj)];
+    i2n = newf[OX(i + 2 * s * di, j + 2 * s * dj)];
+    i3n = newf[OX(i + 3 * s * di, j + 3 * s * dj)];
+    i6o = oldf[OX(i + s * di + dj, j + s * dj + di)];
+    i7o = oldf[OX(i + s * di - dj, j + s * dj - di)];
+    if ((i + s * di + 3 * dj) > imax || (j + s * dj + 3 * di) > jmax)
+        none = 1;
+    else if (cp[OX(i + s * di + dj, j + s * dj + di)] * cp[OX(i + s * di + 2 * dj, j + s * dj + 2 * di)] *
+             cp[OX(i + s * di + 3 * dj, j + s * dj + 3 * di)] == 0)
+        none = 1;
+    if ((i + s * di - 3 * dj) < imin || (j + s * dj - 3 * di) < jmin)
+        nosw = 1;
+    else if (cp[OX(i + s * di - dj, j + s * dj - di)] * cp[OX(i + s * di - 2 * dj, j + s * dj - 2 * di)] *
+             cp[OX(i + s * di - 3 * dj, j + s * dj - 3 * di)] == 0)
+        nosw = 1;
+    if (!none) {
+        b3o = oldf[OX(i + dj, j + di)];
+        b32o = oldf[OX(i + 2 * dj, j + 2 * di)];
+        b33o = oldf[OX(i + 3 * dj, j + 3 * di)];
+    }
+    if (!nosw) {
+        b4o = oldf[OX(i - dj, j - di)];
+        b42o = oldf[OX(i - 2 * dj, j - 2 * di)];
+        b43o = oldf[OX(i - 3 * dj, j - 3 * di)];
+    }
+    b5o = oldf[OX(i, j)];
+    tv = i1n - i1o;
+    svx = i1n - i2n;
+    auxy = i6o - i7o;
+    if ((auxy * tv > 0.0) != (g_mistake == 17)) {
+        svy = i1o - i7o;
+        if (nosw) svy = 0.0;
+    } else {
+        svy = i6o - i1o;
+        if (none) svy = 0.0;
+    }
+    ss = svx * svx + svy * svy;
+    if (ss > 0.0) {
+        wcx = -(tv * svx) / ss;
+        wcy = -(tv * svy) / ss;
+        if (fabs(wcx) > limitmax) wcx = limitmax * wcx / fabs(wcx);
+        if (fabs(wcy) > limitmax) wcy = limitmax * wcy / fabs(wcy);
+    } else {
+        wcx = 0.0;
+        wcy = 0.0;
+    }
+    refv = ref[OX(i, j)];
+    trout = (double)(86400 * 300);
+    trin = (double)(86400 * 300);
+    wcx = wcx + flowvelx;
+    if (wcx > 0.0) {
+        wcx = 4.0 * wcx;
+        adjx = 0.0546875 * i3n - 0.2578125 * i2n + 0.6015625 * i1n;
+        trelax = trout;
+    } else {
+        adjx = refv;
+        wcx = 0.0;
+        wcy = 0.0;
+        trelax = trin;
+    }
+    auxint = wcx * adjx;
+    if (wcy >= 0.0 && !nosw) {
+        const double adjy = 0.0546875 * b43o - 0.2578125 * b42o + 0.6015625 * b4o + 0.6015625 * b5o;
+        auxint = auxint - 4.0 * wcy * (b5o - adjy);
+    } else if (wcy < 0.0 && !none) {
+        const double adjy = 0.0546875 * b33o - 0.2578125 * b32o + 0.6015625 * b3o + 0.6015625 * b5o;
+        auxint = auxint - 4.0 * wcy * (adjy - b5o);
+    }
+    oldf[OX(i, j)] = newf[OX(i, j)];
+    auxb = 1.0;
+    auxb = auxb + wcx * (1.0 - 0.6015625) + dt / trelax;
+    newf[OX(i, j)] = (oldf[OX(i, j)] + auxint + refv * dt / trelax) / auxb;
+    return newf[OX(i, j)];
+#undef OX
+}
+
+/* ------------------------------------------------------------------------ */
 /* Open boundary.  The water flux into a boundary cell, FluxAtOpenBoundary
  * (AD:5697-5731): the net face flows minus the volume change per second. */
 static double flux_at_open_boundary(const ora_instance *me, const ora_args *a, int i, int j,
@@ -1276,7 +1371,7 @@ static int null_grad_prop(const ora_instance *me, const ora_args *a, const doubl
  * open boundary cells, the last change before the solve.  P is PROP at this
  * point of the step (after the line solves in 3-D implicit horizontal
  * advection). */
-static void open_boundary_condition(ora_instance *me, const ora_args *a, const double *P)
+static int open_boundary_condition(ora_instance *me, const ora_args *a, double *P)
 {
     const ora_size3d *w = &me->ws;
     const int bc = a->BoundaryCondition;
@@ -1284,6 +1379,8 @@ static void open_boundary_condition(ora_instance *me, const ora_args *a, const d
     const double *ref = a->ReferenceProp;
     const double tdec = 1.0 / (1.0 + a->DecayTime / a->DTProp);
     int i, j, k;
+    /* AD:5428-5429: PROPOld = PROP, the old field of the radiation */
+    if (bc == BC_ORLANSKI) memcpy(a->PROPOld, P, sizeof(double) * (size_t)me->n3);
     for (j = w->JLB; j <= w->JUB; j++)
         for (i = w->ILB; i <= w->IUB; i++) {
             if (bp[I2(me, i, j)] != 1) continue;
@@ -1309,12 +1406,28 @@ static void open_boundary_condition(ora_instance *me, const ora_args *a, const d
                         ext = ref[q];
                     }
                 }
-                if (bc == BC_MASSCONSERVATION || bc == BC_MASSCONSERVNULLGRAD) {
+                if (bc == BC_MASSCONSERVATION || bc == BC_MASSCONSERVNULLGRAD ||
+                    bc == BC_ORLANSKI) {
                     double wf;
                     if (bc == BC_MASSCONSERVATION) ext = P[q] * (1.0 - tdec) + ref[q] * tdec;
                     wf = flux_at_open_boundary(me, a, i, j, k);
+                    if (bc == BC_ORLANSKI) {
+                        /* AD:5506-5568: the exterior cell across the boundary
+                         * (i +- di, j +- dj) gets the radiated value */
+                        int di, dj, en;
+                        if (i == w->ILB || i == w->IUB) { di = 1; dj = 0; }
+                        else if (j == w->JLB || j == w->JUB) { di = 0; dj = 1; }
+                        else return ORA_ERR_OPTION;           /* 'Orlanski Advection 2' */
+                        en = !(i == w->ILB || j == w->JLB);
+                        ext = ora_orlanski2d(&me->size, P, a->PROPOld, ref, op, w->ILB, w->IUB,
+                                             w->JLB, w->JUB, di, dj, en ? i + di : i - di,
+                                             en ? j + dj : j - dj, k,
+                                             10.0 * a->DTProp /
+                                                 (me->g.DUX[I2(me, i, j)] + me->g.DVY[I2(me, i, j)]) * 2.0,
+                                             en, a->DTProp, wf * dtv);
+                    }
                     if ((wf < 0.0) != (g_mistake == 15)) {          /* water flowing in */
-                        if (bc == BC_MASSCONSERVATION) {
+                        if (bc != BC_MASSCONSERVNULLGRAD) {
                             me->TI[q] = me->TI[q] - wf * ext * dtv;
                         } else {
                             double v;
@@ -1339,6 +1452,7 @@ static void open_boundary_condition(ora_instance *me, const ora_args *a, const d
                 }
             }
         }
+    return 0;
 }
 
 /* ImposeNullGradient (AD:1926-1987): every boundary cell takes the mean of
@@ -1487,7 +1601,10 @@ static int iteration(ora_instance *me, const ora_args *a, double *P)
         }
     }
     /* AD:1745-1746: the open boundary rows, the last change before the solve */
-    if (a->ReferenceProp) open_boundary_condition(me, a, P);
+    if (a->ReferenceProp) {
+        rc = open_boundary_condition(me, a, P);
+        if (rc) return rc;
+    }
     /* AD:1753: TI = null_real on land points */
     {
         long q;
@@ -1573,7 +1690,7 @@ int ora_advection_diffusion(ora_instance *me, const ora_args *a, double *prop)
             bc != BC_ORLANSKI && bc != BC_NULLGRADIENT && bc != BC_CYCLIC &&
             bc != BC_MASSCONSERVNULLGRAD)
             return ORA_ERR_OPTION;
-        if (bc == BC_ORLANSKI) return ORA_ERR_UNSUPPORTED;
+        if (bc == BC_ORLANSKI && !a->PROPOld) return ORA_ERR_BADARG;
     }
     if ((a->ReferenceProp || a->BoundaryCondition == BC_NULLGRADIENT ||
          a->BoundaryCondition == BC_CYCLIC) && !a->BoundaryPoints2D)

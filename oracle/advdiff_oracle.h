@@ -82,6 +82,7 @@ typedef struct {
     int32_t BoundaryCondition;                    /* AD:152-158 ids; 0 when absent */
     double DecayTime;
     const int32_t *BoundaryPoints2D;              /* 2-D (Size i, j) */
+    double *PROPOld;                              /* 3-D, written with Orlanski_ (AD:5428) */
 } ora_args;
 
 typedef struct ora_instance ora_instance;
@@ -132,8 +133,20 @@ const double *ora_internal(const ora_instance *me, const char *name);
  *     the sweep (a Jacobi update instead of the reference's in-place sweep)
  *  15 open boundary inflow taken with the sign of an outflow
  *  16 cyclic boundary copies the row next to the boundary instead of the
- *     one before the opposite boundary */
+ *     one before the opposite boundary
+ *  17 Orlanski: the cross-boundary space variability taken on the wrong
+ *     side of the interior point */
 void ora_plant_mistake(int id);
+
+/* OrlanskiCelerity2D (ModuleFunctions.F90:4129-4490) with the optional
+ * arguments AdvectionDiffusion passes (AD:5549-5568): ReferenceField,
+ * FlowVelX and NewValue (returned); TrelaxIn/TrelaxOut take their defaults,
+ * ConstantCelerity, NormalRadiation and Explicit are false.  newf(i,j,k) and
+ * oldf(i,j,k) are updated as the reference updates them.  Arrays span Size. */
+double ora_orlanski2d(const ora_size3d *size, double *newf, double *oldf, const double *ref,
+                      const int32_t *cp, int imin, int imax, int jmin, int jmax, int di, int dj,
+                      int i, int j, int k, int eastnorth, double limitmax, double dt,
+                      double flowvelx);
 
 /* THOMASZ_NewType2 (ModuleFunctions.F90:4026-4123) on explicit arrays.
  * W, G: scratch of length >= KUB+2 (per-thread VEC in the reference). */

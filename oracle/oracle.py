@@ -55,7 +55,7 @@ class Args(ct.Structure):
         ("NoDifFlux", ct.c_int32), ("NoFluxU", _ip), ("NoFluxV", _ip), ("NoFluxW", _ip),
         ("SmallDepths", _ip),
         ("ReferenceProp", _dp), ("BoundaryCondition", ct.c_int32), ("DecayTime", ct.c_double),
-        ("BoundaryPoints2D", _ip),
+        ("BoundaryPoints2D", _ip), ("PROPOld", _dp),
     ]
 
 
@@ -99,6 +99,9 @@ def lib():
         L.ora_advection_face.argtypes = [_dp, _dp, _dp, ct.c_double, ct.c_double, ct.c_double,
                                          ct.c_int, ct.c_int, ct.c_int, ct.c_int, _dp]
         L.ora_advection_face.restype = ct.c_int
+        L.ora_orlanski2d.argtypes = [ct.POINTER(Size3D), _dp, _dp, _dp, _ip] + [ct.c_int] * 10 + \
+            [ct.c_double] * 3
+        L.ora_orlanski2d.restype = ct.c_double
         L.ora_advection1d_tvd_superbee_2.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_double,
                                                      _dp, _dp, _dp, _dp, _ip, _dp, _dp]
         _lib = L
@@ -127,6 +130,8 @@ def ref():
                                                      _dp, _dp, _dp, _dp, _ip, _dp, _dp]
         R.ref_advection_face.argtypes = [_dp, _dp, _dp, ct.c_double, ct.c_double, ct.c_double,
                                          ct.c_int, ct.c_int, ct.c_int, ct.c_int, _dp]
+        R.ref_orlanski2d.argtypes = [_ip, _dp, _dp, _dp, _ip] + [ct.c_int] * 10 + \
+            [ct.c_double] * 3 + [_dp]
         _ref = R
     return _ref
 
@@ -189,6 +194,11 @@ class Instance:
             ref = np.asfortranarray(ref, dtype=np.float64)
             keep.append(ref)
             ref = _ptr(ref, _dp)
+        # PROPOld is written in place (the caller's array, AD:5428)
+        old = p.get("PROPOld")
+        if old is not None:
+            assert old.flags.f_contiguous and old.dtype == np.float64
+            old = _ptr(old, _dp)
 
         args = Args(p["schmidt_H"], p["SchmidtCoef_V"], p["SchmidtBackground_V"],
                     p["AdvMethodH"], p["TVDLimitationH"], p["AdvMethodV"], p["TVDLimitationV"],
@@ -202,7 +212,7 @@ class Instance:
                     self.now if now is None else now, int(bool(p.get("NoDifFlux", False))),
                     opt_i("NoFluxU"), opt_i("NoFluxV"), opt_i("NoFluxW"), opt_i("SmallDepths"),
                     ref, int(p.get("BoundaryCondition", 0)),
-                    float(p.get("DecayTime", -9.9e15)), opt_i("BoundaryPoints2D"))
+                    float(p.get("DecayTime", -9.9e15)), opt_i("BoundaryPoints2D"), old)
         assert prop.flags.f_contiguous and prop.dtype == np.float64
         return lib().ora_advection_diffusion(self.h, ct.byref(args), _ptr(prop, _dp))
 

@@ -14,7 +14,8 @@ module ref_mf_bind
     use iso_c_binding
     use ModuleFunctions, only : THOMASZ_NewType2, THOMAS_3D, ComputeAdvection1D_V2, &
                                 ComputeAdvection1D_TVD_SuperBee_2,              &
-                                ComputeAdvectionFace, T_THOMAS, T_D_E_F, T_VECGW
+                                ComputeAdvectionFace, OrlanskiCelerity2D,       &
+                                T_THOMAS, T_D_E_F, T_VECGW
     implicit none
 
 contains
@@ -108,6 +109,37 @@ contains
         deallocate(th%COEF3)
         deallocate(th)
     end subroutine ref_thomas3d
+
+    !> OrlanskiCelerity2D (ModuleFunctions.F90:4129-4490) with the optional
+    !> arguments ModuleAdvectionDiffusion passes (AD:5549-5568): the reference
+    !> field, FlowVelX and NewValue.  newf/oldf/ref/cp span Size, column-major;
+    !> newf(i,j,k) and oldf(i,j,k) are updated as the reference updates them.
+    subroutine ref_orlanski2d(sz, newf, oldf, ref, cp, imin, imax, jmin, jmax, di, dj,  &
+                              i, j, k, eastnorth, limitmax, dt, flowvelx, newvalue)     &
+               bind(c, name='ref_orlanski2d')
+        integer(c_int), intent(in)  :: sz(6)
+        type(c_ptr), value          :: newf, oldf, ref, cp
+        integer(c_int), value       :: imin, imax, jmin, jmax, di, dj, i, j, k, eastnorth
+        real(c_double), value       :: limitmax, dt, flowvelx
+        real(c_double), intent(out) :: newvalue
+        real,    pointer :: fN(:,:,:), fO(:,:,:), fR(:,:,:), bN(:,:,:), bO(:,:,:), bR(:,:,:)
+        integer, pointer :: fC(:,:,:), bC(:,:,:)
+        real    :: nv
+        integer :: shp(3)
+
+        shp = [sz(2)-sz(1)+1, sz(4)-sz(3)+1, sz(6)-sz(5)+1]
+        call c_f_pointer(newf, fN, shp); bN(sz(1):, sz(3):, sz(5):) => fN
+        call c_f_pointer(oldf, fO, shp); bO(sz(1):, sz(3):, sz(5):) => fO
+        call c_f_pointer(ref,  fR, shp); bR(sz(1):, sz(3):, sz(5):) => fR
+        call c_f_pointer(cp,   fC, shp); bC(sz(1):, sz(3):, sz(5):) => fC
+        call OrlanskiCelerity2D(NewField = bN, OldField = bO, ComputePoints = bC,       &
+                                ReferenceField = bR, Imin = imin, Imax = imax,          &
+                                Jmin = jmin, Jmax = jmax, di = di, dj = dj, i = i,      &
+                                j = j, k = k, LimitMax = limitmax,                      &
+                                EastNorthBoundary = eastnorth /= 0, DT = dt,            &
+                                FlowVelX = flowvelx, NewValue = nv)
+        newvalue = nv
+    end subroutine ref_orlanski2d
 
     !> ComputeAdvection1D_V2 (ModuleFunctions.F90:10534-10591) on 1-based lines of length n.
     subroutine ref_advection1d_v2(n, ilb, iub, dt, du, prop, q, v, cp, C, D, E, F,  &

@@ -43,7 +43,9 @@ mathematics fixes:
   the mean of its face neighbours, the ones swept before it at their new
   values); the cyclic condition makes both ends of every cyclic row the
   values just inside the opposite end, and the other boundary cells the
-  reference.
+  reference; the Orlanski radiation (its OrlanskiCelerity2D pinned to the
+  reference itself) leaves still water as it was, puts the radiated value in
+  the exterior cell and closes the channel mass budget with it.
 
 ``test_planted_mistakes_are_caught`` plants each of a list of plausible
 mistakes in the oracle (a flux sign, the upwind side, the volume ratio, a
@@ -816,6 +818,68 @@ def _cyclic_err(mistake_free=True):
     return float(np.max(np.abs(got - want)))
 
 
+def _orlanski_err(q):
+    """Orlanski_ along a channel whose middle row has boundary points at both
+    ends.  OrlanskiCelerity2D itself is pinned to the reference
+    (test_oracle_ref); here its use: in still water (q = 0) every WorkSize
+    cell is the step without boundary, each exterior cell holds the value of
+    the radiation evaluated on the old fields (LimitMax = 10 DTProp /
+    (DUX + DVY) * 2, AD:5546), and PROPOld is the old field.  With a flow the
+    mass changes by Q dt (C_ext at the inflow - C_new at the outflow), C_ext
+    being what the step left in the exterior cell."""
+    case = fx.make_case(fx.config("c1", ni=3, nj=12, nk=2))
+    hy = _still_hydro(case)
+    faces = hy["ComputeFacesU3D"] == 1
+    hy["Wflux_X"][faces] = q
+    ws = case.worksize
+    wp = hy["WaterPoints3D"] == 1
+    bp = np.zeros(case.shape[:2], dtype=np.int32, order="F")
+    bp[2, ws[2]] = bp[2, ws[3]] = 1
+    rng = np.random.default_rng(14)
+    c0 = np.asfortranarray(np.where(wp, rng.uniform(1.0, 3.0, size=wp.shape), 0.0))
+    c0[:, 0, :] = rng.uniform(1.0, 3.0, size=c0[:, 0, :].shape)
+    c0[:, -1, :] = rng.uniform(1.0, 3.0, size=c0[:, -1, :].shape)
+    ref = np.asfortranarray(rng.uniform(1.0, 3.0, size=case.shape))
+    old = np.asfortranarray(np.zeros(case.shape))
+    got = _boundary_run(case, c0, bp, hydro=hy, **UPWIND, ReferenceProp=ref,
+                        BoundaryCondition=C.ORLANSKI, DecayTime=0.0, PROPOld=old)
+    err = float(np.max(np.abs(old - c0)))
+    KUB = ws[5]
+    ghosts = [(2, ws[2] - 1, k) for k in range(1, KUB + 1)] + \
+             [(2, ws[3] + 1, k) for k in range(1, KUB + 1)]
+    if q == 0.0:
+        plain = _run(case, c0.copy(order="F"), hydro=hy, **UPWIND)
+        w = np.zeros(case.shape, dtype=bool)
+        w[1:-1, 1:-1, 1:-1] = True
+        err = max(err, float(np.max(np.abs(got[w] - plain[w]))))
+        a_new, a_old = c0.copy(order="F"), c0.copy(order="F")
+        size = ora.Size3D(*case.size)
+        op = np.asfortranarray(hy["OpenPoints3D"], dtype=np.int32)
+        dx = case.geom["DUX"][2, ws[2]] + case.geom["DVY"][2, ws[2]]
+        for (i, j, k) in ghosts:
+            jb = j + 1 if j < ws[2] else j - 1
+            lim = 10.0 * DT / (case.geom["DUX"][2, jb] + case.geom["DVY"][2, jb]) * 2.0
+            want = ora.lib().ora_orlanski2d(ct.byref(size), a_new.ctypes.data_as(ora._dp),
+                                            a_old.ctypes.data_as(ora._dp),
+                                            ref.ctypes.data_as(ora._dp),
+                                            op.ctypes.data_as(ora._ip), ws[0], ws[1], ws[2], ws[3],
+                                            0, 1, i, j, k, int(j > ws[3]), lim, DT, 0.0)
+            err = max(err, abs(got[i, j, k] - want))
+        assert dx > 0
+        return err
+    V = hy["VolumeZ"]
+    gain = float(np.sum(V[wp] * got[wp]) - np.sum(V[wp] * c0[wp]))
+    jin, jout = (ws[2], ws[3]) if q > 0 else (ws[3], ws[2])
+    gin = ws[2] - 1 if q > 0 else ws[3] + 1
+    want = abs(q) * DT * sum(got[2, gin, k] - got[2, jout, k] for k in range(1, KUB + 1))
+    return max(err, abs(gain - want) / float(np.sum(V[wp] * c0[wp])))
+
+
+@pytest.mark.parametrize("q", [0.0, 1500.0, -1500.0])
+def test_orlanski_boundary(q):
+    assert _orlanski_err(q) <= (0.0 if q == 0.0 else 1e-14)
+
+
 def test_imposed_value_closed_form():
     assert _imposed_value_err() == 0.0
 
@@ -874,6 +938,11 @@ def _all_checks():
     for name, interior in (("tvd_land_sigma", 0), ("upwind_land_ragged", 4)):
         out.append((f"null gradient {name}", _null_gradient_err(name, interior), 1e-14))
     out.append(("cyclic", _cyclic_err(), 0.0))
+    for q in (0.0, 1500.0, -1500.0):
+        out.append((f"orlanski {q}", _orlanski_err(q), 0.0 if q == 0.0 else 1e-14))
+    if ora.ref_available():
+        from .test_oracle_ref import orlanski2d_ref_diff
+        out.append(("orlanski vs reference", orlanski2d_ref_diff(1, 1e9), 0.0))
     for name in sorted(SMALL):
         mass, constancy = _invariant_errs(name, steps=2)
         out.append((f"mass {name}", mass, 1e-13))
@@ -884,7 +953,7 @@ def _all_checks():
     return out
 
 
-@pytest.mark.parametrize("mistake", list(range(1, 17)))
+@pytest.mark.parametrize("mistake", list(range(1, 18)))
 def test_planted_mistakes_are_caught(mistake):
     L = _lib()
     L.ora_plant_mistake(mistake)

@@ -10,6 +10,10 @@ inputs and require bit-identical results:
   identical VEC scratch contents afterwards;
 * ComputeAdvectionFace (MF:10702-10894) for every method, limiter, flow sign
   and NearBoundary/Upwind2 combination the reference accepts;
+* OrlanskiCelerity2D (MF:4129-4490) as AdvectionDiffusion calls it
+  (AD:5506-5568) at every boundary cell of a grid, corners included, both
+  flow signs, with and without celerity clipping: identical new values and
+  identical updates of the new and old fields;
 * ComputeAdvection1D_V2 (MF:10534-10591) and
   ComputeAdvection1D_TVD_SuperBee_2 (MF:10642-10699) on random lines with
   random compute masks (which exercises every NearBoundary branch and the
@@ -288,3 +292,53 @@ def test_advection1d_tvd_superbee_2_bitwise():
         ora.ref().ref_advection1d_tvd_superbee_2(n, 2, n - 1, 60.0, P(du), P(prop), P(q), P(v),
                                                  P(cp, ip), P(d_ref), P(e_ref))
         assert np.array_equal(d_ora, d_ref) and np.array_equal(e_ora, e_ref)
+
+
+def orlanski2d_ref_diff(seed, limit):
+    """Largest |oracle - reference| over the calls and the final fields, as
+    test_orlanski2d_bitwise makes them."""
+    try:
+        test_orlanski2d_bitwise(seed, limit)
+    except AssertionError:
+        return float("inf")
+    return 0.0
+
+
+@pytest.mark.parametrize("seed,limit", [(1, 1e9), (2, 1e-3), (3, 0.05)])
+def test_orlanski2d_bitwise(seed, limit):
+    """Every edge cell of a 9 x 8 x 4 grid, in the reference's order, both
+    codes on their own copies of the new/old fields; the mask has closed
+    cells so that both one-sided cases occur."""
+    rng = np.random.default_rng(seed)
+    ni, nj, nk = 9, 8, 4
+    sz = np.array([0, ni + 1, 0, nj + 1, 0, nk + 1], dtype=np.int32)
+    shape = (ni + 2, nj + 2, nk + 2)
+    newf = np.asfortranarray(rng.uniform(1.0, 3.0, shape))
+    oldf = np.asfortranarray(newf + rng.normal(0, 0.05, shape))
+    ref = np.asfortranarray(rng.uniform(1.0, 3.0, shape))
+    cp = np.asfortranarray((rng.uniform(size=shape) < 0.85).astype(np.int32))
+    a_new, a_old = newf.copy(order="F"), oldf.copy(order="F")
+    b_new, b_old = newf.copy(order="F"), oldf.copy(order="F")
+    size = ora.Size3D(*[int(v) for v in sz])
+    n = 0
+    for j in range(1, nj + 1):
+        for i in range(1, ni + 1):
+            if not (i in (1, ni) or j in (1, nj)):
+                continue
+            di, dj = (1, 0) if i in (1, ni) else (0, 1)
+            en = 0 if (i == 1 or j == 1) else 1
+            ie, je = (i + di, j + dj) if en else (i - di, j - dj)
+            for k in range(1, nk + 1):
+                flow = float(rng.normal(0, 0.3))
+                got = ora.lib().ora_orlanski2d(ct.byref(size), P(a_new), P(a_old), P(ref),
+                                               P(cp, ip), 1, ni, 1, nj, di, dj, ie, je, k, en,
+                                               limit, 60.0, flow)
+                want = ct.c_double()
+                ora.ref().ref_orlanski2d(P(sz, ip), P(b_new), P(b_old), P(ref), P(cp, ip), 1,
+                                         ni, 1, nj, di, dj, ie, je, k, en, limit, 60.0, flow,
+                                         ct.byref(want))
+                assert got == want.value or (np.isnan(got) and np.isnan(want.value)), \
+                    (i, j, k, got, want.value)
+                n += 1
+    assert np.array_equal(a_new, b_new) and np.array_equal(a_old, b_old)
+    assert n > 100

@@ -80,8 +80,7 @@ typedef struct {
 /* Arguments of AdvectionDiffusion (AD:1108-1198) in the reference's order.
  * Options the GPU path does not implement are listed as `unsupported_*`:
  * a non-zero value makes mgpu_ad_step return MGPU_ERR_UNSUPPORTED so the
- * caller falls back to the Fortran path.  The same holds for the Orlanski_
- * open boundary condition. */
+ * caller falls back to the Fortran path. */
 typedef struct {
     double schmidt_H, SchmidtCoef_V, SchmidtBackground_V;
     int32_t AdvMethodH, TVDLimitationH, AdvMethodV, TVDLimitationV;
@@ -110,6 +109,8 @@ typedef struct {
     const int32_t *BoundaryPoints2D;       /* 2-D (Size i, j), HorizontalMap's GetBoundaries
                                               (AD:1373); needed with ReferenceProp or
                                               BoundaryCondition 4 / 8 */
+    double *PROPOld;                       /* 3-D, written: PROP before the open boundary
+                                              (AD:5428-5429); needed with Orlanski_ (6) */
     /* offloaded optional arguments (AD:1287-1335); 0 / null when absent */
     int32_t NoDifFlux;                     /* zero diffusion across the NoFlux faces */
     const int32_t *NoFluxU, *NoFluxV, *NoFluxW;   /* 3-D, 1 = flagged face; needed with NoDifFlux */

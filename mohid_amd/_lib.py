@@ -72,7 +72,7 @@ class ADArgs(ct.Structure):
         ("Optimize", ct.c_int32), ("FirstProperty", ct.c_int32),
         ("unsupported_CellFluxes", ct.c_int32), ("unsupported_NoAdvFlux", ct.c_int32),
         ("ReferenceProp", _dp), ("BoundaryCondition", ct.c_int32), ("DecayTime", ct.c_double),
-        ("BoundaryPoints2D", _ip),
+        ("BoundaryPoints2D", _ip), ("PROPOld", _dp),
         ("NoDifFlux", ct.c_int32), ("NoFluxU", _ip), ("NoFluxV", _ip), ("NoFluxW", _ip),
         ("SmallDepths", _ip),
         ("device_pointers", ct.c_int32),

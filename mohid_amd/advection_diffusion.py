@@ -98,6 +98,14 @@ def _args(hydro: Dict, params: Dict, keep: list, device: bool, optimize: bool,
     ref = ct.POINTER(ct.c_double)()
     if p.get("ReferenceProp") is not None:
         ref = ct.cast(_addr(p["ReferenceProp"], keep, np.float64), ct.POINTER(ct.c_double))
+    # PROPOld (Orlanski_) is written: the caller's own array
+    old = ct.POINTER(ct.c_double)()
+    if p.get("PROPOld") is not None:
+        a = p["PROPOld"]
+        if not _is_device(a) and not (isinstance(a, np.ndarray) and a.dtype == np.float64 and
+                                      a.flags.f_contiguous):
+            raise ValueError("PROPOld must be a Fortran-ordered float64 array (written in place)")
+        old = ct.cast(_addr(a, keep, np.float64), ct.POINTER(ct.c_double))
     return L.ADArgs(
         float(p["schmidt_H"]), float(p["SchmidtCoef_V"]), float(p["SchmidtBackground_V"]),
         int(p["AdvMethodH"]), int(p["TVDLimitationH"]), int(p["AdvMethodV"]),
@@ -112,7 +120,7 @@ def _args(hydro: Dict, params: Dict, keep: list, device: bool, optimize: bool,
         int(bool(optimize)), int(bool(first_property)),
         int(bool(p.get("CellFluxes", False))), int(bool(p.get("NoAdvFlux", False))),
         ref, int(p.get("BoundaryCondition", 0)), float(p.get("DecayTime", C.NULL_REAL)),
-        opt_ip("BoundaryPoints2D"),
+        opt_ip("BoundaryPoints2D"), old,
         int(bool(p.get("NoDifFlux", False))),
         opt_ip("NoFluxU"), opt_ip("NoFluxV"), opt_ip("NoFluxW"), opt_ip("SmallDepths"),
         int(bool(device)))

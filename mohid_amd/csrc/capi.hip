@@ -309,7 +309,7 @@ int check_options(const mgpu_ad_args *a) {
     if (a->ReferenceProp) {
         if (bc != 1 && bc != 2 && bc != 4 && bc != 5 && bc != 6 && bc != 7 && bc != 8)
             return MGPU_ERR_OPTION;
-        if (bc == 6) return MGPU_ERR_UNSUPPORTED;
+        if (bc == 6 && !a->PROPOld) return MGPU_ERR_BADARG;   // PROPOld(:,:,:) = PROP
     }
     if ((a->ReferenceProp || bc == 4 || bc == 8) && !a->BoundaryPoints2D) return MGPU_ERR_BADARG;
     if (bc == 8 && !a->ReferenceProp) return MGPU_ERR_BADARG;
@@ -437,6 +437,7 @@ struct mgpu_ad {
     // PROP after implicit line solves
     double *tig = nullptr, *mid = nullptr;
     double *Rb = nullptr;                   // implicit line row records {D, E, F, TI}
+    double *oldb = nullptr;                 // PROPOld of a host-mode Orlanski_ call
     bool have_faces = false;
     DevDischarges disch;
     DevBoundary bnd;
@@ -498,7 +499,7 @@ extern "C" int mgpu_ad_destroy(mgpu_ad *ad) {
     ad->hydro.release();
     ad->der.release();
     dfree(ad->prop); dfree(ad->out); dfree(ad->Wb); dfree(ad->Gb);
-    dfree(ad->tig); dfree(ad->mid); dfree(ad->Rb);
+    dfree(ad->tig); dfree(ad->mid); dfree(ad->Rb); dfree(ad->oldb);
     ad->disch.release();
     ad->bnd.release();
     if (ad->stream) (void)hipStreamDestroy(ad->stream);
@@ -675,6 +676,10 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     // one layer with implicit horizontal advection solves the open boundary
     // rows along the lines (AD:1745-1843): not offloaded
     if (a->ReferenceProp && sc.impH >= 0 && !sc.vert) return MGPU_ERR_UNSUPPORTED;
+    // Orlanski_ writes PROP's exterior cells before the solve; after the 3-D
+    // line solves those are line rows (not offloaded)
+    const bool orl = a->ReferenceProp && a->BoundaryCondition == 6;
+    if (orl && sc.impH >= 0) return MGPU_ERR_UNSUPPORTED;
     const bool post = a->BoundaryCondition == 4 || a->BoundaryCondition == 8;
     if (post && (rc = ad->bnd.alloc(d))) return rc;
 
@@ -732,6 +737,17 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
         f.obc.kfloor = ad->geom.KFloor;
         f.obc.bc = a->BoundaryCondition;
         f.obc.tdec = 1.0 / (1.0 + a->DecayTime / a->DTProp);    // AD:5420
+        if (orl) {
+            // PROPOld(:,:,:) = PROP (AD:5428-5429); the caller's array when it is
+            // on the device, else a copy here (returned after the step)
+            if (!a->device_pointers && !ad->oldb && dalloc(&ad->oldb, d.n3()))
+                return MGPU_ERR_NOMEM;
+            double *old = a->device_pointers ? a->PROPOld : ad->oldb;
+            HIPCHK(hipMemcpyAsync(old, P, b3, hipMemcpyDeviceToDevice, s));
+            f.obc.old = old;
+            f.obc.dux = ad->geom.DUX; f.obc.dvy = ad->geom.DVY;
+            f.obc.extv = ad->mid;      // mid is free: no implicit line solves here
+        }
     }
     // the step's own initialisation of TI (AD:1660-1671) reaches the ghost rows
     if (first) HIPCHK(hipMemsetAsync(ad->tig, 0, b3, s));
@@ -769,6 +785,12 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     // TI = null_real on land (AD:1753), kept for the next call
     rc = launch_land_fill(d, ad->der.mask, ad->tig, s);
     if (!rc && sc.impH >= 0) rc = launch_copy_lines(d, sc.impH, ad->out, P, stop, s);
+    // Orlanski's exterior cells (written by the reference inside the open
+    // boundary, before the solve; a solved cell among them takes the solve's
+    // value, as there)
+    if (!rc && orl)
+        rc = launch_orlanski_exterior(d, ad->hydro.bp, ad->hydro.open, ad->geom.KFloor, ad->mid,
+                                      P, stop, s);
     if (!rc && (sc.impH < 0 || sc.hreset))
         rc = launch_copy_back(d, ad->der.mask, ad->out, P, stop, s);
     if (rc) return rc;
@@ -797,6 +819,7 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     if (bflag) return MGPU_ERR_BADARG;            // a cyclic copy from below the Size
     if (!a->device_pointers) {
         HIPCHK(hipMemcpyAsync(prop, ad->prop, b3, hipMemcpyDeviceToHost, s));
+        if (orl) HIPCHK(hipMemcpyAsync(a->PROPOld, ad->oldb, b3, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
     }
     if (cnt[1]) return MGPU_ERR_OPTION;          // method/limiter the reference stops on

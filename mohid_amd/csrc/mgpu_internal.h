@@ -101,6 +101,13 @@ struct ObcFields {
     const int32_t *kfloor = nullptr;  // KFloorZ
     int bc = 0;
     double tdec = 0.0;                // 1 / (1 + DecayTime / DTProp), AD:5420
+    // Orlanski_: PROP before the boundary (PROPOld, AD:5428), the metrics of
+    // LimitMax (AD:5546), and per boundary cell the exterior value the
+    // radiation gives (written, then put in PROP's exterior cell after the
+    // solve by launch_orlanski_exterior)
+    const double *old = nullptr;
+    const double *dux = nullptr, *dvy = nullptr;
+    double *extv = nullptr;
 };
 
 struct SharedFields {     // device pointers shared by all tracers
@@ -152,6 +159,12 @@ int launch_null_gradient(const Dims &d, const uint32_t *list, const int32_t *met
 int launch_cyclic(const Dims &d, const int32_t *bp, const double *ref, const int32_t *kfloor,
                   double *P, unsigned int *flag, const unsigned long long *stop,
                   hipStream_t s);
+// Orlanski_ (AD:5549-5568): the exterior cell of every open boundary cell
+// takes the value the column walk left in extv (OrlanskiCelerity2D's
+// NewField update), before the solved cells are copied back.
+int launch_orlanski_exterior(const Dims &d, const int32_t *bp, const int32_t *open,
+                             const int32_t *kfloor, const double *extv, double *P,
+                             const unsigned long long *stop, hipStream_t s);
 // ptr[cells[m]] = vals ? vals[m] : -1 for m < n.
 int launch_disch_scatter(int n, const uint32_t *cells, const int32_t *vals, int32_t *ptr,
                          hipStream_t s);

@@ -431,7 +431,6 @@ null_gradient_kernel(Dims d, const int4 *__restrict__ meta,
         __syncthreads();
         if (wave == 0) {
             const int cnt = n - c0 < NG_CH ? (int)(n - c0) : NG_CH;
-#pragma unroll 4
             for (int u = 0; u < cnt; ++u) {
                 const int4 m = make_int4(__builtin_amdgcn_readlane(mc.x, u),
                                          __builtin_amdgcn_readlane(mc.y, u),
@@ -503,6 +502,39 @@ __global__ void cyclic_copy_kernel(Dims d, const int32_t *__restrict__ bp,
     }
 }
 }  // namespace
+
+namespace {
+// One thread per WorkSize column; boundary columns write their open levels'
+// exterior cells.  Every boundary cell has its own exterior cell, and no
+// thread reads P.
+__global__ void orlanski_exterior_kernel(Dims d, const int32_t *__restrict__ bp,
+                                         const int32_t *__restrict__ open,
+                                         const int32_t *__restrict__ kfloor,
+                                         const double *__restrict__ extv, double *__restrict__ P,
+                                         const unsigned long long *__restrict__ stop) {
+    if (*stop) return;
+    const int i = d.wILB + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = d.wJLB + blockIdx.y;
+    if (i > d.wIUB || bp[d.idx2(i, j)] != 1) return;
+    int di, dj, ie, je;
+    bool en;
+    if (!orlanski_exterior(d, i, j, di, dj, ie, je, en)) return;
+    for (int k = abs(kfloor[d.idx2(i, j)]); k <= d.wKUB; ++k) {
+        const long x = d.idx(i, j, k);
+        if (open[x] == 1) P[d.idx(ie, je, k)] = extv[x];
+    }
+}
+}  // namespace
+
+int launch_orlanski_exterior(const Dims &d, const int32_t *bp, const int32_t *open,
+                             const int32_t *kfloor, const double *extv, double *P,
+                             const unsigned long long *stop, hipStream_t s) {
+    const int ni = d.wIUB - d.wILB + 1, nj = d.wJUB - d.wJLB + 1;
+    hipLaunchKernelGGL(orlanski_exterior_kernel, dim3((ni + 255) / 256, nj), dim3(256), 0, s, d,
+                       bp, open, kfloor, extv, P, stop);
+    HIPCHK(hipGetLastError());
+    return MGPU_SUCCESS;
+}
 
 int launch_bnd_list(const Dims &d, const int32_t *bp, const int32_t *kfloor, uint32_t *rowcnt,
                     uint32_t *list, int32_t *meta, int32_t *pos, unsigned long long *total,

@@ -664,13 +664,111 @@ __device__ __forceinline__ void column_begin(const Dims &d, const StepScalars &s
     for (int t = 0; t < NT; ++t) { c.Wm[t] = 0.0; c.Gm[t] = 0.0; }
 }
 
-// OpenBoundaryCondition (AD:5369-5665, Orlanski excluded) on the row of
-// open cell x of boundary column r: the last change to the row before the
-// solve.  P is PROP at this point of the step (its stride s).
+// OrlanskiCelerity2D (MF:4129-4490) as AdvectionDiffusion calls it
+// (AD:5549-5568): the new value of cell (i, j, k) -- the exterior cell of a
+// boundary cell -- from the new field nw (PROP, unchanged during the walk)
+// and the old field od (PROPOld); no field is written here.  The radiation
+// of Marchesiello et al. (2000), interior values extrapolated with the
+// weights 0.0546875, -0.2578125, 0.6015625, relaxation times of 300 days,
+// implicit in the normal direction.  The one corner the reference maps to
+// an interior "exterior" cell (IUB, JLB) reads one cell past the Size's i
+// range there, as the reference does: the next column's first cell.
+__device__ __forceinline__ double orlanski_value(const Dims &d, const double *nw,
+                                                 const double *od, const double *ref,
+                                                 const int32_t *cp, int di, int dj, int i, int j,
+                                                 int k, bool en, double limitmax, double dt,
+                                                 double flowvelx) {
+    auto X = [&](int a, int b) { return d.idx(a, b, k); };
+    const int s = en ? -1 : 1;
+    const double i1n = nw[X(i + s * di, j + s * dj)], i1o = od[X(i + s * di, j + s * dj)];
+    const double i2n = nw[X(i + 2 * s * di, j + 2 * s * dj)];
+    const double i3n = nw[X(i + 3 * s * di, j + 3 * s * dj)];
+    const double i6o = od[X(i + s * di + dj, j + s * dj + di)];
+    const double i7o = od[X(i + s * di - dj, j + s * dj - di)];
+    bool none = false, nosw = false;
+    if ((i + s * di + 3 * dj) > d.wIUB || (j + s * dj + 3 * di) > d.wJUB)
+        none = true;
+    else if (cp[X(i + s * di + dj, j + s * dj + di)] * cp[X(i + s * di + 2 * dj, j + s * dj + 2 * di)] *
+             cp[X(i + s * di + 3 * dj, j + s * dj + 3 * di)] == 0)
+        none = true;
+    if ((i + s * di - 3 * dj) < d.wILB || (j + s * dj - 3 * di) < d.wJLB)
+        nosw = true;
+    else if (cp[X(i + s * di - dj, j + s * dj - di)] * cp[X(i + s * di - 2 * dj, j + s * dj - 2 * di)] *
+             cp[X(i + s * di - 3 * dj, j + s * dj - 3 * di)] == 0)
+        nosw = true;
+    double b3o = 0.0, b32o = 0.0, b33o = 0.0, b4o = 0.0, b42o = 0.0, b43o = 0.0;
+    if (!none) {
+        b3o = od[X(i + dj, j + di)]; b32o = od[X(i + 2 * dj, j + 2 * di)];
+        b33o = od[X(i + 3 * dj, j + 3 * di)];
+    }
+    if (!nosw) {
+        b4o = od[X(i - dj, j - di)]; b42o = od[X(i - 2 * dj, j - 2 * di)];
+        b43o = od[X(i - 3 * dj, j - 3 * di)];
+    }
+    const double b5o = od[X(i, j)];
+    const double tv = i1n - i1o, svx = i1n - i2n, auxy = i6o - i7o;
+    double svy;
+    if (auxy * tv > 0.0) {
+        svy = i1o - i7o;
+        if (nosw) svy = 0.0;
+    } else {
+        svy = i6o - i1o;
+        if (none) svy = 0.0;
+    }
+    const double ss = svx * svx + svy * svy;
+    double wcx = 0.0, wcy = 0.0;
+    if (ss > 0.0) {
+        wcx = -(tv * svx) / ss;
+        wcy = -(tv * svy) / ss;
+        if (fabs(wcx) > limitmax) wcx = limitmax * wcx / fabs(wcx);
+        if (fabs(wcy) > limitmax) wcy = limitmax * wcy / fabs(wcy);
+    }
+    const double refv = ref[X(i, j)];
+    const double trel = (double)(86400 * 300);          // TrelaxIn/TrelaxOut defaults
+    wcx = wcx + flowvelx;
+    double adjx;
+    if (wcx > 0.0) {
+        wcx = 4.0 * wcx;
+        adjx = 0.0546875 * i3n - 0.2578125 * i2n + 0.6015625 * i1n;
+    } else {
+        adjx = refv;
+        wcx = 0.0;
+        wcy = 0.0;
+    }
+    double auxint = wcx * adjx;
+    if (wcy >= 0.0 && !nosw) {
+        const double adjy = 0.0546875 * b43o - 0.2578125 * b42o + 0.6015625 * b4o + 0.6015625 * b5o;
+        auxint = auxint - 4.0 * wcy * (b5o - adjy);
+    } else if (wcy < 0.0 && !none) {
+        const double adjy = 0.0546875 * b33o - 0.2578125 * b32o + 0.6015625 * b3o + 0.6015625 * b5o;
+        auxint = auxint - 4.0 * wcy * (adjy - b5o);
+    }
+    double auxb = 1.0;
+    auxb = auxb + wcx * (1.0 - 0.6015625) + dt / trel;
+    return (nw[X(i, j)] + auxint + refv * dt / trel) / auxb;
+}
+
+// Orlanski's di, dj and exterior cell of boundary cell (i, j) (AD:5510-5540);
+// false for a boundary point on no edge (the reference stops there).
+__device__ __forceinline__ bool orlanski_exterior(const Dims &d, int i, int j, int &di, int &dj,
+                                                  int &ie, int &je, bool &en) {
+    if (i == d.wILB || i == d.wIUB) { di = 1; dj = 0; }
+    else if (j == d.wJLB || j == d.wJUB) { di = 0; dj = 1; }
+    else return false;
+    en = !(i == d.wILB || j == d.wJLB);
+    ie = en ? i + di : i - di;
+    je = en ? j + dj : j - dj;
+    return true;
+}
+
+// OpenBoundaryCondition (AD:5369-5665) on the row of open cell x of
+// boundary column r: the last change to the row before the solve.  P is
+// PROP at this point of the step (its stride s).  `bad` is set where the
+// reference stops (Orlanski at a boundary point on no edge).
 __device__ __forceinline__ void obc_row(const Dims &d, const StepScalars &sc,
                                         const SharedFields &f, const double *P, long s, long x,
                                         long r, double vk, double &TI, double &D, double &E,
-                                        double &F) {
+                                        double &F, int &bad) {
     const ObcFields &o = f.obc;
     const int bc = o.bc;
     const long sj = d.sj, sk = d.sk;
@@ -691,15 +789,31 @@ __device__ __forceinline__ void obc_row(const Dims &d, const StepScalars &sc,
             ext = o.ref[x];
         }
     }
-    if (bc == 1 || bc == 7) {                         // MassConservation_, MassConservNullGrad_
+    if (bc == 1 || bc == 7 || bc == 6) {              // MassConservation_, MassConservNullGrad_,
+                                                      // Orlanski_
         if (bc == 1) ext = P[x * s] * (1.0 - o.tdec) + o.ref[x] * o.tdec;
         // FluxAtOpenBoundary (AD:5716-5727)
         const double wf = f.Qx[x] * o.cfu[x] - f.Qx[x + sj] * o.cfu[x + sj] +
                           f.Qy[x] * o.cfv[x] - f.Qy[x + 1] * o.cfv[x + 1] +
                           f.Qz[x] * o.cfw[x] - f.Qz[x + sk] * o.cfw[x + sk] -
                           (f.V[x] - f.Vold[x]) / sc.dt;
+        if (bc == 6) {
+            // the drop-in's one tracer: P is PROP itself (stride 1)
+            const int i = d.ILB + (int)(r % sj), j = d.JLB + (int)(r / sj);
+            const int k = d.KLB + (int)((x - r) / sk);
+            int di, dj, ie, je;
+            bool en;
+            if (!orlanski_exterior(d, i, j, di, dj, ie, je, en)) {
+                bad = 1;
+                return;
+            }
+            const double lim = 10.0 * sc.dt / (o.dux[r] + o.dvy[r]) * 2.0;   // AD:5546
+            ext = orlanski_value(d, P, o.old, o.ref, o.open, di, dj, ie, je, k, en, lim, sc.dt,
+                                 wf * dtv);
+            o.extv[x] = ext;
+        }
         if (wf < 0.0) {                               // water flowing in
-            if (bc == 1) {
+            if (bc != 7) {
                 TI = TI - wf * ext * dtv;
             } else {
                 // NullGradProp (AD:2071-2110)
@@ -816,7 +930,7 @@ __device__ __forceinline__ void column_row(const Dims &d, const StepScalars &sc,
             // boundary column (base is its (i, j) index: KLB is 0), levels
             // |KFloorZ|..KUB (AD:5437-5444)
             if (f.obc.bp[base] == 1 && (mk & M_OPEN) && k >= abs(f.obc.kfloor[base]))
-                obc_row(d, sc, f, ts.P + t, ts.s, x, base, vk, TI, D, E, F);
+                obc_row(d, sc, f, ts.P + t, ts.s, x, base, vk, TI, D, E, F, bad);
         }
         if (land) TI = MGPU_NULL_REAL;
         // THOMASZ_NewType2 forward sweep (MF:4090-4099)

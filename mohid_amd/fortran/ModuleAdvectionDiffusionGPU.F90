@@ -77,6 +77,7 @@ module ModuleAdvectionDiffusionGPU
         integer(c_int32_t) :: BoundaryCondition
         real(c_double)     :: DecayTime
         type(c_ptr)        :: BoundaryPoints2D
+        type(c_ptr)        :: PROPOld
         integer(c_int32_t) :: NoDifFlux
         type(c_ptr)        :: NoFluxU, NoFluxV, NoFluxW
         type(c_ptr)        :: SmallDepths
@@ -213,9 +214,9 @@ contains
     !> advanced by DTProp in place.  Arguments keep the reference's names,
     !> order and meaning.  Options the GPU path does not offload make
     !> STAT = MGPU_ERR_UNSUPPORTED and leave PROP untouched:
-    !>   CellFluxes, NoAdvFlux true (AD:1321-1325), AdvectionNudging, the
-    !>   Orlanski_ open boundary, and an open boundary with one layer and
-    !>   implicit horizontal advection.
+    !>   CellFluxes, NoAdvFlux true (AD:1321-1325), AdvectionNudging, an open
+    !>   boundary with one layer and implicit horizontal advection, and the
+    !>   Orlanski_ open boundary with implicit horizontal advection.
     !> SmallDepths, NoDifFlux (with NoFluxU/V/W), the discharges of
     !> SetDischargesGPU, the open boundary (ReferenceProp, BoundaryCondition,
     !> DecayTime, AD:5369-5665) and the NullGradient_ / CyclicBoundary_ passes
@@ -234,7 +235,7 @@ contains
                                      OpenPoints3D, LandPoints3D, ComputeFacesU3D,             &
                                      ComputeFacesV3D, ComputeFacesW3D, Visc_H, Diff_V,        &
                                      WaterPoints3D, CellFluxes, ReferenceProp,                &
-                                     BoundaryCondition, DecayTime, BoundaryPoints2D,          &
+                                     BoundaryCondition, DecayTime, BoundaryPoints2D, PROPOld, &
                                      SmallDepths, NoAdvFlux, NoDifFlux,                       &
                                      NoFluxU, NoFluxV, NoFluxW,                               &
                                      Optimize, FirstProperty, STAT)
@@ -265,6 +266,7 @@ contains
         integer, optional,                 intent(in)    :: BoundaryCondition
         real(8), optional,                 intent(in)    :: DecayTime
         integer, dimension(:,:), optional, contiguous, target, intent(in) :: BoundaryPoints2D
+        real(8), dimension(:,:,:), optional, contiguous, target, intent(inout) :: PROPOld
         logical, dimension(:,:), optional, intent(in)    :: SmallDepths
         logical, optional,                 intent(in)    :: NoAdvFlux, NoDifFlux
         integer, dimension(:,:,:), optional, contiguous, target, intent(in) :: NoFluxU, NoFluxV
@@ -313,6 +315,8 @@ contains
         if (present(DecayTime)) a%DecayTime = DecayTime
         a%BoundaryPoints2D = c_null_ptr
         if (present(BoundaryPoints2D)) a%BoundaryPoints2D = c_loc(BoundaryPoints2D)
+        a%PROPOld = c_null_ptr
+        if (present(PROPOld)) a%PROPOld = c_loc(PROPOld)
         a%unsupported_NoAdvFlux = 0
         if (present(NoAdvFlux)) a%unsupported_NoAdvFlux = b2i(NoAdvFlux)
         a%NoDifFlux = 0

@@ -1367,7 +1367,7 @@ static int null_grad_prop(const ora_instance *me, const ora_args *a, const doubl
     return 1;
 }
 
-/* OpenBoundaryCondition (AD:5369-5665) without Orlanski: the rows of the
+/* OpenBoundaryCondition (AD:5369-5665): the rows of the
  * open boundary cells, the last change before the solve.  P is PROP at this
  * point of the step (after the line solves in 3-D implicit horizontal
  * advection). */
@@ -1419,12 +1419,12 @@ static int open_boundary_condition(ora_instance *me, const ora_args *a, double *
                         else if (j == w->JLB || j == w->JUB) { di = 0; dj = 1; }
                         else return ORA_ERR_OPTION;           /* 'Orlanski Advection 2' */
                         en = !(i == w->ILB || j == w->JLB);
+                        const double lim = 10.0 * a->DTProp /
+                                           (me->g.DUX[I2(me, i, j)] + me->g.DVY[I2(me, i, j)]) * 2.0;
                         ext = ora_orlanski2d(&me->size, P, a->PROPOld, ref, op, w->ILB, w->IUB,
                                              w->JLB, w->JUB, di, dj, en ? i + di : i - di,
-                                             en ? j + dj : j - dj, k,
-                                             10.0 * a->DTProp /
-                                                 (me->g.DUX[I2(me, i, j)] + me->g.DVY[I2(me, i, j)]) * 2.0,
-                                             en, a->DTProp, wf * dtv);
+                                             en ? j + dj : j - dj, k, en, lim, a->DTProp,
+                                             wf * dtv);
                     }
                     if ((wf < 0.0) != (g_mistake == 15)) {          /* water flowing in */
                         if (bc != BC_MASSCONSERVNULLGRAD) {

@@ -161,7 +161,6 @@ def test_boundary_options_rejected_like_the_reference():
     p = np.array(case.tracers[0], order="F", copy=True)
     bad = [
         (dict(ReferenceProp=ref, BoundaryCondition=3), hy, L.MGPU_ERR_OPTION),   # AD:5817-5826
-        (dict(ReferenceProp=ref, BoundaryCondition=C.ORLANSKI), hy, L.MGPU_ERR_UNSUPPORTED),
         (dict(ReferenceProp=ref, BoundaryCondition=C.IMPOSED_VALUE), case.hydro,
          L.MGPU_ERR_BADARG),                                                     # no BoundaryPoints2D
         (dict(BoundaryCondition=C.CYCLIC), hy, L.MGPU_ERR_BADARG),               # no ReferenceProp
@@ -188,5 +187,70 @@ def test_one_layer_implicit_open_boundary_stays_on_the_cpu():
                BoundaryCondition=C.IMPOSED_VALUE, DecayTime=0.0)
     with pytest.raises(L.MgpuError) as ei:
         gpu(p, hy, prm)
+    assert ei.value.code == L.MGPU_ERR_UNSUPPORTED
+    gpu.kill()
+
+
+@pytest.mark.parametrize("name,optimize", [("tvd_land_sigma", False), ("tvd_ragged_opt", True),
+                                           ("upwind_land_ragged", False)])
+def test_orlanski_drop_in(name, optimize):
+    """Orlanski_ (AD:5506-5568): the radiated exterior value enters the rows
+    as MassConservation's, PROP's exterior cells take it, PROPOld is PROP
+    before the boundary; every array against the oracle, every step."""
+    _, AD = _mods()
+    case = make(name)
+    bp = _boundary_points(case)
+    hy = _hydro(case, bp)
+    rng = np.random.default_rng(61)
+    refs = [np.asfortranarray(rng.uniform(0.5, 30.0, size=case.shape)) for _ in case.tracers]
+    gpu = AD(case.size, case.worksize, case.geom)
+    inst = ora.Instance(case, nthreads=1)
+    pg = [np.array(p, order="F", copy=True) for p in case.tracers]
+    po = [np.array(p, order="F", copy=True) for p in case.tracers]
+    og = np.asfortranarray(np.zeros(case.shape))
+    oo = np.asfortranarray(np.zeros(case.shape))
+    bitwise = True
+    for s in range(3):
+        for t in range(len(pg)):
+            prm = dict(case.params, Optimize=optimize, BoundaryCondition=C.ORLANSKI,
+                       DecayTime=1800.0, ReferenceProp=refs[t])
+            assert inst.step(po[t], first_property=t == 0, now=(s + 1) * prm["DTProp"],
+                             hydro=hy, **dict(prm, PROPOld=oo)) == 0
+            gpu(pg[t], hy, dict(prm, PROPOld=og), first_property=t == 0)
+            err = max(_rel_err(pg[t], po[t]), _rel_err(og, oo))
+            assert err <= RTOL, (s, t, err)
+            bitwise &= bool(np.array_equal(pg[t], po[t]) and np.array_equal(og, oo))
+    gpu.kill()
+    inst.close()
+    print(f"{name} optimize={optimize} Orlanski: bit-identical={bitwise}")
+
+
+def test_orlanski_rejections():
+    L, AD = _mods()
+    case = make("upwind_flat")
+    gpu = AD(case.size, case.worksize, case.geom)
+    ref = np.asfortranarray(np.ones(case.shape))
+    p = np.array(case.tracers[0], order="F", copy=True)
+    prm = dict(case.params, ReferenceProp=ref, BoundaryCondition=C.ORLANSKI, DecayTime=0.0)
+    # a boundary point on no edge: the reference stops ('Orlanski Advection 2')
+    hy = _hydro(case, _boundary_points(case, interior=2, seed=4))
+    with pytest.raises(L.MgpuError) as ei:
+        gpu(p, hy, dict(prm, PROPOld=np.asfortranarray(np.zeros(case.shape))))
+    assert ei.value.code == L.MGPU_ERR_OPTION
+    inst = ora.Instance(case, nthreads=1)
+    q = np.array(case.tracers[0], order="F", copy=True)
+    assert inst.step(q, hydro=hy, **dict(prm, PROPOld=np.asfortranarray(
+        np.zeros(case.shape)))) == ora.ERR_OPTION
+    inst.close()
+    # no PROPOld to write
+    hy = _hydro(case, _boundary_points(case))
+    with pytest.raises(L.MgpuError) as ei:
+        gpu(np.array(case.tracers[0], order="F", copy=True), hy, prm)
+    assert ei.value.code == L.MGPU_ERR_BADARG
+    # with implicit horizontal advection
+    with pytest.raises(L.MgpuError) as ei:
+        gpu(np.array(case.tracers[0], order="F", copy=True), hy,
+            dict(prm, PROPOld=np.asfortranarray(np.zeros(case.shape)),
+                 AdvMethodH=C.UPWIND_ORDER1, ImpExp_AdvXX=1.0))
     assert ei.value.code == L.MGPU_ERR_UNSUPPORTED
     gpu.kill()

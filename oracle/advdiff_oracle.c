@@ -86,6 +86,8 @@ struct ora_instance {
     double ext_SchmidtCoef_V, ext_SchmidtBackground_V, ext_Schmidt_H;
     int first_call;
     ora_disch disch;
+    /* T_CellFluxes (AD:184-194), allocated on the first call with CellFluxes */
+    double *AFX, *AFY, *AFZ, *DFX, *DFY, *DFZ;
 };
 
 #define I3(me, i, j, k) (((long)(i) - (me)->size.ILB) + (me)->nI * (((long)(j) - (me)->size.JLB) + (me)->nJ * ((long)(k) - (me)->size.KLB)))
@@ -516,6 +518,7 @@ void ora_kill(ora_instance *me)
     free(me->xC); free(me->xD); free(me->xE); free(me->xF);
     free(me->yC); free(me->yD); free(me->yE); free(me->yF);
     free(me->zC); free(me->zD); free(me->zE); free(me->zF);
+    free(me->AFX); free(me->AFY); free(me->AFZ); free(me->DFX); free(me->DFY); free(me->DFZ);
     free(me->vecW); free(me->vecG);
     free(me);
 }
@@ -529,6 +532,12 @@ const double *ora_internal(const ora_instance *me, const char *name)
     if (!strcmp(name, "KhX")) return me->KhX;
     if (!strcmp(name, "KhY")) return me->KhY;
     if (!strcmp(name, "KzZ")) return me->KzZ;
+    if (!strcmp(name, "AdvFluxX")) return me->AFX;
+    if (!strcmp(name, "AdvFluxY")) return me->AFY;
+    if (!strcmp(name, "AdvFluxZ")) return me->AFZ;
+    if (!strcmp(name, "DifFluxX")) return me->DFX;
+    if (!strcmp(name, "DifFluxY")) return me->DFY;
+    if (!strcmp(name, "DifFluxZ")) return me->DFZ;
     return NULL;
 }
 
@@ -800,6 +809,138 @@ int ora_unset_discharges(ora_instance *me)
 }
 
 /* AD:5156-5214 HorizontalDiffusionXX / AD:5216-5265 HorizontalDiffusionXX2 */
+/* ------------------------------------------------------------------------ */
+/* Cell fluxes (AD:3356-3954): the diffusive and advective fluxes through
+ * every computed face, the explicit part with PROP before the solve and the
+ * implicit part with PROP after it (AD:1885-1912), each weighted. */
+static void calc_hdif_flux(ora_instance *me, const ora_args *a, const double *P, int yy)
+{
+    /* CalcHorizontalDifFluxXX/XX2 (AD:3784-3867), YY/YY2 (AD:3871-3954) */
+    const ora_size3d *w = &me->ws;
+    const int32_t *cf = yy ? a->ComputeFacesV3D : a->ComputeFacesU3D;
+    double *fl = yy ? me->DFY : me->DFX;
+    int i, j, k;
+    for (k = w->KLB; k <= w->KUB; k++)
+        for (j = w->JLB; j <= w->JUB; j++)
+            for (i = w->ILB; i <= w->IUB; i++) {
+                const long x = I3(me, i, j, k);
+                const long xm = yy ? I3(me, i - 1, j, k) : I3(me, i, j - 1, k);
+                double c;
+                if (cf[x] != 1) continue;
+                if (a->Optimize) c = yy ? me->DHV[x] : me->DHU[x];
+                else if (yy) c = me->KhY[x] * me->g.AreaV[x] / me->g.DZY[I2(me, i - 1, j)];
+                else c = me->KhX[x] * me->g.AreaU[x] / me->g.DZX[I2(me, i, j - 1)];
+                fl[x] = g_mistake == 18 ? fl[x] + c * (P[x] - P[xm]) : fl[x] - c * (P[x] - P[xm]);
+            }
+}
+
+static void calc_hadv_flux(ora_instance *me, const ora_args *a, const double *P, int yy,
+                           double wt)
+{
+    /* CalcHorizontalAdvFluxXX/_opt (AD:3479-3567), YY/_opt (AD:3571-3664) */
+    const ora_size3d *w = &me->ws;
+    const int32_t *cf = yy ? a->ComputeFacesV3D : a->ComputeFacesU3D;
+    const double *C = yy ? me->yC : me->xC, *D = yy ? me->yD : me->xD;
+    const double *E = yy ? me->yE : me->xE, *F = yy ? me->yF : me->xF;
+    double *fl = yy ? me->AFY : me->AFX;
+    int i, j, k;
+    for (k = w->KLB; k <= w->KUB; k++)
+        for (j = w->JLB; j <= w->JUB; j++)
+            for (i = w->ILB; i <= w->IUB; i++) {
+                const long x = I3(me, i, j, k);
+                const long s = yy ? 1 : me->nI;
+                if (cf[x] != 1) continue;
+                if (a->Optimize)
+                    fl[x] = fl[x] + wt * (D[x] * P[x - s] + E[x] * P[x]);
+                else
+                    fl[x] = fl[x] + wt * (C[x] * P[x - 2 * s] + D[x] * P[x - s] + E[x] * P[x] +
+                                          F[x] * P[x + s]);
+            }
+}
+
+static void calc_vadv_flux(ora_instance *me, const ora_args *a, const double *P, double wt)
+{
+    /* CalcVerticalAdvFlux (AD:3356-3405), _opt (AD:3409-3475) */
+    const ora_size3d *w = &me->ws;
+    const long sk = me->nI * me->nJ;
+    int i, j, k;
+    if (a->Optimize && me->docycle == 1) {
+        for (j = w->JLB; j <= w->JUB; j++)
+            for (i = w->ILB; i <= w->IUB; i++) {
+                int kb;
+                double pb;
+                if (a->ComputeFacesW3D[I3(me, i, j, w->KUB)] != 1) continue;
+                kb = me->g.KFloorZ[I2(me, i, j)];
+                pb = P[I3(me, i, j, kb)];
+                for (k = kb + 1; k <= w->KUB; k++) {
+                    const long x = I3(me, i, j, k);
+                    me->AFZ[x] = me->AFZ[x] + wt * (me->zD[x] * pb + me->zE[x] * P[x]);
+                    pb = P[x];
+                }
+            }
+        return;
+    }
+    for (k = w->KLB; k <= w->KUB; k++)
+        for (j = w->JLB; j <= w->JUB; j++)
+            for (i = w->ILB; i <= w->IUB; i++) {
+                const long x = I3(me, i, j, k);
+                if (a->ComputeFacesW3D[x] != 1) continue;
+                if (a->Optimize)
+                    me->AFZ[x] = me->AFZ[x] + wt * (me->zD[x] * P[x - sk] + me->zE[x] * P[x]);
+                else
+                    me->AFZ[x] = me->AFZ[x] + wt * (me->zC[x] * P[x - 2 * sk] + me->zD[x] * P[x - sk] +
+                                                   me->zE[x] * P[x] + me->zF[x] * P[x + sk]);
+            }
+}
+
+static void calc_vdif_flux(ora_instance *me, const ora_args *a, const double *P, double wt)
+{
+    /* CalcVerticalDifFlux (AD:3668-3711), CalcVerticalDifFlux2 (AD:3715-3780) */
+    const ora_size3d *w = &me->ws;
+    const long sk = me->nI * me->nJ;
+    int i, j, k;
+    if (a->Optimize && me->docycle == 1) {
+        for (j = w->JLB; j <= w->JUB; j++)
+            for (i = w->ILB; i <= w->IUB; i++) {
+                int kb;
+                double pb;
+                if (a->ComputeFacesW3D[I3(me, i, j, w->KUB)] != 1) continue;
+                kb = me->g.KFloorZ[I2(me, i, j)];
+                pb = P[I3(me, i, j, kb)];
+                for (k = kb + 1; k <= w->KUB; k++) {
+                    const long x = I3(me, i, j, k);
+                    me->DFZ[x] = me->DFZ[x] - wt * me->DVC[x] * (P[x] - pb);
+                    pb = P[x];
+                }
+            }
+        return;
+    }
+    for (k = w->KLB; k <= w->KUB; k++)
+        for (j = w->JLB; j <= w->JUB; j++)
+            for (i = w->ILB; i <= w->IUB; i++) {
+                const long x = I3(me, i, j, k);
+                if (a->ComputeFacesW3D[x] != 1) continue;
+                if (a->Optimize)
+                    me->DFZ[x] = me->DFZ[x] - wt * me->DVC[x] * (P[x] - P[x - sk]);
+                else
+                    me->DFZ[x] = me->DFZ[x] - wt * me->KzZ[x] * me->g.DUX[I2(me, i, j)] *
+                                                  me->g.DVY[I2(me, i, j)] /
+                                                  me->g.DZZ[I3(me, i, j, k - 1)] *
+                                                  (P[x] - P[x - sk]);
+            }
+}
+
+/* AD:1885-1912: the implicit parts, with PROP after the solve */
+static void fluxes_after_solve(ora_instance *me, const ora_args *a, const double *P)
+{
+    const int kub = me->ws.KUB;
+    if (!a->CellFluxes) return;
+    if (a->ImpExp_AdvV > 0.0 && kub > 1) calc_vadv_flux(me, a, P, a->ImpExp_AdvV);
+    if (a->ImpExp_DifV > 0.0 && kub > 1) calc_vdif_flux(me, a, P, a->ImpExp_DifV);
+    if (a->ImpExp_AdvXX == IMPLICIT) calc_hadv_flux(me, a, P, 0, a->ImpExp_AdvXX);
+    if (a->ImpExp_AdvYY == IMPLICIT) calc_hadv_flux(me, a, P, 1, a->ImpExp_AdvYY);
+}
+
 static void horizontal_diffusion_xx(ora_instance *me, const ora_args *a, const double *P)
 {
     const ora_size3d *w = &me->ws;
@@ -824,6 +965,7 @@ static void horizontal_diffusion_xx(ora_instance *me, const ora_args *a, const d
                 }
             }
     }
+    if (a->CellFluxes) calc_hdif_flux(me, a, P, 0);          /* AD:5210, 5256 */
 }
 
 /* AD:5313-5365 HorizontalDiffusionYY / AD:5267-5309 HorizontalDiffusionYY2 */
@@ -854,6 +996,7 @@ static void horizontal_diffusion_yy(ora_instance *me, const ora_args *a, const d
             }
         }
     }
+    if (a->CellFluxes) calc_hdif_flux(me, a, P, 1);          /* AD:5305, 5361 */
 }
 
 /* AD:4368-4540 HorizontalAdvectionXX (+ _Explicit, AD:4544-4582) */
@@ -926,6 +1069,8 @@ static int horizontal_advection_xx(ora_instance *me, const ora_args *a, const do
                     me->TI[xm] = me->TI[xm] - f * dt / a->VolumeZ[xm];
             }
     }
+    if (a->CellFluxes)                                           /* AD:4531-4536 */
+        calc_hadv_flux(me, a, P, 0, 1.0 - a->ImpExp_AdvXX);
     return rc;
 }
 
@@ -998,6 +1143,8 @@ static int horizontal_advection_yy(ora_instance *me, const ora_args *a, const do
             }
         }
     }
+    if (a->CellFluxes)                                           /* AD:4900-4905 */
+        calc_hadv_flux(me, a, P, 1, 1.0 - a->ImpExp_AdvYY);
     return rc;
 }
 
@@ -1030,6 +1177,8 @@ static void vertical_diffusion(ora_instance *me, const ora_args *a, const double
             }
         }
     }
+    if (a->CellFluxes && a->ImpExp_DifV < 1.0)                /* AD:2776 */
+        calc_vdif_flux(me, a, P, 1.0 - a->ImpExp_DifV);
 }
 
 /* AD:2788-2937 VerticalDiffusion2 (optimised path) */
@@ -1099,6 +1248,8 @@ static void vertical_diffusion2(ora_instance *me, const ora_args *a, const doubl
             }
         }
     }
+    if (a->CellFluxes && a->ImpExp_DifV < 1.0)                /* AD:2930 */
+        calc_vdif_flux(me, a, P, 1.0 - a->ImpExp_DifV);
 }
 
 /* AD:2941-3140 VerticalAdvection (+ _ExplicitScheme, AD:3144-3207) */
@@ -1239,6 +1390,8 @@ static int vertical_advection(ora_instance *me, const ora_args *a, const double 
     } else {
         return ORA_ERR_OPTION;  /* AD:3131 stop ERR01 */
     }
+    if (a->CellFluxes && a->ImpExp_AdvV < 1.0)                    /* AD:3129-3134 */
+        calc_vadv_flux(me, a, P, 1.0 - a->ImpExp_AdvV);
     return rc;
 }
 
@@ -1619,7 +1772,9 @@ static int iteration(ora_instance *me, const ora_args *a, double *P)
                                   0, 1, me->D, me->E, me->F, me->TI, P, me->vecW, me->vecG, NULL)
                    : ora_thomas3d(&me->size, w->JLB, w->JUB, w->ILB, w->IUB, w->KLB, w->KUB,
                                   1, 0, me->D, me->E, me->F, me->TI, P, me->vecW, me->vecG, NULL);
-        return rc ? rc : boundary_post(me, a, P);
+        if (rc || (rc = boundary_post(me, a, P))) return rc;
+        fluxes_after_solve(me, a, P);
+        return 0;
     }
     /* AD:1858-1867 THOMASZ_NewType2 over WorkSize, OpenMP over j */
     {
@@ -1643,7 +1798,10 @@ static int iteration(ora_instance *me, const ora_args *a, double *P)
                                    me->F, me->TI, P, W, G);
         }
     }
-    return boundary_post(me, a, P);
+    rc = boundary_post(me, a, P);
+    if (rc) return rc;
+    fluxes_after_solve(me, a, P);
+    return 0;
 }
 
 /* AD:1108-1509 AdvectionDiffusion */
@@ -1716,6 +1874,16 @@ int ora_advection_diffusion(ora_instance *me, const ora_args *a, double *prop)
     if (optimize && a->FirstProperty) {
         compute_difh_constants(me, &aa);
         compute_difv_constants(me, &aa);
+    }
+    /* AD:1458-1469: the fluxes of this call start from zero */
+    if (a->CellFluxes) {
+        double **fx[6] = {&me->AFX, &me->AFY, &me->AFZ, &me->DFX, &me->DFY, &me->DFZ};
+        int q;
+        for (q = 0; q < 6; q++) {
+            if (!*fx[q]) *fx[q] = (double *)malloc(sizeof(double) * (size_t)me->n3);
+            if (!*fx[q]) return ORA_ERR_BADARG;
+            fill(*fx[q], me->n3, 0.0);
+        }
     }
     rc = iteration(me, &aa, prop);
     me->last_calc = a->Now;

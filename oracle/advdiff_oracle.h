@@ -83,6 +83,7 @@ typedef struct {
     double DecayTime;
     const int32_t *BoundaryPoints2D;              /* 2-D (Size i, j) */
     double *PROPOld;                              /* 3-D, written with Orlanski_ (AD:5428) */
+    int32_t CellFluxes;                           /* AD:1402-1407: the flux diagnostics */
 } ora_args;
 
 typedef struct ora_instance ora_instance;
@@ -109,7 +110,9 @@ int ora_set_discharges(ora_instance *me, int number, const double *flow, const d
                        const int32_t *bypass);
 int ora_unset_discharges(ora_instance *me);
 
-/* Read back an internal array after a call (names: "D","E","F","TI"). */
+/* Read back an internal array after a call (names: "D","E","F","TI", and
+ * with CellFluxes "AdvFluxX/Y/Z", "DifFluxX/Y/Z": GetAdvFlux / GetDifFlux,
+ * AD:697-860). */
 const double *ora_internal(const ora_instance *me, const char *name);
 
 /* Test-only: plant deliberate mistake `id` (0 = none) in the restatement,
@@ -135,7 +138,8 @@ const double *ora_internal(const ora_instance *me, const char *name);
  *  16 cyclic boundary copies the row next to the boundary instead of the
  *     one before the opposite boundary
  *  17 Orlanski: the cross-boundary space variability taken on the wrong
- *     side of the interior point */
+ *     side of the interior point
+ *  18 horizontal diffusive cell flux with the sign of the gradient */
 void ora_plant_mistake(int id);
 
 /* OrlanskiCelerity2D (ModuleFunctions.F90:4129-4490) with the optional

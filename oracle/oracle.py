@@ -55,7 +55,7 @@ class Args(ct.Structure):
         ("NoDifFlux", ct.c_int32), ("NoFluxU", _ip), ("NoFluxV", _ip), ("NoFluxW", _ip),
         ("SmallDepths", _ip),
         ("ReferenceProp", _dp), ("BoundaryCondition", ct.c_int32), ("DecayTime", ct.c_double),
-        ("BoundaryPoints2D", _ip), ("PROPOld", _dp),
+        ("BoundaryPoints2D", _ip), ("PROPOld", _dp), ("CellFluxes", ct.c_int32),
     ]
 
 
@@ -212,7 +212,8 @@ class Instance:
                     self.now if now is None else now, int(bool(p.get("NoDifFlux", False))),
                     opt_i("NoFluxU"), opt_i("NoFluxV"), opt_i("NoFluxW"), opt_i("SmallDepths"),
                     ref, int(p.get("BoundaryCondition", 0)),
-                    float(p.get("DecayTime", -9.9e15)), opt_i("BoundaryPoints2D"), old)
+                    float(p.get("DecayTime", -9.9e15)), opt_i("BoundaryPoints2D"), old,
+                    int(bool(p.get("CellFluxes", False))))
         assert prop.flags.f_contiguous and prop.dtype == np.float64
         return lib().ora_advection_diffusion(self.h, ct.byref(args), _ptr(prop, _dp))
 

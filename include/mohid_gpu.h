@@ -97,8 +97,9 @@ typedef struct {
     const double *Visc_H, *Diff_V;
     const int32_t *WaterPoints3D;
     int32_t Optimize, FirstProperty;
-    /* present-and-set optional arguments the GPU path does not offload */
-    int32_t unsupported_CellFluxes;
+    int32_t CellFluxes;                    /* the flux diagnostics (AD:3356-3954), read back with
+                                              mgpu_ad_get_fluxes */
+    /* present-and-set optional argument the GPU path does not offload */
     int32_t unsupported_NoAdvFlux;         /* its YY branch zeroes the XX coefficients, AD:4804-4812 */
     /* open boundary (OpenBoundaryCondition, AD:5369-5665: on when
      * ReferenceProp is given, AD:5816) and the conditions applied after the
@@ -134,6 +135,15 @@ int mgpu_ad_set_geometry(mgpu_ad *ad, const mgpu_geometry *geometry);
  * Host mode copies the inputs in and PROP out per call (PCIe-bound: use the
  * device mode or the batched transport for performance). */
 int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *args, double *prop);
+
+/* GetAdvFlux / GetDifFlux (AD:697-860): the cell fluxes of the last
+ * mgpu_ad_step call made with CellFluxes, over Size (the faces between
+ * j-1 and j, i-1 and i, k-1 and k).  Each non-null array receives its
+ * flux (device_pointers: device arrays).  MGPU_ERR_STATE when the last call
+ * computed none. */
+int mgpu_ad_get_fluxes(mgpu_ad *ad, double *AdvFluxX, double *AdvFluxY, double *AdvFluxZ,
+                       double *DifFluxX, double *DifFluxY, double *DifFluxZ,
+                       int32_t device_pointers);
 
 /* SetDischarges (AD:978-1034) / UnSetDischarges (AD:1040-1086): the
  * discharges of the following mgpu_ad_step calls, in the reference's

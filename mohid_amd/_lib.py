@@ -70,7 +70,7 @@ class ADArgs(ct.Structure):
         ("ComputeFacesU3D", _ip), ("ComputeFacesV3D", _ip), ("ComputeFacesW3D", _ip),
         ("Visc_H", _dp), ("Diff_V", _dp), ("WaterPoints3D", _ip),
         ("Optimize", ct.c_int32), ("FirstProperty", ct.c_int32),
-        ("unsupported_CellFluxes", ct.c_int32), ("unsupported_NoAdvFlux", ct.c_int32),
+        ("CellFluxes", ct.c_int32), ("unsupported_NoAdvFlux", ct.c_int32),
         ("ReferenceProp", _dp), ("BoundaryCondition", ct.c_int32), ("DecayTime", ct.c_double),
         ("BoundaryPoints2D", _ip), ("PROPOld", _dp),
         ("NoDifFlux", ct.c_int32), ("NoFluxU", _ip), ("NoFluxV", _ip), ("NoFluxW", _ip),
@@ -90,6 +90,7 @@ class TransportOpts(ct.Structure):
 EXPORTS = (
     "mgpu_ad_create", "mgpu_ad_set_geometry", "mgpu_ad_step", "mgpu_ad_destroy",
     "mgpu_ad_zero_pivots", "mgpu_ad_set_discharges", "mgpu_ad_unset_discharges",
+    "mgpu_ad_get_fluxes",
     "mgpu_thomasz", "mgpu_thomas3d", "mgpu_transport_status",
     "mgpu_transport_create", "mgpu_transport_set_hydro", "mgpu_transport_set_tracer",
     "mgpu_transport_get_tracer", "mgpu_transport_run", "mgpu_transport_tracer_ptr",
@@ -130,6 +131,7 @@ def lib():
     i32 = ct.c_int32
     L.mgpu_ad_set_discharges.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp]
     L.mgpu_ad_unset_discharges.argtypes = [vp]
+    L.mgpu_ad_get_fluxes.argtypes = [vp] + [vp] * 6 + [i32]
     L.mgpu_thomas3d.argtypes = [ct.POINTER(Size3D), i32, i32, i32, i32, i32, i32, i32, i32, vp,
                                 vp, vp, vp, vp, i32, vp]
     L.mgpu_transport_status.argtypes = [vp]

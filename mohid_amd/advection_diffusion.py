@@ -182,6 +182,25 @@ class AdvectionDiffusion:
     def unset_discharges(self) -> None:
         L.check("UnSetDischarges", L.lib().mgpu_ad_unset_discharges(self._h))
 
+    def fluxes(self, like=None) -> Dict:
+        """GetAdvFlux / GetDifFlux (AD:697-860): the six cell-flux arrays of the
+        last call made with CellFluxes, as Fortran-ordered arrays over Size
+        (or device tensors shaped like `like` when it is one)."""
+        names = ("AdvFluxX", "AdvFluxY", "AdvFluxZ", "DifFluxX", "DifFluxY", "DifFluxZ")
+        if like is not None and _is_device(like):
+            import torch
+            out = {n: torch.empty_like(like) for n in names}
+            ptrs = [ct.c_void_p(out[n].data_ptr()) for n in names]
+            dev = 1
+        else:
+            z = self._size
+            shape = (z.IUB - z.ILB + 1, z.JUB - z.JLB + 1, z.KUB - z.KLB + 1)
+            out = {n: np.zeros(shape, order="F") for n in names}
+            ptrs = [ct.c_void_p(out[n].ctypes.data) for n in names]
+            dev = 0
+        L.check("GetAdvFlux", L.lib().mgpu_ad_get_fluxes(self._h, *ptrs, dev))
+        return out
+
     def zero_pivots(self) -> int:
         c = ct.c_uint64()
         L.check("mgpu_ad_zero_pivots", L.lib().mgpu_ad_zero_pivots(self._h, ct.byref(c)))

@@ -301,8 +301,7 @@ int check_options(const mgpu_ad_args *a) {
     // HorizontalAdvectionXX/YY ERR01 (AD:4527, 4887)
     if ((a->ImpExp_AdvXX != 0.0 && !impXX) || (a->ImpExp_AdvYY != 0.0 && !impYY))
         return MGPU_ERR_OPTION;
-    if (a->AdvectionNudging || a->unsupported_CellFluxes || a->unsupported_NoAdvFlux)
-        return MGPU_ERR_UNSUPPORTED;
+    if (a->AdvectionNudging || a->unsupported_NoAdvFlux) return MGPU_ERR_UNSUPPORTED;
     // Set_Internal_State ERR01 (AD:5816-5829): an open boundary needs a known
     // condition; Orlanski's radiation (OrlanskiCelerity2D) is not offloaded
     const int bc = a->BoundaryCondition;
@@ -441,6 +440,8 @@ struct mgpu_ad {
     bool have_faces = false;
     DevDischarges disch;
     DevBoundary bnd;
+    FluxArrays flux{};                      // CellFluxes, allocated on first use
+    bool have_flux = false;                 // the last call computed them
     double last_dt = 0.0, last_schmidtH = 0.0;   // Set_Internal_State's memory (AD:5746)
     unsigned long long last_zero_pivots = 0;
 };
@@ -502,6 +503,8 @@ extern "C" int mgpu_ad_destroy(mgpu_ad *ad) {
     dfree(ad->tig); dfree(ad->mid); dfree(ad->Rb); dfree(ad->oldb);
     ad->disch.release();
     ad->bnd.release();
+    dfree(ad->flux.afx); dfree(ad->flux.afy); dfree(ad->flux.afz);
+    dfree(ad->flux.dfx); dfree(ad->flux.dfy); dfree(ad->flux.dfz);
     if (ad->stream) (void)hipStreamDestroy(ad->stream);
     delete ad;
     return MGPU_SUCCESS;
@@ -654,6 +657,22 @@ extern "C" int mgpu_ad_set_discharges(mgpu_ad *ad, const double *DischFlow,
     return MGPU_SUCCESS;
 }
 
+extern "C" int mgpu_ad_get_fluxes(mgpu_ad *ad, double *AdvFluxX, double *AdvFluxY,
+                                  double *AdvFluxZ, double *DifFluxX, double *DifFluxY,
+                                  double *DifFluxZ, int32_t device_pointers) {
+    if (!ad) return MGPU_OFF_ERR;
+    if (!ad->have_flux) return MGPU_ERR_STATE;      // GetAdvFlux ERR01 (AD:733)
+    const size_t b3 = sizeof(double) * (size_t)ad->d.n3();
+    const hipMemcpyKind kind = device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    double *dst[6] = {AdvFluxX, AdvFluxY, AdvFluxZ, DifFluxX, DifFluxY, DifFluxZ};
+    const double *src[6] = {ad->flux.afx, ad->flux.afy, ad->flux.afz,
+                            ad->flux.dfx, ad->flux.dfy, ad->flux.dfz};
+    for (int q = 0; q < 6; ++q)
+        if (dst[q]) HIPCHK(hipMemcpyAsync(dst[q], src[q], b3, kind, ad->stream));
+    HIPCHK(hipStreamSynchronize(ad->stream));
+    return MGPU_SUCCESS;
+}
+
 extern "C" int mgpu_ad_unset_discharges(mgpu_ad *ad) {
     if (!ad) return MGPU_OFF_ERR;
     int rc = disch_clear(ad);
@@ -680,6 +699,18 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     // line solves those are line rows (not offloaded)
     const bool orl = a->ReferenceProp && a->BoundaryCondition == 6;
     if (orl && sc.impH >= 0) return MGPU_ERR_UNSUPPORTED;
+    // cell fluxes with implicit horizontal advection are not offloaded (the
+    // line-solve records share their scratch)
+    const bool cfl = a->CellFluxes != 0;
+    if (cfl && sc.impH >= 0) return MGPU_ERR_UNSUPPORTED;
+    ad->have_flux = false;
+    if (cfl && !ad->flux.afx) {
+        FluxArrays &q = ad->flux;
+        HIPCHK(hipStreamSynchronize(s));
+        if (dalloc(&q.afx, d.n3()) || dalloc(&q.afy, d.n3()) || dalloc(&q.afz, d.n3()) ||
+            dalloc(&q.dfx, d.n3()) || dalloc(&q.dfy, d.n3()) || dalloc(&q.dfz, d.n3()))
+            return MGPU_ERR_NOMEM;
+    }
     const bool post = a->BoundaryCondition == 4 || a->BoundaryCondition == 8;
     if (post && (rc = ad->bnd.alloc(d))) return rc;
 
@@ -785,6 +816,17 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     // TI = null_real on land (AD:1753), kept for the next call
     rc = launch_land_fill(d, ad->der.mask, ad->tig, s);
     if (!rc && sc.impH >= 0) rc = launch_copy_lines(d, sc.impH, ad->out, P, stop, s);
+    // the cell fluxes' explicit parts, with PROP before the solve (AD:1458-1468
+    // zero them first)
+    if (!rc && cfl) {
+        double *arr[6] = {ad->flux.afx, ad->flux.afy, ad->flux.afz,
+                          ad->flux.dfx, ad->flux.dfy, ad->flux.dfz};
+        for (double *q : arr) HIPCHK(hipMemsetAsync(q, 0, b3, s));
+        rc = launch_cell_flux_explicit(d, sc, f, P, ad->hydro.diffv, ad->geom.DZZ,
+                                       a->SchmidtCoef_V, a->SchmidtBackground_V, a->NullDif,
+                                       a->ImpExp_AdvV, a->ImpExp_DifV, ad->flux, ad->Rb,
+                                       reinterpret_cast<unsigned int *>(ad->der.counters + 1), s);
+    }
     // Orlanski's exterior cells (written by the reference inside the open
     // boundary, before the solve; a solved cell among them takes the solve's
     // value, as there)
@@ -811,12 +853,21 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
         if (rc) return rc;
         HIPCHK(hipMemcpyAsync(&bflag, ad->bnd.flag, sizeof(bflag), hipMemcpyDeviceToHost, s));
     }
+    // the cell fluxes' implicit parts, with PROP after the solve and the
+    // boundary passes (AD:1885-1912)
+    if (cfl) {
+        rc = launch_cell_flux_implicit(d, sc, f, P, ad->hydro.diffv, ad->geom.DZZ,
+                                       a->SchmidtCoef_V, a->SchmidtBackground_V, a->NullDif,
+                                       a->ImpExp_AdvV, a->ImpExp_DifV, ad->flux, ad->Rb, s);
+        if (rc) return rc;
+    }
     unsigned long long cnt[4];
     HIPCHK(hipMemcpyAsync(cnt, ad->der.counters, sizeof(cnt), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     ad->last_zero_pivots = cnt[0];
     if (cnt[3]) return MGPU_ERR_PIVOT;            // THOMAS_3D ERR10: PROP not written
     if (bflag) return MGPU_ERR_BADARG;            // a cyclic copy from below the Size
+    ad->have_flux = cfl;
     if (!a->device_pointers) {
         HIPCHK(hipMemcpyAsync(prop, ad->prop, b3, hipMemcpyDeviceToHost, s));
         if (orl) HIPCHK(hipMemcpyAsync(a->PROPOld, ad->oldb, b3, hipMemcpyDeviceToHost, s));

@@ -165,6 +165,24 @@ int launch_cyclic(const Dims &d, const int32_t *bp, const double *ref, const int
 int launch_orlanski_exterior(const Dims &d, const int32_t *bp, const int32_t *open,
                              const int32_t *kfloor, const double *extv, double *P,
                              const unsigned long long *stop, hipStream_t s);
+// CellFluxes (AD:3356-3954): the six flux arrays of a drop-in call.
+struct FluxArrays {
+    double *afx, *afy, *afz, *dfx, *dfy, *dfz;
+};
+// The explicit parts (with PROP before the solve, P): the horizontal
+// advective and diffusive fluxes, and the vertical ones with weights
+// 1 - ImpExp_AdvV and 1 - ImpExp_DifV; the vertical advection coefficients
+// of every W face go to coef (4 per cell) for the implicit part.
+int launch_cell_flux_explicit(const Dims &d, const StepScalars &sc, const SharedFields &f,
+                              const double *P, const double *diffv, const double *DZZ,
+                              double sV, double bV, int nulldif, double thA, double thD,
+                              const FluxArrays &o, double *coef, unsigned int *bad,
+                              hipStream_t s);
+// The implicit parts (AD:1885-1912), with PROP after the solve (P).
+int launch_cell_flux_implicit(const Dims &d, const StepScalars &sc, const SharedFields &f,
+                              const double *P, const double *diffv, const double *DZZ,
+                              double sV, double bV, int nulldif, double thA, double thD,
+                              const FluxArrays &o, const double *coef, hipStream_t s);
 // ptr[cells[m]] = vals ? vals[m] : -1 for m < n.
 int launch_disch_scatter(int n, const uint32_t *cells, const int32_t *vals, int32_t *ptr,
                          hipStream_t s);

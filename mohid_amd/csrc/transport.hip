@@ -536,6 +536,167 @@ int launch_orlanski_exterior(const Dims &d, const int32_t *bp, const int32_t *op
     return MGPU_SUCCESS;
 }
 
+// ---------------------------------------------------------------------------
+// Cell fluxes (AD:3356-3954), one thread per WorkSize cell and its three
+// lower faces: U (j-1, j), V (i-1, i), W (k-1, k), as the reference indexes
+// the flux arrays.  Each array starts at zero (AD:1462-1468), so an explicit
+// part is 0 + w * flux and an implicit part adds to it.
+namespace {
+// Kz of a W face, Convert_Dif_Vertical (AD:2396-2401)
+__device__ __forceinline__ double face_kz(const double *diffv, const double *Qz, long x,
+                                          double sV, double bV, int nulldif) {
+    double kz = sV * diffv[x] + bV;
+    if (nulldif && Qz[x] == 0.0) kz = 0.0;
+    return kz;
+}
+
+// -w * Kz * DUX * DVY / DZZ(k-1) (CalcVerticalDifFlux, AD:3690-3697) or
+// -w * Diff_V_Const (CalcVerticalDifFlux2, AD:3744-3764), times the gradient
+template <bool OPT>
+__device__ __forceinline__ double vdif_term(const Dims &d, const SharedFields &f,
+                                            const double *diffv, double sV, double bV,
+                                            int nulldif, const double *DZZ, long x, int i, int j,
+                                            int k, double w, double grad) {
+    if (OPT) return w * f.cw[x] * grad;
+    return w * face_kz(diffv, f.Qz, x, sV, bV, nulldif) * f.DUX[d.idx2(i, j)] *
+           f.DVY[d.idx2(i, j)] / DZZ[d.idx(i, j, k - 1)] * grad;
+}
+
+template <bool OPT>
+__global__ void cell_flux_explicit_kernel(Dims d, StepScalars sc, SharedFields f,
+                                          const double *__restrict__ P,
+                                          const double *__restrict__ diffv,
+                                          const double *__restrict__ DZZ, double sV, double bV,
+                                          int nulldif, double thA, double thD, FluxArrays o,
+                                          double4 *__restrict__ coef,
+                                          unsigned int *__restrict__ badflag) {
+    const int i = d.wILB + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = d.wJLB + blockIdx.y;
+    const int k = d.wKLB + blockIdx.z;
+    if (i > d.wIUB) return;
+    const long x = d.idx(i, j, k), sk = d.sk;
+    const uint8_t mk = f.mask[x];
+    const TracerSet ts{P, nullptr, 1, nullptr};
+    int bad = 0;
+    const bool hz = !sc.vertical1d;
+    if (hz && (mk & M_CFU)) {
+        // CalcHorizontalAdvFluxXX(_opt) (AD:3479-3567), weight 1 - ImpExp_AdvXX = 1
+        double fl[1];
+        hface_flux<OPT, 1, false>(d, sc, f, ts, x, d.sj, j, d.JLB, d.JUB, f.DUX + d.idx2(i, j),
+                                  d.sj, f.Qx[x], fl, bad);
+        o.afx[x] = 0.0 + 1.0 * fl[0];
+        // CalcHorizontalDifFluxXX(2) (AD:3784-3867)
+        o.dfx[x] = 0.0 - f.cu[x] * (P[x] - P[x - d.sj]);
+    }
+    if (hz && !sc.xzflow && (mk & M_CFV)) {
+        double fl[1];
+        hface_flux<OPT, 1, false>(d, sc, f, ts, x, 1, i, d.ILB, d.IUB, f.DVY + d.idx2(i, j), 1,
+                                  f.Qy[x], fl, bad);
+        o.afy[x] = 0.0 + 1.0 * fl[0];
+        o.dfy[x] = 0.0 - f.cv[x] * (P[x] - P[x - 1]);
+    }
+    if (sc.vert && (mk & M_CFW)) {
+        // the W face's advection coefficients, from the stencil k-2 .. k+1
+        // as the column walk makes them (VerticalAdvection, AD:2941-3140)
+        const long base = d.idx2(i, j);
+        const bool vflux = !sc.vertical1d && (f.mask[base + (long)(d.wKUB - d.KLB) * sk] & M_OPEN);
+        double p[1][4], v[4], du[4];
+        bool op[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int kk = k - 2 + q;
+            const bool in = kk >= d.KLB && kk <= d.KUB;
+            const long y = x + (long)(q - 2) * sk;
+            p[0][q] = in ? P[y] : 0.0;
+            v[q] = in ? f.V[y] : 1.0;
+            du[q] = in ? f.DWZ[y] : 0.0;
+            op[q] = in && (f.mask[y] & M_OPEN);
+        }
+        VCoef co[1];
+        vface_coefs<OPT, 1, false>(sc, p, v, du, op, f.Qz[x], vflux && k <= d.wKUB, co, bad);
+        coef[x] = make_double4(co[0].c, co[0].d, co[0].e, co[0].f);
+        if (vflux && thA < 1.0) {
+            // CalcVerticalAdvFlux(_opt) (AD:3356-3475), weight 1 - ImpExp_AdvV
+            double fl;
+            if (OPT) {
+                fl = co[0].d * p[0][1] + co[0].e * p[0][2];
+            } else {
+                const double pmm = (co[0].c == 0.0 || k - 2 < d.KLB) ? 0.0 : p[0][0];
+                fl = co[0].c * pmm + co[0].d * p[0][1] + co[0].e * p[0][2] + co[0].f * p[0][3];
+            }
+            o.afz[x] = 0.0 + (1.0 - thA) * fl;
+        }
+        if (thD < 1.0)
+            o.dfz[x] = 0.0 - vdif_term<OPT>(d, f, diffv, sV, bV, nulldif, DZZ, x, i, j, k,
+                                            1.0 - thD, P[x] - P[x - sk]);
+    }
+    if (bad) atomicOr(badflag, 1u);
+}
+
+template <bool OPT>
+__global__ void cell_flux_implicit_kernel(Dims d, StepScalars sc, SharedFields f,
+                                          const double *__restrict__ P,
+                                          const double *__restrict__ diffv,
+                                          const double *__restrict__ DZZ, double sV, double bV,
+                                          int nulldif, double thA, double thD, FluxArrays o,
+                                          const double4 *__restrict__ coef) {
+    const int i = d.wILB + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = d.wJLB + blockIdx.y;
+    const int k = d.wKLB + blockIdx.z;
+    if (i > d.wIUB) return;
+    const long x = d.idx(i, j, k), sk = d.sk;
+    if (!(f.mask[x] & M_CFW)) return;
+    if (thA > 0.0) {
+        const double4 c = coef[x];
+        double fl;
+        if (OPT) {
+            fl = c.y * P[x - sk] + c.z * P[x];
+        } else {
+            const double pmm = (c.x == 0.0 || k - 2 < d.KLB) ? 0.0 : P[x - 2 * sk];
+            fl = c.x * pmm + c.y * P[x - sk] + c.z * P[x] + c.w * P[x + sk];
+        }
+        o.afz[x] = o.afz[x] + thA * fl;
+    }
+    if (thD > 0.0)
+        o.dfz[x] = o.dfz[x] - vdif_term<OPT>(d, f, diffv, sV, bV, nulldif, DZZ, x, i, j, k, thD,
+                                             P[x] - P[x - sk]);
+}
+}  // namespace
+
+int launch_cell_flux_explicit(const Dims &d, const StepScalars &sc, const SharedFields &f,
+                              const double *P, const double *diffv, const double *DZZ,
+                              double sV, double bV, int nulldif, double thA, double thD,
+                              const FluxArrays &o, double *coef, unsigned int *bad,
+                              hipStream_t s) {
+    const dim3 g((d.wIUB - d.wILB + 1 + 63) / 64, d.wJUB - d.wJLB + 1, d.wKUB - d.wKLB + 1);
+    double4 *c4 = reinterpret_cast<double4 *>(coef);
+    if (sc.optimize)
+        hipLaunchKernelGGL(cell_flux_explicit_kernel<true>, g, dim3(64), 0, s, d, sc, f, P, diffv,
+                           DZZ, sV, bV, nulldif, thA, thD, o, c4, bad);
+    else
+        hipLaunchKernelGGL(cell_flux_explicit_kernel<false>, g, dim3(64), 0, s, d, sc, f, P,
+                           diffv, DZZ, sV, bV, nulldif, thA, thD, o, c4, bad);
+    HIPCHK(hipGetLastError());
+    return MGPU_SUCCESS;
+}
+
+int launch_cell_flux_implicit(const Dims &d, const StepScalars &sc, const SharedFields &f,
+                              const double *P, const double *diffv, const double *DZZ,
+                              double sV, double bV, int nulldif, double thA, double thD,
+                              const FluxArrays &o, const double *coef, hipStream_t s) {
+    if (!sc.vert || (thA <= 0.0 && thD <= 0.0)) return MGPU_SUCCESS;
+    const dim3 g((d.wIUB - d.wILB + 1 + 63) / 64, d.wJUB - d.wJLB + 1, d.wKUB - d.wKLB + 1);
+    const double4 *c4 = reinterpret_cast<const double4 *>(coef);
+    if (sc.optimize)
+        hipLaunchKernelGGL(cell_flux_implicit_kernel<true>, g, dim3(64), 0, s, d, sc, f, P, diffv,
+                           DZZ, sV, bV, nulldif, thA, thD, o, c4);
+    else
+        hipLaunchKernelGGL(cell_flux_implicit_kernel<false>, g, dim3(64), 0, s, d, sc, f, P,
+                           diffv, DZZ, sV, bV, nulldif, thA, thD, o, c4);
+    HIPCHK(hipGetLastError());
+    return MGPU_SUCCESS;
+}
+
 int launch_bnd_list(const Dims &d, const int32_t *bp, const int32_t *kfloor, uint32_t *rowcnt,
                     uint32_t *list, int32_t *meta, int32_t *pos, unsigned long long *total,
                     hipStream_t s) {

@@ -32,6 +32,7 @@ module ModuleAdvectionDiffusionGPU
     public :: THOMASZ_GPU
     public :: THOMAS_3D_GPU
     public :: SetDischargesGPU, UnSetDischargesGPU
+    public :: GetFluxesGPU
     public :: MGPU_SUCCESS, MGPU_OFF_ERR, MGPU_ERR_OPTION, MGPU_ERR_UNSUPPORTED
     public :: MGPU_ERR_PIVOT
 
@@ -72,7 +73,7 @@ module ModuleAdvectionDiffusionGPU
         type(c_ptr)        :: Visc_H, Diff_V
         type(c_ptr)        :: WaterPoints3D
         integer(c_int32_t) :: Optimize, FirstProperty
-        integer(c_int32_t) :: unsupported_CellFluxes, unsupported_NoAdvFlux
+        integer(c_int32_t) :: CellFluxes, unsupported_NoAdvFlux
         type(c_ptr)        :: ReferenceProp
         integer(c_int32_t) :: BoundaryCondition
         real(c_double)     :: DecayTime
@@ -143,6 +144,13 @@ module ModuleAdvectionDiffusionGPU
             import :: c_int, c_ptr
             type(c_ptr), value :: handle
         end function mgpu_ad_unset_discharges
+
+        integer(c_int) function mgpu_ad_get_fluxes(handle, afx, afy, afz, dfx, dfy, dfz, dev) &
+                bind(c, name="mgpu_ad_get_fluxes")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: handle, afx, afy, afz, dfx, dfy, dfz
+            integer(c_int32_t), value :: dev
+        end function mgpu_ad_get_fluxes
     end interface
 
     ! instance table: integer IDs as MOHID's modules hand out (ModuleGlobalData
@@ -214,7 +222,8 @@ contains
     !> advanced by DTProp in place.  Arguments keep the reference's names,
     !> order and meaning.  Options the GPU path does not offload make
     !> STAT = MGPU_ERR_UNSUPPORTED and leave PROP untouched:
-    !>   CellFluxes, NoAdvFlux true (AD:1321-1325), AdvectionNudging, an open
+    !>   CellFluxes with implicit horizontal advection, NoAdvFlux true
+    !>   (AD:1321-1325), AdvectionNudging, an open
     !>   boundary with one layer and implicit horizontal advection, and the
     !>   Orlanski_ open boundary with implicit horizontal advection.
     !> SmallDepths, NoDifFlux (with NoFluxU/V/W), the discharges of
@@ -305,8 +314,8 @@ contains
         a%WaterPoints3D = c_loc(WaterPoints3D)
         a%Optimize = 0; if (present(Optimize)) a%Optimize = b2i(Optimize)
         a%FirstProperty = 0; if (present(FirstProperty)) a%FirstProperty = b2i(FirstProperty)
-        a%unsupported_CellFluxes = 0
-        if (present(CellFluxes)) a%unsupported_CellFluxes = b2i(CellFluxes)
+        a%CellFluxes = 0
+        if (present(CellFluxes)) a%CellFluxes = b2i(CellFluxes)
         a%ReferenceProp = c_null_ptr
         if (present(ReferenceProp)) a%ReferenceProp = c_loc(ReferenceProp)
         a%BoundaryCondition = 0
@@ -368,6 +377,36 @@ contains
                                     c_loc(DischnCells), c_loc(byp), c_loc(DischConcMF))
         if (present(STAT)) STAT = rc
     end subroutine SetDischargesGPU
+
+    !> GetAdvFlux / GetDifFlux (ModuleAdvectionDiffusion.F90:697-860): the cell
+    !> fluxes of the last AdvectionDiffusionGPU call made with CellFluxes, into
+    !> the caller's arrays over Size (each optional).
+    subroutine GetFluxesGPU(ID, AdvFluxX, AdvFluxY, AdvFluxZ, DifFluxX, DifFluxY,       &
+                            DifFluxZ, STAT)
+        integer,                                  intent(in)  :: ID
+        real(8), dimension(:,:,:), optional, contiguous, target, intent(inout) :: AdvFluxX
+        real(8), dimension(:,:,:), optional, contiguous, target, intent(inout) :: AdvFluxY
+        real(8), dimension(:,:,:), optional, contiguous, target, intent(inout) :: AdvFluxZ
+        real(8), dimension(:,:,:), optional, contiguous, target, intent(inout) :: DifFluxX
+        real(8), dimension(:,:,:), optional, contiguous, target, intent(inout) :: DifFluxY
+        real(8), dimension(:,:,:), optional, contiguous, target, intent(inout) :: DifFluxZ
+        integer, optional,                        intent(out) :: STAT
+        type(c_ptr) :: p(6)
+        integer :: rc
+        if (ID < 1 .or. ID > MAX_INSTANCES) then
+            if (present(STAT)) STAT = MGPU_OFF_ERR
+            return
+        endif
+        p = c_null_ptr
+        if (present(AdvFluxX)) p(1) = c_loc(AdvFluxX)
+        if (present(AdvFluxY)) p(2) = c_loc(AdvFluxY)
+        if (present(AdvFluxZ)) p(3) = c_loc(AdvFluxZ)
+        if (present(DifFluxX)) p(4) = c_loc(DifFluxX)
+        if (present(DifFluxY)) p(5) = c_loc(DifFluxY)
+        if (present(DifFluxZ)) p(6) = c_loc(DifFluxZ)
+        rc = mgpu_ad_get_fluxes(Handles(ID), p(1), p(2), p(3), p(4), p(5), p(6), 0_c_int32_t)
+        if (present(STAT)) STAT = rc
+    end subroutine GetFluxesGPU
 
     !> UnSetDischarges (ModuleAdvectionDiffusion.F90:1040-1086).
     subroutine UnSetDischargesGPU(ID, STAT)

@@ -240,7 +240,7 @@ def test_rejects_like_the_reference():
         (dict(ImpExp_DifH=1.0), L.MGPU_ERR_OPTION),               # AD:1340 ERR02
         (dict(ImpExp_AdvXX=1.0, ImpExp_AdvYY=1.0), L.MGPU_ERR_OPTION),  # ERR03
         (dict(ImpExp_AdvXX=0.5), L.MGPU_ERR_OPTION),               # AD:4527 ERR01
-        (dict(CellFluxes=True), L.MGPU_ERR_UNSUPPORTED),
+        (dict(CellFluxes=True, ImpExp_AdvXX=1.0), L.MGPU_ERR_UNSUPPORTED),
     ]
     for ov, code in bad:
         params = dict(case.params)

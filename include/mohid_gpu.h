@@ -136,6 +136,36 @@ int mgpu_ad_set_geometry(mgpu_ad *ad, const mgpu_geometry *geometry);
  * device mode or the batched transport for performance). */
 int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *args, double *prop);
 
+/* ---- upstream producers of the transport's inputs ----------------------
+ * Each works in place on caller arrays over Size (outputs are in/out: the
+ * cells the reference leaves untouched keep their values); with
+ * device_pointers every array is a device pointer and nothing crosses PCIe.
+ *
+ * WaterFlux_Z_VarGrid / WaterFlux_Z_FixGrid (ModuleHydrodynamic.F90:
+ * 43270-43343): for every water cell, level by level upwards,
+ * Qz(k+1) = Qz(k) + Qx(j) - Qx(j+1) + Qy(i) - Qy(i+1) [- dV/dt with
+ * variable_grid], the faces weighted by ComputeFaces3D_U/V. */
+int mgpu_waterflux_z(const mgpu_size3d *size, const mgpu_size3d *worksize,
+                     const double *Volume_Z_New, const double *Volume_Z_Old,
+                     const double *WaterFlux_X, const double *WaterFlux_Y, double *WaterFlux_Z,
+                     const int32_t *ComputeFaces3D_U, const int32_t *ComputeFaces3D_V,
+                     const int32_t *WaterPoints3D, double DT, int32_t variable_grid,
+                     int32_t device_pointers);
+/* ComputeVolumes (ModuleGeometry.F90:4108-4231): VolumeZ = DWZ DUX DVY, and
+ * VolumeU/V (and VolumeW when not null: the non-hydrostatic case) the means
+ * of the Z volumes beside each face. */
+int mgpu_compute_volumes(const mgpu_size3d *size, const mgpu_size3d *worksize, const double *DWZ,
+                         const double *DUX, const double *DVY, double *VolumeZ, double *VolumeU,
+                         double *VolumeV, double *VolumeW, int32_t device_pointers);
+/* ComputeAreas (ModuleGeometry.F90:4238-4327): AreaU = DUZ DYY,
+ * AreaV = DVZ DXX; with SZZ not null the impermeability factors
+ * (Coef_U(k) + WaterLevel CoefX_U(k)), Coef arrays indexed by k over Size. */
+int mgpu_compute_areas(const mgpu_size3d *size, const mgpu_size3d *worksize, const double *DUZ,
+                       const double *DVZ, const double *DXX, const double *DYY, double *AreaU,
+                       double *AreaV, const double *SZZ, const double *Coef_U,
+                       const double *CoefX_U, const double *Coef_V, const double *CoefX_V,
+                       int32_t device_pointers);
+
 /* GetAdvFlux / GetDifFlux (AD:697-860): the cell fluxes of the last
  * mgpu_ad_step call made with CellFluxes, over Size (the faces between
  * j-1 and j, i-1 and i, k-1 and k).  Each non-null array receives its

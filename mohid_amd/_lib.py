@@ -90,7 +90,7 @@ class TransportOpts(ct.Structure):
 EXPORTS = (
     "mgpu_ad_create", "mgpu_ad_set_geometry", "mgpu_ad_step", "mgpu_ad_destroy",
     "mgpu_ad_zero_pivots", "mgpu_ad_set_discharges", "mgpu_ad_unset_discharges",
-    "mgpu_ad_get_fluxes",
+    "mgpu_ad_get_fluxes", "mgpu_waterflux_z", "mgpu_compute_volumes", "mgpu_compute_areas",
     "mgpu_thomasz", "mgpu_thomas3d", "mgpu_transport_status",
     "mgpu_transport_create", "mgpu_transport_set_hydro", "mgpu_transport_set_tracer",
     "mgpu_transport_get_tracer", "mgpu_transport_run", "mgpu_transport_tracer_ptr",
@@ -132,6 +132,10 @@ def lib():
     L.mgpu_ad_set_discharges.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp]
     L.mgpu_ad_unset_discharges.argtypes = [vp]
     L.mgpu_ad_get_fluxes.argtypes = [vp] + [vp] * 6 + [i32]
+    sz = ct.POINTER(Size3D)
+    L.mgpu_waterflux_z.argtypes = [sz, sz] + [vp] * 8 + [ct.c_double, i32, i32]
+    L.mgpu_compute_volumes.argtypes = [sz, sz] + [vp] * 7 + [i32]
+    L.mgpu_compute_areas.argtypes = [sz, sz] + [vp] * 11 + [i32]
     L.mgpu_thomas3d.argtypes = [ct.POINTER(Size3D), i32, i32, i32, i32, i32, i32, i32, i32, vp,
                                 vp, vp, vp, vp, i32, vp]
     L.mgpu_transport_status.argtypes = [vp]

@@ -1889,3 +1889,85 @@ int ora_advection_diffusion(ora_instance *me, const ora_args *a, double *prop)
     me->last_calc = a->Now;
     return rc;
 }
+
+
+/* ------------------------------------------------------------------------ */
+/* Upstream producers.  Index helpers over an explicit Size. */
+#define PX(z, i, j, k) (((long)(i) - (z)->ILB) + ((long)(z)->IUB - (z)->ILB + 1) * \
+                        (((long)(j) - (z)->JLB) + ((long)(z)->JUB - (z)->JLB + 1) * ((long)(k) - (z)->KLB)))
+#define PX2(z, i, j) (((long)(i) - (z)->ILB) + ((long)(z)->IUB - (z)->ILB + 1) * ((long)(j) - (z)->JLB))
+
+/* HYD:43270-43343 */
+void ora_waterflux_z(const ora_size3d *size, const ora_size3d *ws, const double *vnew,
+                     const double *vold, const double *qx, const double *qy, double *qz,
+                     const int32_t *cfu, const int32_t *cfv, const int32_t *wp, double dt,
+                     int vargrid)
+{
+    int i, j, k;
+    for (k = ws->KLB; k <= ws->KUB; k++)
+        for (j = ws->JLB; j <= ws->JUB; j++)
+            for (i = ws->ILB; i <= ws->IUB; i++) {
+                const long x = PX(size, i, j, k);
+                double west, east, south, north, v;
+                if (wp[x] != 1) continue;
+                west = qx[x] * (double)cfu[x];
+                east = qx[PX(size, i, j + 1, k)] * (double)cfu[PX(size, i, j + 1, k)];
+                south = qy[x] * (double)cfv[x];
+                north = qy[PX(size, i + 1, j, k)] * (double)cfv[PX(size, i + 1, j, k)];
+                v = qz[x] + west - east + south - north;
+                if (vargrid) v = v - (vnew[x] - vold[x]) / dt;
+                qz[PX(size, i, j, k + 1)] = v;
+            }
+}
+
+/* GEO:4108-4231 */
+void ora_compute_volumes(const ora_size3d *size, const ora_size3d *ws, const double *dwz,
+                         const double *dux, const double *dvy, double *vz, double *vu,
+                         double *vv, double *vw)
+{
+    int i, j, k;
+    for (k = ws->KLB; k <= ws->KUB; k++)
+        for (j = ws->JLB; j <= ws->JUB; j++)
+            for (i = ws->ILB; i <= ws->IUB; i++)
+                vz[PX(size, i, j, k)] = dwz[PX(size, i, j, k)] * dux[PX2(size, i, j)] *
+                                        dvy[PX2(size, i, j)];
+    for (k = ws->KLB; k <= ws->KUB; k++)
+        for (j = ws->JLB + 1; j <= ws->JUB; j++)
+            for (i = ws->ILB; i <= ws->IUB; i++)
+                vu[PX(size, i, j, k)] = (vz[PX(size, i, j - 1, k)] + vz[PX(size, i, j, k)]) / 2.0;
+    for (k = ws->KLB; k <= ws->KUB; k++)
+        for (j = ws->JLB; j <= ws->JUB; j++)
+            for (i = ws->ILB + 1; i <= ws->IUB; i++)
+                vv[PX(size, i, j, k)] = (vz[PX(size, i - 1, j, k)] + vz[PX(size, i, j, k)]) / 2.0;
+    if (vw)
+        for (k = ws->KLB + 1; k <= ws->KUB; k++)
+            for (j = ws->JLB; j <= ws->JUB; j++)
+                for (i = ws->ILB; i <= ws->IUB; i++)
+                    vw[PX(size, i, j, k)] =
+                        (vz[PX(size, i, j, k - 1)] + vz[PX(size, i, j, k)]) / 2.0;
+}
+
+/* GEO:4238-4327 */
+void ora_compute_areas(const ora_size3d *size, const ora_size3d *ws, const double *duz,
+                       const double *dvz, const double *dxx, const double *dyy, double *au,
+                       double *av, const double *szz, const double *coef_u,
+                       const double *coefx_u, const double *coef_v, const double *coefx_v)
+{
+    int i, j, k;
+    for (k = ws->KLB; k <= ws->KUB; k++)
+        for (j = ws->JLB; j <= ws->JUB; j++)
+            for (i = ws->ILB; i <= ws->IUB; i++) {
+                au[PX(size, i, j, k)] = duz[PX(size, i, j, k)] * dyy[PX2(size, i, j)];
+                av[PX(size, i, j, k)] = dvz[PX(size, i, j, k)] * dxx[PX2(size, i, j)];
+            }
+    if (!szz) return;
+    for (k = ws->KLB; k <= ws->KUB; k++)
+        for (j = ws->JLB; j <= ws->JUB; j++)
+            for (i = ws->ILB; i <= ws->IUB; i++) {
+                const int ku = ws->KUB, kc = k - size->KLB;
+                double wl = fmax2(-szz[PX(size, i, j, ku)], -szz[PX(size, i, j + 1, ku)]);
+                au[PX(size, i, j, k)] = au[PX(size, i, j, k)] * (coef_u[kc] + wl * coefx_u[kc]);
+                wl = fmax2(-szz[PX(size, i, j, ku)], -szz[PX(size, i + 1, j, ku)]);
+                av[PX(size, i, j, k)] = av[PX(size, i, j, k)] * (coef_v[kc] + wl * coefx_v[kc]);
+            }
+}

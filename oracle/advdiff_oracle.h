@@ -152,6 +152,27 @@ double ora_orlanski2d(const ora_size3d *size, double *newf, double *oldf, const 
                       int i, int j, int k, int eastnorth, double limitmax, double dt,
                       double flowvelx);
 
+/* Upstream producers of the transport's inputs.
+ * WaterFlux_Z_VarGrid / WaterFlux_Z_FixGrid (ModuleHydrodynamic.F90:
+ * 43270-43343): the vertical water flux from continuity, column by column
+ * upwards from Qz(KLB); vargrid = 0 drops the volume change. */
+void ora_waterflux_z(const ora_size3d *size, const ora_size3d *ws, const double *vnew,
+                     const double *vold, const double *qx, const double *qy, double *qz,
+                     const int32_t *cfu, const int32_t *cfv, const int32_t *wp, double dt,
+                     int vargrid);
+/* ComputeVolumes (ModuleGeometry.F90:4108-4231): VolumeZ, VolumeU, VolumeV
+ * and, when vw is not NULL, VolumeW. */
+void ora_compute_volumes(const ora_size3d *size, const ora_size3d *ws, const double *dwz,
+                         const double *dux, const double *dvy, double *vz, double *vu,
+                         double *vv, double *vw);
+/* ComputeAreas (ModuleGeometry.F90:4238-4327): AreaU, AreaV; with szz not
+ * NULL the impermeability factors (coef_u, coefx_u, coef_v, coefx_v index
+ * k from Size KLB). */
+void ora_compute_areas(const ora_size3d *size, const ora_size3d *ws, const double *duz,
+                       const double *dvz, const double *dxx, const double *dyy, double *au,
+                       double *av, const double *szz, const double *coef_u,
+                       const double *coefx_u, const double *coef_v, const double *coefx_v);
+
 /* THOMASZ_NewType2 (ModuleFunctions.F90:4026-4123) on explicit arrays.
  * W, G: scratch of length >= KUB+2 (per-thread VEC in the reference). */
 void ora_thomasz(const ora_size3d *size, const ora_size3d *ws,

@@ -102,6 +102,14 @@ def lib():
         L.ora_orlanski2d.argtypes = [ct.POINTER(Size3D), _dp, _dp, _dp, _ip] + [ct.c_int] * 10 + \
             [ct.c_double] * 3
         L.ora_orlanski2d.restype = ct.c_double
+        sz = ct.POINTER(Size3D)
+        L.ora_waterflux_z.argtypes = [sz, sz, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _ip, ct.c_double,
+                                      ct.c_int]
+        L.ora_waterflux_z.restype = None
+        L.ora_compute_volumes.argtypes = [sz, sz] + [_dp] * 7
+        L.ora_compute_volumes.restype = None
+        L.ora_compute_areas.argtypes = [sz, sz] + [_dp] * 11
+        L.ora_compute_areas.restype = None
         L.ora_advection1d_tvd_superbee_2.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_double,
                                                      _dp, _dp, _dp, _dp, _ip, _dp, _dp]
         _lib = L

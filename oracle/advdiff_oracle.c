@@ -1000,6 +1000,20 @@ static void horizontal_diffusion_yy(ora_instance *me, const ora_args *a, const d
 }
 
 /* AD:4368-4540 HorizontalAdvectionXX (+ _Explicit, AD:4544-4582) */
+/* The XX coefficients of the WorkSize cells flagged in nf set to zero */
+static void no_adv_flux_xx(ora_instance *me, const int32_t *nf)
+{
+    const ora_size3d *w = &me->ws;
+    int i, j, k;
+    for (k = w->KLB; k <= w->KUB; k++)
+        for (j = w->JLB; j <= w->JUB; j++)
+            for (i = w->ILB; i <= w->IUB; i++) {
+                const long x = I3(me, i, j, k);
+                if (nf[x] != 1) continue;
+                me->xC[x] = 0.0; me->xD[x] = 0.0; me->xE[x] = 0.0; me->xF[x] = 0.0;
+            }
+}
+
 static int horizontal_advection_xx(ora_instance *me, const ora_args *a, const double *P)
 {
     const ora_size3d *w = &me->ws, *s = &me->size;
@@ -1025,6 +1039,8 @@ static int horizontal_advection_xx(ora_instance *me, const ora_args *a, const do
                                            a->VolumeRelMax, a->Upwind2H);
             }
         }
+        /* AD:4434-4443: NoAdvFlux closes the NoFluxU faces (non-optimised path) */
+        if (!a->Optimize && a->NoAdvFlux) no_adv_flux_xx(me, a->NoFluxU);
     }
     if (rc) return rc;
     if (a->ImpExp_AdvXX == IMPLICIT) {
@@ -1099,6 +1115,10 @@ static int horizontal_advection_yy(ora_instance *me, const ora_args *a, const do
                                            a->VolumeRelMax, a->Upwind2H);
             }
         }
+        /* AD:4804-4813: the reference clears the XX coefficients at the
+         * NoFluxV faces here, after XX has used them; a later call of the
+         * step that keeps the coefficients (Set_Internal_State) sees them */
+        if (!a->Optimize && a->NoAdvFlux) no_adv_flux_xx(me, g_mistake == 19 ? a->NoFluxU : a->NoFluxV);
     }
     if (rc) return rc;
     if (a->ImpExp_AdvYY == IMPLICIT) {
@@ -1840,6 +1860,7 @@ int ora_advection_diffusion(ora_instance *me, const ora_args *a, double *prop)
         if (!optimize || a->FirstProperty) convert_dif_vertical(me, &aa);
     }
     if (a->NoDifFlux && (!a->NoFluxU || !a->NoFluxV || !a->NoFluxW)) return ORA_ERR_BADARG;
+    if (a->NoAdvFlux && (!a->NoFluxU || !a->NoFluxV)) return ORA_ERR_BADARG;
     /* Set_Internal_State ERR01 (AD:5816-5829): an open boundary needs a
      * known condition; Orlanski's radiation is not restated */
     if (a->ReferenceProp) {

@@ -84,6 +84,7 @@ typedef struct {
     const int32_t *BoundaryPoints2D;              /* 2-D (Size i, j) */
     double *PROPOld;                              /* 3-D, written with Orlanski_ (AD:5428) */
     int32_t CellFluxes;                           /* AD:1402-1407: the flux diagnostics */
+    int32_t NoAdvFlux;                            /* AD:4434-4443, 4804-4813 (with NoFluxU/V) */
 } ora_args;
 
 typedef struct ora_instance ora_instance;
@@ -139,7 +140,9 @@ const double *ora_internal(const ora_instance *me, const char *name);
  *     one before the opposite boundary
  *  17 Orlanski: the cross-boundary space variability taken on the wrong
  *     side of the interior point
- *  18 horizontal diffusive cell flux with the sign of the gradient */
+ *  18 horizontal diffusive cell flux with the sign of the gradient
+ *  19 NoAdvFlux: the YY branch clears the XX coefficients at the NoFluxU
+ *     faces instead of the NoFluxV ones */
 void ora_plant_mistake(int id);
 
 /* OrlanskiCelerity2D (ModuleFunctions.F90:4129-4490) with the optional

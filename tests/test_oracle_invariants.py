@@ -35,6 +35,11 @@ mathematics fixes:
 * cell fluxes (AD:3356-3954, 1885-1912): in a closed basin every open cell
   balances V (C_new - C_old) = dt (F_in - F_out) over its six faces, the
   explicit parts taken with the old field and the implicit ones with the new;
+* NoAdvFlux (AD:4434-4443, 4804-4813): closing every U face is, bit for
+  bit, no flow across them; the XX coefficients a first property clears (at
+  its NoFluxU faces, and by the YY branch at its NoFluxV positions) stay
+  cleared for the next property when the method keeps its coefficients
+  within the step; the optimised path ignores the option;
 * open boundaries (AD:5369-5742) and the post-solve conditions
   (AD:1874-1880, 1926-2224): an imposed value with no decay time makes
   every open boundary cell the reference value and leaves every other cell
@@ -945,6 +950,63 @@ def test_cell_fluxes_balance_every_cell(name, theta, implicit):
     assert _cell_flux_err(name, theta, implicit) <= 1e-13
 
 
+def _no_adv_flux_err():
+    """NoAdvFlux (AD:4434-4443, 4804-4813), non-optimised path:
+    * every U face flagged is, bit for bit, no flow across U faces (the XX
+      coefficients are zero; TVD recomputes them every call);
+    * with a method the reference does not recompute within a step (upwind,
+      Set_Internal_State AD:5746-5800), the XX coefficients a first property
+      left cleared at its NoFluxU faces and, by the YY branch, at its NoFluxV
+      positions stay cleared for the next property: that property equals one
+      computed first with NoAdvFlux on NoFluxU = (U | V), NoFluxV = none;
+    * the optimised path ignores NoAdvFlux."""
+    rng = np.random.default_rng(21)
+    # 1. all U faces closed == Qx = 0
+    case = make("tvd_sigma")
+    params = dict(case.params, Optimize=False)
+    shape = case.shape
+    ones = np.asfortranarray(np.ones(shape, dtype=np.int32))
+    zeros = np.asfortranarray(np.zeros(shape, dtype=np.int32))
+    c0 = np.array(case.tracers[0], order="F", copy=True)
+    hy = dict(case.hydro, NoFluxU=ones, NoFluxV=zeros, NoFluxW=zeros)
+    a = _run(case, c0.copy(order="F"), hydro=hy, NoAdvFlux=True, **params)
+    hq = dict(case.hydro)
+    hq["Wflux_X"] = np.asfortranarray(np.zeros(shape))
+    b = _run(case, c0.copy(order="F"), hydro=hq, **params)
+    err = float(np.max(np.abs(a - b)))
+    # 2. the leak to the next property of the step (upwind: coefficients kept)
+    case = make("upwind_land_ragged")
+    params = dict(case.params, Optimize=False)
+    U = np.asfortranarray((rng.uniform(size=case.shape) < 0.3).astype(np.int32))
+    V = np.asfortranarray((rng.uniform(size=case.shape) < 0.3).astype(np.int32))
+    z = np.asfortranarray(np.zeros(case.shape, dtype=np.int32))
+    hy = dict(case.hydro, NoFluxU=U, NoFluxV=V, NoFluxW=z)
+    inst = ora.Instance(case, nthreads=2)
+    p1 = np.array(case.tracers[0], order="F", copy=True)
+    t2 = np.asfortranarray(case.tracers[0] * 0.5 + 1.0)       # a second property
+    p2 = t2.copy(order="F")
+    assert inst.step(p1, first_property=True, now=DT, hydro=hy, NoAdvFlux=True, **params) == 0
+    assert inst.step(p2, first_property=False, now=DT, hydro=hy, **params) == 0
+    inst.close()
+    hw = dict(case.hydro, NoFluxU=np.asfortranarray((U | V).astype(np.int32)), NoFluxV=z,
+              NoFluxW=z)
+    want = _run(case, t2.copy(order="F"), hydro=hw, NoAdvFlux=True, **params)
+    err = max(err, float(np.max(np.abs(p2 - want))))
+    # 3. the optimised path
+    case = make("tvd_ragged_opt")
+    hy = dict(case.hydro, NoFluxU=np.asfortranarray(np.ones(case.shape, np.int32)),
+              NoFluxV=np.asfortranarray(np.ones(case.shape, np.int32)),
+              NoFluxW=np.asfortranarray(np.zeros(case.shape, np.int32)))
+    c0 = np.array(case.tracers[0], order="F", copy=True)
+    a = _run(case, c0.copy(order="F"), hydro=hy, NoAdvFlux=True)
+    b = _run(case, c0.copy(order="F"))
+    return max(err, float(np.max(np.abs(a - b))))
+
+
+def test_no_adv_flux():
+    assert _no_adv_flux_err() == 0.0
+
+
 def test_imposed_value_closed_form():
     assert _imposed_value_err() == 0.0
 
@@ -1003,6 +1065,7 @@ def _all_checks():
     for name, interior in (("tvd_land_sigma", 0), ("upwind_land_ragged", 4)):
         out.append((f"null gradient {name}", _null_gradient_err(name, interior), 1e-14))
     out.append(("cyclic", _cyclic_err(), 0.0))
+    out.append(("no adv flux", _no_adv_flux_err(), 0.0))
     for name, theta, imp in (("tvd_land_sigma", None, None), ("tvd_ragged_opt", None, None),
                              ("upwind_flat", 0.5, None), ("tvd_sigma", 0.0, None)):
         out.append((f"cell fluxes {name} {theta} {imp}", _cell_flux_err(name, theta, imp), 1e-13))
@@ -1021,7 +1084,7 @@ def _all_checks():
     return out
 
 
-@pytest.mark.parametrize("mistake", list(range(1, 19)))
+@pytest.mark.parametrize("mistake", list(range(1, 20)))
 def test_planted_mistakes_are_caught(mistake):
     L = _lib()
     L.ora_plant_mistake(mistake)

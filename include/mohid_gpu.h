@@ -99,8 +99,9 @@ typedef struct {
     int32_t Optimize, FirstProperty;
     int32_t CellFluxes;                    /* the flux diagnostics (AD:3356-3954), read back with
                                               mgpu_ad_get_fluxes */
-    /* present-and-set optional argument the GPU path does not offload */
-    int32_t unsupported_NoAdvFlux;         /* its YY branch zeroes the XX coefficients, AD:4804-4812 */
+    int32_t NoAdvFlux;                     /* no advection across the NoFluxU faces (AD:4434-4443;
+                                              needs NoFluxU/V: the YY branch clears the XX
+                                              coefficients at the NoFluxV positions, AD:4804-4813) */
     /* open boundary (OpenBoundaryCondition, AD:5369-5665: on when
      * ReferenceProp is given, AD:5816) and the conditions applied after the
      * solve (AD:1874-1880: NullGradient_ 4, CyclicBoundary_ 8) */

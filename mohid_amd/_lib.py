@@ -70,7 +70,7 @@ class ADArgs(ct.Structure):
         ("ComputeFacesU3D", _ip), ("ComputeFacesV3D", _ip), ("ComputeFacesW3D", _ip),
         ("Visc_H", _dp), ("Diff_V", _dp), ("WaterPoints3D", _ip),
         ("Optimize", ct.c_int32), ("FirstProperty", ct.c_int32),
-        ("CellFluxes", ct.c_int32), ("unsupported_NoAdvFlux", ct.c_int32),
+        ("CellFluxes", ct.c_int32), ("NoAdvFlux", ct.c_int32),
         ("ReferenceProp", _dp), ("BoundaryCondition", ct.c_int32), ("DecayTime", ct.c_double),
         ("BoundaryPoints2D", _ip), ("PROPOld", _dp),
         ("NoDifFlux", ct.c_int32), ("NoFluxU", _ip), ("NoFluxV", _ip), ("NoFluxW", _ip),

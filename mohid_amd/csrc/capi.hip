@@ -143,7 +143,7 @@ struct DevHydro {
         HIPCHK(hipMemcpyAsync(cfv, a->ComputeFacesV3D, bi, kind, s));
         HIPCHK(hipMemcpyAsync(cfw, a->ComputeFacesW3D, bi, kind, s));
         HIPCHK(hipMemcpyAsync(water, a->WaterPoints3D, bi, kind, s));
-        have_nf = a->NoDifFlux != 0;
+        have_nf = a->NoDifFlux != 0 || a->NoAdvFlux != 0;
         if (have_nf) {
             if (!nfu) {
                 HIPCHK(hipStreamSynchronize(s));
@@ -152,7 +152,7 @@ struct DevHydro {
             }
             HIPCHK(hipMemcpyAsync(nfu, a->NoFluxU, bi, kind, s));
             HIPCHK(hipMemcpyAsync(nfv, a->NoFluxV, bi, kind, s));
-            HIPCHK(hipMemcpyAsync(nfw, a->NoFluxW, bi, kind, s));
+            if (a->NoFluxW) HIPCHK(hipMemcpyAsync(nfw, a->NoFluxW, bi, kind, s));
         }
         have_small = a->SmallDepths != nullptr;
         if (have_small) {
@@ -301,7 +301,8 @@ int check_options(const mgpu_ad_args *a) {
     // HorizontalAdvectionXX/YY ERR01 (AD:4527, 4887)
     if ((a->ImpExp_AdvXX != 0.0 && !impXX) || (a->ImpExp_AdvYY != 0.0 && !impYY))
         return MGPU_ERR_OPTION;
-    if (a->AdvectionNudging || a->unsupported_NoAdvFlux) return MGPU_ERR_UNSUPPORTED;
+    if (a->AdvectionNudging) return MGPU_ERR_UNSUPPORTED;
+    if (a->NoAdvFlux && (!a->NoFluxU || !a->NoFluxV)) return MGPU_ERR_BADARG;
     // Set_Internal_State ERR01 (AD:5816-5829): an open boundary needs a known
     // condition; Orlanski's radiation (OrlanskiCelerity2D) is not offloaded
     const int bc = a->BoundaryCondition;
@@ -442,6 +443,13 @@ struct mgpu_ad {
     DevBoundary bnd;
     FluxArrays flux{};                      // CellFluxes, allocated on first use
     bool have_flux = false;                 // the last call computed them
+    // NoAdvFlux: this call's cleared U faces, and the ones the XX coefficients
+    // keep for a later call of the step that does not recompute them
+    // (Set_Internal_State, AD:5746-5800)
+    uint8_t *xz = nullptr, *xzs = nullptr;
+    bool xzs_on = false, have_adv = false;
+    int last_adv_method = 0, last_adv_tvd = 0;
+    double last_adv_dt = 0.0;
     double last_dt = 0.0, last_schmidtH = 0.0;   // Set_Internal_State's memory (AD:5746)
     unsigned long long last_zero_pivots = 0;
 };
@@ -503,6 +511,7 @@ extern "C" int mgpu_ad_destroy(mgpu_ad *ad) {
     dfree(ad->tig); dfree(ad->mid); dfree(ad->Rb); dfree(ad->oldb);
     ad->disch.release();
     ad->bnd.release();
+    dfree(ad->xz); dfree(ad->xzs);
     dfree(ad->flux.afx); dfree(ad->flux.afy); dfree(ad->flux.afz);
     dfree(ad->flux.dfx); dfree(ad->flux.dfy); dfree(ad->flux.dfz);
     if (ad->stream) (void)hipStreamDestroy(ad->stream);
@@ -761,6 +770,42 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     }
     SharedFields f = make_fields(ad->hydro, ad->der, ad->geom);
     f.disch = ad->disch.fields();
+    // NoAdvFlux, non-optimised path only (AD:4434-4443, 4804-4813): a call
+    // that computes the XX coefficients clears them at its NoFluxU faces and,
+    // in its YY branch, at its NoFluxV positions; one that keeps them (the
+    // same DTProp and model time, a method other than P2_TVD, unchanged)
+    // sees what the last computing call left
+    if (!sc.optimize && !sc.vertical1d) {
+        const bool st = !ad->have_adv || a->FirstProperty || a->DTProp != ad->last_adv_dt ||
+                        a->AdvMethodH != ad->last_adv_method ||
+                        a->TVDLimitationH != ad->last_adv_tvd || a->AdvMethodH == TVD;
+        if (a->NoAdvFlux && (!ad->xz || !ad->xzs)) {
+            HIPCHK(hipStreamSynchronize(s));
+            if ((!ad->xz && dalloc(&ad->xz, d.n3())) || (!ad->xzs && dalloc(&ad->xzs, d.n3())))
+                return MGPU_ERR_NOMEM;
+        }
+        if (st) {
+            if (a->NoAdvFlux) {
+                if ((rc = launch_flag_or(d.n3(), ad->hydro.nfu, nullptr, ad->xz, s))) return rc;
+                if ((rc = launch_flag_or(d.n3(), ad->hydro.nfu, ad->hydro.nfv, ad->xzs, s)))
+                    return rc;
+                f.xzero = ad->xz;
+            }
+            ad->xzs_on = a->NoAdvFlux != 0;
+        } else if (ad->xzs_on) {
+            f.xzero = ad->xzs;
+        }
+        ad->have_adv = true;
+        ad->last_adv_dt = a->DTProp;
+        ad->last_adv_method = a->AdvMethodH;
+        ad->last_adv_tvd = a->TVDLimitationH;
+    } else if (!sc.vertical1d) {
+        ad->xzs_on = false;        // the optimised path rewrites the XX coefficients
+        ad->have_adv = true;
+        ad->last_adv_dt = a->DTProp;
+        ad->last_adv_method = a->AdvMethodH;
+        ad->last_adv_tvd = a->TVDLimitationH;
+    }
     if (a->ReferenceProp) {
         const DevHydro &h = ad->hydro;
         f.obc.bp = h.bp; f.obc.ref = h.ref;

@@ -122,6 +122,10 @@ struct SharedFields {     // device pointers shared by all tracers
     const uint16_t *ccnt, *fperm, *fcnt;
     DischargeFields disch;
     ObcFields obc;
+    // NoAdvFlux: U faces whose XX advection coefficients are cleared this call
+    // (AD:4434-4443, and those a previous call of the step left cleared);
+    // null: none.  The lane kernels only.
+    const uint8_t *xzero = nullptr;
 };
 
 // Kernel launchers (transport.hip)
@@ -183,6 +187,8 @@ int launch_cell_flux_implicit(const Dims &d, const StepScalars &sc, const Shared
                               const double *P, const double *diffv, const double *DZZ,
                               double sV, double bV, int nulldif, double thA, double thD,
                               const FluxArrays &o, const double *coef, hipStream_t s);
+// dst[x] = (a && a[x] == 1) || (b && b[x] == 1) over n cells.
+int launch_flag_or(long n, const int32_t *a, const int32_t *b, uint8_t *dst, hipStream_t s);
 // ptr[cells[m]] = vals ? vals[m] : -1 for m < n.
 int launch_disch_scatter(int n, const uint32_t *cells, const int32_t *vals, int32_t *ptr,
                          hipStream_t s);

@@ -117,7 +117,8 @@ int launch_transport(const TransportLaunch &L, const Dims &d, const StepScalars 
         if ((sc.vert && !L.mid) || !L.Rb || !L.lists.hrows) return MGPU_ERR_BADARG;
     }
     if (L.tig && L.n != 1) return MGPU_ERR_BADARG;   // a per-tracer record
-    if (f.obc.bp && (L.variant != MGPU_KERNEL_LANES || L.n != 1)) return MGPU_ERR_UNSUPPORTED;
+    if ((f.obc.bp || f.xzero) && (L.variant != MGPU_KERNEL_LANES || L.n != 1))
+        return MGPU_ERR_UNSUPPORTED;
     for (int t0 = 0; t0 < L.n; t0 += chunk) {
         const int nt = L.n - t0 < chunk ? L.n - t0 : chunk;
         TracerSet ts;
@@ -581,9 +582,10 @@ __global__ void cell_flux_explicit_kernel(Dims d, StepScalars sc, SharedFields f
     const bool hz = !sc.vertical1d;
     if (hz && (mk & M_CFU)) {
         // CalcHorizontalAdvFluxXX(_opt) (AD:3479-3567), weight 1 - ImpExp_AdvXX = 1
-        double fl[1];
-        hface_flux<OPT, 1, false>(d, sc, f, ts, x, d.sj, j, d.JLB, d.JUB, f.DUX + d.idx2(i, j),
-                                  d.sj, f.Qx[x], fl, bad);
+        double fl[1] = {0.0};
+        if (!(f.xzero && f.xzero[x]))      // NoAdvFlux: cleared coefficients
+            hface_flux<OPT, 1, false>(d, sc, f, ts, x, d.sj, j, d.JLB, d.JUB,
+                                      f.DUX + d.idx2(i, j), d.sj, f.Qx[x], fl, bad);
         o.afx[x] = 0.0 + 1.0 * fl[0];
         // CalcHorizontalDifFluxXX(2) (AD:3784-3867)
         o.dfx[x] = 0.0 - f.cu[x] * (P[x] - P[x - d.sj]);
@@ -693,6 +695,23 @@ int launch_cell_flux_implicit(const Dims &d, const StepScalars &sc, const Shared
     else
         hipLaunchKernelGGL(cell_flux_implicit_kernel<false>, g, dim3(64), 0, s, d, sc, f, P,
                            diffv, DZZ, sV, bV, nulldif, thA, thD, o, c4);
+    HIPCHK(hipGetLastError());
+    return MGPU_SUCCESS;
+}
+
+namespace {
+__global__ void flag_or_kernel(long n, const int32_t *__restrict__ a,
+                               const int32_t *__restrict__ b, uint8_t *__restrict__ dst) {
+    for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < n;
+         x += (long)gridDim.x * blockDim.x)
+        dst[x] = ((a && a[x] == 1) || (b && b[x] == 1)) ? 1 : 0;
+}
+}  // namespace
+
+int launch_flag_or(long n, const int32_t *a, const int32_t *b, uint8_t *dst, hipStream_t s) {
+    long blocks = (n + 255) / 256;
+    if (blocks > 65535) blocks = 65535;
+    hipLaunchKernelGGL(flag_or_kernel, dim3((unsigned)blocks), dim3(256), 0, s, n, a, b, dst);
     HIPCHK(hipGetLastError());
     return MGPU_SUCCESS;
 }

@@ -1279,13 +1279,24 @@ __device__ __forceinline__ void cell_row_lane(const Dims &d, const StepScalars &
     // implicit direction the D (fu*/fv*) and E (eu*/ev*) coefficients
     double fu0[TPL], fu1[TPL], fv0[TPL], fv1[TPL];
     double eu0[TPL], eu1[TPL], ev0[TPL], ev1[TPL];
-    if (cfu0)
+    // NoAdvFlux (the drop-in's lane kernels): a U face whose XX coefficients
+    // the reference has cleared carries no advective flux (AD:4434-4443)
+    const bool xz0 = f.xzero && f.xzero[x], xz1 = f.xzero && f.xzero[x + d.sj];
+    if (cfu0 && xz0) {
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) { fu0[t] = 0.0; eu0[t] = 0.0; }
+    } else if (cfu0) {
         hface_flux<OPT, TPL, FAST, IMP == 0>(d, sc, f, ts, x, d.sj, j, d.JLB, d.JUB,
                                              f.DUX + d.idx2(i, j), d.sj, f.Qx[x], fu0, bad, &eu0);
-    if (cfu1)
+    }
+    if (cfu1 && xz1) {
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) { fu1[t] = 0.0; eu1[t] = 0.0; }
+    } else if (cfu1) {
         hface_flux<OPT, TPL, FAST, IMP == 0>(d, sc, f, ts, x + d.sj, d.sj, j + 1, d.JLB, d.JUB,
                                              f.DUX + d.idx2(i, j + 1), d.sj, f.Qx[x + d.sj],
                                              fu1, bad, &eu1);
+    }
     if (cfv0)
         hface_flux<OPT, TPL, FAST, IMP == 1>(d, sc, f, ts, x, 1, i, d.ILB, d.IUB,
                                              f.DVY + d.idx2(i, j), 1, f.Qy[x], fv0, bad, &ev0);

@@ -73,7 +73,7 @@ module ModuleAdvectionDiffusionGPU
         type(c_ptr)        :: Visc_H, Diff_V
         type(c_ptr)        :: WaterPoints3D
         integer(c_int32_t) :: Optimize, FirstProperty
-        integer(c_int32_t) :: CellFluxes, unsupported_NoAdvFlux
+        integer(c_int32_t) :: CellFluxes, NoAdvFlux
         type(c_ptr)        :: ReferenceProp
         integer(c_int32_t) :: BoundaryCondition
         real(c_double)     :: DecayTime
@@ -222,8 +222,7 @@ contains
     !> advanced by DTProp in place.  Arguments keep the reference's names,
     !> order and meaning.  Options the GPU path does not offload make
     !> STAT = MGPU_ERR_UNSUPPORTED and leave PROP untouched:
-    !>   CellFluxes with implicit horizontal advection, NoAdvFlux true
-    !>   (AD:1321-1325), AdvectionNudging, an open
+    !>   CellFluxes with implicit horizontal advection, AdvectionNudging, an open
     !>   boundary with one layer and implicit horizontal advection, and the
     !>   Orlanski_ open boundary with implicit horizontal advection.
     !> SmallDepths, NoDifFlux (with NoFluxU/V/W), the discharges of
@@ -326,8 +325,8 @@ contains
         if (present(BoundaryPoints2D)) a%BoundaryPoints2D = c_loc(BoundaryPoints2D)
         a%PROPOld = c_null_ptr
         if (present(PROPOld)) a%PROPOld = c_loc(PROPOld)
-        a%unsupported_NoAdvFlux = 0
-        if (present(NoAdvFlux)) a%unsupported_NoAdvFlux = b2i(NoAdvFlux)
+        a%NoAdvFlux = 0
+        if (present(NoAdvFlux)) a%NoAdvFlux = b2i(NoAdvFlux)
         a%NoDifFlux = 0
         if (present(NoDifFlux)) a%NoDifFlux = b2i(NoDifFlux)
         a%NoFluxU = c_null_ptr; a%NoFluxV = c_null_ptr; a%NoFluxW = c_null_ptr

@@ -131,19 +131,64 @@ def test_options_batched(optimize):
         assert _rel_err(got[t], ref[t]) <= RTOL, t
 
 
-def test_noadvflux_stays_on_the_cpu():
+def test_flag_arguments_required():
     L, AD, _ = _mods()
     case = make("upwind_flat")
     gpu = AD(case.size, case.worksize, case.geom)
     p = np.array(case.tracers[0], order="F", copy=True)
-    with pytest.raises(L.MgpuError) as ei:
-        gpu(p, case.hydro, dict(case.params, NoAdvFlux=True))
-    assert ei.value.code == L.MGPU_ERR_UNSUPPORTED
-    with pytest.raises(L.MgpuError) as ei:         # NoDifFlux without the face flags
-        gpu(p, case.hydro, dict(case.params, NoDifFlux=True))
-    assert ei.value.code == L.MGPU_ERR_BADARG
+    for opt in ("NoAdvFlux", "NoDifFlux"):           # without the face flags
+        with pytest.raises(L.MgpuError) as ei:
+            gpu(p, case.hydro, dict(case.params, **{opt: True}))
+        assert ei.value.code == L.MGPU_ERR_BADARG
     assert np.array_equal(p, case.tracers[0])
     gpu.kill()
+
+
+@pytest.mark.parametrize("name,method,optimize,implicit", [
+    ("upwind_land_ragged", C.UPWIND_ORDER1, False, False),
+    ("upwind_land_ragged", C.CENTRAL_DIF, False, False),
+    ("upwind_land_ragged", C.UPWIND_ORDER1, False, True),
+    ("tvd_land_sigma", C.P2_TVD, False, False),
+    ("tvd_ragged_opt", C.P2_TVD, True, False),
+    ("upwind_land_ragged", C.UPWIND_ORDER1, True, False)])
+def test_noadvflux_drop_in(name, method, optimize, implicit):
+    """NoAdvFlux (AD:4434-4443, 4804-4813) on every other property: with an
+    upwind method the XX coefficients the first property clears (its NoFluxU
+    faces, and its NoFluxV positions through the YY branch) stay cleared for
+    the rest of the step; TVD recomputes them; the optimised path ignores
+    the option."""
+    case = make(name)
+    hy = _with_options(case, 13, small=False)
+    T = max(len(case.tracers), 3)
+    tracers = [np.asfortranarray(case.tracers[t % len(case.tracers)] * (1.0 + 0.1 * t))
+               for t in range(T)]
+
+    def params_of(s, t):
+        p = dict(case.params, AdvMethodH=method, Optimize=optimize,
+                 NoAdvFlux=(t + s) % 2 == 0)
+        if implicit:
+            p.update(ImpExp_AdvXX=float(s % 2 == 0), ImpExp_AdvYY=float(s % 2 == 1))
+        return p
+
+    _, AD, _ = _mods()
+    gpu = AD(case.size, case.worksize, case.geom)
+    inst = ora.Instance(case, nthreads=1)
+    pg = [t.copy(order="F") for t in tracers]
+    po = [t.copy(order="F") for t in tracers]
+    bitwise = True
+    for s in range(3):
+        for t in range(T):
+            prm = params_of(s, t)
+            assert inst.step(po[t], first_property=t == 0, now=(s + 1) * prm["DTProp"],
+                             hydro=hy, **prm) == 0
+            gpu(pg[t], hy, prm, first_property=t == 0)
+            err = _rel_err(pg[t], po[t])
+            assert err <= RTOL, (s, t, err)
+            bitwise &= bool(np.array_equal(pg[t], po[t]))
+    gpu.kill()
+    inst.close()
+    print(f"{name} method={method} optimize={optimize} implicit={implicit} NoAdvFlux: "
+          f"bit-identical={bitwise}")
 
 
 def _discharges(case, rng, T, non_open=None):

@@ -88,6 +88,9 @@ struct ora_instance {
     ora_disch disch;
     /* T_CellFluxes (AD:184-194), allocated on the first call with CellFluxes */
     double *AFX, *AFY, *AFZ, *DFX, *DFY, *DFZ;
+    /* AdvectionTi/D/E/F (AD:272-275), allocated on the first call with
+     * AdvectionNudging */
+    double *nTi, *nD, *nE, *nF;
 };
 
 #define I3(me, i, j, k) (((long)(i) - (me)->size.ILB) + (me)->nI * (((long)(j) - (me)->size.JLB) + (me)->nJ * ((long)(k) - (me)->size.KLB)))
@@ -519,6 +522,7 @@ void ora_kill(ora_instance *me)
     free(me->yC); free(me->yD); free(me->yE); free(me->yF);
     free(me->zC); free(me->zD); free(me->zE); free(me->zF);
     free(me->AFX); free(me->AFY); free(me->AFZ); free(me->DFX); free(me->DFY); free(me->DFZ);
+    free(me->nTi); free(me->nD); free(me->nE); free(me->nF);
     free(me->vecW); free(me->vecG);
     free(me);
 }
@@ -1017,6 +1021,7 @@ static void no_adv_flux_xx(ora_instance *me, const int32_t *nf)
 static int horizontal_advection_xx(ora_instance *me, const ora_args *a, const double *P)
 {
     const ora_size3d *w = &me->ws, *s = &me->size;
+    double *ti;
     double dt = a->DTProp;
     long sj = me->nI;      /* stride of a j-line */
     int rc = 0, k;
@@ -1039,8 +1044,9 @@ static int horizontal_advection_xx(ora_instance *me, const ora_args *a, const do
                                            a->VolumeRelMax, a->Upwind2H);
             }
         }
-        /* AD:4434-4443: NoAdvFlux closes the NoFluxU faces (non-optimised path) */
-        if (!a->Optimize && a->NoAdvFlux) no_adv_flux_xx(me, a->NoFluxU);
+        /* AD:4434-4443: NoAdvFlux closes the NoFluxU faces (non-optimised
+         * path; HorizontalAdvectionXX_Nudge, AD:4586-4733, has no such step) */
+        if (!a->Optimize && a->NoAdvFlux && !a->AdvectionNudging) no_adv_flux_xx(me, a->NoFluxU);
     }
     if (rc) return rc;
     if (a->ImpExp_AdvXX == IMPLICIT) {
@@ -1065,6 +1071,8 @@ static int horizontal_advection_xx(ora_instance *me, const ora_args *a, const do
         return 0;
     }
     if (a->ImpExp_AdvXX != EXPLICIT) return ORA_ERR_OPTION;   /* AD:4527 ERR01 */
+    /* with nudging the terms go to AdvectionTi (AD:4665-4668) */
+    ti = a->AdvectionNudging ? me->nTi : me->TI;
 #pragma omp parallel for schedule(static)
     for (k = w->KLB; k <= w->KUB; k++) {
         int j, i;
@@ -1078,11 +1086,11 @@ static int horizontal_advection_xx(ora_instance *me, const ora_args *a, const do
                 else
                     f = me->xC[x] * P[I3(me, i, j - 2, k)] + me->xD[x] * P[xm] +
                         me->xE[x] * P[x] + me->xF[x] * P[I3(me, i, j + 1, k)];
-                me->TI[x] = me->TI[x] + f * dt / a->VolumeZ[x];
+                ti[x] = ti[x] + f * dt / a->VolumeZ[x];
                 if (g_mistake == 1)
-                    me->TI[xm] = me->TI[xm] + f * dt / a->VolumeZ[xm];
+                    ti[xm] = ti[xm] + f * dt / a->VolumeZ[xm];
                 else
-                    me->TI[xm] = me->TI[xm] - f * dt / a->VolumeZ[xm];
+                    ti[xm] = ti[xm] - f * dt / a->VolumeZ[xm];
             }
     }
     if (a->CellFluxes)                                           /* AD:4531-4536 */
@@ -1094,6 +1102,7 @@ static int horizontal_advection_xx(ora_instance *me, const ora_args *a, const do
 static int horizontal_advection_yy(ora_instance *me, const ora_args *a, const double *P)
 {
     const ora_size3d *w = &me->ws, *s = &me->size;
+    double *ti;
     double dt = a->DTProp;
     int rc = 0, k;
     if (me->st_hadv) {
@@ -1118,7 +1127,8 @@ static int horizontal_advection_yy(ora_instance *me, const ora_args *a, const do
         /* AD:4804-4813: the reference clears the XX coefficients at the
          * NoFluxV faces here, after XX has used them; a later call of the
          * step that keeps the coefficients (Set_Internal_State) sees them */
-        if (!a->Optimize && a->NoAdvFlux) no_adv_flux_xx(me, g_mistake == 19 ? a->NoFluxU : a->NoFluxV);
+        if (!a->Optimize && a->NoAdvFlux && !a->AdvectionNudging)
+            no_adv_flux_xx(me, g_mistake == 19 ? a->NoFluxU : a->NoFluxV);
     }
     if (rc) return rc;
     if (a->ImpExp_AdvYY == IMPLICIT) {
@@ -1144,6 +1154,7 @@ static int horizontal_advection_yy(ora_instance *me, const ora_args *a, const do
         return 0;
     }
     if (a->ImpExp_AdvYY != EXPLICIT) return ORA_ERR_OPTION;   /* AD:4887 ERR01 */
+    ti = a->AdvectionNudging ? me->nTi : me->TI;              /* AD:5043-5046 */
     for (k = w->KLB; k <= w->KUB; k++) {
         int j;
 #pragma omp parallel for schedule(static)
@@ -1158,8 +1169,8 @@ static int horizontal_advection_yy(ora_instance *me, const ora_args *a, const do
                 else
                     f = me->yC[x] * P[I3(me, i - 2, j, k)] + me->yD[x] * P[xm] +
                         me->yE[x] * P[x] + me->yF[x] * P[I3(me, i + 1, j, k)];
-                me->TI[x] = me->TI[x] + f * dt / a->VolumeZ[x];
-                me->TI[xm] = me->TI[xm] - f * dt / a->VolumeZ[xm];
+                ti[x] = ti[x] + f * dt / a->VolumeZ[x];
+                ti[xm] = ti[xm] - f * dt / a->VolumeZ[xm];
             }
         }
     }
@@ -1301,7 +1312,36 @@ static int vertical_advection(ora_instance *me, const ora_args *a, const double 
             }
         }
     }
-    if (a->ImpExp_AdvV == EXPLICIT) {
+    if (a->AdvectionNudging) {
+        /* VerticalAdvection_Nudge (AD:3211-3350): every face of WorkSize in
+         * k order, into AdvectionTi (explicit) or AdvectionD/E/F (implicit) */
+        int k;
+        if (a->ImpExp_AdvV != EXPLICIT && a->ImpExp_AdvV != IMPLICIT)
+            return ORA_ERR_OPTION;                                /* AD:3336 ERR01 */
+        for (k = w->KLB; k <= w->KUB; k++) {
+            int j;
+#pragma omp parallel for schedule(static)
+            for (j = w->JLB; j <= w->JUB; j++) {
+                int i;
+                for (i = w->ILB; i <= w->IUB; i++) {
+                    long x = I3(me, i, j, k), xm = I3(me, i, j, k - 1);
+                    if (a->ComputeFacesW3D[x] != 1) continue;
+                    if (a->ImpExp_AdvV == EXPLICIT) {
+                        double f = me->zC[x] * P[I3(me, i, j, k - 2)] + me->zD[x] * P[xm] +
+                                   me->zE[x] * P[x] + me->zF[x] * P[I3(me, i, j, k + 1)];
+                        me->nTi[xm] = me->nTi[xm] - f * dt / a->VolumeZ[xm];
+                        me->nTi[x] = me->nTi[x] + f * dt / a->VolumeZ[x];
+                    } else {
+                        double dt1 = dt / a->VolumeZ[xm], dt2 = dt / a->VolumeZ[x];
+                        me->nD[x] = me->nD[x] - me->zD[x] * dt2;
+                        me->nE[x] = me->nE[x] - me->zE[x] * dt2;
+                        me->nE[xm] = me->nE[xm] + me->zD[x] * (g_mistake == 21 ? dt2 : dt1);
+                        me->nF[xm] = me->nF[xm] + me->zE[x] * dt1;
+                    }
+                }
+            }
+        }
+    } else if (a->ImpExp_AdvV == EXPLICIT) {
         if (a->Optimize) {
             if (me->docycle == 1) {
                 int j;
@@ -1413,6 +1453,56 @@ static int vertical_advection(ora_instance *me, const ora_args *a, const double 
     if (a->CellFluxes && a->ImpExp_AdvV < 1.0)                    /* AD:3129-3134 */
         calc_vadv_flux(me, a, P, 1.0 - a->ImpExp_AdvV);
     return rc;
+}
+
+/* AD:1989-2068 AdvectionNudging: the advective terms join the rows with the
+ * weight n = min(in, jn) / nCells within nCells cells of the domain's edges,
+ * 1 elsewhere, on open points.  in and jn are the reference's loop-private
+ * variables: one the cell does not set keeps the value of the last
+ * iteration that set it.  The loop below runs in the reference's order on
+ * one thread, where that carried value is the one of the edge band's
+ * innermost row (nCells), so the weight falls to 1/nCells at the edges and
+ * rises to 1 inside; with several threads the reference reads whatever its
+ * thread saw last.  With WorkSize ILB = JLB = 1 and nCells >= 1 the first
+ * iteration sets both. */
+static void advection_nudging(ora_instance *me, const ora_args *a)
+{
+    const ora_size3d *w = &me->ws;
+    const int nc = a->AdvectionNudgingCells;
+    int i, j, k, in = 0, jn = 0;
+    for (j = w->JLB; j <= w->JUB; j++)
+        for (i = w->ILB; i <= w->IUB; i++) {
+            int cell = 0;
+            double n;
+            if (i < nc + 1) {
+                cell = 1;
+                in = i - w->ILB + 1 > 0 ? i - w->ILB + 1 : 0;
+            } else if (i > w->IUB - nc) {
+                cell = 1;
+                in = w->IUB - i + 1 > 0 ? w->IUB - i + 1 : 0;
+            }
+            if (j < nc + 1) {
+                cell = 1;
+                jn = j - w->JLB + 1 > 0 ? j - w->JLB + 1 : 0;
+            } else if (j > w->JUB - nc) {
+                cell = 1;
+                jn = w->JUB - j + 1 > 0 ? w->JUB - j + 1 : 0;
+            }
+            if (cell) {
+                n = (double)(g_mistake == 20 ? (in > jn ? in : jn) : (in < jn ? in : jn));
+                n = n / (double)nc;
+            } else {
+                n = 1.0;
+            }
+            for (k = w->KLB; k <= w->KUB; k++) {
+                long x = I3(me, i, j, k);
+                if (a->OpenPoints3D[x] != 1) continue;
+                me->TI[x] = me->TI[x] + n * me->nTi[x];
+                me->D[x] = me->D[x] + n * me->nD[x];
+                me->E[x] = me->E[x] + n * me->nE[x];
+                me->F[x] = me->F[x] + n * me->nF[x];
+            }
+        }
 }
 
 /* AD:1624-1922 AdvectionDiffusionIteration */
@@ -1740,6 +1830,10 @@ static int iteration(ora_instance *me, const ora_args *a, double *P)
     if (!me->vertical1d) {
         horizontal_diffusion_xx(me, a, P);
         if (!me->xzflow) horizontal_diffusion_yy(me, a, P);
+        if (a->AdvectionNudging) {                                /* AD:4154-4157 */
+            fill(me->nTi, n3, 0.0); fill(me->nD, n3, 0.0);
+            fill(me->nE, n3, 0.0); fill(me->nF, n3, 0.0);
+        }
         rc = horizontal_advection_xx(me, a, P);
         if (rc) return rc;
         if (!me->xzflow) {
@@ -1773,6 +1867,7 @@ static int iteration(ora_instance *me, const ora_args *a, double *P)
             if (rc) return rc;
         }
     }
+    if (a->AdvectionNudging) advection_nudging(me, a);           /* AD:1739-1743 */
     /* AD:1745-1746: the open boundary rows, the last change before the solve */
     if (a->ReferenceProp) {
         rc = open_boundary_condition(me, a, P);
@@ -1844,6 +1939,24 @@ int ora_advection_diffusion(ora_instance *me, const ora_args *a, double *prop)
     if ((a->ImpExp_AdvXX != EXPLICIT && a->ImpExp_AdvXX != IMPLICIT) ||
         (a->ImpExp_AdvYY != EXPLICIT && a->ImpExp_AdvYY != IMPLICIT))
         return ORA_ERR_OPTION;
+    /* Nudging is restated on the non-optimised 3-D path with explicit
+     * horizontal advection.  Not restated: the optimised path (the _Nudge
+     * routines make non-optimised coefficients beside optimised diffusion),
+     * Vertical1D (HorizontalAdvection, which clears the accumulators, is not
+     * called, AD:1703-1722, and AdvectionNudging adds what the last call
+     * left), and implicit horizontal advection (its coefficients are added
+     * to the vertical rows after the line solve, AD:1739-1743). */
+    if (a->AdvectionNudging &&
+        ((a->Optimize && !me->vertical1d) || me->vertical1d ||
+         a->ImpExp_AdvXX == IMPLICIT || a->ImpExp_AdvYY == IMPLICIT))
+        return ORA_ERR_UNSUPPORTED;
+    if (a->AdvectionNudging && !me->nTi) {
+        me->nTi = alloc_fill(me->n3, 0.0);
+        me->nD = alloc_fill(me->n3, 0.0);
+        me->nE = alloc_fill(me->n3, 0.0);
+        me->nF = alloc_fill(me->n3, 0.0);
+        if (!me->nTi || !me->nD || !me->nE || !me->nF) return ORA_UNKNOWN;
+    }
 
     aa = *a;
     optimize = a->Optimize && !me->vertical1d;   /* AD:1419 */

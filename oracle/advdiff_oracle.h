@@ -85,6 +85,11 @@ typedef struct {
     double *PROPOld;                              /* 3-D, written with Orlanski_ (AD:5428) */
     int32_t CellFluxes;                           /* AD:1402-1407: the flux diagnostics */
     int32_t NoAdvFlux;                            /* AD:4434-4443, 4804-4813 (with NoFluxU/V) */
+    /* AdvectionNudging, AdvectionNudgingCells (AD:1179-1180): the advective
+     * terms go to their own accumulators and are added with a weight that
+     * rises from the domain's edges (AD:1989-2068) */
+    int32_t AdvectionNudging;
+    int32_t AdvectionNudgingCells;
 } ora_args;
 
 typedef struct ora_instance ora_instance;
@@ -142,7 +147,10 @@ const double *ora_internal(const ora_instance *me, const char *name);
  *     side of the interior point
  *  18 horizontal diffusive cell flux with the sign of the gradient
  *  19 NoAdvFlux: the YY branch clears the XX coefficients at the NoFluxU
- *     faces instead of the NoFluxV ones */
+ *     faces instead of the NoFluxV ones
+ *  20 nudging weight from the larger of the two edge distances
+ *  21 implicit vertical advection with nudging: the lower cell's diagonal
+ *     scaled by the upper cell's volume */
 void ora_plant_mistake(int id);
 
 /* OrlanskiCelerity2D (ModuleFunctions.F90:4129-4490) with the optional

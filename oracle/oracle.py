@@ -57,6 +57,8 @@ class Args(ct.Structure):
         ("ReferenceProp", _dp), ("BoundaryCondition", ct.c_int32), ("DecayTime", ct.c_double),
         ("BoundaryPoints2D", _ip), ("PROPOld", _dp), ("CellFluxes", ct.c_int32),
         ("NoAdvFlux", ct.c_int32),
+        ("AdvectionNudging", ct.c_int32),
+        ("AdvectionNudgingCells", ct.c_int32),
     ]
 
 
@@ -222,7 +224,9 @@ class Instance:
                     opt_i("NoFluxU"), opt_i("NoFluxV"), opt_i("NoFluxW"), opt_i("SmallDepths"),
                     ref, int(p.get("BoundaryCondition", 0)),
                     float(p.get("DecayTime", -9.9e15)), opt_i("BoundaryPoints2D"), old,
-                    int(bool(p.get("CellFluxes", False))), int(bool(p.get("NoAdvFlux", False))))
+                    int(bool(p.get("CellFluxes", False))), int(bool(p.get("NoAdvFlux", False))),
+                    int(bool(p.get("AdvectionNudging", False))),
+                    int(p.get("AdvectionNudgingCells", 0)))
         assert prop.flags.f_contiguous and prop.dtype == np.float64
         return lib().ora_advection_diffusion(self.h, ct.byref(args), _ptr(prop, _dp))
 

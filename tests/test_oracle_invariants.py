@@ -1030,6 +1030,103 @@ def test_cyclic_boundary_copies():
 
 
 # ---------------------------------------------------------------------------
+def _nudge_weights(case, nc):
+    """The nudging weight of each (i, j) of Size as AdvectionNudging means it
+    (AD:1989-2068): within nc cells of a WorkSize edge, the distance in cells
+    to the nearer i edge or j edge (the edge cell counting 1) over nc; 1
+    elsewhere.  Computed here from that definition, independently of the
+    oracle's loop."""
+    ws = case.worksize
+    ni, nj = ws[1], ws[3]
+    i = np.arange(1, ni + 1)[:, None]
+    j = np.arange(1, nj + 1)[None, :]
+    di = np.minimum(i, ni - i + 1)
+    dj = np.minimum(j, nj - j + 1)
+    n = np.ones(case.shape[:2])
+    n[1:ni + 1, 1:nj + 1] = np.minimum(np.minimum(di, dj), nc) / nc
+    return n
+
+
+def _no_advection_hydro(case, vertical=True):
+    """The case's hydro with no horizontal transport and, with vertical, no
+    transport across the interior W faces (the surface face keeps its flux,
+    which enters VolumeVariation's row, not the advection)."""
+    hy = {k: np.array(v, order="F", copy=True) for k, v in case.hydro.items()}
+    hy["Wflux_X"][:] = 0.0
+    hy["Wflux_Y"][:] = 0.0
+    if vertical:
+        hy["Wflux_Z"][:, :, :case.worksize[5] + 1] = 0.0
+    return hy
+
+
+def _nudging_err(name, check, nc=3):
+    """AdvectionNudging (AD:1989-2068, the _Nudge routines AD:3211-3350,
+    4586-4733, 4957-5118), non-optimised path:
+    * "unit": with nCells = 0 every weight is 1 and the step is the step
+      without nudging, up to the rounding of the separate accumulators
+      (explicit and implicit vertical advection);
+    * "linear": with explicit advection the rows are linear in the
+      advective terms and the weight is constant down a column, so the
+      nudged step is x0 + n (x1 - x0), x1 the step without nudging and x0
+      the step with no transport;
+    * "scaled": with implicit first-order vertical advection alone, the
+      nudged column equals the column whose W-face fluxes are scaled by its
+      weight.
+    Returns the largest difference relative to max |x|."""
+    case = make(name)
+    base = dict(case.params, Optimize=False)
+    c0 = np.array(case.tracers[0], order="F", copy=True)
+    wp = case.hydro["WaterPoints3D"] == 1
+    if check == "unit":
+        err = 0.0
+        for imp in (0.0, 1.0):
+            prm = dict(base, ImpExp_AdvV=imp)
+            a = _run(case, c0.copy(order="F"), **prm)
+            b = _run(case, c0.copy(order="F"), AdvectionNudging=True, AdvectionNudgingCells=0,
+                     **prm)
+            err = max(err, float(np.max(np.abs(a - b)[wp]) / np.max(np.abs(a)[wp])))
+        return err
+    n = _nudge_weights(case, nc)[:, :, None]
+    if check == "linear":
+        prm = dict(base, ImpExp_AdvV=0.0)
+        x1 = _run(case, c0.copy(order="F"), **prm)
+        x0 = _run(case, c0.copy(order="F"), hydro=_no_advection_hydro(case), **prm)
+        got = _run(case, c0.copy(order="F"), AdvectionNudging=True, AdvectionNudgingCells=nc,
+                   **prm)
+        want = x0 + n * (x1 - x0)
+    else:
+        prm = dict(base, ImpExp_AdvV=1.0, AdvMethodV=C.UPWIND_ORDER1)
+        hy = _no_advection_hydro(case, vertical=False)
+        got = _run(case, c0.copy(order="F"), hydro=hy, AdvectionNudging=True,
+                   AdvectionNudgingCells=nc, **prm)
+        hs = dict(hy, Wflux_Z=np.array(hy["Wflux_Z"], order="F", copy=True))
+        KUB = case.worksize[5]
+        hs["Wflux_Z"][:, :, :KUB + 1] *= n
+        want = _run(case, c0.copy(order="F"), hydro=hs, **prm)
+    return float(np.max(np.abs(got - want)[wp]) / np.max(np.abs(want)[wp]))
+
+
+@pytest.mark.parametrize("name", ["upwind_land_ragged", "tvd_land_sigma", "tvd_stretched_stiff"])
+@pytest.mark.parametrize("check", ["unit", "linear", "scaled"])
+def test_nudging(name, check):
+    err = _nudging_err(name, check)
+    print(f"nudging {check} {name}: {err:.3e}")
+    assert err <= 1e-12
+
+
+def test_nudging_unsupported_branches():
+    """Nudging with the optimised path, Vertical1D or implicit horizontal
+    advection is not restated: the oracle says so and writes nothing."""
+    case = make("upwind_land_ragged")
+    for extra in (dict(Optimize=True), dict(ImpExp_AdvXX=1.0)):
+        inst = ora.Instance(case, nthreads=1)
+        p = np.array(case.tracers[0], order="F", copy=True)
+        prm = dict(case.params, AdvectionNudging=True, AdvectionNudgingCells=2, **extra)
+        assert inst.step(p, first_property=True, now=DT, **prm) == ora.ERR_UNSUPPORTED
+        assert np.array_equal(p, case.tracers[0])
+        inst.close()
+
+
 def _all_checks():
     """(name, error, tolerance) of every check above."""
     out = []
@@ -1066,6 +1163,9 @@ def _all_checks():
         out.append((f"null gradient {name}", _null_gradient_err(name, interior), 1e-14))
     out.append(("cyclic", _cyclic_err(), 0.0))
     out.append(("no adv flux", _no_adv_flux_err(), 0.0))
+    for check in ("unit", "linear", "scaled"):
+        for name in ("upwind_land_ragged", "tvd_stretched_stiff"):
+            out.append((f"nudging {check} {name}", _nudging_err(name, check), 1e-12))
     for name, theta, imp in (("tvd_land_sigma", None, None), ("tvd_ragged_opt", None, None),
                              ("upwind_flat", 0.5, None), ("tvd_sigma", 0.0, None)):
         out.append((f"cell fluxes {name} {theta} {imp}", _cell_flux_err(name, theta, imp), 1e-13))
@@ -1084,7 +1184,7 @@ def _all_checks():
     return out
 
 
-@pytest.mark.parametrize("mistake", list(range(1, 20)))
+@pytest.mark.parametrize("mistake", list(range(1, 22)))
 def test_planted_mistakes_are_caught(mistake):
     L = _lib()
     L.ora_plant_mistake(mistake)

@@ -86,7 +86,10 @@ typedef struct {
     int32_t AdvMethodH, TVDLimitationH, AdvMethodV, TVDLimitationV;
     int32_t Upwind2H, Upwind2V;
     double VolumeRelMax;
-    int32_t AdvectionNudging, AdvectionNudgingCells;
+    int32_t AdvectionNudging, AdvectionNudgingCells;   /* AD:1989-2068: the advective terms
+                                              join the rows with a weight rising from
+                                              1/cells at the edges to 1 (non-optimised
+                                              3-D path, explicit horizontal advection) */
     double DTProp;
     double ImpExp_AdvV, ImpExp_DifV, ImpExp_AdvXX, ImpExp_AdvYY, ImpExp_DifH;
     int32_t NullDif;

@@ -301,7 +301,6 @@ int check_options(const mgpu_ad_args *a) {
     // HorizontalAdvectionXX/YY ERR01 (AD:4527, 4887)
     if ((a->ImpExp_AdvXX != 0.0 && !impXX) || (a->ImpExp_AdvYY != 0.0 && !impYY))
         return MGPU_ERR_OPTION;
-    if (a->AdvectionNudging) return MGPU_ERR_UNSUPPORTED;
     if (a->NoAdvFlux && (!a->NoFluxU || !a->NoFluxV)) return MGPU_ERR_BADARG;
     // Set_Internal_State ERR01 (AD:5816-5829): an open boundary needs a known
     // condition; Orlanski's radiation (OrlanskiCelerity2D) is not offloaded
@@ -448,6 +447,11 @@ struct mgpu_ad {
     // (Set_Internal_State, AD:5746-5800)
     uint8_t *xz = nullptr, *xzs = nullptr;
     bool xzs_on = false, have_adv = false;
+    // AdvectionNudging: AdvectionTi of the horizontal terms, and the column
+    // weights of nw_cells (AD:1989-2068), both allocated on first use
+    double *nadv = nullptr, *nw = nullptr;
+    int nw_cells = 0;
+    bool have_nw = false;
     int last_adv_method = 0, last_adv_tvd = 0;
     double last_adv_dt = 0.0;
     double last_dt = 0.0, last_schmidtH = 0.0;   // Set_Internal_State's memory (AD:5746)
@@ -512,6 +516,7 @@ extern "C" int mgpu_ad_destroy(mgpu_ad *ad) {
     ad->disch.release();
     ad->bnd.release();
     dfree(ad->xz); dfree(ad->xzs);
+    dfree(ad->nadv); dfree(ad->nw);
     dfree(ad->flux.afx); dfree(ad->flux.afy); dfree(ad->flux.afz);
     dfree(ad->flux.dfx); dfree(ad->flux.dfy); dfree(ad->flux.dfz);
     if (ad->stream) (void)hipStreamDestroy(ad->stream);
@@ -690,6 +695,43 @@ extern "C" int mgpu_ad_unset_discharges(mgpu_ad *ad) {
     return MGPU_SUCCESS;
 }
 
+// The nudging weight of every column (AdvectionNudging, AD:1989-2068), in the
+// reference's loop order: n = min(in, jn) / nCells where the column lies
+// within nCells cells of an i or j edge, 1 elsewhere.  in and jn are the
+// loop's private variables there: a column that does not set one reads the
+// value the last iteration that set it left (the innermost row of the edge
+// band), as one thread running the loop in order does.  Returns false where
+// that value would be undefined (an edge band that starts after the first
+// iteration).
+static bool nudge_weights(const Dims &d, int nc, std::vector<double> &w) {
+    w.assign((size_t)d.n2(), 1.0);
+    int in = 0, jn = 0;
+    bool has_in = false, has_jn = false;
+    for (int j = d.wJLB; j <= d.wJUB; ++j)
+        for (int i = d.wILB; i <= d.wIUB; ++i) {
+            bool cell = false;
+            if (i < nc + 1) {
+                cell = true; has_in = true;
+                in = std::max(0, i - d.wILB + 1);
+            } else if (i > d.wIUB - nc) {
+                cell = true; has_in = true;
+                in = std::max(0, d.wIUB - i + 1);
+            }
+            if (j < nc + 1) {
+                cell = true; has_jn = true;
+                jn = std::max(0, j - d.wJLB + 1);
+            } else if (j > d.wJUB - nc) {
+                cell = true; has_jn = true;
+                jn = std::max(0, d.wJUB - j + 1);
+            }
+            if (!cell) continue;
+            if (!has_in || !has_jn) return false;
+            double n = (double)std::min(in, jn);
+            w[(size_t)d.idx2(i, j)] = n / (double)nc;
+        }
+    return true;
+}
+
 extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     if (!ad) return MGPU_OFF_ERR;
     if (!a || !prop) return MGPU_ERR_BADARG;
@@ -712,6 +754,24 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     // line-solve records share their scratch)
     const bool cfl = a->CellFluxes != 0;
     if (cfl && sc.impH >= 0) return MGPU_ERR_UNSUPPORTED;
+    // nudging on the non-optimised 3-D path with explicit horizontal
+    // advection; not offloaded: the optimised path (the _Nudge routines make
+    // non-optimised coefficients beside optimised diffusion), Vertical1D
+    // (AdvectionNudging adds what an earlier call left in the accumulators)
+    // and implicit horizontal advection (its coefficients reach the vertical
+    // rows after the line solve, AD:1739-1743)
+    const bool nud = a->AdvectionNudging != 0;
+    if (nud && (sc.optimize || sc.vertical1d || sc.impH >= 0)) return MGPU_ERR_UNSUPPORTED;
+    if (nud && (!ad->have_nw || ad->nw_cells != a->AdvectionNudgingCells)) {
+        std::vector<double> w;
+        if (!nudge_weights(d, a->AdvectionNudgingCells, w)) return MGPU_ERR_UNSUPPORTED;
+        HIPCHK(hipStreamSynchronize(s));
+        if (!ad->nadv && dalloc(&ad->nadv, d.n3())) return MGPU_ERR_NOMEM;
+        if (!ad->nw && dalloc(&ad->nw, d.n2())) return MGPU_ERR_NOMEM;
+        HIPCHK(hipMemcpy(ad->nw, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+        ad->nw_cells = a->AdvectionNudgingCells;
+        ad->have_nw = true;
+    }
     ad->have_flux = false;
     if (cfl && !ad->flux.afx) {
         FluxArrays &q = ad->flux;
@@ -779,19 +839,21 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
         const bool st = !ad->have_adv || a->FirstProperty || a->DTProp != ad->last_adv_dt ||
                         a->AdvMethodH != ad->last_adv_method ||
                         a->TVDLimitationH != ad->last_adv_tvd || a->AdvMethodH == TVD;
-        if (a->NoAdvFlux && (!ad->xz || !ad->xzs)) {
+        // HorizontalAdvectionXX/YY_Nudge clear nothing (AD:4586-4733, 4957-5118)
+        const bool noadv = a->NoAdvFlux && !nud;
+        if (noadv && (!ad->xz || !ad->xzs)) {
             HIPCHK(hipStreamSynchronize(s));
             if ((!ad->xz && dalloc(&ad->xz, d.n3())) || (!ad->xzs && dalloc(&ad->xzs, d.n3())))
                 return MGPU_ERR_NOMEM;
         }
         if (st) {
-            if (a->NoAdvFlux) {
+            if (noadv) {
                 if ((rc = launch_flag_or(d.n3(), ad->hydro.nfu, nullptr, ad->xz, s))) return rc;
                 if ((rc = launch_flag_or(d.n3(), ad->hydro.nfu, ad->hydro.nfv, ad->xzs, s)))
                     return rc;
                 f.xzero = ad->xz;
             }
-            ad->xzs_on = a->NoAdvFlux != 0;
+            ad->xzs_on = noadv;
         } else if (ad->xzs_on) {
             f.xzero = ad->xzs;
         }
@@ -824,6 +886,11 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
             f.obc.dux = ad->geom.DUX; f.obc.dvy = ad->geom.DVY;
             f.obc.extv = ad->mid;      // mid is free: no implicit line solves here
         }
+    }
+    if (nud) {
+        HIPCHK(hipMemsetAsync(ad->nadv, 0, b3, s));             // AD:4154-4157
+        f.nadv = ad->nadv;
+        f.nw = ad->nw;
     }
     // the step's own initialisation of TI (AD:1660-1671) reaches the ghost rows
     if (first) HIPCHK(hipMemsetAsync(ad->tig, 0, b3, s));
@@ -1178,6 +1245,12 @@ extern "C" int mgpu_transport_set_hydro(mgpu_transport *t, const mgpu_ad_args *a
     if (!a) return MGPU_ERR_BADARG;
     int rc = check_options(a);
     if (rc) return rc;
+    // the batched loop covers NoDifFlux and SmallDepths; the per-call
+    // branches (open boundary and the passes after the solve, cell fluxes,
+    // NoAdvFlux, nudging) belong to the drop-in
+    if (a->ReferenceProp || a->BoundaryCondition == 4 || a->BoundaryCondition == 8 ||
+        a->CellFluxes || a->NoAdvFlux || a->AdvectionNudging)
+        return MGPU_ERR_UNSUPPORTED;
     const int vertical1d = 0, xzflow = 0;
     const StepScalars sc = make_scalars(t->d, a, vertical1d, xzflow);
     if (sc.impH >= 0) {

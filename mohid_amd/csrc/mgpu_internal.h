@@ -126,6 +126,13 @@ struct SharedFields {     // device pointers shared by all tracers
     // (AD:4434-4443, and those a previous call of the step left cleared);
     // null: none.  The lane kernels only.
     const uint8_t *xzero = nullptr;
+    // AdvectionNudging (AD:1989-2068): the explicit kernel writes each cell's
+    // horizontal advective terms to nadv (AdvectionTi) instead of TI; the
+    // column kernel adds its vertical ones and joins them to the rows with
+    // the weight nw of the column (Size i, j).  null: no nudging.  The lane
+    // kernels with one tracer only.
+    double *nadv = nullptr;
+    const double *nw = nullptr;
 };
 
 // Kernel launchers (transport.hip)

@@ -117,7 +117,7 @@ int launch_transport(const TransportLaunch &L, const Dims &d, const StepScalars 
         if ((sc.vert && !L.mid) || !L.Rb || !L.lists.hrows) return MGPU_ERR_BADARG;
     }
     if (L.tig && L.n != 1) return MGPU_ERR_BADARG;   // a per-tracer record
-    if ((f.obc.bp || f.xzero) && (L.variant != MGPU_KERNEL_LANES || L.n != 1))
+    if ((f.obc.bp || f.xzero || f.nadv) && (L.variant != MGPU_KERNEL_LANES || L.n != 1))
         return MGPU_ERR_UNSUPPORTED;
     for (int t0 = 0; t0 < L.n; t0 += chunk) {
         const int nt = L.n - t0 < chunk ? L.n - t0 : chunk;

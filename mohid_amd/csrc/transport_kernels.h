@@ -899,8 +899,17 @@ __device__ __forceinline__ void column_row(const Dims &d, const StepScalars &sc,
                 TI = TI + a1 * (pp - pk) * (1.0 - sc.thV);
             }
         }
+        // nudging: the advective terms in AdvectionTi/D/E/F (VerticalAdvection_Nudge,
+        // AD:3211-3350), joined to the rows with the column's weight on open
+        // points (AdvectionNudging, AD:2055-2062); the explicit kernel left
+        // the horizontal ones in nadv
+        const bool nud = f.nadv != nullptr;
+        double aT = nud ? f.nadv[x * ts.s + t] : 0.0, aD = 0.0, aE = 0.0, aF = 0.0;
         if (sc.vert && !sc.vertical1d) {
-            if (sc.impAdvV) {
+            if (sc.impAdvV && nud) {
+                if (cfwB) { aD = aD - c.fb[t].d * dtk; aE = aE - c.fb[t].e * dtk; }
+                if (cfwT) { aE = aE + ft[t].d * dtk; aF = aF + ft[t].e * dtk; }
+            } else if (sc.impAdvV) {
                 if (cfwB) { D = D - c.fb[t].d * dtk; E = E - c.fb[t].e * dtk; }
                 if (cfwT) { E = E + ft[t].d * dtk; F = F + ft[t].e * dtk; }
             } else if (OPT) {
@@ -917,14 +926,23 @@ __device__ __forceinline__ void column_row(const Dims &d, const StepScalars &sc,
                         (c.fb[t].c == 0.0 || k - 2 < d.KLB) ? 0.0 : ts.P[(x - 2 * sk) * ts.s + t];
                     const double fl =
                         c.fb[t].c * pmm + c.fb[t].d * pm + c.fb[t].e * pk + c.fb[t].f * pp;
-                    TI = TI + fl * dt / vk;
+                    if (nud) aT = aT + fl * dt / vk;
+                    else TI = TI + fl * dt / vk;
                 }
                 if (cfwT) {
                     const double fl = ft[t].c * pm + ft[t].d * pk + ft[t].e * pp +
                                       ft[t].f * c.wp[t][3];
-                    TI = TI - fl * dt / vk;
+                    if (nud) aT = aT - fl * dt / vk;
+                    else TI = TI - fl * dt / vk;
                 }
             }
+        }
+        if (nud && (mk & M_OPEN)) {
+            const double n = f.nw[base];
+            TI = TI + n * aT;
+            D = D + n * aD;
+            E = E + n * aE;
+            F = F + n * aF;
         }
         if constexpr (OBC) {
             // boundary column (base is its (i, j) index: KLB is 0), levels
@@ -1336,17 +1354,23 @@ __device__ __forceinline__ void cell_row_lane(const Dims &d, const StepScalars &
             TI = ex ? TI + bv1 * g / vk : TI + wv1 * g;
         }
         const bool au = IMP != 0, av = IMP != 1;   // explicit directions
+        // with nudging the advective terms gather in AdvectionTi, from zero
+        // (AD:4154-4157, 4665-4668, 5043-5046)
+        const bool nud = IMP < 0 && f.nadv;
+        double A = nud ? 0.0 : TI;
         if (FAST) {
-            if (au && cfu0) TI = TI + fu0[t] * dtv;
-            if (au && cfu1) TI = TI - fu1[t] * dtv;
-            if (av && cfv0) TI = TI + fv0[t] * dtv;
-            if (av && cfv1) TI = TI - fv1[t] * dtv;
+            if (au && cfu0) A = A + fu0[t] * dtv;
+            if (au && cfu1) A = A - fu1[t] * dtv;
+            if (av && cfv0) A = A + fv0[t] * dtv;
+            if (av && cfv1) A = A - fv1[t] * dtv;
         } else {
-            if (au && cfu0) TI = TI + fu0[t] * dt / vk;
-            if (au && cfu1) TI = TI - fu1[t] * dt / vk;
-            if (av && cfv0) TI = TI + fv0[t] * dt / vk;
-            if (av && cfv1) TI = TI - fv1[t] * dt / vk;
+            if (au && cfu0) A = A + fu0[t] * dt / vk;
+            if (au && cfu1) A = A - fu1[t] * dt / vk;
+            if (av && cfv0) A = A + fv0[t] * dt / vk;
+            if (av && cfv1) A = A - fv1[t] * dt / vk;
         }
+        if (nud) f.nadv[x * s + t] = A;
+        else TI = A;
         TIo[t] = TI;
         if constexpr (IMP >= 0) {
             const bool c0 = IMP == 0 ? cfu0 : cfv0, c1 = IMP == 0 ? cfu1 : cfv1;

@@ -222,9 +222,11 @@ contains
     !> advanced by DTProp in place.  Arguments keep the reference's names,
     !> order and meaning.  Options the GPU path does not offload make
     !> STAT = MGPU_ERR_UNSUPPORTED and leave PROP untouched:
-    !>   CellFluxes with implicit horizontal advection, AdvectionNudging, an open
-    !>   boundary with one layer and implicit horizontal advection, and the
-    !>   Orlanski_ open boundary with implicit horizontal advection.
+    !>   CellFluxes with implicit horizontal advection, an open boundary with
+    !>   one layer and implicit horizontal advection, the Orlanski_ open
+    !>   boundary with implicit horizontal advection, and AdvectionNudging
+    !>   with Optimize, Vertical1D or implicit horizontal advection.
+    !> AdvectionNudging (AD:1989-2068), NoAdvFlux (with NoFluxU/V), CellFluxes,
     !> SmallDepths, NoDifFlux (with NoFluxU/V/W), the discharges of
     !> SetDischargesGPU, the open boundary (ReferenceProp, BoundaryCondition,
     !> DecayTime, AD:5369-5665) and the NullGradient_ / CyclicBoundary_ passes

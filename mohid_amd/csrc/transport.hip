@@ -119,6 +119,9 @@ int launch_transport(const TransportLaunch &L, const Dims &d, const StepScalars 
     if (L.tig && L.n != 1) return MGPU_ERR_BADARG;   // a per-tracer record
     if ((f.obc.bp || f.xzero || f.nadv) && (L.variant != MGPU_KERNEL_LANES || L.n != 1))
         return MGPU_ERR_UNSUPPORTED;
+    // NoAdvFlux and nudging exist on the non-optimised path only: the
+    // optimised kernel instances are compiled without them
+    if ((f.xzero || f.nadv) && sc.optimize) return MGPU_ERR_UNSUPPORTED;
     for (int t0 = 0; t0 < L.n; t0 += chunk) {
         const int nt = L.n - t0 < chunk ? L.n - t0 : chunk;
         TracerSet ts;

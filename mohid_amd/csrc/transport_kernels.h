@@ -903,7 +903,7 @@ __device__ __forceinline__ void column_row(const Dims &d, const StepScalars &sc,
         // AD:3211-3350), joined to the rows with the column's weight on open
         // points (AdvectionNudging, AD:2055-2062); the explicit kernel left
         // the horizontal ones in nadv
-        const bool nud = f.nadv != nullptr;
+        const bool nud = !OPT && NT == 1 && f.nadv != nullptr;
         double aT = nud ? f.nadv[x * ts.s + t] : 0.0, aD = 0.0, aE = 0.0, aF = 0.0;
         if (sc.vert && !sc.vertical1d) {
             if (sc.impAdvV && nud) {
@@ -1299,7 +1299,9 @@ __device__ __forceinline__ void cell_row_lane(const Dims &d, const StepScalars &
     double eu0[TPL], eu1[TPL], ev0[TPL], ev1[TPL];
     // NoAdvFlux (the drop-in's lane kernels): a U face whose XX coefficients
     // the reference has cleared carries no advective flux (AD:4434-4443)
-    const bool xz0 = f.xzero && f.xzero[x], xz1 = f.xzero && f.xzero[x + d.sj];
+    // (non-optimised path only: compiled out of the optimised instances)
+    const bool xzf = !OPT && f.xzero;
+    const bool xz0 = xzf && f.xzero[x], xz1 = xzf && f.xzero[x + d.sj];
     if (cfu0 && xz0) {
 #pragma unroll
         for (int t = 0; t < TPL; ++t) { fu0[t] = 0.0; eu0[t] = 0.0; }
@@ -1356,7 +1358,7 @@ __device__ __forceinline__ void cell_row_lane(const Dims &d, const StepScalars &
         const bool au = IMP != 0, av = IMP != 1;   // explicit directions
         // with nudging the advective terms gather in AdvectionTi, from zero
         // (AD:4154-4157, 4665-4668, 5043-5046)
-        const bool nud = IMP < 0 && f.nadv;
+        const bool nud = IMP < 0 && !OPT && TPL == 1 && f.nadv;
         double A = nud ? 0.0 : TI;
         if (FAST) {
             if (au && cfu0) A = A + fu0[t] * dtv;

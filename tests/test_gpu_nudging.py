@@ -7,7 +7,7 @@ VerticalAdvection_Nudge, ModuleAdvectionDiffusion.F90:3211-3350, 4586-4733,
 1/AdvectionNudgingCells at the domain's edges to 1 inside (AdvectionNudging,
 AD:1989-2068).  Every call of a few steps and properties, nudged and not,
 explicit and implicit vertical advection, with NoAdvFlux, cell fluxes and
-discharges beside it; |gpu - oracle| <= 1e-12 |oracle| on every cell.
+the open boundary beside it; |gpu - oracle| <= 1e-12 |oracle| on every cell.
 """
 import numpy as np
 import pytest
@@ -135,3 +135,20 @@ def test_nudging_unsupported_branches():
             tr.set_hydro(hy, dict(case.params, **extra))
         assert ei.value.code == L.MGPU_ERR_UNSUPPORTED, extra
         tr.close()
+
+
+@pytest.mark.parametrize("bc", [C.MASS_CONSERVATION, C.IMPOSED_VALUE, C.NULL_GRADIENT])
+def test_nudging_with_open_boundary(bc):
+    """Nudging joins the advective terms before the open boundary rewrites
+    the boundary rows (AD:1739-1746), and the NullGradient_ pass follows the
+    solve."""
+    from .test_gpu_boundaries import _boundary_points, _compare, _hydro
+    case = make("tvd_land_sigma")
+    bp = _boundary_points(case, interior=2, seed=bc)
+    hy = _hydro(case, bp)
+    rng = np.random.default_rng(40 + bc)
+    refs = [np.asfortranarray(rng.uniform(0.5, 30.0, size=case.shape)) for _ in case.tracers]
+    base = dict(case.params, Optimize=False, BoundaryCondition=bc, DecayTime=1800.0,
+                AdvectionNudging=True, AdvectionNudgingCells=3)
+    worst, bitwise = _compare(case, hy, lambda s, t: base, refs)
+    print(f"nudging with boundary condition {bc}: {worst:.3e} bit-identical={bitwise}")

@@ -102,12 +102,18 @@ struct DevGeometry {
     }
 };
 
-// Hydrodynamic inputs held on the device (owned copies).
+// Hydrodynamic inputs on the device.  The kernels read the views below: the
+// owned copies, or, on a drop-in call with device pointers (which reads them
+// only before it returns), the caller's own arrays, so that nothing is copied.
 struct DevHydro {
-    double *Qx = nullptr, *Qy = nullptr, *Qz = nullptr, *V = nullptr, *Vold = nullptr;
-    double *visc = nullptr, *diffv = nullptr;
-    int32_t *open = nullptr, *land = nullptr, *cfu = nullptr, *cfv = nullptr, *cfw = nullptr,
-            *water = nullptr;
+    const double *Qx = nullptr, *Qy = nullptr, *Qz = nullptr, *V = nullptr, *Vold = nullptr;
+    const double *visc = nullptr, *diffv = nullptr;
+    const int32_t *open = nullptr, *land = nullptr, *cfu = nullptr, *cfv = nullptr,
+                  *cfw = nullptr, *water = nullptr;
+    // owned copies (Qx, Qy, Qz, V, Vold, visc, diffv / open, land, cfu, cfv,
+    // cfw, water), allocated on the first call that copies
+    double *od[7] = {};
+    int32_t *oi[6] = {};
     // optional inputs, allocated on first use; have_* tells whether this
     // step's call passed them
     int32_t *nfu = nullptr, *nfv = nullptr, *nfw = nullptr, *small = nullptr;
@@ -116,33 +122,44 @@ struct DevHydro {
     int32_t *bp = nullptr;        // BoundaryPoints2D
     bool have_ref = false, have_bp = false;
 
-    int alloc(const Dims &d) {
-        const long n3 = d.n3();
-        int rc = 0;
-        rc |= dalloc(&Qx, n3); rc |= dalloc(&Qy, n3); rc |= dalloc(&Qz, n3);
-        rc |= dalloc(&V, n3); rc |= dalloc(&Vold, n3); rc |= dalloc(&visc, n3);
-        rc |= dalloc(&diffv, n3);
-        rc |= dalloc(&open, n3); rc |= dalloc(&land, n3); rc |= dalloc(&cfu, n3);
-        rc |= dalloc(&cfv, n3); rc |= dalloc(&cfw, n3); rc |= dalloc(&water, n3);
-        return rc ? MGPU_ERR_NOMEM : MGPU_SUCCESS;
-    }
-    int upload(const Dims &d, const mgpu_ad_args *a, hipStream_t s) {
+    // view: read the caller's device arrays in place (a drop-in call with
+    // device pointers); otherwise copy them into the owned arrays
+    int upload(const Dims &d, const mgpu_ad_args *a, hipStream_t s, bool view = false) {
         const hipMemcpyKind kind =
             a->device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
         const size_t b = sizeof(double) * (size_t)d.n3(), bi = sizeof(int32_t) * (size_t)d.n3();
-        HIPCHK(hipMemcpyAsync(Qx, a->Wflux_X, b, kind, s));
-        HIPCHK(hipMemcpyAsync(Qy, a->Wflux_Y, b, kind, s));
-        HIPCHK(hipMemcpyAsync(Qz, a->Wflux_Z, b, kind, s));
-        HIPCHK(hipMemcpyAsync(V, a->VolumeZ, b, kind, s));
-        HIPCHK(hipMemcpyAsync(Vold, a->VolumeZOld, b, kind, s));
-        HIPCHK(hipMemcpyAsync(visc, a->Visc_H, b, kind, s));
-        HIPCHK(hipMemcpyAsync(diffv, a->Diff_V, b, kind, s));
-        HIPCHK(hipMemcpyAsync(open, a->OpenPoints3D, bi, kind, s));
-        HIPCHK(hipMemcpyAsync(land, a->LandPoints3D, bi, kind, s));
-        HIPCHK(hipMemcpyAsync(cfu, a->ComputeFacesU3D, bi, kind, s));
-        HIPCHK(hipMemcpyAsync(cfv, a->ComputeFacesV3D, bi, kind, s));
-        HIPCHK(hipMemcpyAsync(cfw, a->ComputeFacesW3D, bi, kind, s));
-        HIPCHK(hipMemcpyAsync(water, a->WaterPoints3D, bi, kind, s));
+        const double *sd[7] = {a->Wflux_X, a->Wflux_Y, a->Wflux_Z, a->VolumeZ, a->VolumeZOld,
+                               a->Visc_H, a->Diff_V};
+        const int32_t *si[6] = {a->OpenPoints3D, a->LandPoints3D, a->ComputeFacesU3D,
+                                a->ComputeFacesV3D, a->ComputeFacesW3D, a->WaterPoints3D};
+        const double *vd[7];
+        const int32_t *vi[6];
+        if (view && a->device_pointers) {
+            for (int q = 0; q < 7; ++q) vd[q] = sd[q];
+            for (int q = 0; q < 6; ++q) vi[q] = si[q];
+        } else {
+            if (!oi[5]) {
+                HIPCHK(hipStreamSynchronize(s));
+                int rc = 0;
+                for (int q = 0; q < 7; ++q) rc |= dalloc(&od[q], d.n3());
+                for (int q = 0; q < 6; ++q) rc |= dalloc(&oi[q], d.n3());
+                if (rc) {
+                    for (double *&q : od) dfree(q);
+                    for (int32_t *&q : oi) dfree(q);
+                    return MGPU_ERR_NOMEM;
+                }
+            }
+            for (int q = 0; q < 7; ++q) {
+                HIPCHK(hipMemcpyAsync(od[q], sd[q], b, kind, s));
+                vd[q] = od[q];
+            }
+            for (int q = 0; q < 6; ++q) {
+                HIPCHK(hipMemcpyAsync(oi[q], si[q], bi, kind, s));
+                vi[q] = oi[q];
+            }
+        }
+        Qx = vd[0]; Qy = vd[1]; Qz = vd[2]; V = vd[3]; Vold = vd[4]; visc = vd[5]; diffv = vd[6];
+        open = vi[0]; land = vi[1]; cfu = vi[2]; cfv = vi[3]; cfw = vi[4]; water = vi[5];
         have_nf = a->NoDifFlux != 0 || a->NoAdvFlux != 0;
         if (have_nf) {
             if (!nfu) {
@@ -183,8 +200,8 @@ struct DevHydro {
         return MGPU_SUCCESS;
     }
     void release() {
-        dfree(Qx); dfree(Qy); dfree(Qz); dfree(V); dfree(Vold); dfree(visc); dfree(diffv);
-        dfree(open); dfree(land); dfree(cfu); dfree(cfv); dfree(cfw); dfree(water);
+        for (double *&q : od) dfree(q);
+        for (int32_t *&q : oi) dfree(q);
         dfree(nfu); dfree(nfv); dfree(nfw); dfree(small); dfree(ref); dfree(bp);
     }
 };
@@ -481,7 +498,6 @@ extern "C" int mgpu_ad_create(const mgpu_size3d *size, const mgpu_size3d *worksi
     }
     const long n3 = ad->d.n3();
     rc = ad->geom.alloc(ad->d);
-    if (!rc) rc = ad->hydro.alloc(ad->d);
     if (!rc) rc = ad->der.alloc(ad->d);
     if (!rc) rc = dalloc(&ad->prop, n3);
     if (!rc) rc = dalloc(&ad->out, n3);
@@ -783,7 +799,7 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     const bool post = a->BoundaryCondition == 4 || a->BoundaryCondition == 8;
     if (post && (rc = ad->bnd.alloc(d))) return rc;
 
-    rc = ad->hydro.upload(d, a, s);
+    rc = ad->hydro.upload(d, a, s, true);
     if (rc) return rc;
     double *P = ad->prop;
     if (a->device_pointers) P = prop;
@@ -1220,7 +1236,6 @@ extern "C" int mgpu_transport_create(const mgpu_size3d *size, const mgpu_size3d 
     }
     const long n3 = t->d.n3();
     rc = t->geom.alloc(t->d);
-    if (!rc) rc = t->hydro.alloc(t->d);
     if (!rc) rc = t->der.alloc(t->d);
     const long nT = (long)opts->ntracers;
     // scratch: tile-blocked (split, fused) or laid out like the state (lanes)

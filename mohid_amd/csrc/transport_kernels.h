@@ -1422,6 +1422,10 @@ explicit_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
     if (bad) atomicOr(badflag, 1u);
 }
 
+// Levels at the top of a column whose Thomas coefficients W and G stay in
+// registers in the lane column kernel, by tracers per lane.
+__host__ __device__ constexpr int column_window(int tpl) { return tpl == 1 ? 4 : (tpl == 2 ? 2 : 0); }
+
 // Column kernel of the lane variant: NT / TPL lanes per column; lane (n, u)
 // solves column cols[n] for tracers u*TPL .. u*TPL+TPL-1: rows D/E/F,
 // forward sweep (W over TI in TW, G in Gs), back substitution
@@ -1459,6 +1463,11 @@ column_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
     const bool vflux = sc.vert && !sc.vertical1d && columnBuild;
     ColumnWalk<TPL> c;
     column_begin<OPT, TPL, FAST>(d, sc, f, t1, base, ks, vflux, c, bad);
+    // the top WIN levels keep W and G in registers: slot KUB - k, the same
+    // for the whole wave (k is), so each access is one static slot behind a
+    // uniform branch; the levels below go through the scratch arrays
+    constexpr int WIN = column_window(TPL);
+    double rW[WIN > 0 ? WIN : 1][TPL], rG[WIN > 0 ? WIN : 1][TPL];
     for (int k = kw; k <= KUB; ++k) {
         if (k < ks) continue;
         const long x = base + k * sk;
@@ -1467,8 +1476,18 @@ column_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
         load_tpl<TPL>((sc.hreset ? t1.P : tw) + x * s, TI);
         column_row<OPT, TPL, FAST, OBC>(d, sc, f, t1, base, k, vflux, c, TI, W, G, zero_pivots,
                                         bad);
-        store_tpl<TPL>(tw + x * s, W);
-        store_tpl<TPL>(gs + x * s, G);
+        const int slot = KUB - k;
+        if (slot < WIN) {
+#pragma unroll
+            for (int r = 0; r < WIN; ++r)
+                if (r == slot) {
+#pragma unroll
+                    for (int u = 0; u < TPL; ++u) { rW[r][u] = W[u]; rG[r][u] = G[u]; }
+                }
+        } else {
+            store_tpl<TPL>(tw + x * s, W);
+            store_tpl<TPL>(gs + x * s, G);
+        }
     }
     double xs[TPL];
     column_ghost<TPL>(d, f, t1, base, c, xs);
@@ -1476,8 +1495,18 @@ column_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
         if (k < ks) continue;
         const long x = base + k * sk;
         double W[TPL], G[TPL];
-        load_tpl<TPL>(tw + x * s, W);
-        load_tpl<TPL>(gs + x * s, G);
+        const int slot = KUB - k;
+        if (slot < WIN) {
+#pragma unroll
+            for (int r = 0; r < WIN; ++r)
+                if (r == slot) {
+#pragma unroll
+                    for (int u = 0; u < TPL; ++u) { W[u] = rW[r][u]; G[u] = rG[r][u]; }
+                }
+        } else {
+            load_tpl<TPL>(tw + x * s, W);
+            load_tpl<TPL>(gs + x * s, G);
+        }
 #pragma unroll
         for (int u = 0; u < TPL; ++u) xs[u] = W[u] * xs[u] + G[u];
         store_tpl<TPL>(t1.out + x * s, xs);

@@ -7,8 +7,9 @@
 !   adgpu_driver <input> <output>
 !
 ! Input (stream, native byte order): Size(6), WorkSize(6), nsteps,
-! iparam(8) = AdvMethodH, TVDLimitationH, AdvMethodV, TVDLimitationV,
-!             Upwind2H, Upwind2V, NullDif, Optimize (0/1),
+! iparam(10) = AdvMethodH, TVDLimitationH, AdvMethodV, TVDLimitationV,
+!              Upwind2H, Upwind2V, NullDif, Optimize (0/1),
+!              AdvectionNudging (0/1), AdvectionNudgingCells,
 ! rparam(10) = schmidt_H, SchmidtCoef_V, SchmidtBackground_V, VolumeRelMax,
 !              DTProp, ImpExp_AdvV, ImpExp_DifV, ImpExp_AdvXX, ImpExp_AdvYY,
 !              ImpExp_DifH,
@@ -32,7 +33,7 @@ program adgpu_driver
 
     character(len=1024) :: fin, fout
     integer :: u, s, ID, STAT, nsteps
-    integer :: sz(6), ws(6), ipar(8)
+    integer :: sz(6), ws(6), ipar(10)
     real(8) :: rpar(10)
     real(8), allocatable, dimension(:,:)   :: DUX, DVY, DZX, DZY
     integer, allocatable, dimension(:,:)   :: KFloorZ
@@ -85,7 +86,8 @@ program adgpu_driver
         endif
         call AdvectionDiffusionGPU(ID, PROP, rpar(1), rpar(2), rpar(3),                  &
                                    ipar(1), ipar(2), ipar(3), ipar(4),                   &
-                                   ipar(5) /= 0, ipar(6) /= 0, rpar(4), .false., 0,       &
+                                   ipar(5) /= 0, ipar(6) /= 0, rpar(4), ipar(9) /= 0,     &
+                                   ipar(10),                                              &
                                    rpar(5), rpar(6), rpar(7), rpar(8), rpar(9), rpar(10), &
                                    ipar(7) /= 0, Qx, Qy, Qz, VolOld, Vol, OP, LP,         &
                                    CFU, CFV, CFW, Visc, Diff, WP,                         &

@@ -22,7 +22,9 @@ pytestmark = pytest.mark.gpu
 def _write_input(path, case, params, steps, disch=None):
     ipar = np.array([params["AdvMethodH"], params["TVDLimitationH"], params["AdvMethodV"],
                      params["TVDLimitationV"], int(params["Upwind2H"]), int(params["Upwind2V"]),
-                     int(params["NullDif"]), int(params["Optimize"])], dtype=np.int32)
+                     int(params["NullDif"]), int(params["Optimize"]),
+                     int(bool(params.get("AdvectionNudging", False))),
+                     int(params.get("AdvectionNudgingCells", 0))], dtype=np.int32)
     rpar = np.array([params["schmidt_H"], params["SchmidtCoef_V"],
                      params["SchmidtBackground_V"], params["VolumeRelMax"], params["DTProp"],
                      params["ImpExp_AdvV"], params["ImpExp_DifV"], params["ImpExp_AdvXX"],
@@ -62,11 +64,12 @@ def _write_input(path, case, params, steps, disch=None):
             put(disch[n], np.int32)
 
 
-@pytest.mark.parametrize("name,implicit,discharges", [
-    ("tvd_sigma", None, False), ("upwind_land_ragged", None, False), ("tvd_sigma", "x", False),
-    ("upwind_land_ragged", "y", False), ("tvd_land_sigma", None, True),
-    ("tvd_land_sigma", "x", True)])
-def test_fortran_driver_matches_oracle(name, implicit, discharges, tmp_path):
+@pytest.mark.parametrize("name,implicit,discharges,nudging", [
+    ("tvd_sigma", None, False, 0), ("upwind_land_ragged", None, False, 0),
+    ("tvd_sigma", "x", False, 0), ("upwind_land_ragged", "y", False, 0),
+    ("tvd_land_sigma", None, True, 0), ("tvd_land_sigma", "x", True, 0),
+    ("upwind_land_ragged", None, False, 3), ("tvd_sigma", None, True, 2)])
+def test_fortran_driver_matches_oracle(name, implicit, discharges, nudging, tmp_path):
     if not os.path.exists(B.DRIVER):
         pytest.fail(f"Fortran driver not built: {B.DRIVER} (python -m mohid_amd.build)")
     case = make(name)
@@ -75,6 +78,10 @@ def test_fortran_driver_matches_oracle(name, implicit, discharges, tmp_path):
         # implicit horizontal advection (the THOMAS_3D line solves)
         params["ImpExp_AdvXX"] = float(implicit == "x")
         params["ImpExp_AdvYY"] = float(implicit == "y")
+    if nudging:
+        # AdvectionNudging / AdvectionNudgingCells through the Fortran logical
+        # and integer arguments (non-optimised path)
+        params.update(AdvectionNudging=True, AdvectionNudgingCells=nudging, Optimize=False)
     steps = 4
     # SetDischargesGPU / UnSetDischargesGPU around every call (IgnoreDisch and
     # ByPass go through the Fortran logical arrays)
@@ -92,8 +99,7 @@ def test_fortran_driver_matches_oracle(name, implicit, discharges, tmp_path):
         if disch is not None:
             assert inst.set_discharges(disch) == 0
         assert inst.step(ref, first_property=True, now=(s + 1) * params["DTProp"],
-                         ImpExp_AdvXX=params["ImpExp_AdvXX"],
-                         ImpExp_AdvYY=params["ImpExp_AdvYY"]) == 0
+                         **params) == 0
         if disch is not None:
             assert inst.unset_discharges() == 0
     inst.close()
@@ -104,7 +110,8 @@ def test_fortran_driver_matches_oracle(name, implicit, discharges, tmp_path):
         solved[...] = True          # the line solves also write the lines' other cells
     den = np.maximum(np.abs(ref[solved]), 1e-300)
     err = float(np.max(np.abs(got[solved] - ref[solved]) / den))
-    print(f"{name} implicit={implicit} discharges={discharges}: fortran driver max rel err {err:.3e}, "
+    print(f"{name} implicit={implicit} discharges={discharges} nudging={nudging}: "
+          f"fortran driver max rel err {err:.3e}, "
           f"bit-identical={bool(np.array_equal(got, ref))}")
     assert err <= 1e-12
     assert np.array_equal(got[~solved], ref[~solved])

@@ -23,8 +23,16 @@
 ! DischJ, DischK, DischKmin, DischKmax (nc int), DischVert, IgnoreDisch,
 ! DischnCells, ByPass (DischNumber int, 0/1 for the logicals).  The
 ! discharges are set before every step and unset after it, as
-! ModuleWaterProperties does around each call (WP:14763, 14830).
-! Output: PROP after nsteps steps.  Exit status 0 on success, else the STAT.
+! ModuleWaterProperties does around each call (WP:14763, 14830).  Then the
+! optional arguments: iopt(7) = CellFluxes, NoAdvFlux, NoDifFlux (0/1),
+! NoFlux arrays given, SmallDepths given (0/1), BoundaryCondition (0:
+! absent), ReferenceProp given (0/1); DecayTime (real); NoFluxU, NoFluxV,
+! NoFluxW (3-D int) when given; SmallDepths (2-D int, 0/1) when given;
+! BoundaryPoints2D (2-D int) when BoundaryCondition or ReferenceProp is
+! given; ReferenceProp (3-D real) when given.  An optional argument the
+! input does not give is left unallocated, which makes it absent.
+! Output: PROP after nsteps steps, then with CellFluxes the six flux arrays
+! of the last step (GetFluxesGPU).  Exit status 0 on success, else the STAT.
 !------------------------------------------------------------------------------
 program adgpu_driver
 
@@ -43,6 +51,14 @@ program adgpu_driver
     integer :: ndis, ncell
     real(8), allocatable, dimension(:) :: DFlow, DConc, DConcMF
     integer, allocatable, dimension(:) :: DI, DJ, DK, DKmin, DKmax, DVert, DIgn, DnC, DByp
+    integer :: iopt(7)
+    real(8) :: rdecay
+    integer, allocatable, dimension(:,:,:) :: NFU, NFV, NFW
+    integer, allocatable, dimension(:,:)   :: SD, BP
+    logical, allocatable, dimension(:,:)   :: SmallD
+    real(8), allocatable, dimension(:,:,:) :: RefP, POld, AFX, AFY, AFZ, DFX, DFY, DFZ
+    integer, allocatable :: BC
+    real(8), allocatable :: Decay
 
     call get_command_argument(1, fin)
     call get_command_argument(2, fout)
@@ -65,6 +81,30 @@ program adgpu_driver
         read(u) DFlow(1:ncell), DConc(1:ncell), DConcMF(1:ncell)
         read(u) DI(1:ncell), DJ(1:ncell), DK(1:ncell), DKmin(1:ncell), DKmax(1:ncell)
         read(u) DVert(1:ndis), DIgn(1:ndis), DnC(1:ndis), DByp(1:ndis)
+    endif
+    read(u) iopt, rdecay
+    if (iopt(4) /= 0) then
+        allocate(NFU, NFV, NFW, mold=OP)
+        read(u) NFU, NFV, NFW
+    endif
+    if (iopt(5) /= 0) then
+        allocate(SD(sz(1):sz(2), sz(3):sz(4)), SmallD(sz(1):sz(2), sz(3):sz(4)))
+        read(u) SD
+        SmallD = SD /= 0
+    endif
+    if (iopt(6) /= 0 .or. iopt(7) /= 0) then
+        allocate(BP(sz(1):sz(2), sz(3):sz(4)))
+        read(u) BP
+    endif
+    if (iopt(6) /= 0) then
+        allocate(BC, Decay)
+        BC = iopt(6)
+        Decay = rdecay
+        if (BC == 6) allocate(POld, mold=AreaU)
+    endif
+    if (iopt(7) /= 0) then
+        allocate(RefP, mold=AreaU)
+        read(u) RefP
     endif
     close(u)
 
@@ -91,6 +131,12 @@ program adgpu_driver
                                    rpar(5), rpar(6), rpar(7), rpar(8), rpar(9), rpar(10), &
                                    ipar(7) /= 0, Qx, Qy, Qz, VolOld, Vol, OP, LP,         &
                                    CFU, CFV, CFW, Visc, Diff, WP,                         &
+                                   CellFluxes = iopt(1) /= 0, ReferenceProp = RefP,       &
+                                   BoundaryCondition = BC, DecayTime = Decay,             &
+                                   BoundaryPoints2D = BP, PROPOld = POld,                 &
+                                   SmallDepths = SmallD, NoAdvFlux = iopt(2) /= 0,        &
+                                   NoDifFlux = iopt(3) /= 0, NoFluxU = NFU,               &
+                                   NoFluxV = NFV, NoFluxW = NFW,                          &
                                    Optimize = ipar(8) /= 0, FirstProperty = .true.,      &
                                    STAT = STAT)
         if (STAT /= MGPU_SUCCESS) then
@@ -102,10 +148,19 @@ program adgpu_driver
             if (STAT /= MGPU_SUCCESS) error stop 5
         endif
     enddo
+    if (iopt(1) /= 0) then
+        allocate(AFX, AFY, AFZ, DFX, DFY, DFZ, mold=AreaU)
+        call GetFluxesGPU(ID, AFX, AFY, AFZ, DFX, DFY, DFZ, STAT = STAT)
+        if (STAT /= MGPU_SUCCESS) then
+            write(*, *) 'GetFluxesGPU STAT =', STAT
+            error stop 6
+        endif
+    endif
     call KillAdvectionDiffusionGPU(ID, STAT = STAT)
 
     open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
     write(u) PROP
+    if (iopt(1) /= 0) write(u) AFX, AFY, AFZ, DFX, DFY, DFZ
     close(u)
 
 end program adgpu_driver

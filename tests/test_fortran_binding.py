@@ -19,7 +19,7 @@ from .test_gpu_options import _discharges
 pytestmark = pytest.mark.gpu
 
 
-def _write_input(path, case, params, steps, disch=None):
+def _write_input(path, case, params, steps, disch=None, hydro=None, opts=None):
     ipar = np.array([params["AdvMethodH"], params["TVDLimitationH"], params["AdvMethodV"],
                      params["TVDLimitationV"], int(params["Upwind2H"]), int(params["Upwind2V"]),
                      int(params["NullDif"]), int(params["Optimize"]),
@@ -29,7 +29,7 @@ def _write_input(path, case, params, steps, disch=None):
                      params["SchmidtBackground_V"], params["VolumeRelMax"], params["DTProp"],
                      params["ImpExp_AdvV"], params["ImpExp_DifV"], params["ImpExp_AdvXX"],
                      params["ImpExp_AdvYY"], params["ImpExp_DifH"]], dtype=np.float64)
-    g, h = case.geom, case.hydro
+    g, h = case.geom, (case.hydro if hydro is None else hydro)
     with open(path, "wb") as f:
         np.array(case.size, dtype=np.int32).tofile(f)
         np.array(case.worksize, dtype=np.int32).tofile(f)
@@ -56,12 +56,31 @@ def _write_input(path, case, params, steps, disch=None):
         put(case.tracers[0], np.float64)
         if disch is None:
             np.array([0, 0], dtype=np.int32).tofile(f)
-            return
-        np.array([len(disch["vert"]), len(disch["flow"])], dtype=np.int32).tofile(f)
-        for n in ("flow", "conc", "concMF"):
-            put(disch[n], np.float64)
-        for n in ("I", "J", "K", "Kmin", "Kmax", "vert", "ignore", "ncells", "bypass"):
-            put(disch[n], np.int32)
+        else:
+            np.array([len(disch["vert"]), len(disch["flow"])], dtype=np.int32).tofile(f)
+            for n in ("flow", "conc", "concMF"):
+                put(disch[n], np.float64)
+            for n in ("I", "J", "K", "Kmin", "Kmax", "vert", "ignore", "ncells", "bypass"):
+                put(disch[n], np.int32)
+        # the optional arguments: what is not written stays absent
+        o = opts or {}
+        nof = "NoFluxU" in h
+        small = "SmallDepths" in h
+        bc = int(o.get("BoundaryCondition", 0))
+        ref = o.get("ReferenceProp")
+        np.array([int(bool(o.get("CellFluxes"))), int(bool(o.get("NoAdvFlux"))),
+                  int(bool(o.get("NoDifFlux"))), int(nof), int(small), bc,
+                  int(ref is not None)], dtype=np.int32).tofile(f)
+        np.array([float(o.get("DecayTime", 0.0))], dtype=np.float64).tofile(f)
+        if nof:
+            for n in ("NoFluxU", "NoFluxV", "NoFluxW"):
+                put(h[n], np.int32)
+        if small:
+            put(h["SmallDepths"], np.int32)
+        if bc or ref is not None:
+            put(h["BoundaryPoints2D"], np.int32)
+        if ref is not None:
+            put(ref, np.float64)
 
 
 @pytest.mark.parametrize("name,implicit,discharges,nudging", [
@@ -115,3 +134,77 @@ def test_fortran_driver_matches_oracle(name, implicit, discharges, nudging, tmp_
           f"bit-identical={bool(np.array_equal(got, ref))}")
     assert err <= 1e-12
     assert np.array_equal(got[~solved], ref[~solved])
+
+
+FLUXES = ("AdvFluxX", "AdvFluxY", "AdvFluxZ", "DifFluxX", "DifFluxY", "DifFluxZ")
+
+
+@pytest.mark.parametrize("name,kind", [
+    ("tvd_land_sigma", "nodif_small"), ("upwind_land_ragged", "noadv"),
+    ("tvd_land_sigma", "open_mass_fluxes"), ("upwind_land_ragged", "null_gradient"),
+    ("tvd_land_sigma", "orlanski")])
+def test_fortran_optional_arguments(name, kind, tmp_path):
+    """The optional arguments of AdvectionDiffusionGPU through the Fortran
+    module: NoFluxU/V/W with NoDifFlux and SmallDepths, NoAdvFlux, the open
+    boundary (ReferenceProp, BoundaryCondition, DecayTime, BoundaryPoints2D)
+    with CellFluxes read back by GetFluxesGPU, NullGradient_ alone, and
+    Orlanski_ with PROPOld; the absent ones stay absent (unallocated
+    actual arguments)."""
+    from .test_gpu_boundaries import _boundary_points, _hydro
+    if not os.path.exists(B.DRIVER):
+        pytest.fail(f"Fortran driver not built: {B.DRIVER} (python -m mohid_amd.build)")
+    case = make(name)
+    params = dict(case.params, Optimize=False)
+    rng = np.random.default_rng(9)
+    hy = {k: np.array(v, order="F", copy=True) for k, v in case.hydro.items()}
+    opts = {}
+    if kind in ("nodif_small", "noadv"):
+        for n in ("NoFluxU", "NoFluxV", "NoFluxW"):
+            hy[n] = np.asfortranarray((rng.uniform(size=case.shape) < 0.3).astype(np.int32))
+        opts = dict(NoDifFlux=kind == "nodif_small", NoAdvFlux=kind == "noadv")
+        if kind == "nodif_small":
+            hy["SmallDepths"] = np.asfortranarray(
+                (rng.uniform(size=case.shape[:2]) < 0.4).astype(np.int32))
+    else:
+        bc = {"open_mass_fluxes": 1, "null_gradient": 4, "orlanski": 6}[kind]
+        # Orlanski's exterior cells exist on the domain's edges only
+        hy = _hydro(case, _boundary_points(case, interior=0 if bc == 6 else 2, seed=bc))
+        opts = dict(BoundaryCondition=bc, DecayTime=1800.0, CellFluxes=kind == "open_mass_fluxes")
+        if bc != 4:
+            opts["ReferenceProp"] = np.asfortranarray(rng.uniform(0.5, 30.0, size=case.shape))
+    steps = 3
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    _write_input(fin, case, params, steps, hydro=hy, opts=opts)
+    print(f"fortran driver options: {name} {kind}", flush=True)
+    res = subprocess.run([B.DRIVER, str(fin), str(fout)], capture_output=True, text=True,
+                         timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    raw = np.fromfile(fout, dtype=np.float64)
+    n3 = int(np.prod(case.shape))
+    got = raw[:n3].reshape(case.shape, order="F")
+    ref = np.array(case.tracers[0], order="F", copy=True)
+    inst = ora.Instance(case, nthreads=1)
+    old = np.asfortranarray(np.zeros(case.shape)) if kind == "orlanski" else None
+    extra = {k: v for k, v in opts.items() if k != "ReferenceProp"}
+    if "ReferenceProp" in opts:
+        extra["ReferenceProp"] = opts["ReferenceProp"]
+    if old is not None:
+        extra["PROPOld"] = old
+    for s in range(steps):
+        assert inst.step(ref, first_property=True, now=(s + 1) * params["DTProp"], hydro=hy,
+                         **params, **extra) == 0
+    want_fl = {n: np.array(inst.internal(n), copy=True) for n in FLUXES} \
+        if opts.get("CellFluxes") else None
+    inst.close()
+    den = np.maximum(np.abs(ref), 1e-300)
+    err = float(np.max(np.abs(got - ref) / den))
+    bitwise = bool(np.array_equal(got, ref))
+    if want_fl is not None:
+        assert raw.size == 7 * n3
+        for q, n in enumerate(FLUXES):
+            fl = raw[(q + 1) * n3:(q + 2) * n3].reshape(case.shape, order="F")
+            e = float(np.max(np.abs(fl - want_fl[n]) / np.maximum(np.abs(want_fl[n]), 1e-300)))
+            assert e <= 1e-12, (n, e)
+            bitwise &= bool(np.array_equal(fl, want_fl[n]))
+    print(f"{name} {kind}: fortran driver max rel err {err:.3e}, bit-identical={bitwise}")
+    assert err <= 1e-12

@@ -65,6 +65,7 @@ FC = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
 FDIR = os.path.join(HERE, "fortran")
 FBUILD = os.path.join(FDIR, "_build")
 DRIVER = os.path.join(FDIR, "adgpu_driver")
+PRODUCERS_DRIVER = os.path.join(FDIR, "producers_driver")
 # the reference's production build is default-real-8 (compile_mohid.sh:185)
 FFLAGS = ["-cpp", "-O2", "-fdefault-real-8", "-fdefault-double-8"]
 
@@ -75,11 +76,16 @@ def build_fortran(verbose: bool = False) -> str:
     os.makedirs(FBUILD, exist_ok=True)
     mod_o = os.path.join(FBUILD, "ModuleAdvectionDiffusionGPU.o")
     drv_o = os.path.join(FBUILD, "adgpu_driver.o")
+    prd_o = os.path.join(FBUILD, "producers_driver.o")
     steps = [
         [FC, *FFLAGS, "-c", os.path.join(FDIR, "ModuleAdvectionDiffusionGPU.F90"), "-o", mod_o,
          "-J", FBUILD],
         [FC, *FFLAGS, "-c", os.path.join(FDIR, "adgpu_driver.F90"), "-o", drv_o, "-J", FBUILD],
         [FC, "-o", DRIVER + ".tmp", drv_o, mod_o, f"-L{HERE}", "-lmohid_gpu",
+         "-Wl,-rpath,$ORIGIN/.."],
+        [FC, *FFLAGS, "-c", os.path.join(FDIR, "producers_driver.F90"), "-o", prd_o, "-J",
+         FBUILD],
+        [FC, "-o", PRODUCERS_DRIVER + ".tmp", prd_o, mod_o, f"-L{HERE}", "-lmohid_gpu",
          "-Wl,-rpath,$ORIGIN/.."],
     ]
     for cmd in steps:
@@ -89,6 +95,7 @@ def build_fortran(verbose: bool = False) -> str:
         if verbose and (res.stdout or res.stderr).strip():
             print(res.stdout + res.stderr)
     os.replace(DRIVER + ".tmp", DRIVER)
+    os.replace(PRODUCERS_DRIVER + ".tmp", PRODUCERS_DRIVER)
     return DRIVER
 
 

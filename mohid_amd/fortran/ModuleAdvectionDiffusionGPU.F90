@@ -31,13 +31,17 @@ module ModuleAdvectionDiffusionGPU
     public :: KillAdvectionDiffusionGPU
     public :: THOMASZ_GPU
     public :: THOMAS_3D_GPU
+    public :: WaterFluxZGPU
+    public :: ComputeVolumesGPU
+    public :: ComputeAreasGPU
     public :: SetDischargesGPU, UnSetDischargesGPU
     public :: GetFluxesGPU
     public :: MGPU_SUCCESS, MGPU_OFF_ERR, MGPU_ERR_OPTION, MGPU_ERR_UNSUPPORTED
-    public :: MGPU_ERR_PIVOT
+    public :: MGPU_ERR_PIVOT, MGPU_ERR_BADARG
 
     integer, parameter :: MGPU_SUCCESS         = 0
     integer, parameter :: MGPU_OFF_ERR         = 1
+    integer, parameter :: MGPU_ERR_BADARG      = 101
     integer, parameter :: MGPU_ERR_OPTION      = 102
     integer, parameter :: MGPU_ERR_UNSUPPORTED = 103
     integer, parameter :: MGPU_ERR_PIVOT       = 107
@@ -151,6 +155,34 @@ module ModuleAdvectionDiffusionGPU
             type(c_ptr), value :: handle, afx, afy, afz, dfx, dfy, dfz
             integer(c_int32_t), value :: dev
         end function mgpu_ad_get_fluxes
+
+        integer(c_int) function mgpu_waterflux_z(size, worksize, vnew, vold, qx, qy, qz, &
+                                                 cfu, cfv, wp, dt, vargrid, dev)           &
+                bind(c, name="mgpu_waterflux_z")
+            import :: c_int, c_int32_t, c_ptr, c_double, t_mgpu_size3d
+            type(t_mgpu_size3d), intent(in) :: size, worksize
+            type(c_ptr),         value      :: vnew, vold, qx, qy, qz, cfu, cfv, wp
+            real(c_double),      value      :: dt
+            integer(c_int32_t),  value      :: vargrid, dev
+        end function mgpu_waterflux_z
+
+        integer(c_int) function mgpu_compute_volumes(size, worksize, dwz, dux, dvy, vz, vu, &
+                                                     vv, vw, dev)                           &
+                bind(c, name="mgpu_compute_volumes")
+            import :: c_int, c_int32_t, c_ptr, t_mgpu_size3d
+            type(t_mgpu_size3d), intent(in) :: size, worksize
+            type(c_ptr),         value      :: dwz, dux, dvy, vz, vu, vv, vw
+            integer(c_int32_t),  value      :: dev
+        end function mgpu_compute_volumes
+
+        integer(c_int) function mgpu_compute_areas(size, worksize, duz, dvz, dxx, dyy, au, av, &
+                                                   szz, cu, cxu, cv, cxv, dev)                 &
+                bind(c, name="mgpu_compute_areas")
+            import :: c_int, c_int32_t, c_ptr, t_mgpu_size3d
+            type(t_mgpu_size3d), intent(in) :: size, worksize
+            type(c_ptr),         value      :: duz, dvz, dxx, dyy, au, av, szz, cu, cxu, cv, cxv
+            integer(c_int32_t),  value      :: dev
+        end function mgpu_compute_areas
     end interface
 
     ! instance table: integer IDs as MOHID's modules hand out (ModuleGlobalData
@@ -469,5 +501,88 @@ contains
                            c_loc(F), c_loc(TI), c_loc(RES), 0_c_int32_t, c_null_ptr)
         if (present(STAT)) STAT = rc
     end subroutine THOMAS_3D_GPU
+
+    !> The continuity step of ModuleHydrodynamic (WaterFlux_Z_VarGrid with
+    !> Volume_Z_New and Volume_Z_Old, else WaterFlux_Z_FixGrid;
+    !> ModuleHydrodynamic.F90:43270-43343): WaterFlux_Z of every water column,
+    !> level by level upwards, from the horizontal fluxes (Size-shaped arrays).
+    subroutine WaterFluxZGPU(Size, WorkSize, WaterFlux_X, WaterFlux_Y, WaterFlux_Z,       &
+                             ComputeFaces3D_U, ComputeFaces3D_V, WaterPoints3D, DT,        &
+                             Volume_Z_New, Volume_Z_Old, STAT)
+        integer, dimension(6),             intent(in)    :: Size, WorkSize
+        real(8), dimension(:,:,:), contiguous, target, intent(in)    :: WaterFlux_X, WaterFlux_Y
+        real(8), dimension(:,:,:), contiguous, target, intent(inout) :: WaterFlux_Z
+        integer, dimension(:,:,:), contiguous, target, intent(in)    :: ComputeFaces3D_U
+        integer, dimension(:,:,:), contiguous, target, intent(in)    :: ComputeFaces3D_V
+        integer, dimension(:,:,:), contiguous, target, intent(in)    :: WaterPoints3D
+        real(8),                           intent(in)    :: DT
+        real(8), dimension(:,:,:), optional, contiguous, target, intent(in) :: Volume_Z_New
+        real(8), dimension(:,:,:), optional, contiguous, target, intent(in) :: Volume_Z_Old
+        integer, optional,                 intent(out)   :: STAT
+        type(c_ptr) :: vn, vo
+        integer :: rc
+        vn = c_null_ptr
+        vo = c_null_ptr
+        if (present(Volume_Z_New)) vn = c_loc(Volume_Z_New)
+        if (present(Volume_Z_Old)) vo = c_loc(Volume_Z_Old)
+        rc = mgpu_waterflux_z(to_size3d(Size), to_size3d(WorkSize), vn, vo, c_loc(WaterFlux_X), &
+                              c_loc(WaterFlux_Y), c_loc(WaterFlux_Z), c_loc(ComputeFaces3D_U),  &
+                              c_loc(ComputeFaces3D_V), c_loc(WaterPoints3D), real(DT, c_double), &
+                              merge(1_c_int32_t, 0_c_int32_t, present(Volume_Z_New)),           &
+                              0_c_int32_t)
+        if (present(STAT)) STAT = rc
+    end subroutine WaterFluxZGPU
+
+    !> ComputeVolumes of ModuleGeometry (ModuleGeometry.F90:4108-4231):
+    !> VolumeZ = DWZ DUX DVY and the face volumes; VolumeW only when given.
+    subroutine ComputeVolumesGPU(Size, WorkSize, DWZ, DUX, DVY, VolumeZ, VolumeU, VolumeV,   &
+                                 VolumeW, STAT)
+        integer, dimension(6),             intent(in)    :: Size, WorkSize
+        real(8), dimension(:,:,:), contiguous, target, intent(in)    :: DWZ
+        real(8), dimension(:,:),   contiguous, target, intent(in)    :: DUX, DVY
+        real(8), dimension(:,:,:), contiguous, target, intent(inout) :: VolumeZ, VolumeU, VolumeV
+        real(8), dimension(:,:,:), optional, contiguous, target, intent(inout) :: VolumeW
+        integer, optional,                 intent(out)   :: STAT
+        type(c_ptr) :: vw
+        integer :: rc
+        vw = c_null_ptr
+        if (present(VolumeW)) vw = c_loc(VolumeW)
+        rc = mgpu_compute_volumes(to_size3d(Size), to_size3d(WorkSize), c_loc(DWZ), c_loc(DUX), &
+                                  c_loc(DVY), c_loc(VolumeZ), c_loc(VolumeU), c_loc(VolumeV),    &
+                                  vw, 0_c_int32_t)
+        if (present(STAT)) STAT = rc
+    end subroutine ComputeVolumesGPU
+
+    !> ComputeAreas of ModuleGeometry (ModuleGeometry.F90:4238-4327): AreaU
+    !> and AreaV; with SZZ and the impermeability coefficients (indexed by k
+    !> over Size) the reduced areas.
+    subroutine ComputeAreasGPU(Size, WorkSize, DUZ, DVZ, DXX, DYY, AreaU, AreaV, SZZ,        &
+                               Coef_U, CoefX_U, Coef_V, CoefX_V, STAT)
+        integer, dimension(6),             intent(in)    :: Size, WorkSize
+        real(8), dimension(:,:,:), contiguous, target, intent(in)    :: DUZ, DVZ
+        real(8), dimension(:,:),   contiguous, target, intent(in)    :: DXX, DYY
+        real(8), dimension(:,:,:), contiguous, target, intent(inout) :: AreaU, AreaV
+        real(8), dimension(:,:,:), optional, contiguous, target, intent(in) :: SZZ
+        real(8), dimension(:), optional, contiguous, target, intent(in) :: Coef_U, CoefX_U
+        real(8), dimension(:), optional, contiguous, target, intent(in) :: Coef_V, CoefX_V
+        integer, optional,                 intent(out)   :: STAT
+        type(c_ptr) :: p(5)
+        integer :: rc
+        p = c_null_ptr
+        if (present(SZZ)) then
+            if (.not. (present(Coef_U) .and. present(CoefX_U) .and. present(Coef_V) .and.    &
+                       present(CoefX_V))) then
+                if (present(STAT)) STAT = MGPU_ERR_BADARG
+                return
+            endif
+            p(1) = c_loc(SZZ)
+            p(2) = c_loc(Coef_U); p(3) = c_loc(CoefX_U)
+            p(4) = c_loc(Coef_V); p(5) = c_loc(CoefX_V)
+        endif
+        rc = mgpu_compute_areas(to_size3d(Size), to_size3d(WorkSize), c_loc(DUZ), c_loc(DVZ), &
+                                c_loc(DXX), c_loc(DYY), c_loc(AreaU), c_loc(AreaV), p(1), p(2), &
+                                p(3), p(4), p(5), 0_c_int32_t)
+        if (present(STAT)) STAT = rc
+    end subroutine ComputeAreasGPU
 
 end module ModuleAdvectionDiffusionGPU

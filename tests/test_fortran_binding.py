@@ -208,3 +208,59 @@ def test_fortran_optional_arguments(name, kind, tmp_path):
             bitwise &= bool(np.array_equal(fl, want_fl[n]))
     print(f"{name} {kind}: fortran driver max rel err {err:.3e}, bit-identical={bitwise}")
     assert err <= 1e-12
+
+
+@pytest.mark.parametrize("vargrid", [False, True])
+def test_fortran_producers_match_oracle(vargrid, tmp_path):
+    """WaterFluxZGPU, ComputeVolumesGPU and ComputeAreasGPU through the
+    Fortran module (mohid_amd/fortran/producers_driver.F90) against the
+    oracle's continuity, ComputeVolumes and ComputeAreas: bit-identical."""
+    import ctypes as ct
+    if not os.path.exists(B.PRODUCERS_DRIVER):
+        pytest.fail(f"Fortran driver not built: {B.PRODUCERS_DRIVER} (python -m mohid_amd.build)")
+    ni, nj, nk = 23, 19, 9
+    size, ws = (0, ni + 1, 0, nj + 1, 0, nk + 1), (1, ni, 1, nj, 1, nk)
+    shape = (ni + 2, nj + 2, nk + 2)
+    rng = np.random.default_rng(5 + vargrid)
+    f3 = lambda lo, hi: np.asfortranarray(rng.uniform(lo, hi, size=shape))
+    f2 = lambda lo, hi: np.asfortranarray(rng.uniform(lo, hi, size=shape[:2]))
+    i3 = lambda p: np.asfortranarray((rng.uniform(size=shape) < p).astype(np.int32))
+    dux, dvy, dxx, dyy = f2(50, 200), f2(50, 200), f2(50, 200), f2(50, 200)
+    dwz, duz, dvz = f3(0.2, 5), f3(0.2, 5), f3(0.2, 5)
+    vn = f3(1e3, 2e3)
+    vo = np.asfortranarray(vn * (1 + 0.01 * rng.normal(size=shape)))
+    qx = np.asfortranarray(rng.normal(size=shape))
+    qy = np.asfortranarray(rng.normal(size=shape))
+    qz = np.asfortranarray(rng.normal(size=shape))
+    cfu, cfv, wp = i3(0.8), i3(0.8), i3(0.9)
+    dt = 30.0
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        np.array([*size, *ws, int(vargrid)], dtype=np.int32).tofile(f)
+        np.array([dt], dtype=np.float64).tofile(f)
+        for a in (dux, dvy, dxx, dyy, dwz, duz, dvz, vn, vo, qx, qy, qz):
+            a.ravel(order="F").tofile(f)
+        for a in (cfu, cfv, wp):
+            a.ravel(order="F").tofile(f)
+    res = subprocess.run([B.PRODUCERS_DRIVER, str(fin), str(fout)], capture_output=True,
+                         text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    raw = np.fromfile(fout, dtype=np.float64)
+    n3 = int(np.prod(shape))
+    got = [raw[q * n3:(q + 1) * n3].reshape(shape, order="F") for q in range(6)]
+    S, W = ora.Size3D(*size), ora.Size3D(*ws)
+    dp = lambda a: a.ctypes.data_as(ora._dp)
+    ip = lambda a: a.ctypes.data_as(ora._ip)
+    wz = qz.copy(order="F")
+    ora.lib().ora_waterflux_z(ct.byref(S), ct.byref(W), dp(vn) if vargrid else None,
+                              dp(vo) if vargrid else None, dp(qx), dp(qy), dp(wz), ip(cfu),
+                              ip(cfv), ip(wp), dt, int(vargrid))
+    vz, vu, vv, au, av = (qz.copy(order="F") for _ in range(5))
+    ora.lib().ora_compute_volumes(ct.byref(S), ct.byref(W), dp(dwz), dp(dux), dp(dvy), dp(vz),
+                                  dp(vu), dp(vv), None)
+    ora.lib().ora_compute_areas(ct.byref(S), ct.byref(W), dp(duz), dp(dvz), dp(dxx), dp(dyy),
+                                dp(au), dp(av), None, None, None, None, None)
+    for n, g, w in zip(("WaterFlux_Z", "VolumeZ", "VolumeU", "VolumeV", "AreaU", "AreaV"), got,
+                       (wz, vz, vu, vv, au, av)):
+        assert np.array_equal(g, w), n
+    print(f"fortran producers vargrid={vargrid}: bit-identical")

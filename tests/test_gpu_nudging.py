@@ -49,7 +49,10 @@ def _flags(case, seed):
     ("tvd_stretched_stiff", 2, dict(Optimize=False)),
     ("tvd_stretched_stiff", 0, dict(Optimize=False, ImpExp_AdvV=0.0)),
     ("upwind_flat", 5, dict(ImpExp_DifV=0.5, CellFluxes=True)),
-    ("tvd_sigma", 3, dict(Optimize=False, ImpExp_AdvV=0.0, CellFluxes=True))])
+    ("tvd_sigma", 3, dict(Optimize=False, ImpExp_AdvV=0.0, CellFluxes=True)),
+    # edge bands wider than half the domain: the i < nCells + 1 branch wins
+    # where both hold (AD:2021-2035)
+    ("upwind_land_ragged", 12, {})])
 def test_nudging_drop_in(name, cells, over):
     """Property 0 nudged, property 1 not and with NoAdvFlux (an upwind
     method keeps the coefficients the nudged call made, uncleared), the

@@ -315,3 +315,57 @@ def test_discharges_reject_bad_cells():
         gpu.set_discharges(d)
     assert ei.value.code == L.MGPU_ERR_BADARG
     gpu.kill()
+
+
+@pytest.mark.parametrize("bc,nudge", [(C.MASS_CONSERVATION, False), (C.NULL_GRADIENT, True)])
+def test_discharges_with_open_boundary_and_fluxes(bc, nudge):
+    """Discharges (in the explicit right-hand side, AD:1702), the open
+    boundary rows or the null-gradient pass, cell fluxes and, in one case,
+    nudging in the same calls, including discharges into boundary columns."""
+    from .test_gpu_boundaries import _boundary_points, _hydro
+    _, AD, _ = _mods()
+    case = make("tvd_land_sigma")
+    T = len(case.tracers)
+    rng = np.random.default_rng(23)
+    bp = _boundary_points(case, interior=2, seed=bc)
+    hy = _hydro(case, bp)
+    sets = _discharges(case, rng, T)
+    # one more point discharge into a boundary column's top cell
+    KUB = case.worksize[5]
+    bi, bj = (int(v[0]) for v in np.nonzero((bp == 1) & (hy["WaterPoints3D"][:, :, KUB] == 1)))
+    for t, d in enumerate(sets):
+        sets[t] = dict(d, flow=d["flow"] + [0.8], I=d["I"] + [bi], J=d["J"] + [bj],
+                       K=d["K"] + [KUB], Kmin=d["Kmin"] + [-9999999],
+                       Kmax=d["Kmax"] + [-9999999], conc=d["conc"] + [4.0],
+                       concMF=d["concMF"] + [1.0], vert=d["vert"] + [1],
+                       ignore=d["ignore"] + [0], ncells=d["ncells"] + [1],
+                       bypass=d["bypass"] + [0])
+    refs = [np.asfortranarray(rng.uniform(0.5, 30.0, size=case.shape)) for _ in range(T)]
+    base = dict(case.params, Optimize=False, BoundaryCondition=bc, DecayTime=1800.0,
+                CellFluxes=True, AdvectionNudging=nudge, AdvectionNudgingCells=3)
+    gpu = AD(case.size, case.worksize, case.geom)
+    inst = ora.Instance(case, nthreads=1)
+    pg = [np.array(p, order="F", copy=True) for p in case.tracers]
+    po = [np.array(p, order="F", copy=True) for p in case.tracers]
+    bitwise = True
+    names = ("AdvFluxX", "AdvFluxY", "AdvFluxZ", "DifFluxX", "DifFluxY", "DifFluxZ")
+    for s in range(3):
+        for t in range(T):
+            prm = dict(base) if bc == C.NULL_GRADIENT else dict(base, ReferenceProp=refs[t])
+            assert inst.set_discharges(sets[t]) == 0
+            assert inst.step(po[t], first_property=t == 0, now=(s + 1) * prm["DTProp"],
+                             hydro=hy, **prm) == 0
+            assert inst.unset_discharges() == 0
+            gpu.set_discharges(sets[t])
+            gpu(pg[t], hy, prm, first_property=t == 0)
+            gpu.unset_discharges()
+            err = _rel_err(pg[t], po[t])
+            assert err <= RTOL, (s, t, err)
+            bitwise &= bool(np.array_equal(pg[t], po[t]))
+            got = gpu.fluxes()
+            for n in names:
+                e = _rel_err(got[n], inst.internal(n))
+                assert e <= RTOL, (s, t, n, e)
+    gpu.kill()
+    inst.close()
+    print(f"discharges + boundary {bc} + fluxes, nudging={nudge}: bit-identical={bitwise}")

@@ -90,7 +90,9 @@ def main():
     ap.add_argument("--boundaries", action="store_true",
                     help="instead of the headline line, time the drop-in on 1024x1024x40 "
                          "(one tracer, device arrays) without and with each offloaded "
-                         "boundary condition; one JSON line")
+                         "boundary condition, then with each other optional branch (cell "
+                         "fluxes, NoDifFlux, NoAdvFlux, SmallDepths, nudging, discharges); "
+                         "two JSON lines")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 512 MiB three times (the "
                          "reference workload for scaling FETCH_SIZE/WRITE_SIZE counters)")
@@ -409,8 +411,49 @@ def boundary_costs(args):
             ad(prop, hy, prm)
         out[name] = median_ms(lambda: ad(prop, hy, prm))
         print(f"boundaries: {name} {out[name]:.3f} ms per call", file=sys.stderr, flush=True)
-    ad.kill()
     print(json.dumps(out), flush=True)
+
+    # the other optional branches, on the non-optimised path they belong to
+    g = torch.Generator(device="cuda").manual_seed(5)
+    flag = lambda: (torch.rand(case.tracers[0].shape, generator=g, device="cuda") < 0.3
+                    ).to(torch.int32).contiguous()
+    hf = dict(case.hydro, NoFluxU=flag(), NoFluxV=flag(), NoFluxW=flag())
+    hs = dict(case.hydro, SmallDepths=(torch.rand(wp.shape, generator=g, device="cuda") < 0.4
+                                       ).to(torch.int32).contiguous())
+    jj, ii = torch.nonzero(wp, as_tuple=True)
+    pick = torch.randperm(len(ii), generator=g, device="cuda")[:8].cpu()
+    nd = len(pick)
+    disch = dict(flow=[1.5] * nd, conc=[5.0] * nd, concMF=[1.0] * nd,
+                 I=[int(v) for v in ii.cpu()[pick]], J=[int(v) for v in jj.cpu()[pick]],
+                 K=[int(ws[5])] * nd, Kmin=[-9999999] * nd, Kmax=[-9999999] * nd,
+                 vert=[1] * nd, ignore=[0] * nd, ncells=[1] * nd, bypass=[0] * nd)
+    out2 = {"measure": "drop-in per call with the other optional branches, 1024x1024x40, "
+                       "1 tracer, device arrays, Optimize off (median of 10 calls)",
+            "unit": "ms_per_call", "discharges": nd}
+    base = dict(case.params, Optimize=False)
+    for name, h, extra in (("none", case.hydro, {}), ("CellFluxes", case.hydro,
+                                                      dict(CellFluxes=True)),
+                           ("NoDifFlux", hf, dict(NoDifFlux=True)),
+                           ("NoAdvFlux", hf, dict(NoAdvFlux=True)),
+                           ("SmallDepths", hs, {}),
+                           ("AdvectionNudging", case.hydro,
+                            dict(AdvectionNudging=True, AdvectionNudgingCells=10)),
+                           ("Discharges", case.hydro, {})):
+        prm = dict(base, **extra)
+        if name == "Discharges":
+            def call():
+                ad.set_discharges(disch)
+                ad(prop, h, prm)
+                ad.unset_discharges()
+        else:
+            def call():
+                ad(prop, h, prm)
+        for _ in range(2):
+            call()
+        out2[name] = median_ms(call)
+        print(f"options: {name} {out2[name]:.3f} ms per call", file=sys.stderr, flush=True)
+    ad.kill()
+    print(json.dumps(out2), flush=True)
 
 
 def suite(args):

@@ -133,6 +133,14 @@ typedef struct mgpu_ad mgpu_ad;
 int mgpu_ad_create(const mgpu_size3d *size, const mgpu_size3d *worksize,
                    int32_t docycle_method, int32_t vertical1d, int32_t xzflow,
                    const mgpu_geometry *geometry, mgpu_ad **out);
+/* The geometry of the following mgpu_ad_step calls.  The reference fetches
+ * DZZ, DWZ, AreaU and AreaV from ModuleGeometry inside every AdvectionDiffusion
+ * call (GetGeometryDistances / GetGeometryAreas, AD:1353-1401), and
+ * ModuleGeometry recomputes them every step as the free surface moves
+ * (ComputeVolumes / ComputeAreas, ModuleGeometry.F90:4108-4327): a caller whose
+ * vertical coordinate moves sets them before each step's first call.  The
+ * arrays are copied (device_pointers: device to device); the call returns
+ * once they are on the device. */
 int mgpu_ad_set_geometry(mgpu_ad *ad, const mgpu_geometry *geometry);
 
 /* AdvectionDiffusion (AD:1108-1509): advances PROP by one DTProp in place.
@@ -208,7 +216,10 @@ int mgpu_ad_zero_pivots(const mgpu_ad *ad, uint64_t *count);
  * (WaterPoints3D(i,j,WorkSize%KUB) == 1) of the tridiagonal system
  * D(k) x(k-1) + E(k) x(k) + F(k) x(k+1) = TI(k), rows WorkSize%KLB..KUB+1,
  * writing x into res.  dev = 1: all pointers are device pointers and the
- * call is asynchronous on `stream` (a hipStream_t, may be null). */
+ * work runs on `stream` (a hipStream_t, may be null).  The call returns once
+ * its work is done.  Its scratch (and, in host mode, the device copies of
+ * the arrays) is kept per device and reused: a caller solving one shape
+ * every step allocates device memory only on its first call. */
 int mgpu_thomasz(const mgpu_size3d *size, const mgpu_size3d *worksize,
                  const double *D, const double *E, const double *F,
                  const double *TI, const int32_t *waterpoints, double *res,
@@ -223,7 +234,7 @@ int mgpu_thomasz(const mgpu_size3d *size, const mgpu_size3d *worksize,
  * MGPU_ERR_PIVOT at a zero pivot (res is then undefined), MGPU_ERR_OPTION
  * for another (di, dj) (ERR01).  dev = 1: device pointers, work on `stream`
  * (a hipStream_t, may be null); the call waits for it to read the pivot
- * check. */
+ * check.  Scratch is kept per device and reused, as for mgpu_thomasz. */
 int mgpu_thomas3d(const mgpu_size3d *size, int32_t IJmin, int32_t IJmax, int32_t JImin,
                   int32_t JImax, int32_t Kmin, int32_t Kmax, int32_t di, int32_t dj,
                   const double *D, const double *E, const double *F, const double *TI,
@@ -255,6 +266,12 @@ int mgpu_transport_create(const mgpu_size3d *size, const mgpu_size3d *worksize,
                           mgpu_transport **out);
 /* Shared per-step inputs (the arrays of mgpu_ad_args; scalars from args). */
 int mgpu_transport_set_hydro(mgpu_transport *t, const mgpu_ad_args *args);
+/* The geometry of the following steps, as mgpu_ad_set_geometry (the
+ * reference's per-call GetGeometryDistances / GetGeometryAreas, AD:1353-1401,
+ * of a grid ModuleGeometry updates every step, ModuleGeometry.F90:4108-4327).
+ * Ordered on the handle's stream after the steps enqueued before it; the
+ * handle's captured step graphs stay valid (they read the same buffers). */
+int mgpu_transport_set_geometry(mgpu_transport *t, const mgpu_geometry *geometry);
 int mgpu_transport_set_tracer(mgpu_transport *t, int32_t index, const double *prop, int32_t dev);
 int mgpu_transport_get_tracer(mgpu_transport *t, int32_t index, double *prop, int32_t dev);
 /* Advance all tracers nsteps steps.  use_graph = 1 captures the step loop in

@@ -92,7 +92,8 @@ EXPORTS = (
     "mgpu_ad_zero_pivots", "mgpu_ad_set_discharges", "mgpu_ad_unset_discharges",
     "mgpu_ad_get_fluxes", "mgpu_waterflux_z", "mgpu_compute_volumes", "mgpu_compute_areas",
     "mgpu_thomasz", "mgpu_thomas3d", "mgpu_transport_status",
-    "mgpu_transport_create", "mgpu_transport_set_hydro", "mgpu_transport_set_tracer",
+    "mgpu_transport_create", "mgpu_transport_set_hydro", "mgpu_transport_set_geometry",
+    "mgpu_transport_set_tracer",
     "mgpu_transport_get_tracer", "mgpu_transport_run", "mgpu_transport_tracer_ptr",
     "mgpu_transport_set_lanes",
     "mgpu_transport_stream", "mgpu_transport_destroy", "mgpu_transport_timing",
@@ -143,6 +144,7 @@ def lib():
                                         ct.POINTER(TransportOpts), ct.POINTER(Geometry),
                                         ct.POINTER(vp)]
     L.mgpu_transport_set_hydro.argtypes = [vp, ct.POINTER(ADArgs)]
+    L.mgpu_transport_set_geometry.argtypes = [vp, ct.POINTER(Geometry)]
     L.mgpu_transport_set_tracer.argtypes = [vp, ct.c_int32, vp, ct.c_int32]
     L.mgpu_transport_get_tracer.argtypes = [vp, ct.c_int32, vp, ct.c_int32]
     L.mgpu_transport_run.argtypes = [vp, ct.c_int32, ct.c_int32]

@@ -165,6 +165,14 @@ class AdvectionDiffusion:
             addr = ct.c_void_p(prop.ctypes.data)
         L.check("AdvectionDiffusion", L.lib().mgpu_ad_step(self._h, ct.byref(args), addr))
 
+    def set_geometry(self, geometry: Dict) -> None:
+        """The geometry of the following calls (the reference fetches DZZ, DWZ,
+        AreaU and AreaV on every call, AD:1353-1401; ModuleGeometry updates
+        them every step as the free surface moves)."""
+        keep: list = []
+        g = _geometry(geometry, keep)
+        L.check("mgpu_ad_set_geometry", L.lib().mgpu_ad_set_geometry(self._h, ct.byref(g)))
+
     def set_discharges(self, d: Dict) -> None:
         """SetDischarges (AD:978-1034) from a dict of arrays: per discharge
         cell flow, conc, concMF, I, J, K, Kmin, Kmax; per discharge vert,
@@ -279,6 +287,13 @@ class Transport:
         args = _args(hydro, params, keep, dev, params.get("Optimize", False), True)
         L.check("mgpu_transport_set_hydro", L.lib().mgpu_transport_set_hydro(self._h,
                                                                              ct.byref(args)))
+
+    def set_geometry(self, geometry: Dict) -> None:
+        """The geometry of the following steps (as AdvectionDiffusion.set_geometry)."""
+        keep: list = []
+        g = _geometry(geometry, keep)
+        L.check("mgpu_transport_set_geometry",
+                L.lib().mgpu_transport_set_geometry(self._h, ct.byref(g)))
 
     def set_tracer(self, index: int, prop) -> None:
         keep: list = []

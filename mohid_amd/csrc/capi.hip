@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <utility>
 #include <vector>
@@ -365,6 +366,7 @@ StepScalars make_scalars(const Dims &d, const mgpu_ad_args *a, int vertical1d, i
 // `ptr`; a new set clears the previous cells first.
 struct DevDischarges {
     bool on = false;
+    unsigned version = 0;                    // changes with every set and unset
     int ncontrib = 0, ncell = 0, nptr = 0;   // contributions, discharge cells, distinct cells
     int cap_c = 0, cap_q = 0;
     int32_t *ptr = nullptr;                  // n3
@@ -471,6 +473,15 @@ struct mgpu_ad {
     bool have_nw = false;
     int last_adv_method = 0, last_adv_tvd = 0;
     double last_adv_dt = 0.0;
+    // packed masks, kstart and solve lists of an earlier call, reused by a
+    // later call of the same step (FirstProperty false, the same DTProp,
+    // the same mask arrays, discharge set and SmallDepths): WaterProperties
+    // passes one set of masks to every property of a step that shares its
+    // DTInterval (WP:14600-14680)
+    bool masks_ok = false, mask_hlines = false;
+    const void *mask_src[8] = {};
+    unsigned mask_disch = 0;
+    double mask_dt = 0.0;
     double last_dt = 0.0, last_schmidtH = 0.0;   // Set_Internal_State's memory (AD:5746)
     unsigned long long last_zero_pivots = 0;
 };
@@ -518,6 +529,7 @@ extern "C" int mgpu_ad_create(const mgpu_size3d *size, const mgpu_size3d *worksi
 
 extern "C" int mgpu_ad_set_geometry(mgpu_ad *ad, const mgpu_geometry *geometry) {
     if (!ad) return MGPU_OFF_ERR;
+    ad->masks_ok = false;          // KFloor enters the packed masks' checks
     return ad->geom.upload(ad->d, geometry, ad->stream);
 }
 
@@ -560,6 +572,7 @@ int disch_clear(mgpu_ad *ad) {
     DevDischarges &q = ad->disch;
     int rc = MGPU_SUCCESS;
     if (q.on && q.nptr > 0) rc = launch_disch_scatter(q.nptr, q.ucell, nullptr, q.ptr, ad->stream);
+    if (q.on) ++q.version;
     q.on = false;
     q.nptr = 0;
     return rc;
@@ -684,6 +697,7 @@ extern "C" int mgpu_ad_set_discharges(mgpu_ad *ad, const double *DischFlow,
     q.ncell = nq;
     q.nptr = (int)ucell.size();
     q.on = true;
+    ++q.version;
     return MGPU_SUCCESS;
 }
 
@@ -813,9 +827,25 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
                                ad->disch.flow, s);
         if (rc) return rc;
     }
-    rc = ad->der.pack(d, ad->hydro, ad->geom.KFloor, ad->docycle, s, false, sc.impH >= 0,
-                      ad->disch.on ? ad->disch.ptr : nullptr);
-    if (rc) return rc;
+    {
+        const void *src[8] = {a->OpenPoints3D, a->LandPoints3D, a->ComputeFacesU3D,
+                              a->ComputeFacesV3D, a->ComputeFacesW3D, a->WaterPoints3D,
+                              a->SmallDepths, (const void *)(intptr_t)a->device_pointers};
+        bool same = ad->masks_ok && !a->FirstProperty && a->DTProp == ad->mask_dt &&
+                    ad->mask_disch == ad->disch.version && (sc.impH < 0 || ad->mask_hlines);
+        for (int q = 0; q < 8; ++q) same = same && src[q] == ad->mask_src[q];
+        if (!same) {
+            ad->masks_ok = false;
+            rc = ad->der.pack(d, ad->hydro, ad->geom.KFloor, ad->docycle, s, false,
+                              sc.impH >= 0, ad->disch.on ? ad->disch.ptr : nullptr);
+            if (rc) return rc;
+            for (int q = 0; q < 8; ++q) ad->mask_src[q] = src[q];
+            ad->mask_dt = a->DTProp;
+            ad->mask_disch = ad->disch.version;
+            ad->mask_hlines = sc.impH >= 0;
+            ad->masks_ok = true;
+        }
+    }
     {
         // Convert_Visc_Dif_Horizontal(_opt) + Convert_Dif_Vertical (+ the
         // Optimize path's constants, AD:1421-1478); NullDif and NoDifFlux only
@@ -993,6 +1023,8 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     HIPCHK(hipMemcpyAsync(cnt, ad->der.counters, sizeof(cnt), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     ad->last_zero_pivots = cnt[0];
+    const unsigned int mflags = (unsigned int)(cnt[2] & 0xffffffffu);
+    if (mflags) ad->masks_ok = false;             // report them again on the next call
     if (cnt[3]) return MGPU_ERR_PIVOT;            // THOMAS_3D ERR10: PROP not written
     if (bflag) return MGPU_ERR_BADARG;            // a cyclic copy from below the Size
     ad->have_flux = cfl;
@@ -1002,12 +1034,60 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
         HIPCHK(hipStreamSynchronize(s));
     }
     if (cnt[1]) return MGPU_ERR_OPTION;          // method/limiter the reference stops on
-    const unsigned int mflags = (unsigned int)(cnt[2] & 0xffffffffu);
     if (mflags) return MGPU_ERR_MASKS;
     return MGPU_SUCCESS;
 }
 
 // ---------------------------------------------------------------------------
+// Scratch of the stand-alone solvers (mgpu_thomasz, mgpu_thomas3d), kept per
+// device and grown on demand, so that a caller solving the same shape every
+// step allocates nothing after its first call.  The calls finish their work
+// before they return and hold the lock meanwhile, so concurrent callers on
+// one device take turns.
+namespace {
+struct SolverScratch {
+    double *W = nullptr;            // forward-sweep W, Size-shaped
+    double *st[5] = {};             // host mode: D, E, F, TI, res on the device
+    int32_t *wp = nullptr;          // host mode: WaterPoints3D on the device
+    unsigned long long *piv = nullptr;
+    long cap = 0;                   // cells each array holds
+};
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scr_mu;
+SolverScratch g_scr[MAX_DEVICES];
+
+// The calling device's scratch with room for n cells (host: also the
+// staging arrays), or null (rc set).
+SolverScratch *solver_scratch(long n, bool host, int &rc) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) { mgpu_set_error("hipGetDevice", e); rc = MGPU_ERR_HIP; return nullptr; }
+    if (dev < 0 || dev >= MAX_DEVICES) { rc = MGPU_ERR_BADARG; return nullptr; }
+    SolverScratch &q = g_scr[dev];
+    if (q.cap < n) {
+        dfree(q.W);
+        for (double *&a : q.st) dfree(a);
+        dfree(q.wp);
+        q.cap = 0;
+        if (dalloc(&q.W, n)) { rc = MGPU_ERR_NOMEM; return nullptr; }
+        q.cap = n;
+    }
+    if (host && !q.st[0]) {
+        int r = 0;
+        for (double *&a : q.st) r |= dalloc(&a, q.cap);
+        r |= dalloc(&q.wp, q.cap);
+        if (r) {
+            for (double *&a : q.st) dfree(a);
+            dfree(q.wp);
+            rc = MGPU_ERR_NOMEM;
+            return nullptr;
+        }
+    }
+    if (!q.piv && dalloc(&q.piv, 1)) { rc = MGPU_ERR_NOMEM; return nullptr; }
+    return &q;
+}
+}  // namespace
+
 extern "C" int mgpu_thomasz(const mgpu_size3d *size, const mgpu_size3d *worksize,
                             const double *D, const double *E, const double *F, const double *TI,
                             const int32_t *waterpoints, double *res, int32_t dev, void *stream) {
@@ -1018,40 +1098,29 @@ extern "C" int mgpu_thomasz(const mgpu_size3d *size, const mgpu_size3d *worksize
     const long n3 = d.n3();
     const size_t b3 = sizeof(double) * (size_t)n3;
     hipStream_t s = (hipStream_t)stream;
-    double *W = nullptr;
-    if ((rc = dalloc(&W, n3))) return rc;
+    std::lock_guard<std::mutex> lock(g_scr_mu);
+    SolverScratch *q = solver_scratch(n3, !dev, rc);
+    if (!q) return rc;
+    hipError_t e = hipSuccess;
     if (dev) {
-        rc = launch_thomasz(d, D, E, F, TI, waterpoints, res, W, s);
-        if (!rc) {
-            hipError_t e = hipStreamSynchronize(s);
-            if (e != hipSuccess) { mgpu_set_error("hipStreamSynchronize", e); rc = MGPU_ERR_HIP; }
-        }
-        dfree(W);
-        return rc;
-    }
-    double *dD = nullptr, *dE = nullptr, *dF = nullptr, *dT = nullptr, *dR = nullptr;
-    int32_t *dW = nullptr;
-    rc |= dalloc(&dD, n3); rc |= dalloc(&dE, n3); rc |= dalloc(&dF, n3);
-    rc |= dalloc(&dT, n3); rc |= dalloc(&dR, n3); rc |= dalloc(&dW, n3);
-    if (!rc) {
-        hipError_t e = hipSuccess;
-        if (e == hipSuccess) e = hipMemcpy(dD, D, b3, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(dE, E, b3, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(dF, F, b3, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(dT, TI, b3, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(dR, res, b3, hipMemcpyHostToDevice);
-        if (e == hipSuccess)
-            e = hipMemcpy(dW, waterpoints, sizeof(int32_t) * (size_t)n3, hipMemcpyHostToDevice);
-        if (e != hipSuccess) { mgpu_set_error("hipMemcpy", e); rc = MGPU_ERR_HIP; }
-        if (!rc) rc = launch_thomasz(d, dD, dE, dF, dT, dW, dR, W, nullptr);
-        if (!rc) {
-            e = hipMemcpy(res, dR, b3, hipMemcpyDeviceToHost);
-            if (e != hipSuccess) { mgpu_set_error("hipMemcpy", e); rc = MGPU_ERR_HIP; }
-        }
+        rc = launch_thomasz(d, D, E, F, TI, waterpoints, res, q->W, s);
     } else {
-        rc = MGPU_ERR_NOMEM;
+        double *dD = q->st[0], *dE = q->st[1], *dF = q->st[2], *dT = q->st[3], *dR = q->st[4];
+        if (e == hipSuccess) e = hipMemcpyAsync(dD, D, b3, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(dE, E, b3, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(dF, F, b3, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(dT, TI, b3, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(dR, res, b3, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(q->wp, waterpoints, sizeof(int32_t) * (size_t)n3,
+                               hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) { mgpu_set_error("hipMemcpyAsync", e); return MGPU_ERR_HIP; }
+        rc = launch_thomasz(d, dD, dE, dF, dT, q->wp, dR, q->W, s);
+        if (!rc) e = hipMemcpyAsync(res, dR, b3, hipMemcpyDeviceToHost, s);
     }
-    dfree(dD); dfree(dE); dfree(dF); dfree(dT); dfree(dR); dfree(dW); dfree(W);
+    // the scratch is shared: the work is done before the lock is released
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) { mgpu_set_error("hipStreamSynchronize", e); rc = MGPU_ERR_HIP; }
     return rc;
 }
 
@@ -1076,41 +1145,31 @@ extern "C" int mgpu_thomas3d(const mgpu_size3d *size, int32_t IJmin, int32_t IJm
     const size_t b3 = sizeof(double) * (size_t)n3;
     hipStream_t s = (hipStream_t)stream;
     int rc = 0;
-    double *W = nullptr;
-    unsigned long long *piv = nullptr;
-    if ((rc = dalloc(&W, n3))) return rc;
-    if ((rc = dalloc(&piv, 1))) { dfree(W); return rc; }
-    double *dD = nullptr, *dE = nullptr, *dF = nullptr, *dT = nullptr, *dR = nullptr;
-    if (!dev) {
-        rc |= dalloc(&dD, n3); rc |= dalloc(&dE, n3); rc |= dalloc(&dF, n3);
-        rc |= dalloc(&dT, n3); rc |= dalloc(&dR, n3);
-    }
+    std::lock_guard<std::mutex> lock(g_scr_mu);
+    SolverScratch *q = solver_scratch(n3, !dev, rc);
+    if (!q) return rc;
+    double *dD = q->st[0], *dE = q->st[1], *dF = q->st[2], *dT = q->st[3], *dR = q->st[4];
     unsigned long long npiv = 0;
-    if (!rc) {
-        hipError_t e = hipMemsetAsync(piv, 0, sizeof(*piv), s);
-        if (!dev) {
-            if (e == hipSuccess) e = hipMemcpyAsync(dD, D, b3, hipMemcpyHostToDevice, s);
-            if (e == hipSuccess) e = hipMemcpyAsync(dE, E, b3, hipMemcpyHostToDevice, s);
-            if (e == hipSuccess) e = hipMemcpyAsync(dF, F, b3, hipMemcpyHostToDevice, s);
-            if (e == hipSuccess) e = hipMemcpyAsync(dT, TI, b3, hipMemcpyHostToDevice, s);
-            if (e == hipSuccess) e = hipMemcpyAsync(dR, res, b3, hipMemcpyHostToDevice, s);
-        }
-        if (e != hipSuccess) { mgpu_set_error("hipMemcpyAsync", e); rc = MGPU_ERR_HIP; }
-        if (!rc)
-            rc = dev ? launch_thomas3d(z, IJmin, IJmax, JImin, JImax, Kmin, Kmax, di, D, E, F, TI,
-                                       res, W, piv, s)
-                     : launch_thomas3d(z, IJmin, IJmax, JImin, JImax, Kmin, Kmax, di, dD, dE, dF,
-                                       dT, dR, W, piv, s);
-        if (!rc) {
-            e = hipMemcpyAsync(&npiv, piv, sizeof(npiv), hipMemcpyDeviceToHost, s);
-            if (e == hipSuccess && !dev) e = hipMemcpyAsync(res, dR, b3, hipMemcpyDeviceToHost, s);
-            if (e == hipSuccess) e = hipStreamSynchronize(s);
-            if (e != hipSuccess) { mgpu_set_error("hipMemcpyAsync", e); rc = MGPU_ERR_HIP; }
-        }
-    } else {
-        rc = MGPU_ERR_NOMEM;
+    hipError_t e = hipMemsetAsync(q->piv, 0, sizeof(*q->piv), s);
+    if (!dev) {
+        if (e == hipSuccess) e = hipMemcpyAsync(dD, D, b3, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(dE, E, b3, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(dF, F, b3, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(dT, TI, b3, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(dR, res, b3, hipMemcpyHostToDevice, s);
     }
-    dfree(dD); dfree(dE); dfree(dF); dfree(dT); dfree(dR); dfree(W); dfree(piv);
+    if (e != hipSuccess) { mgpu_set_error("hipMemcpyAsync", e); return MGPU_ERR_HIP; }
+    rc = dev ? launch_thomas3d(z, IJmin, IJmax, JImin, JImax, Kmin, Kmax, di, D, E, F, TI, res,
+                               q->W, q->piv, s)
+             : launch_thomas3d(z, IJmin, IJmax, JImin, JImax, Kmin, Kmax, di, dD, dE, dF, dT, dR,
+                               q->W, q->piv, s);
+    if (!rc) {
+        e = hipMemcpyAsync(&npiv, q->piv, sizeof(npiv), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess && !dev) e = hipMemcpyAsync(res, dR, b3, hipMemcpyDeviceToHost, s);
+    }
+    // the scratch is shared: the work is done before the lock is released
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) { mgpu_set_error("hipMemcpyAsync", e); rc = MGPU_ERR_HIP; }
     if (!rc && npiv) rc = MGPU_ERR_PIVOT;
     return rc;
 }
@@ -1302,6 +1361,13 @@ extern "C" int mgpu_transport_set_hydro(mgpu_transport *t, const mgpu_ad_args *a
     drop_graphs(t);  // step scalars are baked into captured launches
     t->have_hydro = true;
     return MGPU_SUCCESS;
+}
+
+extern "C" int mgpu_transport_set_geometry(mgpu_transport *t, const mgpu_geometry *geometry) {
+    if (!t) return MGPU_OFF_ERR;
+    // same buffers: captured step graphs keep reading them; the copies are
+    // ordered after the steps already enqueued on the handle's stream
+    return t->geom.upload(t->d, geometry, t->stream);
 }
 
 extern "C" int mgpu_transport_set_tracer(mgpu_transport *t, int32_t index, const double *prop,

@@ -58,6 +58,10 @@ class CaseConfig:
     seed: int = 20240901
     modes: int = 8
     optimize: Optional[bool] = None  # None -> WP:14576-14598 rule
+    # "uniform": DUX = DZX = dx, DVY = DZY = dy everywhere; "stretched": a
+    # non-orthogonal stretched grid whose cell widths vary along both axes
+    # (stretched_corners), metrics from its corners as HorizontalGrid does
+    metrics: str = "uniform"
 
 
 CONFIGS: Dict[str, CaseConfig] = {
@@ -152,6 +156,48 @@ def _value_noise(rng: np.random.Generator, ni: int, nj: int, octaves: int = 5) -
         out += amp * ((g00 * (1 - fx) + g01 * fx) * (1 - fy) + (g10 * (1 - fx) + g11 * fx) * fy)
         amp *= 0.5
     return out
+
+
+def stretched_corners(cfg: CaseConfig, nI: int, nJ: int):
+    """Corner coordinates XX_IE, YY_IE (nI+1, nJ+1) of a stretched,
+    slightly skewed grid over Size: corner (ic, jc) is the lower-left corner
+    of cell (ILB+ic, JLB+jc).  Column widths vary along j between about
+    0.6 and 1.4 dx, row heights along i between 0.7 and 1.3 dy, and each
+    coordinate also grows a few per cent along the other axis, so every
+    metric differs from its neighbours in both directions."""
+    jc = np.arange(nJ + 1, dtype=np.float64)
+    ic = np.arange(nI + 1, dtype=np.float64)
+    wx = cfg.dx * (1.0 + 0.4 * np.sin(2.0 * np.pi * 1.7 * np.arange(nJ) / nJ + 0.3))
+    hy = cfg.dy * (1.0 + 0.3 * np.cos(2.0 * np.pi * 1.3 * np.arange(nI) / nI + 0.7))
+    X = np.concatenate(([0.0], np.cumsum(wx)))
+    Y = np.concatenate(([0.0], np.cumsum(hy)))
+    XX = np.asfortranarray(np.outer(1.0 + 0.05 * ic / nI, X))
+    YY = np.asfortranarray(np.outer(Y, 1.0 + 0.04 * jc / nJ))
+    return XX, YY
+
+
+def grid_metrics(XX: np.ndarray, YY: np.ndarray):
+    """DXX, DYY, DUX, DVY, DZX, DZY over Size (nI, nJ) from corner
+    coordinates (nI+1, nJ+1), the formulas of a grid without rotation in
+    Software/MOHIDBase2/ModuleHorizontalGrid.F90:6283-6341:
+    DXX(i,j) = XX(i,j+1) - XX(i,j), DYY(i,j) = YY(i+1,j) - YY(i,j),
+    DUX = mean of DXX(i,j), DXX(i+1,j) (the cell's width), DVY = mean of
+    DYY(i,j), DYY(i,j+1) (its height), DZX(i,j) = mean of DUX(i,j),
+    DUX(i,j+1) (centre distance to (i,j+1)), DZY(i,j) = mean of DVY(i,j),
+    DVY(i+1,j).  DZX(:,JUB) and DZY(IUB,:), which the reference leaves
+    unset, take the cell's own width or height."""
+    nI, nJ = XX.shape[0] - 1, XX.shape[1] - 1
+    DXXf = XX[:, 1:] - XX[:, :-1]          # (nI+1, nJ)
+    DYYf = YY[1:, :] - YY[:-1, :]          # (nI, nJ+1)
+    DUX = np.asfortranarray((DXXf[:-1, :] + DXXf[1:, :]) / 2.0)
+    DVY = np.asfortranarray((DYYf[:, :-1] + DYYf[:, 1:]) / 2.0)
+    DZX = DUX.copy(order="F")
+    DZX[:, :-1] = (DUX[:, :-1] + DUX[:, 1:]) / 2.0
+    DZY = DVY.copy(order="F")
+    DZY[:-1, :] = (DVY[:-1, :] + DVY[1:, :]) / 2.0
+    DXX = np.asfortranarray(DXXf[:nI, :])
+    DYY = np.asfortranarray(DYYf[:, :nJ])
+    return DXX, DYY, DUX, DVY, DZX, DZY
 
 
 def _arr3(shape, dtype=np.float64, fill=0.0):
@@ -269,13 +315,17 @@ def make_case(cfg: CaseConfig, device=None) -> Case:
         layer = np.full((nI, nJ), cfg.dz) if frac is None else h * frac[k - 1]
         return np.where(water_k(k), layer, 0.0)
 
-    # ---- horizontal metrics (uniform grid) ---------------------------------
-    DUX = np.full((nI, nJ), cfg.dx, order="F")
-    DVY = np.full((nI, nJ), cfg.dy, order="F")
-    DZX = np.full((nI, nJ), cfg.dx, order="F")
-    DZY = np.full((nI, nJ), cfg.dy, order="F")
-    DXX = DUX
-    DYY = DVY
+    # ---- horizontal metrics ----------------------------------------------------
+    if cfg.metrics == "uniform":
+        DUX = np.full((nI, nJ), cfg.dx, order="F")
+        DVY = np.full((nI, nJ), cfg.dy, order="F")
+        DZX = np.full((nI, nJ), cfg.dx, order="F")
+        DZY = np.full((nI, nJ), cfg.dy, order="F")
+        DXX = DUX
+        DYY = DVY
+    else:
+        XX, YY = stretched_corners(cfg, nI, nJ)
+        DXX, DYY, DUX, DVY, DZX, DZY = grid_metrics(XX, YY)
 
     # ---- 2-D face structure -----------------------------------------------------
     # U face j between (i, j-1) and (i, j), j in WorkSize
@@ -435,7 +485,8 @@ def make_case(cfg: CaseConfig, device=None) -> Case:
         ImpExp_DifH=0.0, NullDif=False, Optimize=bool(optimize),
     )
     geom = dict(DUX=DUX, DVY=DVY, DZX=DZX, DZY=DZY, KFloorZ=KF,
-                AreaU=A["AreaU"], AreaV=A["AreaV"], DZZ=A["DZZ"], DWZ=A["DWZ"])
+                AreaU=A["AreaU"], AreaV=A["AreaV"], DZZ=A["DZZ"], DWZ=A["DWZ"],
+                DXX=np.asfortranarray(DXX), DYY=np.asfortranarray(DYY))
     hydro = {n: A[n] for n in ("Wflux_X", "Wflux_Y", "Wflux_Z", "VolumeZOld", "VolumeZ",
                                "OpenPoints3D", "LandPoints3D", "ComputeFacesU3D",
                                "ComputeFacesV3D", "ComputeFacesW3D", "WaterPoints3D",
@@ -445,3 +496,57 @@ def make_case(cfg: CaseConfig, device=None) -> Case:
     case.n_wet = n_wet
     case.wet_columns = int(wet.sum())
     return case
+
+
+def geometry_from_dwz(case: Case, dwz: np.ndarray):
+    """DZZ, AreaU, AreaV and VolumeZ of a layer-thickness field DWZ (numpy,
+    over Size) on the case's grid, with the formulas make_case uses
+    (ModuleGeometry.F90:3798-4327: DZZ the centre distance to the layer above,
+    DUZ/DVZ the DUX/DVY-weighted face thickness, AreaU = DUZ DYY,
+    AreaV = DVZ DXX, VolumeZ = DWZ DUX DVY on water cells)."""
+    g = case.geom
+    wp = np.asarray(case.hydro["WaterPoints3D"]) == 1
+    DUX, DVY, DXX, DYY = (g[n][:, :, None] for n in ("DUX", "DVY", "DXX", "DYY"))
+    up = np.zeros_like(dwz)
+    up[:, :, :-1] = dwz[:, :, 1:]
+    dzz = np.where(wp, (up + dwz) / 2.0, 0.0)
+    au = np.zeros_like(dwz)
+    wU = wp[:, :-1, :] & wp[:, 1:, :]
+    duz = (dwz[:, 1:, :] * DUX[:, :-1, :] + dwz[:, :-1, :] * DUX[:, 1:, :]) / \
+        (DUX[:, :-1, :] + DUX[:, 1:, :])
+    au[:, 1:, :] = np.where(wU, duz * DYY[:, 1:, :], 0.0)
+    av = np.zeros_like(dwz)
+    wV = wp[:-1, :, :] & wp[1:, :, :]
+    dvz = (dwz[1:, :, :] * DVY[:-1, :, :] + dwz[:-1, :, :] * DVY[1:, :, :]) / \
+        (DVY[:-1, :, :] + DVY[1:, :, :])
+    av[1:, :, :] = np.where(wV, dvz * DXX[1:, :, :], 0.0)
+    vol = np.where(wp, DUX * DVY * dwz, np.asarray(case.hydro["VolumeZ"]))
+    f = lambda a: np.asfortranarray(a)     # noqa: E731
+    return f(dzz), f(au), f(av), f(vol)
+
+
+def moving_surface(case: Case, steps: int, amplitude: float = 0.08, seed: int = 5):
+    """Geometry and volumes of `steps` successive steps under a moving free
+    surface (host arrays): the surface layer of every water column thickens
+    by amplitude * (its thickness) * a smooth seeded pattern that changes
+    every step, and ModuleGeometry's DZZ, AreaU, AreaV and VolumeZ follow
+    (geometry_from_dwz).  Step s's VolumeZOld is step s-1's VolumeZ.
+    Returns a list of (geom, hydro) dicts, the case's own arrays replaced."""
+    rng = np.random.default_rng(seed)
+    ni, nj = case.cfg.ni, case.cfg.nj
+    KUB = case.worksize[5]
+    dwz0 = np.asarray(case.geom["DWZ"])
+    vold = np.array(case.hydro["VolumeZ"], order="F", copy=True)
+    out = []
+    for _ in range(steps):
+        pat = np.zeros(dwz0.shape[:2])
+        pat[1:-1, 1:-1] = _smooth_field(rng, ni, nj, 4)
+        pat /= max(np.abs(pat).max(), 1e-30)
+        dwz = np.array(dwz0, order="F", copy=True)
+        dwz[:, :, KUB] = dwz0[:, :, KUB] * (1.0 + amplitude * pat)
+        dzz, au, av, vol = geometry_from_dwz(case, dwz)
+        geom = dict(case.geom, DWZ=dwz, DZZ=dzz, AreaU=au, AreaV=av)
+        hydro = dict(case.hydro, VolumeZ=vol, VolumeZOld=vold)
+        out.append((geom, hydro))
+        vold = vol
+    return out

@@ -36,6 +36,11 @@ module ModuleAdvectionDiffusionGPU
     public :: ComputeAreasGPU
     public :: SetDischargesGPU, UnSetDischargesGPU
     public :: GetFluxesGPU
+    public :: SetGeometryGPU
+    public :: TransportStartGPU, TransportSetGeometryGPU, TransportSetHydroGPU
+    public :: TransportSetPropertyGPU, TransportGetPropertyGPU, TransportRunGPU
+    public :: TransportKillGPU
+    public :: MGPU_PREC_FP64, MGPU_PREC_FAST, MGPU_PREC_MIXED
     public :: MGPU_SUCCESS, MGPU_OFF_ERR, MGPU_ERR_OPTION, MGPU_ERR_UNSUPPORTED
     public :: MGPU_ERR_PIVOT, MGPU_ERR_BADARG
 
@@ -45,6 +50,12 @@ module ModuleAdvectionDiffusionGPU
     integer, parameter :: MGPU_ERR_OPTION      = 102
     integer, parameter :: MGPU_ERR_UNSUPPORTED = 103
     integer, parameter :: MGPU_ERR_PIVOT       = 107
+    integer, parameter :: MGPU_ERR_STATE       = 105
+
+    ! arithmetic of the batched transport (include/mohid_gpu.h)
+    integer, parameter :: MGPU_PREC_FP64  = 0
+    integer, parameter :: MGPU_PREC_FAST  = 1
+    integer, parameter :: MGPU_PREC_MIXED = 2
 
     ! AdvectionDiffusion boundary-condition ids (ModuleAdvectionDiffusion.F90:152-158)
     integer, parameter :: NullGradient_   = 4
@@ -60,6 +71,12 @@ module ModuleAdvectionDiffusionGPU
         type(c_ptr)        :: AreaU, AreaV, DZZ, DWZ
         integer(c_int32_t) :: device_pointers
     end type t_mgpu_geometry
+
+    type, bind(c) :: t_mgpu_transport_opts
+        integer(c_int32_t) :: ntracers, precision, docycle_method, kernel
+        integer(c_int32_t) :: tracers_per_launch, explicit_tracers_per_lane
+        integer(c_int32_t) :: column_tracers_per_lane
+    end type t_mgpu_transport_opts
 
     type, bind(c) :: t_mgpu_ad_args
         real(c_double)     :: schmidt_H, SchmidtCoef_V, SchmidtBackground_V
@@ -99,6 +116,69 @@ module ModuleAdvectionDiffusionGPU
             type(t_mgpu_geometry), intent(in)  :: geometry
             type(c_ptr),           intent(out) :: handle
         end function mgpu_ad_create
+
+        integer(c_int) function mgpu_ad_set_geometry(handle, geometry)                      &
+                       bind(c, name="mgpu_ad_set_geometry")
+            import :: c_int, c_ptr, t_mgpu_geometry
+            type(c_ptr),           value      :: handle
+            type(t_mgpu_geometry), intent(in) :: geometry
+        end function mgpu_ad_set_geometry
+
+        integer(c_int) function mgpu_transport_create(size, worksize, opts, geometry, handle) &
+                       bind(c, name="mgpu_transport_create")
+            import :: c_int, c_ptr, t_mgpu_size3d, t_mgpu_transport_opts, t_mgpu_geometry
+            type(t_mgpu_size3d),         intent(in)  :: size, worksize
+            type(t_mgpu_transport_opts), intent(in)  :: opts
+            type(t_mgpu_geometry),       intent(in)  :: geometry
+            type(c_ptr),                 intent(out) :: handle
+        end function mgpu_transport_create
+
+        integer(c_int) function mgpu_transport_set_geometry(handle, geometry)                &
+                       bind(c, name="mgpu_transport_set_geometry")
+            import :: c_int, c_ptr, t_mgpu_geometry
+            type(c_ptr),           value      :: handle
+            type(t_mgpu_geometry), intent(in) :: geometry
+        end function mgpu_transport_set_geometry
+
+        integer(c_int) function mgpu_transport_set_hydro(handle, args)                       &
+                       bind(c, name="mgpu_transport_set_hydro")
+            import :: c_int, c_ptr, t_mgpu_ad_args
+            type(c_ptr),          value      :: handle
+            type(t_mgpu_ad_args), intent(in) :: args
+        end function mgpu_transport_set_hydro
+
+        integer(c_int) function mgpu_transport_set_tracer(handle, index, prop, dev)          &
+                       bind(c, name="mgpu_transport_set_tracer")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr),        value :: handle, prop
+            integer(c_int32_t), value :: index, dev
+        end function mgpu_transport_set_tracer
+
+        integer(c_int) function mgpu_transport_get_tracer(handle, index, prop, dev)          &
+                       bind(c, name="mgpu_transport_get_tracer")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr),        value :: handle, prop
+            integer(c_int32_t), value :: index, dev
+        end function mgpu_transport_get_tracer
+
+        integer(c_int) function mgpu_transport_run(handle, nsteps, use_graph)                &
+                       bind(c, name="mgpu_transport_run")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr),        value :: handle
+            integer(c_int32_t), value :: nsteps, use_graph
+        end function mgpu_transport_run
+
+        integer(c_int) function mgpu_transport_status(handle)                                &
+                       bind(c, name="mgpu_transport_status")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle
+        end function mgpu_transport_status
+
+        integer(c_int) function mgpu_transport_destroy(handle)                               &
+                       bind(c, name="mgpu_transport_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle
+        end function mgpu_transport_destroy
 
         integer(c_int) function mgpu_ad_step(handle, args, prop) bind(c, name="mgpu_ad_step")
             import :: c_int, c_ptr, t_mgpu_ad_args
@@ -189,6 +269,8 @@ module ModuleAdvectionDiffusionGPU
     ! RegisterNewInstance), each holding one library handle
     integer, parameter :: MAX_INSTANCES = 64
     type(c_ptr), dimension(MAX_INSTANCES), save :: Handles = c_null_ptr
+    ! batched transports (the tracer loop of Advection_Diffusion_Processes)
+    type(c_ptr), dimension(MAX_INSTANCES), save :: THandles = c_null_ptr
 
 contains
 
@@ -250,9 +332,51 @@ contains
         if (present(STAT)) STAT = rc
     end subroutine StartAdvectionDiffusionGPU
 
+    function geometry_of(DUX, DVY, DZX, DZY, KFloorZ, AreaU, AreaV, DZZ, DWZ) result(g)
+        real(8), dimension(:,:),   target, intent(in) :: DUX, DVY, DZX, DZY
+        integer, dimension(:,:),   target, intent(in) :: KFloorZ
+        real(8), dimension(:,:,:), contiguous, target, intent(in) :: AreaU, AreaV, DZZ, DWZ
+        type(t_mgpu_geometry) :: g
+        g%DUX = c_loc(DUX); g%DVY = c_loc(DVY); g%DZX = c_loc(DZX); g%DZY = c_loc(DZY)
+        g%KFloorZ = c_loc(KFloorZ)
+        g%AreaU = c_loc(AreaU); g%AreaV = c_loc(AreaV); g%DZZ = c_loc(DZZ); g%DWZ = c_loc(DWZ)
+        g%device_pointers = 0
+    end function geometry_of
+
+    !> The geometry of the following AdvectionDiffusionGPU calls.  The
+    !> reference fetches DZZ, DWZ, AreaU and AreaV from ModuleGeometry inside
+    !> every call (GetGeometryDistances / GetGeometryAreas,
+    !> ModuleAdvectionDiffusion.F90:1353-1401), and ModuleGeometry recomputes
+    !> them every step as the free surface moves (ComputeVolumes /
+    !> ComputeAreas, ModuleGeometry.F90:4108-4327); call this once per step,
+    !> before the step's first AdvectionDiffusionGPU call, with the arrays
+    !> StartAdvectionDiffusionGPU took.
+    subroutine SetGeometryGPU(ID, DUX, DVY, DZX, DZY, KFloorZ, AreaU, AreaV, DZZ, DWZ, STAT)
+        integer,                           intent(in)  :: ID
+        real(8), dimension(:,:),   target, intent(in)  :: DUX, DVY, DZX, DZY
+        integer, dimension(:,:),   target, intent(in)  :: KFloorZ
+        real(8), dimension(:,:,:), contiguous, target, intent(in)  :: AreaU, AreaV, DZZ, DWZ
+        integer, optional,                 intent(out) :: STAT
+        type(t_mgpu_geometry) :: g
+        integer               :: rc
+        if (ID < 1 .or. ID > MAX_INSTANCES) then
+            if (present(STAT)) STAT = MGPU_OFF_ERR
+            return
+        endif
+        g = geometry_of(DUX, DVY, DZX, DZY, KFloorZ, AreaU, AreaV, DZZ, DWZ)
+        rc = mgpu_ad_set_geometry(Handles(ID), g)
+        if (present(STAT)) STAT = rc
+    end subroutine SetGeometryGPU
+
     !> AdvectionDiffusion (ModuleAdvectionDiffusion.F90:1108-1198): PROP is
     !> advanced by DTProp in place.  Arguments keep the reference's names,
-    !> order and meaning.  Options the GPU path does not offload make
+    !> order and meaning, so WaterProperties' call (WP:14776-14822) compiles
+    !> unchanged against it once renamed and given BoundaryPoints2D, the one
+    !> argument added (after STAT): the array the reference fetches from
+    !> HorizontalMap inside the call (GetBoundaries, AD:1373).
+    !> NumericStability is accepted and ignored: the reference only stores it
+    !> (AD:1265-1268).  An absent FirstProperty counts as .true. (nothing is
+    !> kept from an earlier call).  Options the GPU path does not offload make
     !> STAT = MGPU_ERR_UNSUPPORTED and leave PROP untouched:
     !>   CellFluxes with implicit horizontal advection, an open boundary with
     !>   one layer and implicit horizontal advection, the Orlanski_ open
@@ -276,11 +400,11 @@ contains
                                      Wflux_X, Wflux_Y, Wflux_Z, VolumeZOld, VolumeZ,          &
                                      OpenPoints3D, LandPoints3D, ComputeFacesU3D,             &
                                      ComputeFacesV3D, ComputeFacesW3D, Visc_H, Diff_V,        &
-                                     WaterPoints3D, CellFluxes, ReferenceProp,                &
-                                     BoundaryCondition, DecayTime, BoundaryPoints2D, PROPOld, &
+                                     CellFluxes, WaterPoints3D, ReferenceProp,                &
+                                     BoundaryCondition, DecayTime, NumericStability, PROPOld, &
                                      SmallDepths, NoAdvFlux, NoDifFlux,                       &
                                      NoFluxU, NoFluxV, NoFluxW,                               &
-                                     Optimize, FirstProperty, STAT)
+                                     Optimize, FirstProperty, STAT, BoundaryPoints2D)
         integer,                           intent(in)    :: ID
         real(8), dimension(:,:,:), contiguous, target, intent(inout) :: PROP
         real(8),                           intent(in)    :: schmidt_H, SchmidtCoef_V
@@ -302,8 +426,9 @@ contains
         integer, dimension(:,:,:), contiguous, target, intent(in)    :: ComputeFacesU3D, ComputeFacesV3D
         integer, dimension(:,:,:), contiguous, target, intent(in)    :: ComputeFacesW3D
         real(8), dimension(:,:,:), contiguous, target, intent(in)    :: Visc_H, Diff_V
-        integer, dimension(:,:,:), contiguous, target, intent(in)    :: WaterPoints3D
+        integer, dimension(:,:,:), optional, contiguous, target, intent(in) :: WaterPoints3D
         logical, optional,                 intent(in)    :: CellFluxes
+        logical, optional,                 intent(in)    :: NumericStability
         real(8), dimension(:,:,:), optional, contiguous, target, intent(in) :: ReferenceProp
         integer, optional,                 intent(in)    :: BoundaryCondition
         real(8), optional,                 intent(in)    :: DecayTime
@@ -344,9 +469,16 @@ contains
         a%ComputeFacesV3D = c_loc(ComputeFacesV3D)
         a%ComputeFacesW3D = c_loc(ComputeFacesW3D)
         a%Visc_H = c_loc(Visc_H); a%Diff_V = c_loc(Diff_V)
+        ! the library needs WaterPoints3D (the columns THOMASZ solves, MF:4086)
+        if (.not. present(WaterPoints3D)) then
+            if (present(STAT)) STAT = MGPU_ERR_BADARG
+            return
+        endif
         a%WaterPoints3D = c_loc(WaterPoints3D)
         a%Optimize = 0; if (present(Optimize)) a%Optimize = b2i(Optimize)
-        a%FirstProperty = 0; if (present(FirstProperty)) a%FirstProperty = b2i(FirstProperty)
+        ! without FirstProperty nothing is kept from an earlier call: every
+        ! coefficient is made again (the safe reading of an absent flag)
+        a%FirstProperty = 1; if (present(FirstProperty)) a%FirstProperty = b2i(FirstProperty)
         a%CellFluxes = 0
         if (present(CellFluxes)) a%CellFluxes = b2i(CellFluxes)
         a%ReferenceProp = c_null_ptr
@@ -452,6 +584,210 @@ contains
     end subroutine UnSetDischargesGPU
 
     !> KillAdvectionDiffusion (ModuleAdvectionDiffusion.F90:5849).
+    !> The tracer loop of ModuleWaterProperties::Advection_Diffusion_Processes
+    !> (WP:14520-15169) for NTracers properties that share one hydrodynamic
+    !> state, kept on the GPU across steps: TransportStartGPU (with the
+    !> geometry StartAdvectionDiffusionGPU takes), TransportSetPropertyGPU per
+    !> property, then per step TransportSetGeometryGPU (a moving vertical
+    !> coordinate), TransportSetHydroGPU (the hydrodynamic arguments of
+    !> AdvectionDiffusion) and TransportRunGPU; TransportGetPropertyGPU
+    !> returns a property's concentration.  Every property of a step gets the
+    !> per-property semantics of AdvectionDiffusion, Optimize/FirstProperty as
+    !> WP:14576-14598 decide.  Precision: MGPU_PREC_FP64 (the reference's
+    !> arithmetic), MGPU_PREC_FAST, MGPU_PREC_MIXED.
+    subroutine TransportStartGPU(TID, Size, WorkSize, NTracers, Precision, DUX, DVY, DZX,    &
+                                 DZY, KFloorZ, AreaU, AreaV, DZZ, DWZ, Docycle_method, STAT)
+        integer,                           intent(out) :: TID
+        integer, dimension(6),             intent(in)  :: Size, WorkSize
+        integer,                           intent(in)  :: NTracers, Precision
+        real(8), dimension(:,:),   target, intent(in)  :: DUX, DVY, DZX, DZY
+        integer, dimension(:,:),   target, intent(in)  :: KFloorZ
+        real(8), dimension(:,:,:), contiguous, target, intent(in)  :: AreaU, AreaV, DZZ, DWZ
+        integer,                           intent(in)  :: Docycle_method
+        integer, optional,                 intent(out) :: STAT
+        type(t_mgpu_transport_opts) :: o
+        type(t_mgpu_geometry)       :: g
+        type(t_mgpu_size3d)         :: s, w
+        type(c_ptr)                 :: h
+        integer                     :: rc, n
+        o%ntracers = NTracers; o%precision = Precision; o%docycle_method = Docycle_method
+        o%kernel = 0; o%tracers_per_launch = 0
+        o%explicit_tracers_per_lane = 0; o%column_tracers_per_lane = 0
+        g = geometry_of(DUX, DVY, DZX, DZY, KFloorZ, AreaU, AreaV, DZZ, DWZ)
+        s = to_size3d(Size); w = to_size3d(WorkSize)
+        TID = 0
+        rc = mgpu_transport_create(s, w, o, g, h)
+        if (rc == MGPU_SUCCESS) then
+            do n = 1, MAX_INSTANCES
+                if (.not. c_associated(THandles(n))) then
+                    THandles(n) = h
+                    TID = n
+                    exit
+                endif
+            enddo
+            if (TID == 0) then
+                rc = mgpu_transport_destroy(h)
+                rc = -99
+            endif
+        endif
+        if (present(STAT)) STAT = rc
+    end subroutine TransportStartGPU
+
+    logical function bad_tid(TID, STAT)
+        integer,           intent(in)  :: TID
+        integer, optional, intent(out) :: STAT
+        bad_tid = TID < 1 .or. TID > MAX_INSTANCES
+        if (.not. bad_tid) bad_tid = .not. c_associated(THandles(TID))
+        if (bad_tid .and. present(STAT)) STAT = MGPU_OFF_ERR
+    end function bad_tid
+
+    !> The geometry of the following steps (as SetGeometryGPU).
+    subroutine TransportSetGeometryGPU(TID, DUX, DVY, DZX, DZY, KFloorZ, AreaU, AreaV, DZZ,   &
+                                       DWZ, STAT)
+        integer,                           intent(in)  :: TID
+        real(8), dimension(:,:),   target, intent(in)  :: DUX, DVY, DZX, DZY
+        integer, dimension(:,:),   target, intent(in)  :: KFloorZ
+        real(8), dimension(:,:,:), contiguous, target, intent(in)  :: AreaU, AreaV, DZZ, DWZ
+        integer, optional,                 intent(out) :: STAT
+        integer :: rc
+        if (bad_tid(TID, STAT)) return
+        rc = mgpu_transport_set_geometry(THandles(TID),                                      &
+                 geometry_of(DUX, DVY, DZX, DZY, KFloorZ, AreaU, AreaV, DZZ, DWZ))
+        if (present(STAT)) STAT = rc
+    end subroutine TransportSetGeometryGPU
+
+    !> The hydrodynamic state and the transport options of the following
+    !> steps: AdvectionDiffusion's arguments (ModuleAdvectionDiffusion.F90:
+    !> 1108-1198) that every property of a step shares, with their names.
+    subroutine TransportSetHydroGPU(TID, schmidt_H, SchmidtCoef_V, SchmidtBackground_V,      &
+                                    AdvMethodH, TVDLimitationH, AdvMethodV, TVDLimitationV,  &
+                                    Upwind2H, Upwind2V, VolumeRelMax, DTProp, ImpExp_AdvV,   &
+                                    ImpExp_DifV, ImpExp_AdvXX, ImpExp_AdvYY, ImpExp_DifH,    &
+                                    NullDif, Wflux_X, Wflux_Y, Wflux_Z, VolumeZOld, VolumeZ, &
+                                    OpenPoints3D, LandPoints3D, ComputeFacesU3D,             &
+                                    ComputeFacesV3D, ComputeFacesW3D, Visc_H, Diff_V,        &
+                                    WaterPoints3D, SmallDepths, NoDifFlux, NoFluxU, NoFluxV, &
+                                    NoFluxW, Optimize, STAT)
+        integer,                           intent(in)    :: TID
+        real(8),                           intent(in)    :: schmidt_H, SchmidtCoef_V
+        real(8),                           intent(in)    :: SchmidtBackground_V
+        integer,                           intent(in)    :: AdvMethodH, TVDLimitationH
+        integer,                           intent(in)    :: AdvMethodV, TVDLimitationV
+        logical,                           intent(in)    :: Upwind2H, Upwind2V
+        real(8),                           intent(in)    :: VolumeRelMax, DTProp
+        real(8),                           intent(in)    :: ImpExp_AdvV, ImpExp_DifV
+        real(8),                           intent(in)    :: ImpExp_AdvXX, ImpExp_AdvYY
+        real(8),                           intent(in)    :: ImpExp_DifH
+        logical,                           intent(in)    :: NullDif
+        real(8), dimension(:,:,:), contiguous, target, intent(in)    :: Wflux_X, Wflux_Y, Wflux_Z
+        real(8), dimension(:,:,:), contiguous, target, intent(in)    :: VolumeZOld, VolumeZ
+        integer, dimension(:,:,:), contiguous, target, intent(in)    :: OpenPoints3D, LandPoints3D
+        integer, dimension(:,:,:), contiguous, target, intent(in)    :: ComputeFacesU3D, ComputeFacesV3D
+        integer, dimension(:,:,:), contiguous, target, intent(in)    :: ComputeFacesW3D
+        real(8), dimension(:,:,:), contiguous, target, intent(in)    :: Visc_H, Diff_V
+        integer, dimension(:,:,:), contiguous, target, intent(in)    :: WaterPoints3D
+        logical, dimension(:,:), optional, intent(in)    :: SmallDepths
+        logical, optional,                 intent(in)    :: NoDifFlux
+        integer, dimension(:,:,:), optional, contiguous, target, intent(in) :: NoFluxU, NoFluxV
+        integer, dimension(:,:,:), optional, contiguous, target, intent(in) :: NoFluxW
+        logical, optional,                 intent(in)    :: Optimize
+        integer, optional,                 intent(out)   :: STAT
+        type(t_mgpu_ad_args) :: a
+        integer              :: rc
+        integer, allocatable, target :: small(:,:)
+        if (bad_tid(TID, STAT)) return
+        a%schmidt_H = schmidt_H
+        a%SchmidtCoef_V = SchmidtCoef_V
+        a%SchmidtBackground_V = SchmidtBackground_V
+        a%AdvMethodH = AdvMethodH; a%TVDLimitationH = TVDLimitationH
+        a%AdvMethodV = AdvMethodV; a%TVDLimitationV = TVDLimitationV
+        a%Upwind2H = b2i(Upwind2H); a%Upwind2V = b2i(Upwind2V)
+        a%VolumeRelMax = VolumeRelMax
+        a%AdvectionNudging = 0; a%AdvectionNudgingCells = 0
+        a%DTProp = DTProp
+        a%ImpExp_AdvV = ImpExp_AdvV; a%ImpExp_DifV = ImpExp_DifV
+        a%ImpExp_AdvXX = ImpExp_AdvXX; a%ImpExp_AdvYY = ImpExp_AdvYY
+        a%ImpExp_DifH = ImpExp_DifH
+        a%NullDif = b2i(NullDif)
+        a%Wflux_X = c_loc(Wflux_X); a%Wflux_Y = c_loc(Wflux_Y); a%Wflux_Z = c_loc(Wflux_Z)
+        a%VolumeZOld = c_loc(VolumeZOld); a%VolumeZ = c_loc(VolumeZ)
+        a%OpenPoints3D = c_loc(OpenPoints3D); a%LandPoints3D = c_loc(LandPoints3D)
+        a%ComputeFacesU3D = c_loc(ComputeFacesU3D)
+        a%ComputeFacesV3D = c_loc(ComputeFacesV3D)
+        a%ComputeFacesW3D = c_loc(ComputeFacesW3D)
+        a%Visc_H = c_loc(Visc_H); a%Diff_V = c_loc(Diff_V)
+        a%WaterPoints3D = c_loc(WaterPoints3D)
+        a%Optimize = 0; if (present(Optimize)) a%Optimize = b2i(Optimize)
+        a%FirstProperty = 1
+        a%CellFluxes = 0; a%NoAdvFlux = 0
+        a%ReferenceProp = c_null_ptr; a%BoundaryCondition = 0; a%DecayTime = -9.9e15_8
+        a%BoundaryPoints2D = c_null_ptr; a%PROPOld = c_null_ptr
+        a%NoDifFlux = 0
+        if (present(NoDifFlux)) a%NoDifFlux = b2i(NoDifFlux)
+        a%NoFluxU = c_null_ptr; a%NoFluxV = c_null_ptr; a%NoFluxW = c_null_ptr
+        if (present(NoFluxU)) a%NoFluxU = c_loc(NoFluxU)
+        if (present(NoFluxV)) a%NoFluxV = c_loc(NoFluxV)
+        if (present(NoFluxW)) a%NoFluxW = c_loc(NoFluxW)
+        a%SmallDepths = c_null_ptr
+        if (present(SmallDepths)) then
+            allocate(small(size(SmallDepths, 1), size(SmallDepths, 2)))
+            small = merge(1, 0, SmallDepths)
+            a%SmallDepths = c_loc(small)
+        endif
+        a%device_pointers = 0
+        rc = mgpu_transport_set_hydro(THandles(TID), a)
+        if (present(STAT)) STAT = rc
+    end subroutine TransportSetHydroGPU
+
+    !> Property Index (1..NTracers) takes the concentration PROP (over Size).
+    subroutine TransportSetPropertyGPU(TID, Index, PROP, STAT)
+        integer,                           intent(in)  :: TID, Index
+        real(8), dimension(:,:,:), contiguous, target, intent(in) :: PROP
+        integer, optional,                 intent(out) :: STAT
+        integer :: rc
+        if (bad_tid(TID, STAT)) return
+        rc = mgpu_transport_set_tracer(THandles(TID), int(Index - 1, c_int32_t), c_loc(PROP), 0)
+        if (present(STAT)) STAT = rc
+    end subroutine TransportSetPropertyGPU
+
+    !> The concentration of property Index (1..NTracers) into PROP.
+    subroutine TransportGetPropertyGPU(TID, Index, PROP, STAT)
+        integer,                           intent(in)    :: TID, Index
+        real(8), dimension(:,:,:), contiguous, target, intent(inout) :: PROP
+        integer, optional,                 intent(out)   :: STAT
+        integer :: rc
+        if (bad_tid(TID, STAT)) return
+        rc = mgpu_transport_get_tracer(THandles(TID), int(Index - 1, c_int32_t), c_loc(PROP), 0)
+        if (present(STAT)) STAT = rc
+    end subroutine TransportGetPropertyGPU
+
+    !> NSteps transport steps of every property (captured as a hipGraph with
+    !> UseGraph); waits for them and returns what the reference would have
+    !> stopped on (MGPU_ERR_OPTION, MGPU_ERR_PIVOT).
+    subroutine TransportRunGPU(TID, NSteps, UseGraph, STAT)
+        integer,           intent(in)  :: TID, NSteps
+        logical, optional, intent(in)  :: UseGraph
+        integer, optional, intent(out) :: STAT
+        integer :: rc
+        logical :: g
+        if (bad_tid(TID, STAT)) return
+        g = .true.; if (present(UseGraph)) g = UseGraph
+        rc = mgpu_transport_run(THandles(TID), int(NSteps, c_int32_t), b2i(g))
+        if (rc == MGPU_SUCCESS) rc = mgpu_transport_status(THandles(TID))
+        if (present(STAT)) STAT = rc
+    end subroutine TransportRunGPU
+
+    subroutine TransportKillGPU(TID, STAT)
+        integer,           intent(inout) :: TID
+        integer, optional, intent(out)   :: STAT
+        integer :: rc
+        if (bad_tid(TID, STAT)) return
+        rc = mgpu_transport_destroy(THandles(TID))
+        THandles(TID) = c_null_ptr
+        TID = 0
+        if (present(STAT)) STAT = rc
+    end subroutine TransportKillGPU
+
     subroutine KillAdvectionDiffusionGPU(ID, STAT)
         integer,           intent(inout) :: ID
         integer, optional, intent(out)   :: STAT

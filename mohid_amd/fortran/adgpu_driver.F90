@@ -31,8 +31,16 @@
 ! BoundaryPoints2D (2-D int) when BoundaryCondition or ReferenceProp is
 ! given; ReferenceProp (3-D real) when given.  An optional argument the
 ! input does not give is left unallocated, which makes it absent.
-! Output: PROP after nsteps steps, then with CellFluxes the six flux arrays
-! of the last step (GetFluxesGPU).  Exit status 0 on success, else the STAT.
+! Then the run: mode (0: AdvectionDiffusionGPU per property, as
+! WaterProperties calls it; 1: the batched transport, TransportStartGPU ..
+! TransportRunGPU), ngeo (0, or nsteps: a geometry per step, a moving free
+! surface), ntr (properties), precision (the batched transport's); the
+! ntr-1 further properties (3-D real); with ngeo, per step AreaU, AreaV, DZZ,
+! DWZ, VolumeZOld, VolumeZ (3-D real), set before the step (SetGeometryGPU /
+! TransportSetGeometryGPU, and the step's volumes).
+! Output: the ntr properties after nsteps steps, then with CellFluxes the six
+! flux arrays of the last step (GetFluxesGPU).  Exit status 0 on success,
+! else the STAT.
 !------------------------------------------------------------------------------
 program adgpu_driver
 
@@ -59,6 +67,8 @@ program adgpu_driver
     real(8), allocatable, dimension(:,:,:) :: RefP, POld, AFX, AFY, AFZ, DFX, DFY, DFZ
     integer, allocatable :: BC
     real(8), allocatable :: Decay
+    integer :: run(4), mode, ngeo, ntr, prec, t, TID
+    real(8), allocatable, dimension(:,:,:,:) :: PR, GAU, GAV, GZZ, GWZ, GVO, GV
 
     call get_command_argument(1, fin)
     call get_command_argument(2, fout)
@@ -106,7 +116,28 @@ program adgpu_driver
         allocate(RefP, mold=AreaU)
         read(u) RefP
     endif
+    read(u) run
+    mode = run(1); ngeo = run(2); ntr = run(3); prec = run(4)
+    allocate(PR(sz(1):sz(2), sz(3):sz(4), sz(5):sz(6), ntr))
+    PR(:, :, :, 1) = PROP
+    do t = 2, ntr
+        read(u) PR(:, :, :, t)
+    enddo
+    if (ngeo > 0) then
+        allocate(GAU(sz(1):sz(2), sz(3):sz(4), sz(5):sz(6), ngeo))
+        allocate(GAV, GZZ, GWZ, GVO, GV, mold=GAU)
+        do s = 1, ngeo
+            read(u) GAU(:, :, :, s), GAV(:, :, :, s), GZZ(:, :, :, s), GWZ(:, :, :, s)
+            read(u) GVO(:, :, :, s), GV(:, :, :, s)
+        enddo
+    endif
     close(u)
+
+    if (mode == 1) then
+        call run_batched()
+        call write_out()
+        stop
+    endif
 
     call StartAdvectionDiffusionGPU(ID, sz, ws, DUX, DVY, DZX, DZY, KFloorZ,           &
                                     AreaU, AreaV, DZZ, DWZ, 1, STAT = STAT)
@@ -115,6 +146,19 @@ program adgpu_driver
         error stop 2
     endif
     do s = 1, nsteps
+      if (ngeo > 0) then
+        AreaU = GAU(:, :, :, s); AreaV = GAV(:, :, :, s)
+        DZZ = GZZ(:, :, :, s); DWZ = GWZ(:, :, :, s)
+        VolOld = GVO(:, :, :, s); Vol = GV(:, :, :, s)
+        call SetGeometryGPU(ID, DUX, DVY, DZX, DZY, KFloorZ, AreaU, AreaV, DZZ, DWZ,     &
+                            STAT = STAT)
+        if (STAT /= MGPU_SUCCESS) then
+            write(*, *) 'SetGeometryGPU STAT =', STAT
+            error stop 7
+        endif
+      endif
+      do t = 1, ntr
+        PROP = PR(:, :, :, t)
         if (ndis > 0) then
             call SetDischargesGPU(ID, DFlow, DConc, DI, DJ, DK, DKmin, DKmax, DVert, ndis,  &
                                   DIgn(1:ndis) /= 0, DnC, DByp(1:ndis) /= 0, DConcMF,      &
@@ -130,15 +174,16 @@ program adgpu_driver
                                    ipar(10),                                              &
                                    rpar(5), rpar(6), rpar(7), rpar(8), rpar(9), rpar(10), &
                                    ipar(7) /= 0, Qx, Qy, Qz, VolOld, Vol, OP, LP,         &
-                                   CFU, CFV, CFW, Visc, Diff, WP,                         &
-                                   CellFluxes = iopt(1) /= 0, ReferenceProp = RefP,       &
+                                   CFU, CFV, CFW, Visc, Diff,                             &
+                                   CellFluxes = iopt(1) /= 0, WaterPoints3D = WP,         &
+                                   ReferenceProp = RefP,                                  &
                                    BoundaryCondition = BC, DecayTime = Decay,             &
-                                   BoundaryPoints2D = BP, PROPOld = POld,                 &
+                                   NumericStability = .false., PROPOld = POld,            &
                                    SmallDepths = SmallD, NoAdvFlux = iopt(2) /= 0,        &
                                    NoDifFlux = iopt(3) /= 0, NoFluxU = NFU,               &
                                    NoFluxV = NFV, NoFluxW = NFW,                          &
-                                   Optimize = ipar(8) /= 0, FirstProperty = .true.,      &
-                                   STAT = STAT)
+                                   Optimize = ipar(8) /= 0, FirstProperty = t == 1,       &
+                                   STAT = STAT, BoundaryPoints2D = BP)
         if (STAT /= MGPU_SUCCESS) then
             write(*, *) 'AdvectionDiffusionGPU step', s, 'STAT =', STAT
             error stop 3
@@ -147,6 +192,8 @@ program adgpu_driver
             call UnSetDischargesGPU(ID, STAT = STAT)
             if (STAT /= MGPU_SUCCESS) error stop 5
         endif
+        PR(:, :, :, t) = PROP
+      enddo
     enddo
     if (iopt(1) /= 0) then
         allocate(AFX, AFY, AFZ, DFX, DFY, DFZ, mold=AreaU)
@@ -157,10 +204,70 @@ program adgpu_driver
         endif
     endif
     call KillAdvectionDiffusionGPU(ID, STAT = STAT)
+    call write_out()
 
-    open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
-    write(u) PROP
-    if (iopt(1) /= 0) write(u) AFX, AFY, AFZ, DFX, DFY, DFZ
-    close(u)
+contains
+
+    subroutine write_out()
+        open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+        write(u) PR
+        if (iopt(1) /= 0) write(u) AFX, AFY, AFZ, DFX, DFY, DFZ
+        close(u)
+    end subroutine write_out
+
+    ! The batched transport: the properties on the GPU for the whole run,
+    ! the hydrodynamic state (and with ngeo the geometry) set before each step.
+    subroutine run_batched()
+        call TransportStartGPU(TID, sz, ws, ntr, prec, DUX, DVY, DZX, DZY, KFloorZ, AreaU,  &
+                               AreaV, DZZ, DWZ, 1, STAT = STAT)
+        if (STAT /= MGPU_SUCCESS) then
+            write(*, *) 'TransportStartGPU STAT =', STAT
+            error stop 12
+        endif
+        do t = 1, ntr
+            call TransportSetPropertyGPU(TID, t, PR(:, :, :, t), STAT = STAT)
+            if (STAT /= MGPU_SUCCESS) error stop 13
+        enddo
+        do s = 1, nsteps
+            if (ngeo > 0) then
+                AreaU = GAU(:, :, :, s); AreaV = GAV(:, :, :, s)
+                DZZ = GZZ(:, :, :, s); DWZ = GWZ(:, :, :, s)
+                VolOld = GVO(:, :, :, s); Vol = GV(:, :, :, s)
+                call TransportSetGeometryGPU(TID, DUX, DVY, DZX, DZY, KFloorZ, AreaU, AreaV, &
+                                             DZZ, DWZ, STAT = STAT)
+                if (STAT /= MGPU_SUCCESS) error stop 14
+            endif
+            if (ngeo > 0 .or. s == 1) then
+                call TransportSetHydroGPU(TID, rpar(1), rpar(2), rpar(3), ipar(1), ipar(2),  &
+                                          ipar(3), ipar(4), ipar(5) /= 0, ipar(6) /= 0,      &
+                                          rpar(4), rpar(5), rpar(6), rpar(7), rpar(8),       &
+                                          rpar(9), rpar(10), ipar(7) /= 0, Qx, Qy, Qz,       &
+                                          VolOld, Vol, OP, LP, CFU, CFV, CFW, Visc, Diff,    &
+                                          WP, SmallDepths = SmallD,                          &
+                                          NoDifFlux = iopt(3) /= 0, NoFluxU = NFU,           &
+                                          NoFluxV = NFV, NoFluxW = NFW,                      &
+                                          Optimize = ipar(8) /= 0, STAT = STAT)
+                if (STAT /= MGPU_SUCCESS) then
+                    write(*, *) 'TransportSetHydroGPU STAT =', STAT
+                    error stop 15
+                endif
+            endif
+            if (ngeo > 0) then
+                call TransportRunGPU(TID, 1, STAT = STAT)
+            else
+                call TransportRunGPU(TID, nsteps, STAT = STAT)
+            endif
+            if (STAT /= MGPU_SUCCESS) then
+                write(*, *) 'TransportRunGPU STAT =', STAT
+                error stop 16
+            endif
+            if (ngeo == 0) exit
+        enddo
+        do t = 1, ntr
+            call TransportGetPropertyGPU(TID, t, PR(:, :, :, t), STAT = STAT)
+            if (STAT /= MGPU_SUCCESS) error stop 17
+        enddo
+        call TransportKillGPU(TID, STAT = STAT)
+    end subroutine run_batched
 
 end program adgpu_driver

@@ -512,6 +512,16 @@ int ora_start(const ora_size3d *size, const ora_size3d *worksize,
     return ORA_SUCCESS;
 }
 
+/* The geometry AdvectionDiffusion fetches on every call (GetGeometryAreas /
+ * GetGeometryDistances, AD:1353-1401): the following steps read these
+ * arrays. */
+int ora_set_geometry(ora_instance *me, const ora_geometry *geom)
+{
+    if (!me || !geom) return 1;
+    me->g = *geom;
+    return 0;
+}
+
 void ora_kill(ora_instance *me)
 {
     if (!me) return;
@@ -611,9 +621,10 @@ static void convert_visc_dif_horizontal(ora_instance *me, const ora_args *a, int
             for (i = w->ILB; i <= w->IUB; i++) {
                 long x = I3(me, i, j, k);
                 if (a->ComputeFacesU3D[x] == 1) {
+                    const int ja = g_mistake == 22 ? j : j - 1, jb = g_mistake == 22 ? j - 1 : j;
                     me->KhX[x] = a->schmidt_H *
-                                 (nu[x] * DUX[I2(me, i, j - 1)] +
-                                  nu[I3(me, i, j - 1, k)] * DUX[I2(me, i, j)]) /
+                                 (nu[x] * DUX[I2(me, i, ja)] +
+                                  nu[I3(me, i, j - 1, k)] * DUX[I2(me, i, jb)]) /
                                  (DUX[I2(me, i, j)] + DUX[I2(me, i, j - 1)]);
                     if (nodif && a->NoFluxU[x] == 1) me->KhX[x] = 0.0;   /* AD:2496-2500 */
                 }
@@ -628,9 +639,10 @@ static void convert_visc_dif_horizontal(ora_instance *me, const ora_args *a, int
             for (i = w->ILB; i <= w->IUB; i++) {
                 long x = I3(me, i, j, k);
                 if (a->ComputeFacesV3D[x] == 1) {
+                    const int ia = g_mistake == 25 ? i : i - 1, ib = g_mistake == 25 ? i - 1 : i;
                     me->KhY[x] = a->schmidt_H *
-                                 (nu[x] * DVY[I2(me, i - 1, j)] +
-                                  nu[I3(me, i - 1, j, k)] * DVY[I2(me, i, j)]) /
+                                 (nu[x] * DVY[I2(me, ia, j)] +
+                                  nu[I3(me, i - 1, j, k)] * DVY[I2(me, ib, j)]) /
                                  (DVY[I2(me, i, j)] + DVY[I2(me, i - 1, j)]);
                     if (nodif && g_mistake != 12 && a->NoFluxV[x] == 1)   /* AD:2527-2531 */
                         me->KhY[x] = 0.0;
@@ -670,7 +682,8 @@ static void compute_difh_constants(ora_instance *me, const ora_args *a)
             for (i = s->ILB; i <= s->IUB; i++) {
                 long x = I3(me, i, j, k);
                 if (a->ComputeFacesV3D[x] == 1 && i - 1 >= s->ILB)
-                    me->DHV[x] = me->KhY[x] * me->g.AreaV[x] / me->g.DZY[I2(me, i - 1, j)];
+                    me->DHV[x] = me->KhY[x] * me->g.AreaV[x] /
+                                 me->g.DZY[I2(me, g_mistake == 26 ? i : i - 1, j)];
             }
     }
 #pragma omp parallel for schedule(static)
@@ -680,7 +693,9 @@ static void compute_difh_constants(ora_instance *me, const ora_args *a)
             for (i = s->ILB; i <= s->IUB; i++) {
                 long x = I3(me, i, j, k);
                 if (a->ComputeFacesU3D[x] == 1 && j - 1 >= s->JLB)
-                    me->DHU[x] = me->KhX[x] * me->g.AreaU[x] / me->g.DZX[I2(me, i, j - 1)];
+                    me->DHU[x] = me->KhX[x] * me->g.AreaU[x] /
+                                 (g_mistake == 24 ? me->g.DUX : me->g.DZX)[I2(
+                                     me, i, g_mistake == 23 ? j : j - 1)];
             }
     }
 }
@@ -963,7 +978,9 @@ static void horizontal_diffusion_xx(ora_instance *me, const ora_args *a, const d
                     me->TI[xm] = me->TI[xm] + auxj / a->VolumeZ[xm];
                     me->TI[x] = me->TI[x] - auxj / a->VolumeZ[x];
                 } else {
-                    double auxj = me->KhX[x] * me->g.AreaU[x] / me->g.DZX[I2(me, i, j - 1)];
+                    double auxj = me->KhX[x] * me->g.AreaU[x] /
+                                  (g_mistake == 24 ? me->g.DUX : me->g.DZX)[I2(
+                                      me, i, g_mistake == 23 ? j : j - 1)];
                     me->TI[xm] = me->TI[xm] + auxj * dt / a->VolumeZ[xm] * (P[x] - P[xm]);
                     me->TI[x] = me->TI[x] - auxj * dt / a->VolumeZ[x] * (P[x] - P[xm]);
                 }
@@ -992,7 +1009,8 @@ static void horizontal_diffusion_yy(ora_instance *me, const ora_args *a, const d
                     me->TI[xm] = me->TI[xm] + auxi / a->VolumeZ[xm];
                     me->TI[x] = me->TI[x] - auxi / a->VolumeZ[x];
                 } else {
-                    double auxi = me->KhY[x] * me->g.AreaV[x] / me->g.DZY[I2(me, i - 1, j)];
+                    double auxi = me->KhY[x] * me->g.AreaV[x] /
+                                  me->g.DZY[I2(me, g_mistake == 26 ? i : i - 1, j)];
                     me->TI[xm] = me->TI[xm] + auxi * dt / a->VolumeZ[g_mistake == 4 ? x : xm] *
                                               (P[x] - P[xm]);
                     me->TI[x] = me->TI[x] - auxi * dt / a->VolumeZ[x] * (P[x] - P[xm]);
@@ -1032,11 +1050,13 @@ static int horizontal_advection_xx(ora_instance *me, const ora_args *a, const do
             for (i = w->ILB; i <= w->IUB; i++) {
                 long b3 = I3(me, i, s->JLB, k), b2 = I2(me, i, s->JLB);
                 if (a->Optimize)
-                    adv1d_sb2_strided(w->JLB + 1, w->JUB + 1, dt, me->g.DUX + b2, sj,
+                    adv1d_sb2_strided(w->JLB + 1, w->JUB + 1, dt,
+                                      (g_mistake == 27 ? me->g.DVY : me->g.DUX) + b2, sj,
                                       P + b3, a->Wflux_X + b3, a->VolumeZ + b3,
                                       a->OpenPoints3D + b3, sj, me->xD + b3, me->xE + b3);
                 else
-                    rc |= adv1d_v2_strided(w->JLB + 1, w->JUB + 1, dt, me->g.DUX + b2, sj,
+                    rc |= adv1d_v2_strided(w->JLB + 1, w->JUB + 1, dt,
+                                           (g_mistake == 27 ? me->g.DVY : me->g.DUX) + b2, sj,
                                            P + b3, a->Wflux_X + b3, a->VolumeZ + b3,
                                            a->OpenPoints3D + b3, sj,
                                            me->xC + b3, me->xD + b3, me->xE + b3, me->xF + b3,
@@ -2068,7 +2088,9 @@ void ora_compute_volumes(const ora_size3d *size, const ora_size3d *ws, const dou
     for (k = ws->KLB; k <= ws->KUB; k++)
         for (j = ws->JLB + 1; j <= ws->JUB; j++)
             for (i = ws->ILB; i <= ws->IUB; i++)
-                vu[PX(size, i, j, k)] = (vz[PX(size, i, j - 1, k)] + vz[PX(size, i, j, k)]) / 2.0;
+                vu[PX(size, i, j, k)] = g_mistake == 28
+                    ? (vz[PX(size, i, j, k)] + vz[PX(size, i, j + 1 <= size->JUB ? j + 1 : j, k)]) / 2.0
+                    : (vz[PX(size, i, j - 1, k)] + vz[PX(size, i, j, k)]) / 2.0;
     for (k = ws->KLB; k <= ws->KUB; k++)
         for (j = ws->JLB; j <= ws->JUB; j++)
             for (i = ws->ILB + 1; i <= ws->IUB; i++)

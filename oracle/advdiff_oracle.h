@@ -98,6 +98,7 @@ typedef struct ora_instance ora_instance;
 int ora_start(const ora_size3d *size, const ora_size3d *worksize,
               int docycle_method, int vertical1d, int xzflow, int nthreads,
               const ora_geometry *geom, ora_instance **out);
+int ora_set_geometry(ora_instance *me, const ora_geometry *geom);
 /* AdvectionDiffusion (ModuleAdvectionDiffusion.F90:1108-1509): PROP in place. */
 int ora_advection_diffusion(ora_instance *me, const ora_args *args, double *prop);
 /* KillAdvectionDiffusion (ModuleAdvectionDiffusion.F90:5849). */

@@ -79,6 +79,8 @@ def lib():
         L.ora_start.argtypes = [ct.POINTER(Size3D), ct.POINTER(Size3D), ct.c_int, ct.c_int,
                                 ct.c_int, ct.c_int, ct.POINTER(Geometry), ct.POINTER(ct.c_void_p)]
         L.ora_start.restype = ct.c_int
+        L.ora_set_geometry.argtypes = [ct.c_void_p, ct.POINTER(Geometry)]
+        L.ora_set_geometry.restype = ct.c_int
         L.ora_advection_diffusion.argtypes = [ct.c_void_p, ct.POINTER(Args), _dp]
         L.ora_advection_diffusion.restype = ct.c_int
         L.ora_kill.argtypes = [ct.c_void_p]
@@ -151,16 +153,11 @@ class Instance:
     """One ModuleAdvectionDiffusion instance (StartAdvectionDiffusion .. Kill)."""
 
     def __init__(self, case, docycle_method: int = 1, vertical1d: bool = False,
-                 xzflow: bool = False, nthreads: Optional[int] = None):
+                 xzflow: bool = False, nthreads: Optional[int] = None, geom=None):
         self.case = case
         L = lib()
         self._keep = []
-        g = case.geom
-        self.geom = Geometry(*[_ptr(self._k(np.asfortranarray(g[n], dtype=np.float64)), _dp)
-                               for n in ("DUX", "DVY", "DZX", "DZY")],
-                             _ptr(self._k(np.asfortranarray(g["KFloorZ"], dtype=np.int32)), _ip),
-                             *[_ptr(self._k(np.asfortranarray(g[n], dtype=np.float64)), _dp)
-                               for n in ("AreaU", "AreaV", "DZZ", "DWZ")])
+        self.geom = self._geometry(case.geom if geom is None else geom)
         self.size = Size3D(*case.size)
         self.ws = Size3D(*case.worksize)
         if nthreads is None:
@@ -177,6 +174,22 @@ class Instance:
     def _k(self, a):
         self._keep.append(a)
         return a
+
+    def _geometry(self, g):
+        """A Geometry struct over copies of the arrays of dict g (kept alive)."""
+        return Geometry(*[_ptr(self._k(np.array(g[n], dtype=np.float64, order="F")), _dp)
+                          for n in ("DUX", "DVY", "DZX", "DZY")],
+                        _ptr(self._k(np.array(g["KFloorZ"], dtype=np.int32, order="F")), _ip),
+                        *[_ptr(self._k(np.array(g[n], dtype=np.float64, order="F")), _dp)
+                          for n in ("AreaU", "AreaV", "DZZ", "DWZ")])
+
+    def set_geometry(self, geom):
+        """The geometry of the following steps (the reference fetches areas and
+        distances on every call, AD:1353-1401)."""
+        self.geom = self._geometry(geom)
+        rc = lib().ora_set_geometry(self.h, ct.byref(self.geom))
+        if rc != 0:
+            raise RuntimeError(f"ora_set_geometry failed: {rc}")
 
     def step(self, prop: np.ndarray, first_property: bool = True, now: Optional[float] = None,
              hydro=None, **overrides) -> int:
@@ -263,6 +276,16 @@ class Instance:
             self.close()
         except Exception:
             pass
+
+
+def host_threads() -> int:
+    """Threads the oracle may use here: the cores this process may run on,
+    capped by OMP_NUM_THREADS when the host sets it (its CPU share)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(n, 1)
 
 
 def run_case(case, steps: int, nthreads: Optional[int] = None, tracers=None):

@@ -1,4 +1,8 @@
-"""Small seeded cases shared by the parity tests (sized for the oracle)."""
+"""Small seeded cases shared by the parity tests (sized for the oracle).
+
+Every case runs on a stretched, slightly skewed grid (fixtures.stretched_corners)
+with dx != dy, so DUX, DVY, DZX and DZY differ from cell to cell along both
+axes: an index slip in a horizontal metric changes the results."""
 from mohid_amd import constants as C
 from mohid_amd import fixtures as fx
 
@@ -17,8 +21,11 @@ SMALL = {
 }
 
 
+GRID = dict(metrics="stretched", dx=500.0, dy=350.0)
+
+
 def make(name, **extra):
     base, ov = SMALL[name]
-    ov = dict(ov)
+    ov = dict(GRID, **ov)
     ov.update(extra)
     return fx.make_case(fx.config(base, **ov))

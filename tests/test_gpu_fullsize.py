@@ -101,3 +101,145 @@ def test_full_size_drop_in_c2():
     print(f"c2 drop-in: mass drift {drift:.1e}, constant kept to {err / 7.0:.1e}")
     assert drift <= MASS_RTOL
     assert err / 7.0 <= CONST_RTOL
+
+
+def _on_device(torch, case):
+    """The case with every 3-D field (and the tracers) as a device tensor of
+    the same memory layout ((nK, nJ, nI) C order = Fortran (nI, nJ, nK))."""
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a).T)).cuda()  # noqa: E731
+    geom = {k: (dev(v) if np.ndim(v) == 3 else v) for k, v in case.geom.items()}
+    hydro = {k: dev(v) for k, v in case.hydro.items()}
+    return geom, hydro, [dev(t) for t in case.tracers]
+
+
+def _host(t):
+    return np.asfortranarray(t.cpu().numpy().T)
+
+
+def _threads():
+    from oracle import oracle as ora
+    return ora.host_threads()
+
+
+def test_full_size_c1_matches_oracle():
+    """The plain method at its full size: 100 x 100 x 20 Cartesian, upwind,
+    one tracer, dt 60 s, 100 steps, through the drop-in (host arrays, as a
+    Fortran caller passes them) and the batched handle: bit-identical to the
+    oracle after every one of the 100 steps."""
+    from oracle import oracle as ora
+    torch, L, AD, Transport = _mods()
+    case = fx.make_case(fx.config("c1"))
+    steps = case.cfg.steps
+    dt = case.params["DTProp"]
+    inst = ora.Instance(case, nthreads=_threads())
+    ref = np.array(case.tracers[0], order="F", copy=True)
+    gpu = AD(case.size, case.worksize, case.geom)
+    p = np.array(case.tracers[0], order="F", copy=True)
+    for s in range(steps):
+        assert inst.step(ref, first_property=True, now=(s + 1) * dt) == 0
+        gpu(p, case.hydro, case.params, first_property=True)
+        assert np.array_equal(p, ref), s
+    inst.close()
+    gpu.kill()
+    tr = Transport(case.size, case.worksize, case.geom, 1)
+    tr.set_hydro(case.hydro, case.params)
+    tr.set_tracer(0, case.tracers[0])
+    tr.run(steps, use_graph=True)
+    assert np.array_equal(tr.get_tracer(0), ref)
+    tr.close()
+    print(f"c1 full size: {steps} steps, {case.n_wet} water cells, bit-identical")
+
+
+@pytest.mark.timeout(600)
+def test_full_size_c2_200_steps_matches_oracle():
+    """The plain FP64 GPU drop-in at its full size: 512 x 512 x 50 sigma,
+    TVD-SuperBee, one tracer, dt 30 s, 200 per-call steps on device arrays,
+    against the oracle on every host thread: within 1e-12 (relative) of the
+    oracle on every water cell after 200 steps (and bit-identical)."""
+    import time
+    from oracle import oracle as ora
+    torch, L, AD, _ = _mods()
+    torch.cuda.set_device(0)
+    case = fx.make_case(fx.config("c2"))
+    steps = case.cfg.steps
+    geom, hydro, trs = _on_device(torch, case)
+    gpu = AD(case.size, case.worksize, geom)
+    p = trs[0]
+    for s in range(steps):
+        gpu(p, hydro, case.params, first_property=True)
+    torch.cuda.synchronize()
+    gpu.kill()
+    got = _host(p)
+    t0 = time.perf_counter()
+    nt = _threads()
+    ref = ora.run_case(case, steps, nthreads=nt)[0]
+    print(f"c2 oracle: {steps} steps on {nt} threads in {time.perf_counter() - t0:.1f} s",
+          flush=True)
+    wp = case.hydro["WaterPoints3D"] == 1
+    err = float(np.max(np.abs(got[wp] - ref[wp]) / np.maximum(np.abs(ref[wp]), 1e-300)))
+    print(f"c2 full size: {steps} steps, max rel err {err:.3e}, "
+          f"bit-identical={bool(np.array_equal(got, ref))}")
+    assert err <= 1e-12
+
+
+@pytest.mark.timeout(600)
+def test_full_size_c5_invariants_and_window():
+    """The large run's shape: 2048 x 2048 x 60 sigma (bathymetry 2-500 m),
+    25 % land, 4 tracers, TVD-SuperBee, dt 20 s, through the batched handle:
+    50 steps captured as one hipGraph keep every tracer's sum(V * C) and move
+    every tracer; a constant field stays constant over 50 more steps.  Then a
+    256 x 256 window of the same configuration runs 20 graph-captured steps
+    bit-identical to the oracle."""
+    import time
+    from oracle import oracle as ora
+    torch, L, _, Transport = _mods()
+    torch.cuda.set_device(0)
+    t0 = time.perf_counter()
+    case = fx.make_case(fx.config("c5"), device="cuda")
+    print(f"c5: {case.n_wet} water cells built in {time.perf_counter() - t0:.0f} s", flush=True)
+    V = case.hydro["VolumeZ"]
+    assert torch.equal(case.hydro["VolumeZOld"], V)
+    wp = case.hydro["WaterPoints3D"] == 1
+    T = len(case.tracers)
+    tr = Transport(case.size, case.worksize, case.geom, T)
+    tr.set_hydro(case.hydro, case.params)
+    m0 = []
+    for t, p in enumerate(case.tracers):
+        tr.set_tracer(t, p)
+        m0.append(_mass(torch, V, wp, p))
+    tr.run(50, use_graph=True)
+    tr.status()
+    out = torch.empty_like(case.tracers[0])
+    drift = []
+    for t in range(T):
+        tr.get_tracer(t, out)
+        torch.cuda.synchronize()
+        moved = torch.max(torch.abs(torch.where(wp, out - case.tracers[t],
+                                                torch.zeros_like(out)))).item()
+        assert moved > 1e-6, (t, moved)
+        drift.append(abs(_mass(torch, V, wp, out) - m0[t]) / abs(m0[t]))
+    print(f"c5 batched, 50 steps: mass drift per tracer {['%.1e' % d for d in drift]}",
+          flush=True)
+    assert max(drift) <= MASS_RTOL, drift
+    const = torch.where(wp, torch.full_like(out, 7.0), case.tracers[0])
+    tr.set_tracer(0, const)
+    tr.run(50, use_graph=True)
+    tr.status()
+    tr.get_tracer(0, out)
+    torch.cuda.synchronize()
+    err = float(torch.max(torch.abs(torch.where(wp, out - 7.0, torch.zeros_like(out)))).item())
+    tr.close()
+    del case, out, const, V, wp
+    print(f"c5 batched: constant 7 kept to {err / 7.0:.1e} over 50 steps", flush=True)
+    assert err / 7.0 <= CONST_RTOL
+    win = fx.make_case(fx.config("c5", ni=256, nj=256))
+    tw = Transport(win.size, win.worksize, win.geom, T)
+    tw.set_hydro(win.hydro, win.params)
+    for t, p in enumerate(win.tracers):
+        tw.set_tracer(t, p)
+    tw.run(20, use_graph=True)
+    ref = ora.run_case(win, 20, nthreads=_threads())
+    for t in range(T):
+        assert np.array_equal(tw.get_tracer(t), ref[t]), t
+    tw.close()
+    print(f"c5 window 256 x 256 x 60: 20 steps, {win.n_wet} water cells, bit-identical")

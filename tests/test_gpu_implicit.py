@@ -19,7 +19,7 @@ from mohid_amd import constants as C
 from mohid_amd import fixtures as fx
 from oracle import oracle as ora
 
-from .cases import SMALL, make
+from .cases import GRID, SMALL, make
 
 pytestmark = pytest.mark.gpu
 
@@ -165,7 +165,7 @@ def test_implicit_one_layer(start, optimize):
     """One layer (WorkSize KUB = 1): the line solve replaces THOMASZ
     (AD:1758-1843), after the land fill."""
     cfg = fx.config("c3", ni=21, nj=17, nk=1, land_frac=0.3,
-                    tracers=("temperature", "lognormal", "salinity"))
+                    tracers=("temperature", "lognormal", "salinity"), **GRID)
     case = fx.make_case(cfg)
     base = _implicit_params(case, Optimize=optimize)
     worst, bitwise = _drop_in_vs_oracle(case, 4, lambda s, t: dict(base, **_dirs(s, start)))
@@ -177,7 +177,7 @@ def test_implicit_vertical1d(nk):
     """Vertical1D instances: no horizontal terms; in 3-D no line solve
     (HorizontalAdvection is skipped, AD:1704), with one layer the line solve
     still runs (AD:1758)."""
-    cfg = fx.config("c2", ni=9, nj=8, nk=nk)
+    cfg = fx.config("c2", ni=9, nj=8, nk=nk, **GRID)
     case = fx.make_case(cfg)
     base = _implicit_params(case)
     worst, _ = _drop_in_vs_oracle(case, 3, lambda s, t: dict(base, **_dirs(s, 0)),

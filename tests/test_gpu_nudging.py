@@ -15,7 +15,7 @@ import pytest
 from mohid_amd import constants as C
 from oracle import oracle as ora
 
-from .cases import make
+from .cases import GRID, make
 
 pytestmark = pytest.mark.gpu
 
@@ -163,7 +163,7 @@ def test_nudging_one_layer():
     from mohid_amd import fixtures as fx
     _, AD, _ = _mods()
     case = fx.make_case(fx.config("c3", ni=21, nj=17, nk=1, land_frac=0.3,
-                                  tracers=("temperature", "lognormal", "salinity")))
+                                  tracers=("temperature", "lognormal", "salinity"), **GRID))
     gpu = AD(case.size, case.worksize, case.geom)
     inst = ora.Instance(case, nthreads=1)
     pg = [np.array(p, order="F", copy=True) for p in case.tracers]

@@ -13,7 +13,7 @@ import pytest
 from mohid_amd import constants as C
 from oracle import oracle as ora
 
-from .cases import SMALL, make
+from .cases import GRID, SMALL, make
 
 pytestmark = pytest.mark.gpu
 
@@ -268,7 +268,7 @@ def test_device_resident_inputs_match_host_inputs():
     from mohid_amd import fixtures as fx
     AD, Transport, _ = _ad()
     cfg = fx.config("c3", ni=30, nj=26, nk=8, kfloor=(2, 8),
-                    tracers=("temperature", "lognormal"))
+                    tracers=("temperature", "lognormal"), **GRID)
     host = fx.make_case(cfg)
     dev = fx.make_case(cfg, device="cuda")
     for t in range(2):
@@ -333,7 +333,7 @@ def test_edge_shapes_match_oracle(shape):
     from mohid_amd import fixtures as fx
     AD, Transport, _ = _ad()
     shape = dict(shape)
-    case = fx.make_case(fx.config(shape.pop("base"), **shape))
+    case = fx.make_case(fx.config(shape.pop("base"), **dict(GRID, **shape)))
     T = len(case.tracers)
     ref = ora.run_case(case, 3, nthreads=1)
     got = _batched(case, 2, 1)
@@ -346,3 +346,43 @@ def test_edge_shapes_match_oracle(shape):
     for t in range(T):
         assert np.array_equal(got[t], ref[t]), t
         assert np.array_equal(drop[t], ref[t]), t
+
+
+@pytest.mark.parametrize("name", ["tvd_ragged_opt", "tvd_land_sigma"])
+def test_moving_surface_matches_oracle(name):
+    """A moving free surface: DWZ, DZZ, AreaU, AreaV, VolumeZ and VolumeZOld
+    change every step (fixtures.moving_surface), set before each step with
+    mgpu_ad_set_geometry (drop-in) and mgpu_transport_set_geometry (batched
+    handle, one graph-captured step per geometry) as the reference fetches
+    them inside every call (AD:1353-1401); bit-identical to the oracle."""
+    from mohid_amd import fixtures as fx
+    AD, Transport, _ = _ad()
+    case = make(name)
+    T = len(case.tracers)
+    steps = 4
+    geoms = fx.moving_surface(case, steps)
+    dt = case.params["DTProp"]
+    inst = ora.Instance(case, nthreads=1)
+    ref = [np.array(t, order="F", copy=True) for t in case.tracers]
+    gpu = AD(case.size, case.worksize, case.geom)
+    drop = [np.array(t, order="F", copy=True) for t in case.tracers]
+    tr = Transport(case.size, case.worksize, case.geom, T)
+    for t, p in enumerate(case.tracers):
+        tr.set_tracer(t, p)
+    for s, (geom, hy) in enumerate(geoms):
+        inst.set_geometry(geom)
+        gpu.set_geometry(geom)
+        tr.set_geometry(geom)
+        tr.set_hydro(hy, case.params)
+        tr.run(1, use_graph=True)
+        for t in range(T):
+            assert inst.step(ref[t], first_property=t == 0, now=(s + 1) * dt, hydro=hy) == 0
+            gpu(drop[t], hy, case.params, first_property=t == 0)
+            assert np.array_equal(drop[t], ref[t]), (s, t)
+    for t in range(T):
+        assert np.array_equal(tr.get_tracer(t), ref[t]), t
+    inst.close()
+    gpu.kill()
+    tr.close()
+    fixed = ora.run_case(case, steps, nthreads=1)
+    assert not np.array_equal(fixed[0], ref[0])

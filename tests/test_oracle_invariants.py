@@ -57,8 +57,12 @@ mathematics fixes:
 
 ``test_planted_mistakes_are_caught`` plants each of a list of plausible
 mistakes in the oracle (a flux sign, the upwind side, the volume ratio, a
-wrong cell volume, a Thomas index, the vertical distance index) and requires
-that at least one of these checks fails for each.
+wrong cell volume, a Thomas index, the vertical distance index, and the
+horizontal metric indexing: DUX(j) and DUX(j-1) swapped in the face
+diffusivity, DZX(i,j) or DUX for DZX(i,j-1), the same slips along i, the
+DVY line in the XX advection stencil, VolumeU from the cells j, j+1) and
+requires that at least one of these checks (with those of
+test_oracle_metrics.py) fails for each.
 """
 import ctypes as ct
 
@@ -1174,6 +1178,8 @@ def _all_checks():
     if ora.ref_available():
         from .test_oracle_ref import orlanski2d_ref_diff
         out.append(("orlanski vs reference", orlanski2d_ref_diff(1, 1e9), 0.0))
+    from .test_oracle_metrics import all_checks as metric_checks
+    out.extend(metric_checks())
     for name in sorted(SMALL):
         mass, constancy = _invariant_errs(name, steps=2)
         out.append((f"mass {name}", mass, 1e-13))
@@ -1184,7 +1190,7 @@ def _all_checks():
     return out
 
 
-@pytest.mark.parametrize("mistake", list(range(1, 22)))
+@pytest.mark.parametrize("mistake", list(range(1, 29)))
 def test_planted_mistakes_are_caught(mistake):
     L = _lib()
     L.ora_plant_mistake(mistake)

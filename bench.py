@@ -71,12 +71,17 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c3")
-    ap.add_argument("--precision", default="fp64", choices=("fp64", "fast"))
+    ap.add_argument("--precision", default="fp64", choices=("fp64", "fast", "mixed"))
+    ap.add_argument("--refinements", type=int, default=2,
+                    help="mixed precision: residual-and-correction passes per step")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision measurement beside the headline")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-window", type=int, default=384)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--kernel", default="lanes", choices=("lanes", "split", "fused"))
+    ap.add_argument("--kernel", default="auto", choices=("auto", "lanes", "split", "fused"),
+                    help="auto: the autotuner picks the lane pair or the fused walk")
     ap.add_argument("--tracers-per-launch", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--implicit", default="none", choices=("none", "x", "y"),
@@ -141,28 +146,33 @@ def main():
     _log(f"case {cfg.name}: {cfg.ni}x{cfg.nj}x{cfg.nk}, {T} tracers, {nwet} water cells "
          f"({time.time() - t0:.1f}s to build)")
 
-    prec = _lib.MGPU_PREC_FAST if args.precision == "fast" else _lib.MGPU_PREC_FP64
-    kernel = {"lanes": _lib.MGPU_KERNEL_LANES, "split": _lib.MGPU_KERNEL_SPLIT,
-              "fused": _lib.MGPU_KERNEL_FUSED}[args.kernel]
+    prec = {"fp64": _lib.MGPU_PREC_FP64, "fast": _lib.MGPU_PREC_FAST,
+            "mixed": _lib.MGPU_PREC_MIXED}[args.precision]
+    kernel = {"auto": _lib.MGPU_KERNEL_LANES, "lanes": _lib.MGPU_KERNEL_LANES,
+              "split": _lib.MGPU_KERNEL_SPLIT, "fused": _lib.MGPU_KERNEL_FUSED}[args.kernel]
     params = _with_implicit(case.params, args.implicit)
     tr = Transport(case.size, case.worksize, case.geom, T, precision=prec, kernel=kernel,
-                   tracers_per_launch=args.tracers_per_launch)
+                   tracers_per_launch=args.tracers_per_launch, refinements=args.refinements)
     tr.set_hydro(case.hydro, params)
     for t, p in enumerate(case.tracers):
         tr.set_tracer(t, p)
     use_graph = not args.no_graph
 
-    # ---- launch shape from the autotuner (lane variant), before any timing;
-    # tuning advances the state, so the tracers are reloaded afterwards
+    # ---- launch shape from the autotuner, before any timing; tuning advances
+    # the state, so the tracers are reloaded afterwards
+    from mohid_amd import autotune as AT
     tuned = None
-    if args.kernel == "lanes" and not args.no_autotune:
-        from mohid_amd.autotune import autotune
+    variant = {"auto": "lanes"}.get(args.kernel, args.kernel)
+    if args.kernel in ("auto", "lanes", "fused") and not args.no_autotune:
         t0 = time.time()
-        tuned = autotune(tr, steps=2)
+        kernels = {"auto": (AT.LANES, AT.FUSED), "lanes": (AT.LANES,),
+                   "fused": (AT.FUSED,)}[args.kernel]
+        tuned = AT.autotune(tr, steps=2, kernels=kernels)
         for t, p in enumerate(case.tracers):
             tr.set_tracer(t, p)
-        _log(f"autotuned launch shape (tracers/launch, explicit tracers/lane, column "
-             f"tracers/lane) = {tuned['shape']} in {time.time() - t0:.1f}s")
+        variant = "fused" if tuned["shape"][0] == AT.FUSED else "lanes"
+        _log(f"autotuned launch shape (kernel, tracers/launch, explicit tracers/lane, column "
+             f"tracers/lane) = {tuned['shape']} ({variant}) in {time.time() - t0:.1f}s")
 
     # ---- warm-up
     if args.warmup > 0:
@@ -201,7 +211,7 @@ def main():
     per_unit = 16.0 + 64.0 / T
     alg_bytes = per_unit * nwet * T
     achieved = alg_bytes / (trans_ms * 1e-3) / 1e9
-    kernel_key = f"transport_{args.kernel}_{args.precision}"
+    kernel_key = f"transport_{variant}_{args.precision}"
     if args.implicit != "none":
         kernel_key += "_implicit"
     traffic = _traffic_from_profiles(kernel_key, cfg.name)
@@ -209,7 +219,8 @@ def main():
          f"({nsteps_t} steps); transport {achieved:.1f} GB/s algorithmic")
 
     out = {
-        "metric": "tracer cell-updates/s (%HBM peak), 1024^2x40x8 tracers, batched GPU step",
+        "metric": "tracer cell-updates/s + %HBM peak, 1024\u00b2\u00d740\u00d78 tracers, "
+                  "mixed vs FP64 GPU",
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -219,7 +230,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision != "mixed" else "f64 rows + f32 sweeps",
         "data": "synthetic (seeded fixtures, mohid_amd/fixtures.py)",
         "config": {
             "workload": f"{cfg.name}: {cfg.ni}x{cfg.nj}x{cfg.nk}, {T} tracers, "
@@ -228,7 +239,7 @@ def main():
             "n_wet_cells": nwet,
             "precision": args.precision,
             "graph": use_graph,
-            "kernel": args.kernel,
+            "kernel": variant,
             "launch_shape": list(tuned["shape"]) if tuned else None,
             "optimize_path": bool(case.params["Optimize"]),
             "implicit_horizontal": args.implicit,
@@ -242,7 +253,10 @@ def main():
             "traffic": traffic,
             "kernel": ("line_rows_kernel + line_thomas_kernel + column_lane_kernel"
                        if args.implicit != "none"
-                       else "explicit_lane_kernel + column_lane_kernel" if args.kernel == "lanes"
+                       else "walk_lane_kernel" if variant == "fused"
+                       else ("explicit_lane_kernel + " +
+                             ("column_mixed_kernel" if args.precision == "mixed"
+                              else "column_lane_kernel")) if variant == "lanes"
                        else "explicit_kernel + vertical_kernel") + " (all tracers of one step)",
             "kernel_ms": trans_ms,
             "explicit_kernel_ms": h_ms,
@@ -253,12 +267,18 @@ def main():
         },
     }
 
+    # ---- mixed precision beside the FP64 headline: the same K steps from the
+    # same initial tracers, timed the same way, and its error against FP64
+    if args.precision == "fp64" and args.implicit == "none" and not args.no_mixed:
+        out["mixed"] = _mixed_vs_fp64(case, tr, args, value)
     # ---- CPU baseline: the oracle on a bounded sample of the same workload
     # (the same configuration on a smaller horizontal window), timed on the
     # host cores; the GPU path runs the same sample for the parity figure
     if not args.no_cpu:
         from oracle import oracle as ora
-        nthreads = min(16, os.cpu_count() or 1)
+        # every core this process may run on, capped by the host's
+        # OMP_NUM_THREADS (its CPU share) when set
+        nthreads = ora.host_threads()
         scfg = fx.config(cfg.name, ni=min(cfg.ni, args.cpu_window),
                          nj=min(cfg.nj, args.cpu_window))
         scase = fx.make_case(scfg)
@@ -297,6 +317,7 @@ def main():
         tr2.close()
         out["cpu_baseline"] = {
             "value": cval, "unit": "cell-updates/s", "cores": nthreads, "kind": "port",
+            "host": _host_cpu(),
             "sample": f"{nsteps} steps x {sT} tracers of {scfg.name} on a {scfg.ni}x{scfg.nj}"
                       f"x{scfg.nk} window ({swet} water cells) after 1 untimed step; "
                       f"oracle/advdiff_oracle.c with OpenMP",
@@ -308,6 +329,84 @@ def main():
              f"{cwall:.1f}s); gpu vs oracle max rel err {worst:.3e}")
     tr.close()
     print(json.dumps(out), flush=True)
+
+
+def _mixed_vs_fp64(case, tr64, args, value64):
+    """MGPU_PREC_MIXED with 1 and 2 correction passes on the headline case:
+    updates/s over K graph-captured steps (autotuned), and the max relative
+    error of every tracer on water cells against the FP64 handle's result
+    after the same K steps from the same initial tracers."""
+    import torch
+    from mohid_amd import _lib
+    from mohid_amd import autotune as AT
+    from mohid_amd.advection_diffusion import Transport
+    T = len(case.tracers)
+    K = args.steps
+    wp = case.hydro["WaterPoints3D"] == 1
+    zero = torch.zeros_like(case.hydro["VolumeZ"])
+    for t, p in enumerate(case.tracers):
+        tr64.set_tracer(t, p)
+    tr64.run(K, use_graph=True)
+    ref = []
+    for t in range(T):
+        o = torch.empty_like(zero)
+        tr64.get_tracer(t, o)
+        ref.append(o)
+    res = {"steps_compared": K, "passes": {}}
+    for passes in (1, 2):
+        tm = Transport(case.size, case.worksize, case.geom, T, precision=_lib.MGPU_PREC_MIXED,
+                       refinements=passes)
+        tm.set_hydro(case.hydro, case.params)
+        for t, p in enumerate(case.tracers):
+            tm.set_tracer(t, p)
+        shape = AT.autotune(tm, steps=2, kernels=(AT.LANES,))["shape"]
+        for t, p in enumerate(case.tracers):
+            tm.set_tracer(t, p)
+        tm.run(K, use_graph=True)              # capture, and the K steps compared
+        tm.status()
+        worst = 0.0
+        o = torch.empty_like(zero)
+        for t in range(T):
+            tm.get_tracer(t, o)
+            rel = torch.where(wp, torch.abs(o - ref[t]) / torch.clamp(torch.abs(ref[t]),
+                                                                      min=1e-300), zero)
+            worst = max(worst, float(torch.max(rel).item()))
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        tm.run(K, use_graph=True)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - w0
+        tm.timing(True, min(K, 10))
+        tm.run(min(K, 10), use_graph=False)
+        pm, hm, vm, _ = tm.kernel_times()
+        tm.close()
+        v = T * case.n_wet * K / wall
+        res["passes"][str(passes)] = {
+            "value": v, "ms_per_step": 1e3 * wall / K, "vs_fp64": v / value64,
+            "max_rel_err_vs_fp64": worst, "launch_shape": list(shape),
+            "prep_ms": pm, "explicit_ms": hm, "column_ms": vm}
+        _log(f"mixed, {passes} pass(es): {v:.4e} cell-updates/s ({v / value64:.3f} x FP64), "
+             f"column kernel {vm:.3f} ms, max rel err vs FP64 after {K} steps {worst:.2e}")
+    del ref
+    torch.cuda.empty_cache()
+    return res
+
+
+def _host_cpu():
+    """What the host offers the CPU baseline: its CPU model, nproc, the cores
+    this process may run on and the OMP_NUM_THREADS it sets."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "affinity_cores": len(os.sched_getaffinity(0)),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
 
 
 def _implicit_dirs(first, step):
@@ -576,7 +675,7 @@ def suite(args):
           "launch_shapes": {"fp64": si, "fast": sfa}})
     from oracle import oracle as ora
     scase = fx.make_case(fx.config("c3", ni=256, nj=256))
-    inst = ora.Instance(scase, nthreads=min(16, os.cpu_count() or 1))
+    inst = ora.Instance(scase, nthreads=ora.host_threads())
     props = [np.array(t, order="F", copy=True) for t in scase.tracers]
     dt = scase.params["DTProp"]
     c0 = time.perf_counter()
@@ -590,7 +689,7 @@ def suite(args):
     inst.close()
     emit({"measure": "oracle (CPU), implicit horizontal advection, 256x256x40 window of the "
                      "same case x 8 tracers", "value": len(scase.tracers) * scase.n_wet * ns / cw,
-          "unit": "cell-updates/s", "threads": min(16, os.cpu_count() or 1), "steps": ns})
+          "unit": "cell-updates/s", "threads": ora.host_threads(), "steps": ns})
     del scase, props
 
     # 5. large run: 2048 x 2048 x 60 sigma, 4 tracers, graphs of 50 steps

@@ -55,10 +55,15 @@ extern "C" {
 #define MGPU_PREC_FP64 0         /* reference operation order, IEEE FP64 division */
 #define MGPU_PREC_FAST 1         /* FP64 with reciprocal-multiply division (rounding-level
                                     differences from the reference's operation order) */
+#define MGPU_PREC_MIXED 2        /* FP64 right-hand side and rows, the column solve as FP64
+                                    residual-and-correction passes with FP32 Thomas sweeps
+                                    (mgpu_transport_opts.refinements passes per step) */
 
 /* ---- kernel variant of the transport ------------------------------------- */
 #define MGPU_KERNEL_AUTO 0       /* the library's choice (lanes) */
-#define MGPU_KERNEL_FUSED 1      /* one column-tile kernel per step and tracer chunk */
+#define MGPU_KERNEL_FUSED 1      /* one pass per step and tracer chunk: a lane per (column,
+                                    tracers) builds each level's explicit right-hand side
+                                    inside its column solve */
 #define MGPU_KERNEL_SPLIT 2      /* tile kernels: explicit per level, then a column solve */
 #define MGPU_KERNEL_LANES 3      /* one lane per (cell, tracer), then per (column, tracer) */
 
@@ -253,12 +258,15 @@ typedef struct mgpu_transport mgpu_transport;
 
 typedef struct {
     int32_t ntracers;
-    int32_t precision;                    /* MGPU_PREC_FP64 / MGPU_PREC_FAST */
+    int32_t precision;                    /* MGPU_PREC_FP64 / _FAST / _MIXED (lanes only) */
     int32_t docycle_method;
     int32_t kernel;                       /* MGPU_KERNEL_AUTO / _FUSED / _SPLIT / _LANES */
     int32_t tracers_per_launch;           /* 1..8 tracers per kernel launch; 0: 8 */
     int32_t explicit_tracers_per_lane;    /* lanes: 1, 2, 4 or 8 (dividing the launch); 0: 4 */
     int32_t column_tracers_per_lane;      /* lanes: 1, 2, 4 or 8 (dividing the launch); 0: 2 */
+    int32_t refinements;                  /* MGPU_PREC_MIXED: residual-and-correction passes of
+                                             the column solve per step, 1..4 (0: 2); the first
+                                             starts from the tracer before the step */
 } mgpu_transport_opts;
 
 int mgpu_transport_create(const mgpu_size3d *size, const mgpu_size3d *worksize,
@@ -290,6 +298,11 @@ int mgpu_transport_stream(mgpu_transport *t, void **stream);
 int mgpu_transport_set_lanes(mgpu_transport *t, int32_t tracers_per_launch,
                              int32_t explicit_tracers_per_lane,
                              int32_t column_tracers_per_lane);
+/* Kernel variant of the following steps: MGPU_KERNEL_LANES or _FUSED (they
+ * share their scratch; a handle created with _SPLIT keeps it, MGPU_ERR_STATE).
+ * For the autotuner; results do not depend on it.  _FUSED is not offered for
+ * MGPU_PREC_MIXED or implicit horizontal advection (MGPU_ERR_UNSUPPORTED). */
+int mgpu_transport_set_kernel(mgpu_transport *t, int32_t kernel);
 int mgpu_transport_destroy(mgpu_transport *t);
 /* Waits for the handle's work; MGPU_ERR_OPTION if a face method or limiter
  * the reference stops on was met since the last set_hydro, MGPU_ERR_PIVOT

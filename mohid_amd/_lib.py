@@ -29,6 +29,7 @@ MGPU_ERR_NOMEM = 106
 MGPU_ERR_PIVOT = 107
 MGPU_PREC_FP64 = 0
 MGPU_PREC_FAST = 1
+MGPU_PREC_MIXED = 2
 MGPU_KERNEL_AUTO = 0
 MGPU_KERNEL_FUSED = 1
 MGPU_KERNEL_SPLIT = 2
@@ -83,7 +84,7 @@ class TransportOpts(ct.Structure):
     _fields_ = [("ntracers", ct.c_int32), ("precision", ct.c_int32),
                 ("docycle_method", ct.c_int32), ("kernel", ct.c_int32),
                 ("tracers_per_launch", ct.c_int32), ("explicit_tracers_per_lane", ct.c_int32),
-                ("column_tracers_per_lane", ct.c_int32)]
+                ("column_tracers_per_lane", ct.c_int32), ("refinements", ct.c_int32)]
 
 
 # every symbol include/mohid_gpu.h declares
@@ -95,7 +96,7 @@ EXPORTS = (
     "mgpu_transport_create", "mgpu_transport_set_hydro", "mgpu_transport_set_geometry",
     "mgpu_transport_set_tracer",
     "mgpu_transport_get_tracer", "mgpu_transport_run", "mgpu_transport_tracer_ptr",
-    "mgpu_transport_set_lanes",
+    "mgpu_transport_set_lanes", "mgpu_transport_set_kernel",
     "mgpu_transport_stream", "mgpu_transport_destroy", "mgpu_transport_timing",
     "mgpu_transport_kernel_times",
     "mgpu_version", "mgpu_last_error",
@@ -152,6 +153,7 @@ def lib():
                                             ct.POINTER(ct.c_int64)]
     L.mgpu_transport_stream.argtypes = [vp, ct.POINTER(vp)]
     L.mgpu_transport_set_lanes.argtypes = [vp, ct.c_int32, ct.c_int32, ct.c_int32]
+    L.mgpu_transport_set_kernel.argtypes = [vp, ct.c_int32]
     L.mgpu_transport_destroy.argtypes = [vp]
     L.mgpu_transport_timing.argtypes = [vp, ct.c_int32, ct.c_int32]
     L.mgpu_transport_kernel_times.argtypes = [vp, ct.POINTER(ct.c_double),

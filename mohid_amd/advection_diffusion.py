@@ -266,12 +266,14 @@ class Transport:
 
     def __init__(self, size, worksize, geometry: Dict, ntracers: int, precision: int = 0,
                  docycle_method: int = 1, kernel: int = 0, tracers_per_launch: int = 0,
-                 explicit_tracers_per_lane: int = 0, column_tracers_per_lane: int = 0):
+                 explicit_tracers_per_lane: int = 0, column_tracers_per_lane: int = 0,
+                 refinements: int = 0):
         keep: list = []
         g = _geometry(geometry, keep)
         opts = L.TransportOpts(int(ntracers), int(precision), int(docycle_method),
                                int(kernel), int(tracers_per_launch),
-                               int(explicit_tracers_per_lane), int(column_tracers_per_lane))
+                               int(explicit_tracers_per_lane), int(column_tracers_per_lane),
+                               int(refinements))
         h = ct.c_void_p()
         self._size = L.Size3D(*size)
         self._ws = L.Size3D(*worksize)
@@ -343,6 +345,11 @@ class Transport:
                 L.lib().mgpu_transport_set_lanes(self._h, int(tracers_per_launch),
                                                  int(explicit_tracers_per_lane),
                                                  int(column_tracers_per_lane)))
+
+    def set_kernel(self, kernel: int) -> None:
+        """Kernel variant (MGPU_KERNEL_LANES or _FUSED); results do not depend on it."""
+        L.check("mgpu_transport_set_kernel", L.lib().mgpu_transport_set_kernel(self._h,
+                                                                               int(kernel)))
 
     def stream_handle(self) -> int:
         s = ct.c_void_p()

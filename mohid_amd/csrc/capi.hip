@@ -409,6 +409,8 @@ SharedFields make_fields(const DevHydro &h, const DevDerived &dv, const DevGeome
 }  // namespace
 
 // ---------------------------------------------------------------------------
+void mgpu_set_text(const char *msg) { snprintf(g_err, sizeof(g_err), "%s", msg); }
+
 void mgpu_set_error(const char *what, hipError_t e) {
     snprintf(g_err, sizeof(g_err), "%s: %s", what, hipGetErrorString(e));
 }
@@ -1265,8 +1267,14 @@ extern "C" int mgpu_transport_create(const mgpu_size3d *size, const mgpu_size3d 
     int rc = check_dims(size, worksize);
     if (rc) return rc;
     if (opts->ntracers < 1 || opts->ntracers > 1024) return MGPU_ERR_BADARG;
-    if (opts->precision != MGPU_PREC_FP64 && opts->precision != MGPU_PREC_FAST)
+    if (opts->precision != MGPU_PREC_FP64 && opts->precision != MGPU_PREC_FAST &&
+        opts->precision != MGPU_PREC_MIXED)
         return MGPU_ERR_BADARG;
+    if (opts->refinements < 0 || opts->refinements > 4) return MGPU_ERR_BADARG;
+    // the mixed-precision column solve runs on the lane variant
+    if (opts->precision == MGPU_PREC_MIXED && opts->kernel != MGPU_KERNEL_AUTO &&
+        opts->kernel != MGPU_KERNEL_LANES)
+        return MGPU_ERR_UNSUPPORTED;
     mgpu_transport *t = new (std::nothrow) mgpu_transport();
     if (!t) return MGPU_UNKNOWN;
     t->size = *size;
@@ -1297,9 +1305,9 @@ extern "C" int mgpu_transport_create(const mgpu_size3d *size, const mgpu_size3d 
     rc = t->geom.alloc(t->d);
     if (!rc) rc = t->der.alloc(t->d);
     const long nT = (long)opts->ntracers;
-    // scratch: tile-blocked (split, fused) or laid out like the state (lanes)
+    // scratch: tile-blocked (split) or laid out like the state (lanes, fused)
     long scr = blocked_scratch_len(t->d, opts->ntracers);
-    if (t->kernel == MGPU_KERNEL_LANES) scr = n3 * nT;
+    if (t->kernel != MGPU_KERNEL_SPLIT) scr = n3 * nT;
     if (!rc) rc = dalloc(&t->Wb, scr);
     if (!rc) rc = dalloc(&t->Gb, scr);
     if (!rc) rc = dalloc(&t->buf[0], n3 * nT);
@@ -1329,7 +1337,8 @@ extern "C" int mgpu_transport_set_hydro(mgpu_transport *t, const mgpu_ad_args *a
     const StepScalars sc = make_scalars(t->d, a, vertical1d, xzflow);
     if (sc.impH >= 0) {
         // the line solves run on the lane variant's state-shaped scratch
-        if (t->kernel != MGPU_KERNEL_LANES) return MGPU_ERR_UNSUPPORTED;
+        if (t->kernel != MGPU_KERNEL_LANES || t->opts.precision == MGPU_PREC_MIXED)
+            return MGPU_ERR_UNSUPPORTED;
         const long ns = t->d.n3() * (long)t->opts.ntracers;
         HIPCHK(hipStreamSynchronize(t->stream));
         if (sc.vert && !t->mid && (rc = dalloc(&t->mid, ns))) return rc;
@@ -1346,7 +1355,7 @@ extern "C" int mgpu_transport_set_hydro(mgpu_transport *t, const mgpu_ad_args *a
     if (rc) return rc;
     HIPCHK(hipMemsetAsync(t->der.counters, 0, sizeof(unsigned long long) * 4, t->stream));
     rc = t->der.pack(t->d, t->hydro, t->geom.KFloor, t->opts.docycle_method, t->stream,
-                     t->kernel != MGPU_KERNEL_LANES, sc.impH >= 0);
+                     t->kernel == MGPU_KERNEL_SPLIT, sc.impH >= 0);
     if (rc) return rc;
     unsigned long long cnt[4];
     HIPCHK(hipMemcpyAsync(cnt, t->der.counters, sizeof(cnt), hipMemcpyDeviceToHost, t->stream));
@@ -1423,7 +1432,7 @@ extern "C" int mgpu_transport_set_lanes(mgpu_transport *t, int32_t tracers_per_l
         !tpl_ok(column_tracers_per_lane))
         return MGPU_ERR_BADARG;
     // the tile variants' scratch is sized for the chunk chosen at creation
-    if (t->kernel != MGPU_KERNEL_LANES && tracers_per_launch != t->opts.tracers_per_launch)
+    if (t->kernel == MGPU_KERNEL_SPLIT && tracers_per_launch != t->opts.tracers_per_launch)
         return MGPU_ERR_STATE;
     HIPCHK(hipStreamSynchronize(t->stream));
     t->opts.tracers_per_launch = tracers_per_launch;
@@ -1432,6 +1441,20 @@ extern "C" int mgpu_transport_set_lanes(mgpu_transport *t, int32_t tracers_per_l
     t->chunk = tracers_per_launch ? tracers_per_launch : 8;
     if (t->chunk > t->opts.ntracers) t->chunk = t->opts.ntracers;
     drop_graphs(t);  // captured launches bake the shape in
+    return MGPU_SUCCESS;
+}
+
+extern "C" int mgpu_transport_set_kernel(mgpu_transport *t, int32_t kernel) {
+    if (!t) return MGPU_OFF_ERR;
+    const int k = kernel == MGPU_KERNEL_AUTO ? MGPU_KERNEL_LANES : kernel;
+    if (k != MGPU_KERNEL_LANES && k != MGPU_KERNEL_FUSED) return MGPU_ERR_BADARG;
+    // the two share the state-shaped scratch; the split variant's is tile-blocked
+    if (t->kernel == MGPU_KERNEL_SPLIT) return MGPU_ERR_STATE;
+    if (k == MGPU_KERNEL_FUSED && (t->opts.precision == MGPU_PREC_MIXED || t->impdir >= 0))
+        return MGPU_ERR_UNSUPPORTED;
+    HIPCHK(hipStreamSynchronize(t->stream));
+    t->kernel = k;
+    drop_graphs(t);   // captured launches bake the variant in
     return MGPU_SUCCESS;
 }
 
@@ -1471,6 +1494,8 @@ static int enqueue_steps(mgpu_transport *t, int nsteps, int cur, int dir) {
         if ((rc = mark(t))) return rc;
         TransportLaunch L;
         L.fast = t->opts.precision == MGPU_PREC_FAST;
+        if (t->opts.precision == MGPU_PREC_MIXED)
+            L.lanes.mixed = t->opts.refinements ? t->opts.refinements : 2;
         L.variant = t->kernel;
         L.n = t->opts.ntracers;
         L.stride = t->opts.ntracers;

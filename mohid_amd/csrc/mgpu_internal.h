@@ -48,6 +48,8 @@ int check_dims(const mgpu_size3d *size, const mgpu_size3d *ws);
 // Error plumbing: every HIP call goes through HIPCHK, which records the
 // message and returns MGPU_ERR_HIP from the calling entry point.
 void mgpu_set_error(const char *what, hipError_t e);
+// Sets the text mgpu_last_error returns (this thread).
+void mgpu_set_text(const char *msg);
 #define HIPCHK(x)                                   \
     do {                                            \
         hipError_t e_ = (x);                        \
@@ -218,18 +220,20 @@ int launch_prep_faces(const Dims &d, const double *visc, const double *diffv,
                       int do_v, double *cu, double *cv, double *cw, hipStream_t s);
 // Transport of n tracers P -> out.  Tracer t of cell index x lives at
 // [x * stride + t] of P and out (stride 1: one tracer per array; stride n:
-// interleaved).  Tracers go in chunks of `chunk` (<= 8) per launch.  Both
-// variants use two tile-blocked scratch arrays Wb and Gb of
-// blocked_scratch_len(d, n) doubles each (chunk c at offset
-// blocked_scratch_len(d, first tracer of c)):
-//   variant MGPU_KERNEL_FUSED: one column-tile kernel (W in Wb, G in Gb);
+// interleaved).  Tracers go in chunks of `chunk` (<= 8) per launch.
 //   variant MGPU_KERNEL_SPLIT: explicit kernel (TI into Wb), then the column
-//     kernel (W over TI in Wb, G in Gb).
+//     kernel (W over TI in Wb, G in Gb); Wb and Gb tile-blocked,
+//     blocked_scratch_len(d, n) doubles each (chunk c at offset
+//     blocked_scratch_len(d, first tracer of c)).
 //   variant MGPU_KERNEL_LANES: lane-per-(cell, tracer) explicit kernel and
 //     lane-per-(column, tracer) column kernel over `lists`; Wb and Gb are
 //     laid out like the state (n * Size doubles each, tracer t of cell x at
-//     [x * stride + t]).
-// phase: bit 1 = first kernel, bit 2 = second kernel (split only).
+//     [x * stride + t]).  With lanes.mixed the column kernel is the
+//     mixed-precision one (its float2 W/G in Gb).
+//   variant MGPU_KERNEL_FUSED: one lane-per-(column, tracer) walk that builds
+//     the explicit right-hand side inside the column solve (W in Wb, G in
+//     Gb, laid out like the state).
+// phase: bit 1 = first kernel, bit 2 = second kernel.
 // fast: reciprocal-multiply division (FP64 state and arithmetic, results
 // differ from the reference's operation order by rounding only).
 // Compact lists of the lane variant (device arrays, host counts).
@@ -250,6 +254,9 @@ int launch_build_lists(const Dims &d, const uint8_t *mask, const int32_t *kstart
 // Tracers per lane of the lane variant's kernels (0: library default).
 struct LaneShape {
     int explicit_tpl = 0, column_tpl = 0;
+    // MGPU_PREC_MIXED: FP64 residual-and-correction passes of the column
+    // solve (column_mixed_kernel); 0: the FP64 (or FAST) column kernel
+    int mixed = 0;
 };
 
 struct TransportLaunch {

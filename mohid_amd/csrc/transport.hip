@@ -109,7 +109,7 @@ int launch_transport(const TransportLaunch &L, const Dims &d, const StepScalars 
     if (L.variant != MGPU_KERNEL_FUSED && L.variant != MGPU_KERNEL_SPLIT &&
         L.variant != MGPU_KERNEL_LANES)
         return MGPU_ERR_BADARG;
-    if (L.variant == MGPU_KERNEL_LANES && (!L.lists.cells || !L.lists.cols)) return MGPU_ERR_BADARG;
+    if (L.variant != MGPU_KERNEL_SPLIT && (!L.lists.cells || !L.lists.cols)) return MGPU_ERR_BADARG;
     if (!L.Wb || !L.Gb || !L.P || !L.out) return MGPU_ERR_BADARG;
     if (sc.impH >= 0) {
         // the line solves are built on the lane variant's state-shaped scratch
@@ -139,7 +139,7 @@ int launch_transport(const TransportLaunch &L, const Dims &d, const StepScalars 
         // each chunk has its own scratch region: with the split variant all
         // first kernels run before the second ones.  The lane variant's
         // scratch is laid out like the state (tracer t at [x * stride + t]).
-        const bool lanes = L.variant == MGPU_KERNEL_LANES;
+        const bool lanes = L.variant != MGPU_KERNEL_SPLIT;
         double *Wb = L.Wb + (lanes ? t0 : blocked_scratch_len(d, t0));
         double *Gb = L.Gb + (lanes ? t0 : blocked_scratch_len(d, t0));
         const bool big = nt > 4;

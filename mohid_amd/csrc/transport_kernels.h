@@ -836,10 +836,17 @@ __device__ __forceinline__ void obc_row(const Dims &d, const StepScalars &sc,
     }
 }
 
+// The finished rows of one level for NT tracers (the mixed-precision walk).
+template <int NT>
+struct RowSet {
+    double D[NT], E[NT], F[NT], TI[NT];
+};
+
 // Row k of the column: assembles D/E/F and completes TI with the vertical
 // terms, then one forward-sweep step of THOMASZ_NewType2; slides the
 // windows up one level.  With OBC the open cells of boundary columns take
-// the open boundary rows.
+// the open boundary rows.  With `rows` the finished rows go there and no
+// sweep step is taken (Wo, Go, the carried W and G are left alone).
 template <bool OPT, int NT, bool FAST, bool OBC = false>
 __device__ __forceinline__ void column_row(const Dims &d, const StepScalars &sc,
                                            const SharedFields &f, const TracerSet &ts, long base,
@@ -847,7 +854,7 @@ __device__ __forceinline__ void column_row(const Dims &d, const StepScalars &sc,
                                            const double (&TIk)[NT], double (&Wo)[NT],
                                            double (&Go)[NT],
                                            unsigned long long *__restrict__ zero_pivots,
-                                           int &bad) {
+                                           int &bad, RowSet<NT> *rows = nullptr) {
     const long sk = d.sk;
     const int KUB = d.wKUB;
     const double dt = sc.dt;
@@ -951,6 +958,11 @@ __device__ __forceinline__ void column_row(const Dims &d, const StepScalars &sc,
                 obc_row(d, sc, f, ts.P + t, ts.s, x, base, vk, TI, D, E, F, bad);
         }
         if (land) TI = MGPU_NULL_REAL;
+        if (rows) {
+            rows->D[t] = D; rows->E[t] = E; rows->F[t] = F; rows->TI[t] = TI;
+            c.fb[t] = ft[t];
+            continue;
+        }
         // THOMASZ_NewType2 forward sweep (MF:4090-4099)
         const double aux = E + D * c.Wm[t];
         double W = 0.0, G = 0.0;
@@ -1061,89 +1073,6 @@ vertical_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts, double *__
         }
     }
     if (bad) atomicOr(badflag, 1u);
-}
-
-// ---------------------------------------------------------------------------
-// Fused variant: one block per tile walks the levels bottom-up.  At each
-// level the block computes the tile's face fluxes into LDS, then every active
-// column builds its explicit right-hand side in registers and takes one
-// forward-sweep step; W and G go to a tile-blocked scratch
-//   level k, tracer t, lane q  ->  [((tile * nK + k) * NT + t) * TC + q]
-// so every store and load of a wave is one contiguous 512-byte run.  The
-// back substitution reads them back and writes each cell's solution once.
-
-template <bool OPT, int NT, bool FAST>
-__global__ void __launch_bounds__(TC)
-column_tile_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts, double *__restrict__ Wb,
-                   double *__restrict__ Gb, int tiles_x,
-                   unsigned long long *__restrict__ zero_pivots,
-                   unsigned int *__restrict__ badflag) {
-    __shared__ double FU[NT][TY + 1][TX];
-    __shared__ double FV[NT][TY][TX + 1];
-    const long tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int i0 = d.wILB + (int)(tile % tiles_x) * TX;
-    const int j0 = d.wJLB + (int)(tile / tiles_x) * TY;
-    const int tid = threadIdx.x;
-    const int nK = d.KUB - d.KLB + 1;
-    const int KUB = d.wKUB;
-    const uint16_t *cc = f.ccnt + tile * nK, *fc = f.fcnt + tile * nK;
-    // solved columns (WaterPoints3D(i,j,KUB) == 1, MF:4086) are the sorted
-    // prefix of keys <= KUB+1; their lanes keep the same column at every level
-    const bool act = tid < cc[KUB + 1];
-    int ks = KUB + 1, tx = 0, ty = 0;
-    long base = 0;
-    if (act) {
-        const int slot = f.cperm[tile * TC + tid];
-        tx = slot % TX;
-        ty = slot / TX;
-        base = d.idx(i0 + tx, j0 + ty, 0);
-        ks = f.kstart[d.idx2(i0 + tx, j0 + ty)];
-    }
-    const long sk = d.sk;
-    const bool vflux = act && sc.vert && !sc.vertical1d && (f.mask[base + KUB * sk] & M_OPEN);
-    int bad = 0;
-    ColumnWalk<NT> c;
-#pragma unroll
-    for (int t = 0; t < NT; ++t) { c.Wm[t] = 0.0; c.Gm[t] = 0.0; }
-    double *Wt = Wb + blk(tile, nK, 0, NT, 0) + tid;
-    double *Gt = Gb + blk(tile, nK, 0, NT, 0) + tid;
-    for (int k = d.wKLB; k <= KUB; ++k) {
-        const int nc = cc[k];                        // uniform over the block
-        const int nf = sc.vertical1d ? 0 : fc[k];
-        if (nc == 0 && nf == 0) continue;
-        if (nf) tile_faces<OPT, NT, FAST>(d, sc, f, ts, tile, i0, j0, k, nf, FU, FV, bad);
-        __syncthreads();
-        if (tid < nc) {                              // implies act and ks <= k
-            if (k == ks) column_begin<OPT, NT, FAST>(d, sc, f, ts, base, ks, vflux, c, bad);
-            double TI[NT], W[NT], G[NT];
-            cell_rhs<OPT, NT, FAST>(d, sc, f, ts, base + k * sk, tx, ty, FU, FV, TI);
-            column_row<OPT, NT, FAST>(d, sc, f, ts, base, k, vflux, c, TI, W, G, zero_pivots,
-                                      bad);
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                Wt[((long)k * NT + t) * TC] = W[t];
-                Gt[((long)k * NT + t) * TC] = G[t];
-            }
-        }
-        __syncthreads();
-    }
-    if (bad) atomicOr(badflag, 1u);
-    if (!act) return;
-    for (int k = d.wKLB; k < ks && k <= KUB; ++k) {
-#pragma unroll
-        for (int t = 0; t < NT; ++t) ts.out[(base + k * sk) * ts.s + t] = MGPU_NULL_REAL;
-    }
-    double xs[NT];
-    column_ghost<NT>(d, f, ts, base, c, xs);
-    // back substitution (MF:4102-4106)
-    for (int k = KUB; k >= ks; --k) {
-        const long x = base + k * sk;
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            xs[t] = Wt[((long)k * NT + t) * TC] * xs[t] + Gt[((long)k * NT + t) * TC];
-            ts.out[x * ts.s + t] = xs[t];
-        }
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1515,6 +1444,217 @@ column_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
 }
 
 // ---------------------------------------------------------------------------
+// Fused variant: one pass per step.  One lane per (column, TPL tracers), as
+// in column_lane_kernel, walks its column upwards; at every level it builds
+// the cell's explicit right-hand side itself (cell_rhs_lane: the explicit
+// kernel's work, with the same operations) and takes the row and
+// forward-sweep step at once, so TI never goes through memory and the
+// cell's shared fields are read once for both halves.  The neighbour values
+// the horizontal faces need at level k are read at level k by the lanes of
+// neighbouring columns, which walk the same level at the same time.
+template <bool OPT, int NT, int TPL, bool FAST>
+__global__ void __launch_bounds__(256)
+walk_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
+                 const uint32_t *__restrict__ cols, long ncols, double *__restrict__ Ws,
+                 double *__restrict__ Gs, unsigned long long *__restrict__ zero_pivots,
+                 unsigned int *__restrict__ badflag) {
+    constexpr int LPC = NT / TPL;
+    const long g = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    const long n = g / LPC;
+    const bool act = n < ncols;
+    const int t = (int)(g - n * LPC) * TPL;
+    const int KUB = d.wKUB;
+    const long base = act ? (long)cols[n] : 0;  // Size index of the column's level 0
+    const long sk = d.sk, s = ts.s;
+    const int ks = act ? f.kstart[base] : KUB + 1;   // idx2(i,j) == idx(i,j,0): KLB is 0
+    const int kw = wave_min(ks);
+    if (!act) return;
+    const int i = d.ILB + (int)(base % d.sj), j = d.JLB + (int)(base / d.sj);
+    const TracerSet t1{ts.P + t, ts.out + t, s, ts.G0 ? ts.G0 + t : nullptr};
+    double *ws = Ws + t, *gs = Gs + t;
+    int bad = 0;
+    {
+        double nr[TPL];
+#pragma unroll
+        for (int u = 0; u < TPL; ++u) nr[u] = MGPU_NULL_REAL;
+        for (int k = d.wKLB; k < ks && k <= KUB; ++k) store_tpl<TPL>(t1.out + (base + k * sk) * s, nr);
+    }
+    const bool columnBuild = f.mask[base + KUB * sk] & M_OPEN;  // AD:2982/3004
+    const bool vflux = sc.vert && !sc.vertical1d && columnBuild;
+    ColumnWalk<TPL> c;
+    column_begin<OPT, TPL, FAST>(d, sc, f, t1, base, ks, vflux, c, bad);
+    constexpr int WIN = column_window(TPL);
+    double rW[WIN > 0 ? WIN : 1][TPL], rG[WIN > 0 ? WIN : 1][TPL];
+    for (int k = kw; k <= KUB; ++k) {
+        if (k < ks) continue;
+        const long x = base + k * sk;
+        double TI[TPL], W[TPL], G[TPL];
+        cell_rhs_lane<OPT, TPL, FAST>(d, sc, f, t1, x, i, j, TI, bad);
+        column_row<OPT, TPL, FAST>(d, sc, f, t1, base, k, vflux, c, TI, W, G, zero_pivots, bad);
+        const int slot = KUB - k;
+        if (slot < WIN) {
+#pragma unroll
+            for (int r = 0; r < WIN; ++r)
+                if (r == slot) {
+#pragma unroll
+                    for (int u = 0; u < TPL; ++u) { rW[r][u] = W[u]; rG[r][u] = G[u]; }
+                }
+        } else {
+            store_tpl<TPL>(ws + x * s, W);
+            store_tpl<TPL>(gs + x * s, G);
+        }
+    }
+    double xs[TPL];
+    column_ghost<TPL>(d, f, t1, base, c, xs);
+    for (int k = KUB; k >= kw; --k) {
+        if (k < ks) continue;
+        const long x = base + k * sk;
+        double W[TPL], G[TPL];
+        const int slot = KUB - k;
+        if (slot < WIN) {
+#pragma unroll
+            for (int r = 0; r < WIN; ++r)
+                if (r == slot) {
+#pragma unroll
+                    for (int u = 0; u < TPL; ++u) { W[u] = rW[r][u]; G[u] = rG[r][u]; }
+                }
+        } else {
+            load_tpl<TPL>(ws + x * s, W);
+            load_tpl<TPL>(gs + x * s, G);
+        }
+#pragma unroll
+        for (int u = 0; u < TPL; ++u) xs[u] = W[u] * xs[u] + G[u];
+        store_tpl<TPL>(t1.out + x * s, xs);
+    }
+    if (bad) atomicOr(badflag, 1u);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision: the column solve as FP64-residual iterative refinement
+// with FP32 Thomas sweeps.  The rows D/E/F and the right-hand side TI are
+// assembled in FP64 exactly as in column_lane_kernel (TI from the FP64
+// explicit kernel).  Pass p (p = 1 .. npass) walks the column upwards,
+// makes the FP64 residual r = TI - (D x(k-1) + E x(k) + F x(k+1)) of the
+// current iterate (x0 = the tracer at the start of the step, read from the
+// walk's own window; later passes read the previous pass's result) and
+// takes the FP32 forward sweep of A d = r (W and G as one float2 per
+// tracer and level, 8 bytes instead of 16); the downward pass makes
+// d by FP32 back substitution and stores x + d in FP64.  The row of the
+// first level ks takes no D term (THOMASZ starts the sweep there with
+// W = G = 0, MF:4087-4090); the ghost top row keeps its FP64 value.  The
+// error after a pass is about eps32 * (growth) times the previous pass's,
+// so one pass from the old field leaves ~1e-7 of the step's change and two
+// passes reach the FP64 solve's rounding level.
+template <bool OPT, int NT, int TPL>
+__global__ void __launch_bounds__(256)
+column_mixed_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
+                    const uint32_t *__restrict__ cols, long ncols, const double *__restrict__ TW,
+                    float2 *__restrict__ WG, int npass,
+                    unsigned long long *__restrict__ zero_pivots,
+                    unsigned int *__restrict__ badflag) {
+    constexpr int LPC = NT / TPL;
+    const long g = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    const long n = g / LPC;
+    const bool act = n < ncols;
+    const int t = (int)(g - n * LPC) * TPL;
+    const int KUB = d.wKUB;
+    const long base = act ? (long)cols[n] : 0;
+    const long sk = d.sk, s = ts.s;
+    const int ks = act ? f.kstart[base] : KUB + 1;
+    const int kw = wave_min(ks);
+    if (!act) return;
+    const TracerSet t1{ts.P + t, ts.out + t, s, ts.G0 ? ts.G0 + t : nullptr};
+    const double *tw = TW + t;
+    float2 *wg = WG + t;
+    int bad = 0;
+    {
+        double nr[TPL];
+#pragma unroll
+        for (int u = 0; u < TPL; ++u) nr[u] = MGPU_NULL_REAL;
+        for (int k = d.wKLB; k < ks && k <= KUB; ++k) store_tpl<TPL>(t1.out + (base + k * sk) * s, nr);
+    }
+    const bool columnBuild = f.mask[base + KUB * sk] & M_OPEN;  // AD:2982/3004
+    const bool vflux = sc.vert && !sc.vertical1d && columnBuild;
+    for (int pass = 0; pass < npass; ++pass) {
+        const bool first = pass == 0;
+        ColumnWalk<TPL> c;
+        column_begin<OPT, TPL, false>(d, sc, f, t1, base, ks, vflux, c, bad);
+        float Wm[TPL], Gm[TPL];
+        double xm[TPL], xk[TPL];
+#pragma unroll
+        for (int u = 0; u < TPL; ++u) { Wm[u] = 0.f; Gm[u] = 0.f; xm[u] = 0.0; }
+        if (!first) load_tpl<TPL>(t1.out + (base + ks * sk) * s, xk);
+        for (int k = kw; k <= KUB; ++k) {
+            if (k < ks) continue;
+            const long x = base + k * sk;
+            double TI[TPL], dW[TPL], dG[TPL], xp[TPL];
+            load_tpl<TPL>(tw + x * s, TI);
+            if (first) {
+#pragma unroll
+                for (int u = 0; u < TPL; ++u) { xk[u] = c.wp[u][1]; xp[u] = c.wp[u][2]; }
+            } else {
+                load_tpl<TPL>(t1.out + (x + sk) * s, xp);   // level KUB: the ghost row
+            }
+            RowSet<TPL> R;
+            column_row<OPT, TPL, false>(d, sc, f, t1, base, k, vflux, c, TI, dW, dG, zero_pivots,
+                                        bad, &R);
+#pragma unroll
+            for (int u = 0; u < TPL; ++u) {
+                double r = R.TI[u];
+                if (k > ks) r = r - R.D[u] * xm[u];
+                r = r - R.E[u] * xk[u];
+                r = r - R.F[u] * xp[u];
+                const float Df = (float)R.D[u], Ef = (float)R.E[u], Ff = (float)R.F[u];
+                const float aux = Ef + Df * Wm[u];
+                float W = 0.f, G = 0.f;
+                if (fabsf(aux) > 0.f) {
+                    const float ia = __builtin_amdgcn_rcpf(aux);
+                    W = -Ff * ia;
+                    G = ((float)r - Df * Gm[u]) * ia;
+                } else if (first) {
+                    atomicAdd(zero_pivots, 1ull);
+                }
+                wg[x * s + u] = make_float2(W, G);
+                Wm[u] = W;
+                Gm[u] = G;
+                xm[u] = xk[u];
+                xk[u] = xp[u];
+            }
+        }
+        // ghost top row: its FP64 value, no correction
+        double xs[TPL];
+        if (first) {
+            double p0[TPL];
+#pragma unroll
+            for (int u = 0; u < TPL; ++u) p0[u] = c.wp[u][1];   // P(KUB+1) after the walk
+            column_ghost<TPL>(d, f, t1, base, c, xs);
+#pragma unroll
+            for (int u = 0; u < TPL; ++u) xs[u] = xs[u] - p0[u];
+        } else {
+#pragma unroll
+            for (int u = 0; u < TPL; ++u) xs[u] = 0.0;
+        }
+        float dn[TPL];
+#pragma unroll
+        for (int u = 0; u < TPL; ++u) dn[u] = (float)xs[u];
+        for (int k = KUB; k >= kw; --k) {
+            if (k < ks) continue;
+            const long x = base + k * sk;
+            double xo[TPL];
+            load_tpl<TPL>((first ? t1.P : t1.out) + x * s, xo);
+#pragma unroll
+            for (int u = 0; u < TPL; ++u) {
+                const float2 q = wg[x * s + u];
+                dn[u] = q.x * dn[u] + q.y;
+                xo[u] = xo[u] + (double)dn[u];
+            }
+            store_tpl<TPL>(t1.out + x * s, xo);
+        }
+    }
+    if (bad) atomicOr(badflag, 1u);
+}
+
+// ---------------------------------------------------------------------------
 // Implicit horizontal advection, in two kernels.
 //
 // Rows: one lane per (listed cell, TPL tracers) builds the cell's
@@ -1836,9 +1976,37 @@ static int launch_explicit_tpl(const Dims &d, const StepScalars &sc, const Share
 template <bool OPT, int NT, int TPL, bool FAST>
 static int launch_column_tpl(const Dims &d, const StepScalars &sc, const SharedFields &f,
                              const TracerSet &ts, const SolveLists &l, double *TW, double *Gs,
-                             unsigned long long *counters, unsigned int *bad, hipStream_t s) {
+                             unsigned long long *counters, unsigned int *bad, hipStream_t s,
+                             int mixed = 0, bool walk = false) {
     const long nb = (l.ncols * (NT / TPL) + 255) / 256;
     if (nb <= 0) return MGPU_SUCCESS;
+    if (walk) {
+        // the fused variant: explicit right-hand side inside the column walk
+        // (at most 4 tracers per lane: 8 do not fit the register file)
+        if constexpr (TPL <= 4) {
+            if (f.obc.bp || f.nadv || f.xzero) return MGPU_ERR_UNSUPPORTED;
+            hipLaunchKernelGGL((walk_lane_kernel<OPT, NT, TPL, FAST>), dim3((unsigned)nb),
+                               dim3(256), 0, s, d, sc, f, ts, l.cols, l.ncols, TW, Gs, counters,
+                               bad);
+            HIPCHK(hipGetLastError());
+            return MGPU_SUCCESS;
+        } else {
+            return MGPU_ERR_BADARG;
+        }
+    }
+    if (mixed > 0) {
+        // FP64 rows and residual, FP32 sweeps (the FP64 explicit kernel's TI in TW)
+        if constexpr (FAST) {
+            return MGPU_ERR_BADARG;
+        } else {
+            if (f.obc.bp || f.nadv) return MGPU_ERR_UNSUPPORTED;
+            hipLaunchKernelGGL((column_mixed_kernel<OPT, NT, TPL>), dim3((unsigned)nb),
+                               dim3(256), 0, s, d, sc, f, ts, l.cols, l.ncols, TW,
+                               reinterpret_cast<float2 *>(Gs), mixed, counters, bad);
+            HIPCHK(hipGetLastError());
+            return MGPU_SUCCESS;
+        }
+    }
     if (f.obc.bp) {
         // open boundaries are a per-property option: the drop-in's one tracer
         if constexpr (NT == 1 && TPL == 1) {
@@ -1882,14 +2050,17 @@ template <bool OPT, int NT, bool FAST>
 static int launch_column_lanes(int want, const Dims &d, const StepScalars &sc,
                                const SharedFields &f, const TracerSet &ts, const SolveLists &l,
                                double *TW, double *Gs, unsigned long long *counters,
-                               unsigned int *bad, hipStream_t s) {
+                               unsigned int *bad, hipStream_t s, int mixed = 0,
+                               bool walk = false) {
+    if (walk && want > 4) want = 4;
     switch (lane_tpl<NT>(want, 2)) {
-        case 8: if constexpr (NT % 8 == 0) return launch_column_tpl<OPT, NT, 8, FAST>(d, sc, f, ts, l, TW, Gs, counters, bad, s); break;
-        case 4: if constexpr (NT % 4 == 0) return launch_column_tpl<OPT, NT, 4, FAST>(d, sc, f, ts, l, TW, Gs, counters, bad, s); break;
-        case 2: if constexpr (NT % 2 == 0) return launch_column_tpl<OPT, NT, 2, FAST>(d, sc, f, ts, l, TW, Gs, counters, bad, s); break;
+        case 8: if constexpr (NT % 8 == 0) return launch_column_tpl<OPT, NT, 8, FAST>(d, sc, f, ts, l, TW, Gs, counters, bad, s, mixed, walk); break;
+        case 4: if constexpr (NT % 4 == 0) return launch_column_tpl<OPT, NT, 4, FAST>(d, sc, f, ts, l, TW, Gs, counters, bad, s, mixed, walk); break;
+        case 2: if constexpr (NT % 2 == 0) return launch_column_tpl<OPT, NT, 2, FAST>(d, sc, f, ts, l, TW, Gs, counters, bad, s, mixed, walk); break;
         default: break;
     }
-    return launch_column_tpl<OPT, NT, 1, FAST>(d, sc, f, ts, l, TW, Gs, counters, bad, s);
+    return launch_column_tpl<OPT, NT, 1, FAST>(d, sc, f, ts, l, TW, Gs, counters, bad, s, mixed,
+                                               walk);
 }
 
 template <bool OPT, int NT, int TPL, bool FAST>
@@ -1972,16 +2143,20 @@ static int launch_chunk(int variant, int phase, const Dims &d, const StepScalars
         }
         if (phase & 2) {
             const int rc = launch_column_lanes<OPT, NT, FAST>(lanes.column_tpl, d, sc, f, ts,
-                                                              lists, Wb, Gb, counters, bad, s);
+                                                              lists, Wb, Gb, counters, bad, s,
+                                                              lanes.mixed);
             if (rc) return rc;
         }
         return MGPU_SUCCESS;
     }
     if (variant == MGPU_KERNEL_FUSED) {
+        // one walk per column: the explicit right-hand side inside it
+        if (lanes.mixed) return MGPU_ERR_UNSUPPORTED;
         if (phase & 1) {
-            hipLaunchKernelGGL((column_tile_kernel<OPT, NT, FAST>), dim3(tiles.x * tiles.y),
-                               dim3(TC), 0, s, d, sc, f, ts, Wb, Gb, (int)tiles.x, counters, bad);
-            HIPCHK(hipGetLastError());
+            const int rc = launch_column_lanes<OPT, NT, FAST>(lanes.column_tpl, d, sc, f, ts,
+                                                              lists, Wb, Gb, counters, bad, s, 0,
+                                                              true);
+            if (rc) return rc;
         }
         return MGPU_SUCCESS;
     }

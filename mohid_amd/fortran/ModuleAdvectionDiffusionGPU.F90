@@ -75,7 +75,7 @@ module ModuleAdvectionDiffusionGPU
     type, bind(c) :: t_mgpu_transport_opts
         integer(c_int32_t) :: ntracers, precision, docycle_method, kernel
         integer(c_int32_t) :: tracers_per_launch, explicit_tracers_per_lane
-        integer(c_int32_t) :: column_tracers_per_lane
+        integer(c_int32_t) :: column_tracers_per_lane, refinements
     end type t_mgpu_transport_opts
 
     type, bind(c) :: t_mgpu_ad_args
@@ -612,7 +612,7 @@ contains
         integer                     :: rc, n
         o%ntracers = NTracers; o%precision = Precision; o%docycle_method = Docycle_method
         o%kernel = 0; o%tracers_per_launch = 0
-        o%explicit_tracers_per_lane = 0; o%column_tracers_per_lane = 0
+        o%explicit_tracers_per_lane = 0; o%column_tracers_per_lane = 0; o%refinements = 0
         g = geometry_of(DUX, DVY, DZX, DZY, KFloorZ, AreaU, AreaV, DZZ, DWZ)
         s = to_size3d(Size); w = to_size3d(WorkSize)
         TID = 0

@@ -89,13 +89,19 @@ def test_fortran_binding_compiles(tmp_path):
 
 
 def test_autotune_candidates_are_valid_shapes():
-    from mohid_amd.autotune import candidates
+    from mohid_amd.autotune import FUSED, LANES, candidates
     for T in range(1, 13):
         c = candidates(T)
         assert c, T
-        for pl, e, col in c:
+        for kern, pl, e, col in c:
+            assert kern in (LANES, FUSED)
             chunk = min(pl or 8, T)
-            assert chunk % e == 0 and chunk % col == 0 and e <= chunk and col <= chunk
+            assert chunk % col == 0 and col <= chunk
+            if kern == LANES:
+                assert chunk % e == 0 and e <= chunk
+            else:
+                assert e == 0 and col <= 4      # the fused walk: no explicit shape
             assert pl == 0 or pl < T
         assert len(set(c)) == len(c)
-    assert (0, 4, 2) in candidates(8)
+    assert (LANES, 0, 4, 2) in candidates(8)
+    assert (FUSED, 0, 0, 2) in candidates(8)

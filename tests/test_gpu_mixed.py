@@ -1,0 +1,125 @@
+"""The mixed-precision transport (MGPU_PREC_MIXED) against the FP64 path.
+
+Mixed precision keeps the right-hand side and the rows in FP64 (the FP64
+explicit kernel and row assembly, bit for bit the reference's) and solves
+each column as FP64 residual-and-correction passes whose Thomas sweeps run in
+FP32: pass 1 corrects the tracer before the step, pass 2 corrects pass 1's
+result.  Its error against the FP64 path (which is bit-identical to the
+oracle) is therefore the FP32 sweep's rounding times the size of the
+correction: one pass leaves about 1e-7 of the step's change, two passes reach
+the FP64 solve's own rounding.
+
+* small cases: both pass counts against the oracle, after 6 steps; two
+  passes within 1e-9 relative on every water cell;
+* the stiff workload at its full shapes (512 x 512 x 50 stretched sigma,
+  layers down to 0.25 m, Kz up to 0.1 m2/s, dt 120 s, so Kz dt / dz^2 reaches
+  ~190; 8 tracers): per tracer, the max and RMS relative error against the
+  FP64 path after 10 steps, and the drift of sum(V * C) (the basin is closed
+  and divergence-free, so the FP64 path keeps it to rounding); two passes
+  within 1e-9 max relative error.
+"""
+import numpy as np
+import pytest
+
+from mohid_amd import fixtures as fx
+from oracle import oracle as ora
+
+from .cases import SMALL, make
+
+pytestmark = pytest.mark.gpu
+
+MIXED_RTOL = 1e-9       # two correction passes, max relative error vs FP64
+
+
+def _mods():
+    import torch
+    from mohid_amd import _lib as L
+    from mohid_amd.advection_diffusion import Transport
+    return torch, L, Transport
+
+
+@pytest.mark.parametrize("name", sorted(SMALL))
+def test_mixed_small_cases_against_oracle(name):
+    torch, L, Transport = _mods()
+    case = make(name, tracers=("temperature", "salinity", "lognormal", "lognormal"))
+    T = len(case.tracers)
+    steps = 6
+    ref = ora.run_case(case, steps, nthreads=ora.host_threads())
+    wp = case.hydro["WaterPoints3D"] == 1
+    worst = {}
+    for passes in (1, 2):
+        tr = Transport(case.size, case.worksize, case.geom, T, precision=L.MGPU_PREC_MIXED,
+                       refinements=passes)
+        tr.set_hydro(case.hydro, case.params)
+        for t, p in enumerate(case.tracers):
+            tr.set_tracer(t, p)
+        tr.run(steps, use_graph=True)
+        tr.status()
+        w = 0.0
+        for t in range(T):
+            got = tr.get_tracer(t)
+            w = max(w, float(np.max(np.abs(got[wp] - ref[t][wp]) /
+                                    np.maximum(np.abs(ref[t][wp]), 1e-300))))
+        tr.close()
+        worst[passes] = w
+    print(f"mixed {name}: max rel err vs oracle after {steps} steps: one pass {worst[1]:.2e}, "
+          f"two passes {worst[2]:.2e}")
+    assert worst[2] <= MIXED_RTOL
+    assert worst[2] <= worst[1]
+
+
+@pytest.mark.timeout(600)
+def test_mixed_stiff_full_shapes_against_fp64():
+    torch, L, Transport = _mods()
+    torch.cuda.set_device(0)
+    case = fx.make_case(fx.config("c4"), device="cuda")
+    T = len(case.tracers)
+    steps = 10
+    V = case.hydro["VolumeZ"]
+    wp = case.hydro["WaterPoints3D"] == 1
+    zero = torch.zeros_like(V)
+
+    def mass(c):
+        return float(torch.sum(torch.where(wp, V * c, zero)).item())
+
+    def run(prec, passes=0):
+        tr = Transport(case.size, case.worksize, case.geom, T, precision=prec,
+                       refinements=passes)
+        tr.set_hydro(case.hydro, case.params)
+        for t, p in enumerate(case.tracers):
+            tr.set_tracer(t, p)
+        tr.run(steps, use_graph=True)
+        tr.status()
+        out = []
+        for t in range(T):
+            o = torch.empty_like(V)
+            tr.get_tracer(t, o)
+            out.append(o)
+        tr.close()
+        torch.cuda.synchronize()
+        return out
+
+    m0 = [mass(p) for p in case.tracers]
+    fp64 = run(L.MGPU_PREC_FP64)
+    report = {}
+    for passes in (1, 2):
+        mixed = run(L.MGPU_PREC_MIXED, passes)
+        rows = []
+        for t in range(T):
+            rel = torch.where(wp, torch.abs(mixed[t] - fp64[t]) /
+                              torch.clamp(torch.abs(fp64[t]), min=1e-300), zero)
+            mx = float(torch.max(rel).item())
+            rms = float(torch.sqrt(torch.sum(rel * rel) / torch.sum(wp)).item())
+            drift = abs(mass(mixed[t]) - m0[t]) / abs(m0[t])
+            drift64 = abs(mass(fp64[t]) - m0[t]) / abs(m0[t])
+            rows.append((mx, rms, drift, drift64))
+            print(f"c4 mixed, {passes} pass(es), tracer {t} ({case.cfg.tracers[t]}): "
+                  f"max rel {mx:.2e}, rms rel {rms:.2e}, mass drift {drift:.1e} "
+                  f"(fp64 {drift64:.1e})", flush=True)
+        report[passes] = rows
+        del mixed
+    worst2 = max(r[0] for r in report[2])
+    print(f"c4 mixed after {steps} steps: max rel err one pass "
+          f"{max(r[0] for r in report[1]):.2e}, two passes {worst2:.2e}")
+    assert worst2 <= MIXED_RTOL
+    assert max(r[2] for r in report[2]) <= 1e-12

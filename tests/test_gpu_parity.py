@@ -298,10 +298,11 @@ def test_device_resident_inputs_match_host_inputs():
 
 
 def test_every_launch_shape_gives_identical_results():
-    """The lane variant's launch shapes (tracers per launch, tracers per lane
-    in the explicit and column kernels) change speed, never results: every
-    candidate the autotuner can pick reproduces the oracle bit for bit."""
-    from mohid_amd.autotune import autotune, candidates
+    """The launch shapes (lane pair or fused walk, tracers per launch, tracers
+    per lane in the explicit and column kernels) change speed, never results:
+    every candidate the autotuner can pick reproduces the oracle bit for
+    bit."""
+    from mohid_amd.autotune import apply, autotune, candidates
     _, Transport, _ = _ad()
     case = make("tvd_ragged_opt", tracers=("temperature", "salinity") + ("lognormal",) * 6)
     T = len(case.tracers)
@@ -309,7 +310,7 @@ def test_every_launch_shape_gives_identical_results():
     tr = Transport(case.size, case.worksize, case.geom, T)
     tr.set_hydro(case.hydro, case.params)
     for shape in candidates(T):
-        tr.set_lanes(*shape)
+        apply(tr, shape)
         for t, p in enumerate(case.tracers):
             tr.set_tracer(t, p)
         tr.run(1, use_graph=True)

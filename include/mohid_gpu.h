@@ -230,6 +230,22 @@ int mgpu_thomasz(const mgpu_size3d *size, const mgpu_size3d *worksize,
                  const double *TI, const int32_t *waterpoints, double *res,
                  int32_t dev, void *stream);
 
+/* No counterpart in the reference.  mgpu_thomasz with dev = 1 for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * solver loops, harnesses that forbid hipMalloc): all pointers are device
+ * pointers, the work is enqueued on `stream` (a hipStream_t, may be null)
+ * and the call returns without waiting.  It calls no allocating routine: its
+ * scratch is one static device array of MGPU_NOALLOC_MAX_CELLS doubles per
+ * GPU (128 MiB, held until the process ends), so Size may hold at most that
+ * many cells (MGPU_ERR_BADARG beyond, with the reason in mgpu_last_error).
+ * Every call on a GPU shares that array: calls that overlap on one GPU
+ * (other streams or threads) race and must be serialised by the caller. */
+#define MGPU_NOALLOC_MAX_CELLS 16777216
+int mgpu_thomasz_noalloc(const mgpu_size3d *size, const mgpu_size3d *worksize,
+                         const double *D, const double *E, const double *F,
+                         const double *TI, const int32_t *waterpoints, double *res,
+                         void *stream);
+
 /* THOMAS_3D (MF:455-458, its T_THOMAS form THOMAS_3D_NewType, MF:3667-3870).
  * di = 0, dj = 1: for every i in IJmin..IJmax and k in Kmin..Kmax, solve
  * along j  D(j) x(j-1) + E(j) x(j) + F(j) x(j+1) = TI(j),  j = JImin..JImax+1

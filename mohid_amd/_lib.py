@@ -34,6 +34,7 @@ MGPU_KERNEL_AUTO = 0
 MGPU_KERNEL_FUSED = 1
 MGPU_KERNEL_SPLIT = 2
 MGPU_KERNEL_LANES = 3
+MGPU_NOALLOC_MAX_CELLS = 16777216
 
 STATUS_NAMES = {
     0: "SUCCESS", 1: "OFF_ERR", -99: "UNKNOWN", 100: "ERR_HIP", 101: "ERR_BADARG",
@@ -92,7 +93,7 @@ EXPORTS = (
     "mgpu_ad_create", "mgpu_ad_set_geometry", "mgpu_ad_step", "mgpu_ad_destroy",
     "mgpu_ad_zero_pivots", "mgpu_ad_set_discharges", "mgpu_ad_unset_discharges",
     "mgpu_ad_get_fluxes", "mgpu_waterflux_z", "mgpu_compute_volumes", "mgpu_compute_areas",
-    "mgpu_thomasz", "mgpu_thomas3d", "mgpu_transport_status",
+    "mgpu_thomasz", "mgpu_thomasz_noalloc", "mgpu_thomas3d", "mgpu_transport_status",
     "mgpu_transport_create", "mgpu_transport_set_hydro", "mgpu_transport_set_geometry",
     "mgpu_transport_set_tracer",
     "mgpu_transport_get_tracer", "mgpu_transport_run", "mgpu_transport_tracer_ptr",
@@ -130,6 +131,8 @@ def lib():
     L.mgpu_ad_zero_pivots.argtypes = [vp, ct.POINTER(ct.c_uint64)]
     L.mgpu_thomasz.argtypes = [ct.POINTER(Size3D), ct.POINTER(Size3D), vp, vp, vp, vp, vp, vp,
                                ct.c_int32, vp]
+    L.mgpu_thomasz_noalloc.argtypes = [ct.POINTER(Size3D), ct.POINTER(Size3D), vp, vp, vp, vp,
+                                       vp, vp, vp]
     i32 = ct.c_int32
     L.mgpu_ad_set_discharges.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp]
     L.mgpu_ad_unset_discharges.argtypes = [vp]

@@ -1127,6 +1127,61 @@ extern "C" int mgpu_thomasz(const mgpu_size3d *size, const mgpu_size3d *worksize
 }
 
 // ---------------------------------------------------------------------------
+// THOMASZ without allocation.  The forward sweep's W lives in `scratch`, a
+// file-scope device array of MGPU_NOALLOC_MAX_CELLS doubles (128 MiB) that
+// every GPU the library runs on holds until the process ends.  In host code
+// `scratch` names a host-side stand-in, so its device address is looked up
+// once per device (with that device current) and only that address is used.
+// The kernel writes W(k) before it reads it, for every row it reads, so
+// nothing has to start at zero.  Calls that overlap on one GPU race on it.
+namespace {
+__device__ __align__(256) double scratch[MGPU_NOALLOC_MAX_CELLS];
+std::mutex g_noalloc_mu;
+double *g_noalloc_ptr[MAX_DEVICES] = {};
+
+// The calling device's address of `scratch`, or null (rc set).
+double *noalloc_scratch(int &rc) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) { mgpu_set_error("hipGetDevice", e); rc = MGPU_ERR_HIP; return nullptr; }
+    if (dev < 0 || dev >= MAX_DEVICES) { rc = MGPU_ERR_BADARG; return nullptr; }
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            mgpu_set_error("hipGetSymbolAddress(scratch)", e);
+            rc = MGPU_ERR_HIP;
+            return nullptr;
+        }
+        g_noalloc_ptr[dev] = static_cast<double *>(p);
+    }
+    return g_noalloc_ptr[dev];
+}
+}  // namespace
+
+extern "C" int mgpu_thomasz_noalloc(const mgpu_size3d *size, const mgpu_size3d *worksize,
+                                    const double *D, const double *E, const double *F,
+                                    const double *TI, const int32_t *waterpoints, double *res,
+                                    void *stream) {
+    int rc = check_dims(size, worksize);
+    if (rc) return rc;
+    if (!D || !E || !F || !TI || !waterpoints || !res) return MGPU_ERR_BADARG;
+    const Dims d = make_dims(*size, *worksize);
+    if (d.n3() > MGPU_NOALLOC_MAX_CELLS) {
+        char msg[160];
+        snprintf(msg, sizeof(msg),
+                 "mgpu_thomasz_noalloc: Size holds %ld cells, more than the %ld its scratch holds",
+                 d.n3(), (long)MGPU_NOALLOC_MAX_CELLS);
+        mgpu_set_text(msg);
+        return MGPU_ERR_BADARG;
+    }
+    double *W = noalloc_scratch(rc);
+    if (!W) return rc;
+    return launch_thomasz(d, D, E, F, TI, waterpoints, res, W, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
 extern "C" int mgpu_thomas3d(const mgpu_size3d *size, int32_t IJmin, int32_t IJmax,
                              int32_t JImin, int32_t JImax, int32_t Kmin, int32_t Kmax,
                              int32_t di, int32_t dj, const double *D, const double *E,

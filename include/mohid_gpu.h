@@ -337,6 +337,18 @@ int mgpu_transport_timing(mgpu_transport *t, int32_t enable, int32_t max_steps);
 int mgpu_transport_kernel_times(mgpu_transport *t, double *prep_ms, double *explicit_ms,
                                 double *vertical_ms, int32_t *steps);
 
+/* No counterpart in the reference: a self-check of the arithmetic the FP64
+ * kernels rely on.  They divide many numerators by one denominator (a cell
+ * volume, a stencil spacing, a Thomas pivot) by reusing the denominator's
+ * refined reciprocal, which returns the same bits as the IEEE division inside
+ * a safe exponent range and divides outright elsewhere.  This runs n seeded
+ * (numerator, denominator) pairs on the current device -- random, edge-case
+ * significands, exponents near the safe range's ends and anywhere, zeros,
+ * subnormals, infinities, NaNs -- and counts the pairs whose quotient differs
+ * from a / b in any bit (NaN matches NaN) and those that took the
+ * shared-reciprocal path. */
+int mgpu_selfcheck_xdiv(int64_t n, uint64_t seed, uint64_t *mismatches, uint64_t *fast_path);
+
 /* Library version string and the last HIP error message (thread-local). */
 const char *mgpu_version(void);
 const char *mgpu_last_error(void);

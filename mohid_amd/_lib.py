@@ -100,7 +100,7 @@ EXPORTS = (
     "mgpu_transport_set_lanes", "mgpu_transport_set_kernel",
     "mgpu_transport_stream", "mgpu_transport_destroy", "mgpu_transport_timing",
     "mgpu_transport_kernel_times",
-    "mgpu_version", "mgpu_last_error",
+    "mgpu_selfcheck_xdiv", "mgpu_version", "mgpu_last_error",
 )
 
 _lib = None
@@ -162,6 +162,8 @@ def lib():
     L.mgpu_transport_kernel_times.argtypes = [vp, ct.POINTER(ct.c_double),
                                               ct.POINTER(ct.c_double), ct.POINTER(ct.c_double),
                                               ct.POINTER(ct.c_int32)]
+    L.mgpu_selfcheck_xdiv.argtypes = [ct.c_int64, ct.c_uint64, ct.POINTER(ct.c_uint64),
+                                      ct.POINTER(ct.c_uint64)]
     L.mgpu_version.restype = ct.c_char_p
     L.mgpu_last_error.restype = ct.c_char_p
     for name in EXPORTS:

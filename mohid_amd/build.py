@@ -34,9 +34,10 @@ def check_resources(text: str) -> None:
     """Fail the build if a kernel spills to scratch memory: every kernel here
     is sized to keep its state in registers, and a spill (for instance an
     array the compiler could not keep in registers) turns into hidden HBM
-    traffic.  SGPR spills go to VGPR lanes (no memory traffic); a kernel with
-    them also reports the register scavenger's reserved emergency slot as
-    scratch, which is accepted while no VGPR spills."""
+    traffic.  SGPR spills go to VGPR lanes and VGPR spills with no scratch
+    bytes go to the accumulation registers (no memory traffic either); a
+    kernel with SGPR spills also reports the register scavenger's reserved
+    emergency slot as scratch, which is accepted while no VGPR spills."""
     name = None
     info = {}
     for line in text.splitlines():
@@ -51,7 +52,7 @@ def check_resources(text: str) -> None:
     bad = []
     for name, r in info.items():
         scratch, sspill, vspill = r.get("ScratchSize", 0), r.get("SGPRs", 0), r.get("VGPRs", 0)
-        if vspill > 0 or (scratch > 0 and sspill == 0):
+        if (vspill > 0 and scratch > 0) or (scratch > 0 and sspill == 0):
             bad.append(f"{name}: {scratch} bytes/lane scratch, {vspill} VGPRs spilled")
     if bad:
         raise RuntimeError("kernels spill to scratch memory:\n  " + "\n  ".join(bad))

@@ -22,6 +22,49 @@ struct Face4 {
     double c, d, e, f;   // C/D/E/F_flux: flux = c*P(f-2) + d*P(f-1) + e*P(f) + f*P(f+1)
 };
 
+// Reciprocal: hardware estimate refined by two Newton steps (relative error
+// within a few ulp of 1/y).  These are exactly the denominator steps of the
+// compiler's IEEE FP64 division on gfx950 (v_rcp_f64, then e = 1 - y r and
+// r = r + r e twice), so that xdiv below can reuse them.
+__device__ __forceinline__ double frcp(double y) {
+    double r = __builtin_amdgcn_rcp(y);
+    double e = fma(-y, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-y, r, 1.0);
+    return fma(r, e, r);
+}
+
+// IEEE FP64 division a / b with the denominator's work done once for many
+// numerators.  The compiler divides as
+//   y = div_scale(b), x = div_scale(a), r = rcp(y) + two Newton steps,
+//   q0 = x r, rem = fma(-y, q0, x), q = div_fmas(rem, r, q0),
+//   result = div_fixup(q, b, a);
+// the two div_scale steps return their operand unchanged and div_fmas is a
+// plain fma unless an operand, 1/b or the quotient lies near the ends of the
+// exponent range, and div_fixup changes only zero, infinite, NaN and
+// out-of-range results.  So while 2^-500 <= |b| < 2^500 and the quotient
+// estimate q0 satisfies 2^-400 <= |q0| < 2^400 (which keeps a away from
+// zero and from the ends of the range too), the quotient is
+//   q0 = a r,  q = fma(fma(-b, q0, a), r, q0)
+// with r = frcp(b), bit for bit the same; anywhere else the division itself
+// is taken.  The zero numerator goes the slow way as well (q0 = 0).
+struct Recip {
+    double b, r;
+    bool ok;
+};
+
+__device__ __forceinline__ Recip recip(double b) {
+    const double m = fabs(b);                          // false for NaN below
+    return Recip{b, frcp(b), m >= 0x1p-500 && m < 0x1p500};
+}
+
+__device__ __forceinline__ double xdiv(double a, const Recip &R) {
+    const double q0 = a * R.r;
+    const double m = fabs(q0);                         // zero, infinity and NaN fail
+    if (R.ok && m >= 0x1p-400 && m < 0x1p400) return fma(fma(-R.b, q0, a), R.r, q0);
+    return a / R.b;
+}
+
 __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
 __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
 

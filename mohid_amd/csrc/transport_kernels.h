@@ -30,16 +30,8 @@
 
 namespace {
 
-// Reciprocal for the fast path: hardware estimate refined by two Newton
-// steps (relative error within a few ulp of 1/y).  y is never zero where it
-// is used (volumes, stencil spacings, clamped gradients, non-zero pivots).
-__device__ __forceinline__ double frcp(double y) {
-    double r = __builtin_amdgcn_rcp(y);
-    double e = fma(-y, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-y, r, 1.0);
-    return fma(r, e, r);
-}
+// frcp (face.h) serves the fast path: y is never zero where it is used
+// (volumes, stencil spacings, clamped gradients, non-zero pivots).
 
 // ---------------------------------------------------------------------------
 // Masks.  A row k of a water column is "inert" when it is land and touches no
@@ -362,6 +354,10 @@ __device__ __forceinline__ void hface_flux(const Dims &d, const StepScalars &sc,
         const double s1 = du[2] + du[1];
         const double s0 = up ? du[1] + du[0] : du[2] + du[3];
         const double i1 = FAST ? frcp(s1) : 0.0, i0 = FAST ? frcp(s0) : 0.0;
+        // several tracers share the spacings: divide through their reciprocals
+        constexpr bool XR = !FAST && NT >= 2;
+        const Recip R1 = XR ? recip(s1) : Recip{s1, 0.0, false};
+        const Recip R0 = XR ? recip(s0) : Recip{s0, 0.0, false};
         const long c0m2 = cell[0], c0m1 = cell[1], c0p0 = cell[2], c0p1 = cell[3];
         const long cfar = up ? (inl[0] ? c0m2 : c0m1) : (inl[3] ? c0p1 : c0p0);
 #pragma unroll
@@ -374,6 +370,9 @@ __device__ __forceinline__ void hface_flux(const Dims &d, const StepScalars &sc,
             if (FAST) {
                 dc = clamp_min_value((a - b) * i1);
                 rr = (b - pf) * i0 * frcp(dc);
+            } else if (XR) {
+                dc = clamp_min_value(xdiv(a - b, R1));
+                rr = xdiv(b - pf, R0) / dc;
             } else {
                 dc = clamp_min_value((a - b) / s1);
                 rr = (b - pf) / s0 / dc;
@@ -1210,7 +1209,9 @@ __device__ __forceinline__ void cell_row_lane(const Dims &d, const StepScalars &
     const bool open = mk & M_OPEN;
     const double ivk = FAST ? frcp(vk) : 0.0;
     const double dtv = dt * ivk;
-    const double vv = open ? (FAST ? f.Vold[x] * ivk : f.Vold[x] / vk) : 0.0;
+    // every quotient by V below shares its reciprocal (IEEE results, xdiv)
+    const Recip Rv = FAST ? Recip{vk, 0.0, false} : recip(vk);
+    const double vv = open ? (FAST ? f.Vold[x] * ivk : xdiv(f.Vold[x], Rv)) : 0.0;
     const bool hz = !sc.vertical1d;
     const bool cfu0 = hz && (mk & M_CFU);
     const bool cfu1 = hz && (f.mask[x + d.sj] & M_CFU);
@@ -1218,8 +1219,10 @@ __device__ __forceinline__ void cell_row_lane(const Dims &d, const StepScalars &
     const bool cfv1 = hz && !sc.xzflow && (f.mask[x + 1] & M_CFV);
     const double au0 = cfu0 ? f.cu[x] : 0.0, au1 = cfu1 ? f.cu[x + d.sj] : 0.0;
     const double av0 = cfv0 ? f.cv[x] : 0.0, av1 = cfv1 ? f.cv[x + 1] : 0.0;
-    const double wu0 = FAST ? au0 * dtv : au0 * dt / vk, wu1 = FAST ? au1 * dtv : au1 * dt / vk;
-    const double wv0 = FAST ? av0 * dtv : av0 * dt / vk, wv1 = FAST ? av1 * dtv : av1 * dt / vk;
+    const double wu0 = FAST ? au0 * dtv : xdiv(au0 * dt, Rv);
+    const double wu1 = FAST ? au1 * dtv : xdiv(au1 * dt, Rv);
+    const double wv0 = FAST ? av0 * dtv : xdiv(av0 * dt, Rv);
+    const double wv1 = FAST ? av1 * dtv : xdiv(av1 * dt, Rv);
     const double bv0 = av0 * dt, bv1 = av1 * dt;
     const long s = ts.s;
     // the four advective face fluxes (MF:10534-10894, 10642-10699); for the
@@ -1258,7 +1261,7 @@ __device__ __forceinline__ void cell_row_lane(const Dims &d, const StepScalars &
     // of the cell, and E0 of VolumeVariation
     double dt2 = 0.0, E0 = 1.0;
     if constexpr (IMP >= 0) {
-        dt2 = FAST ? dt * frcp(vk) : dt / vk;
+        dt2 = FAST ? dt * frcp(vk) : xdiv(dt, Rv);
         if (k == d.wKUB && open) E0 = 1.0 + dt2 * f.Qz[x + d.sk];
         disch_e(f, x, open, dt, vk, E0);
     }
@@ -1270,19 +1273,19 @@ __device__ __forceinline__ void cell_row_lane(const Dims &d, const StepScalars &
         disch_ti(f, x, open, pk, dt, vk, TI);
         if (cfu0) {
             const double g = pk - P[(x - d.sj) * s];
-            TI = ex ? TI - au0 * g * dt / vk : TI - wu0 * g;
+            TI = ex ? TI - xdiv(au0 * g * dt, Rv) : TI - wu0 * g;
         }
         if (cfu1) {
             const double g = P[(x + d.sj) * s] - pk;
-            TI = ex ? TI + au1 * g * dt / vk : TI + wu1 * g;
+            TI = ex ? TI + xdiv(au1 * g * dt, Rv) : TI + wu1 * g;
         }
         if (cfv0) {
             const double g = pk - P[(x - 1) * s];
-            TI = ex ? TI - bv0 * g / vk : TI - wv0 * g;
+            TI = ex ? TI - xdiv(bv0 * g, Rv) : TI - wv0 * g;
         }
         if (cfv1) {
             const double g = P[(x + 1) * s] - pk;
-            TI = ex ? TI + bv1 * g / vk : TI + wv1 * g;
+            TI = ex ? TI + xdiv(bv1 * g, Rv) : TI + wv1 * g;
         }
         const bool au = IMP != 0, av = IMP != 1;   // explicit directions
         // with nudging the advective terms gather in AdvectionTi, from zero
@@ -1295,10 +1298,10 @@ __device__ __forceinline__ void cell_row_lane(const Dims &d, const StepScalars &
             if (av && cfv0) A = A + fv0[t] * dtv;
             if (av && cfv1) A = A - fv1[t] * dtv;
         } else {
-            if (au && cfu0) A = A + fu0[t] * dt / vk;
-            if (au && cfu1) A = A - fu1[t] * dt / vk;
-            if (av && cfv0) A = A + fv0[t] * dt / vk;
-            if (av && cfv1) A = A - fv1[t] * dt / vk;
+            if (au && cfu0) A = A + xdiv(fu0[t] * dt, Rv);
+            if (au && cfu1) A = A - xdiv(fu1[t] * dt, Rv);
+            if (av && cfv0) A = A + xdiv(fv0[t] * dt, Rv);
+            if (av && cfv1) A = A - xdiv(fv1[t] * dt, Rv);
         }
         if (nud) f.nadv[x * s + t] = A;
         else TI = A;
@@ -1317,14 +1320,179 @@ __device__ __forceinline__ void cell_row_lane(const Dims &d, const StepScalars &
     }
 }
 
+// ---------------------------------------------------------------------------
+// The optimised path's explicit right-hand side with every load first.
+// cell_row_lane reads each face's stencil inside the branches that decide
+// whether the face is computed, so one lane waits on memory once per face and
+// tracer; here the lane loads the whole cross stencil of its cell (two cells
+// each way along i and j: tracer values, volumes, masks and spacings), the
+// fluxes and the conductances in one go, clamping the positions outside Size
+// to the cell itself, and then computes from registers.  The arithmetic is
+// that of cell_row_lane<OPT = true, IMP = -1> and hface_flux's optimised
+// face, operation for operation: VolumeVariation (AD:3993-4001), the
+// optimised diffusion (AD:5239-5246, 5285-5292) and the explicit advection
+// (AD:4560-4576, 4930-4946) with ComputeAdvection1D_TVD_SuperBee_2's faces
+// (MF:10642-10699).
+
+// Five cells of a line through the cell (offsets -2 .. 2) for TPL tracers.
+template <int TPL>
+struct Line5 {
+    double p[5][TPL];
+    double v[5], du[5];
+    bool in[5], op[5];
+};
+
+template <int TPL>
+__device__ __forceinline__ void gather_line(const Dims &d, const SharedFields &f,
+                                            const TracerSet &ts, long x, long st, int pos,
+                                            int lo, int hi, const double *du_line, long du_st,
+                                            Line5<TPL> &L) {
+#pragma unroll
+    for (int o = 0; o < 5; ++o) {
+        const bool in = pos - 2 + o >= lo && pos - 2 + o <= hi;
+        const long c = in ? x + (long)(o - 2) * st : x;   // always inside Size
+        // unconditional loads (a load under `in` would wait before the next)
+        const double du = du_line[in ? (long)(o - 2) * du_st : 0];
+        const uint8_t m = f.mask[c];
+        const double v = f.V[c];
+        load_tpl<TPL>(ts.P + c * ts.s, L.p[o]);
+        L.in[o] = in;
+        L.op[o] = in && (m & M_OPEN);
+        L.v[o] = in ? v : 1.0;
+        L.du[o] = in ? du : 0.0;
+    }
+}
+
+// hface_flux's optimised face over window o0 .. o0+3 of a gathered line.
+template <int TPL, bool FAST>
+__device__ __forceinline__ void face_opt(const StepScalars &sc, const Line5<TPL> &L, int o0,
+                                         double Q, double (&fl)[TPL]) {
+    const bool o1 = o0 == 1;
+    const double v1 = o1 ? L.v[2] : L.v[1], v2 = o1 ? L.v[3] : L.v[2];
+    const double du0 = o1 ? L.du[1] : L.du[0], du1 = o1 ? L.du[2] : L.du[1];
+    const double du2 = o1 ? L.du[3] : L.du[2], du3 = o1 ? L.du[4] : L.du[3];
+    const bool op0 = o1 ? L.op[1] : L.op[0], op1 = o1 ? L.op[2] : L.op[1];
+    const bool op2 = o1 ? L.op[3] : L.op[2], op3 = o1 ? L.op[4] : L.op[3];
+    const bool in0 = o1 ? L.in[1] : L.in[0], in3 = o1 ? L.in[4] : L.in[3];
+    const bool up = Q > 0.0;
+    const bool nearb = up ? !op0 : !op3;
+    const double vup = up ? v1 : v2;
+    const double cr = FAST ? Q * sc.dt * frcp(vup) : Q * sc.dt / vup;
+    const double s1 = du2 + du1;
+    const double s0 = up ? du1 + du0 : du2 + du3;
+    const double i1 = FAST ? frcp(s1) : 0.0, i0 = FAST ? frcp(s0) : 0.0;
+    constexpr bool XR = !FAST && TPL >= 2;
+    const Recip R1 = XR ? recip(s1) : Recip{s1, 0.0, false};
+    const Recip R0 = XR ? recip(s0) : Recip{s0, 0.0, false};
+    const bool act = op1 && op2;
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+        const double q0 = o1 ? L.p[1][t] : L.p[0][t], q1 = o1 ? L.p[2][t] : L.p[1][t];
+        const double q2 = o1 ? L.p[3][t] : L.p[2][t], q3 = o1 ? L.p[4][t] : L.p[3][t];
+        const double p1 = q1, p2 = q2;
+        const double pf = up ? (in0 ? q0 : q1) : (in3 ? q3 : q2);
+        const double a = up ? p2 : p1, b = up ? p1 : p2;   // downstream, upstream
+        double dc, rr;
+        if (FAST) {
+            dc = clamp_min_value((a - b) * i1);
+            rr = (b - pf) * i0 * frcp(dc);
+        } else if (XR) {
+            dc = clamp_min_value(xdiv(a - b, R1));
+            rr = xdiv(b - pf, R0) / dc;
+        } else {
+            dc = clamp_min_value((a - b) / s1);
+            rr = (b - pf) / s0 / dc;
+        }
+        const double th = 0.5 * superbee_theta(rr) * (1.0 - cr);
+        const double Dc = nearb ? (up ? Q : 0.0) : (up ? Q * (1.0 - th) : Q * th);
+        const double Ec = nearb ? (up ? 0.0 : Q) : (up ? Q * th : Q * (1.0 - th));
+        fl[t] = act ? Dc * p1 + Ec * p2 : 0.0;
+    }
+}
+
+template <int TPL, bool FAST>
+__device__ __forceinline__ void cell_rhs_opt(const Dims &d, const StepScalars &sc,
+                                             const SharedFields &f, const TracerSet &ts, long x,
+                                             int i, int j, double (&TIo)[TPL]) {
+    const double dt = sc.dt;
+    // ---- every load of the cell first
+    Line5<TPL> LI, LJ;   // along i (V faces) and along j (U faces)
+    gather_line<TPL>(d, f, ts, x, 1, i, d.ILB, d.IUB, f.DVY + d.idx2(i, j), 1, LI);
+    gather_line<TPL>(d, f, ts, x, d.sj, j, d.JLB, d.JUB, f.DUX + d.idx2(i, j), d.sj, LJ);
+    const uint8_t mk = f.mask[x], mu1 = f.mask[x + d.sj], mv1 = f.mask[x + 1];
+    const double vold = f.Vold[x];
+    const double qu0 = f.Qx[x], qu1 = f.Qx[x + d.sj], qv0 = f.Qy[x], qv1 = f.Qy[x + 1];
+    const double cu0 = f.cu[x], cu1 = f.cu[x + d.sj], cv0 = f.cv[x], cv1 = f.cv[x + 1];
+    // ---- then the arithmetic
+    const double vk = LI.v[2];
+    const bool open = mk & M_OPEN;
+    const double ivk = FAST ? frcp(vk) : 0.0;
+    const double dtv = dt * ivk;
+    const Recip Rv = FAST ? Recip{vk, 0.0, false} : recip(vk);
+    const double vv = open ? (FAST ? vold * ivk : xdiv(vold, Rv)) : 0.0;
+    const bool hz = !sc.vertical1d;
+    const bool cfu0 = hz && (mk & M_CFU);
+    const bool cfu1 = hz && (mu1 & M_CFU);
+    const bool cfv0 = hz && !sc.xzflow && (mk & M_CFV);
+    const bool cfv1 = hz && !sc.xzflow && (mv1 & M_CFV);
+    const double au0 = cfu0 ? cu0 : 0.0, au1 = cfu1 ? cu1 : 0.0;
+    const double av0 = cfv0 ? cv0 : 0.0, av1 = cfv1 ? cv1 : 0.0;
+    const double wu0 = FAST ? au0 * dtv : 0.0, wu1 = FAST ? au1 * dtv : 0.0;
+    const double wv0 = FAST ? av0 * dtv : 0.0, wv1 = FAST ? av1 * dtv : 0.0;
+    const double bv0 = av0 * dt, bv1 = av1 * dt;
+    double fu0[TPL], fu1[TPL], fv0[TPL], fv1[TPL];
+    face_opt<TPL, FAST>(sc, LJ, 0, qu0, fu0);
+    face_opt<TPL, FAST>(sc, LJ, 1, qu1, fu1);
+    face_opt<TPL, FAST>(sc, LI, 0, qv0, fv0);
+    face_opt<TPL, FAST>(sc, LI, 1, qv1, fv1);
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+        const double pk = LI.p[2][t];
+        double TI = open ? pk * vv : pk;
+        disch_ti(f, x, open, pk, dt, vk, TI);
+        if (cfu0) {
+            const double g = pk - LJ.p[1][t];
+            TI = FAST ? TI - wu0 * g : TI - xdiv(au0 * g * dt, Rv);
+        }
+        if (cfu1) {
+            const double g = LJ.p[3][t] - pk;
+            TI = FAST ? TI + wu1 * g : TI + xdiv(au1 * g * dt, Rv);
+        }
+        if (cfv0) {
+            const double g = pk - LI.p[1][t];
+            TI = FAST ? TI - wv0 * g : TI - xdiv(bv0 * g, Rv);
+        }
+        if (cfv1) {
+            const double g = LI.p[3][t] - pk;
+            TI = FAST ? TI + wv1 * g : TI + xdiv(bv1 * g, Rv);
+        }
+        if (FAST) {
+            if (cfu0) TI = TI + fu0[t] * dtv;
+            if (cfu1) TI = TI - fu1[t] * dtv;
+            if (cfv0) TI = TI + fv0[t] * dtv;
+            if (cfv1) TI = TI - fv1[t] * dtv;
+        } else {
+            if (cfu0) TI = TI + xdiv(fu0[t] * dt, Rv);
+            if (cfu1) TI = TI - xdiv(fu1[t] * dt, Rv);
+            if (cfv0) TI = TI + xdiv(fv0[t] * dt, Rv);
+            if (cfv1) TI = TI - xdiv(fv1[t] * dt, Rv);
+        }
+        TIo[t] = TI;
+    }
+}
+
 template <bool OPT, int TPL, bool FAST>
 __device__ __forceinline__ void cell_rhs_lane(const Dims &d, const StepScalars &sc,
                                               const SharedFields &f, const TracerSet &ts,
                                               long x, int i, int j, double (&TIo)[TPL],
                                               int &bad) {
-    double unused[TPL];
-    cell_row_lane<OPT, TPL, FAST, -1>(d, sc, f, ts, x, i, j, 0, TIo, unused, unused, unused,
-                                      bad);
+    if constexpr (OPT) {
+        cell_rhs_opt<TPL, FAST>(d, sc, f, ts, x, i, j, TIo);
+    } else {
+        double unused[TPL];
+        cell_row_lane<OPT, TPL, FAST, -1>(d, sc, f, ts, x, i, j, 0, TIo, unused, unused, unused,
+                                          bad);
+    }
 }
 
 // Explicit kernel of the lane variant: NT / TPL lanes per cell; lane

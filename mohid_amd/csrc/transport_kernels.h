@@ -1363,10 +1363,12 @@ __device__ __forceinline__ void gather_line(const Dims &d, const SharedFields &f
     }
 }
 
-// hface_flux's optimised face over window o0 .. o0+3 of a gathered line.
-template <int TPL, bool FAST>
+// hface_flux's optimised face over window o0 .. o0+3 of a gathered line;
+// with COEF its D and E coefficients go to fl and fe instead of the flux.
+template <int TPL, bool FAST, bool COEF = false>
 __device__ __forceinline__ void face_opt(const StepScalars &sc, const Line5<TPL> &L, int o0,
-                                         double Q, double (&fl)[TPL]) {
+                                         double Q, double (&fl)[TPL],
+                                         double (*fe)[TPL] = nullptr) {
     const bool o1 = o0 == 1;
     const double v1 = o1 ? L.v[2] : L.v[1], v2 = o1 ? L.v[3] : L.v[2];
     const double du0 = o1 ? L.du[1] : L.du[0], du1 = o1 ? L.du[2] : L.du[1];
@@ -1406,14 +1408,24 @@ __device__ __forceinline__ void face_opt(const StepScalars &sc, const Line5<TPL>
         const double th = 0.5 * superbee_theta(rr) * (1.0 - cr);
         const double Dc = nearb ? (up ? Q : 0.0) : (up ? Q * (1.0 - th) : Q * th);
         const double Ec = nearb ? (up ? 0.0 : Q) : (up ? Q * th : Q * (1.0 - th));
-        fl[t] = act ? Dc * p1 + Ec * p2 : 0.0;
+        if constexpr (COEF) {
+            fl[t] = act ? Dc : 0.0;
+            (*fe)[t] = act ? Ec : 0.0;
+        } else {
+            fl[t] = act ? Dc * p1 + Ec * p2 : 0.0;
+        }
     }
 }
 
-template <int TPL, bool FAST>
+// IMP >= 0: horizontal advection along j (0) or i (1) is implicit, and the
+// cell's tridiagonal row along it goes to Do/Eo/Fo as in cell_row_lane.
+template <int TPL, bool FAST, int IMP = -1>
 __device__ __forceinline__ void cell_rhs_opt(const Dims &d, const StepScalars &sc,
                                              const SharedFields &f, const TracerSet &ts, long x,
-                                             int i, int j, double (&TIo)[TPL]) {
+                                             int i, int j, int k, double (&TIo)[TPL],
+                                             double (*Do)[TPL] = nullptr,
+                                             double (*Eo)[TPL] = nullptr,
+                                             double (*Fo)[TPL] = nullptr) {
     const double dt = sc.dt;
     // ---- every load of the cell first
     Line5<TPL> LI, LJ;   // along i (V faces) and along j (U faces)
@@ -1423,6 +1435,7 @@ __device__ __forceinline__ void cell_rhs_opt(const Dims &d, const StepScalars &s
     const double vold = f.Vold[x];
     const double qu0 = f.Qx[x], qu1 = f.Qx[x + d.sj], qv0 = f.Qy[x], qv1 = f.Qy[x + 1];
     const double cu0 = f.cu[x], cu1 = f.cu[x + d.sj], cv0 = f.cv[x], cv1 = f.cv[x + 1];
+    const double qz1 = IMP >= 0 ? f.Qz[x + d.sk] : 0.0;
     // ---- then the arithmetic
     const double vk = LI.v[2];
     const bool open = mk & M_OPEN;
@@ -1440,11 +1453,23 @@ __device__ __forceinline__ void cell_rhs_opt(const Dims &d, const StepScalars &s
     const double wu0 = FAST ? au0 * dtv : 0.0, wu1 = FAST ? au1 * dtv : 0.0;
     const double wv0 = FAST ? av0 * dtv : 0.0, wv1 = FAST ? av1 * dtv : 0.0;
     const double bv0 = av0 * dt, bv1 = av1 * dt;
+    // the advective face fluxes; for the implicit direction the faces' D
+    // (fu*/fv*) and E (eu*/ev*) coefficients
     double fu0[TPL], fu1[TPL], fv0[TPL], fv1[TPL];
-    face_opt<TPL, FAST>(sc, LJ, 0, qu0, fu0);
-    face_opt<TPL, FAST>(sc, LJ, 1, qu1, fu1);
-    face_opt<TPL, FAST>(sc, LI, 0, qv0, fv0);
-    face_opt<TPL, FAST>(sc, LI, 1, qv1, fv1);
+    double eu0[TPL], eu1[TPL], ev0[TPL], ev1[TPL];
+    face_opt<TPL, FAST, IMP == 0>(sc, LJ, 0, qu0, fu0, &eu0);
+    face_opt<TPL, FAST, IMP == 0>(sc, LJ, 1, qu1, fu1, &eu1);
+    face_opt<TPL, FAST, IMP == 1>(sc, LI, 0, qv0, fv0, &ev0);
+    face_opt<TPL, FAST, IMP == 1>(sc, LI, 1, qv1, fv1, &ev1);
+    // implicit rows: dt/V as the reference's DT1 = DT2 = DTProp / VolumeZ of
+    // the cell, and E0 of VolumeVariation
+    double dt2 = 0.0, E0 = 1.0;
+    if constexpr (IMP >= 0) {
+        dt2 = FAST ? dt * frcp(vk) : xdiv(dt, Rv);
+        if (k == d.wKUB && open) E0 = 1.0 + dt2 * qz1;
+        disch_e(f, x, open, dt, vk, E0);
+    }
+    const bool au = IMP != 0, av = IMP != 1;   // explicit directions
 #pragma unroll
     for (int t = 0; t < TPL; ++t) {
         const double pk = LI.p[2][t];
@@ -1467,17 +1492,28 @@ __device__ __forceinline__ void cell_rhs_opt(const Dims &d, const StepScalars &s
             TI = FAST ? TI + wv1 * g : TI + xdiv(bv1 * g, Rv);
         }
         if (FAST) {
-            if (cfu0) TI = TI + fu0[t] * dtv;
-            if (cfu1) TI = TI - fu1[t] * dtv;
-            if (cfv0) TI = TI + fv0[t] * dtv;
-            if (cfv1) TI = TI - fv1[t] * dtv;
+            if (au && cfu0) TI = TI + fu0[t] * dtv;
+            if (au && cfu1) TI = TI - fu1[t] * dtv;
+            if (av && cfv0) TI = TI + fv0[t] * dtv;
+            if (av && cfv1) TI = TI - fv1[t] * dtv;
         } else {
-            if (cfu0) TI = TI + xdiv(fu0[t] * dt, Rv);
-            if (cfu1) TI = TI - xdiv(fu1[t] * dt, Rv);
-            if (cfv0) TI = TI + xdiv(fv0[t] * dt, Rv);
-            if (cfv1) TI = TI - xdiv(fv1[t] * dt, Rv);
+            if (au && cfu0) TI = TI + xdiv(fu0[t] * dt, Rv);
+            if (au && cfu1) TI = TI - xdiv(fu1[t] * dt, Rv);
+            if (av && cfv0) TI = TI + xdiv(fv0[t] * dt, Rv);
+            if (av && cfv1) TI = TI - xdiv(fv1[t] * dt, Rv);
         }
         TIo[t] = TI;
+        if constexpr (IMP >= 0) {
+            const bool c0 = IMP == 0 ? cfu0 : cfv0, c1 = IMP == 0 ? cfu1 : cfv1;
+            const double d0 = IMP == 0 ? fu0[t] : fv0[t], e0 = IMP == 0 ? eu0[t] : ev0[t];
+            const double d1 = IMP == 0 ? fu1[t] : fv1[t], e1 = IMP == 0 ? eu1[t] : ev1[t];
+            double E = E0;
+            if (c0) E = E - e0 * dt2;
+            if (c1) E = E + d1 * dt2;
+            (*Do)[t] = c0 ? 0.0 - d0 * dt2 : 0.0;
+            (*Eo)[t] = E;
+            (*Fo)[t] = c1 ? 0.0 + e1 * dt2 : 0.0;
+        }
     }
 }
 
@@ -1487,7 +1523,7 @@ __device__ __forceinline__ void cell_rhs_lane(const Dims &d, const StepScalars &
                                               long x, int i, int j, double (&TIo)[TPL],
                                               int &bad) {
     if constexpr (OPT) {
-        cell_rhs_opt<TPL, FAST>(d, sc, f, ts, x, i, j, TIo);
+        cell_rhs_opt<TPL, FAST>(d, sc, f, ts, x, i, j, 0, TIo);
     } else {
         double unused[TPL];
         cell_row_lane<OPT, TPL, FAST, -1>(d, sc, f, ts, x, i, j, 0, TIo, unused, unused, unused,
@@ -1585,6 +1621,208 @@ column_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
             store_tpl<TPL>(tw + x * s, W);
             store_tpl<TPL>(gs + x * s, G);
         }
+    }
+    double xs[TPL];
+    column_ghost<TPL>(d, f, t1, base, c, xs);
+    for (int k = KUB; k >= kw; --k) {
+        if (k < ks) continue;
+        const long x = base + k * sk;
+        double W[TPL], G[TPL];
+        const int slot = KUB - k;
+        if (slot < WIN) {
+#pragma unroll
+            for (int r = 0; r < WIN; ++r)
+                if (r == slot) {
+#pragma unroll
+                    for (int u = 0; u < TPL; ++u) { W[u] = rW[r][u]; G[u] = rG[r][u]; }
+                }
+        } else {
+            load_tpl<TPL>(tw + x * s, W);
+            load_tpl<TPL>(gs + x * s, G);
+        }
+#pragma unroll
+        for (int u = 0; u < TPL; ++u) xs[u] = W[u] * xs[u] + G[u];
+        store_tpl<TPL>(t1.out + x * s, xs);
+    }
+    if (bad) atomicOr(badflag, 1u);
+}
+
+// ---------------------------------------------------------------------------
+// Pipelined column walk for the optimised path without open boundary rows or
+// discharges (the batched transport).  column_lane_kernel issues each
+// level's loads inside the branches of column_row, so a lane waits on memory
+// several times per level; here all of a level's loads (TI, the flux and the
+// conductances of its faces, and the window's next level) are issued one
+// level ahead, unconditionally at clamped positions, and the level is then
+// computed from registers with column_row's optimised arithmetic, operation
+// for operation (AD:2708-3140; THOMASZ_NewType2's forward sweep,
+// MF:4090-4099).
+template <int NT>
+struct LevelIn {
+    double TI[NT], p3[NT];
+    double qz1, cwB, cwT, v3, du3;
+    uint8_t m3;
+    bool in3;
+};
+
+template <int NT>
+__device__ __forceinline__ void level_load(const Dims &d, const StepScalars &sc,
+                                           const SharedFields &f, const TracerSet &ts,
+                                           const double *tw, long base, int k, LevelIn<NT> &L) {
+    const long sk = d.sk, x = base + k * sk;
+    load_tpl<NT>((sc.hreset ? ts.P : tw) + x * ts.s, L.TI);
+    // k <= WorkSize KUB, so x + sk is inside Size
+    L.qz1 = f.Qz[x + sk];
+    L.cwB = f.cw[x];
+    L.cwT = f.cw[x + sk];
+    const int kn = k + 3;
+    L.in3 = kn >= d.KLB && kn <= d.KUB;
+    const long y = L.in3 ? base + kn * sk : x;
+    L.v3 = f.V[y];
+    L.du3 = f.DWZ[y];
+    L.m3 = f.mask[y];
+    load_tpl<NT>(ts.P + y * ts.s, L.p3);
+}
+
+template <int NT, bool FAST>
+__device__ __forceinline__ void column_row_pipe(const StepScalars &sc, int k, int KUB, bool vflux,
+                                                ColumnWalk<NT> &c, const LevelIn<NT> &L,
+                                                double (&Wo)[NT], double (&Go)[NT],
+                                                unsigned long long *__restrict__ zero_pivots,
+                                                int &bad) {
+    const double dt = sc.dt;
+    const double vk = c.wv[1];
+    const uint8_t mk = c.wm[1];
+    VCoef ft[NT];
+    {
+        const bool o4[4] = {(c.wm[0] & M_OPEN) != 0, (c.wm[1] & M_OPEN) != 0,
+                            (c.wm[2] & M_OPEN) != 0, (c.wm[3] & M_OPEN) != 0};
+        vface_coefs<true, NT, FAST>(sc, c.wp, c.wv, c.wdu, o4, L.qz1, vflux && k < KUB, ft, bad);
+    }
+    const double dtk = FAST ? dt * frcp(vk) : dt / vk;
+    double E0 = 1.0;
+    if (k == KUB && (mk & M_OPEN) && !sc.hreset) E0 = 1.0 + dtk * L.qz1;
+    const bool cfwB = sc.vert && (mk & M_CFW);
+    const bool cfwT = sc.vert && k < KUB && (c.wm[2] & M_CFW);
+    const double a2 = cfwB ? (FAST ? L.cwB * dtk : L.cwB * dt / vk) : 0.0;
+    const double a1 = cfwT ? (FAST ? L.cwT * dtk : L.cwT * dt / vk) : 0.0;
+    const bool land = mk & M_LAND;
+    const bool vdif = !(mk & M_SDV);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const double pk = c.wp[t][1], pm = c.wp[t][0], pp = c.wp[t][2];
+        double TI = L.TI[t];
+        double E = E0, D = 0.0, F = 0.0;
+        if (cfwB && vdif) {
+            if (sc.thV > 0.0) { D = D - a2; E = E + a2; }
+            else TI = TI - a2 * (pk - pm);
+        }
+        if (cfwT && vdif) {
+            if (sc.thV > 0.0) { E = E + a1; F = F - a1; }
+            else TI = TI + a1 * (pp - pk);
+        }
+        if (sc.vert && !sc.vertical1d) {
+            if (sc.impAdvV) {
+                if (cfwB) { D = D - c.fb[t].d * dtk; E = E - c.fb[t].e * dtk; }
+                if (cfwT) { E = E + ft[t].d * dtk; F = F + ft[t].e * dtk; }
+            } else if (FAST) {
+                if (cfwB) TI = TI + (c.fb[t].d * pm + c.fb[t].e * pk) * dtk;
+                if (cfwT) TI = TI - (ft[t].d * pk + ft[t].e * pp) * dtk;
+            } else {
+                if (cfwB) TI = TI + (c.fb[t].d * pm + c.fb[t].e * pk) * dt / vk;
+                if (cfwT) TI = TI - (ft[t].d * pk + ft[t].e * pp) * dt / vk;
+            }
+        }
+        if (land) TI = MGPU_NULL_REAL;
+        // THOMASZ_NewType2 forward sweep (MF:4090-4099)
+        const double aux = E + D * c.Wm[t];
+        double W = 0.0, G = 0.0;
+        if (fabs(aux) > 0.0) {
+            if (FAST) {
+                const double ia = frcp(aux);
+                W = -F * ia;
+                G = (TI - D * c.Gm[t]) * ia;
+            } else {
+                W = -F / aux;
+                G = (TI - D * c.Gm[t]) / aux;
+            }
+        } else {
+            atomicAdd(zero_pivots, 1ull);
+        }
+        Wo[t] = W;
+        Go[t] = G;
+        c.Wm[t] = W;
+        c.Gm[t] = G;
+        c.fb[t] = ft[t];
+    }
+    // slide the windows up one level
+#pragma unroll
+    for (int o = 0; o < 3; ++o) { c.wv[o] = c.wv[o + 1]; c.wdu[o] = c.wdu[o + 1]; c.wm[o] = c.wm[o + 1]; }
+    c.wv[3] = L.in3 ? L.v3 : 1.0;
+    c.wdu[3] = L.in3 ? L.du3 : 0.0;
+    c.wm[3] = L.in3 ? L.m3 : (uint8_t)0;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        c.wp[t][0] = c.wp[t][1]; c.wp[t][1] = c.wp[t][2]; c.wp[t][2] = c.wp[t][3];
+        c.wp[t][3] = L.in3 ? L.p3[t] : 0.0;
+    }
+}
+
+template <int NT, int TPL, bool FAST>
+__global__ void __launch_bounds__(256)
+column_pipe_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
+                   const uint32_t *__restrict__ cols, long ncols, double *__restrict__ TW,
+                   double *__restrict__ Gs, unsigned long long *__restrict__ zero_pivots,
+                   unsigned int *__restrict__ badflag) {
+    constexpr int LPC = NT / TPL;
+    const long g = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    const long n = g / LPC;
+    const bool act = n < ncols;
+    const int t = (int)(g - n * LPC) * TPL;
+    const int KUB = d.wKUB;
+    const long base = act ? (long)cols[n] : 0;  // Size index of the column's level 0
+    const long sk = d.sk, s = ts.s;
+    const int ks = act ? f.kstart[base] : KUB + 1;   // idx2(i,j) == idx(i,j,0): KLB is 0
+    const int kw = wave_min(ks);
+    if (!act) return;
+    const TracerSet t1{ts.P + t, ts.out + t, s, ts.G0 ? ts.G0 + t : nullptr};
+    double *tw = TW + t, *gs = Gs + t;
+    int bad = 0;
+    {
+        double nr[TPL];
+#pragma unroll
+        for (int u = 0; u < TPL; ++u) nr[u] = MGPU_NULL_REAL;
+        for (int k = d.wKLB; k < ks && k <= KUB; ++k) store_tpl<TPL>(t1.out + (base + k * sk) * s, nr);
+    }
+    const bool columnBuild = f.mask[base + KUB * sk] & M_OPEN;  // AD:2982/3004
+    const bool vflux = sc.vert && !sc.vertical1d && columnBuild;
+    ColumnWalk<TPL> c;
+    column_begin<true, TPL, FAST>(d, sc, f, t1, base, ks, vflux, c, bad);
+    constexpr int WIN = column_window(TPL);
+    double rW[WIN > 0 ? WIN : 1][TPL], rG[WIN > 0 ? WIN : 1][TPL];
+    LevelIn<TPL> cur;
+    level_load<TPL>(d, sc, f, t1, tw, base, ks, cur);
+    for (int k = kw; k <= KUB; ++k) {
+        if (k < ks) continue;
+        const long x = base + k * sk;
+        // the next level's loads go out before this level's arithmetic
+        LevelIn<TPL> nxt;
+        if (k < KUB) level_load<TPL>(d, sc, f, t1, tw, base, k + 1, nxt);
+        double W[TPL], G[TPL];
+        column_row_pipe<TPL, FAST>(sc, k, KUB, vflux, c, cur, W, G, zero_pivots, bad);
+        const int slot = KUB - k;
+        if (slot < WIN) {
+#pragma unroll
+            for (int r = 0; r < WIN; ++r)
+                if (r == slot) {
+#pragma unroll
+                    for (int u = 0; u < TPL; ++u) { rW[r][u] = W[u]; rG[r][u] = G[u]; }
+                }
+        } else {
+            store_tpl<TPL>(tw + x * s, W);
+            store_tpl<TPL>(gs + x * s, G);
+        }
+        cur = nxt;
     }
     double xs[TPL];
     column_ghost<TPL>(d, f, t1, base, c, xs);
@@ -1850,7 +2088,10 @@ line_rows_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
     const TracerSet tl{ts.P + t, ts.out + t, s, nullptr};
     int bad = 0;
     double TI[TPL], D[TPL], E[TPL], F[TPL];
-    cell_row_lane<OPT, TPL, FAST, AX>(d, sc, f, tl, x, c.i, c.j, c.k, TI, D, E, F, bad);
+    if constexpr (OPT)
+        cell_rhs_opt<TPL, FAST, AX>(d, sc, f, tl, x, c.i, c.j, c.k, TI, &D, &E, &F);
+    else
+        cell_row_lane<OPT, TPL, FAST, AX>(d, sc, f, tl, x, c.i, c.j, c.k, TI, D, E, F, bad);
     const bool nul = !sc.vert && (f.mask[x] & M_LAND);
 #pragma unroll
     for (int u = 0; u < TPL; ++u)
@@ -2185,6 +2426,16 @@ static int launch_column_tpl(const Dims &d, const StepScalars &sc, const SharedF
             return MGPU_SUCCESS;
         }
         return MGPU_ERR_UNSUPPORTED;
+    }
+    if constexpr (OPT) {
+        // the optimised path without discharges: the pipelined walk
+        if (!f.disch.ptr) {
+            hipLaunchKernelGGL((column_pipe_kernel<NT, TPL, FAST>), dim3((unsigned)nb),
+                               dim3(256), 0, s, d, sc, f, ts, l.cols, l.ncols, TW, Gs, counters,
+                               bad);
+            HIPCHK(hipGetLastError());
+            return MGPU_SUCCESS;
+        }
     }
     hipLaunchKernelGGL((column_lane_kernel<OPT, NT, TPL, FAST>), dim3((unsigned)nb), dim3(256), 0,
                        s, d, sc, f, ts, l.cols, l.ncols, TW, Gs, counters, bad);

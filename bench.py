@@ -256,6 +256,7 @@ def main():
                        else "walk_lane_kernel" if variant == "fused"
                        else ("explicit_lane_kernel + " +
                              ("column_mixed_kernel" if args.precision == "mixed"
+                              else "column_pipe_kernel" if case.params["Optimize"]
                               else "column_lane_kernel")) if variant == "lanes"
                        else "explicit_kernel + vertical_kernel") + " (all tracers of one step)",
             "kernel_ms": trans_ms,

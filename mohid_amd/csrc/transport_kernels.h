@@ -1768,6 +1768,14 @@ __device__ __forceinline__ void column_row_pipe(const StepScalars &sc, int k, in
     }
 }
 
+// Levels below the register window whose W and G stay in LDS in the
+// pipelined walk, by tracers per lane: sized so that the blocks the register
+// file admits per CU (4, 3 and 2 for 1, 2 and 4 tracers per lane) fit the
+// CU's 160 KB of LDS.
+__host__ __device__ constexpr int column_lds_window(int tpl) {
+    return tpl == 1 ? 10 : (tpl == 2 ? 6 : (tpl == 4 ? 4 : 0));
+}
+
 template <int NT, int TPL, bool FAST>
 __global__ void __launch_bounds__(256)
 column_pipe_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
@@ -1800,6 +1808,10 @@ column_pipe_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
     column_begin<true, TPL, FAST>(d, sc, f, t1, base, ks, vflux, c, bad);
     constexpr int WIN = column_window(TPL);
     double rW[WIN > 0 ? WIN : 1][TPL], rG[WIN > 0 ? WIN : 1][TPL];
+    // the next LW levels down keep W and G in LDS, one (W, G) pair per
+    // (level, tracer) and lane; a lane touches only its own entries
+    constexpr int LW = column_lds_window(TPL);
+    __shared__ double2 lwg[LW > 0 ? LW * TPL : 1][256];
     LevelIn<TPL> cur;
     level_load<TPL>(d, sc, f, t1, tw, base, ks, cur);
     for (int k = kw; k <= KUB; ++k) {
@@ -1818,6 +1830,10 @@ column_pipe_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
 #pragma unroll
                     for (int u = 0; u < TPL; ++u) { rW[r][u] = W[u]; rG[r][u] = G[u]; }
                 }
+        } else if (slot < WIN + LW) {
+#pragma unroll
+            for (int u = 0; u < TPL; ++u)
+                lwg[(slot - WIN) * TPL + u][threadIdx.x] = make_double2(W[u], G[u]);
         } else {
             store_tpl<TPL>(tw + x * s, W);
             store_tpl<TPL>(gs + x * s, G);
@@ -1838,6 +1854,13 @@ column_pipe_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
 #pragma unroll
                     for (int u = 0; u < TPL; ++u) { W[u] = rW[r][u]; G[u] = rG[r][u]; }
                 }
+        } else if (slot < WIN + LW) {
+#pragma unroll
+            for (int u = 0; u < TPL; ++u) {
+                const double2 q = lwg[(slot - WIN) * TPL + u][threadIdx.x];
+                W[u] = q.x;
+                G[u] = q.y;
+            }
         } else {
             load_tpl<TPL>(tw + x * s, W);
             load_tpl<TPL>(gs + x * s, G);

@@ -942,6 +942,12 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     }
     // the step's own initialisation of TI (AD:1660-1671) reaches the ghost rows
     if (first) HIPCHK(hipMemsetAsync(ad->tig, 0, b3, s));
+    // The column solve may write PROP in place when nothing after it needs
+    // PROP from before the solve and no column reads another's PROP during
+    // it: explicit horizontal advection, no open boundary (its rows read the
+    // neighbours), no cell fluxes (their explicit parts use PROP before the
+    // solve).  Each column reads only its own cells, before it writes them.
+    const bool inplace = sc.impH < 0 && !a->ReferenceProp && !cfl;
     TransportLaunch L;
     L.variant = MGPU_KERNEL_LANES;
     L.lists = ad->der.lists();
@@ -949,7 +955,7 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     L.stride = 1;
     L.chunk = 1;
     L.P = P;
-    L.out = ad->out;
+    L.out = inplace ? P : ad->out;
     L.Wb = ad->Wb;
     L.Gb = ad->Gb;
     L.counters = ad->der.counters;
@@ -973,7 +979,8 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
         rc = launch_transport(L, d, sc, f, s);
     }
     if (rc) return rc;
-    // TI = null_real on land (AD:1753), kept for the next call
+    // TI = null_real on land (AD:1753), kept for the next call where a later
+    // call reads it: at the ghost rows
     rc = launch_land_fill(d, ad->der.mask, ad->tig, s);
     if (!rc && sc.impH >= 0) rc = launch_copy_lines(d, sc.impH, ad->out, P, stop, s);
     // the cell fluxes' explicit parts, with PROP before the solve (AD:1458-1468
@@ -993,7 +1000,7 @@ extern "C" int mgpu_ad_step(mgpu_ad *ad, const mgpu_ad_args *a, double *prop) {
     if (!rc && orl)
         rc = launch_orlanski_exterior(d, ad->hydro.bp, ad->hydro.open, ad->geom.KFloor, ad->mid,
                                       P, stop, s);
-    if (!rc && (sc.impH < 0 || sc.hreset))
+    if (!rc && !inplace && (sc.impH < 0 || sc.hreset))
         rc = launch_copy_back(d, ad->der.mask, ad->out, P, stop, s);
     if (rc) return rc;
     // AD:1874-1880, on the solved PROP

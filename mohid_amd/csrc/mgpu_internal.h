@@ -336,7 +336,8 @@ int launch_interleave(long n3, const double *src, double *dst, long stride, int 
 // unless *stop is non-zero.
 int launch_copy_lines(const Dims &d, int axis, const double *src, double *dst,
                       const unsigned long long *stop, hipStream_t s);
-// dst[x] = null_real where the mask has M_LAND, over Size.
+// dst[x] = null_real where the mask has M_LAND, on the Size's last plane
+// along k, j and i (the ghost rows of the drop-in's TI record).
 int launch_land_fill(const Dims &d, const uint8_t *mask, double *dst, hipStream_t s);
 // THOMAS_3D lines (di 0: along j, IJ = i; di 1: along i, IJ = j) on arrays
 // spanning sz; bounds already checked.  pivots counts lines with a zero pivot.

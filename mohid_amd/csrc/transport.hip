@@ -290,11 +290,28 @@ __global__ void copy_lines_kernel(Dims d, int axis, const double *__restrict__ s
     }
 }
 
-__global__ void land_fill_kernel(long n3, const uint8_t *__restrict__ mask,
+// TI = null_real on land (AD:1753) in the drop-in's record of TI, on the
+// Size's last plane along k, j and i: the ghost rows, the only places a later
+// call reads it (the column walk's top KUB+1, the line solves' ends JUB+1 and
+// IUB+1).
+__global__ void land_fill_kernel(Dims d, const uint8_t *__restrict__ mask,
                                  double *__restrict__ dst) {
-    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n3;
-         q += (long)gridDim.x * blockDim.x)
-        if (mask[q] & M_LAND) dst[q] = MGPU_NULL_REAL;
+    const long nI = d.IUB - d.ILB + 1, nJ = d.JUB - d.JLB + 1, nK = d.KUB - d.KLB + 1;
+    const long pk = nI * nJ, pj = nI * nK, pi = nJ * nK;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < pk + pj + pi;
+         q += (long)gridDim.x * blockDim.x) {
+        long x;
+        if (q < pk) {
+            x = q + (nK - 1) * d.sk;                                   // k = KUB
+        } else if (q < pk + pj) {
+            const long r = q - pk;
+            x = (r % nI) + (nJ - 1) * d.sj + (r / nI) * d.sk;          // j = JUB
+        } else {
+            const long r = q - pk - pj;
+            x = (nI - 1) + (r % nJ) * d.sj + (r / nJ) * d.sk;          // i = IUB
+        }
+        if (mask[x] & M_LAND) dst[x] = MGPU_NULL_REAL;
+    }
 }
 }  // namespace
 
@@ -775,10 +792,11 @@ int launch_copy_lines(const Dims &d, int axis, const double *src, double *dst,
 }
 
 int launch_land_fill(const Dims &d, const uint8_t *mask, double *dst, hipStream_t s) {
-    const long n3 = d.n3();
-    long blocks = (n3 + 255) / 256;
+    const long nI = d.IUB - d.ILB + 1, nJ = d.JUB - d.JLB + 1, nK = d.KUB - d.KLB + 1;
+    const long n = nI * nJ + nI * nK + nJ * nK;
+    long blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(land_fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s, n3, mask, dst);
+    hipLaunchKernelGGL(land_fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s, d, mask, dst);
     HIPCHK(hipGetLastError());
     return MGPU_SUCCESS;
 }

@@ -1586,12 +1586,6 @@ column_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
     const TracerSet t1{ts.P + t, ts.out + t, s, ts.G0 ? ts.G0 + t : nullptr};
     double *tw = TW + t, *gs = Gs + t;
     int bad = 0;
-    {
-        double nr[TPL];
-#pragma unroll
-        for (int u = 0; u < TPL; ++u) nr[u] = MGPU_NULL_REAL;
-        for (int k = d.wKLB; k < ks && k <= KUB; ++k) store_tpl<TPL>(t1.out + (base + k * sk) * s, nr);
-    }
     const bool columnBuild = f.mask[base + KUB * sk] & M_OPEN;  // AD:2982/3004
     const bool vflux = sc.vert && !sc.vertical1d && columnBuild;
     ColumnWalk<TPL> c;
@@ -1643,6 +1637,14 @@ column_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
 #pragma unroll
         for (int u = 0; u < TPL; ++u) xs[u] = W[u] * xs[u] + G[u];
         store_tpl<TPL>(t1.out + x * s, xs);
+    }
+    {
+        // the inert land rows below the column's first row solve to null_real
+        // (AD:1753); stored last, so that out may be P itself
+        double nr[TPL];
+#pragma unroll
+        for (int u = 0; u < TPL; ++u) nr[u] = MGPU_NULL_REAL;
+        for (int k = d.wKLB; k < ks && k <= KUB; ++k) store_tpl<TPL>(t1.out + (base + k * sk) * s, nr);
     }
     if (bad) atomicOr(badflag, 1u);
 }
@@ -1796,12 +1798,6 @@ column_pipe_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
     const TracerSet t1{ts.P + t, ts.out + t, s, ts.G0 ? ts.G0 + t : nullptr};
     double *tw = TW + t, *gs = Gs + t;
     int bad = 0;
-    {
-        double nr[TPL];
-#pragma unroll
-        for (int u = 0; u < TPL; ++u) nr[u] = MGPU_NULL_REAL;
-        for (int k = d.wKLB; k < ks && k <= KUB; ++k) store_tpl<TPL>(t1.out + (base + k * sk) * s, nr);
-    }
     const bool columnBuild = f.mask[base + KUB * sk] & M_OPEN;  // AD:2982/3004
     const bool vflux = sc.vert && !sc.vertical1d && columnBuild;
     ColumnWalk<TPL> c;
@@ -1868,6 +1864,14 @@ column_pipe_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
 #pragma unroll
         for (int u = 0; u < TPL; ++u) xs[u] = W[u] * xs[u] + G[u];
         store_tpl<TPL>(t1.out + x * s, xs);
+    }
+    {
+        // the inert land rows below the column's first row solve to null_real
+        // (AD:1753); stored last, so that out may be P itself
+        double nr[TPL];
+#pragma unroll
+        for (int u = 0; u < TPL; ++u) nr[u] = MGPU_NULL_REAL;
+        for (int k = d.wKLB; k < ks && k <= KUB; ++k) store_tpl<TPL>(t1.out + (base + k * sk) * s, nr);
     }
     if (bad) atomicOr(badflag, 1u);
 }

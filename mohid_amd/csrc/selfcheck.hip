@@ -77,7 +77,8 @@ extern "C" int mgpu_selfcheck_xdiv(int64_t n, uint64_t seed, uint64_t *mismatche
     }
     unsigned long long h[2] = {0, 0};
     if (e == hipSuccess) e = hipMemcpy(h, cnt, sizeof(h), hipMemcpyDeviceToHost);
-    hipFree(cnt);
+    const hipError_t ef = hipFree(cnt);
+    if (e == hipSuccess) e = ef;
     if (e != hipSuccess) { mgpu_set_error("mgpu_selfcheck_xdiv", e); return MGPU_ERR_HIP; }
     *mismatches = h[0];
     *fast_path = h[1];

@@ -16,7 +16,7 @@ from mohid_amd import fixtures as fx
 
 pytestmark = pytest.mark.gpu
 
-MASS_RTOL = 1e-12       # relative drift of sum(V * C) over a few steps
+MASS_RTOL = 1e-12       # relative drift of sum(V * C): rounding only, the basins are closed
 CONST_RTOL = 1e-12      # max |C - c| / c on water points
 
 
@@ -184,12 +184,13 @@ def test_full_size_c2_200_steps_matches_oracle():
 
 @pytest.mark.timeout(600)
 def test_full_size_c5_invariants_and_window():
-    """The large run's shape: 2048 x 2048 x 60 sigma (bathymetry 2-500 m),
-    25 % land, 4 tracers, TVD-SuperBee, dt 20 s, through the batched handle:
-    50 steps captured as one hipGraph keep every tracer's sum(V * C) and move
-    every tracer; a constant field stays constant over 50 more steps.  Then a
-    256 x 256 window of the same configuration runs 20 graph-captured steps
-    bit-identical to the oracle."""
+    """The large run as configured: 2048 x 2048 x 60 sigma (bathymetry
+    2-500 m), 25 % land, 4 tracers, TVD-SuperBee, dt 20 s, through the batched
+    handle for 500 steps in hipGraph-captured chunks of 50: after every chunk
+    each tracer's sum(V * C) is kept, and every tracer has moved; a constant
+    field stays constant over 50 more steps.  Then a 256 x 256 window of the
+    same configuration runs 20 graph-captured steps bit-identical to the
+    oracle."""
     import time
     from oracle import oracle as ora
     torch, L, _, Transport = _mods()
@@ -207,20 +208,21 @@ def test_full_size_c5_invariants_and_window():
     for t, p in enumerate(case.tracers):
         tr.set_tracer(t, p)
         m0.append(_mass(torch, V, wp, p))
-    tr.run(50, use_graph=True)
-    tr.status()
     out = torch.empty_like(case.tracers[0])
-    drift = []
-    for t in range(T):
-        tr.get_tracer(t, out)
-        torch.cuda.synchronize()
-        moved = torch.max(torch.abs(torch.where(wp, out - case.tracers[t],
-                                                torch.zeros_like(out)))).item()
-        assert moved > 1e-6, (t, moved)
-        drift.append(abs(_mass(torch, V, wp, out) - m0[t]) / abs(m0[t]))
-    print(f"c5 batched, 50 steps: mass drift per tracer {['%.1e' % d for d in drift]}",
-          flush=True)
-    assert max(drift) <= MASS_RTOL, drift
+    for chunk in range(10):
+        tr.run(50, use_graph=True)
+        tr.status()
+        drift = []
+        for t in range(T):
+            tr.get_tracer(t, out)
+            torch.cuda.synchronize()
+            moved = torch.max(torch.abs(torch.where(wp, out - case.tracers[t],
+                                                    torch.zeros_like(out)))).item()
+            assert moved > 1e-6, (chunk, t, moved)
+            drift.append(abs(_mass(torch, V, wp, out) - m0[t]) / abs(m0[t]))
+        print(f"c5 batched, {50 * (chunk + 1)} steps: mass drift per tracer "
+              f"{['%.1e' % d for d in drift]}", flush=True)
+        assert max(drift) <= MASS_RTOL, (chunk, drift)
     const = torch.where(wp, torch.full_like(out, 7.0), case.tracers[0])
     tr.set_tracer(0, const)
     tr.run(50, use_graph=True)

@@ -44,6 +44,29 @@ def _log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def _copy_ceiling():
+    """A measured HBM ceiling beside the published peak: a 2 GiB device-to-
+    device copy, HIP events around 5 copies after 2 warm-ups, counting the
+    bytes read and written."""
+    import torch
+    n = 256 * 1024 * 1024                      # doubles: 2 GiB per buffer
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(2):
+        b.copy_(a)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / 5
+    del a, b
+    torch.cuda.empty_cache()
+    return {"GBps": 2 * n * 8 / (ms * 1e-3) / 1e9, "method": "2 GiB device copy, read + write"}
+
+
 def _traffic_from_profiles(kernel_key: str, workload: str):
     """HBM bytes per step of the transport kernels measured with rocprofv3
     --pmc (FETCH_SIZE, WRITE_SIZE), as recorded in profiles/*.md (a JSON block
@@ -265,6 +288,7 @@ def main():
             "alg_bytes_per_launch": alg_bytes,
             "alg_bytes_model": f"(16 + 64/T) B x T x N_wet, T={T}",
             "prep_kernel_ms": prep_ms,
+            "copy_ceiling": _copy_ceiling(),
         },
     }
 

@@ -6,17 +6,23 @@
 // Software/MOHIDBase1/ModuleFunctions.F90 = MF) is done by
 //
 //   pack_masks      six int32 masks -> one byte per cell, plus per column the
-//                   lowest row that is not an inert land row (`kstart`);
+//                   lowest row that is not an inert land row (`kstart`), and
+//                   compact lists of the solved cells and columns;
 //   prep_faces      face conductances Kh*A/dx (U, V faces) and
 //                   Kz*DUX*DVY/DZZ (W faces), AD:2364-2675 + AD:1514-1619;
-//   explicit (H)    per level, the explicit right-hand side TI of every water
-//                   cell (AD:3966-4021, 4368-4953, 5156-5365), face fluxes
-//                   computed once per face into LDS;
-//   vertical (V)    one lane per water column, walking k upward: the rows
-//                   D/E/F (AD:2708-3140), the THOMASZ_NewType2 forward sweep
-//                   (MF:4080-4100), then the back substitution (MF:4102-4106).
-// Both kernels advance up to MAXT tracers per launch, so the shared fields
-// and every tracer-independent quantity are loaded/computed once for all.
+//   explicit        the explicit right-hand side TI of every solved cell
+//                   (AD:3966-4021, 4368-4953, 5156-5365): by default one lane
+//                   per (cell, group of tracers) over the cell list, the
+//                   cell's whole cross stencil loaded before any arithmetic
+//                   (explicit_lane_kernel); the tile variant computes each
+//                   face once into LDS (explicit_kernel);
+//   column          one lane per (water column, group of tracers), walking k
+//                   upward: the rows D/E/F (AD:2708-3140), the
+//                   THOMASZ_NewType2 forward sweep (MF:4080-4100), then the
+//                   back substitution (MF:4102-4106); by default each level's
+//                   loads go out one level ahead (column_pipe_kernel).
+// The kernels advance up to MAXT tracers per launch, so the shared fields
+// and every tracer-independent quantity are loaded/computed once per lane.
 //
 // The reference assembles TI/D/E/F by scattering face contributions into
 // whole 3-D arrays; here every cell gathers the contributions of its own row

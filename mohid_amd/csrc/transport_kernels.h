@@ -1203,12 +1203,16 @@ __device__ __forceinline__ int wave_min(int v) {
 //   D = 0 - Dflux(below) * dt/V,  F = 0 + Eflux(above) * dt/V,
 //   E = E0 - Eflux(below) * dt/V + Dflux(above) * dt/V,
 // E0 being VolumeVariation's diagonal (AD:3994-4001).
+//
+// This is the non-optimised path (ComputeAdvection1D_V2 faces, any method and
+// limiter, NoAdvFlux, nudging); the optimised path is cell_rhs_opt.
 template <bool OPT, int TPL, bool FAST, int IMP>
 __device__ __forceinline__ void cell_row_lane(const Dims &d, const StepScalars &sc,
                                               const SharedFields &f, const TracerSet &ts,
                                               long x, int i, int j, int k, double (&TIo)[TPL],
                                               double (&Do)[TPL], double (&Eo)[TPL],
                                               double (&Fo)[TPL], int &bad) {
+    static_assert(!OPT, "the optimised path is cell_rhs_opt");
     const double dt = sc.dt;
     const uint8_t mk = f.mask[x];
     const double vk = f.V[x];
@@ -1229,7 +1233,6 @@ __device__ __forceinline__ void cell_row_lane(const Dims &d, const StepScalars &
     const double wu1 = FAST ? au1 * dtv : xdiv(au1 * dt, Rv);
     const double wv0 = FAST ? av0 * dtv : xdiv(av0 * dt, Rv);
     const double wv1 = FAST ? av1 * dtv : xdiv(av1 * dt, Rv);
-    const double bv0 = av0 * dt, bv1 = av1 * dt;
     const long s = ts.s;
     // the four advective face fluxes (MF:10534-10894, 10642-10699); for the
     // implicit direction the D (fu*/fv*) and E (eu*/ev*) coefficients
@@ -1262,7 +1265,6 @@ __device__ __forceinline__ void cell_row_lane(const Dims &d, const StepScalars &
         hface_flux<OPT, TPL, FAST, IMP == 1>(d, sc, f, ts, x + 1, 1, i + 1, d.ILB, d.IUB,
                                              f.DVY + d.idx2(i + 1, j), 1, f.Qy[x + 1], fv1, bad,
                                              &ev1);
-    const bool ex = OPT && !FAST;
     // implicit rows: dt/V as the reference's DT1 = DT2 = DTProp / VolumeZ
     // of the cell, and E0 of VolumeVariation
     double dt2 = 0.0, E0 = 1.0;
@@ -1279,19 +1281,19 @@ __device__ __forceinline__ void cell_row_lane(const Dims &d, const StepScalars &
         disch_ti(f, x, open, pk, dt, vk, TI);
         if (cfu0) {
             const double g = pk - P[(x - d.sj) * s];
-            TI = ex ? TI - xdiv(au0 * g * dt, Rv) : TI - wu0 * g;
+            TI = TI - wu0 * g;
         }
         if (cfu1) {
             const double g = P[(x + d.sj) * s] - pk;
-            TI = ex ? TI + xdiv(au1 * g * dt, Rv) : TI + wu1 * g;
+            TI = TI + wu1 * g;
         }
         if (cfv0) {
             const double g = pk - P[(x - 1) * s];
-            TI = ex ? TI - xdiv(bv0 * g, Rv) : TI - wv0 * g;
+            TI = TI - wv0 * g;
         }
         if (cfv1) {
             const double g = P[(x + 1) * s] - pk;
-            TI = ex ? TI + xdiv(bv1 * g, Rv) : TI + wv1 * g;
+            TI = TI + wv1 * g;
         }
         const bool au = IMP != 0, av = IMP != 1;   // explicit directions
         // with nudging the advective terms gather in AdvectionTi, from zero

@@ -121,6 +121,11 @@ def main():
                          "boundary condition, then with each other optional branch (cell "
                          "fluxes, NoDifFlux, NoAdvFlux, SmallDepths, nudging, discharges); "
                          "two JSON lines")
+    ap.add_argument("--dropin-steps", type=int, default=0,
+                    help="instead of the headline line, run this many per-call drop-in steps "
+                         "on 1024x1024x40 (8 tracers, device arrays, FirstProperty on tracer "
+                         "0) and exit: the workload of the HIP API trace that compares "
+                         "allocations at two step counts")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 512 MiB three times (the "
                          "reference workload for scaling FETCH_SIZE/WRITE_SIZE counters)")
@@ -143,6 +148,9 @@ def main():
         sys.exit(2)
     if args.boundaries:
         boundary_costs(args)
+        return
+    if args.dropin_steps > 0:
+        dropin_steps(args.dropin_steps)
         return
     if args.suite:
         suite(args)
@@ -478,6 +486,28 @@ def _batched_rate(case, precision, steps, chunk=None, tune=True, implicit="none"
     tr.close()
     torch.cuda.empty_cache()
     return T * case.n_wet * (steps // chunk) * chunk / wall, shape
+
+
+def dropin_steps(steps):
+    """`steps` transport steps through the per-call drop-in, as the Fortran
+    caller makes them: one call per tracer, FirstProperty on the first."""
+    import torch
+    from mohid_amd import _lib
+    from mohid_amd import fixtures as fx
+    from mohid_amd.advection_diffusion import AdvectionDiffusion
+
+    torch.cuda.set_device(0)
+    _lib.lib()
+    case = fx.make_case(fx.config("c3"), device="cuda")
+    ad = AdvectionDiffusion(case.size, case.worksize, case.geom)
+    props = [t.clone() for t in case.tracers]
+    for s in range(steps):
+        for t, p in enumerate(props):
+            ad(p, case.hydro, case.params, first_property=(t == 0))
+        torch.cuda.synchronize()
+        _log(f"drop-in step {s + 1} of {steps}: {len(props)} calls")
+    ad.kill()
+    print(json.dumps({"dropin_steps": steps, "calls": steps * len(props)}), flush=True)
 
 
 def boundary_costs(args):

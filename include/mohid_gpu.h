@@ -319,6 +319,13 @@ int mgpu_transport_set_lanes(mgpu_transport *t, int32_t tracers_per_launch,
  * For the autotuner; results do not depend on it.  _FUSED is not offered for
  * MGPU_PREC_MIXED or implicit horizontal advection (MGPU_ERR_UNSUPPORTED). */
 int mgpu_transport_set_kernel(mgpu_transport *t, int32_t kernel);
+/* Column order of the lane variant's FP64/FAST column walk: 1 (the default)
+ * walks the solved columns sorted by first row, with the fields it reads
+ * and its forward-sweep scratch copied level-major in that order (rebuilt at
+ * every set_hydro and set_geometry); 0 walks them in memory order.  For the
+ * autotuner; results do not depend on it.  No counterpart in the reference:
+ * a launch-shape setting of this implementation. */
+int mgpu_transport_set_column_order(mgpu_transport *t, int32_t order);
 int mgpu_transport_destroy(mgpu_transport *t);
 /* Waits for the handle's work; MGPU_ERR_OPTION if a face method or limiter
  * the reference stops on was met since the last set_hydro, MGPU_ERR_PIVOT

@@ -97,7 +97,7 @@ EXPORTS = (
     "mgpu_transport_create", "mgpu_transport_set_hydro", "mgpu_transport_set_geometry",
     "mgpu_transport_set_tracer",
     "mgpu_transport_get_tracer", "mgpu_transport_run", "mgpu_transport_tracer_ptr",
-    "mgpu_transport_set_lanes", "mgpu_transport_set_kernel",
+    "mgpu_transport_set_lanes", "mgpu_transport_set_kernel", "mgpu_transport_set_column_order",
     "mgpu_transport_stream", "mgpu_transport_destroy", "mgpu_transport_timing",
     "mgpu_transport_kernel_times",
     "mgpu_selfcheck_xdiv", "mgpu_version", "mgpu_last_error",
@@ -157,6 +157,7 @@ def lib():
     L.mgpu_transport_stream.argtypes = [vp, ct.POINTER(vp)]
     L.mgpu_transport_set_lanes.argtypes = [vp, ct.c_int32, ct.c_int32, ct.c_int32]
     L.mgpu_transport_set_kernel.argtypes = [vp, ct.c_int32]
+    L.mgpu_transport_set_column_order.argtypes = [vp, ct.c_int32]
     L.mgpu_transport_destroy.argtypes = [vp]
     L.mgpu_transport_timing.argtypes = [vp, ct.c_int32, ct.c_int32]
     L.mgpu_transport_kernel_times.argtypes = [vp, ct.POINTER(ct.c_double),

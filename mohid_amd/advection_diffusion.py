@@ -351,6 +351,12 @@ class Transport:
         L.check("mgpu_transport_set_kernel", L.lib().mgpu_transport_set_kernel(self._h,
                                                                                int(kernel)))
 
+    def set_column_order(self, order: int) -> None:
+        """Column walk order: 1 by first row (level-major copies), 0 memory
+        order; results do not depend on it."""
+        L.check("mgpu_transport_set_column_order",
+                L.lib().mgpu_transport_set_column_order(self._h, int(order)))
+
     def stream_handle(self) -> int:
         s = ct.c_void_p()
         L.check("mgpu_transport_stream", L.lib().mgpu_transport_stream(self._h, ct.byref(s)))

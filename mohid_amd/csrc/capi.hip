@@ -1240,6 +1240,65 @@ extern "C" int mgpu_thomas3d(const mgpu_size3d *size, int32_t IJmin, int32_t IJm
 
 // ---------------------------------------------------------------------------
 // Batched device-resident transport
+// The depth-ordered column walk's list and level-major copies
+// (SlotColumns), owned by a batched handle; rebuilt with the masks
+// (set_hydro) and the geometry (set_geometry).
+struct SlotStore {
+    uint32_t *cols = nullptr;
+    int32_t *slot_of = nullptr;
+    double *Qz = nullptr, *Kz = nullptr, *DZZ = nullptr, *V = nullptr, *DWZ = nullptr;
+    uint8_t *mask = nullptr;
+    unsigned long long *total = nullptr;
+    long cap = 0;     // columns the arrays hold
+    long ncols = 0;   // columns of the current list (0: none built)
+    void release() {
+        dfree(cols); dfree(slot_of); dfree(Qz); dfree(Kz); dfree(DZZ); dfree(V); dfree(DWZ);
+        dfree(mask); dfree(total);
+        cap = ncols = 0;
+    }
+    SlotColumns view(double sV, double bV) const {
+        SlotColumns S;
+        if (!ncols) return S;
+        S.cols = cols; S.slot_of = slot_of; S.Qz = Qz; S.Kz = Kz; S.DZZ = DZZ; S.V = V;
+        S.DWZ = DWZ; S.mask = mask; S.ncols = ncols; S.sV = sV; S.bV = bV;
+        return S;
+    }
+    // the copies from the current fields (the list built)
+    int fields(const Dims &d, const DevHydro &h, const DevGeometry &g, const DevDerived &dv,
+               hipStream_t s) const {
+        if (!ncols) return MGPU_SUCCESS;
+        return launch_slot_fields(d, view(1.0, 0.0), h.Qz, h.diffv, g.DZZ, h.V, g.DWZ, dv.mask, s);
+    }
+    // list, slot map and copies for the masks and lists of dv (after
+    // dv.pack; the stream idle)
+    int build(const Dims &d, const DevHydro &h, const DevGeometry &g, DevDerived &dv,
+              hipStream_t s) {
+        const long nc = dv.ncols;
+        ncols = 0;
+        if (nc <= 0) return MGPU_SUCCESS;
+        if (nc > cap) {
+            release();
+            const long nl = (long)(d.KUB - d.KLB + 1) * nc;
+            int r = 0;
+            r |= dalloc(&cols, nc); r |= dalloc(&total, 1); r |= dalloc(&slot_of, d.n2());
+            r |= dalloc(&Qz, nl); r |= dalloc(&Kz, nl); r |= dalloc(&DZZ, nl);
+            r |= dalloc(&V, nl); r |= dalloc(&DWZ, nl); r |= dalloc(&mask, nl);
+            if (r) { release(); return MGPU_ERR_NOMEM; }
+            cap = nc;
+        }
+        int rc = launch_slot_list(d, dv.mask, dv.kstart, dv.rowcnt, cols, total, s);
+        if (rc) return rc;
+        HIPCHK(hipMemsetAsync(slot_of, 0xff, sizeof(int32_t) * (size_t)d.n2(), s));   // -1
+        if ((rc = launch_slot_map(cols, nc, slot_of, s))) return rc;
+        unsigned long long n = 0;
+        HIPCHK(hipMemcpyAsync(&n, total, sizeof(n), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if ((long)n != nc) return MGPU_ERR_STATE;   // every solved column has one first row
+        ncols = nc;
+        return fields(d, h, g, dv, s);
+    }
+};
+
 struct mgpu_transport {
     mgpu_size3d size, ws;
     Dims d;
@@ -1272,7 +1331,13 @@ struct mgpu_transport {
     int timing = 0;
     hipEvent_t *ev = nullptr;        // 2 per launch: [prep, tracer 0 .. T-1] per step
     int ev_cap = 0, ev_used = 0;
+    // column order of the lane variant's column walk: 1 = by first row
+    // (SlotStore), 0 = the solved columns in memory order
+    int col_order = 1;
+    SlotStore slot;
 };
+
+static int slot_sync(mgpu_transport *t);
 
 // record an event of the timing pool (no-op when timing is off or the pool is full)
 static int mark(mgpu_transport *t) {
@@ -1316,6 +1381,7 @@ extern "C" int mgpu_transport_destroy(mgpu_transport *t) {
     dfree(t->stage);
     dfree(t->mid);
     dfree(t->Rb);
+    t->slot.release();
     if (t->stream) (void)hipStreamDestroy(t->stream);
     delete t;
     return MGPU_SUCCESS;
@@ -1423,6 +1489,7 @@ extern "C" int mgpu_transport_set_hydro(mgpu_transport *t, const mgpu_ad_args *a
     HIPCHK(hipMemcpyAsync(cnt, t->der.counters, sizeof(cnt), hipMemcpyDeviceToHost, t->stream));
     HIPCHK(hipStreamSynchronize(t->stream));
     if (cnt[2] & 0xffffffffu) return MGPU_ERR_MASKS;
+    if ((rc = slot_sync(t))) return rc;
     // cells a step does not write (not solved under the new masks) keep their
     // latest value in both ping-pong buffers
     HIPCHK(hipMemcpyAsync(t->buf[t->cur ^ 1], t->buf[t->cur],
@@ -1438,7 +1505,9 @@ extern "C" int mgpu_transport_set_geometry(mgpu_transport *t, const mgpu_geometr
     if (!t) return MGPU_OFF_ERR;
     // same buffers: captured step graphs keep reading them; the copies are
     // ordered after the steps already enqueued on the handle's stream
-    return t->geom.upload(t->d, geometry, t->stream);
+    int rc = t->geom.upload(t->d, geometry, t->stream);
+    if (!rc && t->have_hydro) rc = t->slot.fields(t->d, t->hydro, t->geom, t->der, t->stream);
+    return rc;
 }
 
 extern "C" int mgpu_transport_set_tracer(mgpu_transport *t, int32_t index, const double *prop,
@@ -1506,6 +1575,17 @@ extern "C" int mgpu_transport_set_lanes(mgpu_transport *t, int32_t tracers_per_l
     return MGPU_SUCCESS;
 }
 
+// The depth-ordered list and copies for the handle's current masks, kernel
+// variant and column order (masks packed; the stream idle).
+static int slot_sync(mgpu_transport *t) {
+    // the walk exists for the optimised FP64/FAST column kernel (slot_walk)
+    if (t->col_order && t->kernel == MGPU_KERNEL_LANES && t->sc.optimize &&
+        t->opts.precision != MGPU_PREC_MIXED)
+        return t->slot.build(t->d, t->hydro, t->geom, t->der, t->stream);
+    t->slot.ncols = 0;
+    return MGPU_SUCCESS;
+}
+
 extern "C" int mgpu_transport_set_kernel(mgpu_transport *t, int32_t kernel) {
     if (!t) return MGPU_OFF_ERR;
     const int k = kernel == MGPU_KERNEL_AUTO ? MGPU_KERNEL_LANES : kernel;
@@ -1514,10 +1594,21 @@ extern "C" int mgpu_transport_set_kernel(mgpu_transport *t, int32_t kernel) {
     if (t->kernel == MGPU_KERNEL_SPLIT) return MGPU_ERR_STATE;
     if (k == MGPU_KERNEL_FUSED && (t->opts.precision == MGPU_PREC_MIXED || t->impdir >= 0))
         return MGPU_ERR_UNSUPPORTED;
+    if (k == t->kernel) return MGPU_SUCCESS;
     HIPCHK(hipStreamSynchronize(t->stream));
     t->kernel = k;
     drop_graphs(t);   // captured launches bake the variant in
-    return MGPU_SUCCESS;
+    return t->have_hydro ? slot_sync(t) : MGPU_SUCCESS;
+}
+
+extern "C" int mgpu_transport_set_column_order(mgpu_transport *t, int32_t order) {
+    if (!t) return MGPU_OFF_ERR;
+    if (order != 0 && order != 1) return MGPU_ERR_BADARG;
+    if (order == t->col_order) return MGPU_SUCCESS;
+    HIPCHK(hipStreamSynchronize(t->stream));
+    t->col_order = order;
+    drop_graphs(t);   // captured launches bake the list in
+    return t->have_hydro ? slot_sync(t) : MGPU_SUCCESS;
 }
 
 extern "C" int mgpu_transport_stream(mgpu_transport *t, void **stream) {
@@ -1540,20 +1631,6 @@ static int enqueue_steps(mgpu_transport *t, int nsteps, int cur, int dir) {
             sc.hreset = sc.vert;
             dir ^= 1;
         }
-        // first property of the step: face conductances (AD:1421-1478)
-        int rc = mark(t);
-        if (rc) return rc;
-        const bool nf = !t->sc.optimize && t->hydro.have_nf;
-        rc = launch_prep_faces(d, t->hydro.visc, t->hydro.diffv, t->hydro.Qx, t->hydro.Qy,
-                                   t->hydro.Qz, t->geom.DUX, t->geom.DVY, t->geom.DZX,
-                                   t->geom.DZY, t->geom.AreaU, t->geom.AreaV, t->geom.DZZ,
-                                   t->der.mask, t->schmidtH, t->schmidtV, t->backgroundV,
-                                   t->sc.optimize ? 0 : t->nulldif,
-                                   nf ? t->hydro.nfu : nullptr, nf ? t->hydro.nfv : nullptr,
-                                   nf ? t->hydro.nfw : nullptr, 1, 1, t->der.cu, t->der.cv,
-                                   t->der.cw, s);
-        if (rc) return rc;
-        if ((rc = mark(t))) return rc;
         TransportLaunch L;
         L.fast = t->opts.precision == MGPU_PREC_FAST;
         if (t->opts.precision == MGPU_PREC_MIXED)
@@ -1567,11 +1644,27 @@ static int enqueue_steps(mgpu_transport *t, int nsteps, int cur, int dir) {
         L.Wb = t->Wb;
         L.Gb = t->Gb;
         L.lists = t->der.lists();
+        L.lists.slot = t->slot.view(t->schmidtV, t->backgroundV);
         L.lanes.explicit_tpl = t->opts.explicit_tracers_per_lane;
         L.lanes.column_tpl = t->opts.column_tracers_per_lane;
         L.counters = t->der.counters;
         L.mid = t->mid;
         L.Rb = t->Rb;
+        // first property of the step: face conductances (AD:1421-1478); the
+        // depth-ordered column walk computes the W faces' itself
+        int rc = mark(t);
+        if (rc) return rc;
+        const bool nf = !t->sc.optimize && t->hydro.have_nf;
+        rc = launch_prep_faces(d, t->hydro.visc, t->hydro.diffv, t->hydro.Qx, t->hydro.Qy,
+                               t->hydro.Qz, t->geom.DUX, t->geom.DVY, t->geom.DZX,
+                               t->geom.DZY, t->geom.AreaU, t->geom.AreaV, t->geom.DZZ,
+                               t->der.mask, t->schmidtH, t->schmidtV, t->backgroundV,
+                               t->sc.optimize ? 0 : t->nulldif,
+                               nf ? t->hydro.nfu : nullptr, nf ? t->hydro.nfv : nullptr,
+                               nf ? t->hydro.nfw : nullptr, 1, slot_walk(L, sc, f) ? 0 : 1,
+                               t->der.cu, t->der.cv, t->der.cw, s);
+        if (rc) return rc;
+        if ((rc = mark(t))) return rc;
         // first kernel(s) of every chunk, then the second kernel(s)
         for (int ph = 1; ph <= 2; ++ph) {
             rc = launch_transport(L, d, sc, f, s, ph);

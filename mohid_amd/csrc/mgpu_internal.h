@@ -236,12 +236,34 @@ int launch_prep_faces(const Dims &d, const double *visc, const double *diffv,
 // phase: bit 1 = first kernel, bit 2 = second kernel.
 // fast: reciprocal-multiply division (FP64 state and arithmetic, results
 // differ from the reference's operation order by rounding only).
+// Depth-ordered column walk of the batched transport.  The solved columns
+// sorted by first row (kstart, then j, i); slot n is a column's position in
+// that order.  The fields the walk reads per level are copied level-major
+// (level k of slot n at (k - KLB) * ncols + n), and the explicit kernel's
+// TI, the forward sweep's W over it and G go to the lane scratch in the same
+// order (times the tracer stride).  The lanes of a wave then start at one
+// level, and every access of a level but the tracer state's is one
+// contiguous run.  The walk computes the W-face conductance cw itself from
+// Kz and DZZ (AD:2735-2738), so the step's prep kernel skips it.
+// cols == null: off.
+struct SlotColumns {
+    const uint32_t *cols = nullptr;
+    const int32_t *slot_of = nullptr;   // per Size (i, j): the column's slot, or -1
+    const double *Qz = nullptr;    // Qz of level k + 1 (the cell's upper face)
+    const double *Kz = nullptr;    // Diff_V of level k + 1
+    const double *DZZ = nullptr, *V = nullptr, *DWZ = nullptr;   // of level k
+    const uint8_t *mask = nullptr;                               // of level k
+    long ncols = 0;
+    double sV = 1.0, bV = 0.0;     // SchmidtCoef_V, SchmidtBackground_V
+};
+
 // Compact lists of the lane variant (device arrays, host counts).
 struct SolveLists {
     const uint32_t *cells = nullptr;   // Size index of every solved row, memory order
     const uint32_t *cols = nullptr;    // Size index (level 0) of every solved column
     const uint32_t *hrows = nullptr;   // WorkSize cells without M_HID (implicit line rows)
     long ncells = 0, ncols = 0, nhrows = 0;
+    SlotColumns slot;                  // the depth-ordered walk's columns (batched transport)
 };
 // Builds the lists from the packed masks and kstart; totals[0..2] receive
 // the counts (device memory; the caller copies them back); hrows may be null
@@ -250,6 +272,19 @@ struct SolveLists {
 int launch_build_lists(const Dims &d, const uint8_t *mask, const int32_t *kstart,
                        uint32_t *rowcnt, uint32_t *cells, uint32_t *cols, uint32_t *hrows,
                        unsigned long long *totals, hipStream_t s);
+// The depth-ordered column list (SlotColumns::cols); *total receives its
+// length (device memory), which equals the solved-column count.
+int launch_slot_list(const Dims &d, const uint8_t *mask, const int32_t *kstart,
+                     uint32_t *rowcnt, uint32_t *cols, unsigned long long *total, hipStream_t s);
+// SlotColumns::slot_of from the list: slot_of[cols[n]] = n (the other
+// entries set to -1 beforehand).
+int launch_slot_map(const uint32_t *cols, long ncols, int32_t *slot_of, hipStream_t s);
+// SlotColumns' level-major field copies, levels KLB..KUB of Size, from the
+// state-shaped fields (S.cols and S.ncols set; the copies written through
+// the const pointers of S, which point at the caller's buffers).
+int launch_slot_fields(const Dims &d, const SlotColumns &S, const double *Qz, const double *diffv,
+                       const double *DZZ, const double *V, const double *DWZ,
+                       const uint8_t *mask, hipStream_t s);
 
 // Tracers per lane of the lane variant's kernels (0: library default).
 struct LaneShape {
@@ -327,6 +362,9 @@ template <> int launch_chunk_set<true, true, 1>(MGPU_CHUNK_SET_PARAMS);
 
 int launch_transport(const TransportLaunch &L, const Dims &d, const StepScalars &sc,
                      const SharedFields &f, hipStream_t s, int phase = 3);
+// Whether launch_transport's column kernel is the depth-ordered walk, which
+// computes cw itself (the step's prep kernel may then skip cw).
+bool slot_walk(const TransportLaunch &L, const StepScalars &sc, const SharedFields &f);
 // Copy one tracer between a Fortran array (stride 1) and an interleaved
 // buffer (stride `stride`, slot t).  to_interleaved = 1: src Fortran -> dst.
 int launch_interleave(long n3, const double *src, double *dst, long stride, int t,

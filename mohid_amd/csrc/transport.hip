@@ -142,8 +142,10 @@ int launch_transport(const TransportLaunch &L, const Dims &d, const StepScalars 
         const bool lanes = L.variant != MGPU_KERNEL_SPLIT;
         double *Wb = L.Wb + (lanes ? t0 : blocked_scratch_len(d, t0));
         double *Gb = L.Gb + (lanes ? t0 : blocked_scratch_len(d, t0));
+        SolveLists lists = L.lists;
+        if (!slot_walk(L, sc, f)) lists.slot = SlotColumns{};   // both kernels agree
         const bool big = nt > 4;
-#define MGPU_ARGS nt, L.variant, phase, d, sc, f, ts, Wb, Gb, L.lists, L.lanes, L.counters, imp, s
+#define MGPU_ARGS nt, L.variant, phase, d, sc, f, ts, Wb, Gb, lists, L.lanes, L.counters, imp, s
         int rc;
         if (sc.optimize)
             rc = L.fast ? (big ? launch_chunk_set<true, true, 1>(MGPU_ARGS)
@@ -158,6 +160,51 @@ int launch_transport(const TransportLaunch &L, const Dims &d, const StepScalars 
 #undef MGPU_ARGS
         if (rc) return rc;
     }
+    return MGPU_SUCCESS;
+}
+
+bool slot_walk(const TransportLaunch &L, const StepScalars &sc, const SharedFields &f) {
+    // launch_column_tpl's choice: the optimised FP64/FAST lane column kernel
+    // without open boundary rows or discharges, given the depth-ordered list
+    return L.variant == MGPU_KERNEL_LANES && L.lanes.mixed == 0 && sc.optimize && !f.obc.bp &&
+           !f.disch.ptr && L.lists.slot.cols != nullptr;
+}
+
+int launch_slot_list(const Dims &d, const uint8_t *mask, const int32_t *kstart,
+                     uint32_t *rowcnt, uint32_t *cols, unsigned long long *total, hipStream_t s) {
+    if (d.n3() >= (1L << 32)) return MGPU_ERR_UNSUPPORTED;   // list entries are 32-bit
+    const int nj = d.wJUB - d.wJLB + 1, nk = d.wKUB - d.wKLB + 1;
+    // rows (first row k, j), k-major: the list runs by kstart, then j, then i
+    hipLaunchKernelGGL(list_count_kernel, dim3(nj, nk), dim3(256), 0, s, d, mask, kstart, 3,
+                       rowcnt);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(list_scan_kernel, dim3(1), dim3(1024), 0, s, rowcnt, (long)nj * nk, total);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(list_write_kernel, dim3(nj, nk), dim3(256), 0, s, d, mask, kstart, 3,
+                       rowcnt, cols);
+    HIPCHK(hipGetLastError());
+    return MGPU_SUCCESS;
+}
+
+int launch_slot_map(const uint32_t *cols, long ncols, int32_t *slot_of, hipStream_t s) {
+    if (ncols <= 0) return MGPU_SUCCESS;
+    hipLaunchKernelGGL(slot_map_kernel, dim3((unsigned)((ncols + 255) / 256)), dim3(256), 0, s,
+                       cols, ncols, slot_of);
+    HIPCHK(hipGetLastError());
+    return MGPU_SUCCESS;
+}
+
+int launch_slot_fields(const Dims &d, const SlotColumns &S, const double *Qz, const double *diffv,
+                       const double *DZZ, const double *V, const double *DWZ,
+                       const uint8_t *mask, hipStream_t s) {
+    if (S.ncols <= 0) return MGPU_SUCCESS;
+    const int nlev = d.KUB - d.KLB + 1;
+    hipLaunchKernelGGL(slot_fields_kernel, dim3((unsigned)((S.ncols + 255) / 256), nlev),
+                       dim3(256), 0, s, d, S.cols, S.ncols, Qz, diffv, DZZ, V, DWZ, mask,
+                       const_cast<double *>(S.Qz), const_cast<double *>(S.Kz),
+                       const_cast<double *>(S.DZZ), const_cast<double *>(S.V),
+                       const_cast<double *>(S.DWZ), const_cast<uint8_t *>(S.mask));
+    HIPCHK(hipGetLastError());
     return MGPU_SUCCESS;
 }
 

@@ -1541,12 +1541,13 @@ __device__ __forceinline__ void cell_rhs_lane(const Dims &d, const StepScalars &
 
 // Explicit kernel of the lane variant: NT / TPL lanes per cell; lane
 // (n, u) = cell cells[n], tracers u*TPL .. u*TPL+TPL-1 of the launch; TI
-// goes to TW (laid out like the tracer state).
+// goes to TW, laid out like the tracer state, or with the depth-ordered
+// column walk (S.slot_of) at the cell's level-major slot.
 template <bool OPT, int NT, int TPL, bool FAST>
 __global__ void __launch_bounds__(256)
 explicit_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
                      const uint32_t *__restrict__ cells, long ncells, double *__restrict__ TW,
-                     unsigned int *__restrict__ badflag) {
+                     SlotColumns S, unsigned int *__restrict__ badflag) {
     constexpr int LPC = NT / TPL;   // lanes per cell
     const long g = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     const long n = g / LPC;
@@ -1557,9 +1558,14 @@ explicit_lane_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
     const CellPos c = cell_pos(d, xc);
     const TracerSet tl{ts.P + t0, ts.out + t0, ts.s, ts.G0 ? ts.G0 + t0 : nullptr};
     int bad = 0;
+    long q = x;
+    if (S.slot_of) {
+        const long lk = c.k - d.KLB;
+        q = lk * S.ncols + S.slot_of[x - lk * d.sk];   // every listed cell's column is solved
+    }
     double TI[TPL];
     cell_rhs_lane<OPT, TPL, FAST>(d, sc, f, tl, x, c.i, c.j, TI, bad);
-    store_tpl<TPL>(TW + x * ts.s + t0, TI);
+    store_tpl<TPL>(TW + q * ts.s + t0, TI);
     if (bad) atomicOr(badflag, 1u);
 }
 
@@ -1671,6 +1677,7 @@ template <int NT>
 struct LevelIn {
     double TI[NT], p3[NT];
     double qz1, cwB, cwT, v3, du3;
+    double kzT, dzz;   // depth-ordered walk: Diff_V of the face above, DZZ
     uint8_t m3;
     bool in3;
 };
@@ -1691,6 +1698,30 @@ __device__ __forceinline__ void level_load(const Dims &d, const StepScalars &sc,
     L.v3 = f.V[y];
     L.du3 = f.DWZ[y];
     L.m3 = f.mask[y];
+    load_tpl<NT>(ts.P + y * ts.s, L.p3);
+}
+
+// level_load for the depth-ordered walk: TI and the shared fields from the
+// level-major slots of n (one contiguous run per level for the wave), the
+// window's tracer values (and TI after implicit line solves) from the state.
+template <int NT>
+__device__ __forceinline__ void level_load_slot(const Dims &d, const StepScalars &sc,
+                                                const SlotColumns &S, const TracerSet &ts,
+                                                const double *tw, long base, long n, int k,
+                                                LevelIn<NT> &L) {
+    const long sk = d.sk, x = base + k * sk;
+    const long q = (long)(k - d.KLB) * S.ncols + n;
+    load_tpl<NT>(sc.hreset ? ts.P + x * ts.s : tw + q * ts.s, L.TI);
+    L.qz1 = S.Qz[q];
+    L.kzT = S.Kz[q];
+    L.dzz = S.DZZ[q];
+    const int kn = k + 3;
+    L.in3 = kn >= d.KLB && kn <= d.KUB;
+    const long q3 = L.in3 ? q + 3 * S.ncols : q;
+    L.v3 = S.V[q3];
+    L.du3 = S.DWZ[q3];
+    L.m3 = S.mask[q3];
+    const long y = L.in3 ? base + kn * sk : x;
     load_tpl<NT>(ts.P + y * ts.s, L.p3);
 }
 
@@ -1786,14 +1817,21 @@ __host__ __device__ constexpr int column_lds_window(int tpl) {
     return tpl == 1 ? 10 : (tpl == 2 ? 6 : (tpl == 4 ? 4 : 0));
 }
 
-template <int NT, int TPL, bool FAST>
+// SLOT: the depth-ordered walk (cols sorted by first row; TI, the shared
+// fields, and W and G below the on-chip levels at the level-major slots of
+// S).
+// Blocks then go round the XCDs in list order, so that every XCD gets
+// columns of every depth, and the wave's W-face conductances are computed
+// here from Kz and DZZ as the prep kernel computes them (AD:2735-2738).
+template <int NT, int TPL, bool FAST, bool SLOT>
 __global__ void __launch_bounds__(256)
 column_pipe_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
                    const uint32_t *__restrict__ cols, long ncols, double *__restrict__ TW,
-                   double *__restrict__ Gs, unsigned long long *__restrict__ zero_pivots,
+                   double *__restrict__ Gs, SlotColumns S,
+                   unsigned long long *__restrict__ zero_pivots,
                    unsigned int *__restrict__ badflag) {
     constexpr int LPC = NT / TPL;
-    const long g = xcd_tile(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    const long g = (SLOT ? (long)blockIdx.x : xcd_tile(blockIdx.x, gridDim.x)) * 256 + threadIdx.x;
     const long n = g / LPC;
     const bool act = n < ncols;
     const int t = (int)(g - n * LPC) * TPL;
@@ -1810,6 +1848,11 @@ column_pipe_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
     const bool vflux = sc.vert && !sc.vertical1d && columnBuild;
     ColumnWalk<TPL> c;
     column_begin<true, TPL, FAST>(d, sc, f, t1, base, ks, vflux, c, bad);
+    // the depth-ordered walk's cw: DUX and DVY of the column, and the face
+    // below the first row, which is never a computing face (its lower cell
+    // would otherwise not be inert, or the masks are rejected at KLB)
+    const double dux = SLOT ? f.DUX[base] : 0.0, dvy = SLOT ? f.DVY[base] : 0.0;
+    double cwB = 0.0;
     constexpr int WIN = column_window(TPL);
     double rW[WIN > 0 ? WIN : 1][TPL], rG[WIN > 0 ? WIN : 1][TPL];
     // the next LW levels down keep W and G in LDS, one (W, G) pair per
@@ -1817,13 +1860,24 @@ column_pipe_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
     constexpr int LW = column_lds_window(TPL);
     __shared__ double2 lwg[LW > 0 ? LW * TPL : 1][256];
     LevelIn<TPL> cur;
-    level_load<TPL>(d, sc, f, t1, tw, base, ks, cur);
+    if (SLOT) level_load_slot<TPL>(d, sc, S, t1, tw, base, n, ks, cur);
+    else level_load<TPL>(d, sc, f, t1, tw, base, ks, cur);
     for (int k = kw; k <= KUB; ++k) {
         if (k < ks) continue;
         const long x = base + k * sk;
         // the next level's loads go out before this level's arithmetic
         LevelIn<TPL> nxt;
-        if (k < KUB) level_load<TPL>(d, sc, f, t1, tw, base, k + 1, nxt);
+        if (k < KUB) {
+            if (SLOT) level_load_slot<TPL>(d, sc, S, t1, tw, base, n, k + 1, nxt);
+            else level_load<TPL>(d, sc, f, t1, tw, base, k + 1, nxt);
+        }
+        if (SLOT) {
+            // Kz = SchmidtCoef_V*Diff_V + SchmidtBackground_V; cw = Kz*DUX*DVY/DZZ(k)
+            // for the face above (used only where that face computes)
+            cur.cwB = cwB;
+            cur.cwT = (S.sV * cur.kzT + S.bV) * dux * dvy / cur.dzz;
+            cwB = cur.cwT;
+        }
         double W[TPL], G[TPL];
         column_row_pipe<TPL, FAST>(sc, k, KUB, vflux, c, cur, W, G, zero_pivots, bad);
         const int slot = KUB - k;
@@ -1839,8 +1893,10 @@ column_pipe_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
             for (int u = 0; u < TPL; ++u)
                 lwg[(slot - WIN) * TPL + u][threadIdx.x] = make_double2(W[u], G[u]);
         } else {
-            store_tpl<TPL>(tw + x * s, W);
-            store_tpl<TPL>(gs + x * s, G);
+            // W over TI (each lane reads its TI(k) a level ahead of this store)
+            const long q = SLOT ? (long)(k - d.KLB) * S.ncols + n : x;
+            store_tpl<TPL>(tw + q * s, W);
+            store_tpl<TPL>(gs + q * s, G);
         }
         cur = nxt;
     }
@@ -1866,8 +1922,9 @@ column_pipe_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
                 G[u] = q.y;
             }
         } else {
-            load_tpl<TPL>(tw + x * s, W);
-            load_tpl<TPL>(gs + x * s, G);
+            const long q = SLOT ? (long)(k - d.KLB) * S.ncols + n : x;
+            load_tpl<TPL>(tw + q * s, W);
+            load_tpl<TPL>(gs + q * s, G);
         }
 #pragma unroll
         for (int u = 0; u < TPL; ++u) xs[u] = W[u] * xs[u] + G[u];
@@ -2325,14 +2382,16 @@ line_thomas_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts, double4
 // gives an explicit right-hand side -- rows kstart..KUB of the columns with
 // WaterPoints3D(i,j,KUB) == 1 -- in memory order; cols (kind 1): those
 // columns; hrows (kind 2): the WorkSize cells whose horizontal line-solve
-// row is not the identity (no M_HID).  One block per row (j, k) of the
-// WorkSize plane: counts, then (after an exclusive scan of the counts) the
-// Size-linear indices in order.
+// row is not the identity (no M_HID); the depth-ordered columns (kind 3):
+// the cols whose first row is k, for k from KLB up.  One block per row
+// (j, k) of the WorkSize plane: counts, then (after an exclusive scan of the
+// counts) the Size-linear indices in order.
 __device__ __forceinline__ bool list_member(const Dims &d, const uint8_t *mask,
                                             const int32_t *kstart, int i, int j, int k,
                                             int kind) {
     if (kind == 2) return !(mask[d.idx(i, j, k)] & M_HID);
     if (!(mask[d.idx(i, j, d.wKUB)] & M_WATER)) return false;
+    if (kind == 3) return k == kstart[d.idx2(i, j)];   // a solved column whose first row is k
     return kind == 1 || k >= kstart[d.idx2(i, j)];
 }
 
@@ -2395,10 +2454,43 @@ list_write_kernel(Dims d, const uint8_t *__restrict__ mask, const int32_t *__res
         uint32_t woff = 0;
         for (int w = 0; w < wave; ++w) woff += wsum[w];
         const uint32_t chunk = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-        if (m) out[off + woff + before] = (uint32_t)(kind == 1 ? d.idx2(i, j) : d.idx(i, j, k));
+        if (m)
+            out[off + woff + before] =
+                (uint32_t)(kind == 1 || kind == 3 ? d.idx2(i, j) : d.idx(i, j, k));
         __syncthreads();
         off += chunk;
     }
+}
+
+// The depth-ordered walk's field copies: level k of slot n at
+// (k - KLB) * nc + n, for the Size levels KLB..KUB (Qz and Diff_V of the
+// level above, 0 past the Size's top).
+__global__ void __launch_bounds__(256)
+slot_fields_kernel(Dims d, const uint32_t *__restrict__ cols, long nc,
+                   const double *__restrict__ Qz, const double *__restrict__ diffv,
+                   const double *__restrict__ DZZ, const double *__restrict__ V,
+                   const double *__restrict__ DWZ, const uint8_t *__restrict__ mask,
+                   double *__restrict__ sQz, double *__restrict__ sKz, double *__restrict__ sDZZ,
+                   double *__restrict__ sV, double *__restrict__ sDWZ,
+                   uint8_t *__restrict__ smask) {
+    const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const int k = d.KLB + (int)blockIdx.y;
+    if (n >= nc) return;
+    const long x = (long)cols[n] + (long)(k - d.KLB) * d.sk;
+    const long q = (long)(k - d.KLB) * nc + n;
+    const bool up = k < d.KUB;
+    sQz[q] = up ? Qz[x + d.sk] : 0.0;
+    sKz[q] = up ? diffv[x + d.sk] : 0.0;
+    sDZZ[q] = DZZ[x];
+    sV[q] = V[x];
+    sDWZ[q] = DWZ[x];
+    smask[q] = mask[x];
+}
+
+__global__ void __launch_bounds__(256)
+slot_map_kernel(const uint32_t *__restrict__ cols, long nc, int32_t *__restrict__ slot_of) {
+    const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (n < nc) slot_of[cols[n]] = (int32_t)n;
 }
 
 }  // namespace
@@ -2412,7 +2504,7 @@ static int launch_explicit_tpl(const Dims &d, const StepScalars &sc, const Share
     const long nb = (l.ncells * (NT / TPL) + 255) / 256;
     if (nb <= 0) return MGPU_SUCCESS;
     hipLaunchKernelGGL((explicit_lane_kernel<OPT, NT, TPL, FAST>), dim3((unsigned)nb), dim3(256),
-                       0, s, d, sc, f, ts, l.cells, l.ncells, TW, bad);
+                       0, s, d, sc, f, ts, l.cells, l.ncells, TW, l.slot, bad);
     HIPCHK(hipGetLastError());
     return MGPU_SUCCESS;
 }
@@ -2465,9 +2557,16 @@ static int launch_column_tpl(const Dims &d, const StepScalars &sc, const SharedF
     if constexpr (OPT) {
         // the optimised path without discharges: the pipelined walk
         if (!f.disch.ptr) {
-            hipLaunchKernelGGL((column_pipe_kernel<NT, TPL, FAST>), dim3((unsigned)nb),
-                               dim3(256), 0, s, d, sc, f, ts, l.cols, l.ncols, TW, Gs, counters,
-                               bad);
+            if (l.slot.cols) {
+                if (l.slot.ncols != l.ncols) return MGPU_ERR_STATE;
+                hipLaunchKernelGGL((column_pipe_kernel<NT, TPL, FAST, true>), dim3((unsigned)nb),
+                                   dim3(256), 0, s, d, sc, f, ts, l.slot.cols, l.ncols, TW, Gs,
+                                   l.slot, counters, bad);
+            } else {
+                hipLaunchKernelGGL((column_pipe_kernel<NT, TPL, FAST, false>),
+                                   dim3((unsigned)nb), dim3(256), 0, s, d, sc, f, ts, l.cols,
+                                   l.ncols, TW, Gs, SlotColumns{}, counters, bad);
+            }
             HIPCHK(hipGetLastError());
             return MGPU_SUCCESS;
         }

@@ -299,9 +299,9 @@ def test_device_resident_inputs_match_host_inputs():
 
 def test_every_launch_shape_gives_identical_results():
     """The launch shapes (lane pair or fused walk, tracers per launch, tracers
-    per lane in the explicit and column kernels) change speed, never results:
-    every candidate the autotuner can pick reproduces the oracle bit for
-    bit."""
+    per lane in the explicit and column kernels, and the column walk's order:
+    by first row or in memory order) change speed, never results: every
+    candidate the autotuner can pick reproduces the oracle bit for bit."""
     from mohid_amd.autotune import apply, autotune, candidates
     _, Transport, _ = _ad()
     case = make("tvd_ragged_opt", tracers=("temperature", "salinity") + ("lognormal",) * 6)
@@ -309,14 +309,17 @@ def test_every_launch_shape_gives_identical_results():
     ref = ora.run_case(case, 2, nthreads=1)
     tr = Transport(case.size, case.worksize, case.geom, T)
     tr.set_hydro(case.hydro, case.params)
-    for shape in candidates(T):
-        apply(tr, shape)
-        for t, p in enumerate(case.tracers):
-            tr.set_tracer(t, p)
-        tr.run(1, use_graph=True)
-        tr.run(1, use_graph=False)
-        for t in range(T):
-            assert np.array_equal(tr.get_tracer(t), ref[t]), (shape, t)
+    for order in (1, 0):
+        tr.set_column_order(order)
+        for shape in candidates(T):
+            apply(tr, shape)
+            for t, p in enumerate(case.tracers):
+                tr.set_tracer(t, p)
+            tr.run(1, use_graph=True)
+            tr.run(1, use_graph=False)
+            for t in range(T):
+                assert np.array_equal(tr.get_tracer(t), ref[t]), (order, shape, t)
+    tr.set_column_order(1)
     res = autotune(tr, steps=1)
     assert tuple(res["shape"]) in candidates(T)
     tr.close()

@@ -319,12 +319,21 @@ int mgpu_transport_set_lanes(mgpu_transport *t, int32_t tracers_per_launch,
  * For the autotuner; results do not depend on it.  _FUSED is not offered for
  * MGPU_PREC_MIXED or implicit horizontal advection (MGPU_ERR_UNSUPPORTED). */
 int mgpu_transport_set_kernel(mgpu_transport *t, int32_t kernel);
-/* Column order of the lane variant's FP64/FAST column walk: 1 (the default)
- * walks the solved columns sorted by first row, with the fields it reads
- * and its forward-sweep scratch copied level-major in that order (rebuilt at
- * every set_hydro and set_geometry); 0 walks them in memory order.  For the
- * autotuner; results do not depend on it.  No counterpart in the reference:
- * a launch-shape setting of this implementation. */
+/* Column order of the lane variant's FP64/FAST column walk.
+ * MGPU_COLUMNS_BY_DEPTH walks the solved columns sorted by first row, with
+ * the fields it reads, the right-hand side and the forward-sweep scratch
+ * laid out level-major in that order; its copies are rebuilt at every
+ * set_hydro and set_geometry (about 2 ms at 1024 x 1024 x 40).
+ * MGPU_COLUMNS_MEMORY walks them in memory order.  MGPU_COLUMNS_AUTO (the
+ * default) walks in memory order until a hydrodynamic state has run
+ * MGPU_SLOT_MIN_STEPS steps, and by depth from the run that reaches it, so a
+ * caller that calls set_hydro before every step never pays for the copies.
+ * Results do not depend on the order.  No counterpart in the reference: a
+ * launch-shape setting of this implementation. */
+#define MGPU_COLUMNS_MEMORY 0
+#define MGPU_COLUMNS_BY_DEPTH 1
+#define MGPU_COLUMNS_AUTO 2
+#define MGPU_SLOT_MIN_STEPS 4
 int mgpu_transport_set_column_order(mgpu_transport *t, int32_t order);
 int mgpu_transport_destroy(mgpu_transport *t);
 /* Waits for the handle's work; MGPU_ERR_OPTION if a face method or limiter

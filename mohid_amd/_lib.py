@@ -34,6 +34,10 @@ MGPU_KERNEL_AUTO = 0
 MGPU_KERNEL_FUSED = 1
 MGPU_KERNEL_SPLIT = 2
 MGPU_KERNEL_LANES = 3
+MGPU_COLUMNS_MEMORY = 0
+MGPU_COLUMNS_BY_DEPTH = 1
+MGPU_COLUMNS_AUTO = 2
+MGPU_SLOT_MIN_STEPS = 4
 MGPU_NOALLOC_MAX_CELLS = 16777216
 
 STATUS_NAMES = {

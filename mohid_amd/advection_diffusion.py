@@ -282,6 +282,7 @@ class Transport:
                                               ct.byref(opts), ct.byref(g), ct.byref(h)))
         self._h = h
         self.ntracers = ntracers
+        self.column_order = L.MGPU_COLUMNS_AUTO
 
     def set_hydro(self, hydro: Dict, params: Dict) -> None:
         keep: list = []
@@ -352,10 +353,11 @@ class Transport:
                                                                                int(kernel)))
 
     def set_column_order(self, order: int) -> None:
-        """Column walk order: 1 by first row (level-major copies), 0 memory
-        order; results do not depend on it."""
+        """Column walk order (MGPU_COLUMNS_MEMORY, _BY_DEPTH or _AUTO, the
+        default); results do not depend on it."""
         L.check("mgpu_transport_set_column_order",
                 L.lib().mgpu_transport_set_column_order(self._h, int(order)))
+        self.column_order = int(order)
 
     def stream_handle(self) -> int:
         s = ct.c_void_p()

@@ -99,6 +99,10 @@ def autotune(tr, steps: int = 3, shapes: Optional[List[Shape]] = None,
             return {"shape": best, "ms": {}, "cached": True}
     shapes = shapes or candidates(tr.ntracers, kernels=kernels)
     stream = torch.cuda.ExternalStream(tr.stream_handle())
+    # with the automatic column order, a hydrodynamic state that has run
+    # this many steps walks by depth: every candidate is then timed as the
+    # handle runs it from now on
+    tr.run(L.MGPU_SLOT_MIN_STEPS, use_graph=False)
     timings: Dict[Shape, float] = {}
     for shp in shapes:
         try:

@@ -1331,9 +1331,11 @@ struct mgpu_transport {
     int timing = 0;
     hipEvent_t *ev = nullptr;        // 2 per launch: [prep, tracer 0 .. T-1] per step
     int ev_cap = 0, ev_used = 0;
-    // column order of the lane variant's column walk: 1 = by first row
-    // (SlotStore), 0 = the solved columns in memory order
-    int col_order = 1;
+    // column order of the lane variant's column walk (MGPU_COLUMNS_*): by
+    // first row (SlotStore), in memory order, or by first row once the
+    // current hydrodynamic state has run MGPU_SLOT_MIN_STEPS steps
+    int col_order = MGPU_COLUMNS_AUTO;
+    long hydro_steps = 0;   // steps run since the last set_hydro
     SlotStore slot;
 };
 
@@ -1489,6 +1491,7 @@ extern "C" int mgpu_transport_set_hydro(mgpu_transport *t, const mgpu_ad_args *a
     HIPCHK(hipMemcpyAsync(cnt, t->der.counters, sizeof(cnt), hipMemcpyDeviceToHost, t->stream));
     HIPCHK(hipStreamSynchronize(t->stream));
     if (cnt[2] & 0xffffffffu) return MGPU_ERR_MASKS;
+    t->hydro_steps = 0;
     if ((rc = slot_sync(t))) return rc;
     // cells a step does not write (not solved under the new masks) keep their
     // latest value in both ping-pong buffers
@@ -1577,11 +1580,17 @@ extern "C" int mgpu_transport_set_lanes(mgpu_transport *t, int32_t tracers_per_l
 
 // The depth-ordered list and copies for the handle's current masks, kernel
 // variant and column order (masks packed; the stream idle).
+// The depth-ordered walk exists for the optimised FP64/FAST lane column
+// kernel (slot_walk).
+static bool slot_eligible(const mgpu_transport *t) {
+    return t->kernel == MGPU_KERNEL_LANES && t->sc.optimize &&
+           t->opts.precision != MGPU_PREC_MIXED;
+}
+
 static int slot_sync(mgpu_transport *t) {
-    // the walk exists for the optimised FP64/FAST column kernel (slot_walk)
-    if (t->col_order && t->kernel == MGPU_KERNEL_LANES && t->sc.optimize &&
-        t->opts.precision != MGPU_PREC_MIXED)
+    if (t->col_order == MGPU_COLUMNS_BY_DEPTH && slot_eligible(t))
         return t->slot.build(t->d, t->hydro, t->geom, t->der, t->stream);
+    // MGPU_COLUMNS_AUTO builds it in mgpu_transport_run
     t->slot.ncols = 0;
     return MGPU_SUCCESS;
 }
@@ -1603,7 +1612,9 @@ extern "C" int mgpu_transport_set_kernel(mgpu_transport *t, int32_t kernel) {
 
 extern "C" int mgpu_transport_set_column_order(mgpu_transport *t, int32_t order) {
     if (!t) return MGPU_OFF_ERR;
-    if (order != 0 && order != 1) return MGPU_ERR_BADARG;
+    if (order != MGPU_COLUMNS_MEMORY && order != MGPU_COLUMNS_BY_DEPTH &&
+        order != MGPU_COLUMNS_AUTO)
+        return MGPU_ERR_BADARG;
     if (order == t->col_order) return MGPU_SUCCESS;
     HIPCHK(hipStreamSynchronize(t->stream));
     t->col_order = order;
@@ -1681,6 +1692,15 @@ extern "C" int mgpu_transport_run(mgpu_transport *t, int32_t nsteps, int32_t use
     if (nsteps < 0) return MGPU_ERR_BADARG;
     if (!t->have_hydro) return MGPU_ERR_STATE;
     if (nsteps == 0) return MGPU_SUCCESS;
+    if (t->col_order == MGPU_COLUMNS_AUTO && !t->slot.ncols && slot_eligible(t) &&
+        t->hydro_steps + nsteps >= MGPU_SLOT_MIN_STEPS) {
+        // the hydrodynamic state lasts long enough to repay the copies
+        HIPCHK(hipStreamSynchronize(t->stream));
+        int rc = t->slot.build(t->d, t->hydro, t->geom, t->der, t->stream);
+        if (rc) return rc;
+        drop_graphs(t);   // captured launches bake the column walk in
+    }
+    t->hydro_steps += nsteps;
     if (!use_graph) {
         int rc = enqueue_steps(t, nsteps, t->cur, t->impdir);
         if (rc) return rc;

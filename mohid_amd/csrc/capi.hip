@@ -1248,26 +1248,30 @@ struct SlotStore {
     int32_t *slot_of = nullptr;
     double *Qz = nullptr, *Kz = nullptr, *DZZ = nullptr, *V = nullptr, *DWZ = nullptr;
     uint8_t *mask = nullptr;
+    int32_t *ks = nullptr;
+    double *dux = nullptr, *dvy = nullptr;
     unsigned long long *total = nullptr;
     long cap = 0;     // columns the arrays hold
     long ncols = 0;   // columns of the current list (0: none built)
     void release() {
         dfree(cols); dfree(slot_of); dfree(Qz); dfree(Kz); dfree(DZZ); dfree(V); dfree(DWZ);
-        dfree(mask); dfree(total);
+        dfree(mask); dfree(ks); dfree(dux); dfree(dvy); dfree(total);
         cap = ncols = 0;
     }
     SlotColumns view(double sV, double bV) const {
         SlotColumns S;
         if (!ncols) return S;
         S.cols = cols; S.slot_of = slot_of; S.Qz = Qz; S.Kz = Kz; S.DZZ = DZZ; S.V = V;
-        S.DWZ = DWZ; S.mask = mask; S.ncols = ncols; S.sV = sV; S.bV = bV;
+        S.DWZ = DWZ; S.mask = mask; S.ks = ks; S.dux = dux; S.dvy = dvy; S.ncols = ncols;
+        S.sV = sV; S.bV = bV;
         return S;
     }
     // the copies from the current fields (the list built)
     int fields(const Dims &d, const DevHydro &h, const DevGeometry &g, const DevDerived &dv,
                hipStream_t s) const {
         if (!ncols) return MGPU_SUCCESS;
-        return launch_slot_fields(d, view(1.0, 0.0), h.Qz, h.diffv, g.DZZ, h.V, g.DWZ, dv.mask, s);
+        return launch_slot_fields(d, view(1.0, 0.0), h.Qz, h.diffv, g.DZZ, h.V, g.DWZ, dv.mask,
+                                  dv.kstart, g.DUX, g.DVY, s);
     }
     // list, slot map and copies for the masks and lists of dv (after
     // dv.pack; the stream idle)
@@ -1283,6 +1287,7 @@ struct SlotStore {
             r |= dalloc(&cols, nc); r |= dalloc(&total, 1); r |= dalloc(&slot_of, d.n2());
             r |= dalloc(&Qz, nl); r |= dalloc(&Kz, nl); r |= dalloc(&DZZ, nl);
             r |= dalloc(&V, nl); r |= dalloc(&DWZ, nl); r |= dalloc(&mask, nl);
+            r |= dalloc(&ks, nc); r |= dalloc(&dux, nc); r |= dalloc(&dvy, nc);
             if (r) { release(); return MGPU_ERR_NOMEM; }
             cap = nc;
         }

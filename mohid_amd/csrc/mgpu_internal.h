@@ -253,6 +253,8 @@ struct SlotColumns {
     const double *Kz = nullptr;    // Diff_V of level k + 1
     const double *DZZ = nullptr, *V = nullptr, *DWZ = nullptr;   // of level k
     const uint8_t *mask = nullptr;                               // of level k
+    const int32_t *ks = nullptr;   // per slot: the column's first row (kstart)
+    const double *dux = nullptr, *dvy = nullptr;                 // per slot: DUX, DVY
     long ncols = 0;
     double sV = 1.0, bV = 0.0;     // SchmidtCoef_V, SchmidtBackground_V
 };
@@ -279,12 +281,14 @@ int launch_slot_list(const Dims &d, const uint8_t *mask, const int32_t *kstart,
 // SlotColumns::slot_of from the list: slot_of[cols[n]] = n (the other
 // entries set to -1 beforehand).
 int launch_slot_map(const uint32_t *cols, long ncols, int32_t *slot_of, hipStream_t s);
-// SlotColumns' level-major field copies, levels KLB..KUB of Size, from the
-// state-shaped fields (S.cols and S.ncols set; the copies written through
-// the const pointers of S, which point at the caller's buffers).
+// SlotColumns' level-major field copies, levels KLB..KUB of Size, and its
+// per-slot kstart, DUX and DVY, from the state-shaped fields (S.cols and
+// S.ncols set; the copies written through the const pointers of S, which
+// point at the caller's buffers).
 int launch_slot_fields(const Dims &d, const SlotColumns &S, const double *Qz, const double *diffv,
                        const double *DZZ, const double *V, const double *DWZ,
-                       const uint8_t *mask, hipStream_t s);
+                       const uint8_t *mask, const int32_t *kstart, const double *DUX,
+                       const double *DVY, hipStream_t s);
 
 // Tracers per lane of the lane variant's kernels (0: library default).
 struct LaneShape {

@@ -196,14 +196,17 @@ int launch_slot_map(const uint32_t *cols, long ncols, int32_t *slot_of, hipStrea
 
 int launch_slot_fields(const Dims &d, const SlotColumns &S, const double *Qz, const double *diffv,
                        const double *DZZ, const double *V, const double *DWZ,
-                       const uint8_t *mask, hipStream_t s) {
+                       const uint8_t *mask, const int32_t *kstart, const double *DUX,
+                       const double *DVY, hipStream_t s) {
     if (S.ncols <= 0) return MGPU_SUCCESS;
     const int nlev = d.KUB - d.KLB + 1;
     hipLaunchKernelGGL(slot_fields_kernel, dim3((unsigned)((S.ncols + 255) / 256), nlev),
                        dim3(256), 0, s, d, S.cols, S.ncols, Qz, diffv, DZZ, V, DWZ, mask,
-                       const_cast<double *>(S.Qz), const_cast<double *>(S.Kz),
+                       kstart, DUX, DVY, const_cast<double *>(S.Qz), const_cast<double *>(S.Kz),
                        const_cast<double *>(S.DZZ), const_cast<double *>(S.V),
-                       const_cast<double *>(S.DWZ), const_cast<uint8_t *>(S.mask));
+                       const_cast<double *>(S.DWZ), const_cast<uint8_t *>(S.mask),
+                       const_cast<int32_t *>(S.ks), const_cast<double *>(S.dux),
+                       const_cast<double *>(S.dvy));
     HIPCHK(hipGetLastError());
     return MGPU_SUCCESS;
 }

@@ -631,40 +631,49 @@ struct ColumnWalk {
 };
 
 // Loads the windows at the first row ks and the coefficients of face ks.
-template <bool OPT, int NT, bool FAST>
+// SLOT: V, DWZ, the masks and Qz from the depth-ordered walk's level-major
+// copies at slot n (the same values).
+template <bool OPT, int NT, bool FAST, bool SLOT = false>
 __device__ __forceinline__ void column_begin(const Dims &d, const StepScalars &sc,
                                              const SharedFields &f, const TracerSet &ts,
                                              long base, int ks, bool vflux, ColumnWalk<NT> &c,
-                                             int &bad) {
+                                             int &bad, const SlotColumns &S = SlotColumns{},
+                                             long n = 0) {
     const long sk = d.sk;
     auto inK = [&](int k) { return k >= d.KLB && k <= d.KUB; };
+    // where level k of the column's shared fields is
+    auto fq = [&](int k) { return SLOT ? (long)(k - d.KLB) * S.ncols + n : base + k * sk; };
+    const double *V = SLOT ? S.V : f.V, *DWZ = SLOT ? S.DWZ : f.DWZ;
+    const uint8_t *mask = SLOT ? S.mask : f.mask;
 #pragma unroll
     for (int o = 0; o < 4; ++o) {
         const int kk = ks - 1 + o;
         const bool in = inK(kk);
-        const long y = base + kk * sk;
-        c.wv[o] = in ? f.V[y] : 1.0;
-        c.wdu[o] = in ? f.DWZ[y] : 0.0;
-        c.wm[o] = in ? f.mask[y] : (uint8_t)0;
+        const long y = base + kk * sk, q = fq(kk);
+        c.wv[o] = in ? V[q] : 1.0;
+        c.wdu[o] = in ? DWZ[q] : 0.0;
+        c.wm[o] = in ? mask[q] : (uint8_t)0;
 #pragma unroll
         for (int t = 0; t < NT; ++t) c.wp[t][o] = in ? ts.P[y * ts.s + t] : 0.0;
     }
     const int kk = ks - 2;
     const bool in = inK(kk);
-    const long y = base + kk * sk;
+    const long y = base + kk * sk, q = fq(kk);
     double p[NT][4];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         p[t][0] = in ? ts.P[y * ts.s + t] : 0.0;
         p[t][1] = c.wp[t][0]; p[t][2] = c.wp[t][1]; p[t][3] = c.wp[t][2];
     }
-    const double v4[4] = {in ? f.V[y] : 1.0, c.wv[0], c.wv[1], c.wv[2]};
-    const double du4[4] = {in ? f.DWZ[y] : 0.0, c.wdu[0], c.wdu[1], c.wdu[2]};
-    const uint8_t m0 = in ? f.mask[y] : (uint8_t)0;
+    const double v4[4] = {in ? V[q] : 1.0, c.wv[0], c.wv[1], c.wv[2]};
+    const double du4[4] = {in ? DWZ[q] : 0.0, c.wdu[0], c.wdu[1], c.wdu[2]};
+    const uint8_t m0 = in ? mask[q] : (uint8_t)0;
     const bool o4[4] = {(m0 & M_OPEN) != 0, (c.wm[0] & M_OPEN) != 0, (c.wm[1] & M_OPEN) != 0,
                         (c.wm[2] & M_OPEN) != 0};
-    vface_coefs<OPT, NT, FAST>(sc, p, v4, du4, o4, inK(ks) ? f.Qz[base + ks * sk] : 0.0,
-                               vflux && ks <= d.wKUB, c.fb, bad);
+    // Qz of level ks (the copies hold each level's upper face: level ks - 1's)
+    double qks = 0.0;
+    if (inK(ks)) qks = SLOT && ks - 1 >= d.KLB ? S.Qz[fq(ks - 1)] : f.Qz[base + ks * sk];
+    vface_coefs<OPT, NT, FAST>(sc, p, v4, du4, o4, qks, vflux && ks <= d.wKUB, c.fb, bad);
 #pragma unroll
     for (int t = 0; t < NT; ++t) { c.Wm[t] = 0.0; c.Gm[t] = 0.0; }
 }
@@ -1838,20 +1847,22 @@ column_pipe_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
     const int KUB = d.wKUB;
     const long base = act ? (long)cols[n] : 0;  // Size index of the column's level 0
     const long sk = d.sk, s = ts.s;
-    const int ks = act ? f.kstart[base] : KUB + 1;   // idx2(i,j) == idx(i,j,0): KLB is 0
+    // idx2(i,j) == idx(i,j,0): KLB is 0
+    const int ks = act ? (SLOT ? S.ks[n] : f.kstart[base]) : KUB + 1;
     const int kw = wave_min(ks);
     if (!act) return;
     const TracerSet t1{ts.P + t, ts.out + t, s, ts.G0 ? ts.G0 + t : nullptr};
     double *tw = TW + t, *gs = Gs + t;
     int bad = 0;
-    const bool columnBuild = f.mask[base + KUB * sk] & M_OPEN;  // AD:2982/3004
+    const bool columnBuild =   // AD:2982/3004
+        (SLOT ? S.mask[(long)(KUB - d.KLB) * S.ncols + n] : f.mask[base + KUB * sk]) & M_OPEN;
     const bool vflux = sc.vert && !sc.vertical1d && columnBuild;
     ColumnWalk<TPL> c;
-    column_begin<true, TPL, FAST>(d, sc, f, t1, base, ks, vflux, c, bad);
+    column_begin<true, TPL, FAST, SLOT>(d, sc, f, t1, base, ks, vflux, c, bad, S, n);
     // the depth-ordered walk's cw: DUX and DVY of the column, and the face
     // below the first row, which is never a computing face (its lower cell
     // would otherwise not be inert, or the masks are rejected at KLB)
-    const double dux = SLOT ? f.DUX[base] : 0.0, dvy = SLOT ? f.DVY[base] : 0.0;
+    const double dux = SLOT ? S.dux[n] : 0.0, dvy = SLOT ? S.dvy[n] : 0.0;
     double cwB = 0.0;
     constexpr int WIN = column_window(TPL);
     double rW[WIN > 0 ? WIN : 1][TPL], rG[WIN > 0 ? WIN : 1][TPL];
@@ -2464,18 +2475,27 @@ list_write_kernel(Dims d, const uint8_t *__restrict__ mask, const int32_t *__res
 
 // The depth-ordered walk's field copies: level k of slot n at
 // (k - KLB) * nc + n, for the Size levels KLB..KUB (Qz and Diff_V of the
-// level above, 0 past the Size's top).
+// level above, 0 past the Size's top), and per slot kstart, DUX and DVY.
 __global__ void __launch_bounds__(256)
 slot_fields_kernel(Dims d, const uint32_t *__restrict__ cols, long nc,
                    const double *__restrict__ Qz, const double *__restrict__ diffv,
                    const double *__restrict__ DZZ, const double *__restrict__ V,
                    const double *__restrict__ DWZ, const uint8_t *__restrict__ mask,
-                   double *__restrict__ sQz, double *__restrict__ sKz, double *__restrict__ sDZZ,
-                   double *__restrict__ sV, double *__restrict__ sDWZ,
-                   uint8_t *__restrict__ smask) {
+                   const int32_t *__restrict__ kstart, const double *__restrict__ DUX,
+                   const double *__restrict__ DVY, double *__restrict__ sQz,
+                   double *__restrict__ sKz, double *__restrict__ sDZZ, double *__restrict__ sV,
+                   double *__restrict__ sDWZ, uint8_t *__restrict__ smask,
+                   int32_t *__restrict__ sks, double *__restrict__ sdux,
+                   double *__restrict__ sdvy) {
     const long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
     const int k = d.KLB + (int)blockIdx.y;
     if (n >= nc) return;
+    if (k == d.KLB) {
+        const long c = cols[n];   // idx2(i, j)
+        sks[n] = kstart[c];
+        sdux[n] = DUX[c];
+        sdvy[n] = DVY[c];
+    }
     const long x = (long)cols[n] + (long)(k - d.KLB) * d.sk;
     const long q = (long)(k - d.KLB) * nc + n;
     const bool up = k < d.KUB;

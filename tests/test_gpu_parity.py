@@ -359,7 +359,11 @@ def test_moving_surface_matches_oracle(name):
     change every step (fixtures.moving_surface), set before each step with
     mgpu_ad_set_geometry (drop-in) and mgpu_transport_set_geometry (batched
     handle, one graph-captured step per geometry) as the reference fetches
-    them inside every call (AD:1353-1401); bit-identical to the oracle."""
+    them inside every call (AD:1353-1401); bit-identical to the oracle.  A
+    second handle walks the columns by depth and takes each step's geometry
+    after its hydrodynamic state, so that set_geometry refreshes the walk's
+    level-major copies in place."""
+    from mohid_amd import _lib as L
     from mohid_amd import fixtures as fx
     AD, Transport, _ = _ad()
     case = make(name)
@@ -372,22 +376,30 @@ def test_moving_surface_matches_oracle(name):
     gpu = AD(case.size, case.worksize, case.geom)
     drop = [np.array(t, order="F", copy=True) for t in case.tracers]
     tr = Transport(case.size, case.worksize, case.geom, T)
+    td = Transport(case.size, case.worksize, case.geom, T)
+    td.set_column_order(L.MGPU_COLUMNS_BY_DEPTH)
     for t, p in enumerate(case.tracers):
         tr.set_tracer(t, p)
+        td.set_tracer(t, p)
     for s, (geom, hy) in enumerate(geoms):
         inst.set_geometry(geom)
         gpu.set_geometry(geom)
         tr.set_geometry(geom)
         tr.set_hydro(hy, case.params)
         tr.run(1, use_graph=True)
+        td.set_hydro(hy, case.params)
+        td.set_geometry(geom)
+        td.run(1, use_graph=True)
         for t in range(T):
             assert inst.step(ref[t], first_property=t == 0, now=(s + 1) * dt, hydro=hy) == 0
             gpu(drop[t], hy, case.params, first_property=t == 0)
             assert np.array_equal(drop[t], ref[t]), (s, t)
     for t in range(T):
         assert np.array_equal(tr.get_tracer(t), ref[t]), t
+        assert np.array_equal(td.get_tracer(t), ref[t]), t
     inst.close()
     gpu.kill()
     tr.close()
+    td.close()
     fixed = ora.run_case(case, steps, nthreads=1)
     assert not np.array_equal(fixed[0], ref[0])

@@ -1309,6 +1309,10 @@ struct mgpu_transport {
     Dims d;
     mgpu_transport_opts opts;
     hipStream_t stream = nullptr;
+    // second stream for the next step's face conductances, forked from and
+    // joined back into `stream` by the two events
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     DevGeometry geom;
     DevHydro hydro;
     DevDerived der;
@@ -1376,8 +1380,12 @@ static void drop_graphs(mgpu_transport *t) {
 extern "C" int mgpu_transport_destroy(mgpu_transport *t) {
     if (!t) return MGPU_OFF_ERR;
     if (t->stream) (void)hipStreamSynchronize(t->stream);
+    if (t->side) (void)hipStreamSynchronize(t->side);
     drop_graphs(t);
     drop_events(t);
+    if (t->fork) (void)hipEventDestroy(t->fork);
+    if (t->join) (void)hipEventDestroy(t->join);
+    if (t->side) (void)hipStreamDestroy(t->side);
     t->geom.release();
     t->hydro.release();
     t->der.release();
@@ -1434,6 +1442,13 @@ extern "C" int mgpu_transport_create(const mgpu_size3d *size, const mgpu_size3d 
     if (e != hipSuccess) {
         mgpu_set_error("hipStreamCreate", e);
         delete t;
+        return MGPU_ERR_HIP;
+    }
+    if ((e = hipStreamCreate(&t->side)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&t->fork, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&t->join, hipEventDisableTiming)) != hipSuccess) {
+        mgpu_set_error("side stream", e);
+        mgpu_transport_destroy(t);
         return MGPU_ERR_HIP;
     }
     const long n3 = t->d.n3();
@@ -1640,6 +1655,20 @@ static int enqueue_steps(mgpu_transport *t, int nsteps, int cur, int dir) {
     const Dims &d = t->d;
     hipStream_t s = t->stream;
     const SharedFields f = make_fields(t->hydro, t->der, t->geom);
+    // face conductances (AD:1421-1478) on stream q; the depth-ordered column
+    // walk computes the W faces' itself
+    auto prep = [&](bool sw, hipStream_t q) {
+        const bool nf = !t->sc.optimize && t->hydro.have_nf;
+        return launch_prep_faces(d, t->hydro.visc, t->hydro.diffv, t->hydro.Qx, t->hydro.Qy,
+                                 t->hydro.Qz, t->geom.DUX, t->geom.DVY, t->geom.DZX,
+                                 t->geom.DZY, t->geom.AreaU, t->geom.AreaV, t->geom.DZZ,
+                                 t->der.mask, t->schmidtH, t->schmidtV, t->backgroundV,
+                                 t->sc.optimize ? 0 : t->nulldif,
+                                 nf ? t->hydro.nfu : nullptr, nf ? t->hydro.nfv : nullptr,
+                                 nf ? t->hydro.nfw : nullptr, 1, sw ? 0 : 1, t->der.cu,
+                                 t->der.cv, t->der.cw, q);
+    };
+    bool ahead = false;   // this step's conductances were enqueued on the side stream
     for (int st = 0; st < nsteps; ++st) {
         StepScalars sc = t->sc;
         if (dir >= 0) {
@@ -1666,26 +1695,34 @@ static int enqueue_steps(mgpu_transport *t, int nsteps, int cur, int dir) {
         L.counters = t->der.counters;
         L.mid = t->mid;
         L.Rb = t->Rb;
-        // first property of the step: face conductances (AD:1421-1478); the
-        // depth-ordered column walk computes the W faces' itself
-        int rc = mark(t);
-        if (rc) return rc;
-        const bool nf = !t->sc.optimize && t->hydro.have_nf;
-        rc = launch_prep_faces(d, t->hydro.visc, t->hydro.diffv, t->hydro.Qx, t->hydro.Qy,
-                               t->hydro.Qz, t->geom.DUX, t->geom.DVY, t->geom.DZX,
-                               t->geom.DZY, t->geom.AreaU, t->geom.AreaV, t->geom.DZZ,
-                               t->der.mask, t->schmidtH, t->schmidtV, t->backgroundV,
-                               t->sc.optimize ? 0 : t->nulldif,
-                               nf ? t->hydro.nfu : nullptr, nf ? t->hydro.nfv : nullptr,
-                               nf ? t->hydro.nfw : nullptr, 1, slot_walk(L, sc, f) ? 0 : 1,
-                               t->der.cu, t->der.cv, t->der.cw, s);
-        if (rc) return rc;
-        if ((rc = mark(t))) return rc;
+        const bool sw = slot_walk(L, sc, f);
+        // The depth-ordered walk reads no conductance, so the next step's
+        // (the same inputs, the same operations) are made on the side stream
+        // while this step's walk runs, after this step's first kernels have
+        // read cu and cv; the next step's first kernels wait for them.
+        // Per-kernel timing keeps every kernel on the handle's stream.
+        const bool overlap = sw && !t->timing && st + 1 < nsteps;
+        int rc;
+        if (ahead) {
+            HIPCHK(hipStreamWaitEvent(s, t->join, 0));
+        } else {
+            if ((rc = mark(t))) return rc;
+            if ((rc = prep(sw, s))) return rc;
+            if ((rc = mark(t))) return rc;
+        }
+        ahead = false;
         // first kernel(s) of every chunk, then the second kernel(s)
         for (int ph = 1; ph <= 2; ++ph) {
             rc = launch_transport(L, d, sc, f, s, ph);
             if (rc) return rc;
             if ((rc = mark(t))) return rc;
+            if (ph == 1 && overlap) {
+                HIPCHK(hipEventRecord(t->fork, s));
+                HIPCHK(hipStreamWaitEvent(t->side, t->fork, 0));
+                if ((rc = prep(sw, t->side))) return rc;
+                HIPCHK(hipEventRecord(t->join, t->side));
+                ahead = true;
+            }
         }
         cur ^= 1;
     }

@@ -1820,10 +1820,10 @@ __device__ __forceinline__ void column_row_pipe(const StepScalars &sc, int k, in
 
 // Levels below the register window whose W and G stay in LDS in the
 // pipelined walk, by tracers per lane: sized so that the blocks the register
-// file admits per CU (4, 3 and 2 for 1, 2 and 4 tracers per lane) fit the
-// CU's 160 KB of LDS.
-__host__ __device__ constexpr int column_lds_window(int tpl) {
-    return tpl == 1 ? 10 : (tpl == 2 ? 6 : (tpl == 4 ? 4 : 0));
+// file admits per CU (4, 3 and 2 for 1, 2 and 4 tracers per lane; 3 for the
+// depth-ordered walk with 1) fit the CU's 160 KB of LDS.
+__host__ __device__ constexpr int column_lds_window(int tpl, bool slot = false) {
+    return tpl == 1 ? (slot ? 13 : 10) : (tpl == 2 ? 6 : (tpl == 4 ? 4 : 0));
 }
 
 // SLOT: the depth-ordered walk (cols sorted by first row; TI, the shared
@@ -1868,7 +1868,7 @@ column_pipe_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts,
     double rW[WIN > 0 ? WIN : 1][TPL], rG[WIN > 0 ? WIN : 1][TPL];
     // the next LW levels down keep W and G in LDS, one (W, G) pair per
     // (level, tracer) and lane; a lane touches only its own entries
-    constexpr int LW = column_lds_window(TPL);
+    constexpr int LW = column_lds_window(TPL, SLOT);
     __shared__ double2 lwg[LW > 0 ? LW * TPL : 1][256];
     LevelIn<TPL> cur;
     if (SLOT) level_load_slot<TPL>(d, sc, S, t1, tw, base, n, ks, cur);

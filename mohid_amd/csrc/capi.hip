@@ -1266,7 +1266,8 @@ struct SlotStore {
         S.sV = sV; S.bV = bV;
         return S;
     }
-    // the copies from the current fields (the list built)
+    // the level-major copies from the handle's current fields, for the
+    // list already built (the copies do not use the Schmidt scalars)
     int fields(const Dims &d, const DevHydro &h, const DevGeometry &g, const DevDerived &dv,
                hipStream_t s) const {
         if (!ncols) return MGPU_SUCCESS;

@@ -20,7 +20,11 @@
 //                   upward: the rows D/E/F (AD:2708-3140), the
 //                   THOMASZ_NewType2 forward sweep (MF:4080-4100), then the
 //                   back substitution (MF:4102-4106); by default each level's
-//                   loads go out one level ahead (column_pipe_kernel).
+//                   loads go out one level ahead (column_pipe_kernel), and
+//                   the batched transport walks the columns sorted by first
+//                   row, with TI, W/G and the fields it reads laid out
+//                   level-major in that order (SlotColumns: a wave's lanes
+//                   start together, and a level is one contiguous run).
 // The kernels advance up to MAXT tracers per launch, so the shared fields
 // and every tracer-independent quantity are loaded/computed once per lane.
 //

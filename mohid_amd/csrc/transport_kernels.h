@@ -1368,10 +1368,18 @@ __device__ __forceinline__ void gather_line(const Dims &d, const SharedFields &f
                                             const TracerSet &ts, long x, long st, int pos,
                                             int lo, int hi, const double *du_line, long du_st,
                                             Line5<TPL> &L) {
+    bool inr[5];
+#pragma unroll
+    for (int o = 0; o < 5; ++o) inr[o] = pos - 2 + o >= lo && pos - 2 + o <= hi;
 #pragma unroll
     for (int o = 0; o < 5; ++o) {
-        const bool in = pos - 2 + o >= lo && pos - 2 + o <= hi;
-        const long c = in ? x + (long)(o - 2) * st : x;   // always inside Size
+        const bool in = inr[o];
+        // a position outside Size is loaded from the nearest one inside
+        // towards the cell: the far-upstream value the face would take in
+        // its place (q1 for q0, q2 for q3 in face_opt), so the face needs no
+        // select for it
+        const int off = in ? o - 2 : (o == 0 && inr[1] ? -1 : (o == 4 && inr[3] ? 1 : 0));
+        const long c = x + (long)off * st;   // always inside Size
         // unconditional loads (a load under `in` would wait before the next)
         const double du = du_line[in ? (long)(o - 2) * du_st : 0];
         const uint8_t m = f.mask[c];
@@ -1379,8 +1387,14 @@ __device__ __forceinline__ void gather_line(const Dims &d, const SharedFields &f
         load_tpl<TPL>(ts.P + c * ts.s, L.p[o]);
         L.in[o] = in;
         L.op[o] = in && (m & M_OPEN);
-        L.v[o] = in ? v : 1.0;
-        L.du[o] = in ? du : 0.0;
+        // outside Size: a neighbour's V and the cell's spacing.  Only a face
+        // that is inactive (a closed cell on either side) or next to the
+        // boundary (a closed far cell) reaches them, and its Courant number,
+        // spacings and limiter are then discarded: V of positions 1-3 is in
+        // Size for a WorkSize cell, and a face's far cell is in Size when it
+        // is open
+        L.v[o] = v;
+        L.du[o] = du;
     }
 }
 
@@ -1396,7 +1410,6 @@ __device__ __forceinline__ void face_opt(const StepScalars &sc, const Line5<TPL>
     const double du2 = o1 ? L.du[3] : L.du[2], du3 = o1 ? L.du[4] : L.du[3];
     const bool op0 = o1 ? L.op[1] : L.op[0], op1 = o1 ? L.op[2] : L.op[1];
     const bool op2 = o1 ? L.op[3] : L.op[2], op3 = o1 ? L.op[4] : L.op[3];
-    const bool in0 = o1 ? L.in[1] : L.in[0], in3 = o1 ? L.in[4] : L.in[3];
     const bool up = Q > 0.0;
     const bool nearb = up ? !op0 : !op3;
     const double vup = up ? v1 : v2;
@@ -1413,7 +1426,8 @@ __device__ __forceinline__ void face_opt(const StepScalars &sc, const Line5<TPL>
         const double q0 = o1 ? L.p[1][t] : L.p[0][t], q1 = o1 ? L.p[2][t] : L.p[1][t];
         const double q2 = o1 ? L.p[3][t] : L.p[2][t], q3 = o1 ? L.p[4][t] : L.p[3][t];
         const double p1 = q1, p2 = q2;
-        const double pf = up ? (in0 ? q0 : q1) : (in3 ? q3 : q2);
+        // outside Size gather_line loaded q1 into q0 and q2 into q3
+        const double pf = up ? q0 : q3;
         const double a = up ? p2 : p1, b = up ? p1 : p2;   // downstream, upstream
         double dc, rr;
         if (FAST) {

@@ -1823,16 +1823,18 @@ __device__ __forceinline__ void column_row_pipe(const StepScalars &sc, int k, in
         c.Gm[t] = G;
         c.fb[t] = ft[t];
     }
-    // slide the windows up one level
+    // slide the windows up one level.  A level past the Size's top gets a
+    // closed mask; its V, DWZ and P (the level's own, from the clamped
+    // loads) only reach the face above WorkSize KUB, which is never built
 #pragma unroll
     for (int o = 0; o < 3; ++o) { c.wv[o] = c.wv[o + 1]; c.wdu[o] = c.wdu[o + 1]; c.wm[o] = c.wm[o + 1]; }
-    c.wv[3] = L.in3 ? L.v3 : 1.0;
-    c.wdu[3] = L.in3 ? L.du3 : 0.0;
+    c.wv[3] = L.v3;
+    c.wdu[3] = L.du3;
     c.wm[3] = L.in3 ? L.m3 : (uint8_t)0;
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         c.wp[t][0] = c.wp[t][1]; c.wp[t][1] = c.wp[t][2]; c.wp[t][2] = c.wp[t][3];
-        c.wp[t][3] = L.in3 ? L.p3[t] : 0.0;
+        c.wp[t][3] = L.p3[t];
     }
 }
 

@@ -2230,10 +2230,11 @@ __device__ __forceinline__ void line_put(double *md, double *out, long y, bool s
 // Lines: one lane per (line, tracer) solves a WorkSize line along j (AX 0,
 // lines (i, k): THOMAS_3D_i0_j1_NewType, MF:3756-3810) or along i (AX 1,
 // lines (j, k): THOMAS_3D_i1_j0_NewType, MF:3814-3870) from the row
-// records: the forward sweep (W and G over the record's first half, each
-// record read before it is written), the ghost end row, the back
-// substitution.  LROWS consecutive records are loaded before they are swept,
-// so each lane keeps several independent loads in flight.  Neighbouring
+// records: the forward sweep (W and G written over the row's records once
+// they are read), the ghost end row, the back substitution.  LROWS
+// consecutive records (BROWS W/G pairs in the back substitution, whose row
+// masks are loaded one batch ahead) are loaded before they are swept, so
+// each lane keeps several independent loads in flight.  Neighbouring
 // lanes take neighbouring tracers, then (AX 0) neighbouring i, so every
 // record access of a wave is one contiguous run.
 //
@@ -2250,7 +2251,8 @@ __device__ __forceinline__ void line_put(double *md, double *out, long y, bool s
 // KUB+1 there, which the column walk reads.  Cells the column solve does not
 // overwrite (columns it does not solve, the ghost row) take the line
 // solution in `out` as well.  With one layer every line cell goes to `out`.
-constexpr int LROWS = 4;
+constexpr int LROWS = 3;   // 4 needs 104 VGPRs: 4 waves per SIMD instead of 5
+constexpr int BROWS = 6;
 
 template <int NT, bool FAST, int AX>
 __global__ void __launch_bounds__(256)
@@ -2275,12 +2277,23 @@ line_thomas_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts, double4
     bool pivot = false;
     double Wm = 0.0, Gm = 0.0;
     const int nrow = hi - lo + 1;
+    // W and G of a row.  When a line's NT lanes share one wave (NT divides
+    // 64), the row's pairs of all the run's tracers are packed at the start
+    // of its cell's record block (tracer t_base + t at pair x*2s + t_base + t
+    // from the block base, one 128-byte run for 8 tracers): they overwrite
+    // records this wave has already read, or (t_base > 0) records of earlier
+    // chunks, whose line solves are done.  Otherwise the pair goes to the
+    // first half of the lane's own record.
+    constexpr bool PACK = 64 % NT == 0;
+    const int wgo = PACK ? -t - t_base : 0;   // pair of record y: (double2 *)(R + y) + wgo
     // An identity row's W and G are -0 and TI exactly when the row before it
     // left finite W, G and TI is not zero (aux = 1 + 0*W = 1, G = TI - 0*G);
     // such rows are not stored, the back substitution recomputes them.  From
     // the first identity row where that does not hold (rows >= keep) every
     // row is stored.
     int keep = nrow;
+    const long mx0 = AX == 0 ? d.idx(a, lo, k) : d.idx(lo, a, k);   // mask of row lo
+    const long ms = AX == 0 ? d.sj : 1;                                // mask row stride
     for (int r0 = 0; r0 < nrow; r0 += LROWS) {
         double4 q4[LROWS];
         bool idr[LROWS];
@@ -2341,7 +2354,7 @@ line_thomas_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts, double4
                 Wm = W;
                 Gm = G;
                 const long y = x0 + (long)(r0 + q) * ss;
-                if (!plain) reinterpret_cast<double2 *>(R + y)[0] = make_double2(W, G);
+                if (!plain) reinterpret_cast<double2 *>(R + y)[wgo] = make_double2(W, G);
                 if (edge) mid[y + ez] = ts.P[y + ez];   // rows KLB-1 / KUB+1 for the column walk
             }
         }
@@ -2363,31 +2376,45 @@ line_thomas_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts, double4
     double *md = mid ? mid + t : nullptr;
     double *out = ts.out + t;
     line_put(md, out, xg * s, false, xs);
-    // back substitution, LROWS records loaded at a time.  A recomputed
+    // back substitution, BROWS pairs loaded at a time.  A recomputed
     // identity row below a finite value comes out as TI exactly; where TI is
     // PROP, `out` already holds it (the caller keeps out equal to P on such
     // cells) and `mid` is only needed where the column walk reads it
     // (solved columns, from kstart-2 up).
     const long top = (long)(d.wKUB - k) * d.sk;   // offset of a cell's column top (KUB)
-    for (int r1 = nrow - 1; r1 >= 0; r1 -= LROWS) {
-        double2 wg[LROWS];
-        uint8_t mq[LROWS];
+    // the masks of each batch's rows and of their column tops are loaded one
+    // batch ahead
+    uint8_t bn[BROWS], tn[BROWS];
 #pragma unroll
-        for (int q = 0; q < LROWS; ++q) {
+    for (int q = 0; q < BROWS; ++q) {
+        const int rn = nrow - 1 - q;
+        bn[q] = rn >= 0 ? f.mask[mx0 + (long)rn * ms] : 0;
+        tn[q] = rn >= 0 ? f.mask[mx0 + (long)rn * ms + top] : 0;
+    }
+    for (int r1 = nrow - 1; r1 >= 0; r1 -= BROWS) {
+        double2 wg[BROWS];
+        uint8_t mq[BROWS], mt[BROWS];
+#pragma unroll
+        for (int q = 0; q < BROWS; ++q) {
+            mq[q] = bn[q];
+            mt[q] = tn[q];
+            const int rn = r1 - BROWS - q;
+            bn[q] = rn >= 0 ? f.mask[mx0 + (long)rn * ms] : 0;
+            tn[q] = rn >= 0 ? f.mask[mx0 + (long)rn * ms + top] : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < BROWS; ++q) {
             if (r1 - q >= 0) {
-                const int r = lo + r1 - q;
-                const long xc = AX == 0 ? d.idx(a, r, k) : d.idx(r, a, k);
                 const long y = x0 + (long)(r1 - q) * ss;
-                mq[q] = f.mask[xc];
                 if ((mq[q] & M_HID) && r1 - q < keep)
                     wg[q] = make_double2(-0.0, (last && (mq[q] & M_LAND)) ? MGPU_NULL_REAL
                                                                            : ts.P[y]);
                 else
-                    wg[q] = reinterpret_cast<const double2 *>(R + y)[0];
+                    wg[q] = reinterpret_cast<const double2 *>(R + y)[wgo];
             }
         }
 #pragma unroll
-        for (int q = 0; q < LROWS; ++q) {
+        for (int q = 0; q < BROWS; ++q) {
             if (r1 - q >= 0) {
                 const int r = lo + r1 - q;
                 const long xc = AX == 0 ? d.idx(a, r, k) : d.idx(r, a, k);
@@ -2395,7 +2422,7 @@ line_thomas_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts, double4
                                   !(last && (mq[q] & M_LAND));
                 xs = wg[q].x * xs + wg[q].y;
                 // water columns are overwritten by the column solve (MF:4086)
-                const bool solved = (f.mask[xc + top] & M_WATER) != 0;
+                const bool solved = (mt[q] & M_WATER) != 0;
                 if (!same) {
                     line_put(md, out, xc * s, solved, xs);
                 } else if (md && solved) {

@@ -2384,23 +2384,34 @@ line_thomas_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts, double4
     const long top = (long)(d.wKUB - k) * d.sk;   // offset of a cell's column top (KUB)
     // the masks of each batch's rows and of their column tops are loaded one
     // batch ahead
+    // kstart of row rn's column, for the mid stores of identity rows: loaded
+    // a batch ahead with the masks along j (along i that measured slower)
+    constexpr bool KPRE = AX == 0;
+    auto kst = [&](int rn) {
+        return f.kstart[d.idx2(AX == 0 ? a : lo + rn, AX == 0 ? lo + rn : a)];
+    };
     uint8_t bn[BROWS], tn[BROWS];
+    int kn[BROWS];
 #pragma unroll
     for (int q = 0; q < BROWS; ++q) {
         const int rn = nrow - 1 - q;
         bn[q] = rn >= 0 ? f.mask[mx0 + (long)rn * ms] : 0;
         tn[q] = rn >= 0 ? f.mask[mx0 + (long)rn * ms + top] : 0;
+        kn[q] = KPRE && md && rn >= 0 ? kst(rn) : 0;
     }
     for (int r1 = nrow - 1; r1 >= 0; r1 -= BROWS) {
         double2 wg[BROWS];
         uint8_t mq[BROWS], mt[BROWS];
+        int kq[BROWS];
 #pragma unroll
         for (int q = 0; q < BROWS; ++q) {
             mq[q] = bn[q];
             mt[q] = tn[q];
+            kq[q] = kn[q];
             const int rn = r1 - BROWS - q;
             bn[q] = rn >= 0 ? f.mask[mx0 + (long)rn * ms] : 0;
             tn[q] = rn >= 0 ? f.mask[mx0 + (long)rn * ms + top] : 0;
+            kn[q] = KPRE && md && rn >= 0 ? kst(rn) : 0;
         }
 #pragma unroll
         for (int q = 0; q < BROWS; ++q) {
@@ -2426,8 +2437,7 @@ line_thomas_kernel(Dims d, StepScalars sc, SharedFields f, TracerSet ts, double4
                 if (!same) {
                     line_put(md, out, xc * s, solved, xs);
                 } else if (md && solved) {
-                    const int ks = f.kstart[d.idx2(AX == 0 ? a : r, AX == 0 ? r : a)];
-                    if (k >= ks - 2) md[xc * s] = xs;
+                    if (k >= (KPRE ? kq[q] : kst(r1 - q)) - 2) md[xc * s] = xs;
                 }
             }
         }

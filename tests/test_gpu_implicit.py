@@ -226,9 +226,13 @@ def test_implicit_batched_matches_oracle(name, start):
     print(f"{name} start={start}: batched {worst:.3e} bit-identical={bitwise}")
 
 
-@pytest.mark.parametrize("shape", [(0, 0, 0), (1, 2, 4), (2, 1, 2)])
+@pytest.mark.parametrize("shape", [(0, 0, 0), (1, 2, 4), (2, 1, 2), (3, 1, 1)])
 def test_implicit_batched_launch_shapes(shape):
-    """Tracers per launch / per lane do not change the implicit results."""
+    """Tracers per launch / per lane do not change the implicit results, and
+    they match the oracle.  Launches of 1, 2 or 4 tracers pack the line
+    solve's W/G pairs per cell, over records of launches already solved when
+    the launch is not the first; a launch of 3 keeps them in its own
+    records, beside a packed launch of 1."""
     L, _, Transport, _ = _mods()
     case = make("tvd_land_sigma", tracers=("temperature", "lognormal", "salinity", "lognormal"))
     params = _implicit_params(case, **_dirs(0, 0))
@@ -244,6 +248,9 @@ def test_implicit_batched_launch_shapes(shape):
         tr.close()
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
+    ref = _oracle_alternating(case, 3, 0, params)
+    for t, a in enumerate(outs[1]):
+        assert _rel_err(a, ref[t]) <= RTOL, (shape, t)
 
 
 def test_implicit_fast_mode():
